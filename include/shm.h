@@ -1,0 +1,141 @@
+/* C ABI of the MI355X sledgehamr time-step library (libshm_<Project>.so).
+ *
+ * One shared library is built per project: it holds that project's cell
+ * functions (Rhs, GravitationalWavesRhs, TagCellForRefinement,
+ * TruncationModifier) compiled into the gfx950 kernels.  Host code talks to
+ * it only through the functions below: plain pointers, sizes and status
+ * codes; no C++ or torch types.
+ *
+ * Conventions
+ *   - Every function returns SHM_OK (0) or a positive status code; the text
+ *     of the last error of a context is available from shm_last_error().
+ *   - Field data are exchanged as FP64 arrays of the valid cells of a level,
+ *     box after box in box-table order, x fastest within a box (the Array4
+ *     order of the reference).  Boxes are inclusive index boxes
+ *     {lo_x, lo_y, lo_z, hi_x, hi_y, hi_z}.
+ *   - Calls on one context are stream-ordered on the device; functions that
+ *     return data synchronise.  One context per host thread.
+ */
+#ifndef SHM_H_
+#define SHM_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct shm_ctx shm_ctx;
+
+enum shm_status {
+    SHM_OK = 0,
+    SHM_ERR_ARG = 1,          /* invalid argument or inputs-file value */
+    SHM_ERR_HIP = 2,          /* HIP / hipFFT runtime failure */
+    SHM_ERR_STATE = 3,        /* call not valid in the current state */
+    SHM_ERR_UNSUPPORTED = 4,  /* configuration not built / not supported */
+    SHM_ERR_INTERNAL = 5
+};
+
+enum shm_precision {
+    SHM_PREC_INPUTS = -1,     /* take shm.precision from the inputs file */
+    SHM_PREC_FP64 = 0,        /* FP64 storage and arithmetic */
+    SHM_PREC_FP32C = 1        /* FP32 storage + FP32 remainder (compensated) */
+};
+
+enum shm_te_flags {
+    SHM_TE_COMPUTE = 1,       /* compute truncation errors */
+    SHM_TE_AVGDOWN = 2,       /* replace coarse data by the child mean */
+    SHM_TE_TAGS = 4           /* write the tag mask and counters */
+};
+
+typedef struct shm_config {
+    const char *inputs;       /* inputs-file text (ParmParse syntax) */
+    int precision;            /* enum shm_precision */
+    int device;               /* HIP device ordinal */
+} shm_config;
+
+typedef struct shm_info_t {
+    int ncomp;                /* components incl. GW tensor when enabled */
+    int nscalars;             /* project fields + momenta */
+    int nfields;              /* project fields */
+    int nghost;
+    int stencil_radius;       /* reach of the cell functions (probed) */
+    int precision;
+    int max_level;
+    int finest_level;
+    int coarse_level_grid_size;
+    int integrator_type;
+    int with_gravitational_waves;
+    int shadow_hierarchy;
+    double L, cfl, t_start, t_end;
+} shm_info_t;
+
+/* Sledgehamr::Sledgehamr + InitSledgehamr (sledgehamr.cpp:15-77): build the
+ * project, parse the inputs file, create level 0 (one periodic box, or
+ * chopped by amr.max_grid_size), probe the cell functions. */
+int shm_create(const shm_config *cfg, shm_ctx **out);
+int shm_destroy(shm_ctx *ctx);
+const char *shm_last_error(const shm_ctx *ctx);
+const char *shm_project_name(const shm_ctx *ctx);
+int shm_get_info(shm_ctx *ctx, shm_info_t *info);
+
+/* Define / replace the boxes of a level (AmrCore::MakeNewLevel* for a
+ * static layout; boxes must be properly nested and even-aligned above 0). */
+int shm_set_box_table(shm_ctx *ctx, int lev, const int *boxes, int nbox);
+int shm_get_box_table(shm_ctx *ctx, int lev, int *boxes, int maxbox, int *nbox);
+int shm_level_info(shm_ctx *ctx, int lev, long long *ncells, double *t,
+                   int *istep);
+int shm_set_level_time(shm_ctx *ctx, int lev, double t, int istep);
+
+/* State transfer for grid_new of a level (or grid_old with old_state=1). */
+int shm_upload(shm_ctx *ctx, int lev, int comp, const double *data);
+int shm_download(shm_ctx *ctx, int lev, int comp, double *data, int old_state);
+
+/* Seeded device-side initial states (kind 1: complex random phase with
+ * modulus 1 + p[0] N(0,1); kind 2: zero). */
+int shm_init_state(shm_ctx *ctx, int lev, int kind, unsigned long long seed,
+                   const double *p, int np);
+
+/* Sledgehamr::FillRhs / FillAddRhs (sledgehamr.h:71-88, macros.h:301-485):
+ * rhs = f(grid_new) or rhs = f(grid_new) + weight * rhs. */
+int shm_fill_rhs(shm_ctx *ctx, int lev, double time, double weight, int add);
+int shm_download_rhs(shm_ctx *ctx, int lev, int comp, double *data);
+
+/* Integrator::Advance(lev) (integrator.cpp:11-31) with the configured
+ * integrator (integrator.type 10, 11, 20, 21, 22). */
+int shm_advance_level(shm_ctx *ctx, int lev);
+
+/* TimeStepper::Advance(0) repeated ncoarse times (time_stepper.cpp:90-133). */
+int shm_evolve(shm_ctx *ctx, int ncoarse);
+
+/* Sledgehamr::CreateShadowLevel (sledgehamr.cpp:234-251). */
+int shm_create_shadow(shm_ctx *ctx);
+
+/* LevelSynchronizer::ComputeTruncationErrors (level_synchronizer.cpp:240-357)
+ * fused with Sledgehamr::ErrorEst/TagWithTruncationCpu (sledgehamr.cpp:280-348,
+ * macros.h:489-566).  mask: one byte per valid cell of lev (may be NULL);
+ * counts: [ntags_total, ntags_user, ntags_trunc[ncomp]] (may be NULL);
+ * te: truncation errors per coarse cell, ncomp planes (may be NULL). */
+int shm_te_tag(shm_ctx *ctx, int lev, int flags, unsigned char *mask,
+               long long *counts, double *te);
+
+/* GravitationalWaves::ComputeSpectrum (gravitational_waves.cpp:52-322).
+ * mode 0: unbinned (unique |k|^2), 1: rounded |k|, 2: nbins log bins.
+ * bins/kvals must hold *nout values (call with bins=NULL to query *nout). */
+int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
+                    int projection_type, double *bins, double *kvals,
+                    int *nout);
+
+int shm_sync(shm_ctx *ctx);
+int shm_autotune(shm_ctx *ctx, int lev);
+int shm_set_tile(shm_ctx *ctx, int variant, int tile_z);
+
+/* Kernel timing with HIP events on the library's stream. */
+int shm_set_timing(shm_ctx *ctx, int on);
+int shm_get_timing(shm_ctx *ctx, double *stage_ms, long long *stage_launches,
+                   long long *stage_cells, double *halo_ms,
+                   long long *halo_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHM_H_ */

@@ -1,0 +1,863 @@
+// ORACLE -- test infrastructure only.  Imported, linked or run solely by
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, always as
+// the checker or the timed CPU baseline, never as the product path.
+//
+// FP64 CPU restatement of sledgehamr's per-level time-step path in the
+// reference's own (unfused) structure: a per-stage Rhs pass over 1024x8x8
+// tiles, separate Copy/Saxpy/LinComb passes, a ghost fill per stage.
+//
+// Functions and the reference code they follow:
+//   fill_rhs / fill_add_rhs   source/macros.h:301-485 (FillRhs, FillAddRhs)
+//   rkn                      source/integrators/rkn.cpp:91-186, tableaux :188-271
+//   lsssprk3                 source/integrators/lsssprk3.cpp:14-31
+//   leapfrog                 source/integrators/leapfrog.cpp:13-57
+//   advance                  source/integrators/integrator.cpp:11-31
+//   fill_boundary            level_synchronizer.cpp:94-214 (periodic int_dir,
+//                            same-level box-to-box)
+//   te_avgdown               source/kernels.h:18-42,
+//                            level_synchronizer.cpp:240-357
+//   tag_with_truncation      source/macros.h:489-566, sledgehamr.cpp:280-348
+//   create_shadow            sledgehamr.cpp:234-251
+//   gw_spectrum              gravitational_waves.cpp:52-322 (CPU branch),
+//                            gravitational_waves.h:101-133 (Lambda)
+//   k tables                 pySledgehamr/Sledgehamr.py:7-47 (CreateUniqueVals,
+//                            IndexToK), type 3 from utils Gradient<3>
+//
+// Parity pins (tests/): Laplacian/Gradient exact on polynomials and on
+// plane-wave symbols; integrator orders of convergence and exact harmonic
+// solutions; SSPRK3 equal to u0 + dt/6 (k1 + k2 + 4 k3); hand-built TE/tag
+// blocks; brute-force DFT and Lambda identities; spectra_ks.hdf5 tables.
+#pragma once
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <array>
+#include <sstream>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "shim/sledgehamr.h"
+
+namespace orc {
+
+struct Box {
+    int lo[3], hi[3];
+    int n(int d) const { return hi[d] - lo[d] + 1; }
+    long long vol() const { return (long long)n(0) * n(1) * n(2); }
+};
+
+/** One box of a MultiFab: valid box grown by g, x fastest, comp slowest. */
+struct Fab {
+    Box box;
+    int g = 0, nc = 0;
+    long long jst = 0, kst = 0, nst = 0;
+    std::vector<double> d;
+    void define(const Box &b, int ghost, int ncomp) {
+        box = b;
+        g = ghost;
+        nc = ncomp;
+        jst = b.n(0) + 2 * g;
+        kst = jst * (b.n(1) + 2 * g);
+        nst = kst * (b.n(2) + 2 * g);
+        d.assign((size_t)(nst * nc), 0.);
+    }
+    double &at(int i, int j, int k, int n) {
+        return d[(size_t)((i - box.lo[0] + g) + (j - box.lo[1] + g) * jst +
+                          (k - box.lo[2] + g) * kst + n * nst)];
+    }
+    amrex::Array4<double> arr() {
+        amrex::Array4<double> a;
+        a.p = d.data();
+        a.jstride = jst;
+        a.kstride = kst;
+        a.nstride = nst;
+        a.begin = {box.lo[0] - g, box.lo[1] - g, box.lo[2] - g};
+        return a;
+    }
+    amrex::Array4<const double> carr() const {
+        amrex::Array4<const double> a;
+        a.p = d.data();
+        a.jstride = jst;
+        a.kstride = kst;
+        a.nstride = nst;
+        a.begin = {box.lo[0] - g, box.lo[1] - g, box.lo[2] - g};
+        return a;
+    }
+};
+
+struct MultiFab {
+    std::vector<Fab> fabs;
+    int nc = 0, g = 0;
+    double t = -DBL_MAX;
+    int istep = 0;
+    void define(const std::vector<Box> &boxes, int ncomp, int ghost) {
+        fabs.assign(boxes.size(), Fab());
+        for (size_t b = 0; b < boxes.size(); ++b) fabs[b].define(boxes[b], ghost, ncomp);
+        nc = ncomp;
+        g = ghost;
+    }
+    bool defined() const { return !fabs.empty(); }
+};
+
+/** MultiFab::Copy(dst, src, scomp, dcomp, ncomp, nghost). */
+inline void Copy(MultiFab &dst, MultiFab &src, int sc, int dc, int nc, int ng) {
+    for (size_t b = 0; b < dst.fabs.size(); ++b) {
+        Fab &D = dst.fabs[b], &S = src.fabs[b];
+#pragma omp parallel for schedule(static)
+        for (int k = D.box.lo[2] - ng; k <= D.box.hi[2] + ng; ++k)
+            for (int n = 0; n < nc; ++n)
+                for (int j = D.box.lo[1] - ng; j <= D.box.hi[1] + ng; ++j)
+                    for (int i = D.box.lo[0] - ng; i <= D.box.hi[0] + ng; ++i)
+                        D.at(i, j, k, dc + n) = S.at(i, j, k, sc + n);
+    }
+}
+
+/** MultiFab::Saxpy(dst, a, src, scomp, dcomp, ncomp, nghost): dst += a*src. */
+inline void Saxpy(MultiFab &dst, double a, MultiFab &src, int sc, int dc, int nc,
+                  int ng) {
+    for (size_t b = 0; b < dst.fabs.size(); ++b) {
+        Fab &D = dst.fabs[b], &S = src.fabs[b];
+#pragma omp parallel for schedule(static)
+        for (int k = D.box.lo[2] - ng; k <= D.box.hi[2] + ng; ++k)
+            for (int n = 0; n < nc; ++n)
+                for (int j = D.box.lo[1] - ng; j <= D.box.hi[1] + ng; ++j)
+                    for (int i = D.box.lo[0] - ng; i <= D.box.hi[0] + ng; ++i)
+                        D.at(i, j, k, dc + n) += a * S.at(i, j, k, sc + n);
+    }
+}
+
+/** MultiFab::LinComb(dst, a, x, xc, b, y, yc, dc, nc, ng): dst = a x + b y. */
+inline void LinComb(MultiFab &dst, double a, MultiFab &x, int xc, double bb,
+                    MultiFab &y, int yc, int dc, int nc, int ng) {
+    for (size_t b = 0; b < dst.fabs.size(); ++b) {
+        Fab &D = dst.fabs[b], &X = x.fabs[b], &Y = y.fabs[b];
+#pragma omp parallel for schedule(static)
+        for (int k = D.box.lo[2] - ng; k <= D.box.hi[2] + ng; ++k)
+            for (int n = 0; n < nc; ++n)
+                for (int j = D.box.lo[1] - ng; j <= D.box.hi[1] + ng; ++j)
+                    for (int i = D.box.lo[0] - ng; i <= D.box.hi[0] + ng; ++i)
+                        D.at(i, j, k, dc + n) =
+                            a * X.at(i, j, k, xc + n) + bb * Y.at(i, j, k, yc + n);
+    }
+}
+
+/** Same-level ghost fill with periodic wrap (FillBoundary, int_dir BCs).
+ *  Ghost cells whose periodic image lies in no box keep their values. */
+inline void fill_boundary(MultiFab &m, int N) {
+    if (m.g == 0) return;
+    auto wrap = [N](int v) { return ((v % N) + N) % N; };
+    for (size_t b = 0; b < m.fabs.size(); ++b) {
+        Fab &D = m.fabs[b];
+        const int g = D.g;
+#pragma omp parallel for schedule(static)
+        for (int k = D.box.lo[2] - g; k <= D.box.hi[2] + g; ++k)
+            for (int j = D.box.lo[1] - g; j <= D.box.hi[1] + g; ++j)
+                for (int i = D.box.lo[0] - g; i <= D.box.hi[0] + g; ++i) {
+                    if (i >= D.box.lo[0] && i <= D.box.hi[0] && j >= D.box.lo[1] &&
+                        j <= D.box.hi[1] && k >= D.box.lo[2] && k <= D.box.hi[2])
+                        continue;
+                    const int wi = wrap(i), wj = wrap(j), wk = wrap(k);
+                    for (size_t s = 0; s < m.fabs.size(); ++s) {
+                        const Box &sb = m.fabs[s].box;
+                        // Shift of the image that lands in box s.
+                        int si = wi, sj = wj, sk = wk;
+                        if (!(si >= sb.lo[0] && si <= sb.hi[0] && sj >= sb.lo[1] &&
+                              sj <= sb.hi[1] && sk >= sb.lo[2] && sk <= sb.hi[2]))
+                            continue;
+                        for (int n = 0; n < m.nc; ++n)
+                            D.at(i, j, k, n) = m.fabs[s].at(si, sj, sk, n);
+                        break;
+                    }
+                }
+    }
+}
+
+/** Mean of the 8 children (amrex::average_down) of every coarse cell that
+ *  lies under a fine box; coarse boxes are the fine boxes coarsened by 2. */
+inline void average_down(MultiFab &fine, MultiFab &crse) {
+    for (size_t b = 0; b < fine.fabs.size(); ++b) {
+        Fab &F = fine.fabs[b];
+        Fab &Cf = crse.fabs[b];
+#pragma omp parallel for schedule(static)
+        for (int k = F.box.lo[2] / 2; k <= (F.box.hi[2] + 1) / 2 - 1; ++k)
+            for (int j = F.box.lo[1] / 2; j <= (F.box.hi[1] + 1) / 2 - 1; ++j)
+                for (int i = F.box.lo[0] / 2; i <= (F.box.hi[0] + 1) / 2 - 1; ++i)
+                    for (int n = 0; n < fine.nc; ++n) {
+                        double c = 0;
+                        for (int dk = 0; dk < 2; ++dk)
+                            for (int dj = 0; dj < 2; ++dj)
+                                for (int di = 0; di < 2; ++di)
+                                    c += F.at(2 * i + di, 2 * j + dj, 2 * k + dk, n);
+                        Cf.at(i, j, k, n) = c * 0.125;
+                    }
+    }
+}
+
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+inline double rng_uniform(uint64_t seed, uint64_t stream, uint64_t idx) {
+    const uint64_t h =
+        splitmix64(splitmix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) + idx);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/** Fourier symbol of Gradient<p> at index a of an N-point axis
+ *  (Sledgehamr.py:18-27; p = 3 from the Gradient<3> weights). */
+inline double index_to_k(int a, int N, int type) {
+    const int nt = a > N / 2 - 1 ? a - N : a;
+    const double th = 2. * M_PI / N * nt;
+    if (type == 1) return std::sin(th);
+    if (type == 2) return (8. * std::sin(th) - std::sin(2. * th)) / 6.;
+    if (type == 3)
+        return (45. * std::sin(th) - 9. * std::sin(2. * th) + std::sin(3. * th)) / 30.;
+    return 0.;
+}
+
+/** Sorted unique i^2 + j^2 + k^2 over fftfreq(N)*N (Sledgehamr.py:7-12). */
+inline std::vector<long long> unique_k2(int N) {
+    std::set<long long> sq;
+    for (int a = 0; a < N; ++a) {
+        const long long f = a < (N + 1) / 2 ? a : a - N;
+        sq.insert(f * f);
+    }
+    std::set<long long> out;
+    for (long long x : sq)
+        for (long long y : sq)
+            for (long long z : sq) out.insert(x + y + z);
+    return std::vector<long long>(out.begin(), out.end());
+}
+
+/** Unnormalised forward DFT of a real N^3 array (x fastest) keeping
+ *  x-frequencies 0..N/2 (the R2C layout, x halved); separable O(N^4). */
+inline void r2c(const std::vector<double> &in, int N,
+                std::vector<std::complex<double>> &out) {
+    const int NH = N / 2 + 1;
+    std::vector<std::complex<double>> tw(N);
+    for (int m = 0; m < N; ++m)
+        tw[m] = std::polar(1.0, -2. * M_PI * (double)m / (double)N);
+    std::vector<std::complex<double>> a((size_t)NH * N * N);
+    // x axis
+#pragma omp parallel for schedule(static)
+    for (long long r = 0; r < (long long)N * N; ++r)
+        for (int kx = 0; kx < NH; ++kx) {
+            std::complex<double> s = 0;
+            for (int x = 0; x < N; ++x)
+                s += in[(size_t)(r * N + x)] * tw[(size_t)((long long)kx * x % N)];
+            a[(size_t)(r * NH + kx)] = s;
+        }
+    // y axis
+    std::vector<std::complex<double>> b(a.size());
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < N; ++z)
+        for (int ky = 0; ky < N; ++ky)
+            for (int kx = 0; kx < NH; ++kx) {
+                std::complex<double> s = 0;
+                for (int y = 0; y < N; ++y)
+                    s += a[(size_t)(((long long)z * N + y) * NH + kx)] *
+                         tw[(size_t)((long long)ky * y % N)];
+                b[(size_t)(((long long)z * N + ky) * NH + kx)] = s;
+            }
+    // z axis
+    out.assign(a.size(), 0.);
+#pragma omp parallel for schedule(static)
+    for (int kz = 0; kz < N; ++kz)
+        for (int ky = 0; ky < N; ++ky)
+            for (int kx = 0; kx < NH; ++kx) {
+                std::complex<double> s = 0;
+                for (int z = 0; z < N; ++z)
+                    s += b[(size_t)(((long long)z * N + ky) * NH + kx)] *
+                         tw[(size_t)((long long)kz * z % N)];
+                out[(size_t)(((long long)kz * N + ky) * NH + kx)] = s;
+            }
+}
+
+/** gw_GetProjection / gw_GetLambda (gravitational_waves.h:101-133). */
+inline double gw_projection(int i, int j, const double abc[3]) {
+    const double norm = abc[0] * abc[0] + abc[1] * abc[1] + abc[2] * abc[2];
+    const int l = std::min(i, j), m = std::max(i, j);
+    return (double)(l == m) - abc[l] * abc[m] / norm;
+}
+inline double gw_lambda(int i, int j, int l, int m, const int abc[3],
+                        const double *itk) {
+    if (abc[0] == 0 && abc[1] == 0 && abc[2] == 0) return 0.;
+    const double d[3] = {itk[abc[0]], itk[abc[1]], itk[abc[2]]};
+    return gw_projection(i, l, d) * gw_projection(j, m, d) -
+           gw_projection(i, j, d) * gw_projection(l, m, d) / 2.;
+}
+
+// ---------------------------------------------------------------------------
+
+struct Config {
+    int N0 = 0, max_level = 0, nghost = 0;
+    double L = 0, cfl = 0, t_start = 0, t_end = 0;
+    bool gw = false;
+    int integrator = 10;
+    std::vector<double> nodes, bbar, b;
+    std::vector<std::vector<double>> tableau;
+    std::vector<double> te_crit, ko;
+    bool ko_on = false;
+    int ko_order = 0;
+    bool shadow_hierarchy = false;
+};
+
+template <class Proj>
+class Sim {
+    using Ops = typename Proj::OrcOps;
+
+  public:
+    Proj prj;
+    Config cfg;
+    int nc = 0;
+    std::vector<int> dimN;
+    std::vector<double> dx, dt;
+    std::vector<MultiFab> grid_new, grid_old;
+    MultiFab shadow, shadow_tmp;
+    std::vector<double> rhs_te;  // last truncation errors (per coarse cell)
+
+    explicit Sim(const std::string &inputs) {
+        auto &T = amrex::ParmParse::table();
+        T.clear();
+        std::istringstream in(inputs);
+        std::string line;
+        while (std::getline(in, line)) {
+            const size_t h = line.find('#');
+            if (h != std::string::npos) line = line.substr(0, h);
+            const size_t e = line.find('=');
+            if (e == std::string::npos) continue;
+            std::string key = line.substr(0, e);
+            key.erase(0, key.find_first_not_of(" \t"));
+            key.erase(key.find_last_not_of(" \t\r") + 1);
+            std::istringstream vs(line.substr(e + 1));
+            std::vector<std::string> v;
+            std::string tok;
+            while (vs >> tok) v.push_back(tok);
+            T[key] = v;
+        }
+        amrex::ParmParse pp("");
+        pp.get("amr.coarse_level_grid_size", cfg.N0);
+        pp.query("amr.max_refinement_levels", cfg.max_level);
+        pp.query("amr.nghost", cfg.nghost);
+        pp.get("sim.L", cfg.L);
+        pp.get("sim.cfl", cfg.cfl);
+        pp.get("sim.t_start", cfg.t_start);
+        pp.get("sim.t_end", cfg.t_end);
+        pp.query("sim.gravitational_waves", cfg.gw);
+        pp.get("integrator.type", cfg.integrator);
+        nc = cfg.gw ? Ops::kNGw : Ops::kNScalars;
+        std::vector<std::string> names;
+        for (auto *f : prj.scalar_fields) names.push_back(f->name);
+        for (const char *n : {"u_xx", "u_yy", "u_zz", "u_xy", "u_xz", "u_yz", "du_xx",
+                              "du_yy", "du_zz", "du_xy", "du_xz", "du_yz"})
+            names.push_back(n);
+        double ted = DBL_MAX;
+        pp.query("amr.te_crit", ted);
+        cfg.te_crit.assign(nc, ted);
+        for (int n = 0; n < nc; ++n) {
+            pp.query(("amr.te_crit_" + names[n]).c_str(), cfg.te_crit[n]);
+            if (cfg.te_crit[n] != DBL_MAX) cfg.shadow_hierarchy = true;
+        }
+        double kod = 0;
+        pp.query("sim.dissipation_strength", kod);
+        cfg.ko.assign(nc, kod);
+        for (int n = 0; n < nc; ++n) {
+            pp.query(("sim.dissipation_strength_" + names[n]).c_str(), cfg.ko[n]);
+            if (cfg.ko[n] > 0) cfg.ko_on = true;
+        }
+        if (cfg.ko_on) {
+            cfg.ko_order = cfg.nghost;
+            pp.query("sim.dissipation_order", cfg.ko_order);
+        }
+        switch (cfg.integrator) {
+            case 20: {
+                amrex::ParmParse pr("integrator.rkn");
+                pr.getarr("weights_bar_b", cfg.bbar);
+                pr.getarr("weights_b", cfg.b);
+                pr.getarr("nodes", cfg.nodes);
+                std::vector<double> bt;
+                pr.getarr("tableau", bt);
+                int k = 0;
+                for (size_t i = 0; i < cfg.nodes.size(); ++i) {
+                    std::vector<double> row;
+                    for (size_t j = 0; j <= i; ++j) row.push_back(bt[k++]);
+                    cfg.tableau.push_back(row);
+                }
+                break;
+            }
+            case 21:
+                cfg.nodes = {0.0, 0.5, 1.0};
+                cfg.tableau = {{0.0}, {1. / 8., 0.0}, {0.0, 0.5, 0.0}};
+                cfg.bbar = {1. / 6., 1. / 3., 0.0};
+                cfg.b = {1. / 6., 4. / 6., 1. / 6.};
+                break;
+            case 22:
+                cfg.nodes = {0.0, 1. / 5., 2. / 3., 1.0};
+                cfg.tableau = {{0.0},
+                               {1. / 50., 0.0},
+                               {-1. / 27., 7. / 27., 0.0},
+                               {3. / 10., -2. / 35., 9. / 35., 0.0}};
+                cfg.bbar = {14. / 336., 100. / 336., 54. / 336., 0.0};
+                cfg.b = {14. / 336., 125. / 336., 162. / 336., 35. / 336.};
+                break;
+        }
+        for (int lev = 0; lev <= cfg.max_level; ++lev) {
+            dimN.push_back(cfg.N0 * (1 << lev));
+            dx.push_back(cfg.L / (double)dimN[lev]);
+            dt.push_back(dx[lev] * cfg.cfl);
+        }
+        grid_new.resize(cfg.max_level + 1);
+        grid_old.resize(cfg.max_level + 1);
+        prj.Init();
+        int mgs = cfg.N0;
+        pp.query("amr.max_grid_size", mgs);
+        std::vector<Box> boxes;
+        for (int z = 0; z < cfg.N0; z += mgs)
+            for (int y = 0; y < cfg.N0; y += mgs)
+                for (int x = 0; x < cfg.N0; x += mgs)
+                    boxes.push_back(Box{{x, y, z}, {x + mgs - 1, y + mgs - 1, z + mgs - 1}});
+        set_level(0, boxes);
+    }
+
+    void set_level(int lev, const std::vector<Box> &boxes) {
+        grid_new[lev].define(boxes, nc, cfg.nghost);
+        grid_old[lev].define(boxes, nc, cfg.nghost);
+        grid_new[lev].t = cfg.t_start;
+        grid_old[lev].t = -DBL_MAX;
+    }
+
+    int finest() const {
+        int f = 0;
+        for (int l = 0; l <= cfg.max_level; ++l)
+            if (grid_new[l].defined()) f = l;
+        return f;
+    }
+
+    std::vector<double> params_rhs(double t, int lev) {
+        std::vector<double> p;
+        prj.SetParamsRhs(p, t, lev);
+        p.resize(std::max<size_t>(p.size(), 64), 0.);
+        return p;
+    }
+    std::vector<double> params_gw(double t, int lev) {
+        std::vector<double> p;
+        prj.SetParamsGravitationalWaveRhs(p, t, lev);
+        p.resize(std::max<size_t>(p.size(), 64), 0.);
+        return p;
+    }
+
+    /** FillRhs / FillAddRhs (macros.h:301-485): per-cell Rhs, GW source and
+     *  backreaction, Kreiss-Oliger; with add: rhs = f + weight*rhs_old. */
+    void fill_rhs(MultiFab &rhs, MultiFab &state, double time, int lev, double dtl,
+                  double dxl, bool add, double weight) {
+        const std::vector<double> p = params_rhs(time, lev);
+        const std::vector<double> pg = params_gw(time, lev);
+        const int ncomp = nc;
+        for (size_t b = 0; b < state.fabs.size(); ++b) {
+            Fab &S = state.fabs[b];
+            Fab &Rf = rhs.fabs[b];
+            const amrex::Array4<const double> sa = S.carr();
+            const amrex::Array4<double> ra = Rf.arr();
+            // 1024 x 8 x 8 tiles (MFIter tiling on CPU).
+            const int tz = 8, ty = 8, tx = 1024;
+            std::vector<std::array<int, 3>> tiles;
+            for (int k0 = S.box.lo[2]; k0 <= S.box.hi[2]; k0 += tz)
+                for (int j0 = S.box.lo[1]; j0 <= S.box.hi[1]; j0 += ty)
+                    for (int i0 = S.box.lo[0]; i0 <= S.box.hi[0]; i0 += tx)
+                        tiles.push_back({i0, j0, k0});
+#pragma omp parallel for schedule(dynamic)
+            for (size_t t = 0; t < tiles.size(); ++t) {
+                const auto [i0, j0, k0] = tiles[t];
+                for (int k = k0; k <= std::min(S.box.hi[2], k0 + tz - 1); ++k)
+                    for (int j = j0; j <= std::min(S.box.hi[1], j0 + ty - 1); ++j)
+                        for (int i = i0; i <= std::min(S.box.hi[0], i0 + tx - 1); ++i) {
+                            double tmp[64];
+                            if (add)
+                                for (int n = 0; n < ncomp; ++n) tmp[n] = ra(i, j, k, n);
+                            Ops::rhs(ra, sa, i, j, k, lev, time, dtl, dxl, p.data());
+                            if (cfg.gw) {
+                                Ops::gw_rhs(ra, sa, i, j, k, lev, time, dtl, dxl,
+                                            pg.data());
+                                Ops::gw_back(ra, sa, i, j, k, lev, time, dtl, dxl,
+                                             p.data(), pg.data());
+                            }
+                            if (cfg.ko_on) {
+                                for (int n = 0; n < ncomp; ++n)
+                                    ra(i, j, k, n) +=
+                                        cfg.ko_order == 2
+                                            ? sledgehamr::kernels::KreissOligerDissipation<2>(
+                                                  sa, i, j, k, n, dxl, cfg.ko[n])
+                                            : sledgehamr::kernels::KreissOligerDissipation<3>(
+                                                  sa, i, j, k, n, dxl, cfg.ko[n]);
+                            }
+                            if (add)
+                                for (int n = 0; n < ncomp; ++n)
+                                    ra(i, j, k, n) += weight * tmp[n];
+                        }
+            }
+        }
+    }
+
+    void fill_patch(MultiFab &m, int lev) {
+        fill_boundary(m, lev < 0 ? cfg.N0 / 2 : dimN[lev]);
+    }
+
+    /** IntegratorLsssprk3::Integrate (lsssprk3.cpp:14-31). */
+    void lsssprk3(MultiFab &mo, MultiFab &mn, int lev, double dtl, double dxl) {
+        const double t0 = mo.t, t1 = t0 + dtl;
+        MultiFab k1;
+        k1.define(boxes_of(mo), nc, cfg.nghost);
+        fill_rhs(k1, mo, t0, lev, dtl, dxl, false, 0.);
+        LinComb(mn, 1, mo, 0, dtl, k1, 0, 0, nc, 0);
+        fill_patch(mn, lev);
+        fill_rhs(k1, mn, t1, lev, dtl, dxl, true, 1.);
+        LinComb(mn, 1, mo, 0, dtl / 4., k1, 0, 0, nc, 0);
+        fill_patch(mn, lev);
+        fill_rhs(k1, mn, t0 + dtl / 2., lev, dtl, dxl, true, 0.25);
+        LinComb(mn, 1, mo, 0, dtl * 2. / 3., k1, 0, 0, nc, 0);
+        fill_patch(mn, lev);
+    }
+
+    /** IntegratorRkn::Integrate (rkn.cpp:91-186). */
+    void rkn(MultiFab &mo, MultiFab &mn, int lev, double dtl, double dxl) {
+        const double t = mo.t;
+        const int ng = cfg.nghost;
+        const int N = nc, Ngrav = cfg.gw ? 12 : 0, Nf = N - Ngrav;
+        const int uN = Nf / 2, uN0 = 0, uN1 = uN;
+        const int gN = Ngrav / 2, gN0 = Nf, gN1 = gN0 + gN;
+        const int s = (int)cfg.nodes.size();
+        std::vector<MultiFab> F(s);
+        for (auto &f : F) f.define(boxes_of(mo), N, ng);
+        for (int i = 0; i < s; ++i) {
+            const double st = t + dtl * cfg.nodes[i];
+            Copy(mn, mo, 0, 0, N, ng);
+            Saxpy(mn, dtl * cfg.nodes[i], mo, uN1, uN0, uN, ng);
+            if (Ngrav > 0) Saxpy(mn, dtl * cfg.nodes[i], mo, gN1, gN0, gN, ng);
+            for (int j = 0; j < i; ++j) {
+                Saxpy(mn, dtl * dtl * cfg.tableau[i][j], F[j], uN1, uN0, uN, ng);
+                if (Ngrav > 0)
+                    Saxpy(mn, dtl * dtl * cfg.tableau[i][j], F[j], gN1, gN0, gN, ng);
+            }
+            fill_patch(mn, lev);
+            fill_rhs(F[i], mn, st, lev, dtl, dxl, false, 0.);
+        }
+        Copy(mn, mo, 0, 0, N, ng);
+        Saxpy(mn, dtl, mn, uN1, uN0, uN, ng);
+        if (Ngrav > 0) Saxpy(mn, dtl, mn, gN1, gN0, gN, ng);
+        for (int i = 0; i < s; ++i) {
+            Saxpy(mn, dtl * dtl * cfg.bbar[i], F[i], uN1, uN0, uN, ng);
+            Saxpy(mn, dtl * cfg.b[i], F[i], uN1, uN1, uN, ng);
+            if (Ngrav == 0) continue;
+            Saxpy(mn, dtl * dtl * cfg.bbar[i], F[i], gN1, gN0, gN, ng);
+            Saxpy(mn, dtl * cfg.b[i], F[i], gN1, gN1, gN, ng);
+        }
+        fill_patch(mn, lev);
+    }
+
+    /** IntegratorLeapfrog::Integrate (leapfrog.cpp:13-57). */
+    void leapfrog(MultiFab &mo, MultiFab &mn, int lev, double dtl, double dxl) {
+        const double t0 = mo.t, t1 = t0 + dtl;
+        const int N = nc, Ngrav = cfg.gw ? 12 : 0, Nf = N - Ngrav;
+        const int uN = Nf / 2, uN0 = 0, uN1 = uN;
+        const int gN = Ngrav / 2, gN0 = Nf, gN1 = gN0 + gN;
+        MultiFab a, vh;
+        a.define(boxes_of(mo), N, cfg.nghost);
+        vh.define(boxes_of(mo), N, cfg.nghost);
+        fill_rhs(a, mo, t0, lev, dtl, dxl, false, 0.);
+        LinComb(vh, 1, mo, 0, dtl / 2., a, 0, 0, N, 0);
+        LinComb(mn, 1, mo, uN0, dtl, vh, uN1, uN0, uN, 0);
+        if (Ngrav > 0) LinComb(mn, 1, mo, gN0, dtl, vh, gN1, gN0, gN, 0);
+        fill_patch(mn, lev);
+        fill_rhs(a, mn, t1, lev, dtl, dxl, false, 0.);
+        LinComb(mn, 1, vh, uN1, dtl / 2., a, uN1, uN1, uN, 0);
+        if (Ngrav > 0) LinComb(mn, 1, vh, gN1, dtl / 2., a, gN1, gN1, gN, 0);
+        fill_patch(mn, lev);
+    }
+
+    static std::vector<Box> boxes_of(const MultiFab &m) {
+        std::vector<Box> b;
+        for (auto &f : m.fabs) b.push_back(f.box);
+        return b;
+    }
+
+    void integrate(MultiFab &mo, MultiFab &mn, int lev, double dtl, double dxl) {
+        if (cfg.integrator == 10)
+            lsssprk3(mo, mn, lev, dtl, dxl);
+        else if (cfg.integrator == 11)
+            leapfrog(mo, mn, lev, dtl, dxl);
+        else
+            rkn(mo, mn, lev, dtl, dxl);
+    }
+
+    /** Integrator::Advance (integrator.cpp:11-31). */
+    void advance(int lev) {
+        if (lev >= 0) std::swap(grid_old[lev], grid_new[lev]);
+        const double dtl = lev < 0 ? dt[0] * 2. : dt[lev];
+        const double dxl = lev < 0 ? dx[0] * 2. : dx[lev];
+        MultiFab &mo = lev < 0 ? shadow_tmp : grid_old[lev];
+        MultiFab &mn = lev < 0 ? shadow : grid_new[lev];
+        if (!mn.defined()) mn.define(boxes_of(mo), nc, cfg.nghost);
+        fill_patch(mo, lev);
+        integrate(mo, mn, lev, dtl, dxl);
+        mn.t = mo.t + dtl;
+        mn.istep = mo.istep + 1;
+    }
+
+    /** Sledgehamr::CreateShadowLevel (sledgehamr.cpp:234-251). */
+    void create_shadow() {
+        std::vector<Box> cb;
+        for (auto &f : grid_old[0].fabs) {
+            Box b;
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = f.box.lo[a] / 2;
+                b.hi[a] = (f.box.hi[a] + 1) / 2 - 1;
+            }
+            cb.push_back(b);
+        }
+        shadow.define(cb, nc, cfg.nghost);
+        shadow_tmp.define(cb, nc, cfg.nghost);
+        shadow_tmp.t = grid_old[0].t;
+        average_down(grid_old[0], shadow_tmp);
+        advance(-1);
+    }
+
+    /** AverageDownWithTruncationError over every coarse cell under the fine
+     *  boxes (kernels.h:18-42; level_synchronizer.cpp:262-299), crse being
+     *  the shadow level (lev 0) or level lev-1.  te is stored per fine box at
+     *  even fine indices (here: per coarse cell, ncomp planes). */
+    void te_avgdown(int lev, std::vector<double> &te, bool avgdown) {
+        MultiFab &fine = grid_new[lev];
+        const long long ncoarse = [&] {
+            long long s = 0;
+            for (auto &f : fine.fabs) s += f.box.vol() / 8;
+            return s;
+        }();
+        te.assign((size_t)ncoarse * nc, 0.);
+        long long off = 0;
+        for (size_t b = 0; b < fine.fabs.size(); ++b) {
+            Fab &F = fine.fabs[b];
+            const int cnx = F.box.n(0) / 2, cny = F.box.n(1) / 2, cnz = F.box.n(2) / 2;
+            for (int k = 0; k < cnz; ++k)
+                for (int j = 0; j < cny; ++j)
+                    for (int i = 0; i < cnx; ++i) {
+                        const int ci = F.box.lo[0] / 2 + i, cj = F.box.lo[1] / 2 + j,
+                                  ck = F.box.lo[2] / 2 + k;
+                        const int ii = 2 * ci, jj = 2 * cj, kk = 2 * ck;
+                        double *crse_cell[64];
+                        for (int n = 0; n < nc; ++n) crse_cell[n] = coarse_ptr(lev, ci, cj, ck, n);
+                        for (int n = 0; n < nc; ++n) {
+                            const double volfrac = 1.0 / 8.0;
+                            double c = F.at(ii, jj, kk, n);
+                            c += F.at(ii + 1, jj, kk, n);
+                            c += F.at(ii, jj + 1, kk, n);
+                            c += F.at(ii, jj, kk + 1, n);
+                            c += F.at(ii + 1, jj + 1, kk, n);
+                            c += F.at(ii + 1, jj, kk + 1, n);
+                            c += F.at(ii, jj + 1, kk + 1, n);
+                            c += F.at(ii + 1, jj + 1, kk + 1, n);
+                            const double avg = volfrac * c;
+                            te[(size_t)(n * ncoarse + off + i + (long long)cnx * (j + (long long)cny * k))] =
+                                std::fabs(*crse_cell[n] - avg);
+                            if (avgdown) *crse_cell[n] = avg;
+                        }
+                    }
+            off += (long long)cnx * cny * cnz;
+        }
+    }
+
+    double *coarse_ptr(int lev, int ci, int cj, int ck, int n) {
+        MultiFab &C = lev == 0 ? shadow : grid_new[lev - 1];
+        for (auto &f : C.fabs)
+            if (ci >= f.box.lo[0] && ci <= f.box.hi[0] && cj >= f.box.lo[1] &&
+                cj <= f.box.hi[1] && ck >= f.box.lo[2] && ck <= f.box.hi[2])
+                return &f.at(ci, cj, ck, n);
+        throw std::runtime_error("coarse cell not covered");
+    }
+
+    /** TagWithTruncationCpu (macros.h:489-566) executed literally: per tile
+     *  and cell in k, j, i order, CLEAR own cell, user tag, TE block SET.
+     *  mask holds one byte per fine valid cell (box-major, x fastest). */
+    void tag(int lev, const std::vector<double> &te, bool with_te,
+             std::vector<uint8_t> &mask, std::vector<long long> &counts) {
+        MultiFab &st = grid_new[lev];
+        const double time = st.t;
+        std::vector<double> ptag, pmod;
+        prj.SetParamsTagCellForRefinement(ptag, time, lev);
+        if (cfg.shadow_hierarchy) prj.SetParamsTruncationModifier(pmod, time, lev);
+        ptag.resize(std::max<size_t>(ptag.size(), 1));
+        pmod.resize(std::max<size_t>(pmod.size(), 1));
+        long long total = 0, user = 0;
+        std::vector<long long> trunc(nc, 0);
+        long long nvalid = 0;
+        for (auto &f : st.fabs) nvalid += f.box.vol();
+        mask.assign((size_t)nvalid, 0);
+        long long moff = 0, coff = 0;
+        long long ncoarse = nvalid / 8;
+        for (size_t b = 0; b < st.fabs.size(); ++b) {
+            Fab &F = st.fabs[b];
+            const amrex::Array4<const double> sa = F.carr();
+            const Box &bx = F.box;
+            const int nx = bx.n(0), ny = bx.n(1);
+            auto mk = [&](int i, int j, int k) -> uint8_t & {
+                return mask[(size_t)(moff + (i - bx.lo[0]) +
+                                     (long long)nx * ((j - bx.lo[1]) +
+                                                      (long long)ny * (k - bx.lo[2])))];
+            };
+            const int cnx = nx / 2, cny = ny / 2;
+            for (int k0 = bx.lo[2]; k0 <= bx.hi[2]; k0 += 8)
+                for (int j0 = bx.lo[1]; j0 <= bx.hi[1]; j0 += 8)
+                    for (int k = k0; k <= std::min(bx.hi[2], k0 + 7); ++k)
+                        for (int j = j0; j <= std::min(bx.hi[1], j0 + 7); ++j)
+                            for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+                                mk(i, j, k) = 0;  // CLEAR
+                                const bool res = Ops::tag(sa, i, j, k, lev, time, dt[lev],
+                                                          dx[lev], ptag.data());
+                                if (res) {
+                                    mk(i, j, k) = 1;
+                                    ++user;
+                                    ++total;
+                                }
+                                if (!with_te) continue;
+                                bool te_res = false;
+                                if (!((i % 2 != 0) || (j % 2 != 0) || (k % 2 != 0))) {
+                                    const long long cidx =
+                                        coff + (i - bx.lo[0]) / 2 +
+                                        (long long)cnx * ((j - bx.lo[1]) / 2 +
+                                                          (long long)cny * ((k - bx.lo[2]) / 2));
+                                    sledgehamr::utils::constexpr_for<0, 64, 1>([&](auto n) {
+                                        if constexpr (n < Ops::kNGw) {
+                                            if (n >= nc) return;
+                                            const double m = Ops::template te_mod<n>(
+                                                sa, i, j, k, lev, time, dt[lev], dx[lev],
+                                                te[(size_t)(n * ncoarse + cidx)], pmod.data());
+                                            if (m >= cfg.te_crit[n]) {
+                                                te_res = true;
+                                                trunc[n] += 8;
+                                            }
+                                        }
+                                    });
+                                }
+                                if (te_res) {
+                                    for (int d = 0; d < 8; ++d)
+                                        mk(i + (d & 1), j + ((d >> 1) & 1), k + (d >> 2)) = 1;
+                                    total += 8 - (int)res;
+                                }
+                            }
+            moff += bx.vol();
+            coff += bx.vol() / 8;
+        }
+        counts.assign(2 + nc, 0);
+        counts[0] = total;
+        counts[1] = user;
+        for (int n = 0; n < nc; ++n) counts[2 + n] = trunc[n];
+    }
+
+    /** GravitationalWaves::ComputeSpectrum, CPU branch.  mode 0 unbinned
+     *  (lower_bound over unique |k|^2), 1 rounded |k|, 2 log bins (new). */
+    void gw_spectrum(int mode, int nbins, int zp, int proj, std::vector<double> &spec,
+                     std::vector<double> &kv) {
+        const int N = cfg.N0 * zp;
+        const int NS = Ops::kNScalars;
+        const int comps[6] = {NS + 6, NS + 9, NS + 10, NS + 7, NS + 11, NS + 8};
+        const int mat[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        std::vector<std::vector<std::complex<double>>> u(6);
+        for (int c = 0; c < 6; ++c) {
+            std::vector<double> dense((size_t)N * N * N, 0.);
+            for (auto &f : grid_new[0].fabs)
+                for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
+                    for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
+                        for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i)
+                            dense[(size_t)(i + (long long)N * (j + (long long)N * k))] =
+                                f.at(i, j, k, comps[c]);
+            r2c(dense, N, u[c]);
+        }
+        std::vector<double> itk(N);
+        for (int a = 0; a < N; ++a) itk[a] = index_to_k(a, N, proj);
+        std::vector<long long> ks;
+        kv.clear();
+        if (mode == 0) {
+            ks = unique_k2(N);
+            for (long long v : ks) kv.push_back((double)v);
+        } else if (mode == 1) {
+            for (int k = 0; k <= std::sqrt(3.) / 2. * (double)N + .5; ++k) {
+                ks.push_back((long long)k * k);
+                kv.push_back((double)k * k);
+            }
+        } else {
+            const double lk = std::log(std::sqrt(3.) * N / 2.);
+            for (int i = 0; i < nbins; ++i) kv.push_back(std::exp(lk * i / nbins));
+        }
+        const int kmax = mode == 2 ? nbins : (int)ks.size();
+        spec.assign(kmax, 0.);
+        const int NH = N / 2 + 1;
+        for (int c = 0; c < N; ++c)
+            for (int b = 0; b < N; ++b)
+                for (int a = 0; a < NH; ++a) {
+                    const double multpl = (a == 0 || a == N / 2) ? 1. : 2.;
+                    const int abc[3] = {a, b, c};
+                    const int li = a >= N / 2 ? a - N : a;
+                    const int lj = b >= N / 2 ? b - N : b;
+                    const int lk = c >= N / 2 ? c - N : c;
+                    const unsigned sq = li * li + lj * lj + lk * lk;
+                    long index;
+                    if (mode == 0) {
+                        index = std::lower_bound(ks.begin(), ks.end(), (long long)sq) - ks.begin();
+                    } else if (mode == 1) {
+                        index = (long)(std::sqrt((double)sq) + .5);
+                    } else {
+                        if (sq == 0) continue;
+                        index = (long)std::floor((double)nbins * 0.5 * std::log((double)sq) /
+                                                 std::log(std::sqrt(3.) * N / 2.));
+                        index = std::max(0L, std::min((long)nbins - 1, index));
+                    }
+                    const size_t e = (size_t)(((long long)c * N + b) * NH + a);
+                    double running = 0;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j)
+                            for (int l = 0; l < 3; ++l)
+                                for (int m = 0; m < 3; ++m)
+                                    running += gw_lambda(i, j, l, m, abc, itk.data()) * multpl *
+                                               (u[mat[i][j]][e].real() * u[mat[l][m]][e].real() +
+                                                u[mat[i][j]][e].imag() * u[mat[l][m]][e].imag());
+                    if (index >= 0 && index < kmax) spec[index] += running;
+                }
+        const double n6 = std::pow((double)N, 6);
+        for (double &v : spec) v /= n6;
+    }
+
+    /** Seeded complex random phase, identical to the device generator. */
+    void init_complex_random_phase(int lev, uint64_t seed, double sigma) {
+        MultiFab &m = grid_new[lev];
+        const int N = dimN[lev];
+        for (auto &f : m.fabs)
+            for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
+                for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
+                    for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i) {
+                        const uint64_t g = (uint64_t)i + (uint64_t)N * ((uint64_t)j + (uint64_t)N * (uint64_t)k);
+                        const double u1 = rng_uniform(seed, 1, g);
+                        const double u2 = rng_uniform(seed, 2, g);
+                        const double u3 = rng_uniform(seed, 3, g);
+                        const double theta = 2. * M_PI * u1;
+                        const double gg = std::sqrt(-2. * std::log(1. - u2)) * std::cos(2. * M_PI * u3);
+                        const double mm = 1. + sigma * gg;
+                        for (int n = 0; n < nc; ++n) f.at(i, j, k, n) = 0.;
+                        f.at(i, j, k, 0) = mm * std::cos(theta);
+                        f.at(i, j, k, 1) = mm * std::sin(theta);
+                    }
+    }
+};
+
+}  // namespace orc

@@ -1,0 +1,178 @@
+// ORACLE -- test infrastructure only (see oracle.hpp).
+// C entry points of the FP64 oracle for one project, mirroring the product's
+// C ABI so tests drive both with the same call sequence.
+#pragma once
+
+#include <cstring>
+#include <string>
+
+#include "oracle.hpp"
+
+#define ORC_EXPORT extern "C" __attribute__((visibility("default")))
+
+#define ORC_DEFINE_API(Proj)                                                   \
+    using OrcSim = orc::Sim<Proj>;                                             \
+    static thread_local std::string orc_err;                                   \
+    template <class F>                                                         \
+    static int orc_guard(F &&f) {                                              \
+        try {                                                                  \
+            f();                                                               \
+            return 0;                                                          \
+        } catch (const std::exception &e) {                                    \
+            orc_err = e.what();                                                \
+            return 1;                                                          \
+        }                                                                      \
+    }                                                                          \
+    static orc::MultiFab &orc_level(OrcSim *s, int lev);                       \
+    static long long orc_cells(OrcSim *s, int lev) {                          \
+        long long n = 0;                                                       \
+        for (auto &f : orc_level(s, lev).fabs) n += f.box.vol();              \
+        return n;                                                              \
+    }                                                                          \
+    template <class F>                                                         \
+    static void orc_valid(orc::MultiFab &m, F &&f) {                           \
+        long long o = 0;                                                       \
+        for (auto &fb : m.fabs)                                                \
+            for (int k = fb.box.lo[2]; k <= fb.box.hi[2]; ++k)                 \
+                for (int j = fb.box.lo[1]; j <= fb.box.hi[1]; ++j)             \
+                    for (int i = fb.box.lo[0]; i <= fb.box.hi[0]; ++i)         \
+                        f(fb, i, j, k, o++);                                   \
+    }                                                                          \
+    static orc::MultiFab &orc_level(OrcSim *s, int lev) {                      \
+        if (lev >= 0) return s->grid_new[lev];                                 \
+        if (!s->shadow.defined()) {                                            \
+            std::vector<orc::Box> cb;                                          \
+            for (auto &f : s->grid_new[0].fabs) {                              \
+                orc::Box b;                                                    \
+                for (int a = 0; a < 3; ++a) {                                  \
+                    b.lo[a] = f.box.lo[a] / 2;                                 \
+                    b.hi[a] = (f.box.hi[a] + 1) / 2 - 1;                       \
+                }                                                              \
+                cb.push_back(b);                                               \
+            }                                                                  \
+            s->shadow.define(cb, s->nc, s->cfg.nghost);                        \
+            s->shadow.t = s->grid_new[0].t;                                    \
+        }                                                                      \
+        return s->shadow;                                                      \
+    }                                                                          \
+    static orc::MultiFab orc_rhs;                                              \
+    ORC_EXPORT const char *orc_last_error() { return orc_err.c_str(); }        \
+    ORC_EXPORT void *orc_create(const char *inputs) {                          \
+        try {                                                                  \
+            return new OrcSim(inputs);                                         \
+        } catch (const std::exception &e) {                                    \
+            orc_err = e.what();                                                \
+            return nullptr;                                                    \
+        }                                                                      \
+    }                                                                          \
+    ORC_EXPORT void orc_destroy(void *h) { delete static_cast<OrcSim *>(h); }  \
+    ORC_EXPORT int orc_ncomp(void *h) { return static_cast<OrcSim *>(h)->nc; } \
+    ORC_EXPORT long long orc_ncells(void *h, int lev) {                        \
+        return orc_cells(static_cast<OrcSim *>(h), lev);                       \
+    }                                                                          \
+    ORC_EXPORT int orc_set_boxes(void *h, int lev, const int *b, int nb) {    \
+        return orc_guard([&] {                                                 \
+            std::vector<orc::Box> v(nb);                                       \
+            for (int i = 0; i < nb; ++i)                                       \
+                for (int a = 0; a < 3; ++a) {                                  \
+                    v[i].lo[a] = b[6 * i + a];                                 \
+                    v[i].hi[a] = b[6 * i + 3 + a];                             \
+                }                                                              \
+            static_cast<OrcSim *>(h)->set_level(lev, v);                       \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_set_time(void *h, int lev, double t, int istep) {       \
+        auto *s = static_cast<OrcSim *>(h);                                    \
+        orc_level(s, lev).t = t;                                               \
+        orc_level(s, lev).istep = istep;                                       \
+        return 0;                                                              \
+    }                                                                          \
+    ORC_EXPORT double orc_time(void *h, int lev) {                             \
+        return orc_level(static_cast<OrcSim *>(h), lev).t;                     \
+    }                                                                          \
+    ORC_EXPORT double orc_index_to_k(int a, int N, int type) {                 \
+        return orc::index_to_k(a, N, type);                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_upload(void *h, int lev, int comp, const double *d) {   \
+        return orc_guard([&] {                                                 \
+            orc_valid(orc_level(static_cast<OrcSim *>(h), lev),                \
+                      [&](orc::Fab &f, int i, int j, int k, long long o) {     \
+                          f.at(i, j, k, comp) = d[o];                          \
+                      });                                                      \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_download(void *h, int lev, int comp, double *d,         \
+                                int old) {                                     \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            orc_valid(old ? s->grid_old[lev] : orc_level(s, lev),              \
+                      [&](orc::Fab &f, int i, int j, int k, long long o) {     \
+                          d[o] = f.at(i, j, k, comp);                          \
+                      });                                                      \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_fill_rhs(void *h, int lev, double time, double w,       \
+                                int add) {                                     \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            if (!add || !orc_rhs.defined())                                    \
+                orc_rhs.define(OrcSim::boxes_of(s->grid_new[lev]), s->nc,      \
+                               s->cfg.nghost);                                 \
+            s->fill_patch(s->grid_new[lev], lev);                              \
+            s->fill_rhs(orc_rhs, s->grid_new[lev], time, lev, s->dt[lev],      \
+                        s->dx[lev], add != 0, w);                              \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_download_rhs(void *h, int comp, double *d) {            \
+        return orc_guard([&] {                                                 \
+            orc_valid(orc_rhs, [&](orc::Fab &f, int i, int j, int k,           \
+                                   long long o) { d[o] = f.at(i, j, k, comp); }); \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_advance(void *h, int lev) {                             \
+        return orc_guard([&] { static_cast<OrcSim *>(h)->advance(lev); });    \
+    }                                                                          \
+    ORC_EXPORT int orc_create_shadow(void *h) {                                \
+        return orc_guard([&] { static_cast<OrcSim *>(h)->create_shadow(); });  \
+    }                                                                          \
+    ORC_EXPORT double orc_shadow_time(void *h) {                               \
+        return static_cast<OrcSim *>(h)->shadow.t;                             \
+    }                                                                          \
+    ORC_EXPORT int orc_te_tag(void *h, int lev, int flags, unsigned char *mk,  \
+                              long long *counts, double *te) {                 \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            std::vector<double> t;                                             \
+            if (flags & 1) s->te_avgdown(lev, t, (flags & 2) != 0);            \
+            std::vector<uint8_t> m;                                            \
+            std::vector<long long> c;                                          \
+            s->tag(lev, t, (flags & 1) != 0, m, c);                            \
+            if (mk) std::memcpy(mk, m.data(), m.size());                       \
+            if (counts) std::memcpy(counts, c.data(), c.size() * 8);           \
+            if (te && !t.empty()) std::memcpy(te, t.data(), t.size() * 8);     \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_gw_spectrum(void *h, int mode, int nbins, int zp,       \
+                                   int proj, double *bins, double *k,          \
+                                   int *nout) {                                \
+        return orc_guard([&] {                                                 \
+            std::vector<double> b, kv;                                         \
+            static_cast<OrcSim *>(h)->gw_spectrum(mode, nbins, zp, proj, b, kv); \
+            if (bins && *nout >= (int)b.size()) {                              \
+                std::memcpy(bins, b.data(), b.size() * 8);                     \
+                std::memcpy(k, kv.data(), kv.size() * 8);                      \
+            }                                                                  \
+            *nout = (int)b.size();                                             \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_init_state(void *h, int lev, int kind,                  \
+                                  unsigned long long seed, const double *p,    \
+                                  int np) {                                    \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            if (kind == 1)                                                     \
+                s->init_complex_random_phase(lev, seed, np > 0 ? p[0] : 0.);   \
+            else                                                               \
+                throw std::runtime_error("unknown initial-state kind");        \
+        });                                                                    \
+    }

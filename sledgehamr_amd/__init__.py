@@ -1,0 +1,281 @@
+"""MI355X (gfx950) implementation of sledgehamr's per-level time-step path.
+
+Python binding over the C ABI in ``include/shm.h``.  Each project lives in
+its own ``lib/libshm_<Project>.so`` built by ``sledgehamr_amd.build``; there is
+no CPU fallback: a missing library or a device error raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["Simulation", "library_path", "PROJECTS", "ShmError",
+           "PREC_FP64", "PREC_FP32C", "TE_COMPUTE", "TE_AVGDOWN", "TE_TAGS"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(_HERE, "lib")
+PROJECTS = ("RealScalar", "ComplexScalar", "PhaseTransitionGW")
+
+PREC_INPUTS, PREC_FP64, PREC_FP32C = -1, 0, 1
+TE_COMPUTE, TE_AVGDOWN, TE_TAGS = 1, 2, 4
+
+# Exported symbols declared in include/shm.h (checked by the CPU tests).
+EXPORTS = (
+    "shm_create", "shm_destroy", "shm_last_error", "shm_project_name",
+    "shm_get_info", "shm_set_box_table", "shm_get_box_table",
+    "shm_level_info", "shm_set_level_time", "shm_upload", "shm_download",
+    "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
+    "shm_evolve", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
+    "shm_sync", "shm_autotune", "shm_set_tile", "shm_set_timing",
+    "shm_get_timing",
+)
+
+
+class ShmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[shm status {code}] {msg}")
+        self.code = code
+
+
+class _Config(C.Structure):
+    _fields_ = [("inputs", C.c_char_p), ("precision", C.c_int),
+                ("device", C.c_int)]
+
+
+class _Info(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "ncomp", "nscalars", "nfields", "nghost", "stencil_radius",
+        "precision", "max_level", "finest_level", "coarse_level_grid_size",
+        "integrator_type", "with_gravitational_waves", "shadow_hierarchy")] + \
+        [(n, C.c_double) for n in ("L", "cfl", "t_start", "t_end")]
+
+
+_LIBS: dict[str, C.CDLL] = {}
+
+
+def library_path(project: str) -> str:
+    return os.path.join(LIBDIR, f"libshm_{project}.so")
+
+
+def load_library(project: str) -> C.CDLL:
+    """Loads ``libshm_<project>.so``; raises if it was not built."""
+    if project in _LIBS:
+        return _LIBS[project]
+    path = library_path(project)
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} not found: build it with `python -m sledgehamr_amd.build`")
+    lib = C.CDLL(path, mode=C.RTLD_LOCAL)
+    vp, i, d, ll = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+    dp, ip, llp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)
+    sig = {
+        "shm_create": (i, [C.POINTER(_Config), C.POINTER(vp)]),
+        "shm_destroy": (i, [vp]),
+        "shm_last_error": (C.c_char_p, [vp]),
+        "shm_project_name": (C.c_char_p, [vp]),
+        "shm_get_info": (i, [vp, C.POINTER(_Info)]),
+        "shm_set_box_table": (i, [vp, i, ip, i]),
+        "shm_get_box_table": (i, [vp, i, ip, i, ip]),
+        "shm_level_info": (i, [vp, i, llp, dp, ip]),
+        "shm_set_level_time": (i, [vp, i, d, i]),
+        "shm_upload": (i, [vp, i, i, dp]),
+        "shm_download": (i, [vp, i, i, dp, i]),
+        "shm_init_state": (i, [vp, i, i, C.c_ulonglong, dp, i]),
+        "shm_fill_rhs": (i, [vp, i, d, d, i]),
+        "shm_download_rhs": (i, [vp, i, i, dp]),
+        "shm_advance_level": (i, [vp, i]),
+        "shm_evolve": (i, [vp, i]),
+        "shm_create_shadow": (i, [vp]),
+        "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
+        "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
+        "shm_sync": (i, [vp]),
+        "shm_autotune": (i, [vp, i]),
+        "shm_set_tile": (i, [vp, i, i]),
+        "shm_set_timing": (i, [vp, i]),
+        "shm_get_timing": (i, [vp, dp, llp, llp, dp, llp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIBS[project] = lib
+    return lib
+
+
+def _inputs_text(inputs) -> str:
+    if isinstance(inputs, str):
+        return inputs
+    lines = []
+    for k, v in inputs.items():
+        if isinstance(v, (list, tuple)):
+            v = " ".join(str(x) for x in v)
+        elif isinstance(v, bool):
+            v = "true" if v else "false"
+        lines.append(f"{k} = {v}")
+    return "\n".join(lines) + "\n"
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Simulation:
+    """One project instance on one GPU (a ``shm_ctx``)."""
+
+    def __init__(self, project: str, inputs, precision: int = PREC_INPUTS,
+                 device: int = 0):
+        self.project = project
+        self.lib = load_library(project)
+        self._ctx = C.c_void_p()
+        self._inputs = _inputs_text(inputs).encode()
+        cfg = _Config(self._inputs, precision, device)
+        rc = self.lib.shm_create(C.byref(cfg), C.byref(self._ctx))
+        if rc != 0:
+            msg = self.lib.shm_last_error(self._ctx).decode() if self._ctx else ""
+            if self._ctx:
+                self.lib.shm_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+            raise ShmError(rc, msg)
+        self.info = self._get_info()
+
+    # -- plumbing ----------------------------------------------------------
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise ShmError(rc, self.lib.shm_last_error(self._ctx).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self.lib.shm_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _get_info(self) -> dict:
+        inf = _Info()
+        self._check(self.lib.shm_get_info(self._ctx, C.byref(inf)))
+        return {n: getattr(inf, n) for n, _ in _Info._fields_}
+
+    # -- levels ------------------------------------------------------------
+    def set_boxes(self, lev: int, boxes) -> None:
+        b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        self._check(self.lib.shm_set_box_table(
+            self._ctx, lev, b.ctypes.data_as(C.POINTER(C.c_int)), len(b)))
+        self.info = self._get_info()
+
+    def boxes(self, lev: int) -> np.ndarray:
+        n = C.c_int()
+        self._check(self.lib.shm_get_box_table(self._ctx, lev, None, 0, C.byref(n)))
+        b = np.zeros((n.value, 6), dtype=np.int32)
+        self._check(self.lib.shm_get_box_table(
+            self._ctx, lev, b.ctypes.data_as(C.POINTER(C.c_int)), n.value,
+            C.byref(n)))
+        return b
+
+    def level_info(self, lev: int) -> tuple[int, float, int]:
+        n, t, s = C.c_longlong(), C.c_double(), C.c_int()
+        self._check(self.lib.shm_level_info(self._ctx, lev, C.byref(n),
+                                            C.byref(t), C.byref(s)))
+        return n.value, t.value, s.value
+
+    def set_time(self, lev: int, t: float, istep: int = 0) -> None:
+        self._check(self.lib.shm_set_level_time(self._ctx, lev, t, istep))
+
+    def ncells(self, lev: int) -> int:
+        return self.level_info(lev)[0]
+
+    # -- data --------------------------------------------------------------
+    def upload(self, lev: int, comp: int, data) -> None:
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        if a.size != self.ncells(lev):
+            raise ValueError("data size does not match the level's valid cells")
+        self._check(self.lib.shm_upload(self._ctx, lev, comp, _dptr(a)))
+
+    def download(self, lev: int, comp: int, old: bool = False) -> np.ndarray:
+        a = np.empty(self.ncells(lev), dtype=np.float64)
+        self._check(self.lib.shm_download(self._ctx, lev, comp, _dptr(a), int(old)))
+        return a
+
+    def rhs(self, lev: int, comp: int) -> np.ndarray:
+        a = np.empty(self.ncells(lev), dtype=np.float64)
+        self._check(self.lib.shm_download_rhs(self._ctx, lev, comp, _dptr(a)))
+        return a
+
+    def init_state(self, lev: int, kind: int, seed: int, params=()) -> None:
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        self._check(self.lib.shm_init_state(
+            self._ctx, lev, kind, seed, _dptr(p) if p.size else None, p.size))
+
+    # -- operations --------------------------------------------------------
+    def fill_rhs(self, lev: int, time: float, weight: float = 0.0,
+                 add: bool = False) -> None:
+        self._check(self.lib.shm_fill_rhs(self._ctx, lev, time, weight, int(add)))
+
+    def advance(self, lev: int = 0, nsteps: int = 1) -> None:
+        for _ in range(nsteps):
+            self._check(self.lib.shm_advance_level(self._ctx, lev))
+
+    def evolve(self, ncoarse: int) -> None:
+        self._check(self.lib.shm_evolve(self._ctx, ncoarse))
+
+    def create_shadow(self) -> None:
+        self._check(self.lib.shm_create_shadow(self._ctx))
+
+    def te_tag(self, lev: int, flags: int = TE_COMPUTE | TE_AVGDOWN | TE_TAGS,
+               want_te: bool = False):
+        n = self.ncells(lev)
+        mask = np.zeros(n, dtype=np.uint8)
+        counts = np.zeros(2 + self.info["ncomp"], dtype=np.int64)
+        te = None
+        if want_te:
+            te = np.zeros((self.info["ncomp"], n // 8), dtype=np.float64)
+        self._check(self.lib.shm_te_tag(
+            self._ctx, lev, flags, mask.ctypes.data_as(C.POINTER(C.c_ubyte)),
+            counts.ctypes.data_as(C.POINTER(C.c_longlong)),
+            _dptr(te) if te is not None else None))
+        return mask, counts, te
+
+    def gw_spectrum(self, mode: int = 0, nbins: int = 256, zero_padding: int = 1,
+                    projection_type: int = 2):
+        n = C.c_int(0)
+        self._check(self.lib.shm_gw_spectrum(self._ctx, mode, nbins, zero_padding,
+                                             projection_type, None, None,
+                                             C.byref(n)))
+        bins = np.zeros(n.value)
+        k = np.zeros(n.value)
+        self._check(self.lib.shm_gw_spectrum(self._ctx, mode, nbins, zero_padding,
+                                             projection_type, _dptr(bins),
+                                             _dptr(k), C.byref(n)))
+        return bins, k
+
+    def sync(self) -> None:
+        self._check(self.lib.shm_sync(self._ctx))
+
+    def autotune(self, lev: int = 0) -> None:
+        self._check(self.lib.shm_autotune(self._ctx, lev))
+
+    def set_tile(self, variant: int, tile_z: int = 0) -> None:
+        self._check(self.lib.shm_set_tile(self._ctx, variant, tile_z))
+
+    def set_timing(self, on: bool) -> None:
+        self._check(self.lib.shm_set_timing(self._ctx, int(on)))
+
+    def timing(self) -> dict:
+        sm, hm = C.c_double(), C.c_double()
+        sl, sc, hl = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        self._check(self.lib.shm_get_timing(self._ctx, C.byref(sm), C.byref(sl),
+                                            C.byref(sc), C.byref(hm), C.byref(hl)))
+        return {"stage_ms": sm.value, "stage_launches": sl.value,
+                "stage_cells": sc.value, "halo_ms": hm.value,
+                "halo_launches": hl.value}

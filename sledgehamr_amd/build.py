@@ -1,0 +1,133 @@
+"""Builds the gfx950 libraries: one ``libshm_<Project>.so`` per project.
+
+The project registry below plays the role of the reference's Make.projects
+(Make.projects:1-18): each entry names the project header (compiled
+unchanged), its class, and which kernel variants to instantiate.
+
+Usage: ``python -m sledgehamr_amd.build [-j N] [--projects A,B]``.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from dataclasses import dataclass
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+BUILD = os.path.join(ROOT, "_build")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+
+@dataclass(frozen=True)
+class Project:
+    name: str          # library name and project.name in the inputs file
+    header: str        # header path relative to the repository root
+    cls: str           # fully qualified class
+    r_mask: int        # bit r: stencil radius r instantiated
+    gw: bool = False   # GW (12 tensor components) variants
+    ko: bool = False   # Kreiss-Oliger variants
+
+
+PROJECTS = [
+    Project("RealScalar", "sledgehamr_amd/projects/RealScalar/RealScalar.h",
+            "RealScalar::RealScalar", 0b1110, ko=True),
+    Project("ComplexScalar", "sledgehamr_amd/projects/ComplexScalar/ComplexScalar.h",
+            "ComplexScalar::ComplexScalar", 0b1110),
+    Project("PhaseTransitionGW",
+            "sledgehamr_amd/projects/PhaseTransitionGW/PhaseTransitionGW.h",
+            "PhaseTransitionGW::PhaseTransitionGW", 0b1000, gw=True),
+]
+
+COMMON_FLAGS = [
+    "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=fast", "-munsafe-fp-atomics",
+    "-Wno-unused-result", "-Wno-unknown-pragmas",
+    "-I", INCLUDE, "-I", os.path.join(INCLUDE, "sledgehamr"), "-I", CSRC,
+]
+
+RUNTIME_SOURCES = ["shm_runtime.cpp", "shm_capi.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"command failed ({r.returncode}): {cmd[-1]}")
+
+
+def _compile_runtime(src: str) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    _run([HIPCC, *COMMON_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src),
+          "-o", obj])
+    return obj
+
+
+def _project_tu(p: Project) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    tu = os.path.join(BUILD, f"proj_{p.name}.hip")
+    text = (
+        f'#include "{os.path.join(ROOT, p.header)}"\n'
+        '#include "shm_project_impl.hpp"\n'
+        f"SHM_REGISTER_PROJECT({p.cls})\n"
+    )
+    if not os.path.exists(tu) or open(tu).read() != text:
+        with open(tu, "w") as f:
+            f.write(text)
+    return tu
+
+
+def _compile_project(p: Project) -> str:
+    tu = _project_tu(p)
+    obj = os.path.join(BUILD, f"proj_{p.name}.o")
+    _run([HIPCC, *COMMON_FLAGS, f"-DSHM_R_MASK={p.r_mask}",
+          f"-DSHM_WITH_GW={int(p.gw)}", f"-DSHM_WITH_KO={int(p.ko)}",
+          "-c", tu, "-o", obj])
+    return obj
+
+
+def _link(p: Project, pobj: str, robjs: list[str]) -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, f"libshm_{p.name}.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, pobj, *robjs,
+          "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, out)
+    return out
+
+
+def build(projects: list[str] | None = None, jobs: int = 8) -> list[str]:
+    """Compiles the runtime and every selected project; returns .so paths."""
+    if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
+        raise RuntimeError("hipcc not found")
+    sel = [p for p in PROJECTS if projects is None or p.name in projects]
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        rfut = [ex.submit(_compile_runtime, s) for s in RUNTIME_SOURCES]
+        pfut = {p.name: ex.submit(_compile_project, p) for p in sel}
+        robjs = [f.result() for f in rfut]
+        pobjs = {k: f.result() for k, f in pfut.items()}
+    return [_link(p, pobjs[p.name], robjs) for p in sel]
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--projects", default="")
+    a = ap.parse_args()
+    sel = [s for s in a.projects.split(",") if s] or None
+    for path in build(sel, a.j):
+        print(path)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,503 @@
+// Truncation-error / tagging, gravitational-wave spectrum and initial-state
+// kernels for gfx950.
+#pragma once
+
+#include <sledgehamr.h>
+
+#include <cstdint>
+
+#include "shm_layout.hpp"
+#include "shm_stage.hpp"
+
+namespace shm {
+
+// ---------------------------------------------------------------------------
+// Counter-based random numbers shared by device generators and the oracle:
+// splitmix64 of (seed, stream, cell index) -> uniform in [0, 1).
+// ---------------------------------------------------------------------------
+
+__host__ __device__ AMREX_FORCE_INLINE uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__host__ __device__ AMREX_FORCE_INLINE double rng_uniform(uint64_t seed,
+                                                          uint64_t stream,
+                                                          uint64_t idx) {
+    const uint64_t h = splitmix64(splitmix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) + idx);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
+// Fused average-down + truncation error + tagging.
+// ---------------------------------------------------------------------------
+
+struct TeTagArgs {
+    const BoxDesc *fboxes;   // fine boxes
+    const BoxDesc *cboxes;   // coarse box holding each fine box's parent cells
+    long long fcstride, ccstride;
+    const void *fine;        // fine state (hi part for FP32C)
+    void *crse;              // coarse state, averaged in place when requested
+    void *crselo;            // coarse remainder (FP32C)
+    const long long *cprefix;  // coarse cells per fine box, prefix (nbox+1)
+    const long long *mprefix;  // fine valid cells per fine box, prefix
+    int nbox;
+    long long total;
+    int ncomp;
+    double *te_out;          // optional: te per coarse cell, ncomp planes of total
+    uint8_t *mask;           // fine-cell tags (0/1), box-major x-fastest
+    unsigned long long *counts;  // [total, user, trunc[ncomp]]
+    const double *te_crit;
+    const double *params_tag;
+    const double *params_mod;
+    int lev;
+    double time, dt, dx;
+    int flags;
+};
+
+enum TeTagFlags : int {
+    kTeCompute = 1,   // compute truncation errors (else user tags only)
+    kTeAvgDown = 2,   // write the child mean into the coarse state
+    kTeTags = 4,      // write the tag mask and counters
+};
+
+template <class S>
+__device__ AMREX_FORCE_INLINE amrex::Array4<const double> plain_view(
+    const void *arr, const BoxDesc &b, long long cstride, int ncomp) {
+    amrex::Array4<const double> v;
+    if constexpr (std::is_same_v<S, float>) {
+        v.mode = amrex::Array4<const double>::kPlain32;
+        v.pf = static_cast<const float *>(arr) + b.base;
+    } else {
+        v.mode = amrex::Array4<const double>::kPlain64;
+        v.p = static_cast<const double *>(arr) + b.base;
+    }
+    v.jstride = b.jstride;
+    v.kstride = b.kstride;
+    v.nstride = cstride;
+    v.begin = {b.lo[0], b.lo[1], b.lo[2]};
+    v.end = {b.lo[0] + b.n[0], b.lo[1] + b.n[1], b.lo[2] + b.n[2]};
+    v.ncomp = ncomp;
+    return v;
+}
+
+__device__ AMREX_FORCE_INLINE unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/** One thread per coarse cell.  Reproduces the effective CPU semantics of
+ *  the reference tag loop (macros.h:489-566): a fine cell is tagged if its
+ *  user criterion holds, or if it is the all-even child of its parent and
+ *  some component passes TruncationModifier<n>(te) >= te_crit[n] (FP64).
+ *  Counters follow macros.h:509-527 (8 per passing component, 8 - user for
+ *  the block total). */
+template <class Ops, Prec P, bool GW>
+__global__ void __launch_bounds__(256) te_tag_kernel(const TeTagArgs a) {
+    using S = typename PrecTraits<P>::S;
+    constexpr int NC = GW ? Ops::kNGw : Ops::kNScalars;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    unsigned long long c_total = 0, c_user = 0, c_trunc[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) c_trunc[n] = 0;
+
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < a.total; t += gstride) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.cprefix[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int b = lo;
+        const BoxDesc fb = a.fboxes[b];
+        const BoxDesc cb = a.cboxes[b];
+        const int cnx = fb.n[0] >> 1, cny = fb.n[1] >> 1;
+        long long r = t - a.cprefix[b];
+        const int cx = (int)(r % cnx);
+        r /= cnx;
+        const int cy = (int)(r % cny);
+        const int cz = (int)(r / cny);
+        const int ii = fb.lo[0] + 2 * cx, jj = fb.lo[1] + 2 * cy,
+                  kk = fb.lo[2] + 2 * cz;
+        const int ci = (fb.lo[0] >> 1) + cx, cj = (fb.lo[1] >> 1) + cy,
+                  ck = (fb.lo[2] >> 1) + cz;
+        const amrex::Array4<const double> fv =
+            plain_view<S>(a.fine, fb, a.fcstride, NC);
+
+        bool te_pass = false;
+        if (a.flags & kTeCompute) {
+            const long long ce = cb.base + (ci - cb.lo[0]) +
+                                 (long long)(cj - cb.lo[1]) * cb.jstride +
+                                 (long long)(ck - cb.lo[2]) * cb.kstride;
+            sledgehamr::utils::constexpr_for<0, NC, 1>([&](auto n) {
+                auto f = [&](int x, int y, int z) { return fv(x, y, z, n); };
+                const double sum = child_sum(f, ii, jj, kk);
+                const long long cen = (long long)n * a.ccstride + ce;
+                double crse;
+                if constexpr (std::is_same_v<S, float>)
+                    crse = (double)static_cast<const float *>(a.crse)[cen];
+                else
+                    crse = static_cast<const double *>(a.crse)[cen];
+                double avg;
+                const double te = te_cell(crse, sum, &avg);
+                if (a.flags & kTeAvgDown) store_state<S>(a.crse, a.crselo, cen, avg);
+                if (a.te_out) a.te_out[(long long)n * a.total + t] = te;
+                const double m = Ops::template te_modifier<n>(
+                    fv, ii, jj, kk, a.lev, a.time, a.dt, a.dx, te, a.params_mod);
+                if (m >= a.te_crit[n]) {
+                    te_pass = true;
+                    c_trunc[n] += 8;
+                }
+            });
+        }
+        if (a.flags & kTeTags) {
+            const int fnx = fb.n[0], fny = fb.n[1];
+            uint8_t *mb = a.mask + a.mprefix[b];
+            int user_even = 0;
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                const int dx_ = d & 1, dy_ = (d >> 1) & 1, dz_ = d >> 2;
+                const bool u = Ops::tag_cell(fv, ii + dx_, jj + dy_, kk + dz_,
+                                             a.lev, a.time, a.dt, a.dx,
+                                             a.params_tag);
+                if (d == 0) user_even = u;
+                c_user += u;
+                c_total += u;
+                const bool tag = u || (d == 0 && te_pass);
+                mb[(2 * cx + dx_) + (long long)(2 * cy + dy_) * fnx +
+                   (long long)(2 * cz + dz_) * fnx * fny] = tag ? 1 : 0;
+            }
+            if (te_pass) c_total += 8 - user_even;
+        }
+    }
+    if (a.flags & kTeTags) {
+        const unsigned long long s_total = wave_sum(c_total);
+        const unsigned long long s_user = wave_sum(c_user);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.counts[0], s_total);
+            atomicAdd(&a.counts[1], s_user);
+        }
+#pragma unroll
+        for (int n = 0; n < NC; ++n) {
+            const unsigned long long s = wave_sum(c_trunc[n]);
+            if ((threadIdx.x & 63) == 0) atomicAdd(&a.counts[2 + n], s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Box-to-box copy between two layouts (coarse data onto a coarsened-fine
+// layout and back, shadow levels).  Segments index dst boxes; src cells are
+// dst cells minus `shift`.
+// ---------------------------------------------------------------------------
+
+struct CopyArgs {
+    void *dst;
+    const void *src;
+    long long dcstride, scstride;
+    const BoxDesc *dboxes;
+    const BoxDesc *sboxes;
+    const HaloSeg *segs;
+    int nseg;
+    long long total;
+    int ncomp;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) copy_kernel(const CopyArgs a) {
+    T *dst = static_cast<T *>(a.dst);
+    const T *src = static_cast<const T *>(a.src);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < a.total; t += stride) {
+        int lo = 0, hi = a.nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.segs[mid].start <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const HaloSeg &sg = a.segs[lo];
+        long long r = t - sg.start;
+        const int x = (int)(r % sg.n[0]);
+        r /= sg.n[0];
+        const int y = (int)(r % sg.n[1]);
+        const int z = (int)(r / sg.n[1]);
+        const int gi = sg.lo[0] + x, gj = sg.lo[1] + y, gk = sg.lo[2] + z;
+        const BoxDesc &db = a.dboxes[sg.dst_box];
+        const BoxDesc &sb = a.sboxes[sg.src_box];
+        const long long de = db.base + (gi - db.lo[0]) +
+                             (long long)(gj - db.lo[1]) * db.jstride +
+                             (long long)(gk - db.lo[2]) * db.kstride;
+        const long long se = sb.base + (gi - sg.shift[0] - sb.lo[0]) +
+                             (long long)(gj - sg.shift[1] - sb.lo[1]) * sb.jstride +
+                             (long long)(gk - sg.shift[2] - sb.lo[2]) * sb.kstride;
+        for (int c = 0; c < a.ncomp; ++c)
+            dst[(long long)c * a.dcstride + de] = src[(long long)c * a.scstride + se];
+    }
+}
+
+/** 8-cell mean of a fine level onto a coarser layout (amrex::average_down,
+ *  used to build the shadow level and to synchronise levels): one thread
+ *  per coarse cell of each fine box, FP64 sums in child_sum order. */
+struct AvgArgs {
+    const BoxDesc *fboxes;
+    const BoxDesc *cboxes;
+    long long fcstride, ccstride;
+    const void *fine;
+    const void *finelo;
+    void *crse;
+    void *crselo;
+    const long long *cprefix;
+    int nbox;
+    long long total;
+    int ncomp;
+};
+
+template <Prec P>
+__global__ void __launch_bounds__(256) avgdown_kernel(const AvgArgs a) {
+    using S = typename PrecTraits<P>::S;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < a.total; t += gstride) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.cprefix[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BoxDesc fb = a.fboxes[lo];
+        const BoxDesc cb = a.cboxes[lo];
+        const int cnx = fb.n[0] >> 1, cny = fb.n[1] >> 1;
+        long long r = t - a.cprefix[lo];
+        const int cx = (int)(r % cnx);
+        r /= cnx;
+        const int cy = (int)(r % cny);
+        const int cz = (int)(r / cny);
+        const int ci = (fb.lo[0] >> 1) + cx, cj = (fb.lo[1] >> 1) + cy,
+                  ck = (fb.lo[2] >> 1) + cz;
+        const long long ce = cb.base + (ci - cb.lo[0]) +
+                             (long long)(cj - cb.lo[1]) * cb.jstride +
+                             (long long)(ck - cb.lo[2]) * cb.kstride;
+        const long long fe = fb.base + 2 * cx + (long long)(2 * cy) * fb.jstride +
+                             (long long)(2 * cz) * fb.kstride;
+        for (int n = 0; n < a.ncomp; ++n) {
+            const long long off = (long long)n * a.fcstride + fe;
+            auto f = [&](int x, int y, int z) {
+                const long long e = off + x + (long long)y * fb.jstride +
+                                    (long long)z * fb.kstride;
+                return load_state<S>(a.fine, a.finelo, e);
+            };
+            const double sum = child_sum(f, 0, 0, 0);
+            store_state<S>(a.crse, a.crselo, (long long)n * a.ccstride + ce,
+                           0.125 * sum);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gravitational-wave spectrum: pack -> hipFFT D2Z -> projected binning.
+// ---------------------------------------------------------------------------
+
+/** Copy one component of one box into a dense N^3 FP64 array (x fastest). */
+struct PackArgs {
+    const void *arr;
+    const void *arrlo;
+    long long cstride;
+    BoxDesc box;
+    int comp;
+    double *out;
+    int N;
+};
+
+template <Prec P>
+__global__ void __launch_bounds__(256) pack_kernel(const PackArgs a) {
+    using S = typename PrecTraits<P>::S;
+    const long long nb = (long long)a.box.n[0] * a.box.n[1] * a.box.n[2];
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nb;
+         t += gstride) {
+        const int x = (int)(t % a.box.n[0]);
+        const int y = (int)((t / a.box.n[0]) % a.box.n[1]);
+        const int z = (int)(t / ((long long)a.box.n[0] * a.box.n[1]));
+        const long long e = (long long)a.comp * a.cstride + a.box.base + x +
+                            (long long)y * a.box.jstride +
+                            (long long)z * a.box.kstride;
+        const long long o = (long long)(a.box.lo[0] + x) +
+                            (long long)a.N * ((a.box.lo[1] + y) +
+                                              (long long)a.N * (a.box.lo[2] + z));
+        a.out[o] = load_state<S>(a.arr, a.arrlo, e);
+    }
+}
+
+struct BinArgs {
+    const double2 *u[6];     // R2C outputs [z][y][x], x halved (N/2+1)
+    int N;
+    const double *index_to_k;
+    int mode;                // 0 unbinned (lookup), 1 round(|n|), 2 log bins
+    const int *sq_to_bin;    // mode 0 table, size 3*(N/2)^2 + 1
+    int nbins;
+    double log_kmax;         // mode 2: ln(sqrt(3) N / 2)
+    double *bins;            // FP64 accumulators
+};
+
+/** sum_{ijlm} Lambda_{ij,lm}(k) A_ij A_lm = Tr(APAP) - Tr(PA)^2 / 2 for a
+ *  symmetric A and the transverse projector P = 1 - k k^T / |k|^2; equal to
+ *  the 81-term contraction of gw_GetLambda (gravitational_waves.h:101-133). */
+__device__ AMREX_FORCE_INLINE double lambda_contract(const double P[3][3],
+                                                     const double A[3][3]) {
+    double B[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            B[i][j] = P[i][0] * A[0][j] + P[i][1] * A[1][j] + P[i][2] * A[2][j];
+    double tr = B[0][0] + B[1][1] + B[2][2];
+    double tr2 = 0.;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) tr2 += B[i][j] * B[j][i];
+    return tr2 - 0.5 * tr * tr;
+}
+
+template <int LDSBINS>
+__global__ void __launch_bounds__(256) gw_bin_kernel(const BinArgs a) {
+    __shared__ double sb[LDSBINS > 0 ? LDSBINS : 1];
+    const bool use_lds = LDSBINS > 0 && a.nbins <= LDSBINS;
+    if (use_lds)
+        for (int i = threadIdx.x; i < a.nbins; i += blockDim.x) sb[i] = 0.;
+    __syncthreads();
+    const int N = a.N, NH = N / 2 + 1;
+    const long long total = (long long)NH * N * N;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < total; t += gstride) {
+        const int ax = (int)(t % NH);
+        const int by = (int)((t / NH) % N);
+        const int cz = (int)(t / ((long long)NH * N));
+        if (ax == 0 && by == 0 && cz == 0) continue;  // Lambda(0) = 0
+        const int li = ax >= N / 2 ? ax - N : ax;
+        const int lj = by >= N / 2 ? by - N : by;
+        const int lk = cz >= N / 2 ? cz - N : cz;
+        const unsigned sq = (unsigned)(li * li + lj * lj + lk * lk);
+        int bin;
+        if (a.mode == 0) {
+            bin = a.sq_to_bin[sq];
+        } else if (a.mode == 1) {
+            bin = (int)(sqrt((double)sq) + 0.5);
+        } else {
+            bin = (int)floor((double)a.nbins * 0.5 * log((double)sq) / a.log_kmax);
+            bin = bin < 0 ? 0 : (bin >= a.nbins ? a.nbins - 1 : bin);
+        }
+        if (bin < 0 || bin >= a.nbins) continue;
+        const double kv[3] = {a.index_to_k[ax], a.index_to_k[by],
+                              a.index_to_k[cz]};
+        const double nrm = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+        double Pm[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Pm[i][j] = (i == j ? 1. : 0.) - kv[i] * kv[j] / nrm;
+        double2 v[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[c] = a.u[c][t];
+        // Component order du_xx, du_xy, du_xz, du_yy, du_yz, du_zz.
+        const double Ar[3][3] = {{v[0].x, v[1].x, v[2].x},
+                                 {v[1].x, v[3].x, v[4].x},
+                                 {v[2].x, v[4].x, v[5].x}};
+        const double Ai[3][3] = {{v[0].y, v[1].y, v[2].y},
+                                 {v[1].y, v[3].y, v[4].y},
+                                 {v[2].y, v[4].y, v[5].y}};
+        const double mult = (ax == 0 || ax == N / 2) ? 1. : 2.;
+        const double val =
+            mult * (lambda_contract(Pm, Ar) + lambda_contract(Pm, Ai));
+        if (use_lds)
+            atomicAdd(&sb[bin], val);
+        else
+            atomicAdd(&a.bins[bin], val);
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < a.nbins; i += blockDim.x)
+            if (sb[i] != 0.) atomicAdd(&a.bins[i], sb[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Seeded initial states (identical formulas in the oracle).
+// ---------------------------------------------------------------------------
+
+struct InitArgs {
+    void *arr;
+    void *arrlo;
+    long long cstride;
+    const BoxDesc *boxes;
+    const long long *vprefix;  // valid cells per box, prefix
+    int nbox;
+    long long total;
+    int N;                     // domain cells per axis at this level
+    int kind;
+    uint64_t seed;
+    int nfields;
+    int ncomp;
+    double p[8];
+};
+
+enum InitKind : int {
+    kInitComplexRandomPhase = 1,  // psi = (1 + p0 N(0,1)) exp(i 2 pi U), pi = 0
+    kInitZero = 2,
+};
+
+template <Prec P>
+__global__ void __launch_bounds__(256) init_kernel(const InitArgs a) {
+    using S = typename PrecTraits<P>::S;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < a.total; t += gstride) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.vprefix[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BoxDesc b = a.boxes[lo];
+        long long r = t - a.vprefix[lo];
+        const int x = (int)(r % b.n[0]);
+        r /= b.n[0];
+        const int y = (int)(r % b.n[1]);
+        const int z = (int)(r / b.n[1]);
+        const long long e =
+            b.base + x + (long long)y * b.jstride + (long long)z * b.kstride;
+        const uint64_t gidx = (uint64_t)(b.lo[0] + x) +
+                              (uint64_t)a.N * ((uint64_t)(b.lo[1] + y) +
+                                               (uint64_t)a.N * (uint64_t)(b.lo[2] + z));
+        double v[32];
+        for (int n = 0; n < a.ncomp && n < 32; ++n) v[n] = 0.;
+        if (a.kind == kInitComplexRandomPhase) {
+            const double u1 = rng_uniform(a.seed, 1, gidx);
+            const double u2 = rng_uniform(a.seed, 2, gidx);
+            const double u3 = rng_uniform(a.seed, 3, gidx);
+            const double theta = 2. * M_PI * u1;
+            const double g = sqrt(-2. * log(1. - u2)) * cos(2. * M_PI * u3);
+            const double m = 1. + a.p[0] * g;
+            v[0] = m * cos(theta);
+            v[1] = m * sin(theta);
+        }
+        for (int n = 0; n < a.ncomp && n < 32; ++n)
+            store_state<S>(a.arr, a.arrlo, (long long)n * a.cstride + e, v[n]);
+    }
+}
+
+}  // namespace shm

@@ -1,0 +1,329 @@
+// extern "C" entry points (include/shm.h) over shm::Runtime.
+#include "../../include/shm.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "shm_runtime.hpp"
+
+extern "C" sledgehamr::Sledgehamr *shm_project_factory();
+
+struct shm_ctx {
+    std::unique_ptr<sledgehamr::Sledgehamr> sim;
+    std::unique_ptr<shm::Runtime> rt;
+    std::string err;
+};
+
+namespace {
+
+template <class F>
+int guard(shm_ctx *ctx, F &&f) {
+    if (!ctx) return SHM_ERR_ARG;
+    try {
+        f();
+        ctx->err.clear();
+        return SHM_OK;
+    } catch (const shm::Error &e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return SHM_ERR_ARG;
+    } catch (...) {
+        ctx->err = "unknown error";
+        return SHM_ERR_INTERNAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+__attribute__((visibility("default"))) int shm_create(const shm_config *cfg,
+                                                      shm_ctx **out) {
+    if (!cfg || !out || !cfg->inputs) return SHM_ERR_ARG;
+    *out = nullptr;
+    auto ctx = std::make_unique<shm_ctx>();
+    const int rc = guard(ctx.get(), [&] {
+        if (hipSetDevice(cfg->device) != hipSuccess)
+            throw shm::Error(SHM_ERR_HIP, "hipSetDevice failed");
+        ctx->sim.reset(shm_project_factory());
+        ctx->rt = std::make_unique<shm::Runtime>(ctx->sim.get(),
+                                                 ctx->sim->ShmKernels());
+        ctx->rt->init_from_inputs(cfg->inputs, cfg->precision);
+    });
+    if (rc != SHM_OK) {
+        // Keep the context so the caller can read the error, then destroy.
+        ctx->rt.reset();
+        *out = ctx.release();
+        return rc;
+    }
+    *out = ctx.release();
+    return SHM_OK;
+}
+
+__attribute__((visibility("default"))) int shm_destroy(shm_ctx *ctx) {
+    if (!ctx) return SHM_ERR_ARG;
+    try {
+        ctx->rt.reset();
+        ctx->sim.reset();
+    } catch (...) {
+    }
+    delete ctx;
+    return SHM_OK;
+}
+
+__attribute__((visibility("default"))) const char *shm_last_error(
+    const shm_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+__attribute__((visibility("default"))) const char *shm_project_name(
+    const shm_ctx *ctx) {
+    return (ctx && ctx->sim) ? ctx->sim->project_name.c_str() : "";
+}
+
+#define SHM_NEED_RT                                                           \
+    if (!ctx->rt) throw shm::Error(SHM_ERR_STATE, "context not initialised")
+
+__attribute__((visibility("default"))) int shm_get_info(shm_ctx *ctx,
+                                                        shm_info_t *info) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!info) throw shm::Error(SHM_ERR_ARG, "null info");
+        sledgehamr::Sledgehamr &S = *ctx->sim;
+        shm::Runtime &R = *ctx->rt;
+        const shm::KernelTable &kt = S.ShmKernels();
+        std::memset(info, 0, sizeof(*info));
+        info->ncomp = R.ncomp();
+        info->nscalars = kt.nscalars;
+        info->nfields = kt.nfields;
+        info->nghost = S.nghost;
+        info->stencil_radius = R.radius();
+        info->precision = R.precision();
+        info->max_level = S.GetMaxLevel();
+        info->finest_level = S.GetFinestLevel();
+        info->coarse_level_grid_size = S.GetDimN(0);
+        info->with_gravitational_waves = S.with_gravitational_waves ? 1 : 0;
+        info->L = S.GetL();
+        info->cfl = S.GetDt(0) / S.GetDx(0);
+        amrex::ParmParse pp("");
+        int it = 0;
+        pp.query("integrator.type", it);
+        info->integrator_type = it;
+        double ts = 0, te = 0;
+        pp.query("sim.t_start", ts);
+        pp.query("sim.t_end", te);
+        info->t_start = ts;
+        info->t_end = te;
+        double crit = DBL_MAX;
+        pp.query("amr.te_crit", crit);
+        info->shadow_hierarchy = crit != DBL_MAX;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_set_box_table(shm_ctx *ctx,
+                                                             int lev,
+                                                             const int *boxes,
+                                                             int nbox) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!boxes || nbox <= 0) throw shm::Error(SHM_ERR_ARG, "empty box table");
+        std::vector<shm::Box3> b(nbox);
+        for (int i = 0; i < nbox; ++i)
+            for (int a = 0; a < 3; ++a) {
+                b[i].lo[a] = boxes[6 * i + a];
+                b[i].hi[a] = boxes[6 * i + 3 + a];
+            }
+        ctx->rt->set_level(lev, b);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_get_box_table(
+    shm_ctx *ctx, int lev, int *boxes, int maxbox, int *nbox) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        const shm::LevelState &ls = ctx->rt->level(lev);
+        if (nbox) *nbox = (int)ls.L.boxes.size();
+        if (boxes)
+            for (int i = 0; i < (int)ls.L.boxes.size() && i < maxbox; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    boxes[6 * i + a] = ls.L.boxes[i].lo[a];
+                    boxes[6 * i + 3 + a] = ls.L.boxes[i].hi[a];
+                }
+    });
+}
+
+__attribute__((visibility("default"))) int shm_level_info(shm_ctx *ctx, int lev,
+                                                          long long *ncells,
+                                                          double *t,
+                                                          int *istep) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        const shm::LevelState &ls = ctx->rt->level(lev);
+        if (ncells) *ncells = ls.L.nvalid();
+        if (t) *t = ls.t;
+        if (istep) *istep = ls.istep;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_set_level_time(shm_ctx *ctx,
+                                                              int lev, double t,
+                                                              int istep) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        shm::LevelState &ls = ctx->rt->level(lev);
+        ls.t = t;
+        ls.istep = istep;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_upload(shm_ctx *ctx, int lev,
+                                                      int comp,
+                                                      const double *data) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!data) throw shm::Error(SHM_ERR_ARG, "null data");
+        ctx->rt->upload(lev, comp, data);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_download(shm_ctx *ctx, int lev,
+                                                        int comp, double *data,
+                                                        int old_state) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!data) throw shm::Error(SHM_ERR_ARG, "null data");
+        ctx->rt->download(lev, comp, data, old_state != 0);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_init_state(
+    shm_ctx *ctx, int lev, int kind, unsigned long long seed, const double *p,
+    int np) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->init_state(lev, kind, seed, p, p ? np : 0);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_fill_rhs(shm_ctx *ctx, int lev,
+                                                        double time,
+                                                        double weight, int add) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->fill_rhs(lev, time, weight, add != 0);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_download_rhs(shm_ctx *ctx,
+                                                            int lev, int comp,
+                                                            double *data) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!data) throw shm::Error(SHM_ERR_ARG, "null data");
+        ctx->rt->download_rhs(lev, comp, data);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_advance_level(shm_ctx *ctx,
+                                                             int lev) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->advance(lev);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_evolve(shm_ctx *ctx,
+                                                      int ncoarse) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->step(ncoarse);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_create_shadow(shm_ctx *ctx) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->create_shadow_level();
+    });
+}
+
+__attribute__((visibility("default"))) int shm_te_tag(shm_ctx *ctx, int lev,
+                                                      int flags,
+                                                      unsigned char *mask,
+                                                      long long *counts,
+                                                      double *te) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->te_tag(lev, flags, mask, counts, te);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_gw_spectrum(
+    shm_ctx *ctx, int mode, int nbins, int zero_padding, int projection_type,
+    double *bins, double *kvals, int *nout) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!nout) throw shm::Error(SHM_ERR_ARG, "null nout");
+        std::vector<double> b, k;
+        ctx->rt->gw_spectrum(mode, nbins, zero_padding, projection_type, b, k);
+        if (bins) {
+            if (*nout < (int)b.size())
+                throw shm::Error(SHM_ERR_ARG, "output arrays too small");
+            std::memcpy(bins, b.data(), b.size() * sizeof(double));
+            if (kvals) std::memcpy(kvals, k.data(), k.size() * sizeof(double));
+        }
+        *nout = (int)b.size();
+    });
+}
+
+__attribute__((visibility("default"))) int shm_sync(shm_ctx *ctx) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->sync();
+    });
+}
+
+__attribute__((visibility("default"))) int shm_autotune(shm_ctx *ctx, int lev) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->autotune(lev);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_set_tile(shm_ctx *ctx,
+                                                        int variant,
+                                                        int tile_z) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (variant >= shm::kNVar) throw shm::Error(SHM_ERR_ARG, "bad tile variant");
+        ctx->rt->variant = variant;
+        ctx->rt->tz = tile_z;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_set_timing(shm_ctx *ctx, int on) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->timing_on = on != 0;
+        ctx->rt->timing = shm::Timing();
+    });
+}
+
+__attribute__((visibility("default"))) int shm_get_timing(
+    shm_ctx *ctx, double *stage_ms, long long *stage_launches,
+    long long *stage_cells, double *halo_ms, long long *halo_launches) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        const shm::Timing &t = ctx->rt->timing;
+        if (stage_ms) *stage_ms = t.stage_ms;
+        if (stage_launches) *stage_launches = t.stage_launches;
+        if (stage_cells) *stage_cells = t.stage_cells;
+        if (halo_ms) *halo_ms = t.halo_ms;
+        if (halo_launches) *halo_launches = t.halo_launches;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Plain-old-data descriptors shared by host runtime and device kernels.
+//
+// HBM layout of a level ("component planes"): every array of a level holds
+// ncomp planes of `cstride` elements.  Inside a plane each box owns a padded
+// brick: x fastest, then y, then z (the Array4 order of the reference), with
+// g ghost layers on every side.  The first valid cell (lo) of a box sits at
+// element `base`, which is 32-element aligned, and row/plane pitches are
+// multiples of 32 elements, so every 64-wide x row of valid cells starts on a
+// 128 B (FP32) / 256 B (FP64) boundary.
+#pragma once
+
+#include <cstdint>
+
+namespace shm {
+
+struct BoxDesc {
+    int lo[3];        // global index of the first valid cell
+    int n[3];         // valid extents
+    long long base;   // element offset of cell lo inside a component plane
+    int jstride;      // elements per row
+    int kstride;      // elements per xy-plane
+};
+
+/** One work unit of a fused stage kernel: an x/y tile of one box marched
+ *  along z over [z0, z1).  Coordinates are box-local (0 = first valid). */
+struct TileDesc {
+    int box;
+    int x0, y0;
+    int z0, z1;
+};
+
+/** Ghost-cell copy segment: destination cells [lo, lo+n) (global index, in
+ *  dst_box's ghost region) take the values of src_box at index - shift,
+ *  where shift is a multiple of the domain size per axis (periodic wrap). */
+struct HaloSeg {
+    int dst_box, src_box;
+    int lo[3];
+    int n[3];
+    int shift[3];
+    long long start;  // prefix sum of segment volumes
+};
+
+/** Coarse/fine pairing for truncation errors: one coarse box per fine box
+ *  (the coarse box is the fine box coarsened by 2). */
+struct TePair {
+    int fine_box;
+    int crse_box;
+};
+
+}  // namespace shm

@@ -1,0 +1,1544 @@
+// Host runtime for the MI355X time-step path.  See shm_runtime.hpp.
+//
+// Reference behaviour followed (file:line in the sledgehamr sources):
+//   Integrator::Advance                 integrators/integrator.cpp:11-31
+//   IntegratorLsssprk3::Integrate       integrators/lsssprk3.cpp:14-31
+//   IntegratorRkn::Integrate, tableaux  integrators/rkn.cpp:25-205
+//   IntegratorLeapfrog::Integrate       integrators/leapfrog.cpp:13-57
+//   FillPatch / FillIntermediatePatch   level_synchronizer.cpp:94-214
+//   ComputeTruncationErrors             level_synchronizer.cpp:240-357
+//   CreateShadowLevel                   sledgehamr.cpp:234-251
+//   ParseInput / ParseInputScalars      sledgehamr.cpp:389-548
+//   per-level dx/dt                     sledgehamr.cpp:27-31
+#include "shm_runtime.hpp"
+
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <set>
+#include <sstream>
+
+#include "shm_aux.hpp"
+
+// ---------------------------------------------------------------------------
+// ParmParse table.
+// ---------------------------------------------------------------------------
+
+namespace {
+std::map<std::string, std::vector<std::string>> &pp_table() {
+    static std::map<std::string, std::vector<std::string>> t;
+    return t;
+}
+}  // namespace
+
+namespace amrex {
+
+void ParmParse::ParseText(const std::string &text) {
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line = line.substr(0, hash);
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        std::string key = line.substr(0, eq);
+        key.erase(0, key.find_first_not_of(" \t"));
+        key.erase(key.find_last_not_of(" \t\r") + 1);
+        if (key.empty()) continue;
+        std::istringstream vs(line.substr(eq + 1));
+        std::vector<std::string> vals;
+        std::string tok;
+        while (vs >> tok) {
+            if (tok.size() >= 2 && tok.front() == '"' && tok.back() == '"')
+                tok = tok.substr(1, tok.size() - 2);
+            vals.push_back(tok);
+        }
+        pp_table()[key] = vals;
+    }
+}
+
+void ParmParse::Clear() { pp_table().clear(); }
+
+bool ParmParse::Contains(const std::string &full_key) {
+    return pp_table().count(full_key) != 0;
+}
+
+const std::vector<std::string> *ParmParse::find(const char *name) const {
+    auto it = pp_table().find(full(name));
+    return it == pp_table().end() ? nullptr : &it->second;
+}
+
+}  // namespace amrex
+
+namespace shm {
+
+#define SHM_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            throw Error(2, std::string("HIP error ") + hipGetErrorString(e_) + \
+                               " at " #call);                                  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device memory and layouts.
+// ---------------------------------------------------------------------------
+
+void DevMem::alloc(size_t bytes) {
+    reset();
+    if (bytes == 0) return;
+    SHM_HIP(hipMalloc(&p_, bytes));
+    SHM_HIP(hipMemset(p_, 0, bytes));
+    bytes_ = bytes;
+}
+
+void DevMem::reset() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    bytes_ = 0;
+}
+
+template <class T>
+static void upload_vec(DevMem &d, const std::vector<T> &v) {
+    d.alloc(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty())
+        SHM_HIP(hipMemcpy(d.get(), v.data(), v.size() * sizeof(T),
+                          hipMemcpyHostToDevice));
+}
+
+static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
+                   bool is_periodic) {
+    boxes = b;
+    g = ghost;
+    N = domain_n;
+    periodic = is_periodic;
+    hbox.clear();
+    vprefix.assign(1, 0);
+    cprefix.assign(1, 0);
+    long long end = 0;
+    for (const Box3 &bx : boxes) {
+        for (int d = 0; d < 3; ++d)
+            if (bx.n(d) <= 0)
+                throw Error(1, "box with non-positive extent");
+        BoxDesc bd;
+        for (int d = 0; d < 3; ++d) {
+            bd.lo[d] = bx.lo[d];
+            bd.n[d] = bx.n(d);
+        }
+        bd.jstride = (int)round_up(bx.n(0) + 2 * g, 32);
+        bd.kstride = bd.jstride * (bx.n(1) + 2 * g);
+        // Place the brick so that the first valid cell is 32-element aligned.
+        const long long start = round_up(end + g, 32) - g;
+        bd.base = start + (long long)g * bd.kstride + (long long)g * bd.jstride + g;
+        end = start + (long long)bd.kstride * (bx.n(2) + 2 * g);
+        hbox.push_back(bd);
+        vprefix.push_back(vprefix.back() + bx.vol());
+        cprefix.push_back(cprefix.back() + (long long)(bx.n(0) / 2) *
+                                               (bx.n(1) / 2) * (bx.n(2) / 2));
+    }
+    cstride = round_up(end + 64, 64);
+    upload_vec(dbox, hbox);
+    upload_vec(dvprefix, vprefix);
+    upload_vec(dcprefix, cprefix);
+
+    // Ghost copy plan: intersect each box's ghost shell with every
+    // (periodically shifted) valid box, itself included for shift != 0.
+    halo.clear();
+    halo_total = 0;
+    const int smax = periodic ? 1 : 0;
+    for (int d = 0; d < (int)boxes.size(); ++d) {
+        const Box3 &db = boxes[d];
+        int glo[3], ghi[3];
+        for (int a = 0; a < 3; ++a) {
+            glo[a] = db.lo[a] - g;
+            ghi[a] = db.hi[a] + g;
+        }
+        if (g == 0) continue;
+        for (int s = 0; s < (int)boxes.size(); ++s) {
+            const Box3 &sb = boxes[s];
+            for (int sz = -smax; sz <= smax; ++sz)
+                for (int sy = -smax; sy <= smax; ++sy)
+                    for (int sx = -smax; sx <= smax; ++sx) {
+                        if (s == d && sx == 0 && sy == 0 && sz == 0) continue;
+                        const int sh[3] = {sx * N, sy * N, sz * N};
+                        HaloSeg seg;
+                        bool empty = false;
+                        for (int a = 0; a < 3; ++a) {
+                            const int lo = std::max(glo[a], sb.lo[a] + sh[a]);
+                            const int hi = std::min(ghi[a], sb.hi[a] + sh[a]);
+                            if (hi < lo) {
+                                empty = true;
+                                break;
+                            }
+                            seg.lo[a] = lo;
+                            seg.n[a] = hi - lo + 1;
+                            seg.shift[a] = sh[a];
+                        }
+                        if (empty) continue;
+                        seg.dst_box = d;
+                        seg.src_box = s;
+                        seg.start = halo_total;
+                        halo_total += (long long)seg.n[0] * seg.n[1] * seg.n[2];
+                        halo.push_back(seg);
+                    }
+        }
+    }
+    upload_vec(dhalo, halo);
+    tiles.clear();
+}
+
+const TileDesc *Layout::get_tiles(int ty, int tz, int *ntiles) {
+    auto key = std::make_pair(ty, tz);
+    auto it = tiles.find(key);
+    if (it == tiles.end()) {
+        std::vector<TileDesc> tv;
+        for (int b = 0; b < (int)boxes.size(); ++b) {
+            const Box3 &bx = boxes[b];
+            for (int z0 = 0; z0 < bx.n(2); z0 += tz)
+                for (int y0 = 0; y0 < bx.n(1); y0 += ty)
+                    for (int x0 = 0; x0 < bx.n(0); x0 += 64) {
+                        TileDesc t;
+                        t.box = b;
+                        t.x0 = x0;
+                        t.y0 = y0;
+                        t.z0 = z0;
+                        t.z1 = std::min(z0 + tz, bx.n(2));
+                        tv.push_back(t);
+                    }
+        }
+        auto mem = std::make_shared<DevMem>();
+        upload_vec(*mem, tv);
+        it = tiles.emplace(key, std::make_pair(mem, (int)tv.size())).first;
+    }
+    *ntiles = it->second.second;
+    return it->second.first->as<TileDesc>();
+}
+
+/** Copy plan from `src` boxes onto `dst` boxes (valid regions, no shift). */
+static std::vector<HaloSeg> copy_plan(const Layout &dst, const Layout &src,
+                                      long long *total) {
+    std::vector<HaloSeg> segs;
+    *total = 0;
+    for (int d = 0; d < (int)dst.boxes.size(); ++d)
+        for (int s = 0; s < (int)src.boxes.size(); ++s) {
+            HaloSeg seg;
+            bool empty = false;
+            for (int a = 0; a < 3; ++a) {
+                const int lo = std::max(dst.boxes[d].lo[a], src.boxes[s].lo[a]);
+                const int hi = std::min(dst.boxes[d].hi[a], src.boxes[s].hi[a]);
+                if (hi < lo) {
+                    empty = true;
+                    break;
+                }
+                seg.lo[a] = lo;
+                seg.n[a] = hi - lo + 1;
+                seg.shift[a] = 0;
+            }
+            if (empty) continue;
+            seg.dst_box = d;
+            seg.src_box = s;
+            seg.start = *total;
+            *total += (long long)seg.n[0] * seg.n[1] * seg.n[2];
+            segs.push_back(seg);
+        }
+    return segs;
+}
+
+static std::vector<Box3> coarsen(const std::vector<Box3> &b) {
+    std::vector<Box3> c;
+    for (const Box3 &x : b) {
+        Box3 y;
+        for (int a = 0; a < 3; ++a) {
+            if ((x.lo[a] & 1) || (x.n(a) & 1))
+                throw Error(1, "box not coarsenable by 2 (odd corner or size)");
+            y.lo[a] = x.lo[a] >> 1;
+            y.hi[a] = ((x.hi[a] + 1) >> 1) - 1;
+        }
+        c.push_back(y);
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// Runtime.
+// ---------------------------------------------------------------------------
+
+constexpr int kPrmSlots = 1024;
+
+Runtime::Runtime(sledgehamr::Sledgehamr *sim, const KernelTable &kt)
+    : sim_(sim), kt_(kt) {
+    int dev = 0;
+    SHM_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SHM_HIP(hipGetDeviceProperties(&prop, dev));
+    ncu_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    SHM_HIP(hipStreamCreate(&stream_));  // blocking: ordered after null-stream memsets
+    prm_dev_.alloc(sizeof(double) * kPrmBlock * kPrmSlots);
+    SHM_HIP(hipHostMalloc((void **)&prm_host_,
+                          sizeof(double) * kPrmBlock * kPrmSlots,
+                          hipHostMallocDefault));
+    std::memset(prm_host_, 0, sizeof(double) * kPrmBlock * kPrmSlots);
+    prm_ev_.assign(kPrmSlots, nullptr);
+    SHM_HIP(hipEventCreate(&ev0_));
+    SHM_HIP(hipEventCreate(&ev1_));
+}
+
+Runtime::~Runtime() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (hipEvent_t e : prm_ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (prm_host_) (void)hipHostFree(prm_host_);
+    levels_.clear();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+LevelState &Runtime::level(int lev) {
+    if (lev == -1) return shadow_;
+    if (!has_level(lev))
+        throw Error(1, "level " + std::to_string(lev) + " is not defined");
+    return levels_[lev];
+}
+
+std::vector<int> Runtime::all_comps() const {
+    std::vector<int> c(ncomp_);
+    for (int n = 0; n < ncomp_; ++n) c[n] = n;
+    return c;
+}
+
+std::vector<int> Runtime::stencil_comps() const {
+    std::vector<int> c;
+    for (int n = 0; n < ncomp_; ++n) {
+        const bool s = ko_ || n < kt_.nfields ||
+                       (gw_ && n >= kt_.nscalars && n < kt_.nscalars + 6);
+        if (s) c.push_back(n);
+    }
+    return c;
+}
+
+void Runtime::init_from_inputs(const std::string &inputs,
+                               int precision_override) {
+    amrex::ParmParse::Clear();
+    amrex::ParmParse::ParseText(inputs);
+    amrex::ParmParse pp("");
+    sledgehamr::Sledgehamr &S = *sim_;
+
+    std::string pname;
+    if (pp.query("project.name", pname) && pname != S.project_name)
+        throw Error(1, "inputs select project '" + pname +
+                           "' but this library holds '" + S.project_name + "'");
+
+    pp.query("amr.nghost", S.nghost);
+    pp.get("amr.coarse_level_grid_size", S.coarse_level_grid_size);
+    if (!sledgehamr::utils::IsPowerOfTwo(S.coarse_level_grid_size))
+        throw Error(1, "amr.coarse_level_grid_size needs to be a power of 2!");
+    S.max_level = 0;
+    pp.query("amr.max_refinement_levels", S.max_level);
+    pp.get("sim.t_start", S.t_start);
+    pp.get("sim.t_end", S.t_end);
+    pp.get("sim.L", S.L);
+    pp.get("sim.cfl", S.cfl);
+    pp.query("sim.gravitational_waves", S.with_gravitational_waves);
+    pp.query("amr.tagging_on_gpu", S.tagging_on_gpu);
+    S.dimN.clear();
+    S.dx.clear();
+    S.dt.clear();
+    for (int lev = 0; lev <= S.max_level; ++lev) {
+        S.dimN.push_back(S.coarse_level_grid_size * (1 << lev));
+        S.dx.push_back(S.L / (double)S.dimN[lev]);
+        S.dt.push_back(S.dx[lev] * S.cfl);
+    }
+
+    gw_ = S.with_gravitational_waves;
+    if (gw_ && !kt_.built_gw)
+        throw Error(4, "gravitational waves requested but this library was "
+                       "built without GW kernels");
+    ncomp_ = gw_ ? kt_.ngw : kt_.nscalars;
+    if ((int)S.scalar_fields.size() != kt_.nscalars)
+        throw Error(3, "scalar field registry does not match the kernel table");
+    std::vector<std::string> names;
+    for (auto *f : S.scalar_fields) names.push_back(f->name);
+    if (gw_)
+        for (const char *n : {"u_xx", "u_yy", "u_zz", "u_xy", "u_xz", "u_yz",
+                              "du_xx", "du_yy", "du_zz", "du_xy", "du_xz",
+                              "du_yz"})
+            names.push_back(n);
+
+    // Truncation-error thresholds and dissipation (ParseInputScalars).
+    S.te_crit.assign(ncomp_, DBL_MAX);
+    double te_default = DBL_MAX;
+    pp.query("amr.te_crit", te_default);
+    if (!(te_default > 0)) throw Error(1, "amr.te_crit needs to be > 0!");
+    S.shadow_hierarchy = false;
+    for (int n = 0; n < ncomp_; ++n) {
+        S.te_crit[n] = te_default;
+        pp.query(("amr.te_crit_" + names[n]).c_str(), S.te_crit[n]);
+        if (!(S.te_crit[n] > 0))
+            throw Error(1, "amr.te_crit_" + names[n] + " needs to be > 0!");
+        if (S.te_crit[n] != DBL_MAX) S.shadow_hierarchy = true;
+    }
+    S.dissipation_strength.assign(ncomp_, 0.);
+    double ko_default = 0;
+    pp.query("sim.dissipation_strength", ko_default);
+    S.with_dissipation = false;
+    for (int n = 0; n < ncomp_; ++n) {
+        S.dissipation_strength[n] = ko_default;
+        pp.query(("sim.dissipation_strength_" + names[n]).c_str(),
+                 S.dissipation_strength[n]);
+        if (S.dissipation_strength[n] < 0)
+            throw Error(1, "dissipation strength needs to be >= 0!");
+        if (S.dissipation_strength[n] > 0) S.with_dissipation = true;
+    }
+    ko_ = S.with_dissipation;
+    if (ko_) {
+        S.dissipation_order = S.nghost;
+        pp.query("sim.dissipation_order", S.dissipation_order);
+        if (S.dissipation_order != 2 && S.dissipation_order != 3)
+            throw Error(1, "Currently only sim.dissipation_order = 2 or 3 supported!");
+        if (!kt_.built_ko)
+            throw Error(4, "dissipation requested but this library was built "
+                           "without Kreiss-Oliger kernels");
+    }
+    ko_dev_.alloc(sizeof(double) * std::max(1, ncomp_));
+    SHM_HIP(hipMemcpy(ko_dev_.get(), S.dissipation_strength.data(),
+                      sizeof(double) * ncomp_, hipMemcpyHostToDevice));
+
+    // Integrator (TimeStepper::SetIntegrator, rkn.cpp:188-271).
+    pp.get("integrator.type", integrator_);
+    switch (integrator_) {
+        case 10:
+        case 11:
+            break;
+        case 20: {
+            amrex::ParmParse pr("integrator.rkn");
+            pr.getarr("weights_bar_b", bbar_);
+            pr.getarr("weights_b", b_);
+            pr.getarr("nodes", nodes_);
+            std::vector<double> bt;
+            pr.getarr("tableau", bt);
+            const int s = (int)nodes_.size();
+            if ((int)bbar_.size() != s || (int)b_.size() != s)
+                throw Error(1, "integrator.weights should be the same length "
+                               "as integrator.nodes");
+            if ((int)bt.size() != s * (s + 1) / 2)
+                throw Error(1, "integrator.tableau incorrect length - should "
+                               "include the Butcher Tableau diagonal.");
+            tableau_.clear();
+            int k = 0;
+            for (int i = 0; i < s; ++i) {
+                std::vector<double> row;
+                for (int j = 0; j <= i; ++j) row.push_back(bt[k++]);
+                if (row.back() != 0.0)
+                    throw Error(1, "RKN integrator currently only supports "
+                                   "explicit Butcher tableaus.");
+                tableau_.push_back(row);
+            }
+            break;
+        }
+        case 21:
+            nodes_ = {0.0, 0.5, 1.0};
+            tableau_ = {{0.0}, {1. / 8., 0.0}, {0.0, 0.5, 0.0}};
+            bbar_ = {1. / 6., 1. / 3., 0.0};
+            b_ = {1. / 6., 4. / 6., 1. / 6.};
+            break;
+        case 22:
+            nodes_ = {0.0, 1. / 5., 2. / 3., 1.0};
+            tableau_ = {{0.0},
+                        {1. / 50., 0.0},
+                        {-1. / 27., 7. / 27., 0.0},
+                        {3. / 10., -2. / 35., 9. / 35., 0.0}};
+            bbar_ = {14. / 336., 100. / 336., 54. / 336., 0.0};
+            b_ = {14. / 336., 125. / 336., 162. / 336., 35. / 336.};
+            break;
+        default:
+            throw Error(4, "#error: Unknown integration type: " +
+                               std::to_string(integrator_));
+    }
+    if (integrator_ >= 20 && (nodes_.empty() || nodes_.size() > 5))
+        throw Error(4, "RKN tableaus with 1..5 stages are supported");
+    if ((integrator_ >= 20 || integrator_ == 11) &&
+        2 * kt_.nfields != kt_.nscalars)
+        throw Error(4, "RKN/leapfrog need one conjugate momentum per field");
+    if (integrator_ == 11 && ko_)
+        throw Error(4, "leapfrog with Kreiss-Oliger dissipation is not supported");
+
+    std::string prec;
+    prec_ = Prec::F32C;
+    if (pp.query("shm.precision", prec))
+        prec_ = (prec == "fp64" || prec == "FP64") ? Prec::F64 : Prec::F32C;
+    if (precision_override == 0) prec_ = Prec::F64;
+    if (precision_override == 1) prec_ = Prec::F32C;
+    pp.query("shm.tile_variant", variant);
+    pp.query("shm.tile_z", tz);
+    pp.query("shm.xcd_split", xcd_split);
+
+    // Project initialisation, then the reach of its cell functions.
+    sim_->Init();
+    probe();
+    if (R_ > S.nghost)
+        throw Error(1, "cell functions reach " + std::to_string(R_) +
+                           " cells but amr.nghost = " + std::to_string(S.nghost));
+
+    // Level 0: the periodic domain, optionally chopped (amr.max_grid_size).
+    const int N0 = S.coarse_level_grid_size;
+    int mgs = N0;
+    pp.query("amr.max_grid_size", mgs);
+    if (mgs <= 0 || N0 % mgs != 0) throw Error(1, "amr.max_grid_size must divide the grid");
+    std::vector<Box3> boxes;
+    for (int z = 0; z < N0; z += mgs)
+        for (int y = 0; y < N0; y += mgs)
+            for (int x = 0; x < N0; x += mgs)
+                boxes.push_back(Box3{{x, y, z}, {x + mgs - 1, y + mgs - 1, z + mgs - 1}});
+    levels_.clear();
+    levels_.resize(S.max_level + 1);
+    set_level(0, boxes);
+    S.finest_level = 0;
+    tuned_variant_.assign(S.max_level + 2, -1);
+    tuned_tz_.assign(S.max_level + 2, 0);
+    regrid_dt_.clear();
+    last_regrid_.clear();
+    double reg_dt = DBL_MAX;
+    pp.query("amr.regrid_dt", reg_dt);
+    for (int lev = 0; lev <= S.max_level; ++lev) {
+        regrid_dt_.push_back(reg_dt / std::pow(2., lev));
+        last_regrid_.push_back(S.t_start);
+    }
+}
+
+void Runtime::probe() {
+    const int gwi = gw_ ? 1 : 0;
+    if (!kt_.probe[gwi]) throw Error(4, "probe kernel missing");
+    DevMem out;
+    out.alloc(sizeof(ProbeOut));
+    std::vector<double> blk(kPrmBlock, 0.);
+    fill_params(blk.data(), sim_->t_start, sim_->dt[0], sim_->dx[0], 0);
+    DevMem d;
+    upload_vec(d, blk);
+    SHM_HIP(kt_.probe[gwi](out.as<ProbeOut>(), d.as<double>() + kPrmUser,
+                           d.as<double>() + kPrmGw, sim_->t_start, sim_->dt[0],
+                           sim_->dx[0], stream_));
+    ProbeOut po;
+    SHM_HIP(hipMemcpyAsync(&po, out.get(), sizeof(po), hipMemcpyDeviceToHost,
+                           stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+    int R = 1;
+    const std::vector<int> sc = stencil_comps();
+    for (int n = 0; n < ncomp_; ++n) {
+        const bool is_st = std::find(sc.begin(), sc.end(), n) != sc.end();
+        if (po.offaxis[n])
+            throw Error(4, "cell functions read component " + std::to_string(n) +
+                               " at a point off the z axis and off the xy "
+                               "plane; the tile view supports star stencils "
+                               "and in-plane diagonals");
+        if (!is_st && po.reach[n] > 0)
+            throw Error(4, "cell functions difference component " +
+                               std::to_string(n) +
+                               " which is stored point-wise (momenta are "
+                               "read at the centre only)");
+        R = std::max(R, po.reach[n]);
+    }
+    if (ko_) R = std::max(R, sim_->dissipation_order);
+    if (R > 3) throw Error(4, "stencil radius above 3 is not supported");
+    if (!((kt_.r_mask >> R) & 1))
+        throw Error(4, "stencil radius " + std::to_string(R) +
+                           " not built into this library");
+    R_ = R;
+}
+
+void Runtime::set_level(int lev, const std::vector<Box3> &boxes) {
+    if (lev < 0 || lev > sim_->max_level) throw Error(1, "level out of range");
+    if ((int)levels_.size() <= lev) levels_.resize(lev + 1);
+    LevelState &ls = levels_[lev];
+    ls = LevelState();
+    const int N = sim_->dimN[lev];
+    for (const Box3 &b : boxes)
+        for (int a = 0; a < 3; ++a)
+            if (b.lo[a] < 0 || b.hi[a] >= N || b.hi[a] < b.lo[a])
+                throw Error(1, "box outside the level domain");
+    ls.L.build(boxes, sim_->nghost, N, /*periodic=*/true);
+    ls.ncomp = ncomp_;
+    ensure_arrays(ls, integrator_);
+    ls.t = sim_->t_start;
+    ls.t_old = -DBL_MAX;
+    ls.istep = 0;
+    if (lev > sim_->finest_level) sim_->finest_level = lev;
+}
+
+void Runtime::ensure_arrays(LevelState &ls, int integrator) {
+    const size_t plane = (size_t)ls.L.cstride;
+    const size_t nc = ls.ncomp;
+    const size_t es = esize();
+    const int npair = kt_.nfields + (gw_ ? 6 : 0);
+    for (int b = 0; b < 2; ++b) {
+        if (!ls.U[b]) ls.U[b].alloc(plane * nc * es);
+        if (prec_ == Prec::F32C && !ls.Ulo[b]) ls.Ulo[b].alloc(plane * nc * 4);
+    }
+    if (integrator == 10) {
+        if (!ls.S1) ls.S1.alloc(plane * nc * es);
+        if (!ls.S2) ls.S2.alloc(plane * nc * es);
+        if (!ls.K) ls.K.alloc(plane * nc * es);
+    } else if (integrator >= 20) {
+        for (size_t i = 0; i + 1 < nodes_.size(); ++i)
+            if (!ls.F[i]) ls.F[i].alloc(plane * npair * es);
+    } else if (integrator == 11) {
+        if (!ls.VH) ls.VH.alloc(plane * npair * 8);
+    }
+}
+
+void Runtime::fill_params(double *blk, double time, double dt, double dx,
+                          int lev) {
+    std::fill(blk, blk + kPrmBlock, 0.);
+    blk[kPrmTime] = time;
+    blk[kPrmDt] = dt;
+    blk[kPrmDx] = dx;
+    std::vector<double> p;
+    sim_->SetParamsRhs(p, time, lev);
+    if ((int)p.size() > kMaxParams) throw Error(4, "too many Rhs parameters");
+    std::copy(p.begin(), p.end(), blk + kPrmUser);
+    if (gw_) {
+        std::vector<double> pg;
+        sim_->SetParamsGravitationalWaveRhs(pg, time, lev);
+        if ((int)pg.size() > kMaxParams) throw Error(4, "too many GW parameters");
+        std::copy(pg.begin(), pg.end(), blk + kPrmGw);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers.
+// ---------------------------------------------------------------------------
+
+void Runtime::launch_stage(int pol, LevelState &ls, StageArgs &a, int lev) {
+    const int pi = (int)prec_;
+    int v = variant >= 0 ? variant
+                         : (lev + 1 < (int)tuned_variant_.size() && lev >= -1 &&
+                                    tuned_variant_[lev + 1] >= 0
+                                ? tuned_variant_[lev + 1]
+                                : 1);
+    const StageEntry &e = kt_.stage[pol][pi][R_][gw_ ? 1 : 0][ko_ ? 1 : 0][v];
+    if (!e.fn) throw Error(4, "stage kernel not built for this configuration");
+    int z = tz > 0 ? tz
+                   : (lev + 1 < (int)tuned_tz_.size() && lev >= -1 &&
+                              tuned_tz_[lev + 1] > 0
+                          ? tuned_tz_[lev + 1]
+                          : 0);
+    if (z <= 0) {
+        // Enough tiles to fill every CU several times, long z runs otherwise.
+        int maxn = 1;
+        long long xy = 0;
+        for (const Box3 &b : ls.L.boxes) {
+            maxn = std::max(maxn, b.n(2));
+            xy += (long long)((b.n(0) + 63) / 64) * ((b.n(1) + e.ty - 1) / e.ty);
+        }
+        z = maxn;
+        while (z > 16 && xy * ((maxn + z - 1) / z) < 8LL * ncu_) z = (z + 1) / 2;
+    }
+    int ntiles = 0;
+    a.tiles = ls.L.get_tiles(e.ty, z, &ntiles);
+    a.boxes = ls.L.dbox.as<BoxDesc>();
+    a.ntiles = ntiles;
+    a.cstride = ls.L.cstride;
+    a.lev = lev;
+    a.ko_strength = ko_dev_.as<double>();
+    a.ko_order = sim_->dissipation_order;
+    if (ntiles == 0) return;
+    const int occ = e.occupancy();
+    int grid = std::min<long long>(ntiles, (long long)occ * ncu_);
+    a.xcd_split = xcd_split ? 1 : 0;
+    if (a.xcd_split) grid = (int)round_up(grid, 8);
+    if (timing_on) SHM_HIP(hipEventRecord(ev0_, stream_));
+    SHM_HIP(e.fn(a, grid, stream_));
+    if (timing_on) {
+        SHM_HIP(hipEventRecord(ev1_, stream_));
+        SHM_HIP(hipEventSynchronize(ev1_));
+        float ms = 0;
+        SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        timing.stage_ms += ms;
+        timing.stage_launches += 1;
+        timing.stage_cells += ls.L.nvalid();
+    }
+}
+
+void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps) {
+    if (ls.L.halo_total == 0 || comps.empty()) return;
+    HaloArgs h;
+    h.arr = arr;
+    h.cstride = ls.L.cstride;
+    h.boxes = ls.L.dbox.as<BoxDesc>();
+    h.segs = ls.L.dhalo.as<HaloSeg>();
+    h.nseg = (int)ls.L.halo.size();
+    h.total = ls.L.halo_total;
+    for (size_t i = 0; i < comps.size(); i += 32) {
+        h.ncomp = (int)std::min<size_t>(32, comps.size() - i);
+        for (int c = 0; c < h.ncomp; ++c) h.comps[c] = comps[i + c];
+        const int grid = (int)std::min<long long>((h.total + 255) / 256, 8LL * ncu_);
+        if (timing_on) SHM_HIP(hipEventRecord(ev0_, stream_));
+        if (prec_ == Prec::F64)
+            hipLaunchKernelGGL(halo_kernel<double>, dim3(grid), dim3(256), 0,
+                               stream_, h);
+        else
+            hipLaunchKernelGGL(halo_kernel<float>, dim3(grid), dim3(256), 0,
+                               stream_, h);
+        SHM_HIP(hipGetLastError());
+        if (timing_on) {
+            SHM_HIP(hipEventRecord(ev1_, stream_));
+            SHM_HIP(hipEventSynchronize(ev1_));
+            float ms = 0;
+            SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            timing.halo_ms += ms;
+            timing.halo_launches += 1;
+        }
+    }
+}
+
+void Runtime::fill_patch(LevelState &ls, int which) {
+    if (ls.ghosts_ok[which]) return;
+    halo(ls, ls.U[which].get(), all_comps());
+    ls.ghosts_ok[which] = true;
+}
+
+// ---------------------------------------------------------------------------
+// Integrators.
+// ---------------------------------------------------------------------------
+
+/** Stage record ring: the host writes a pinned slot, the stream copies it to
+ *  the matching device slot; a slot is rewritten only after its copy ran. */
+static const double *push_params(double *host_ring,
+                                 std::vector<hipEvent_t> &evs, int &slot,
+                                 double *dev_ring, hipStream_t s,
+                                 const std::vector<double> &blk) {
+    const int i = slot;
+    slot = (slot + 1) % kPrmSlots;
+    if (evs[i]) SHM_HIP(hipEventSynchronize(evs[i]));
+    else SHM_HIP(hipEventCreateWithFlags(&evs[i], hipEventDisableTiming));
+    double *h = host_ring + (size_t)i * kPrmBlock;
+    std::copy(blk.begin(), blk.end(), h);
+    double *d = dev_ring + (size_t)i * kPrmBlock;
+    SHM_HIP(hipMemcpyAsync(d, h, sizeof(double) * kPrmBlock,
+                           hipMemcpyHostToDevice, s));
+    SHM_HIP(hipEventRecord(evs[i], s));
+    return d;
+}
+
+void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
+                        double t0) {
+    const int old = ls.cur ^ 1, nw = ls.cur;
+    std::vector<double> blk(kPrmBlock);
+    auto prm = [&](double time) {
+        fill_params(blk.data(), time, dt, dx, lev);
+        return push_params(prm_host_, prm_ev_, prm_slot_,
+                           prm_dev_.as<double>(), stream_, blk);
+    };
+    const int npair = kt_.nfields + (gw_ ? 6 : 0);
+    std::vector<int> pairs(npair);
+    for (int p = 0; p < npair; ++p) pairs[p] = p;
+    void *ulo_old = prec_ == Prec::F32C ? ls.Ulo[old].get() : nullptr;
+    void *ulo_new = prec_ == Prec::F32C ? ls.Ulo[nw].get() : nullptr;
+    ls.ghosts_ok[nw] = false;
+
+    if (integrator_ == 10) {
+        const double t1 = t0 + dt;
+        StageArgs a{};
+        a.prm = prm(t0);
+        a.x = ls.U[old].get();
+        a.flags = PolSsprk::kWriteK | PolSsprk::kXisU0;
+        a.kout = ls.K.get();
+        a.y = ls.S1.get();
+        a.c[1] = dt;
+        launch_stage(kPolSsprk, ls, a, lev);
+        halo(ls, ls.S1.get(), stencil_comps());
+
+        StageArgs b{};
+        b.prm = prm(t1);
+        b.x = ls.S1.get();
+        b.u0 = ls.U[old].get();
+        b.kin = ls.K.get();
+        b.kout = ls.K.get();
+        b.y = ls.S2.get();
+        b.flags = PolSsprk::kHasKin | PolSsprk::kWriteK;
+        b.c[0] = 1.;
+        b.c[1] = dt / 4.;
+        launch_stage(kPolSsprk, ls, b, lev);
+        halo(ls, ls.S2.get(), stencil_comps());
+
+        StageArgs c{};
+        c.prm = prm(t0 + dt / 2.);
+        c.x = ls.S2.get();
+        c.u0 = ls.U[old].get();
+        c.u0lo = ulo_old;
+        c.kin = ls.K.get();
+        c.unew = ls.U[nw].get();
+        c.unewlo = ulo_new;
+        c.flags = PolSsprk::kHasKin | PolSsprk::kFinal;
+        c.c[0] = 0.25;
+        c.c[1] = dt * 2. / 3.;
+        launch_stage(kPolSsprk, ls, c, lev);
+    } else if (integrator_ >= 20) {
+        const int s = (int)nodes_.size();
+        for (int i = 0; i < s; ++i) {
+            StageArgs a{};
+            a.prm = prm(t0 + dt * nodes_[i]);
+            a.u0 = ls.U[old].get();
+            a.u0lo = ulo_old;
+            a.c[0] = dt * nodes_[i];
+            a.nprev = i;
+            for (int j = 0; j < i; ++j) {
+                a.c[1 + j] = dt * dt * tableau_[i][j];
+                a.f[j] = ls.F[j].get();
+            }
+            if (i < s - 1) {
+                a.fout = ls.F[i].get();
+                launch_stage(kPolRkn, ls, a, lev);
+                halo(ls, ls.F[i].get(), pairs);
+            } else {
+                a.flags = PolRkn::kFinal;
+                a.c[6] = dt;
+                for (int j = 0; j < i; ++j) {
+                    a.c[7 + j] = dt * dt * bbar_[j];
+                    a.c[12 + j] = dt * b_[j];
+                }
+                a.c[11] = dt * dt * bbar_[i];
+                a.c[16] = dt * b_[i];
+                a.unew = ls.U[nw].get();
+                a.unewlo = ulo_new;
+                launch_stage(kPolRkn, ls, a, lev);
+            }
+        }
+    } else if (integrator_ == 11) {
+        const double t1 = t0 + dt;
+        StageArgs a{};
+        a.prm = prm(t0);
+        a.x = ls.U[old].get();
+        a.u0 = ls.U[old].get();
+        a.u0lo = ulo_old;
+        a.vh = ls.VH.get();
+        a.unew = ls.U[nw].get();
+        a.unewlo = ulo_new;
+        a.c[0] = dt / 2.;
+        a.c[1] = dt;
+        launch_stage(kPolLeapfrog, ls, a, lev);
+        halo(ls, ls.U[nw].get(), stencil_comps());
+
+        StageArgs b{};
+        b.prm = prm(t1);
+        b.x = ls.U[nw].get();
+        b.vh = ls.VH.get();
+        b.unew = ls.U[nw].get();
+        b.unewlo = ulo_new;
+        b.flags = PolLeapfrog::kKick2;
+        b.c[0] = dt / 2.;
+        launch_stage(kPolLeapfrog, ls, b, lev);
+    }
+    // FillIntermediatePatch at the end of the step.
+    halo(ls, ls.U[nw].get(), all_comps());
+    ls.ghosts_ok[nw] = true;
+}
+
+void Runtime::advance(int lev) {
+    LevelState &ls = level(lev);
+    ls.cur ^= 1;  // swap grid_old / grid_new
+    const int old = ls.cur ^ 1;
+    ls.t_old = ls.t;
+    const double dt = sim_->dt[lev], dx = sim_->dx[lev];
+    fill_patch(ls, old);
+    integrate(ls, lev, dt, dx, ls.t_old);
+    ls.t = ls.t_old + dt;
+    ls.istep += 1;
+}
+
+void Runtime::fill_rhs(int lev, double time, double weight, bool add) {
+    LevelState &ls = level(lev);
+    if (!ls.RHS) ls.RHS.alloc((size_t)ls.L.cstride * ls.ncomp * esize());
+    fill_patch(ls, ls.cur);
+    std::vector<double> blk(kPrmBlock);
+    fill_params(blk.data(), time, sim_->dt[std::max(lev, 0)],
+                sim_->dx[std::max(lev, 0)], lev);
+    StageArgs a{};
+    a.prm = push_params(prm_host_, prm_ev_, prm_slot_,
+                        prm_dev_.as<double>(), stream_, blk);
+    a.x = ls.U[ls.cur].get();
+    a.kin = ls.RHS.get();
+    a.kout = ls.RHS.get();
+    a.flags = PolSsprk::kFillOnly | (add ? PolSsprk::kHasKin : 0);
+    a.c[0] = weight;
+    launch_stage(kPolSsprk, ls, a, lev);
+}
+
+// ---------------------------------------------------------------------------
+// Shadow level, truncation errors and tags.
+// ---------------------------------------------------------------------------
+
+void Runtime::avg_down(LevelState &fine, int fw, LevelState &crse, int cw) {
+    AvgArgs a{};
+    a.fboxes = fine.L.dbox.as<BoxDesc>();
+    a.cboxes = crse.L.dbox.as<BoxDesc>();
+    a.fcstride = fine.L.cstride;
+    a.ccstride = crse.L.cstride;
+    a.fine = fine.U[fw].get();
+    a.finelo = prec_ == Prec::F32C ? fine.Ulo[fw].get() : nullptr;
+    a.crse = crse.U[cw].get();
+    a.crselo = prec_ == Prec::F32C ? crse.Ulo[cw].get() : nullptr;
+    a.cprefix = fine.L.dcprefix.as<long long>();
+    a.nbox = (int)fine.L.boxes.size();
+    a.total = fine.L.cprefix.back();
+    a.ncomp = ncomp_;
+    const int grid = (int)std::max<long long>(1, std::min<long long>((a.total + 255) / 256, 8LL * ncu_));
+    if (prec_ == Prec::F64)
+        hipLaunchKernelGGL(avgdown_kernel<Prec::F64>, dim3(grid), dim3(256), 0,
+                           stream_, a);
+    else
+        hipLaunchKernelGGL(avgdown_kernel<Prec::F32C>, dim3(grid), dim3(256), 0,
+                           stream_, a);
+    SHM_HIP(hipGetLastError());
+    crse.ghosts_ok[cw] = false;
+}
+
+void Runtime::define_shadow() {
+    LevelState &l0 = level(0);
+    if (!shadow_.defined() || shadow_.L.boxes.size() != l0.L.boxes.size()) {
+        shadow_ = LevelState();
+        shadow_.L.build(coarsen(l0.L.boxes), sim_->nghost,
+                        sim_->coarse_level_grid_size / 2, true);
+        shadow_.ncomp = ncomp_;
+        ensure_arrays(shadow_, integrator_);
+        shadow_.t = l0.t;
+    }
+}
+
+void Runtime::create_shadow_level() {
+    LevelState &l0 = level(0);
+    const int old = l0.cur ^ 1;
+    if (l0.t_old == -DBL_MAX)
+        throw Error(3, "no valid grid_old on level 0 to build the shadow level");
+    define_shadow();
+    // shadow_level_tmp = average_down(grid_old[0]) at grid_old[0].t
+    avg_down(l0, old, shadow_, 0);
+    shadow_.t = l0.t_old;
+    // Integrator::Advance(-1): one step of 2 dt on 2 dx from buffer 0 into 1.
+    shadow_.cur = 1;
+    shadow_.t_old = shadow_.t;
+    fill_patch(shadow_, 0);
+    const double dt = sim_->dt[0] * 2., dx = sim_->dx[0] * 2.;
+    integrate(shadow_, -1, dt, dx, shadow_.t_old);
+    shadow_.t = shadow_.t_old + dt;
+}
+
+void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
+                     double *te_host) {
+    LevelState &fine = level(lev);
+    const double time = fine.t;
+    LevelState *crse = nullptr;
+    LevelState mirror;
+    std::vector<HaloSeg> plan_in, plan_out;
+    long long tot_in = 0, tot_out = 0;
+    DevMem dplan_in, dplan_out;
+    if (flags & kTeCompute) {
+        if (lev == 0) {
+            if (!shadow_.defined())
+                throw Error(3, "no shadow level for truncation errors on level 0");
+            if (std::fabs(shadow_.t - fine.t) >= std::fabs(shadow_.t) * 1e-8 &&
+                shadow_.t != fine.t)
+                throw Error(3, "Shadow level not sync'd with coarse level!");
+            crse = &shadow_;
+        } else {
+            LevelState &cl = level(lev - 1);
+            mirror.L.build(coarsen(fine.L.boxes), 0, sim_->dimN[lev - 1], true);
+            mirror.ncomp = ncomp_;
+            mirror.U[0].alloc((size_t)mirror.L.cstride * ncomp_ * esize());
+            if (prec_ == Prec::F32C)
+                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * 4);
+            mirror.cur = 0;
+            plan_in = copy_plan(mirror.L, cl.L, &tot_in);
+            plan_out = copy_plan(cl.L, mirror.L, &tot_out);
+            upload_vec(dplan_in, plan_in);
+            upload_vec(dplan_out, plan_out);
+            auto run_copy = [&](void *dst, const void *src, const Layout &dl,
+                                const Layout &sl, const DevMem &plan, int nseg,
+                                long long total, size_t es) {
+                if (total == 0) return;
+                CopyArgs c{};
+                c.dst = dst;
+                c.src = src;
+                c.dcstride = dl.cstride;
+                c.scstride = sl.cstride;
+                c.dboxes = dl.dbox.as<BoxDesc>();
+                c.sboxes = sl.dbox.as<BoxDesc>();
+                c.segs = plan.as<HaloSeg>();
+                c.nseg = nseg;
+                c.total = total;
+                c.ncomp = ncomp_;
+                const int grid = (int)std::min<long long>((total + 255) / 256, 8LL * ncu_);
+                if (es == 8)
+                    hipLaunchKernelGGL(copy_kernel<double>, dim3(grid), dim3(256),
+                                       0, stream_, c);
+                else
+                    hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256),
+                                       0, stream_, c);
+                SHM_HIP(hipGetLastError());
+            };
+            run_copy(mirror.U[0].get(), cl.U[cl.cur].get(), mirror.L, cl.L,
+                     dplan_in, (int)plan_in.size(), tot_in, esize());
+            if (prec_ == Prec::F32C)
+                run_copy(mirror.Ulo[0].get(), cl.Ulo[cl.cur].get(), mirror.L,
+                         cl.L, dplan_in, (int)plan_in.size(), tot_in, 4);
+            crse = &mirror;
+        }
+    }
+    const sledgehamr::Sledgehamr &S = *sim_;
+    std::vector<double> ptag, pmod;
+    sim_->SetParamsTagCellForRefinement(ptag, time, lev);
+    if (S.shadow_hierarchy) sim_->SetParamsTruncationModifier(pmod, time, lev);
+    DevMem dcrit, dptag, dpmod, dmask, dcounts, dte;
+    upload_vec(dcrit, S.te_crit);
+    ptag.resize(std::max<size_t>(ptag.size(), 1));
+    pmod.resize(std::max<size_t>(pmod.size(), 1));
+    upload_vec(dptag, ptag);
+    upload_vec(dpmod, pmod);
+    dmask.alloc(std::max<long long>(1, fine.L.nvalid()));
+    dcounts.alloc(sizeof(unsigned long long) * (2 + ncomp_));
+    const long long total = fine.L.cprefix.back();
+    if (te_host && (flags & kTeCompute)) dte.alloc(sizeof(double) * total * ncomp_);
+
+    TeTagArgs a{};
+    a.fboxes = fine.L.dbox.as<BoxDesc>();
+    a.cboxes = crse ? crse->L.dbox.as<BoxDesc>() : fine.L.dbox.as<BoxDesc>();
+    a.fcstride = fine.L.cstride;
+    a.ccstride = crse ? crse->L.cstride : 0;
+    a.fine = fine.U[fine.cur].get();
+    a.crse = crse ? crse->U[crse->cur].get() : nullptr;
+    a.crselo = (crse && prec_ == Prec::F32C) ? crse->Ulo[crse->cur].get() : nullptr;
+    a.cprefix = fine.L.dcprefix.as<long long>();
+    a.mprefix = fine.L.dvprefix.as<long long>();
+    a.nbox = (int)fine.L.boxes.size();
+    a.total = total;
+    a.ncomp = ncomp_;
+    a.te_out = dte ? dte.as<double>() : nullptr;
+    a.mask = dmask.as<uint8_t>();
+    a.counts = dcounts.as<unsigned long long>();
+    a.te_crit = dcrit.as<double>();
+    a.params_tag = dptag.as<double>();
+    a.params_mod = dpmod.as<double>();
+    a.lev = lev;
+    a.time = time;
+    a.dt = sim_->dt[lev];
+    a.dx = sim_->dx[lev];
+    a.flags = flags;
+    const TeTagFn fn = kt_.te_tag[(int)prec_][gw_ ? 1 : 0];
+    if (!fn) throw Error(4, "tagging kernel not built");
+    if (total > 0) {
+        const int grid = (int)std::min<long long>((total + 255) / 256, 8LL * ncu_);
+        SHM_HIP(fn(a, grid, stream_));
+    }
+    if (crse && crse == &mirror && (flags & kTeAvgDown) && tot_out > 0) {
+        LevelState &cl = level(lev - 1);
+        CopyArgs c{};
+        c.dst = cl.U[cl.cur].get();
+        c.src = mirror.U[0].get();
+        c.dcstride = cl.L.cstride;
+        c.scstride = mirror.L.cstride;
+        c.dboxes = cl.L.dbox.as<BoxDesc>();
+        c.sboxes = mirror.L.dbox.as<BoxDesc>();
+        c.segs = dplan_out.as<HaloSeg>();
+        c.nseg = (int)plan_out.size();
+        c.total = tot_out;
+        c.ncomp = ncomp_;
+        const int grid = (int)std::min<long long>((tot_out + 255) / 256, 8LL * ncu_);
+        if (prec_ == Prec::F64) {
+            hipLaunchKernelGGL(copy_kernel<double>, dim3(grid), dim3(256), 0,
+                               stream_, c);
+        } else {
+            hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256), 0,
+                               stream_, c);
+            c.dst = cl.Ulo[cl.cur].get();
+            c.src = mirror.Ulo[0].get();
+            hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256), 0,
+                               stream_, c);
+        }
+        SHM_HIP(hipGetLastError());
+        cl.ghosts_ok[cl.cur] = false;
+    }
+    if (mask)
+        SHM_HIP(hipMemcpyAsync(mask, dmask.get(), fine.L.nvalid(),
+                               hipMemcpyDeviceToHost, stream_));
+    std::vector<unsigned long long> hc(2 + ncomp_);
+    SHM_HIP(hipMemcpyAsync(hc.data(), dcounts.get(), hc.size() * 8,
+                           hipMemcpyDeviceToHost, stream_));
+    if (te_host && dte)
+        SHM_HIP(hipMemcpyAsync(te_host, dte.get(), sizeof(double) * total * ncomp_,
+                               hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+    if (counts)
+        for (size_t i = 0; i < hc.size(); ++i) counts[i] = (long long)hc[i];
+}
+
+// ---------------------------------------------------------------------------
+// Data movement and initial states.
+// ---------------------------------------------------------------------------
+
+void Runtime::upload(int lev, int comp, const double *host) {
+    if (lev == -1) define_shadow();
+    LevelState &ls = level(lev);
+    if (comp < 0 || comp >= ncomp_) throw Error(1, "component out of range");
+    const int w = ls.cur;
+    for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
+        const Box3 &bx = ls.L.boxes[b];
+        const BoxDesc &bd = ls.L.hbox[b];
+        const long long nb = bx.vol();
+        const double *src = host + ls.L.vprefix[b];
+        // Rows are contiguous in x; copy row by row through a staging buffer.
+        std::vector<char> hi((size_t)nb * esize()), lo;
+        if (prec_ == Prec::F32C) lo.resize((size_t)nb * 4);
+        for (long long i = 0; i < nb; ++i) {
+            if (prec_ == Prec::F64) {
+                reinterpret_cast<double *>(hi.data())[i] = src[i];
+            } else {
+                const float h = (float)src[i];
+                reinterpret_cast<float *>(hi.data())[i] = h;
+                reinterpret_cast<float *>(lo.data())[i] = (float)(src[i] - (double)h);
+            }
+        }
+        const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
+        const size_t es = esize();
+        // Rows of consecutive planes are not equally spaced (ghost rows in
+        // between), so copy plane by plane.
+        for (int z = 0; z < nz; ++z) {
+            char *base = static_cast<char *>(ls.U[w].get()) +
+                         ((size_t)comp * ls.L.cstride + bd.base +
+                          (size_t)z * bd.kstride) * es;
+            SHM_HIP(hipMemcpy2D(base, (size_t)bd.jstride * es,
+                                hi.data() + (size_t)z * nx * ny * es, (size_t)nx * es,
+                                (size_t)nx * es, ny, hipMemcpyHostToDevice));
+            if (prec_ == Prec::F32C) {
+                char *bl = static_cast<char *>(ls.Ulo[w].get()) +
+                           ((size_t)comp * ls.L.cstride + bd.base +
+                            (size_t)z * bd.kstride) * 4;
+                SHM_HIP(hipMemcpy2D(bl, (size_t)bd.jstride * 4,
+                                    lo.data() + (size_t)z * nx * ny * 4,
+                                    (size_t)nx * 4, (size_t)nx * 4, ny,
+                                    hipMemcpyHostToDevice));
+            }
+        }
+    }
+    ls.ghosts_ok[w] = false;
+}
+
+void Runtime::download(int lev, int comp, double *host, bool old_state) {
+    LevelState &ls = level(lev);
+    if (comp < 0 || comp >= ncomp_) throw Error(1, "component out of range");
+    sync();
+    const int w = old_state ? (ls.cur ^ 1) : ls.cur;
+    const size_t es = esize();
+    for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
+        const Box3 &bx = ls.L.boxes[b];
+        const BoxDesc &bd = ls.L.hbox[b];
+        const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
+        std::vector<char> hi((size_t)bx.vol() * es), lo;
+        if (prec_ == Prec::F32C) lo.resize((size_t)bx.vol() * 4);
+        for (int z = 0; z < nz; ++z) {
+            const char *base = static_cast<const char *>(ls.U[w].get()) +
+                               ((size_t)comp * ls.L.cstride + bd.base +
+                                (size_t)z * bd.kstride) * es;
+            SHM_HIP(hipMemcpy2D(hi.data() + (size_t)z * nx * ny * es,
+                                (size_t)nx * es, base, (size_t)bd.jstride * es,
+                                (size_t)nx * es, ny, hipMemcpyDeviceToHost));
+            if (prec_ == Prec::F32C) {
+                const char *bl = static_cast<const char *>(ls.Ulo[w].get()) +
+                                 ((size_t)comp * ls.L.cstride + bd.base +
+                                  (size_t)z * bd.kstride) * 4;
+                SHM_HIP(hipMemcpy2D(lo.data() + (size_t)z * nx * ny * 4,
+                                    (size_t)nx * 4, bl, (size_t)bd.jstride * 4,
+                                    (size_t)nx * 4, ny, hipMemcpyDeviceToHost));
+            }
+        }
+        double *dst = host + ls.L.vprefix[b];
+        for (long long i = 0; i < bx.vol(); ++i) {
+            if (prec_ == Prec::F64)
+                dst[i] = reinterpret_cast<const double *>(hi.data())[i];
+            else
+                dst[i] = (double)reinterpret_cast<const float *>(hi.data())[i] +
+                         (double)reinterpret_cast<const float *>(lo.data())[i];
+        }
+    }
+}
+
+void Runtime::download_rhs(int lev, int comp, double *host) {
+    LevelState &ls = level(lev);
+    if (!ls.RHS) throw Error(3, "no rhs has been computed on this level");
+    sync();
+    const size_t es = esize();
+    for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
+        const Box3 &bx = ls.L.boxes[b];
+        const BoxDesc &bd = ls.L.hbox[b];
+        const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
+        std::vector<char> buf((size_t)bx.vol() * es);
+        for (int z = 0; z < nz; ++z) {
+            const char *base = static_cast<const char *>(ls.RHS.get()) +
+                               ((size_t)comp * ls.L.cstride + bd.base +
+                                (size_t)z * bd.kstride) * es;
+            SHM_HIP(hipMemcpy2D(buf.data() + (size_t)z * nx * ny * es,
+                                (size_t)nx * es, base, (size_t)bd.jstride * es,
+                                (size_t)nx * es, ny, hipMemcpyDeviceToHost));
+        }
+        double *dst = host + ls.L.vprefix[b];
+        for (long long i = 0; i < bx.vol(); ++i)
+            dst[i] = prec_ == Prec::F64
+                         ? reinterpret_cast<const double *>(buf.data())[i]
+                         : (double)reinterpret_cast<const float *>(buf.data())[i];
+    }
+}
+
+void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
+                         int np) {
+    LevelState &ls = level(lev);
+    InitArgs a{};
+    a.arr = ls.U[ls.cur].get();
+    a.arrlo = prec_ == Prec::F32C ? ls.Ulo[ls.cur].get() : nullptr;
+    a.cstride = ls.L.cstride;
+    a.boxes = ls.L.dbox.as<BoxDesc>();
+    a.vprefix = ls.L.dvprefix.as<long long>();
+    a.nbox = (int)ls.L.boxes.size();
+    a.total = ls.L.nvalid();
+    a.N = ls.L.N;
+    a.kind = kind;
+    a.seed = seed;
+    a.nfields = kt_.nfields;
+    a.ncomp = ncomp_;
+    for (int i = 0; i < np && i < 8; ++i) a.p[i] = p[i];
+    if (ncomp_ > 32) throw Error(4, "initial-state generator supports <= 32 components");
+    const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
+    if (prec_ == Prec::F64)
+        hipLaunchKernelGGL(init_kernel<Prec::F64>, dim3(grid), dim3(256), 0,
+                           stream_, a);
+    else
+        hipLaunchKernelGGL(init_kernel<Prec::F32C>, dim3(grid), dim3(256), 0,
+                           stream_, a);
+    SHM_HIP(hipGetLastError());
+    ls.ghosts_ok[ls.cur] = false;
+}
+
+void Runtime::sync() { SHM_HIP(hipStreamSynchronize(stream_)); }
+
+// ---------------------------------------------------------------------------
+// Autotuner: time the first stage of the configured integrator (it writes
+// only stage scratch) for each tile variant and a few z-chunk lengths.
+// ---------------------------------------------------------------------------
+
+void Runtime::autotune(int lev) {
+    LevelState &ls = level(lev);
+    const int old = ls.cur;
+    fill_patch(ls, old);
+    const double dt = sim_->dt[lev], dx = sim_->dx[lev];
+    std::vector<double> blk(kPrmBlock);
+    fill_params(blk.data(), ls.t, dt, dx, lev);
+    const double *prm = push_params(prm_host_, prm_ev_, prm_slot_,
+                                    prm_dev_.as<double>(), stream_, blk);
+    int pol = integrator_ == 10 ? kPolSsprk
+                                : (integrator_ == 11 ? kPolLeapfrog : kPolRkn);
+    StageArgs a{};
+    a.prm = prm;
+    if (pol == kPolSsprk) {
+        a.x = ls.U[old].get();
+        a.flags = PolSsprk::kWriteK | PolSsprk::kXisU0;
+        a.kout = ls.K.get();
+        a.y = ls.S1.get();
+        a.c[1] = dt;
+    } else if (pol == kPolRkn) {
+        if (nodes_.size() < 2) return;
+        a.u0 = ls.U[old].get();
+        a.fout = ls.F[0].get();
+    } else {
+        return;  // leapfrog kick 1 writes u_new fields; keep the default
+    }
+    int maxn = 1;
+    for (const Box3 &b : ls.L.boxes) maxn = std::max(maxn, b.n(2));
+    std::vector<int> zs = {0};
+    for (int z = 16; z <= maxn; z *= 2) zs.push_back(z);
+    const int saved_v = variant, saved_tz = tz;
+    const bool saved_t = timing_on;
+    timing_on = false;
+    double best = 1e30;
+    int bv = 1, bz = 0;
+    for (int v = 0; v < kNVar; ++v) {
+        const StageEntry &e = kt_.stage[pol][(int)prec_][R_][gw_ ? 1 : 0][ko_ ? 1 : 0][v];
+        if (!e.fn) continue;
+        for (int z : zs) {
+            variant = v;
+            tz = z;
+            for (int w = 0; w < 2; ++w) {
+                StageArgs c = a;
+                launch_stage(pol, ls, c, lev);
+            }
+            SHM_HIP(hipEventRecord(ev0_, stream_));
+            const int reps = 5;
+            for (int r = 0; r < reps; ++r) {
+                StageArgs c = a;
+                launch_stage(pol, ls, c, lev);
+            }
+            SHM_HIP(hipEventRecord(ev1_, stream_));
+            SHM_HIP(hipEventSynchronize(ev1_));
+            float ms = 0;
+            SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            if (ms < best) {
+                best = ms;
+                bv = v;
+                bz = z;
+            }
+        }
+    }
+    variant = saved_v;
+    tz = saved_tz;
+    timing_on = saved_t;
+    tuned_variant_[lev + 1] = bv;
+    tuned_tz_[lev + 1] = bz;
+}
+
+// ---------------------------------------------------------------------------
+// Gravitational-wave spectrum (GravitationalWaves::ComputeSpectrum).
+// ---------------------------------------------------------------------------
+
+static double index_to_k(int a, int N, int type) {
+    const int nt = a >= N / 2 ? a - N : a;
+    const double th = 2. * M_PI * nt / N;
+    switch (type) {
+        case 1:
+            return std::sin(th);
+        case 2:
+            return (8. * std::sin(th) - std::sin(2. * th)) / 6.;
+        case 3:
+            return (45. * std::sin(th) - 9. * std::sin(2. * th) +
+                    std::sin(3. * th)) / 30.;
+    }
+    return 0.;
+}
+
+void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
+                          std::vector<double> &bins,
+                          std::vector<double> &kvals) {
+    if (!gw_) throw Error(3, "gravitational waves are not enabled");
+    if (proj < 1 || proj > 3) throw Error(1, "projection_type must be 1, 2 or 3");
+    if (zp < 1 || !sledgehamr::utils::IsPowerOfTwo(zp))
+        throw Error(1, "zero padding must be a power of 2");
+    LevelState &l0 = level(0);
+    const int N = sim_->coarse_level_grid_size * zp;
+    const int NH = N / 2 + 1;
+    const int NS = kt_.nscalars;
+    const int comps[6] = {NS + 6, NS + 9, NS + 10, NS + 7, NS + 11, NS + 8};
+
+    std::vector<double> itk(N);
+    for (int a = 0; a < N; ++a) itk[a] = index_to_k(a, N, proj);
+    std::vector<int> sq2bin;
+    kvals.clear();
+    if (mode == 0) {
+        std::set<long long> vals;
+        std::vector<long long> k2(N);
+        for (int a = 0; a < N; ++a) {
+            const long long f = a >= N / 2 ? a - N : a;
+            k2[a] = f * f;
+        }
+        std::set<long long> u(k2.begin(), k2.end());
+        std::vector<long long> uv(u.begin(), u.end());
+        for (long long x : uv)
+            for (long long y : uv)
+                for (long long z : uv) vals.insert(x + y + z);
+        const long long maxsq = 3LL * (N / 2) * (N / 2);
+        sq2bin.assign(maxsq + 1, -1);
+        int i = 0;
+        for (long long v : vals) {
+            if (v <= maxsq) sq2bin[v] = i;
+            kvals.push_back((double)v);
+            ++i;
+        }
+        nbins = (int)vals.size();
+    } else if (mode == 1) {
+        nbins = 0;
+        for (int k = 0; k <= std::sqrt(3.) / 2. * (double)N + .5; ++k) {
+            kvals.push_back((double)k * k);
+            ++nbins;
+        }
+    } else {
+        if (nbins <= 0) throw Error(1, "log binning needs nbins > 0");
+        const double lk = std::log(std::sqrt(3.) * N / 2.);
+        for (int i = 0; i < nbins; ++i) kvals.push_back(std::exp(lk * i / nbins));
+    }
+
+    DevMem dense, out[6], ditk, dsq, dbins;
+    dense.alloc(sizeof(double) * (size_t)N * N * N);
+    hipfftHandle plan;
+    if (hipfftPlan3d(&plan, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS)
+        throw Error(2, "hipfftPlan3d failed");
+    hipfftSetStream(plan, stream_);
+    for (int c = 0; c < 6; ++c) {
+        out[c].alloc(sizeof(double2) * (size_t)NH * N * N);
+        SHM_HIP(hipMemsetAsync(dense.get(), 0, dense.bytes(), stream_));
+        for (size_t b = 0; b < l0.L.boxes.size(); ++b) {
+            PackArgs p{};
+            p.arr = l0.U[l0.cur].get();
+            p.arrlo = prec_ == Prec::F32C ? l0.Ulo[l0.cur].get() : nullptr;
+            p.cstride = l0.L.cstride;
+            p.box = l0.L.hbox[b];
+            p.comp = comps[c];
+            p.out = dense.as<double>();
+            p.N = N;
+            const long long nb = l0.L.boxes[b].vol();
+            const int grid = (int)std::min<long long>((nb + 255) / 256, 8LL * ncu_);
+            if (prec_ == Prec::F64)
+                hipLaunchKernelGGL(pack_kernel<Prec::F64>, dim3(grid), dim3(256),
+                                   0, stream_, p);
+            else
+                hipLaunchKernelGGL(pack_kernel<Prec::F32C>, dim3(grid), dim3(256),
+                                   0, stream_, p);
+            SHM_HIP(hipGetLastError());
+        }
+        if (hipfftExecD2Z(plan, dense.as<hipfftDoubleReal>(),
+                          out[c].as<hipfftDoubleComplex>()) != HIPFFT_SUCCESS) {
+            hipfftDestroy(plan);
+            throw Error(2, "hipfftExecD2Z failed");
+        }
+    }
+    hipfftDestroy(plan);
+    upload_vec(ditk, itk);
+    if (mode == 0) upload_vec(dsq, sq2bin);
+    dbins.alloc(sizeof(double) * std::max(1, nbins));
+    BinArgs ba{};
+    for (int c = 0; c < 6; ++c) ba.u[c] = out[c].as<double2>();
+    ba.N = N;
+    ba.index_to_k = ditk.as<double>();
+    ba.mode = mode;
+    ba.sq_to_bin = mode == 0 ? dsq.as<int>() : nullptr;
+    ba.nbins = nbins;
+    ba.log_kmax = std::log(std::sqrt(3.) * N / 2.);
+    ba.bins = dbins.as<double>();
+    const long long total = (long long)NH * N * N;
+    const int grid = (int)std::min<long long>((total + 255) / 256, 4LL * ncu_);
+    hipLaunchKernelGGL(gw_bin_kernel<2048>, dim3(grid), dim3(256), 0, stream_,
+                       ba);
+    SHM_HIP(hipGetLastError());
+    bins.assign(nbins, 0.);
+    SHM_HIP(hipMemcpyAsync(bins.data(), dbins.get(), sizeof(double) * nbins,
+                           hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+    const double n6 = std::pow((double)N, 6);
+    for (double &v : bins) v /= n6;
+}
+
+// ---------------------------------------------------------------------------
+// Time stepper (TimeStepper::Advance, time_stepper.cpp:90-311) for a static
+// box layout: the schedule decides when truncation errors are computed; the
+// regrid itself is replaced by tagging (ErrorEst) on the current layout.
+// ---------------------------------------------------------------------------
+
+void Runtime::step(int ncoarse) {
+    for (int n = 0; n < ncoarse; ++n) scheduled_step(0);
+}
+
+void Runtime::scheduled_step(int lev) {
+    sledgehamr::Sledgehamr &S = *sim_;
+    LevelState &ls = level(lev);
+    const double time = ls.t;
+    const int istep = ls.istep;
+    if (S.shadow_hierarchy && lev < S.finest_level) {
+        auto find = [&](double t) {
+            for (size_t i = 0; i < schedule_.size(); ++i)
+                if (schedule_[i].second == t) return (int)i;
+            return -1;
+        };
+        const double tnext = time + S.dt[lev];
+        int id = find(tnext);
+        bool already = id >= 0 && schedule_[id].first == lev;
+        bool ok = !already && ls.t_old != time && lev < S.max_level &&
+                  !(istep % 2 == 0 && lev > 0);
+        const double opp = lev > 0 ? time + 3. * S.dt[lev] : time + 2. * S.dt[lev];
+        if (ok && !S.DoCreateLevelIf(lev + 1, opp)) ok = false;
+        if (ok && opp <= last_regrid_[lev] + regrid_dt_[lev]) ok = false;
+        if (ok && lev == 0 && ls.t_old == -DBL_MAX) ok = false;
+        if (ok) {
+            if (id < 0)
+                schedule_.emplace_back(lev, tnext);
+            else
+                schedule_[id].first = std::min(schedule_[id].first, lev);
+            if (lev == 0) create_shadow_level();
+        }
+    }
+    if (lev == 0 && !shadow_.defined()) S.BeforeTimestep(ls.t);
+    advance(lev);
+    if (lev < S.finest_level) {
+        scheduled_step(lev + 1);
+        scheduled_step(lev + 1);
+    }
+    synchronize_levels(lev);
+    // DoRegridIfScheduled: tag on the static layout.
+    for (size_t i = 0; i < schedule_.size(); ++i) {
+        if (schedule_[i].second == ls.t && schedule_[i].first == lev) {
+            for (int l = lev; l < S.finest_level; ++l) last_regrid_[l] = ls.t;
+            schedule_.erase(schedule_.begin() + i);
+            break;
+        }
+    }
+    if (lev == 0)
+        for (int l = 1; l <= S.finest_level; ++l)
+            if (has_level(l)) levels_[l].t = levels_[0].t;
+}
+
+void Runtime::synchronize_levels(int lev) {
+    sledgehamr::Sledgehamr &S = *sim_;
+    LevelState &ls = level(lev);
+    bool need_te = false;
+    for (auto &s : schedule_)
+        if (s.second == ls.t && lev >= s.first) need_te = true;
+    if (lev < S.finest_level && has_level(lev + 1)) {
+        if (!(S.shadow_hierarchy && need_te)) {
+            LevelState &fine = level(lev + 1);
+            // amrex::average_down onto the coarse boxes covering each fine box.
+            LevelState mirror;
+            mirror.L.build(coarsen(fine.L.boxes), 0, S.dimN[lev], true);
+            mirror.ncomp = ncomp_;
+            mirror.U[0].alloc((size_t)mirror.L.cstride * ncomp_ * esize());
+            if (prec_ == Prec::F32C)
+                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * 4);
+            avg_down(fine, fine.cur, mirror, 0);
+            long long tot = 0;
+            std::vector<HaloSeg> plan = copy_plan(ls.L, mirror.L, &tot);
+            DevMem dplan;
+            upload_vec(dplan, plan);
+            if (tot > 0) {
+                CopyArgs c{};
+                c.dst = ls.U[ls.cur].get();
+                c.src = mirror.U[0].get();
+                c.dcstride = ls.L.cstride;
+                c.scstride = mirror.L.cstride;
+                c.dboxes = ls.L.dbox.as<BoxDesc>();
+                c.sboxes = mirror.L.dbox.as<BoxDesc>();
+                c.segs = dplan.as<HaloSeg>();
+                c.nseg = (int)plan.size();
+                c.total = tot;
+                c.ncomp = ncomp_;
+                const int grid = (int)std::min<long long>((tot + 255) / 256, 8LL * ncu_);
+                if (prec_ == Prec::F64) {
+                    hipLaunchKernelGGL(copy_kernel<double>, dim3(grid), dim3(256),
+                                       0, stream_, c);
+                } else {
+                    hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256),
+                                       0, stream_, c);
+                    c.dst = ls.Ulo[ls.cur].get();
+                    c.src = mirror.Ulo[0].get();
+                    hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256),
+                                       0, stream_, c);
+                }
+                SHM_HIP(hipGetLastError());
+            }
+            sync();
+            ls.ghosts_ok[ls.cur] = false;
+        }
+    }
+    if (lev >= 1 - (S.shadow_hierarchy ? 1 : 0) && need_te) {
+        te_tag(lev, kTeCompute | kTeAvgDown | kTeTags, nullptr, nullptr, nullptr);
+        if (lev == 0) shadow_ = LevelState();
+    }
+}
+
+}  // namespace shm

@@ -1,0 +1,205 @@
+// Host runtime: owns device memory, box/tile tables, copy plans and the
+// stage sequences of the integrators for one project instance.
+#pragma once
+
+#include <sledgehamr.h>
+
+#include <cfloat>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "shm_layout.hpp"
+#include "shm_stage.hpp"
+#include "shm_table.hpp"
+
+namespace shm {
+
+struct Box3 {
+    int lo[3], hi[3];  // inclusive, global index space of the level
+    int n(int d) const { return hi[d] - lo[d] + 1; }
+    long long vol() const { return (long long)n(0) * n(1) * n(2); }
+};
+
+/** Owning device allocation. */
+class DevMem {
+  public:
+    DevMem() = default;
+    ~DevMem() { reset(); }
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    DevMem(DevMem &&o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+        o.p_ = nullptr;
+        o.bytes_ = 0;
+    }
+    DevMem &operator=(DevMem &&o) noexcept {
+        if (this != &o) {
+            reset();
+            p_ = o.p_;
+            bytes_ = o.bytes_;
+            o.p_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
+    void alloc(size_t bytes);  // zero-filled
+    void reset();
+    void *get() const { return p_; }
+    template <class T>
+    T *as() const { return static_cast<T *>(p_); }
+    size_t bytes() const { return bytes_; }
+    explicit operator bool() const { return p_ != nullptr; }
+
+  private:
+    void *p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+/** Box layout of one level (or of a coarsened copy of one). */
+struct Layout {
+    int N = 0;          // domain cells per axis at this level
+    int g = 0;          // ghost width
+    bool periodic = true;
+    std::vector<Box3> boxes;
+    std::vector<BoxDesc> hbox;
+    DevMem dbox;
+    long long cstride = 0;  // elements per component plane
+    std::vector<long long> vprefix;  // valid cells per box (prefix, nbox+1)
+    DevMem dvprefix;
+    std::vector<long long> cprefix;  // coarse (half) cells per box (prefix)
+    DevMem dcprefix;
+    std::vector<HaloSeg> halo;
+    DevMem dhalo;
+    long long halo_total = 0;
+    std::map<std::pair<int, int>, std::pair<std::shared_ptr<DevMem>, int>> tiles;
+
+    void build(const std::vector<Box3> &b, int ghost, int domain_n,
+               bool is_periodic);
+    const TileDesc *get_tiles(int ty, int tz, int *ntiles);
+    long long nvalid() const { return vprefix.empty() ? 0 : vprefix.back(); }
+};
+
+/** Device state of one level: the old/new state pair and stage scratch. */
+struct LevelState {
+    Layout L;
+    int ncomp = 0;
+    DevMem U[2], Ulo[2];
+    int cur = 0;               // index of grid_new in U
+    bool ghosts_ok[2] = {false, false};
+    DevMem S1, S2, K;          // SSPRK3 stage states and k
+    DevMem F[5];               // RKN forces
+    DevMem VH;                 // leapfrog half-step momenta (FP64)
+    DevMem RHS;                // FillRhs target
+    double t = -DBL_MAX;       // time of grid_new
+    double t_old = -DBL_MAX;   // time of grid_old
+    int istep = 0;
+    bool defined() const { return !L.boxes.empty(); }
+};
+
+struct Timing {
+    double stage_ms = 0, halo_ms = 0;
+    long long stage_launches = 0, halo_launches = 0;
+    long long stage_cells = 0;  // valid cells updated by stage launches
+};
+
+class Runtime {
+  public:
+    Runtime(sledgehamr::Sledgehamr *sim, const KernelTable &kt);
+    ~Runtime();
+
+    // Setup.
+    void init_from_inputs(const std::string &inputs, int precision_override);
+    void set_level(int lev, const std::vector<Box3> &boxes);
+
+    // Data movement (valid cells, box-major, x fastest).
+    void upload(int lev, int comp, const double *host);
+    void download(int lev, int comp, double *host, bool old_state = false);
+    void download_rhs(int lev, int comp, double *host);
+    void init_state(int lev, int kind, uint64_t seed, const double *p, int np);
+
+    // Reference operations.
+    void fill_rhs(int lev, double time, double weight, bool add);
+    void advance(int lev);               // Integrator::Advance(lev)
+    void create_shadow_level();          // Sledgehamr::CreateShadowLevel
+    void te_tag(int lev, int flags, uint8_t *mask, long long *counts,
+                double *te_host);
+    void gw_spectrum(int mode, int nbins, int zero_padding, int proj_type,
+                     std::vector<double> &bins, std::vector<double> &kvals);
+    void step(int ncoarse);              // TimeStepper::Advance(0)
+
+    void sync();
+    void autotune(int lev);
+
+    // State queries.
+    int ncomp() const { return ncomp_; }
+    int nfields() const { return kt_.nfields; }
+    int radius() const { return R_; }
+    int precision() const { return (int)prec_; }
+    LevelState &level(int lev);
+    bool has_level(int lev) const {
+        return lev >= 0 && lev < (int)levels_.size() && levels_[lev].defined();
+    }
+    int nlevels() const { return (int)levels_.size(); }
+    sledgehamr::Sledgehamr *sim() { return sim_; }
+
+    Timing timing;
+    bool timing_on = false;
+    int variant = -1;   // tile variant, -1 = autotune/default
+    int tz = 0;         // z-chunk length, 0 = automatic
+    bool xcd_split = true;
+
+  private:
+    void define_shadow();
+    void fill_params(double *blk, double time, double dt, double dx, int lev);
+    void ensure_arrays(LevelState &ls, int integrator);
+    size_t esize() const { return prec_ == Prec::F64 ? 8 : 4; }
+    void launch_stage(int pol, LevelState &ls, StageArgs &a, int lev);
+    void halo(LevelState &ls, void *arr, const std::vector<int> &comps);
+    void fill_patch(LevelState &ls, int which);
+    void integrate(LevelState &ls, int lev, double dt, double dx,
+                   double t_old);
+    void probe();
+    std::vector<int> stencil_comps() const;
+    std::vector<int> all_comps() const;
+    bool te_pass_fused(int lev) const;
+    void avg_down(LevelState &fine, int fine_which, LevelState &crse,
+                  int crse_which);
+    void scheduled_step(int lev);
+    void synchronize_levels(int lev);
+
+    sledgehamr::Sledgehamr *sim_;
+    const KernelTable &kt_;
+    Prec prec_ = Prec::F32C;
+    int ncomp_ = 0;
+    int R_ = 1;
+    bool gw_ = false, ko_ = false;
+    int integrator_ = 10;
+    std::vector<double> nodes_, bbar_, b_;
+    std::vector<std::vector<double>> tableau_;
+    std::vector<LevelState> levels_;
+    LevelState shadow_;
+    hipStream_t stream_ = nullptr;
+    int ncu_ = 256;
+    DevMem prm_dev_;
+    double *prm_host_ = nullptr;
+    std::vector<hipEvent_t> prm_ev_;
+    int prm_slot_ = 0;
+    DevMem ko_dev_;
+    std::vector<int> tuned_variant_;
+    std::vector<int> tuned_tz_;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    // regrid/TE schedule of the time stepper
+    std::vector<std::pair<int, double>> schedule_;
+    std::vector<double> regrid_dt_, last_regrid_;
+    double te_lowest_time_ = -DBL_MAX;
+};
+
+/** Last error text for the C ABI (per context). */
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+}  // namespace shm

@@ -1,0 +1,689 @@
+// Fused stage kernels for gfx950.
+//
+// One kernel template evaluates a project's Rhs() and applies an integrator
+// stage update in registers, so each stored array is read once and written at
+// most once per stage.  Work is a persistent loop over a device tile table
+// that spans every box of a level: a tile is 64 cells in x (one wavefront per
+// row, fully coalesced) by TY = NW*RY rows in y, marched along z.
+//
+// The z-march keeps, per thread and stencil component, a register queue of
+// the 2R+1 values of its column; the centre plane with its x/y ghost ring is
+// staged in LDS (double buffered, one barrier per plane).  The project's
+// Rhs() reads the tile through amrex::Array4<const double> in tile mode and
+// writes into a register sink; point-wise components (momenta) are read at
+// the centre only.  Plane z+R+1, its halo ring and the point-wise values of
+// plane z+1 are prefetched one plane ahead.
+//
+// Reference behaviour reproduced: the Rhs cell loop with the optional GW
+// source/backreaction and Kreiss-Oliger term (source/macros.h:301-485), and
+// the stage combinations of IntegratorLsssprk3 (lsssprk3.cpp:14-31),
+// IntegratorRkn (rkn.cpp:25-186) and IntegratorLeapfrog (leapfrog.cpp:13-57).
+#pragma once
+
+#include <sledgehamr.h>
+
+#include <type_traits>
+
+#include "shm_layout.hpp"
+
+namespace shm {
+
+enum class Prec : int { F64 = 0, F32C = 1 };
+
+template <Prec P>
+struct PrecTraits;
+template <>
+struct PrecTraits<Prec::F64> {
+    using S = double;  // stored state
+};
+template <>
+struct PrecTraits<Prec::F32C> {
+    using S = float;  // stored state; a second float array holds the remainder
+};
+
+/** Per-stage parameter record in device memory (written by the host before
+ *  the stage runs, so a captured graph replays with fresh values). */
+constexpr int kPrmTime = 0, kPrmDt = 1, kPrmDx = 2;
+constexpr int kPrmUser = 8;    // SetParamsRhs() values
+constexpr int kMaxParams = 64;
+constexpr int kPrmGw = kPrmUser + kMaxParams;  // SetParamsGravitationalWaveRhs()
+constexpr int kPrmBlock = kPrmGw + kMaxParams;
+
+/** Kernel argument block shared by all stage policies. */
+struct StageArgs {
+    const BoxDesc *boxes;
+    const TileDesc *tiles;
+    int ntiles;
+    int lev;
+    long long cstride;
+    const double *prm;          // stage record: time, dt, dx, params, gw params
+    const double *ko_strength;
+    int ko_order;
+    int flags;
+    int nprev;
+    int xcd_split;
+    const void *x;      // stage-state source
+    const void *u0;     // old state (hi part for FP32C)
+    const void *u0lo;   // old state remainder (FP32C)
+    const void *kin;    // SSPRK3 k in / FillAddRhs previous rhs
+    void *kout;         // SSPRK3 k out / FillRhs out
+    void *y;            // SSPRK3 stage state out
+    void *unew;         // new state (hi)
+    void *unewlo;       // new state remainder
+    const void *f[5];   // RKN forces of earlier stages
+    void *fout;         // RKN force of this stage
+    void *vh;           // leapfrog half-step momenta (FP64)
+    double c[20];       // stage coefficients
+};
+
+/** Which components need neighbour access (stencil) in the fused kernel.
+ *  Default: the project's fields plus the six u_ij when GW are on; with
+ *  Kreiss-Oliger dissipation every component is differenced. */
+template <class Ops, bool GW, bool KO>
+struct CompSet {
+    static constexpr int NC = GW ? Ops::kNGw : Ops::kNScalars;
+    static constexpr int NF = Ops::kNFields;
+    static constexpr bool stencil(int n) {
+        return KO || n < NF ||
+               (GW && n >= Ops::kNScalars && n < Ops::kNScalars + 6);
+    }
+    static constexpr int count() {
+        int c = 0;
+        for (int n = 0; n < NC; ++n) c += stencil(n) ? 1 : 0;
+        return c;
+    }
+    static constexpr int NS = count();
+    static constexpr int NP = NC - NS;
+    static constexpr int slot(int n) {
+        if (!stencil(n)) return -1;
+        int c = 0;
+        for (int m = 0; m < n; ++m) c += stencil(m) ? 1 : 0;
+        return c;
+    }
+    static constexpr int comp_of_slot(int s) {
+        int c = 0;
+        for (int n = 0; n < NC; ++n)
+            if (stencil(n)) {
+                if (c == s) return n;
+                ++c;
+            }
+        return -1;
+    }
+    static constexpr int comp_of_point(int p) {
+        int c = 0;
+        for (int n = 0; n < NC; ++n)
+            if (!stencil(n)) {
+                if (c == p) return n;
+                ++c;
+            }
+        return -1;
+    }
+    // (field, momentum) pairs as the RKN/leapfrog integrators pair them
+    // (rkn.cpp:32-44): first and second halves of the scalars, then of the
+    // twelve GW components.
+    static constexpr int NPAIR = NF + (GW ? 6 : 0);
+    static constexpr int pair_field(int p) {
+        return p < NF ? p : Ops::kNScalars + (p - NF);
+    }
+    static constexpr int pair_mom(int p) {
+        return p < NF ? NF + p : Ops::kNScalars + 6 + (p - NF);
+    }
+    static constexpr int pair_of_field(int n) {
+        return n < NF ? n
+                      : ((GW && n >= Ops::kNScalars && n < Ops::kNScalars + 6)
+                             ? NF + (n - Ops::kNScalars)
+                             : -1);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Stage policies: how the stage state is formed and what a cell writes.
+// ---------------------------------------------------------------------------
+
+template <class T>
+__device__ AMREX_FORCE_INLINE T ld(const void *p, long long e) {
+    return static_cast<const T *>(p)[e];
+}
+template <class T>
+__device__ AMREX_FORCE_INLINE void st(void *p, long long e, T v) {
+    static_cast<T *>(p)[e] = v;
+}
+
+/** Write a full-precision value as FP64, or as an FP32 pair hi + lo with
+ *  hi = fl32(u) and lo = fl32(u - hi) (compensated storage). */
+template <class S>
+__device__ AMREX_FORCE_INLINE void store_state(void *hi, void *lo, long long e,
+                                               double u) {
+    if constexpr (std::is_same_v<S, float>) {
+        const float h = (float)u;
+        st<float>(hi, e, h);
+        st<float>(lo, e, (float)(u - (double)h));
+    } else {
+        st<double>(hi, e, u);
+    }
+}
+template <class S>
+__device__ AMREX_FORCE_INLINE double load_state(const void *hi, const void *lo,
+                                                long long e) {
+    if constexpr (std::is_same_v<S, float>)
+        return (double)ld<float>(hi, e) + (double)ld<float>(lo, e);
+    else
+        return ld<double>(hi, e);
+}
+
+/** Low-storage SSPRK3 stage / FillRhs / FillAddRhs.
+ *  k = f(X) + c0 * k_in;  then either y = u0 + c1 * k, or at the final stage
+ *  u_new = u0 + c1 * k with compensated storage, or (fill mode) rhs_out = k. */
+struct PolSsprk {
+    enum : int { kHasKin = 1, kWriteK = 2, kFinal = 4, kXisU0 = 8, kFillOnly = 16 };
+
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, int n,
+                                                   long long e) {
+        return ld<S>(a.x, n * a.cstride + e);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
+                                                        int n, long long e) {
+        return (double)ld<S>(a.x, n * a.cstride + e);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
+                                                    long long e0,
+                                                    const double *rhs,
+                                                    const double *ctr) {
+#pragma unroll
+        for (int n = 0; n < CS::NC; ++n) {
+            const long long e = n * a.cstride + e0;
+            double k = rhs[n];
+            if (a.flags & kHasKin) k += a.c[0] * (double)ld<S>(a.kin, e);
+            if (a.flags & kFillOnly) {
+                st<S>(a.kout, e, (S)k);
+                continue;
+            }
+            if (a.flags & kWriteK) st<S>(a.kout, e, (S)k);
+            const double u0 =
+                (a.flags & kXisU0) ? ctr[n] : (double)ld<S>(a.u0, e);
+            if (!(a.flags & kFinal)) {
+                st<S>(a.y, e, (S)(u0 + a.c[1] * k));
+            } else {
+                double u = u0;
+                if constexpr (std::is_same_v<S, float>)
+                    u += (double)ld<float>(a.u0lo, e);
+                store_state<S>(a.unew, a.unewlo, e, u + a.c[1] * k);
+            }
+        }
+    }
+};
+
+/** Runge-Kutta-Nystroem stage i (rkn.cpp:52-119).  Stage state of a field:
+ *  y = phi0 + c0*pi0 + sum_j c[1+j] F_j; momenta stay at pi0.  Non-final
+ *  stages write F_i (the momentum right-hand sides); the final stage writes
+ *  phi = phi0 + c6*pi0 + sum_j c[7+j] F_j and pi = pi0 + sum_j c[12+j] F_j,
+ *  with F of this stage taken from registers. */
+struct PolRkn {
+    enum : int { kFinal = 4 };
+
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static double state(const StageArgs &a, int n,
+                                                      long long e) {
+        const int p = CS::pair_of_field(n);
+        if (p < 0) return (double)ld<S>(a.u0, n * a.cstride + e);
+        double y = (double)ld<S>(a.u0, n * a.cstride + e);
+        y += a.c[0] * (double)ld<S>(a.u0, CS::pair_mom(p) * a.cstride + e);
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+            if (j < a.nprev) y += a.c[1 + j] * (double)ld<S>(a.f[j], p * a.cstride + e);
+        return y;
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, int n,
+                                                   long long e) {
+        return (S)state<CS, S>(a, n, e);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
+                                                        int n, long long e) {
+        return (double)(S)state<CS, S>(a, n, e);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
+                                                    long long e0,
+                                                    const double *rhs,
+                                                    const double *ctr) {
+#pragma unroll
+        for (int p = 0; p < CS::NPAIR; ++p) {
+            const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
+            const double fi = rhs[nm];
+            if (!(a.flags & kFinal)) {
+                st<S>(a.fout, p * a.cstride + e0, (S)fi);
+                continue;
+            }
+            const long long ef = nf * a.cstride + e0, em = nm * a.cstride + e0;
+            const double pi0 = load_state<S>(a.u0, a.u0lo, em);
+            double phi = load_state<S>(a.u0, a.u0lo, ef);
+            phi += a.c[6] * pi0;
+            double pi = pi0;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                if (j < a.nprev) {
+                    const double fj = (double)ld<S>(a.f[j], p * a.cstride + e0);
+                    phi += a.c[7 + j] * fj;
+                    pi += a.c[12 + j] * fj;
+                }
+            }
+            // c[7+nprev], c[12+nprev] weight this stage's force; the host
+            // places them at index 11 and 16 so the unrolled loop stays fixed.
+            phi += a.c[11] * fi;
+            pi += a.c[16] * fi;
+            store_state<S>(a.unew, a.unewlo, ef, phi);
+            store_state<S>(a.unew, a.unewlo, em, pi);
+        }
+    }
+};
+
+/** Leapfrog kick-drift-kick (leapfrog.cpp:41-56).
+ *  Kick 1 (X = u0): vh = pi0 + c0*a(pi) (stored FP64), phi1 = phi0 + c1*vh.
+ *  Kick 2 (X = u_new with phi1 and the momenta u_new held before the step,
+ *  exactly as the reference's second Rhs sees them): pi1 = vh + c0*a(pi). */
+struct PolLeapfrog {
+    enum : int { kKick2 = 32 };
+
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, int n,
+                                                   long long e) {
+        return ld<S>(a.x, n * a.cstride + e);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
+                                                        int n, long long e) {
+        return (double)ld<S>(a.x, n * a.cstride + e);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
+                                                    long long e0,
+                                                    const double *rhs,
+                                                    const double *ctr) {
+#pragma unroll
+        for (int p = 0; p < CS::NPAIR; ++p) {
+            const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
+            const long long ev = p * a.cstride + e0;
+            if (!(a.flags & kKick2)) {
+                const double pi0 = load_state<S>(a.u0, a.u0lo, nm * a.cstride + e0);
+                const double vh = pi0 + a.c[0] * rhs[nm];
+                st<double>(a.vh, ev, vh);
+                const long long ef = nf * a.cstride + e0;
+                store_state<S>(a.unew, a.unewlo, ef,
+                               load_state<S>(a.u0, a.u0lo, ef) + a.c[1] * vh);
+            } else {
+                const double vh = ld<double>(a.vh, ev);
+                store_state<S>(a.unew, a.unewlo, nm * a.cstride + e0,
+                               vh + a.c[0] * rhs[nm]);
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// The fused stage kernel.
+// ---------------------------------------------------------------------------
+
+/** Halo-ring cell h of a (64 + 2R) x (TY + 2R) plane: rows above, rows
+ *  below, then the R columns left and right of each interior row.  Returns
+ *  tile-local (x, y), both >= -R. */
+template <int R, int TY>
+__device__ AMREX_FORCE_INLINE void ring_xy(int h, int &x, int &y) {
+    constexpr int PX = 64 + 2 * R;
+    if (h < R * PX) {
+        y = -R + h / PX;
+        x = -R + h % PX;
+    } else if (h < 2 * R * PX) {
+        h -= R * PX;
+        y = TY + h / PX;
+        x = -R + h % PX;
+    } else {
+        h -= 2 * R * PX;
+        y = h / (2 * R);
+        const int c = h % (2 * R);
+        x = c < R ? -R + c : 64 + (c - R);
+    }
+}
+
+template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int NW, int RY>
+__global__ void __launch_bounds__(64 * NW)
+    stage_kernel(const StageArgs a) {
+    using S = typename PrecTraits<P>::S;
+    using CS = CompSet<Ops, GW, KO>;
+    constexpr int NC = CS::NC;
+    constexpr int NS = CS::NS;
+    constexpr int NP = CS::NP;
+    constexpr int TX = 64;
+    constexpr int TY = NW * RY;
+    constexpr int PX = TX + 2 * R, PY = TY + 2 * R;
+    constexpr int Q = 2 * R + 1;
+    constexpr int NT = TX * NW;
+    constexpr int H = PX * PY - TX * TY;
+    constexpr int HPT = (H + NT - 1) / NT;
+    static_assert(R >= 1 && R <= 3, "stencil radius must be 1..3");
+
+    __shared__ S lds[2][NS][PY][PX];
+
+    signed char slot[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) slot[n] = (signed char)CS::slot(n);
+
+    const int tx = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+
+    // Persistent walk over the tile table.  With xcd_split the tiles are cut
+    // into 8 contiguous ranges, one per XCD (workgroups are dealt to XCDs
+    // round-robin), so tiles sharing halos run behind the same L2.
+    int t, tstep, tend;
+    if (a.xcd_split) {
+        const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+        const int per = (a.ntiles + 7) >> 3;
+        t = xcd * per + l;
+        tend = min(a.ntiles, (xcd + 1) * per);
+        tstep = gridDim.x >> 3;
+    } else {
+        t = blockIdx.x;
+        tend = a.ntiles;
+        tstep = gridDim.x;
+    }
+
+    const double time = a.prm[kPrmTime], dt = a.prm[kPrmDt], dx = a.prm[kPrmDx];
+    const double *params = a.prm + kPrmUser;
+    const double *gw_params = a.prm + kPrmGw;
+
+    unsigned step = 0;
+    for (; t < tend; t += tstep) {
+        const TileDesc td = a.tiles[t];
+        const BoxDesc bd = a.boxes[td.box];
+        const int nx = bd.n[0], ny = bd.n[1];
+        const int js = bd.jstride, ks = bd.kstride;
+        const long long base = bd.base;
+        const int lx = td.x0 + tx;
+        const bool ldx = lx < nx + R;
+        const bool cpx = lx < nx;
+
+        int ly[RY];
+        bool ldy[RY], cpy[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            ly[r] = td.y0 + w + r * NW;
+            ldy[r] = ly[r] < ny + R;
+            cpy[r] = ly[r] < ny;
+        }
+
+        auto cell = [&](int x, int y, int z) -> long long {
+            return base + (long long)(x + y * js) + (long long)z * ks;
+        };
+
+        S q[NS][RY][Q];
+        S nq[NS][RY];
+        S hv[NS][HPT];
+        double pv[NP > 0 ? NP : 1][RY];
+
+        // Prologue: planes z0-R .. z0+R-1 into q[..][1..2R], plane z0+R into nq.
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const bool ok = ldx && ldy[r];
+#pragma unroll
+                for (int m = 1; m < Q; ++m)
+                    q[s][r][m] = ok ? Pol::template load_st<CS, S>(
+                                          a, CS::comp_of_slot(s),
+                                          cell(lx, ly[r], td.z0 - R + m - 1))
+                                    : (S)0;
+                nq[s][r] = ok ? Pol::template load_st<CS, S>(
+                                    a, CS::comp_of_slot(s),
+                                    cell(lx, ly[r], td.z0 + R))
+                              : (S)0;
+            }
+        auto load_halo = [&](int z) {
+#pragma unroll
+            for (int m = 0; m < HPT; ++m) {
+                const int h = threadIdx.x + m * NT;
+                int hx = 0, hy = 0;
+                if (h < H) ring_xy<R, TY>(h, hx, hy);
+                const int gx = td.x0 + hx, gy = td.y0 + hy;
+                const bool ok = h < H && gx < nx + R && gy < ny + R;
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    hv[s][m] = ok ? Pol::template load_st<CS, S>(
+                                        a, CS::comp_of_slot(s), cell(gx, gy, z))
+                                  : (S)0;
+            }
+        };
+        auto load_point = [&](int z) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int r = 0; r < RY; ++r)
+                    pv[p][r] = (cpx && cpy[r])
+                                   ? Pol::template load_pt<CS, S>(
+                                         a, CS::comp_of_point(p),
+                                         cell(lx, ly[r], z))
+                                   : 0.;
+        };
+        load_halo(td.z0);
+        load_point(td.z0);
+
+        for (int z = td.z0; z < td.z1; ++z, ++step) {
+            const int buf = step & 1;
+            // Advance the register queues by one plane.
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int r = 0; r < RY; ++r) {
+#pragma unroll
+                    for (int m = 0; m < Q - 1; ++m) q[s][r][m] = q[s][r][m + 1];
+                    q[s][r][Q - 1] = nq[s][r];
+                }
+            // Stage plane z: own centres and the halo ring.
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+#pragma unroll
+                for (int r = 0; r < RY; ++r)
+                    lds[buf][s][w + r * NW + R][tx + R] = q[s][r][R];
+#pragma unroll
+                for (int m = 0; m < HPT; ++m) {
+                    const int h = threadIdx.x + m * NT;
+                    if (h < H) {
+                        int hx, hy;
+                        ring_xy<R, TY>(h, hx, hy);
+                        lds[buf][s][hy + R][hx + R] = hv[s][m];
+                    }
+                }
+            }
+            double pc[NP > 0 ? NP : 1][RY];
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int r = 0; r < RY; ++r) pc[p][r] = pv[p][r];
+
+            // Prefetch plane z+1 (queue head z+1+R, halo ring, point values).
+            if (z + 1 < td.z1) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int r = 0; r < RY; ++r)
+                        nq[s][r] = (ldx && ldy[r])
+                                       ? Pol::template load_st<CS, S>(
+                                             a, CS::comp_of_slot(s),
+                                             cell(lx, ly[r], z + 1 + R))
+                                       : (S)0;
+                load_halo(z + 1);
+                load_point(z + 1);
+            }
+            __syncthreads();
+
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                if (!(cpx && cpy[r])) continue;
+                const int gi = bd.lo[0] + lx, gj = bd.lo[1] + ly[r],
+                          gk = bd.lo[2] + z;
+                double ctr[NC];
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    ctr[CS::comp_of_slot(s)] = (double)q[s][r][R];
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    ctr[CS::comp_of_point(p)] = pc[p][r];
+
+                TileCtx tc;
+                tc.ci = gi;
+                tc.cj = gj;
+                tc.ck = gk;
+                tc.R = R;
+                tc.f32 = std::is_same_v<S, float> ? 1 : 0;
+                tc.lds_pitch = PX;
+                tc.lds_cstride = PX * PY;
+                tc.zq_cstride = RY * Q;
+                tc.lds = &lds[buf][0][w + r * NW + R][tx + R];
+                tc.zq = &q[0][r][0];
+                tc.ctr = ctr;
+                tc.slot = slot;
+                amrex::Array4<const double> sv;
+                sv.mode = amrex::Array4<const double>::kTile;
+                sv.ext = &tc;
+                sv.ncomp = NC;
+
+                double rhs[NC];
+#pragma unroll
+                for (int n = 0; n < NC; ++n) rhs[n] = 0.;
+                amrex::Array4<double> rv;
+                rv.mode = amrex::Array4<double>::kSink;
+                rv.sink = rhs;
+                rv.ncomp = NC;
+
+                Ops::rhs(rv, sv, gi, gj, gk, a.lev, time, dt, dx, params);
+                if constexpr (GW) {
+                    Ops::gw_rhs(rv, sv, gi, gj, gk, a.lev, time, dt, dx,
+                                gw_params);
+                    Ops::gw_backreaction(rv, sv, gi, gj, gk, a.lev, time, dt,
+                                         dx, params, gw_params);
+                }
+                if constexpr (KO) {
+                    if (a.ko_order == 2) {
+#pragma unroll
+                        for (int n = 0; n < NC; ++n)
+                            rhs[n] += sledgehamr::kernels::KreissOligerDissipation<2>(
+                                sv, gi, gj, gk, n, dx, a.ko_strength[n]);
+                    } else if (a.ko_order == 3) {
+#pragma unroll
+                        for (int n = 0; n < NC; ++n)
+                            rhs[n] += sledgehamr::kernels::KreissOligerDissipation<3>(
+                                sv, gi, gj, gk, n, dx, a.ko_strength[n]);
+                    }
+                }
+                Pol::template store<CS, S>(a, cell(lx, ly[r], z), rhs, ctr);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost-cell fill from a copy plan (periodic wrap and box-to-box).
+// ---------------------------------------------------------------------------
+
+struct HaloArgs {
+    void *arr;
+    long long cstride;
+    const BoxDesc *boxes;
+    const HaloSeg *segs;
+    int nseg;
+    long long total;
+    int ncomp;
+    int comps[32];
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) halo_kernel(const HaloArgs a) {
+    T *arr = static_cast<T *>(a.arr);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < a.total; t += stride) {
+        int lo = 0, hi = a.nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.segs[mid].start <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const HaloSeg &sg = a.segs[lo];
+        long long r = t - sg.start;
+        const int x = (int)(r % sg.n[0]);
+        r /= sg.n[0];
+        const int y = (int)(r % sg.n[1]);
+        const int z = (int)(r / sg.n[1]);
+        const int gi = sg.lo[0] + x, gj = sg.lo[1] + y, gk = sg.lo[2] + z;
+        const BoxDesc &db = a.boxes[sg.dst_box];
+        const BoxDesc &sb = a.boxes[sg.src_box];
+        const long long de = db.base + (gi - db.lo[0]) +
+                             (long long)(gj - db.lo[1]) * db.jstride +
+                             (long long)(gk - db.lo[2]) * db.kstride;
+        const long long se = sb.base + (gi - sg.shift[0] - sb.lo[0]) +
+                             (long long)(gj - sg.shift[1] - sb.lo[1]) * sb.jstride +
+                             (long long)(gk - sg.shift[2] - sb.lo[2]) * sb.kstride;
+        for (int c = 0; c < a.ncomp; ++c) {
+            const long long cb = (long long)a.comps[c] * a.cstride;
+            arr[cb + de] = arr[cb + se];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Probe: run the project's cell functions once with a recording view.
+// ---------------------------------------------------------------------------
+
+struct ProbeOut {
+    int ncomp;
+    int reach[ProbeCtx::kMaxComp];
+    int zreach[ProbeCtx::kMaxComp];
+    int offaxis[ProbeCtx::kMaxComp];
+    int tag_reach[ProbeCtx::kMaxComp];
+};
+
+template <class Ops, bool GW>
+__global__ void probe_kernel(ProbeOut *out, const double *params,
+                             const double *gw_params, double time, double dt,
+                             double dx) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    constexpr int NC = GW ? Ops::kNGw : Ops::kNScalars;
+    ProbeCtx pc;
+    pc.ci = 7;
+    pc.cj = 9;
+    pc.ck = 11;
+    pc.ncomp = NC;
+    for (int n = 0; n < ProbeCtx::kMaxComp; ++n) {
+        pc.reach[n] = 0;
+        pc.zreach[n] = 0;
+        pc.offaxis[n] = 0;
+    }
+    amrex::Array4<const double> sv;
+    sv.mode = amrex::Array4<const double>::kProbe;
+    sv.ext = &pc;
+    sv.ncomp = NC;
+    double rhs[NC];
+    for (int n = 0; n < NC; ++n) rhs[n] = 0.;
+    amrex::Array4<double> rv;
+    rv.mode = amrex::Array4<double>::kSink;
+    rv.sink = rhs;
+    Ops::rhs(rv, sv, 7, 9, 11, 0, time, dt, dx, params);
+    if constexpr (GW) {
+        Ops::gw_rhs(rv, sv, 7, 9, 11, 0, time, dt, dx, gw_params);
+        Ops::gw_backreaction(rv, sv, 7, 9, 11, 0, time, dt, dx, params,
+                             gw_params);
+    }
+    out->ncomp = NC;
+    for (int n = 0; n < ProbeCtx::kMaxComp; ++n) {
+        out->reach[n] = pc.reach[n];
+        out->zreach[n] = pc.zreach[n];
+        out->offaxis[n] = pc.offaxis[n];
+    }
+}
+
+}  // namespace shm

@@ -1,0 +1,160 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/_build/liboracle_<P>.so).
+
+Test infrastructure only: tests use it as the checker; it is never the
+product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_LIBS: dict[str, C.CDLL] = {}
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "-j4"], check=True)
+
+
+def _load(project: str) -> C.CDLL:
+    if project in _LIBS:
+        return _LIBS[project]
+    path = os.path.join(ORACLE_DIR, "_build", f"liboracle_{project}.so")
+    if not os.path.exists(path):
+        build_oracle()
+    lib = C.CDLL(path, mode=C.RTLD_LOCAL)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    dp, ip, llp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)
+    sig = {
+        "orc_last_error": (C.c_char_p, []),
+        "orc_create": (vp, [C.c_char_p]),
+        "orc_destroy": (None, [vp]),
+        "orc_ncomp": (i, [vp]),
+        "orc_ncells": (C.c_longlong, [vp, i]),
+        "orc_set_boxes": (i, [vp, i, ip, i]),
+        "orc_set_time": (i, [vp, i, d, i]),
+        "orc_time": (d, [vp, i]),
+        "orc_index_to_k": (d, [i, i, i]),
+        "orc_upload": (i, [vp, i, i, dp]),
+        "orc_download": (i, [vp, i, i, dp, i]),
+        "orc_fill_rhs": (i, [vp, i, d, d, i]),
+        "orc_download_rhs": (i, [vp, i, dp]),
+        "orc_advance": (i, [vp, i]),
+        "orc_create_shadow": (i, [vp]),
+        "orc_shadow_time": (d, [vp]),
+        "orc_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
+        "orc_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
+        "orc_init_state": (i, [vp, i, i, C.c_ulonglong, dp, i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIBS[project] = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def inputs_text(inputs) -> str:
+    if isinstance(inputs, str):
+        return inputs
+    out = []
+    for k, v in inputs.items():
+        if isinstance(v, (list, tuple)):
+            v = " ".join(str(x) for x in v)
+        elif isinstance(v, bool):
+            v = "true" if v else "false"
+        out.append(f"{k} = {v}")
+    return "\n".join(out) + "\n"
+
+
+class Oracle:
+    def __init__(self, project: str, inputs):
+        self.lib = _load(project)
+        self.h = self.lib.orc_create(inputs_text(inputs).encode())
+        if not self.h:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        self.ncomp = self.lib.orc_ncomp(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+
+    def ncells(self, lev):
+        return self.lib.orc_ncells(self.h, lev)
+
+    def set_boxes(self, lev, boxes):
+        b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        self._ck(self.lib.orc_set_boxes(self.h, lev,
+                                        b.ctypes.data_as(C.POINTER(C.c_int)), len(b)))
+
+    def set_time(self, lev, t, istep=0):
+        self._ck(self.lib.orc_set_time(self.h, lev, t, istep))
+
+    def time(self, lev):
+        return self.lib.orc_time(self.h, lev)
+
+    def upload(self, lev, comp, data):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        assert a.size == self.ncells(lev)
+        self._ck(self.lib.orc_upload(self.h, lev, comp, _dp(a)))
+
+    def download(self, lev, comp, old=False):
+        a = np.empty(self.ncells(lev))
+        self._ck(self.lib.orc_download(self.h, lev, comp, _dp(a), int(old)))
+        return a
+
+    def fill_rhs(self, lev, time, weight=0.0, add=False):
+        self._ck(self.lib.orc_fill_rhs(self.h, lev, time, weight, int(add)))
+
+    def rhs(self, lev, comp):
+        a = np.empty(self.ncells(lev))
+        self._ck(self.lib.orc_download_rhs(self.h, comp, _dp(a)))
+        return a
+
+    def advance(self, lev=0, nsteps=1):
+        for _ in range(nsteps):
+            self._ck(self.lib.orc_advance(self.h, lev))
+
+    def create_shadow(self):
+        self._ck(self.lib.orc_create_shadow(self.h))
+
+    def te_tag(self, lev, flags=7, want_te=False):
+        n = self.ncells(lev)
+        mask = np.zeros(n, dtype=np.uint8)
+        counts = np.zeros(2 + self.ncomp, dtype=np.int64)
+        te = np.zeros((self.ncomp, n // 8)) if want_te else None
+        self._ck(self.lib.orc_te_tag(self.h, lev, flags,
+                                     mask.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                     counts.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                     _dp(te) if te is not None else None))
+        return mask, counts, te
+
+    def gw_spectrum(self, mode=0, nbins=256, zero_padding=1, projection_type=2):
+        n = C.c_int(0)
+        self._ck(self.lib.orc_gw_spectrum(self.h, mode, nbins, zero_padding,
+                                          projection_type, None, None, C.byref(n)))
+        b, k = np.zeros(n.value), np.zeros(n.value)
+        self._ck(self.lib.orc_gw_spectrum(self.h, mode, nbins, zero_padding,
+                                          projection_type, _dp(b), _dp(k), C.byref(n)))
+        return b, k
+
+    def init_state(self, lev, kind, seed, params=()):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        self._ck(self.lib.orc_init_state(self.h, lev, kind, seed,
+                                         _dp(p) if p.size else None, p.size))
+
+    def index_to_k(self, a, N, ptype):
+        return self.lib.orc_index_to_k(a, N, ptype)
