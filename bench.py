@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-level time step on one MI355X.
+
+Workload (default): complex scalar on a 512^3 periodic level, 4th-order
+Laplacian, V = (lambda/4)(|psi|^2-1)^2 with lambda = 1 and radiation-era
+damping 2/eta, low-storage SSPRK3 (integrator.type = 10), L = 128
+(dx = 0.25), cfl = 0.2 (dt = 0.05), eta starting at 0.1; initial phases
+seeded uniform in [0, 2 pi), moduli 1 + N(0, 0.05), momenta 0.
+
+A "step" is one SSPRK3 step of the level (3 RK stages).  `value` is
+cell-updates per second per RK stage of the FP32-compensated path
+(valid cells x stages / wall time), inputs resident in HBM.  The same
+seeded run on the FP64 path gives `fp64` and the field error.
+
+roofline: the fused stage kernel's algorithmic bytes per launch (each stored
+array read once and written at most once per stage: 74.67 B per cell-stage
+for FP32 + 32-bit remainder, 128 B for FP64) over its mean launch time,
+measured with HIP events on the library's stream inside the timed region.
+
+cpu_baseline: the FP64 oracle in the reference's unfused structure (Rhs pass,
+LinComb passes, ghost fill per stage; OpenMP), one step of the same 512^3
+workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = ("cell-updates/s per RK stage + % HBM roofline; FP32-comp vs FP64 "
+          "at ≤1e-6 rel err")
+# Algorithmic bytes per cell-stage of the fused low-storage SSPRK3 schedule
+# with C = 4 components: 12 words per component per step (3 stages), plus one
+# read and one write of the remainder array per component per step (FP32C).
+BYTES_PER_CELL_STAGE = {"fp32c": (12 * 4 * 4 + 2 * 4 * 4) / 3.0,
+                        "fp64": 12 * 4 * 8 / 3.0}
+
+
+def inputs(N, L, cfl, t0):
+    return {
+        "project.name": "ComplexScalar",
+        "project.lambda": 1.0,
+        "sim.t_start": t0,
+        "sim.t_end": 2.0,
+        "sim.L": L,
+        "sim.cfl": cfl,
+        "amr.coarse_level_grid_size": N,
+        "amr.nghost": 2,
+        "integrator.type": 10,
+    }
+
+
+def run_gpu(shm, inp, prec, steps, warmup, seed):
+    sim = shm.Simulation("ComplexScalar", inp, precision=prec)
+    sim.init_state(0, 1, seed, [0.05])
+    sim.autotune(0)
+    sim.advance(0, warmup)
+    sim.sync()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.advance(0, steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    tm = sim.timing()
+    sim.set_timing(False)
+    return sim, t1 - t0, tm
+
+
+def cpu_baseline(inp, seed):
+    from oracle_py import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc = Oracle("ComplexScalar", inp)
+    orc.init_state(0, 1, seed, [0.05])
+    n = orc.ncells(0)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < 5 and (steps == 0 or time.perf_counter() - t0 < 10.0):
+        orc.advance(0, 1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": n * 3 * steps / dt, "unit": "cell-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{steps} SSPRK3 step(s) (3 stages each) of the same "
+                      f"{round(n ** (1/3))}^3 workload, FP64 oracle in the "
+                      "reference's unfused structure, OpenMP "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=20240401)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs each level on a "
+              "single MI355X (multi-GPU sharding is design-only); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import sledgehamr_amd as shm
+
+    N = a.n
+    inp = inputs(N, L=N * 0.25, cfl=0.2, t0=0.1)
+    stages = 3
+    sim32, sec32, tm32 = run_gpu(shm, inp, shm.PREC_FP32C, a.steps, a.warmup, a.seed)
+    cells = sim32.ncells(0)
+    rate32 = cells * stages * a.steps / sec32
+
+    result = {
+        "metric": METRIC,
+        "value": rate32,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": sec32 / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f32 remainder storage (compensated); Rhs arithmetic f64",
+        "data": "synthetic (seeded random-phase complex field)",
+        "config": {"workload": f"complex scalar {N}^3 periodic, 4th-order "
+                               "Laplacian, SSPRK3, lambda=1, radiation damping",
+                   "N": N, "components": 4, "integrator": "SSPRK3 (type 10)",
+                   "L": N * 0.25, "cfl": 0.2, "seed": a.seed},
+    }
+
+    # Roofline of the dominant (fused stage) kernel.
+    launches = tm32["stage_launches"]
+    avg_ms = tm32["stage_ms"] / max(launches, 1)
+    bpl = BYTES_PER_CELL_STAGE["fp32c"] * tm32["stage_cells"] / max(launches, 1)
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_ssprk3_complex512.json")
+    if os.path.exists(tfile) and N == 512:
+        with open(tfile) as f:
+            traffic = json.load(f).get("fp32c_bytes_per_launch")
+    result["roofline"] = {"bound": "hbm", "achieved": achieved,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                          "kernel": "shm::stage_kernel<ComplexScalar, PolSsprk, F32C>",
+                          "avg_launch_ms": avg_ms,
+                          "algorithmic_bytes_per_launch": bpl}
+    result["halo_ms_per_step"] = tm32["halo_ms"] / a.steps
+
+    if not a.no_fp64:
+        sim64, sec64, tm64 = run_gpu(shm, inp, shm.PREC_FP64, a.steps, a.warmup,
+                                     a.seed)
+        rate64 = cells * stages * a.steps / sec64
+        l64 = tm64["stage_launches"]
+        avg64 = tm64["stage_ms"] / max(l64, 1)
+        b64 = BYTES_PER_CELL_STAGE["fp64"] * tm64["stage_cells"] / max(l64, 1)
+        err = 0.0
+        for c in range(4):
+            x32 = sim32.download(0, c)
+            x64 = sim64.download(0, c)
+            err = max(err, float(np.max(np.abs(x32 - x64)) /
+                                 max(np.max(np.abs(x64)), 1e-300)))
+            del x32, x64
+        result["fp64"] = {"value": rate64, "ms_per_step": sec64 / a.steps * 1e3,
+                          "roofline_frac": b64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        result["speedup_fp32c_vs_fp64"] = rate32 / rate64
+        result["fp32c_rel_err_vs_fp64"] = err
+        sim64.close()
+    sim32.close()
+
+    if not a.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(inp, a.seed)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -307,6 +307,7 @@ __attribute__((visibility("default"))) int shm_set_tile(shm_ctx *ctx,
 __attribute__((visibility("default"))) int shm_set_timing(shm_ctx *ctx, int on) {
     return guard(ctx, [&] {
         SHM_NEED_RT;
+        ctx->rt->collect_timing();
         ctx->rt->timing_on = on != 0;
         ctx->rt->timing = shm::Timing();
     });
@@ -317,6 +318,7 @@ __attribute__((visibility("default"))) int shm_get_timing(
     long long *stage_cells, double *halo_ms, long long *halo_launches) {
     return guard(ctx, [&] {
         SHM_NEED_RT;
+        ctx->rt->collect_timing();
         const shm::Timing &t = ctx->rt->timing;
         if (stage_ms) *stage_ms = t.stage_ms;
         if (stage_launches) *stage_launches = t.stage_launches;

@@ -290,6 +290,10 @@ Runtime::Runtime(sledgehamr::Sledgehamr *sim, const KernelTable &kt)
 
 Runtime::~Runtime() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (EventPair &p : ev_pool_) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
     for (hipEvent_t e : prm_ev_)
         if (e) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -300,7 +304,10 @@ Runtime::~Runtime() {
 }
 
 LevelState &Runtime::level(int lev) {
-    if (lev == -1) return shadow_;
+    if (lev == -1) {
+        define_shadow();
+        return shadow_;
+    }
     if (!has_level(lev))
         throw Error(1, "level " + std::to_string(lev) + " is not defined");
     return levels_[lev];
@@ -620,7 +627,11 @@ void Runtime::launch_stage(int pol, LevelState &ls, StageArgs &a, int lev) {
                                     tuned_variant_[lev + 1] >= 0
                                 ? tuned_variant_[lev + 1]
                                 : 1);
-    const StageEntry &e = kt_.stage[pol][pi][R_][gw_ ? 1 : 0][ko_ ? 1 : 0][v];
+    const auto &row = kt_.stage[pol][pi][R_][gw_ ? 1 : 0][ko_ ? 1 : 0];
+    // Variants over the LDS budget are not instantiated: fall back to the
+    // nearest smaller tile.
+    while (v > 0 && !row[v].fn) --v;
+    const StageEntry &e = row[v];
     if (!e.fn) throw Error(4, "stage kernel not built for this configuration");
     int z = tz > 0 ? tz
                    : (lev + 1 < (int)tuned_tz_.size() && lev >= -1 &&
@@ -651,15 +662,11 @@ void Runtime::launch_stage(int pol, LevelState &ls, StageArgs &a, int lev) {
     int grid = std::min<long long>(ntiles, (long long)occ * ncu_);
     a.xcd_split = xcd_split ? 1 : 0;
     if (a.xcd_split) grid = (int)round_up(grid, 8);
-    if (timing_on) SHM_HIP(hipEventRecord(ev0_, stream_));
+    EventPair *ep = timing_on ? timed_pair(0) : nullptr;
+    if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
     SHM_HIP(e.fn(a, grid, stream_));
-    if (timing_on) {
-        SHM_HIP(hipEventRecord(ev1_, stream_));
-        SHM_HIP(hipEventSynchronize(ev1_));
-        float ms = 0;
-        SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-        timing.stage_ms += ms;
-        timing.stage_launches += 1;
+    if (ep) {
+        SHM_HIP(hipEventRecord(ep->b, stream_));
         timing.stage_cells += ls.L.nvalid();
     }
 }
@@ -677,7 +684,8 @@ void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps) {
         h.ncomp = (int)std::min<size_t>(32, comps.size() - i);
         for (int c = 0; c < h.ncomp; ++c) h.comps[c] = comps[i + c];
         const int grid = (int)std::min<long long>((h.total + 255) / 256, 8LL * ncu_);
-        if (timing_on) SHM_HIP(hipEventRecord(ev0_, stream_));
+        EventPair *ep = timing_on ? timed_pair(1) : nullptr;
+        if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
         if (prec_ == Prec::F64)
             hipLaunchKernelGGL(halo_kernel<double>, dim3(grid), dim3(256), 0,
                                stream_, h);
@@ -685,14 +693,7 @@ void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps) {
             hipLaunchKernelGGL(halo_kernel<float>, dim3(grid), dim3(256), 0,
                                stream_, h);
         SHM_HIP(hipGetLastError());
-        if (timing_on) {
-            SHM_HIP(hipEventRecord(ev1_, stream_));
-            SHM_HIP(hipEventSynchronize(ev1_));
-            float ms = 0;
-            SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-            timing.halo_ms += ms;
-            timing.halo_launches += 1;
-        }
+        if (ep) SHM_HIP(hipEventRecord(ep->b, stream_));
     }
 }
 
@@ -899,7 +900,7 @@ void Runtime::avg_down(LevelState &fine, int fw, LevelState &crse, int cw) {
 }
 
 void Runtime::define_shadow() {
-    LevelState &l0 = level(0);
+    LevelState &l0 = levels_.at(0);
     if (!shadow_.defined() || shadow_.L.boxes.size() != l0.L.boxes.size()) {
         shadow_ = LevelState();
         shadow_.L.build(coarsen(l0.L.boxes), sim_->nghost,
@@ -1222,6 +1223,37 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
 }
 
 void Runtime::sync() { SHM_HIP(hipStreamSynchronize(stream_)); }
+
+/** Event pair for one timed launch, from a pool that grows on demand. */
+Runtime::EventPair *Runtime::timed_pair(int kind) {
+    if (ev_used_ == ev_pool_.size()) {
+        EventPair p{};
+        SHM_HIP(hipEventCreate(&p.a));
+        SHM_HIP(hipEventCreate(&p.b));
+        ev_pool_.push_back(p);
+    }
+    EventPair *p = &ev_pool_[ev_used_++];
+    p->kind = kind;
+    return p;
+}
+
+/** Resolve recorded launch timings into `timing` (after a synchronisation). */
+void Runtime::collect_timing() {
+    if (ev_used_ == 0) return;
+    sync();
+    for (size_t i = 0; i < ev_used_; ++i) {
+        float ms = 0;
+        SHM_HIP(hipEventElapsedTime(&ms, ev_pool_[i].a, ev_pool_[i].b));
+        if (ev_pool_[i].kind == 0) {
+            timing.stage_ms += ms;
+            timing.stage_launches += 1;
+        } else {
+            timing.halo_ms += ms;
+            timing.halo_launches += 1;
+        }
+    }
+    ev_used_ = 0;
+}
 
 // ---------------------------------------------------------------------------
 // Autotuner: time the first stage of the configured integrator (it writes

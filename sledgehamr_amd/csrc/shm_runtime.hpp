@@ -131,6 +131,7 @@ class Runtime {
 
     void sync();
     void autotune(int lev);
+    void collect_timing();
 
     // State queries.
     int ncomp() const { return ncomp_; }
@@ -151,6 +152,13 @@ class Runtime {
     bool xcd_split = true;
 
   private:
+    struct EventPair {
+        hipEvent_t a, b;
+        int kind;  // 0 stage kernel, 1 halo kernel
+    };
+    EventPair *timed_pair(int kind);
+    std::vector<EventPair> ev_pool_;
+    size_t ev_used_ = 0;
     void define_shadow();
     void fill_params(double *blk, double time, double dt, double dx, int lev);
     void ensure_arrays(LevelState &ls, int integrator);
