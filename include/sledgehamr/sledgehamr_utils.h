@@ -51,6 +51,32 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE constexpr GradWeights grad_weights(int 
 
 }  // namespace detail
 
+/** FP32 ring sums on an FP32 tile (fused stage kernels): x/y neighbours
+ *  from the staged LDS plane, z neighbours from the register queue.  Returns
+ *  false when the request is not the tile's own cell of a stencil slot, in
+ *  which case the caller takes the generic path. */
+template <int P>
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool tile_lap_f32(
+    amrex::Array4<amrex::Real const> const &state, int i, int j, int k, int c,
+    float &sum) {
+    if (state.mode != amrex::Array4<const double>::kTile) return false;
+    const shm::TileCtx &tc = *static_cast<const shm::TileCtx *>(state.ext);
+    if (!tc.f32 || i != tc.ci || j != tc.cj || k != tc.ck) return false;
+    const int s = tc.slot[c];
+    if (s < 0) return false;
+    constexpr detail::LapWeights W = detail::lap_weights(P);
+    const float *L = static_cast<const float *>(tc.lds) + s * tc.lds_cstride;
+    const float *Z = static_cast<const float *>(tc.zq) + s * tc.zq_cstride + tc.R;
+    float acc = (float)W.w[0] * Z[0];
+    constexpr_for<1, P + 1, 1>([&](auto m) {
+        const float ring = L[m] + L[-m] + L[m * tc.lds_pitch] +
+                           L[-m * tc.lds_pitch] + Z[m] + Z[-m];
+        acc += (float)W.w[m] * ring;
+    });
+    sum = acc;
+    return true;
+}
+
 /** Laplacian of component c at cell (i,j,k) with squared spacing dx2. */
 template <int P>
 AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double
@@ -61,6 +87,9 @@ Laplacian(amrex::Array4<amrex::Real const> const &state, const int i,
         return 0.;
     } else {
         constexpr detail::LapWeights W = detail::lap_weights(P);
+        const double scale = 1. / (W.den * dx2);  // loop invariant
+        float fs;
+        if (tile_lap_f32<P>(state, i, j, k, c, fs)) return (double)fs * scale;
         double acc = W.w[0] * state(i, j, k, c);
         constexpr_for<1, P + 1, 1>([&](auto m) {
             const double ring = state(i + m, j, k, c) + state(i - m, j, k, c) +
@@ -68,7 +97,7 @@ Laplacian(amrex::Array4<amrex::Real const> const &state, const int i,
                                 state(i, j, k + m, c) + state(i, j, k - m, c);
             acc += W.w[m] * ring;
         });
-        return acc / (W.den * dx2);
+        return acc * scale;
     }
 }
 
@@ -86,12 +115,34 @@ Gradient(amrex::Array4<amrex::Real const> const &state, const int i,
         const int ax = axis == 'x' ? 1 : 0, ay = axis == 'y' ? 1 : 0,
                   az = axis == 'z' ? 1 : 0;
         if (ax + ay + az == 0) return 0.;
+        const double scale = 1. / (W.den * dx);  // loop invariant
+        if (state.mode == amrex::Array4<const double>::kTile) {
+            const shm::TileCtx &tc = *static_cast<const shm::TileCtx *>(state.ext);
+            const int s = tc.slot[c];
+            if (tc.f32 && s >= 0 && i == tc.ci && j == tc.cj && k == tc.ck) {
+                float acc = 0.f;
+                if (az) {
+                    const float *Z = static_cast<const float *>(tc.zq) +
+                                     s * tc.zq_cstride + tc.R;
+                    constexpr_for<1, P + 1, 1>(
+                        [&](auto m) { acc += (float)W.g[m] * (Z[m] - Z[-m]); });
+                } else {
+                    const float *L =
+                        static_cast<const float *>(tc.lds) + s * tc.lds_cstride;
+                    const int st = ax ? 1 : tc.lds_pitch;
+                    constexpr_for<1, P + 1, 1>([&](auto m) {
+                        acc += (float)W.g[m] * (L[m * st] - L[-m * st]);
+                    });
+                }
+                return (double)acc * scale;
+            }
+        }
         double acc = 0.;
         constexpr_for<1, P + 1, 1>([&](auto m) {
             acc += W.g[m] * (state(i + m * ax, j + m * ay, k + m * az, c) -
                              state(i - m * ax, j - m * ay, k - m * az, c));
         });
-        return acc / (W.den * dx);
+        return acc * scale;
     }
 }
 

@@ -33,7 +33,7 @@ class Project:
     header: str        # header path relative to the repository root
     cls: str           # fully qualified class
     r_mask: int        # bit r: stencil radius r instantiated
-    gw: bool = False   # GW (12 tensor components) variants
+    gw_modes: int = 1  # bit 0: without the GW tensor, bit 1: with it
     ko: bool = False   # Kreiss-Oliger variants
 
 
@@ -41,10 +41,10 @@ PROJECTS = [
     Project("RealScalar", "sledgehamr_amd/projects/RealScalar/RealScalar.h",
             "RealScalar::RealScalar", 0b1110, ko=True),
     Project("ComplexScalar", "sledgehamr_amd/projects/ComplexScalar/ComplexScalar.h",
-            "ComplexScalar::ComplexScalar", 0b1110),
+            "ComplexScalar::ComplexScalar", 0b0100),
     Project("PhaseTransitionGW",
             "sledgehamr_amd/projects/PhaseTransitionGW/PhaseTransitionGW.h",
-            "PhaseTransitionGW::PhaseTransitionGW", 0b1000, gw=True),
+            "PhaseTransitionGW::PhaseTransitionGW", 0b1000, gw_modes=2),
 ]
 
 COMMON_FLAGS = [
@@ -73,34 +73,45 @@ def _compile_runtime(src: str) -> str:
     return obj
 
 
-def _project_tu(p: Project) -> str:
+def _project_units(p: Project) -> list[tuple[str, list[str]]]:
+    """(source, extra flags) of the project's translation units: one per
+    (precision, stencil radius) holding the fused stage kernels, and a main
+    unit with the tagging/probe kernels and the project registration."""
     os.makedirs(BUILD, exist_ok=True)
-    tu = os.path.join(BUILD, f"proj_{p.name}.hip")
-    text = (
-        f'#include "{os.path.join(ROOT, p.header)}"\n'
-        '#include "shm_project_impl.hpp"\n'
-        f"SHM_REGISTER_PROJECT({p.cls})\n"
-    )
-    if not os.path.exists(tu) or open(tu).read() != text:
-        with open(tu, "w") as f:
-            f.write(text)
-    return tu
+    units = [("main", [])]
+    for prec in (0, 1):
+        for r in (1, 2, 3):
+            if (p.r_mask >> r) & 1:
+                units.append((f"p{prec}_r{r}",
+                              [f"-DSHM_TU_PREC={prec}", f"-DSHM_TU_R={r}"]))
+    out = []
+    for tag, flags in units:
+        tu = os.path.join(BUILD, f"proj_{p.name}_{tag}.hip")
+        text = (
+            f'#include "{os.path.join(ROOT, p.header)}"\n'
+            f"#define SHM_PROJECT_CLASS {p.cls}\n"
+            '#include "shm_project_impl.hpp"\n'
+        )
+        if not os.path.exists(tu) or open(tu).read() != text:
+            with open(tu, "w") as f:
+                f.write(text)
+        out.append((tu, flags))
+    return out
 
 
-def _compile_project(p: Project) -> str:
-    tu = _project_tu(p)
-    obj = os.path.join(BUILD, f"proj_{p.name}.o")
+def _compile_unit(p: Project, tu: str, flags: list[str]) -> str:
+    obj = os.path.splitext(tu)[0] + ".o"
     _run([HIPCC, *COMMON_FLAGS, f"-DSHM_R_MASK={p.r_mask}",
-          f"-DSHM_WITH_GW={int(p.gw)}", f"-DSHM_WITH_KO={int(p.ko)}",
+          f"-DSHM_GW_MODES={p.gw_modes}", f"-DSHM_WITH_KO={int(p.ko)}", *flags,
           "-c", tu, "-o", obj])
     return obj
 
 
-def _link(p: Project, pobj: str, robjs: list[str]) -> str:
+def _link(p: Project, pobjs: list[str], robjs: list[str]) -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, f"libshm_{p.name}.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, pobj, *robjs,
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *pobjs, *robjs,
           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, out)
     return out
@@ -113,9 +124,10 @@ def build(projects: list[str] | None = None, jobs: int = 8) -> list[str]:
     sel = [p for p in PROJECTS if projects is None or p.name in projects]
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         rfut = [ex.submit(_compile_runtime, s) for s in RUNTIME_SOURCES]
-        pfut = {p.name: ex.submit(_compile_project, p) for p in sel}
+        pfut = {p.name: [ex.submit(_compile_unit, p, tu, fl)
+                         for tu, fl in _project_units(p)] for p in sel}
         robjs = [f.result() for f in rfut]
-        pobjs = {k: f.result() for k, f in pfut.items()}
+        pobjs = {k: [f.result() for f in v] for k, v in pfut.items()}
     return [_link(p, pobjs[p.name], robjs) for p in sel]
 
 

@@ -17,6 +17,8 @@ struct BoxDesc {
     int lo[3];        // global index of the first valid cell
     int n[3];         // valid extents
     long long base;   // element offset of cell lo inside a component plane
+    long long org;    // element offset of cell lo - (g, g, g): every cell of
+                      // the ghosted brick is org + an unsigned 32-bit offset
     int jstride;      // elements per row
     int kstride;      // elements per xy-plane
 };

@@ -1,10 +1,15 @@
-// Instantiation of the kernel table for one project.  Included (after the
-// project header) by the generated per-project translation unit.
+// Instantiation of the kernel table for one project.
 //
+// The generated translation units of a project include the project header,
+// define SHM_PROJECT_CLASS and include this file:
+//   * one TU per (precision, stencil radius) with SHM_TU_PREC / SHM_TU_R set
+//     instantiates the fused stage kernels of that pair;
+//   * the main TU (neither macro set) instantiates the tagging and probe
+//     kernels and registers the project class.
 // Build-time selection (defaults in brackets):
-//   SHM_R_MASK  bit r set -> stencil radius r instantiated      [0xE: 1,2,3]
-//   SHM_WITH_GW instantiate the 12-component GW variants        [0]
-//   SHM_WITH_KO instantiate Kreiss-Oliger variants              [0]
+//   SHM_R_MASK    bit r set -> stencil radius r instantiated    [0xE: 1,2,3]
+//   SHM_GW_MODES  bit 0: without GW tensor, bit 1: with GW      [1]
+//   SHM_WITH_KO   Kreiss-Oliger variants (default tile only)    [0]
 #pragma once
 
 #include "shm_aux.hpp"
@@ -14,8 +19,8 @@
 #ifndef SHM_R_MASK
 #define SHM_R_MASK 0xE
 #endif
-#ifndef SHM_WITH_GW
-#define SHM_WITH_GW 0
+#ifndef SHM_GW_MODES
+#define SHM_GW_MODES 1
 #endif
 #ifndef SHM_WITH_KO
 #define SHM_WITH_KO 0
@@ -23,38 +28,140 @@
 
 namespace shm {
 
-template <class Pol>
-struct PolOf;
-template <>
-struct PolOf<PolSsprk> {
-    static constexpr int id = kPolSsprk;
-};
-template <>
-struct PolOf<PolRkn> {
-    static constexpr int id = kPolRkn;
-};
-template <>
-struct PolOf<PolLeapfrog> {
-    static constexpr int id = kPolLeapfrog;
-};
+void fill_table_aux(KernelTable &kt);
+void fill_table_p0_r1(KernelTable &kt);
+void fill_table_p0_r2(KernelTable &kt);
+void fill_table_p0_r3(KernelTable &kt);
+void fill_table_p1_r1(KernelTable &kt);
+void fill_table_p1_r2(KernelTable &kt);
+void fill_table_p1_r3(KernelTable &kt);
 
-template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int V>
+template <class Ops>
+const KernelTable &project_kernel_table() {
+    static const KernelTable table = [] {
+        KernelTable kt;
+        kt.nscalars = Ops::kNScalars;
+        kt.nfields = Ops::kNFields;
+        kt.ngw = Ops::kNGw;
+        kt.built_gw = (SHM_GW_MODES & 2) != 0;
+        kt.built_ko = SHM_WITH_KO != 0;
+        kt.r_mask = SHM_R_MASK;
+#if SHM_R_MASK & 2
+        fill_table_p0_r1(kt);
+        fill_table_p1_r1(kt);
+#endif
+#if SHM_R_MASK & 4
+        fill_table_p0_r2(kt);
+        fill_table_p1_r2(kt);
+#endif
+#if SHM_R_MASK & 8
+        fill_table_p0_r3(kt);
+        fill_table_p1_r3(kt);
+#endif
+        fill_table_aux(kt);
+        return kt;
+    }();
+    return table;
+}
+
+template <int K>
+struct KindPol;
+template <> struct KindPol<kSsprk1> { using T = PolSsprk<kWriteK | kXisU0>; };
+template <> struct KindPol<kSsprk2> { using T = PolSsprk<kHasKin | kWriteK>; };
+template <> struct KindPol<kSsprk3> { using T = PolSsprk<kHasKin | kFinal>; };
+template <> struct KindPol<kFillRhs> { using T = PolSsprk<kFillOnly>; };
+template <> struct KindPol<kFillAddRhs> { using T = PolSsprk<kFillOnly | kHasKin>; };
+template <> struct KindPol<kRkn0> { using T = PolRkn<0, false>; };
+template <> struct KindPol<kRkn1> { using T = PolRkn<1, false>; };
+template <> struct KindPol<kRkn2> { using T = PolRkn<2, false>; };
+template <> struct KindPol<kRkn3> { using T = PolRkn<3, false>; };
+template <> struct KindPol<kRknF0> { using T = PolRkn<0, true>; };
+template <> struct KindPol<kRknF1> { using T = PolRkn<1, true>; };
+template <> struct KindPol<kRknF2> { using T = PolRkn<2, true>; };
+template <> struct KindPol<kRknF3> { using T = PolRkn<3, true>; };
+template <> struct KindPol<kRknF4> { using T = PolRkn<4, true>; };
+template <> struct KindPol<kLeap1> { using T = PolLeapfrog<false>; };
+template <> struct KindPol<kLeap2> { using T = PolLeapfrog<true>; };
+
+template <class Ops, int K, Prec P, int R, bool GW, bool KO, int V>
 hipError_t launch_stage(const StageArgs &a, int grid, hipStream_t s) {
     constexpr TileVariant v = kVariants[V];
-    hipLaunchKernelGGL((stage_kernel<Ops, Pol, P, R, GW, KO, v.nw, v.ry>),
+    hipLaunchKernelGGL((stage_kernel<Ops, typename KindPol<K>::T, P, R, GW, KO,
+                                     v.nw, v.ry>),
                        dim3(grid), dim3(64 * v.nw), 0, s, a);
     return hipGetLastError();
 }
 
-template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int V>
+template <class Ops, int K, Prec P, int R, bool GW, bool KO, int V>
 int occupancy_stage() {
     constexpr TileVariant v = kVariants[V];
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, stage_kernel<Ops, Pol, P, R, GW, KO, v.nw, v.ry>, 64 * v.nw,
-            0) != hipSuccess)
+            &nb,
+            stage_kernel<Ops, typename KindPol<K>::T, P, R, GW, KO, v.nw, v.ry>,
+            64 * v.nw, 0) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
+}
+
+template <class Ops, int K, Prec P, int R, bool GW, bool KO>
+void fill_variants(KernelTable &kt) {
+    auto put = [&](auto vc) {
+        constexpr int V = decltype(vc)::value;
+        using S = typename PrecTraits<P>::S;
+        constexpr int TY = kVariants[V].nw * kVariants[V].ry;
+        constexpr long lds = 2L * CompSet<Ops, GW, KO>::NS * (TY + 2 * R) *
+                             (64 + 2 * R) * (long)sizeof(S);
+        // Keep at least two workgroups per CU (160 KB LDS); dissipation
+        // variants only in the default tile shape.
+        if constexpr (lds <= 80 * 1024 && (!KO || V == kDefaultVariant)) {
+            StageEntry &e = kt.stage[K][(int)P][R][GW ? 1 : 0][KO ? 1 : 0][V];
+            e.fn = &launch_stage<Ops, K, P, R, GW, KO, V>;
+            e.occupancy = &occupancy_stage<Ops, K, P, R, GW, KO, V>;
+            e.threads = 64 * kVariants[V].nw;
+            e.ty = kVariants[V].nw * kVariants[V].ry;
+        }
+    };
+    put(std::integral_constant<int, 0>{});
+    put(std::integral_constant<int, 1>{});
+    put(std::integral_constant<int, 2>{});
+}
+
+template <class Ops, Prec P, int R, bool GW, bool KO>
+void fill_kinds(KernelTable &kt) {
+    fill_variants<Ops, kSsprk1, P, R, GW, KO>(kt);
+    fill_variants<Ops, kSsprk2, P, R, GW, KO>(kt);
+    fill_variants<Ops, kSsprk3, P, R, GW, KO>(kt);
+    fill_variants<Ops, kFillRhs, P, R, GW, KO>(kt);
+    fill_variants<Ops, kFillAddRhs, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn0, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn1, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn2, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn3, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRknF0, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRknF1, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRknF2, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRknF3, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRknF4, P, R, GW, KO>(kt);
+    // Leapfrog writes u_new momenta in place while the stage reads them
+    // point-wise; with dissipation the momenta are differenced, so that
+    // combination is not instantiated.
+    if constexpr (!KO) {
+        fill_variants<Ops, kLeap1, P, R, GW, KO>(kt);
+        fill_variants<Ops, kLeap2, P, R, GW, KO>(kt);
+    }
+}
+
+template <class Ops, Prec P, int R>
+void fill_radius(KernelTable &kt) {
+    if constexpr (SHM_GW_MODES & 1) {
+        fill_kinds<Ops, P, R, false, false>(kt);
+        if constexpr (SHM_WITH_KO) fill_kinds<Ops, P, R, false, true>(kt);
+    }
+    if constexpr (SHM_GW_MODES & 2) {
+        fill_kinds<Ops, P, R, true, false>(kt);
+        if constexpr (SHM_WITH_KO) fill_kinds<Ops, P, R, true, true>(kt);
+    }
 }
 
 template <class Ops, Prec P, bool GW>
@@ -72,89 +179,38 @@ hipError_t launch_probe(ProbeOut *out, const double *p, const double *pg,
     return hipGetLastError();
 }
 
-template <class Ops, class Pol, Prec P, int R, bool GW, bool KO>
-void fill_variants(KernelTable &kt) {
-    auto put = [&](auto vc) {
-        constexpr int V = decltype(vc)::value;
-        using S = typename PrecTraits<P>::S;
-        constexpr int TY = kVariants[V].nw * kVariants[V].ry;
-        constexpr long lds = 2L * CompSet<Ops, GW, KO>::NS * (TY + 2 * R) *
-                             (64 + 2 * R) * (long)sizeof(S);
-        // Keep at least two workgroups per CU (160 KB LDS).
-        if constexpr (lds <= 80 * 1024) {
-            StageEntry &e =
-                kt.stage[PolOf<Pol>::id][(int)P][R][GW ? 1 : 0][KO ? 1 : 0][V];
-            e.fn = &launch_stage<Ops, Pol, P, R, GW, KO, V>;
-            e.occupancy = &occupancy_stage<Ops, Pol, P, R, GW, KO, V>;
-            e.threads = 64 * kVariants[V].nw;
-            e.ty = kVariants[V].nw * kVariants[V].ry;
-        }
-    };
-    put(std::integral_constant<int, 0>{});
-    put(std::integral_constant<int, 1>{});
-    put(std::integral_constant<int, 2>{});
-}
-
-template <class Ops, Prec P, int R, bool GW, bool KO>
-void fill_policies(KernelTable &kt) {
-    fill_variants<Ops, PolSsprk, P, R, GW, KO>(kt);
-    fill_variants<Ops, PolRkn, P, R, GW, KO>(kt);
-    // Leapfrog writes u_new momenta in place while the stage reads them
-    // point-wise; with dissipation the momenta are differenced, so that
-    // combination is not instantiated.
-    if constexpr (!KO) fill_variants<Ops, PolLeapfrog, P, R, GW, KO>(kt);
-}
-
-template <class Ops, Prec P, int R>
-void fill_radius(KernelTable &kt) {
-    if constexpr ((SHM_R_MASK >> R) & 1) {
-        fill_policies<Ops, P, R, false, false>(kt);
-        if constexpr (SHM_WITH_KO) fill_policies<Ops, P, R, false, true>(kt);
-        if constexpr (SHM_WITH_GW) {
-            fill_policies<Ops, P, R, true, false>(kt);
-            if constexpr (SHM_WITH_KO) fill_policies<Ops, P, R, true, true>(kt);
-        }
-    }
-}
-
-template <class Ops>
-const KernelTable &project_kernel_table() {
-    static const KernelTable table = [] {
-        KernelTable kt;
-        kt.nscalars = Ops::kNScalars;
-        kt.nfields = Ops::kNFields;
-        kt.ngw = Ops::kNGw;
-        kt.built_gw = SHM_WITH_GW != 0;
-        kt.built_ko = SHM_WITH_KO != 0;
-        kt.r_mask = SHM_R_MASK;
-        fill_radius<Ops, Prec::F64, 1>(kt);
-        fill_radius<Ops, Prec::F64, 2>(kt);
-        fill_radius<Ops, Prec::F64, 3>(kt);
-        fill_radius<Ops, Prec::F32C, 1>(kt);
-        fill_radius<Ops, Prec::F32C, 2>(kt);
-        fill_radius<Ops, Prec::F32C, 3>(kt);
-        kt.te_tag[0][0] = &launch_te_tag<Ops, Prec::F64, false>;
-        kt.te_tag[1][0] = &launch_te_tag<Ops, Prec::F32C, false>;
-        kt.probe[0] = &launch_probe<Ops, false>;
-        if constexpr (SHM_WITH_GW) {
-            kt.te_tag[0][1] = &launch_te_tag<Ops, Prec::F64, true>;
-            kt.te_tag[1][1] = &launch_te_tag<Ops, Prec::F32C, true>;
-            kt.probe[1] = &launch_probe<Ops, true>;
-        }
-        return kt;
-    }();
-    return table;
-}
-
 }  // namespace shm
 
-namespace sledgehamr {
-class Sledgehamr;
+#ifdef SHM_PROJECT_CLASS
+using ShmProjectOps = SHM_PROJECT_CLASS::ShmOps;
+
+#if defined(SHM_TU_PREC) && defined(SHM_TU_R)
+namespace shm {
+void SHM_PP_CAT(SHM_PP_CAT(fill_table_p, SHM_TU_PREC),
+                SHM_PP_CAT(_r, SHM_TU_R))(KernelTable &kt) {
+    fill_radius<ShmProjectOps, (Prec)SHM_TU_PREC, SHM_TU_R>(kt);
 }
+}  // namespace shm
+#else
+namespace shm {
+void fill_table_aux(KernelTable &kt) {
+    if constexpr (SHM_GW_MODES & 1) {
+        kt.te_tag[0][0] = &launch_te_tag<ShmProjectOps, Prec::F64, false>;
+        kt.te_tag[1][0] = &launch_te_tag<ShmProjectOps, Prec::F32C, false>;
+        kt.probe[0] = &launch_probe<ShmProjectOps, false>;
+    }
+    if constexpr (SHM_GW_MODES & 2) {
+        kt.te_tag[0][1] = &launch_te_tag<ShmProjectOps, Prec::F64, true>;
+        kt.te_tag[1][1] = &launch_te_tag<ShmProjectOps, Prec::F32C, true>;
+        kt.probe[1] = &launch_probe<ShmProjectOps, true>;
+    }
+}
+}  // namespace shm
 
 /** Registers the project class of this library: shm_create() builds it. */
-#define SHM_REGISTER_PROJECT(cls)                                              \
-    extern "C" __attribute__((visibility("default")))                         \
-    sledgehamr::Sledgehamr *shm_project_factory() {                            \
-        return new cls();                                                      \
-    }
+extern "C" __attribute__((visibility("default"))) sledgehamr::Sledgehamr *
+shm_project_factory() {
+    return new SHM_PROJECT_CLASS();
+}
+#endif
+#endif

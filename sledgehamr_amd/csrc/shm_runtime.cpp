@@ -135,6 +135,7 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
         // Place the brick so that the first valid cell is 32-element aligned.
         const long long start = round_up(end + g, 32) - g;
         bd.base = start + (long long)g * bd.kstride + (long long)g * bd.jstride + g;
+        bd.org = start;
         end = start + (long long)bd.kstride * (bx.n(2) + 2 * g);
         hbox.push_back(bd);
         vprefix.push_back(vprefix.back() + bx.vol());
@@ -620,17 +621,24 @@ void Runtime::fill_params(double *blk, double time, double dt, double dx,
 // Launch helpers.
 // ---------------------------------------------------------------------------
 
-void Runtime::launch_stage(int pol, LevelState &ls, StageArgs &a, int lev) {
+void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
     const int pi = (int)prec_;
     int v = variant >= 0 ? variant
                          : (lev + 1 < (int)tuned_variant_.size() && lev >= -1 &&
                                     tuned_variant_[lev + 1] >= 0
                                 ? tuned_variant_[lev + 1]
-                                : 1);
-    const auto &row = kt_.stage[pol][pi][R_][gw_ ? 1 : 0][ko_ ? 1 : 0];
-    // Variants over the LDS budget are not instantiated: fall back to the
-    // nearest smaller tile.
-    while (v > 0 && !row[v].fn) --v;
+                                : kDefaultVariant);
+    const auto &row = kt_.stage[kind][pi][R_][gw_ ? 1 : 0][ko_ ? 1 : 0];
+    // Variants over the LDS budget (and non-default dissipation variants)
+    // are not instantiated: take the nearest built tile shape.
+    if (!row[v].fn) {
+        int best = -1;
+        for (int d = 1; d < kNVar && best < 0; ++d) {
+            if (v - d >= 0 && row[v - d].fn) best = v - d;
+            else if (v + d < kNVar && row[v + d].fn) best = v + d;
+        }
+        if (best >= 0) v = best;
+    }
     const StageEntry &e = row[v];
     if (!e.fn) throw Error(4, "stage kernel not built for this configuration");
     int z = tz > 0 ? tz
@@ -657,6 +665,7 @@ void Runtime::launch_stage(int pol, LevelState &ls, StageArgs &a, int lev) {
     a.lev = lev;
     a.ko_strength = ko_dev_.as<double>();
     a.ko_order = sim_->dissipation_order;
+    a.ghost = ls.L.g;
     if (ntiles == 0) return;
     const int occ = e.occupancy();
     int grid = std::min<long long>(ntiles, (long long)occ * ncu_);
@@ -747,11 +756,10 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         StageArgs a{};
         a.prm = prm(t0);
         a.x = ls.U[old].get();
-        a.flags = PolSsprk::kWriteK | PolSsprk::kXisU0;
         a.kout = ls.K.get();
         a.y = ls.S1.get();
         a.c[1] = dt;
-        launch_stage(kPolSsprk, ls, a, lev);
+        launch_stage(kSsprk1, ls, a, lev);
         halo(ls, ls.S1.get(), stencil_comps());
 
         StageArgs b{};
@@ -761,10 +769,9 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         b.kin = ls.K.get();
         b.kout = ls.K.get();
         b.y = ls.S2.get();
-        b.flags = PolSsprk::kHasKin | PolSsprk::kWriteK;
         b.c[0] = 1.;
         b.c[1] = dt / 4.;
-        launch_stage(kPolSsprk, ls, b, lev);
+        launch_stage(kSsprk2, ls, b, lev);
         halo(ls, ls.S2.get(), stencil_comps());
 
         StageArgs c{};
@@ -775,10 +782,9 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         c.kin = ls.K.get();
         c.unew = ls.U[nw].get();
         c.unewlo = ulo_new;
-        c.flags = PolSsprk::kHasKin | PolSsprk::kFinal;
         c.c[0] = 0.25;
         c.c[1] = dt * 2. / 3.;
-        launch_stage(kPolSsprk, ls, c, lev);
+        launch_stage(kSsprk3, ls, c, lev);
     } else if (integrator_ >= 20) {
         const int s = (int)nodes_.size();
         for (int i = 0; i < s; ++i) {
@@ -794,10 +800,9 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             }
             if (i < s - 1) {
                 a.fout = ls.F[i].get();
-                launch_stage(kPolRkn, ls, a, lev);
+                launch_stage(kRkn0 + i, ls, a, lev);
                 halo(ls, ls.F[i].get(), pairs);
             } else {
-                a.flags = PolRkn::kFinal;
                 a.c[6] = dt;
                 for (int j = 0; j < i; ++j) {
                     a.c[7 + j] = dt * dt * bbar_[j];
@@ -807,7 +812,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                 a.c[16] = dt * b_[i];
                 a.unew = ls.U[nw].get();
                 a.unewlo = ulo_new;
-                launch_stage(kPolRkn, ls, a, lev);
+                launch_stage(kRknF0 + i, ls, a, lev);
             }
         }
     } else if (integrator_ == 11) {
@@ -822,7 +827,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         a.unewlo = ulo_new;
         a.c[0] = dt / 2.;
         a.c[1] = dt;
-        launch_stage(kPolLeapfrog, ls, a, lev);
+        launch_stage(kLeap1, ls, a, lev);
         halo(ls, ls.U[nw].get(), stencil_comps());
 
         StageArgs b{};
@@ -831,9 +836,8 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         b.vh = ls.VH.get();
         b.unew = ls.U[nw].get();
         b.unewlo = ulo_new;
-        b.flags = PolLeapfrog::kKick2;
         b.c[0] = dt / 2.;
-        launch_stage(kPolLeapfrog, ls, b, lev);
+        launch_stage(kLeap2, ls, b, lev);
     }
     // FillIntermediatePatch at the end of the step.
     halo(ls, ls.U[nw].get(), all_comps());
@@ -865,9 +869,8 @@ void Runtime::fill_rhs(int lev, double time, double weight, bool add) {
     a.x = ls.U[ls.cur].get();
     a.kin = ls.RHS.get();
     a.kout = ls.RHS.get();
-    a.flags = PolSsprk::kFillOnly | (add ? PolSsprk::kHasKin : 0);
     a.c[0] = weight;
-    launch_stage(kPolSsprk, ls, a, lev);
+    launch_stage(add ? kFillAddRhs : kFillRhs, ls, a, lev);
 }
 
 // ---------------------------------------------------------------------------
@@ -1269,17 +1272,16 @@ void Runtime::autotune(int lev) {
     fill_params(blk.data(), ls.t, dt, dx, lev);
     const double *prm = push_params(prm_host_, prm_ev_, prm_slot_,
                                     prm_dev_.as<double>(), stream_, blk);
-    int pol = integrator_ == 10 ? kPolSsprk
-                                : (integrator_ == 11 ? kPolLeapfrog : kPolRkn);
+    const int pol = integrator_ == 10 ? kSsprk1
+                                      : (integrator_ == 11 ? kLeap1 : kRkn0);
     StageArgs a{};
     a.prm = prm;
-    if (pol == kPolSsprk) {
+    if (pol == kSsprk1) {
         a.x = ls.U[old].get();
-        a.flags = PolSsprk::kWriteK | PolSsprk::kXisU0;
         a.kout = ls.K.get();
         a.y = ls.S1.get();
         a.c[1] = dt;
-    } else if (pol == kPolRkn) {
+    } else if (pol == kRkn0) {
         if (nodes_.size() < 2) return;
         a.u0 = ls.U[old].get();
         a.fout = ls.F[0].get();

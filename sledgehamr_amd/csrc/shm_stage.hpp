@@ -62,6 +62,7 @@ struct StageArgs {
     int flags;
     int nprev;
     int xcd_split;
+    int ghost;          // ghost width g of the level's bricks
     const void *x;      // stage-state source
     const void *u0;     // old state (hi part for FP32C)
     const void *u0lo;   // old state remainder (FP32C)
@@ -138,146 +139,176 @@ struct CompSet {
 
 // ---------------------------------------------------------------------------
 // Stage policies: how the stage state is formed and what a cell writes.
+//
+// Addressing: every array of a level shares one layout, so a cell is named
+// by (org, off): org = uniform element offset of the box brick's corner
+// (SGPRs), off = unsigned 32-bit offset inside the brick (one VGPR shared
+// by all arrays and components).  Component n of array X lives at
+// X + org + n * cstride + off, which maps onto global_load/store with a
+// scalar base and a 32-bit vector offset.
 // ---------------------------------------------------------------------------
 
+// The byte offset is formed in 32 bits so the address is (uniform 64-bit
+// base) + zext(32-bit offset): the scalar-base form of global_load/store.
 template <class T>
-__device__ AMREX_FORCE_INLINE T ld(const void *p, long long e) {
-    return static_cast<const T *>(p)[e];
+__device__ AMREX_FORCE_INLINE T ld(const void *p, long long base, unsigned off) {
+    const char *b = reinterpret_cast<const char *>(static_cast<const T *>(p) + base);
+    return *reinterpret_cast<const T *>(b + (unsigned)(off * (unsigned)sizeof(T)));
 }
 template <class T>
-__device__ AMREX_FORCE_INLINE void st(void *p, long long e, T v) {
-    static_cast<T *>(p)[e] = v;
+__device__ AMREX_FORCE_INLINE void st(void *p, long long base, unsigned off, T v) {
+    char *b = reinterpret_cast<char *>(static_cast<T *>(p) + base);
+    *reinterpret_cast<T *>(b + (unsigned)(off * (unsigned)sizeof(T))) = v;
 }
 
 /** Write a full-precision value as FP64, or as an FP32 pair hi + lo with
  *  hi = fl32(u) and lo = fl32(u - hi) (compensated storage). */
 template <class S>
-__device__ AMREX_FORCE_INLINE void store_state(void *hi, void *lo, long long e,
+__device__ AMREX_FORCE_INLINE void store_state(void *hi, void *lo,
+                                               long long base, unsigned off,
                                                double u) {
     if constexpr (std::is_same_v<S, float>) {
         const float h = (float)u;
-        st<float>(hi, e, h);
-        st<float>(lo, e, (float)(u - (double)h));
+        st<float>(hi, base, off, h);
+        st<float>(lo, base, off, (float)(u - (double)h));
     } else {
-        st<double>(hi, e, u);
+        st<double>(hi, base, off, u);
     }
 }
 template <class S>
 __device__ AMREX_FORCE_INLINE double load_state(const void *hi, const void *lo,
-                                                long long e) {
+                                                long long base, unsigned off) {
     if constexpr (std::is_same_v<S, float>)
-        return (double)ld<float>(hi, e) + (double)ld<float>(lo, e);
+        return (double)ld<float>(hi, base, off) + (double)ld<float>(lo, base, off);
     else
-        return ld<double>(hi, e);
+        return ld<double>(hi, base, off);
 }
 
-/** Low-storage SSPRK3 stage / FillRhs / FillAddRhs.
- *  k = f(X) + c0 * k_in;  then either y = u0 + c1 * k, or at the final stage
- *  u_new = u0 + c1 * k with compensated storage, or (fill mode) rhs_out = k. */
-struct PolSsprk {
-    enum : int { kHasKin = 1, kWriteK = 2, kFinal = 4, kXisU0 = 8, kFillOnly = 16 };
+/** Same, addressed by a 64-bit element index (auxiliary kernels). */
+template <class S>
+__device__ AMREX_FORCE_INLINE void store_state_at(void *hi, void *lo,
+                                                  long long e, double u) {
+    store_state<S>(hi, lo, e, 0u, u);
+}
+template <class S>
+__device__ AMREX_FORCE_INLINE double load_state_at(const void *hi,
+                                                   const void *lo, long long e) {
+    return load_state<S>(hi, lo, e, 0u);
+}
 
+/** Low-storage SSPRK3 stage / FillRhs / FillAddRhs, flags fixed at compile
+ *  time.  k = f(X) + c0 * k_in;  then either y = u0 + c1 * k, or at the
+ *  final stage u_new = u0 + c1 * k with compensated storage, or (fill mode)
+ *  rhs_out = k. */
+enum SsprkFlags : int { kHasKin = 1, kWriteK = 2, kFinal = 4, kXisU0 = 8, kFillOnly = 16 };
+
+template <int F>
+struct PolSsprk {
     template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, int n,
-                                                   long long e) {
-        return ld<S>(a.x, n * a.cstride + e);
+    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
+                                                   int n, unsigned off) {
+        return ld<S>(a.x, org + n * a.cstride, off);
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
-                                                        int n, long long e) {
-        return (double)ld<S>(a.x, n * a.cstride + e);
+                                                        long long org, int n,
+                                                        unsigned off) {
+        return (double)ld<S>(a.x, org + n * a.cstride, off);
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
-                                                    long long e0,
+                                                    long long org, unsigned off,
                                                     const double *rhs,
                                                     const double *ctr) {
 #pragma unroll
         for (int n = 0; n < CS::NC; ++n) {
-            const long long e = n * a.cstride + e0;
+            const long long b = org + n * a.cstride;
             double k = rhs[n];
-            if (a.flags & kHasKin) k += a.c[0] * (double)ld<S>(a.kin, e);
-            if (a.flags & kFillOnly) {
-                st<S>(a.kout, e, (S)k);
-                continue;
-            }
-            if (a.flags & kWriteK) st<S>(a.kout, e, (S)k);
-            const double u0 =
-                (a.flags & kXisU0) ? ctr[n] : (double)ld<S>(a.u0, e);
-            if (!(a.flags & kFinal)) {
-                st<S>(a.y, e, (S)(u0 + a.c[1] * k));
+            if constexpr ((F & kHasKin) != 0) k += a.c[0] * (double)ld<S>(a.kin, b, off);
+            if constexpr ((F & kFillOnly) != 0) {
+                st<S>(a.kout, b, off, (S)k);
             } else {
-                double u = u0;
-                if constexpr (std::is_same_v<S, float>)
-                    u += (double)ld<float>(a.u0lo, e);
-                store_state<S>(a.unew, a.unewlo, e, u + a.c[1] * k);
+                if constexpr ((F & kWriteK) != 0) st<S>(a.kout, b, off, (S)k);
+                double u0;
+                if constexpr ((F & kXisU0) != 0)
+                    u0 = ctr[n];
+                else
+                    u0 = (double)ld<S>(a.u0, b, off);
+                if constexpr ((F & kFinal) == 0) {
+                    st<S>(a.y, b, off, (S)(u0 + a.c[1] * k));
+                } else {
+                    if constexpr (std::is_same_v<S, float>)
+                        u0 += (double)ld<float>(a.u0lo, b, off);
+                    store_state<S>(a.unew, a.unewlo, b, off, u0 + a.c[1] * k);
+                }
             }
         }
     }
 };
 
-/** Runge-Kutta-Nystroem stage i (rkn.cpp:52-119).  Stage state of a field:
- *  y = phi0 + c0*pi0 + sum_j c[1+j] F_j; momenta stay at pi0.  Non-final
- *  stages write F_i (the momentum right-hand sides); the final stage writes
- *  phi = phi0 + c6*pi0 + sum_j c[7+j] F_j and pi = pi0 + sum_j c[12+j] F_j,
- *  with F of this stage taken from registers. */
+/** Runge-Kutta-Nystroem stage with NP earlier stages (rkn.cpp:52-119).
+ *  Stage state of a field: y = phi0 + c0*pi0 + sum_j c[1+j] F_j; momenta
+ *  stay at pi0.  Non-final stages write F (the momentum right-hand sides);
+ *  the final stage writes phi = phi0 + c6*pi0 + sum_j c[7+j] F_j and
+ *  pi = pi0 + sum_j c[12+j] F_j, with this stage's F (weights c[11], c[16])
+ *  taken from registers. */
+template <int NPREV, bool FINAL>
 struct PolRkn {
-    enum : int { kFinal = 4 };
+    static_assert(NPREV >= 0 && NPREV <= 4, "RKN supports up to 5 stages");
 
     template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static double state(const StageArgs &a, int n,
-                                                      long long e) {
+    __device__ AMREX_FORCE_INLINE static double state(const StageArgs &a,
+                                                      long long org, int n,
+                                                      unsigned off) {
         const int p = CS::pair_of_field(n);
-        if (p < 0) return (double)ld<S>(a.u0, n * a.cstride + e);
-        double y = (double)ld<S>(a.u0, n * a.cstride + e);
-        y += a.c[0] * (double)ld<S>(a.u0, CS::pair_mom(p) * a.cstride + e);
+        if (p < 0) return (double)ld<S>(a.u0, org + n * a.cstride, off);
+        double y = (double)ld<S>(a.u0, org + n * a.cstride, off);
+        y += a.c[0] * (double)ld<S>(a.u0, org + CS::pair_mom(p) * a.cstride, off);
 #pragma unroll
-        for (int j = 0; j < 5; ++j)
-            if (j < a.nprev) y += a.c[1 + j] * (double)ld<S>(a.f[j], p * a.cstride + e);
+        for (int j = 0; j < NPREV; ++j)
+            y += a.c[1 + j] * (double)ld<S>(a.f[j], org + p * a.cstride, off);
         return y;
     }
     template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, int n,
-                                                   long long e) {
-        return (S)state<CS, S>(a, n, e);
+    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
+                                                   int n, unsigned off) {
+        return (S)state<CS, S>(a, org, n, off);
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
-                                                        int n, long long e) {
-        return (double)(S)state<CS, S>(a, n, e);
+                                                        long long org, int n,
+                                                        unsigned off) {
+        return (double)(S)state<CS, S>(a, org, n, off);
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
-                                                    long long e0,
+                                                    long long org, unsigned off,
                                                     const double *rhs,
                                                     const double *ctr) {
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
             const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
             const double fi = rhs[nm];
-            if (!(a.flags & kFinal)) {
-                st<S>(a.fout, p * a.cstride + e0, (S)fi);
-                continue;
-            }
-            const long long ef = nf * a.cstride + e0, em = nm * a.cstride + e0;
-            const double pi0 = load_state<S>(a.u0, a.u0lo, em);
-            double phi = load_state<S>(a.u0, a.u0lo, ef);
-            phi += a.c[6] * pi0;
-            double pi = pi0;
+            const long long bp = org + p * a.cstride;
+            if constexpr (!FINAL) {
+                st<S>(a.fout, bp, off, (S)fi);
+            } else {
+                const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
+                const double pi0 = load_state<S>(a.u0, a.u0lo, bm, off);
+                double phi = load_state<S>(a.u0, a.u0lo, bf, off);
+                phi += a.c[6] * pi0;
+                double pi = pi0;
 #pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                if (j < a.nprev) {
-                    const double fj = (double)ld<S>(a.f[j], p * a.cstride + e0);
+                for (int j = 0; j < NPREV; ++j) {
+                    const double fj = (double)ld<S>(a.f[j], bp, off);
                     phi += a.c[7 + j] * fj;
                     pi += a.c[12 + j] * fj;
                 }
+                phi += a.c[11] * fi;
+                pi += a.c[16] * fi;
+                store_state<S>(a.unew, a.unewlo, bf, off, phi);
+                store_state<S>(a.unew, a.unewlo, bm, off, pi);
             }
-            // c[7+nprev], c[12+nprev] weight this stage's force; the host
-            // places them at index 11 and 16 so the unrolled loop stays fixed.
-            phi += a.c[11] * fi;
-            pi += a.c[16] * fi;
-            store_state<S>(a.unew, a.unewlo, ef, phi);
-            store_state<S>(a.unew, a.unewlo, em, pi);
         }
     }
 };
@@ -286,39 +317,38 @@ struct PolRkn {
  *  Kick 1 (X = u0): vh = pi0 + c0*a(pi) (stored FP64), phi1 = phi0 + c1*vh.
  *  Kick 2 (X = u_new with phi1 and the momenta u_new held before the step,
  *  exactly as the reference's second Rhs sees them): pi1 = vh + c0*a(pi). */
+template <bool KICK2>
 struct PolLeapfrog {
-    enum : int { kKick2 = 32 };
-
     template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, int n,
-                                                   long long e) {
-        return ld<S>(a.x, n * a.cstride + e);
+    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
+                                                   int n, unsigned off) {
+        return ld<S>(a.x, org + n * a.cstride, off);
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
-                                                        int n, long long e) {
-        return (double)ld<S>(a.x, n * a.cstride + e);
+                                                        long long org, int n,
+                                                        unsigned off) {
+        return (double)ld<S>(a.x, org + n * a.cstride, off);
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
-                                                    long long e0,
+                                                    long long org, unsigned off,
                                                     const double *rhs,
                                                     const double *ctr) {
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
             const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
-            const long long ev = p * a.cstride + e0;
-            if (!(a.flags & kKick2)) {
-                const double pi0 = load_state<S>(a.u0, a.u0lo, nm * a.cstride + e0);
+            const long long bp = org + p * a.cstride;
+            const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
+            if constexpr (!KICK2) {
+                const double pi0 = load_state<S>(a.u0, a.u0lo, bm, off);
                 const double vh = pi0 + a.c[0] * rhs[nm];
-                st<double>(a.vh, ev, vh);
-                const long long ef = nf * a.cstride + e0;
-                store_state<S>(a.unew, a.unewlo, ef,
-                               load_state<S>(a.u0, a.u0lo, ef) + a.c[1] * vh);
+                st<double>(a.vh, bp, off, vh);
+                store_state<S>(a.unew, a.unewlo, bf, off,
+                               load_state<S>(a.u0, a.u0lo, bf, off) + a.c[1] * vh);
             } else {
-                const double vh = ld<double>(a.vh, ev);
-                store_state<S>(a.unew, a.unewlo, nm * a.cstride + e0,
-                               vh + a.c[0] * rhs[nm]);
+                const double vh = ld<double>(a.vh, bp, off);
+                store_state<S>(a.unew, a.unewlo, bm, off, vh + a.c[0] * rhs[nm]);
             }
         }
     }
@@ -374,6 +404,20 @@ __global__ void __launch_bounds__(64 * NW)
 
     const int tx = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
+    const int g = a.ghost;
+    // Halo-ring cells of this thread (tile-local) and their LDS slots.
+    int hx[HPT], hy[HPT], hl[HPT];
+#pragma unroll
+    for (int m = 0; m < HPT; ++m) {
+        const int h = threadIdx.x + m * NT;
+        hx[m] = 0;
+        hy[m] = 0;
+        if (h < H) ring_xy<R, TY>(h, hx[m], hy[m]);
+        hl[m] = (hy[m] + R) * PX + (hx[m] + R);
+    }
+    const double time = a.prm[kPrmTime], dt = a.prm[kPrmDt], dx = a.prm[kPrmDx];
+    const double *params = a.prm + kPrmUser;
+    const double *gw_params = a.prm + kPrmGw;
 
     // Persistent walk over the tile table.  With xcd_split the tiles are cut
     // into 8 contiguous ranges, one per XCD (workgroups are dealt to XCDs
@@ -391,33 +435,42 @@ __global__ void __launch_bounds__(64 * NW)
         tstep = gridDim.x;
     }
 
-    const double time = a.prm[kPrmTime], dt = a.prm[kPrmDt], dx = a.prm[kPrmDx];
-    const double *params = a.prm + kPrmUser;
-    const double *gw_params = a.prm + kPrmGw;
-
     unsigned step = 0;
     for (; t < tend; t += tstep) {
         const TileDesc td = a.tiles[t];
         const BoxDesc bd = a.boxes[td.box];
         const int nx = bd.n[0], ny = bd.n[1];
-        const int js = bd.jstride, ks = bd.kstride;
-        const long long base = bd.base;
+        const unsigned js = bd.jstride, ks = bd.kstride;
+        const long long org = bd.org;
         const int lx = td.x0 + tx;
         const bool ldx = lx < nx + R;
         const bool cpx = lx < nx;
 
-        int ly[RY];
+        // Brick offsets of this thread's cells and halo-ring cells at z = -g.
+        unsigned coff[RY];
         bool ldy[RY], cpy[RY];
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            ly[r] = td.y0 + w + r * NW;
-            ldy[r] = ly[r] < ny + R;
-            cpy[r] = ly[r] < ny;
+            const int ly = td.y0 + w + r * NW;
+            ldy[r] = ly < ny + R;
+            cpy[r] = ly < ny;
+            coff[r] = (unsigned)(lx + g) + (unsigned)(ly + g) * js;
         }
-
-        auto cell = [&](int x, int y, int z) -> long long {
-            return base + (long long)(x + y * js) + (long long)z * ks;
-        };
+        // Loads are unconditional: a cell no valid cell reads (beyond the
+        // ghost reach of a partial tile) loads the brick corner instead, and
+        // its value is never used.
+        unsigned hoff[HPT];
+#pragma unroll
+        for (int m = 0; m < HPT; ++m) {
+            const int h = threadIdx.x + m * NT;
+            const int gx = td.x0 + hx[m], gy = td.y0 + hy[m];
+            const bool ok = h < H && gx < nx + R && gy < ny + R;
+            hoff[m] = ok ? (unsigned)(gx + g) + (unsigned)(gy + g) * js : 0u;
+        }
+        unsigned qoff[RY];  // queue loads: own column, clamped like the ring
+#pragma unroll
+        for (int r = 0; r < RY; ++r) qoff[r] = (ldx && ldy[r]) ? coff[r] : 0u;
+        auto zoff = [&](int z) { return (unsigned)(z + g) * ks; };
 
         S q[NS][RY][Q];
         S nq[NS][RY];
@@ -429,43 +482,31 @@ __global__ void __launch_bounds__(64 * NW)
         for (int s = 0; s < NS; ++s)
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
-                const bool ok = ldx && ldy[r];
 #pragma unroll
                 for (int m = 1; m < Q; ++m)
-                    q[s][r][m] = ok ? Pol::template load_st<CS, S>(
-                                          a, CS::comp_of_slot(s),
-                                          cell(lx, ly[r], td.z0 - R + m - 1))
-                                    : (S)0;
-                nq[s][r] = ok ? Pol::template load_st<CS, S>(
-                                    a, CS::comp_of_slot(s),
-                                    cell(lx, ly[r], td.z0 + R))
-                              : (S)0;
+                    q[s][r][m] = Pol::template load_st<CS, S>(
+                        a, org, CS::comp_of_slot(s),
+                        qoff[r] + zoff(td.z0 - R + m - 1));
+                nq[s][r] = Pol::template load_st<CS, S>(
+                    a, org, CS::comp_of_slot(s), qoff[r] + zoff(td.z0 + R));
             }
         auto load_halo = [&](int z) {
+            const unsigned zo = zoff(z);
 #pragma unroll
-            for (int m = 0; m < HPT; ++m) {
-                const int h = threadIdx.x + m * NT;
-                int hx = 0, hy = 0;
-                if (h < H) ring_xy<R, TY>(h, hx, hy);
-                const int gx = td.x0 + hx, gy = td.y0 + hy;
-                const bool ok = h < H && gx < nx + R && gy < ny + R;
+            for (int m = 0; m < HPT; ++m)
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    hv[s][m] = ok ? Pol::template load_st<CS, S>(
-                                        a, CS::comp_of_slot(s), cell(gx, gy, z))
-                                  : (S)0;
-            }
+                    hv[s][m] = Pol::template load_st<CS, S>(
+                        a, org, CS::comp_of_slot(s), hoff[m] + zo);
         };
         auto load_point = [&](int z) {
+            const unsigned zo = zoff(z);
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int r = 0; r < RY; ++r)
-                    pv[p][r] = (cpx && cpy[r])
-                                   ? Pol::template load_pt<CS, S>(
-                                         a, CS::comp_of_point(p),
-                                         cell(lx, ly[r], z))
-                                   : 0.;
+                    pv[p][r] = Pol::template load_pt<CS, S>(
+                        a, org, CS::comp_of_point(p), qoff[r] + zo);
         };
         load_halo(td.z0);
         load_point(td.z0);
@@ -488,14 +529,9 @@ __global__ void __launch_bounds__(64 * NW)
                 for (int r = 0; r < RY; ++r)
                     lds[buf][s][w + r * NW + R][tx + R] = q[s][r][R];
 #pragma unroll
-                for (int m = 0; m < HPT; ++m) {
-                    const int h = threadIdx.x + m * NT;
-                    if (h < H) {
-                        int hx, hy;
-                        ring_xy<R, TY>(h, hx, hy);
-                        lds[buf][s][hy + R][hx + R] = hv[s][m];
-                    }
-                }
+                for (int m = 0; m < HPT; ++m)
+                    if (m < HPT - 1 || threadIdx.x + m * NT < H)
+                        (&lds[buf][s][0][0])[hl[m]] = hv[s][m];
             }
             double pc[NP > 0 ? NP : 1][RY];
 #pragma unroll
@@ -505,24 +541,23 @@ __global__ void __launch_bounds__(64 * NW)
 
             // Prefetch plane z+1 (queue head z+1+R, halo ring, point values).
             if (z + 1 < td.z1) {
+                const unsigned zq = zoff(z + 1 + R);
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
 #pragma unroll
                     for (int r = 0; r < RY; ++r)
-                        nq[s][r] = (ldx && ldy[r])
-                                       ? Pol::template load_st<CS, S>(
-                                             a, CS::comp_of_slot(s),
-                                             cell(lx, ly[r], z + 1 + R))
-                                       : (S)0;
+                        nq[s][r] = Pol::template load_st<CS, S>(
+                            a, org, CS::comp_of_slot(s), qoff[r] + zq);
                 load_halo(z + 1);
                 load_point(z + 1);
             }
             __syncthreads();
 
+            const unsigned zc = zoff(z);
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
                 if (!(cpx && cpy[r])) continue;
-                const int gi = bd.lo[0] + lx, gj = bd.lo[1] + ly[r],
+                const int gi = bd.lo[0] + lx, gj = bd.lo[1] + td.y0 + w + r * NW,
                           gk = bd.lo[2] + z;
                 double ctr[NC];
 #pragma unroll
@@ -578,7 +613,7 @@ __global__ void __launch_bounds__(64 * NW)
                                 sv, gi, gj, gk, n, dx, a.ko_strength[n]);
                     }
                 }
-                Pol::template store<CS, S>(a, cell(lx, ly[r], z), rhs, ctr);
+                Pol::template store<CS, S>(a, org, coff[r] + zc, rhs, ctr);
             }
         }
     }

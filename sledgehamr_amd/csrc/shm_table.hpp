@@ -12,7 +12,27 @@ struct StageArgs;
 struct TeTagArgs;
 struct ProbeOut;
 
-enum PolId : int { kPolSsprk = 0, kPolRkn = 1, kPolLeapfrog = 2, kNPol = 3 };
+/** One instantiated stage program per kind: the integrator stage is a
+ *  compile-time property of the kernel, so its store path has no branches. */
+enum StageKind : int {
+    kSsprk1 = 0,   // k = f(u0); y = u0 + dt k
+    kSsprk2,       // k = f(y1) + k; y = u0 + dt/4 k
+    kSsprk3,       // k = f(y2) + k/4; u = u0 + 2dt/3 k (compensated)
+    kFillRhs,      // rhs = f(u)
+    kFillAddRhs,   // rhs = f(u) + w rhs
+    kRkn0,         // RKN non-final stage with 0..3 earlier stages
+    kRkn1,
+    kRkn2,
+    kRkn3,
+    kRknF0,        // RKN final stage with 0..4 earlier stages
+    kRknF1,
+    kRknF2,
+    kRknF3,
+    kRknF4,
+    kLeap1,        // leapfrog kick + drift
+    kLeap2,        // leapfrog second kick
+    kNKind
+};
 
 /** Tile shape variants: NW wavefronts per workgroup, RY rows per thread
  *  (tile = 64 x NW*RY cells in x/y).  The autotuner picks one per level. */
@@ -20,7 +40,8 @@ struct TileVariant {
     int nw, ry;
 };
 constexpr int kNVar = 3;
-constexpr TileVariant kVariants[kNVar] = {{4, 1}, {4, 2}, {4, 4}};
+constexpr TileVariant kVariants[kNVar] = {{4, 1}, {8, 1}, {16, 1}};
+constexpr int kDefaultVariant = 1;
 
 using StageFn = hipError_t (*)(const StageArgs &, int grid, hipStream_t);
 using OccFn = int (*)();
@@ -39,8 +60,8 @@ struct KernelTable {
     int nscalars = 0, nfields = 0, ngw = 0;
     bool built_gw = false, built_ko = false;
     int r_mask = 0;
-    // [policy][precision][R][gw][ko][variant]
-    StageEntry stage[kNPol][2][4][2][2][kNVar];
+    // [kind][precision][R][gw][ko][variant]
+    StageEntry stage[kNKind][2][4][2][2][kNVar];
     TeTagFn te_tag[2][2] = {};  // [precision][gw]
     ProbeFn probe[2] = {};      // [gw]
 };

@@ -19,12 +19,11 @@ void Rhs(const amrex::Array4<double>& rhs,
          const double time, const double dt, const double dx,
          const double* params) {
     const double lambda = params[0];
+    const double damping = params[1];  // 2/eta at this stage time
     double Psi1 = state(i, j, k, Scalar::Psi1);
     double Psi2 = state(i, j, k, Scalar::Psi2);
     double Pi1  = state(i, j, k, Scalar::Pi1);
     double Pi2  = state(i, j, k, Scalar::Pi2);
-
-    double eta = time;
 
     constexpr int order = 2;
     double laplacian_Psi1 = sledgehamr::utils::Laplacian<order>(
@@ -36,8 +35,8 @@ void Rhs(const amrex::Array4<double>& rhs,
 
     rhs(i, j, k, Scalar::Psi1) =  Pi1;
     rhs(i, j, k, Scalar::Psi2) =  Pi2;
-    rhs(i, j, k, Scalar::Pi1)  = -Pi1*2./eta + laplacian_Psi1 - Psi1*potential;
-    rhs(i, j, k, Scalar::Pi2)  = -Pi2*2./eta + laplacian_Psi2 - Psi2*potential;
+    rhs(i, j, k, Scalar::Pi1)  = -Pi1*damping + laplacian_Psi1 - Psi1*potential;
+    rhs(i, j, k, Scalar::Pi2)  = -Pi2*damping + laplacian_Psi2 - Psi2*potential;
 }
 
 SLEDGEHAMR_FINISH_SETUP
@@ -53,7 +52,9 @@ class ComplexScalar : public sledgehamr::Sledgehamr {
 
     void SetParamsRhs(std::vector<double>& params, const double time,
                       const int lev) override {
-        params = {lambda};
+        // Conformal time eta = time; the damping 2/eta is evaluated once per
+        // stage here instead of once per cell.
+        params = {lambda, 2. / time};
     }
 
   private:
