@@ -204,27 +204,55 @@ enum SsprkFlags : int { kHasKin = 1, kWriteK = 2, kFinal = 4, kXisU0 = 8, kFillO
 
 template <int F>
 struct PolSsprk {
+    template <class S>
+    using Ext = S;
+    // Per component: k_in, u0, u0 remainder -- whichever this stage reads.
+    template <class CS, class S>
+    static constexpr int per_comp() {
+        constexpr bool fill = (F & kFillOnly) != 0;
+        return ((F & kHasKin) ? 1 : 0) + ((!fill && !(F & kXisU0)) ? 1 : 0) +
+               ((!fill && (F & kFinal) && std::is_same_v<S, float>) ? 1 : 0);
+    }
+    template <class CS, class S>
+    static constexpr int next() { return CS::NC * per_comp<CS, S>(); }
+
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
                                                    int n, unsigned off) {
         return ld<S>(a.x, org + n * a.cstride, off);
     }
     template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
-                                                        long long org, int n,
-                                                        unsigned off) {
-        return (double)ld<S>(a.x, org + n * a.cstride, off);
+    __device__ AMREX_FORCE_INLINE static S load_pt(const StageArgs &a,
+                                                   long long org, int n,
+                                                   unsigned off) {
+        return ld<S>(a.x, org + n * a.cstride, off);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
+                                                       long long org, unsigned off,
+                                                       S *e) {
+        constexpr bool fill = (F & kFillOnly) != 0;
+        int i = 0;
+#pragma unroll
+        for (int n = 0; n < CS::NC; ++n) {
+            const long long b = org + n * a.cstride;
+            if constexpr ((F & kHasKin) != 0) e[i++] = ld<S>(a.kin, b, off);
+            if constexpr (!fill && !(F & kXisU0)) e[i++] = ld<S>(a.u0, b, off);
+            if constexpr (!fill && (F & kFinal) && std::is_same_v<S, float>)
+                e[i++] = ld<float>(a.u0lo, b, off);
+        }
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
-                                                    const double *ctr) {
+                                                    const double *ctr, const S *e) {
+        int i = 0;
 #pragma unroll
         for (int n = 0; n < CS::NC; ++n) {
             const long long b = org + n * a.cstride;
             double k = rhs[n];
-            if constexpr ((F & kHasKin) != 0) k += a.c[0] * (double)ld<S>(a.kin, b, off);
+            if constexpr ((F & kHasKin) != 0) k += a.c[0] * (double)e[i++];
             if constexpr ((F & kFillOnly) != 0) {
                 st<S>(a.kout, b, off, (S)k);
             } else {
@@ -233,12 +261,11 @@ struct PolSsprk {
                 if constexpr ((F & kXisU0) != 0)
                     u0 = ctr[n];
                 else
-                    u0 = (double)ld<S>(a.u0, b, off);
+                    u0 = (double)e[i++];
                 if constexpr ((F & kFinal) == 0) {
                     st<S>(a.y, b, off, (S)(u0 + a.c[1] * k));
                 } else {
-                    if constexpr (std::is_same_v<S, float>)
-                        u0 += (double)ld<float>(a.u0lo, b, off);
+                    if constexpr (std::is_same_v<S, float>) u0 += (double)e[i++];
                     store_state<S>(a.unew, a.unewlo, b, off, u0 + a.c[1] * k);
                 }
             }
@@ -255,6 +282,14 @@ struct PolSsprk {
 template <int NPREV, bool FINAL>
 struct PolRkn {
     static_assert(NPREV >= 0 && NPREV <= 4, "RKN supports up to 5 stages");
+    template <class S>
+    using Ext = S;
+    // Final stage, per pair: phi0 and pi0 (with remainders) and F_j.
+    template <class CS, class S>
+    static constexpr int next() {
+        return FINAL ? CS::NPAIR * (2 * (std::is_same_v<S, float> ? 2 : 1) + NPREV)
+                     : 0;
+    }
 
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static double state(const StageArgs &a,
@@ -275,32 +310,58 @@ struct PolRkn {
         return (S)state<CS, S>(a, org, n, off);
     }
     template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
-                                                        long long org, int n,
-                                                        unsigned off) {
-        return (double)(S)state<CS, S>(a, org, n, off);
+    __device__ AMREX_FORCE_INLINE static S load_pt(const StageArgs &a,
+                                                   long long org, int n,
+                                                   unsigned off) {
+        return (S)state<CS, S>(a, org, n, off);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
+                                                       long long org, unsigned off,
+                                                       S *e) {
+        if constexpr (FINAL) {
+            int i = 0;
+#pragma unroll
+            for (int p = 0; p < CS::NPAIR; ++p) {
+                const long long bf = org + CS::pair_field(p) * a.cstride;
+                const long long bm = org + CS::pair_mom(p) * a.cstride;
+                e[i++] = ld<S>(a.u0, bf, off);
+                e[i++] = ld<S>(a.u0, bm, off);
+                if constexpr (std::is_same_v<S, float>) {
+                    e[i++] = ld<float>(a.u0lo, bf, off);
+                    e[i++] = ld<float>(a.u0lo, bm, off);
+                }
+#pragma unroll
+                for (int j = 0; j < NPREV; ++j)
+                    e[i++] = ld<S>(a.f[j], org + p * a.cstride, off);
+            }
+        }
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
-                                                    const double *ctr) {
+                                                    const double *ctr, const S *e) {
+        int i = 0;
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
             const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
             const double fi = rhs[nm];
-            const long long bp = org + p * a.cstride;
             if constexpr (!FINAL) {
-                st<S>(a.fout, bp, off, (S)fi);
+                st<S>(a.fout, org + p * a.cstride, off, (S)fi);
             } else {
                 const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
-                const double pi0 = load_state<S>(a.u0, a.u0lo, bm, off);
-                double phi = load_state<S>(a.u0, a.u0lo, bf, off);
+                double phi = (double)e[i++];
+                double pi0 = (double)e[i++];
+                if constexpr (std::is_same_v<S, float>) {
+                    phi += (double)e[i++];
+                    pi0 += (double)e[i++];
+                }
                 phi += a.c[6] * pi0;
                 double pi = pi0;
 #pragma unroll
                 for (int j = 0; j < NPREV; ++j) {
-                    const double fj = (double)ld<S>(a.f[j], bp, off);
+                    const double fj = (double)e[i++];
                     phi += a.c[7 + j] * fj;
                     pi += a.c[12 + j] * fj;
                 }
@@ -319,35 +380,58 @@ struct PolRkn {
  *  exactly as the reference's second Rhs sees them): pi1 = vh + c0*a(pi). */
 template <bool KICK2>
 struct PolLeapfrog {
+    template <class S>
+    using Ext = double;
+    // Kick 1 per pair: pi0 and phi0 (full precision); kick 2: vh.
+    template <class CS, class S>
+    static constexpr int next() { return CS::NPAIR * (KICK2 ? 1 : 2); }
+
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
                                                    int n, unsigned off) {
         return ld<S>(a.x, org + n * a.cstride, off);
     }
     template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static double load_pt(const StageArgs &a,
-                                                        long long org, int n,
-                                                        unsigned off) {
-        return (double)ld<S>(a.x, org + n * a.cstride, off);
+    __device__ AMREX_FORCE_INLINE static S load_pt(const StageArgs &a,
+                                                   long long org, int n,
+                                                   unsigned off) {
+        return ld<S>(a.x, org + n * a.cstride, off);
+    }
+    template <class CS, class S>
+    __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
+                                                       long long org, unsigned off,
+                                                       double *e) {
+        int i = 0;
+#pragma unroll
+        for (int p = 0; p < CS::NPAIR; ++p) {
+            if constexpr (!KICK2) {
+                e[i++] = load_state<S>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+                e[i++] = load_state<S>(a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
+            } else {
+                e[i++] = ld<double>(a.vh, org + p * a.cstride, off);
+            }
+        }
     }
     template <class CS, class S>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
-                                                    const double *ctr) {
+                                                    const double *ctr,
+                                                    const double *e) {
+        int i = 0;
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
             const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
             const long long bp = org + p * a.cstride;
             const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
             if constexpr (!KICK2) {
-                const double pi0 = load_state<S>(a.u0, a.u0lo, bm, off);
+                const double pi0 = e[i++];
+                const double phi0 = e[i++];
                 const double vh = pi0 + a.c[0] * rhs[nm];
                 st<double>(a.vh, bp, off, vh);
-                store_state<S>(a.unew, a.unewlo, bf, off,
-                               load_state<S>(a.u0, a.u0lo, bf, off) + a.c[1] * vh);
+                store_state<S>(a.unew, a.unewlo, bf, off, phi0 + a.c[1] * vh);
             } else {
-                const double vh = ld<double>(a.vh, bp, off);
+                const double vh = e[i++];
                 store_state<S>(a.unew, a.unewlo, bm, off, vh + a.c[0] * rhs[nm]);
             }
         }
@@ -472,10 +556,13 @@ __global__ void __launch_bounds__(64 * NW)
         for (int r = 0; r < RY; ++r) qoff[r] = (ldx && ldy[r]) ? coff[r] : 0u;
         auto zoff = [&](int z) { return (unsigned)(z + g) * ks; };
 
+        using E = typename Pol::template Ext<S>;
+        constexpr int NE = Pol::template next<CS, S>();
         S q[NS][RY][Q];
         S nq[NS][RY];
         S hv[NS][HPT];
-        double pv[NP > 0 ? NP : 1][RY];
+        S pv[NP > 0 ? NP : 1][RY];
+        E ev[RY][NE > 0 ? NE : 1];  // store-time inputs of the next plane
 
         // Prologue: planes z0-R .. z0+R-1 into q[..][1..2R], plane z0+R into nq.
 #pragma unroll
@@ -507,6 +594,11 @@ __global__ void __launch_bounds__(64 * NW)
                 for (int r = 0; r < RY; ++r)
                     pv[p][r] = Pol::template load_pt<CS, S>(
                         a, org, CS::comp_of_point(p), qoff[r] + zo);
+            if constexpr (NE > 0) {
+#pragma unroll
+                for (int r = 0; r < RY; ++r)
+                    Pol::template load_ext<CS, S>(a, org, qoff[r] + zo, ev[r]);
+            }
         };
         load_halo(td.z0);
         load_point(td.z0);
@@ -533,11 +625,16 @@ __global__ void __launch_bounds__(64 * NW)
                     if (m < HPT - 1 || threadIdx.x + m * NT < H)
                         (&lds[buf][s][0][0])[hl[m]] = hv[s][m];
             }
-            double pc[NP > 0 ? NP : 1][RY];
+            S pc[NP > 0 ? NP : 1][RY];
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int r = 0; r < RY; ++r) pc[p][r] = pv[p][r];
+            E ec[RY][NE > 0 ? NE : 1];
+#pragma unroll
+            for (int r = 0; r < RY; ++r)
+#pragma unroll
+                for (int i = 0; i < NE; ++i) ec[r][i] = ev[r][i];
 
             // Prefetch plane z+1 (queue head z+1+R, halo ring, point values).
             if (z + 1 < td.z1) {
@@ -565,7 +662,7 @@ __global__ void __launch_bounds__(64 * NW)
                     ctr[CS::comp_of_slot(s)] = (double)q[s][r][R];
 #pragma unroll
                 for (int p = 0; p < NP; ++p)
-                    ctr[CS::comp_of_point(p)] = pc[p][r];
+                    ctr[CS::comp_of_point(p)] = (double)pc[p][r];
 
                 TileCtx tc;
                 tc.ci = gi;
@@ -613,7 +710,7 @@ __global__ void __launch_bounds__(64 * NW)
                                 sv, gi, gj, gk, n, dx, a.ko_strength[n]);
                     }
                 }
-                Pol::template store<CS, S>(a, org, coff[r] + zc, rhs, ctr);
+                Pol::template store<CS, S>(a, org, coff[r] + zc, rhs, ctr, ec[r]);
             }
         }
     }
