@@ -21,6 +21,10 @@ struct BoxDesc {
                       // the ghosted brick is org + an unsigned 32-bit offset
     int jstride;      // elements per row
     int kstride;      // elements per xy-plane
+    int wrap;         // bit a: the box spans the periodic domain along axis
+                      // a, so stage kernels wrap that index instead of
+                      // reading ghost cells
+    int pad_;
 };
 
 /** One work unit of a fused stage kernel: an x/y tile of one box marched

@@ -112,11 +112,12 @@ static void upload_vec(DevMem &d, const std::vector<T> &v) {
 static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
 void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
-                   bool is_periodic) {
+                   bool is_periodic, bool is_independent) {
     boxes = b;
     g = ghost;
     N = domain_n;
     periodic = is_periodic;
+    independent = is_independent;
     hbox.clear();
     vprefix.assign(1, 0);
     cprefix.assign(1, 0);
@@ -136,6 +137,10 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
         const long long start = round_up(end + g, 32) - g;
         bd.base = start + (long long)g * bd.kstride + (long long)g * bd.jstride + g;
         bd.org = start;
+        bd.wrap = 0;
+        bd.pad_ = 0;
+        for (int a = 0; a < 3; ++a)
+            if (independent || (periodic && bx.n(a) == N)) bd.wrap |= 1 << a;
         end = start + (long long)bd.kstride * (bx.n(2) + 2 * g);
         hbox.push_back(bd);
         vprefix.push_back(vprefix.back() + bx.vol());
@@ -149,9 +154,10 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
 
     // Ghost copy plan: intersect each box's ghost shell with every
     // (periodically shifted) valid box, itself included for shift != 0.
-    halo.clear();
-    halo_total = 0;
-    const int smax = periodic ? 1 : 0;
+    // Independent boxes are each periodic on themselves.
+    halo_full.clear();
+    halo_full_total = 0;
+    const int smax = (periodic || independent) ? 1 : 0;
     for (int d = 0; d < (int)boxes.size(); ++d) {
         const Box3 &db = boxes[d];
         int glo[3], ghi[3];
@@ -161,12 +167,16 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
         }
         if (g == 0) continue;
         for (int s = 0; s < (int)boxes.size(); ++s) {
+            if (independent && s != d) continue;
             const Box3 &sb = boxes[s];
             for (int sz = -smax; sz <= smax; ++sz)
                 for (int sy = -smax; sy <= smax; ++sy)
                     for (int sx = -smax; sx <= smax; ++sx) {
                         if (s == d && sx == 0 && sy == 0 && sz == 0) continue;
-                        const int sh[3] = {sx * N, sy * N, sz * N};
+                        const int per[3] = {independent ? db.n(0) : N,
+                                            independent ? db.n(1) : N,
+                                            independent ? db.n(2) : N};
+                        const int sh[3] = {sx * per[0], sy * per[1], sz * per[2]};
                         HaloSeg seg;
                         bool empty = false;
                         for (int a = 0; a < 3; ++a) {
@@ -183,12 +193,23 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
                         if (empty) continue;
                         seg.dst_box = d;
                         seg.src_box = s;
-                        seg.start = halo_total;
-                        halo_total += (long long)seg.n[0] * seg.n[1] * seg.n[2];
-                        halo.push_back(seg);
+                        seg.start = halo_full_total;
+                        halo_full_total += (long long)seg.n[0] * seg.n[1] * seg.n[2];
+                        halo_full.push_back(seg);
                     }
         }
     }
+    // Stage kernels wrap boxes that span the domain along all axes, so their
+    // ghosts are left out of the per-stage plan.
+    halo.clear();
+    halo_total = 0;
+    for (HaloSeg seg : halo_full) {
+        if (hbox[seg.dst_box].wrap == 7) continue;
+        seg.start = halo_total;
+        halo_total += (long long)seg.n[0] * seg.n[1] * seg.n[2];
+        halo.push_back(seg);
+    }
+    upload_vec(dhalo_full, halo_full);
     upload_vec(dhalo, halo);
     tiles.clear();
 }
@@ -680,15 +701,17 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
     }
 }
 
-void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps) {
-    if (ls.L.halo_total == 0 || comps.empty()) return;
+void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps,
+                   bool full) {
+    const long long total = full ? ls.L.halo_full_total : ls.L.halo_total;
+    if (total == 0 || comps.empty()) return;
     HaloArgs h;
     h.arr = arr;
     h.cstride = ls.L.cstride;
     h.boxes = ls.L.dbox.as<BoxDesc>();
-    h.segs = ls.L.dhalo.as<HaloSeg>();
-    h.nseg = (int)ls.L.halo.size();
-    h.total = ls.L.halo_total;
+    h.segs = (full ? ls.L.dhalo_full : ls.L.dhalo).as<HaloSeg>();
+    h.nseg = (int)(full ? ls.L.halo_full : ls.L.halo).size();
+    h.total = total;
     for (size_t i = 0; i < comps.size(); i += 32) {
         h.ncomp = (int)std::min<size_t>(32, comps.size() - i);
         for (int c = 0; c < h.ncomp; ++c) h.comps[c] = comps[i + c];
@@ -993,6 +1016,10 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
             crse = &mirror;
         }
     }
+    // User criteria may read neighbours of a cell: complete every ghost cell
+    // of the fine state first (stage kernels wrap periodic boxes instead).
+    halo(fine, fine.U[fine.cur].get(), all_comps(), /*full=*/true);
+    fine.ghosts_ok[fine.cur] = false;
     const sledgehamr::Sledgehamr &S = *sim_;
     std::vector<double> ptag, pmod;
     sim_->SetParamsTagCellForRefinement(ptag, time, lev);

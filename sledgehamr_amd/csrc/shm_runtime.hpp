@@ -62,6 +62,7 @@ struct Layout {
     int N = 0;          // domain cells per axis at this level
     int g = 0;          // ghost width
     bool periodic = true;
+    bool independent = false;  // every box is its own periodic domain
     std::vector<Box3> boxes;
     std::vector<BoxDesc> hbox;
     DevMem dbox;
@@ -70,13 +71,16 @@ struct Layout {
     DevMem dvprefix;
     std::vector<long long> cprefix;  // coarse (half) cells per box (prefix)
     DevMem dcprefix;
-    std::vector<HaloSeg> halo;
+    std::vector<HaloSeg> halo;       // ghosts read by stage kernels
     DevMem dhalo;
     long long halo_total = 0;
+    std::vector<HaloSeg> halo_full;  // every ghost cell
+    DevMem dhalo_full;
+    long long halo_full_total = 0;
     std::map<std::pair<int, int>, std::pair<std::shared_ptr<DevMem>, int>> tiles;
 
     void build(const std::vector<Box3> &b, int ghost, int domain_n,
-               bool is_periodic);
+               bool is_periodic, bool is_independent = false);
     const TileDesc *get_tiles(int ty, int tz, int *ntiles);
     long long nvalid() const { return vprefix.empty() ? 0 : vprefix.back(); }
 };
@@ -164,7 +168,8 @@ class Runtime {
     void ensure_arrays(LevelState &ls, int integrator);
     size_t esize() const { return prec_ == Prec::F64 ? 8 : 4; }
     void launch_stage(int pol, LevelState &ls, StageArgs &a, int lev);
-    void halo(LevelState &ls, void *arr, const std::vector<int> &comps);
+    void halo(LevelState &ls, void *arr, const std::vector<int> &comps,
+              bool full = false);
     void fill_patch(LevelState &ls, int which);
     void integrate(LevelState &ls, int lev, double dt, double dx,
                    double t_old);

@@ -155,10 +155,12 @@ __device__ AMREX_FORCE_INLINE T ld(const void *p, long long base, unsigned off) 
     const char *b = reinterpret_cast<const char *>(static_cast<const T *>(p) + base);
     return *reinterpret_cast<const T *>(b + (unsigned)(off * (unsigned)sizeof(T)));
 }
+// Stage outputs are written once and not read again by the same kernel:
+// non-temporal stores keep them from displacing the stencil halos in L2.
 template <class T>
 __device__ AMREX_FORCE_INLINE void st(void *p, long long base, unsigned off, T v) {
     char *b = reinterpret_cast<char *>(static_cast<T *>(p) + base);
-    *reinterpret_cast<T *>(b + (unsigned)(off * (unsigned)sizeof(T))) = v;
+    __builtin_nontemporal_store(v, reinterpret_cast<T *>(b + (unsigned)(off * (unsigned)sizeof(T))));
 }
 
 /** Write a full-precision value as FP64, or as an FP32 pair hi + lo with
@@ -523,9 +525,21 @@ __global__ void __launch_bounds__(64 * NW)
     for (; t < tend; t += tstep) {
         const TileDesc td = a.tiles[t];
         const BoxDesc bd = a.boxes[td.box];
-        const int nx = bd.n[0], ny = bd.n[1];
+        const int nx = bd.n[0], ny = bd.n[1], nz = bd.n[2];
         const unsigned js = bd.jstride, ks = bd.kstride;
         const long long org = bd.org;
+        const int wrap = bd.wrap;
+        // Along axes the box spans periodically, indices within the stencil
+        // reach of the box are wrapped onto the box itself (no ghost reads).
+        auto wx = [&](int v) {
+            return (wrap & 1) ? (v < 0 ? v + nx : (v >= nx ? v - nx : v)) : v;
+        };
+        auto wy = [&](int v) {
+            return (wrap & 2) ? (v < 0 ? v + ny : (v >= ny ? v - ny : v)) : v;
+        };
+        auto wz = [&](int v) {
+            return (wrap & 4) ? (v < 0 ? v + nz : (v >= nz ? v - nz : v)) : v;
+        };
         const int lx = td.x0 + tx;
         const bool ldx = lx < nx + R;
         const bool cpx = lx < nx;
@@ -538,7 +552,7 @@ __global__ void __launch_bounds__(64 * NW)
             const int ly = td.y0 + w + r * NW;
             ldy[r] = ly < ny + R;
             cpy[r] = ly < ny;
-            coff[r] = (unsigned)(lx + g) + (unsigned)(ly + g) * js;
+            coff[r] = (unsigned)(wx(lx) + g) + (unsigned)(wy(ly) + g) * js;
         }
         // Loads are unconditional: a cell no valid cell reads (beyond the
         // ghost reach of a partial tile) loads the brick corner instead, and
@@ -549,12 +563,12 @@ __global__ void __launch_bounds__(64 * NW)
             const int h = threadIdx.x + m * NT;
             const int gx = td.x0 + hx[m], gy = td.y0 + hy[m];
             const bool ok = h < H && gx < nx + R && gy < ny + R;
-            hoff[m] = ok ? (unsigned)(gx + g) + (unsigned)(gy + g) * js : 0u;
+            hoff[m] = ok ? (unsigned)(wx(gx) + g) + (unsigned)(wy(gy) + g) * js : 0u;
         }
         unsigned qoff[RY];  // queue loads: own column, clamped like the ring
 #pragma unroll
         for (int r = 0; r < RY; ++r) qoff[r] = (ldx && ldy[r]) ? coff[r] : 0u;
-        auto zoff = [&](int z) { return (unsigned)(z + g) * ks; };
+        auto zoff = [&](int z) { return (unsigned)(wz(z) + g) * ks; };
 
         using E = typename Pol::template Ext<S>;
         constexpr int NE = Pol::template next<CS, S>();
