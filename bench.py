@@ -8,14 +8,18 @@ damping 2/eta, low-storage SSPRK3 (integrator.type = 10), L = 128
 seeded uniform in [0, 2 pi), moduli 1 + N(0, 0.05), momenta 0.
 
 A "step" is one SSPRK3 step of the level (3 RK stages).  `value` is
-cell-updates per second per RK stage of the FP32-compensated path
-(valid cells x stages / wall time), inputs resident in HBM.  The same
-seeded run on the FP64 path gives `fp64` and the field error.
+cell-updates per second per RK stage of the FP32-compensated path (FP32
+state + bf16 remainder; valid cells x stages / wall time), inputs resident in
+HBM.  The same seeded run on the FP64 path gives `fp64` and the field error;
+`fp32c` is the FP32 + FP32-remainder variant.
 
-roofline: the fused stage kernel's algorithmic bytes per launch (each stored
-array read once and written at most once per stage: 74.67 B per cell-stage
-for FP32 + 32-bit remainder, 128 B for FP64) over its mean launch time,
-measured with HIP events on the library's stream inside the timed region.
+roofline: the fused stage kernel's algorithmic bytes per launch over its mean
+launch time, measured with HIP events on the library's stream inside the
+timed region.  Algorithmic bytes: every stored array read once and written
+at most once per stage, stage states rebuilt from u0 and k (8 words per
+component and step, plus one read and one write of the remainder): 48 B per
+cell-stage for FP32 + bf16 remainder, 53.3 B with an FP32 remainder, 85.3 B
+for FP64 (C = 4).
 
 cpu_baseline: the FP64 oracle in the reference's unfused structure (Rhs pass,
 LinComb passes, ghost fill per stage; OpenMP), one step of the same 512^3
@@ -37,10 +41,13 @@ HBM_PEAK_GBS = 8000.0
 METRIC = ("cell-updates/s per RK stage + % HBM roofline; FP32-comp vs FP64 "
           "at ≤1e-6 rel err")
 # Algorithmic bytes per cell-stage of the fused low-storage SSPRK3 schedule
-# with C = 4 components: 12 words per component per step (3 stages), plus one
-# read and one write of the remainder array per component per step (FP32C).
-BYTES_PER_CELL_STAGE = {"fp32c": (12 * 4 * 4 + 2 * 4 * 4) / 3.0,
-                        "fp64": 12 * 4 * 8 / 3.0}
+# with C = 4 components.  Per component and step: stage 1 reads u0, writes k1;
+# stage 2 reads u0, k1, writes k2; stage 3 reads u0, k2, writes u (8 words),
+# plus one read and one write of the remainder for the compensated paths.
+C = 4
+BYTES_PER_CELL_STAGE = {"fp32c16": (8 * 4 + 2 * 2) * C / 3.0,
+                        "fp32c": (8 * 4 + 2 * 4) * C / 3.0,
+                        "fp64": 8 * 8 * C / 3.0}
 
 
 def inputs(N, L, cfl, t0):
@@ -114,7 +121,8 @@ def main():
     N = a.n
     inp = inputs(N, L=N * 0.25, cfl=0.2, t0=0.1)
     stages = 3
-    sim32, sec32, tm32 = run_gpu(shm, inp, shm.PREC_FP32C, a.steps, a.warmup, a.seed)
+    sim32, sec32, tm32 = run_gpu(shm, inp, shm.PREC_FP32C16, a.steps, a.warmup,
+                                 a.seed)
     cells = sim32.ncells(0)
     rate32 = cells * stages * a.steps / sec32
 
@@ -129,7 +137,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f32 remainder storage (compensated); Rhs arithmetic f64",
+        "dtype": "f32 + bf16 remainder storage (compensated); stencil sums f32, "
+                 "stage sums f64",
         "data": "synthetic (seeded random-phase complex field)",
         "config": {"workload": f"complex scalar {N}^3 periodic, 4th-order "
                                "Laplacian, SSPRK3, lambda=1, radiation damping",
@@ -140,7 +149,7 @@ def main():
     # Roofline of the dominant (fused stage) kernel.
     launches = tm32["stage_launches"]
     avg_ms = tm32["stage_ms"] / max(launches, 1)
-    bpl = BYTES_PER_CELL_STAGE["fp32c"] * tm32["stage_cells"] / max(launches, 1)
+    bpl = BYTES_PER_CELL_STAGE["fp32c16"] * tm32["stage_cells"] / max(launches, 1)
     achieved = bpl / (avg_ms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_ssprk3_complex512.json")
@@ -150,7 +159,7 @@ def main():
     result["roofline"] = {"bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                          "kernel": "shm::stage_kernel<ComplexScalar, PolSsprk, F32C>",
+                          "kernel": "shm::stage_kernel<ComplexScalar, PolSsprk<1..3>, F32C16>",
                           "avg_launch_ms": avg_ms,
                           "algorithmic_bytes_per_launch": bpl}
     result["halo_ms_per_step"] = tm32["halo_ms"] / a.steps
@@ -173,6 +182,23 @@ def main():
                           "roofline_frac": b64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS}
         result["speedup_fp32c_vs_fp64"] = rate32 / rate64
         result["fp32c_rel_err_vs_fp64"] = err
+        simc, secc, tmc = run_gpu(shm, inp, shm.PREC_FP32C, a.steps, a.warmup,
+                                  a.seed)
+        lc = tmc["stage_launches"]
+        bc = BYTES_PER_CELL_STAGE["fp32c"] * tmc["stage_cells"] / max(lc, 1)
+        errc = 0.0
+        for c in range(4):
+            xc = simc.download(0, c)
+            x64 = sim64.download(0, c)
+            errc = max(errc, float(np.max(np.abs(xc - x64)) /
+                                   max(np.max(np.abs(x64)), 1e-300)))
+            del xc, x64
+        result["fp32c"] = {"value": cells * stages * a.steps / secc,
+                           "roofline_frac": bc / (tmc["stage_ms"] / max(lc, 1) * 1e-3)
+                           / 1e9 / HBM_PEAK_GBS,
+                           "rel_err_vs_fp64": errc,
+                           "remainder": "fp32"}
+        simc.close()
         sim64.close()
     sim32.close()
 

@@ -37,7 +37,8 @@ enum shm_status {
 enum shm_precision {
     SHM_PREC_INPUTS = -1,     /* take shm.precision from the inputs file */
     SHM_PREC_FP64 = 0,        /* FP64 storage and arithmetic */
-    SHM_PREC_FP32C = 1        /* FP32 storage + FP32 remainder (compensated) */
+    SHM_PREC_FP32C = 1,       /* FP32 storage + FP32 remainder (compensated) */
+    SHM_PREC_FP32C16 = 2      /* FP32 storage + bf16 remainder (compensated) */
 };
 
 enum shm_te_flags {

@@ -12,13 +12,14 @@ import os
 import numpy as np
 
 __all__ = ["Simulation", "library_path", "PROJECTS", "ShmError",
-           "PREC_FP64", "PREC_FP32C", "TE_COMPUTE", "TE_AVGDOWN", "TE_TAGS"]
+           "PREC_FP64", "PREC_FP32C", "PREC_FP32C16", "TE_COMPUTE",
+           "TE_AVGDOWN", "TE_TAGS"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(_HERE, "lib")
 PROJECTS = ("RealScalar", "ComplexScalar", "PhaseTransitionGW")
 
-PREC_INPUTS, PREC_FP64, PREC_FP32C = -1, 0, 1
+PREC_INPUTS, PREC_FP64, PREC_FP32C, PREC_FP32C16 = -1, 0, 1, 2
 TE_COMPUTE, TE_AVGDOWN, TE_TAGS = 1, 2, 4
 
 # Exported symbols declared in include/shm.h (checked by the CPU tests).

@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) te_tag_kernel(const TeTagArgs a) {
                     crse = static_cast<const double *>(a.crse)[cen];
                 double avg;
                 const double te = te_cell(crse, sum, &avg);
-                if (a.flags & kTeAvgDown) store_state_at<S>(a.crse, a.crselo, cen, avg);
+                if (a.flags & kTeAvgDown) store_state_at<PrecTraits<P>>(a.crse, a.crselo, cen, avg);
                 if (a.te_out) a.te_out[(long long)n * a.total + t] = te;
                 const double m = Ops::template te_modifier<n>(
                     fv, ii, jj, kk, a.lev, a.time, a.dt, a.dx, te, a.params_mod);
@@ -294,11 +294,11 @@ __global__ void __launch_bounds__(256) avgdown_kernel(const AvgArgs a) {
             auto f = [&](int x, int y, int z) {
                 const long long e = off + x + (long long)y * fb.jstride +
                                     (long long)z * fb.kstride;
-                return load_state_at<S>(a.fine, a.finelo, e);
+                return load_state_at<PrecTraits<P>>(a.fine, a.finelo, e);
             };
             const double sum = child_sum(f, 0, 0, 0);
-            store_state_at<S>(a.crse, a.crselo, (long long)n * a.ccstride + ce,
-                              0.125 * sum);
+            store_state_at<PrecTraits<P>>(a.crse, a.crselo,
+                                          (long long)n * a.ccstride + ce, 0.125 * sum);
         }
     }
 }
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(256) pack_kernel(const PackArgs a) {
         const long long o = (long long)(a.box.lo[0] + x) +
                             (long long)a.N * ((a.box.lo[1] + y) +
                                               (long long)a.N * (a.box.lo[2] + z));
-        a.out[o] = load_state_at<S>(a.arr, a.arrlo, e);
+        a.out[o] = load_state_at<PrecTraits<P>>(a.arr, a.arrlo, e);
     }
 }
 
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(256) init_kernel(const InitArgs a) {
             v[1] = m * sin(theta);
         }
         for (int n = 0; n < a.ncomp && n < 32; ++n)
-            store_state_at<S>(a.arr, a.arrlo, (long long)n * a.cstride + e, v[n]);
+            store_state_at<PrecTraits<P>>(a.arr, a.arrlo, (long long)n * a.cstride + e, v[n]);
     }
 }
 

@@ -35,6 +35,9 @@ void fill_table_p0_r3(KernelTable &kt);
 void fill_table_p1_r1(KernelTable &kt);
 void fill_table_p1_r2(KernelTable &kt);
 void fill_table_p1_r3(KernelTable &kt);
+void fill_table_p2_r1(KernelTable &kt);
+void fill_table_p2_r2(KernelTable &kt);
+void fill_table_p2_r3(KernelTable &kt);
 
 template <class Ops>
 const KernelTable &project_kernel_table() {
@@ -49,14 +52,17 @@ const KernelTable &project_kernel_table() {
 #if SHM_R_MASK & 2
         fill_table_p0_r1(kt);
         fill_table_p1_r1(kt);
+        fill_table_p2_r1(kt);
 #endif
 #if SHM_R_MASK & 4
         fill_table_p0_r2(kt);
         fill_table_p1_r2(kt);
+        fill_table_p2_r2(kt);
 #endif
 #if SHM_R_MASK & 8
         fill_table_p0_r3(kt);
         fill_table_p1_r3(kt);
+        fill_table_p2_r3(kt);
 #endif
         fill_table_aux(kt);
         return kt;
@@ -66,11 +72,11 @@ const KernelTable &project_kernel_table() {
 
 template <int K>
 struct KindPol;
-template <> struct KindPol<kSsprk1> { using T = PolSsprk<kWriteK | kXisU0>; };
-template <> struct KindPol<kSsprk2> { using T = PolSsprk<kHasKin | kWriteK>; };
-template <> struct KindPol<kSsprk3> { using T = PolSsprk<kHasKin | kFinal>; };
-template <> struct KindPol<kFillRhs> { using T = PolSsprk<kFillOnly>; };
-template <> struct KindPol<kFillAddRhs> { using T = PolSsprk<kFillOnly | kHasKin>; };
+template <> struct KindPol<kSsprk1> { using T = PolSsprk<kSsStage1>; };
+template <> struct KindPol<kSsprk2> { using T = PolSsprk<kSsStage2>; };
+template <> struct KindPol<kSsprk3> { using T = PolSsprk<kSsStage3>; };
+template <> struct KindPol<kFillRhs> { using T = PolSsprk<kSsFill>; };
+template <> struct KindPol<kFillAddRhs> { using T = PolSsprk<kSsFillAdd>; };
 template <> struct KindPol<kRkn0> { using T = PolRkn<0, false>; };
 template <> struct KindPol<kRkn1> { using T = PolRkn<1, false>; };
 template <> struct KindPol<kRkn2> { using T = PolRkn<2, false>; };
@@ -197,11 +203,13 @@ void fill_table_aux(KernelTable &kt) {
     if constexpr (SHM_GW_MODES & 1) {
         kt.te_tag[0][0] = &launch_te_tag<ShmProjectOps, Prec::F64, false>;
         kt.te_tag[1][0] = &launch_te_tag<ShmProjectOps, Prec::F32C, false>;
+        kt.te_tag[2][0] = &launch_te_tag<ShmProjectOps, Prec::F32C16, false>;
         kt.probe[0] = &launch_probe<ShmProjectOps, false>;
     }
     if constexpr (SHM_GW_MODES & 2) {
         kt.te_tag[0][1] = &launch_te_tag<ShmProjectOps, Prec::F64, true>;
         kt.te_tag[1][1] = &launch_te_tag<ShmProjectOps, Prec::F32C, true>;
+        kt.te_tag[2][1] = &launch_te_tag<ShmProjectOps, Prec::F32C16, true>;
         kt.probe[1] = &launch_probe<ShmProjectOps, true>;
     }
 }

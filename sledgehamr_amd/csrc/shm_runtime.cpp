@@ -271,6 +271,19 @@ static std::vector<HaloSeg> copy_plan(const Layout &dst, const Layout &src,
     return segs;
 }
 
+/** Box-to-box copy of ncomp components of element size es (8, 4 or 2). */
+static void launch_copy(const CopyArgs &c, size_t es, int ncu, hipStream_t s) {
+    if (c.total == 0) return;
+    const int grid = (int)std::min<long long>((c.total + 255) / 256, 8LL * ncu);
+    if (es == 8)
+        hipLaunchKernelGGL(copy_kernel<double>, dim3(grid), dim3(256), 0, s, c);
+    else if (es == 4)
+        hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256), 0, s, c);
+    else
+        hipLaunchKernelGGL(copy_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, c);
+    SHM_HIP(hipGetLastError());
+}
+
 static std::vector<Box3> coarsen(const std::vector<Box3> &b) {
     std::vector<Box3> c;
     for (const Box3 &x : b) {
@@ -498,11 +511,20 @@ void Runtime::init_from_inputs(const std::string &inputs,
         throw Error(4, "leapfrog with Kreiss-Oliger dissipation is not supported");
 
     std::string prec;
-    prec_ = Prec::F32C;
-    if (pp.query("shm.precision", prec))
-        prec_ = (prec == "fp64" || prec == "FP64") ? Prec::F64 : Prec::F32C;
+    prec_ = Prec::F32C16;
+    if (pp.query("shm.precision", prec)) {
+        if (prec == "fp64" || prec == "FP64")
+            prec_ = Prec::F64;
+        else if (prec == "fp32c" || prec == "FP32C")
+            prec_ = Prec::F32C;
+        else if (prec == "fp32c16" || prec == "FP32C16")
+            prec_ = Prec::F32C16;
+        else
+            throw Error(1, "shm.precision must be fp64, fp32c or fp32c16");
+    }
     if (precision_override == 0) prec_ = Prec::F64;
     if (precision_override == 1) prec_ = Prec::F32C;
+    if (precision_override == 2) prec_ = Prec::F32C16;
     pp.query("shm.tile_variant", variant);
     pp.query("shm.tile_z", tz);
     pp.query("shm.xcd_split", xcd_split);
@@ -606,12 +628,11 @@ void Runtime::ensure_arrays(LevelState &ls, int integrator) {
     const int npair = kt_.nfields + (gw_ ? 6 : 0);
     for (int b = 0; b < 2; ++b) {
         if (!ls.U[b]) ls.U[b].alloc(plane * nc * es);
-        if (prec_ == Prec::F32C && !ls.Ulo[b]) ls.Ulo[b].alloc(plane * nc * 4);
+        if (comp() && !ls.Ulo[b]) ls.Ulo[b].alloc(plane * nc * losize());
     }
     if (integrator == 10) {
-        if (!ls.S1) ls.S1.alloc(plane * nc * es);
-        if (!ls.S2) ls.S2.alloc(plane * nc * es);
-        if (!ls.K) ls.K.alloc(plane * nc * es);
+        if (!ls.K[0]) ls.K[0].alloc(plane * nc * es);
+        if (!ls.K[1]) ls.K[1].alloc(plane * nc * es);
     } else if (integrator >= 20) {
         for (size_t i = 0; i + 1 < nodes_.size(); ++i)
             if (!ls.F[i]) ls.F[i].alloc(plane * npair * es);
@@ -770,43 +791,41 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
     const int npair = kt_.nfields + (gw_ ? 6 : 0);
     std::vector<int> pairs(npair);
     for (int p = 0; p < npair; ++p) pairs[p] = p;
-    void *ulo_old = prec_ == Prec::F32C ? ls.Ulo[old].get() : nullptr;
-    void *ulo_new = prec_ == Prec::F32C ? ls.Ulo[nw].get() : nullptr;
+    void *ulo_old = comp() ? ls.Ulo[old].get() : nullptr;
+    void *ulo_new = comp() ? ls.Ulo[nw].get() : nullptr;
     ls.ghosts_ok[nw] = false;
 
     if (integrator_ == 10) {
+        // Low-storage SSPRK3 with the stage states rebuilt from u0 and k:
+        // k1 -> K[0], k2 -> K[1] (k is read at neighbours, so not in place).
         const double t1 = t0 + dt;
         StageArgs a{};
         a.prm = prm(t0);
         a.x = ls.U[old].get();
-        a.kout = ls.K.get();
-        a.y = ls.S1.get();
-        a.c[1] = dt;
+        a.kout = ls.K[0].get();
         launch_stage(kSsprk1, ls, a, lev);
-        halo(ls, ls.S1.get(), stencil_comps());
+        halo(ls, ls.K[0].get(), stencil_comps());
 
         StageArgs b{};
         b.prm = prm(t1);
-        b.x = ls.S1.get();
         b.u0 = ls.U[old].get();
-        b.kin = ls.K.get();
-        b.kout = ls.K.get();
-        b.y = ls.S2.get();
+        b.kin = ls.K[0].get();
+        b.kout = ls.K[1].get();
         b.c[0] = 1.;
-        b.c[1] = dt / 4.;
+        b.c[2] = dt;
         launch_stage(kSsprk2, ls, b, lev);
-        halo(ls, ls.S2.get(), stencil_comps());
+        halo(ls, ls.K[1].get(), stencil_comps());
 
         StageArgs c{};
         c.prm = prm(t0 + dt / 2.);
-        c.x = ls.S2.get();
         c.u0 = ls.U[old].get();
         c.u0lo = ulo_old;
-        c.kin = ls.K.get();
+        c.kin = ls.K[1].get();
         c.unew = ls.U[nw].get();
         c.unewlo = ulo_new;
         c.c[0] = 0.25;
         c.c[1] = dt * 2. / 3.;
+        c.c[2] = dt / 4.;
         launch_stage(kSsprk3, ls, c, lev);
     } else if (integrator_ >= 20) {
         const int s = (int)nodes_.size();
@@ -907,20 +926,28 @@ void Runtime::avg_down(LevelState &fine, int fw, LevelState &crse, int cw) {
     a.fcstride = fine.L.cstride;
     a.ccstride = crse.L.cstride;
     a.fine = fine.U[fw].get();
-    a.finelo = prec_ == Prec::F32C ? fine.Ulo[fw].get() : nullptr;
+    a.finelo = comp() ? fine.Ulo[fw].get() : nullptr;
     a.crse = crse.U[cw].get();
-    a.crselo = prec_ == Prec::F32C ? crse.Ulo[cw].get() : nullptr;
+    a.crselo = comp() ? crse.Ulo[cw].get() : nullptr;
     a.cprefix = fine.L.dcprefix.as<long long>();
     a.nbox = (int)fine.L.boxes.size();
     a.total = fine.L.cprefix.back();
     a.ncomp = ncomp_;
     const int grid = (int)std::max<long long>(1, std::min<long long>((a.total + 255) / 256, 8LL * ncu_));
-    if (prec_ == Prec::F64)
-        hipLaunchKernelGGL(avgdown_kernel<Prec::F64>, dim3(grid), dim3(256), 0,
-                           stream_, a);
-    else
-        hipLaunchKernelGGL(avgdown_kernel<Prec::F32C>, dim3(grid), dim3(256), 0,
-                           stream_, a);
+    switch (prec_) {
+        case Prec::F64:
+            hipLaunchKernelGGL(avgdown_kernel<Prec::F64>, dim3(grid), dim3(256),
+                               0, stream_, a);
+            break;
+        case Prec::F32C:
+            hipLaunchKernelGGL(avgdown_kernel<Prec::F32C>, dim3(grid), dim3(256),
+                               0, stream_, a);
+            break;
+        case Prec::F32C16:
+            hipLaunchKernelGGL(avgdown_kernel<Prec::F32C16>, dim3(grid),
+                               dim3(256), 0, stream_, a);
+            break;
+    }
     SHM_HIP(hipGetLastError());
     crse.ghosts_ok[cw] = false;
 }
@@ -977,8 +1004,8 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
             mirror.L.build(coarsen(fine.L.boxes), 0, sim_->dimN[lev - 1], true);
             mirror.ncomp = ncomp_;
             mirror.U[0].alloc((size_t)mirror.L.cstride * ncomp_ * esize());
-            if (prec_ == Prec::F32C)
-                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * 4);
+            if (comp())
+                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * losize());
             mirror.cur = 0;
             plan_in = copy_plan(mirror.L, cl.L, &tot_in);
             plan_out = copy_plan(cl.L, mirror.L, &tot_out);
@@ -999,20 +1026,13 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
                 c.nseg = nseg;
                 c.total = total;
                 c.ncomp = ncomp_;
-                const int grid = (int)std::min<long long>((total + 255) / 256, 8LL * ncu_);
-                if (es == 8)
-                    hipLaunchKernelGGL(copy_kernel<double>, dim3(grid), dim3(256),
-                                       0, stream_, c);
-                else
-                    hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256),
-                                       0, stream_, c);
-                SHM_HIP(hipGetLastError());
+                launch_copy(c, es, ncu_, stream_);
             };
             run_copy(mirror.U[0].get(), cl.U[cl.cur].get(), mirror.L, cl.L,
                      dplan_in, (int)plan_in.size(), tot_in, esize());
-            if (prec_ == Prec::F32C)
+            if (comp())
                 run_copy(mirror.Ulo[0].get(), cl.Ulo[cl.cur].get(), mirror.L,
-                         cl.L, dplan_in, (int)plan_in.size(), tot_in, 4);
+                         cl.L, dplan_in, (int)plan_in.size(), tot_in, losize());
             crse = &mirror;
         }
     }
@@ -1042,7 +1062,7 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     a.ccstride = crse ? crse->L.cstride : 0;
     a.fine = fine.U[fine.cur].get();
     a.crse = crse ? crse->U[crse->cur].get() : nullptr;
-    a.crselo = (crse && prec_ == Prec::F32C) ? crse->Ulo[crse->cur].get() : nullptr;
+    a.crselo = (crse && comp()) ? crse->Ulo[crse->cur].get() : nullptr;
     a.cprefix = fine.L.dcprefix.as<long long>();
     a.mprefix = fine.L.dvprefix.as<long long>();
     a.nbox = (int)fine.L.boxes.size();
@@ -1078,19 +1098,12 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
         c.nseg = (int)plan_out.size();
         c.total = tot_out;
         c.ncomp = ncomp_;
-        const int grid = (int)std::min<long long>((tot_out + 255) / 256, 8LL * ncu_);
-        if (prec_ == Prec::F64) {
-            hipLaunchKernelGGL(copy_kernel<double>, dim3(grid), dim3(256), 0,
-                               stream_, c);
-        } else {
-            hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256), 0,
-                               stream_, c);
+        launch_copy(c, esize(), ncu_, stream_);
+        if (comp()) {
             c.dst = cl.Ulo[cl.cur].get();
             c.src = mirror.Ulo[0].get();
-            hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256), 0,
-                               stream_, c);
+            launch_copy(c, losize(), ncu_, stream_);
         }
-        SHM_HIP(hipGetLastError());
         cl.ghosts_ok[cl.cur] = false;
     }
     if (mask)
@@ -1111,88 +1124,110 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
 // Data movement and initial states.
 // ---------------------------------------------------------------------------
 
-void Runtime::upload(int lev, int comp, const double *host) {
+/** Host-side split of an FP64 value into the stored hi and remainder. */
+static float host_bf16_round(float r) {
+    uint32_t b;
+    std::memcpy(&b, &r, 4);
+    b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+    float o;
+    std::memcpy(&o, &b, 4);
+    return o;
+}
+
+void Runtime::upload(int lev, int comp_id, const double *host) {
     if (lev == -1) define_shadow();
     LevelState &ls = level(lev);
-    if (comp < 0 || comp >= ncomp_) throw Error(1, "component out of range");
+    if (comp_id < 0 || comp_id >= ncomp_) throw Error(1, "component out of range");
     const int w = ls.cur;
+    const size_t es = esize(), ls_ = losize();
     for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
         const Box3 &bx = ls.L.boxes[b];
         const BoxDesc &bd = ls.L.hbox[b];
         const long long nb = bx.vol();
         const double *src = host + ls.L.vprefix[b];
-        // Rows are contiguous in x; copy row by row through a staging buffer.
-        std::vector<char> hi((size_t)nb * esize()), lo;
-        if (prec_ == Prec::F32C) lo.resize((size_t)nb * 4);
+        std::vector<char> hi((size_t)nb * es), lo;
+        if (comp()) lo.resize((size_t)nb * ls_);
         for (long long i = 0; i < nb; ++i) {
             if (prec_ == Prec::F64) {
                 reinterpret_cast<double *>(hi.data())[i] = src[i];
             } else {
                 const float h = (float)src[i];
                 reinterpret_cast<float *>(hi.data())[i] = h;
-                reinterpret_cast<float *>(lo.data())[i] = (float)(src[i] - (double)h);
+                const float r = (float)(src[i] - (double)h);
+                if (prec_ == Prec::F32C) {
+                    reinterpret_cast<float *>(lo.data())[i] = r;
+                } else {
+                    const float q = host_bf16_round(r);
+                    uint32_t u;
+                    std::memcpy(&u, &q, 4);
+                    reinterpret_cast<uint16_t *>(lo.data())[i] = (uint16_t)(u >> 16);
+                }
             }
         }
         const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
-        const size_t es = esize();
         // Rows of consecutive planes are not equally spaced (ghost rows in
         // between), so copy plane by plane.
         for (int z = 0; z < nz; ++z) {
-            char *base = static_cast<char *>(ls.U[w].get()) +
-                         ((size_t)comp * ls.L.cstride + bd.base +
-                          (size_t)z * bd.kstride) * es;
-            SHM_HIP(hipMemcpy2D(base, (size_t)bd.jstride * es,
+            const size_t e0 = (size_t)comp_id * ls.L.cstride + bd.base +
+                              (size_t)z * bd.kstride;
+            SHM_HIP(hipMemcpy2D(static_cast<char *>(ls.U[w].get()) + e0 * es,
+                                (size_t)bd.jstride * es,
                                 hi.data() + (size_t)z * nx * ny * es, (size_t)nx * es,
                                 (size_t)nx * es, ny, hipMemcpyHostToDevice));
-            if (prec_ == Prec::F32C) {
-                char *bl = static_cast<char *>(ls.Ulo[w].get()) +
-                           ((size_t)comp * ls.L.cstride + bd.base +
-                            (size_t)z * bd.kstride) * 4;
-                SHM_HIP(hipMemcpy2D(bl, (size_t)bd.jstride * 4,
-                                    lo.data() + (size_t)z * nx * ny * 4,
-                                    (size_t)nx * 4, (size_t)nx * 4, ny,
+            if (comp())
+                SHM_HIP(hipMemcpy2D(static_cast<char *>(ls.Ulo[w].get()) + e0 * ls_,
+                                    (size_t)bd.jstride * ls_,
+                                    lo.data() + (size_t)z * nx * ny * ls_,
+                                    (size_t)nx * ls_, (size_t)nx * ls_, ny,
                                     hipMemcpyHostToDevice));
-            }
         }
     }
     ls.ghosts_ok[w] = false;
 }
 
-void Runtime::download(int lev, int comp, double *host, bool old_state) {
+void Runtime::download(int lev, int comp_id, double *host, bool old_state) {
     LevelState &ls = level(lev);
-    if (comp < 0 || comp >= ncomp_) throw Error(1, "component out of range");
+    if (comp_id < 0 || comp_id >= ncomp_) throw Error(1, "component out of range");
     sync();
     const int w = old_state ? (ls.cur ^ 1) : ls.cur;
-    const size_t es = esize();
+    const size_t es = esize(), ls_ = losize();
     for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
         const Box3 &bx = ls.L.boxes[b];
         const BoxDesc &bd = ls.L.hbox[b];
         const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
         std::vector<char> hi((size_t)bx.vol() * es), lo;
-        if (prec_ == Prec::F32C) lo.resize((size_t)bx.vol() * 4);
+        if (comp()) lo.resize((size_t)bx.vol() * ls_);
         for (int z = 0; z < nz; ++z) {
-            const char *base = static_cast<const char *>(ls.U[w].get()) +
-                               ((size_t)comp * ls.L.cstride + bd.base +
-                                (size_t)z * bd.kstride) * es;
+            const size_t e0 = (size_t)comp_id * ls.L.cstride + bd.base +
+                              (size_t)z * bd.kstride;
             SHM_HIP(hipMemcpy2D(hi.data() + (size_t)z * nx * ny * es,
-                                (size_t)nx * es, base, (size_t)bd.jstride * es,
-                                (size_t)nx * es, ny, hipMemcpyDeviceToHost));
-            if (prec_ == Prec::F32C) {
-                const char *bl = static_cast<const char *>(ls.Ulo[w].get()) +
-                                 ((size_t)comp * ls.L.cstride + bd.base +
-                                  (size_t)z * bd.kstride) * 4;
-                SHM_HIP(hipMemcpy2D(lo.data() + (size_t)z * nx * ny * 4,
-                                    (size_t)nx * 4, bl, (size_t)bd.jstride * 4,
-                                    (size_t)nx * 4, ny, hipMemcpyDeviceToHost));
-            }
+                                (size_t)nx * es,
+                                static_cast<const char *>(ls.U[w].get()) + e0 * es,
+                                (size_t)bd.jstride * es, (size_t)nx * es, ny,
+                                hipMemcpyDeviceToHost));
+            if (comp())
+                SHM_HIP(hipMemcpy2D(lo.data() + (size_t)z * nx * ny * ls_,
+                                    (size_t)nx * ls_,
+                                    static_cast<const char *>(ls.Ulo[w].get()) + e0 * ls_,
+                                    (size_t)bd.jstride * ls_, (size_t)nx * ls_, ny,
+                                    hipMemcpyDeviceToHost));
         }
         double *dst = host + ls.L.vprefix[b];
         for (long long i = 0; i < bx.vol(); ++i) {
-            if (prec_ == Prec::F64)
+            if (prec_ == Prec::F64) {
                 dst[i] = reinterpret_cast<const double *>(hi.data())[i];
-            else
-                dst[i] = (double)reinterpret_cast<const float *>(hi.data())[i] +
-                         (double)reinterpret_cast<const float *>(lo.data())[i];
+            } else {
+                double v = (double)reinterpret_cast<const float *>(hi.data())[i];
+                if (prec_ == Prec::F32C) {
+                    v += (double)reinterpret_cast<const float *>(lo.data())[i];
+                } else {
+                    const uint32_t u = (uint32_t)reinterpret_cast<const uint16_t *>(lo.data())[i] << 16;
+                    float f;
+                    std::memcpy(&f, &u, 4);
+                    v += (double)f;
+                }
+                dst[i] = v;
+            }
         }
     }
 }
@@ -1228,7 +1263,7 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
     LevelState &ls = level(lev);
     InitArgs a{};
     a.arr = ls.U[ls.cur].get();
-    a.arrlo = prec_ == Prec::F32C ? ls.Ulo[ls.cur].get() : nullptr;
+    a.arrlo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
     a.cstride = ls.L.cstride;
     a.boxes = ls.L.dbox.as<BoxDesc>();
     a.vprefix = ls.L.dvprefix.as<long long>();
@@ -1242,12 +1277,20 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
     for (int i = 0; i < np && i < 8; ++i) a.p[i] = p[i];
     if (ncomp_ > 32) throw Error(4, "initial-state generator supports <= 32 components");
     const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
-    if (prec_ == Prec::F64)
-        hipLaunchKernelGGL(init_kernel<Prec::F64>, dim3(grid), dim3(256), 0,
-                           stream_, a);
-    else
-        hipLaunchKernelGGL(init_kernel<Prec::F32C>, dim3(grid), dim3(256), 0,
-                           stream_, a);
+    switch (prec_) {
+        case Prec::F64:
+            hipLaunchKernelGGL(init_kernel<Prec::F64>, dim3(grid), dim3(256), 0,
+                               stream_, a);
+            break;
+        case Prec::F32C:
+            hipLaunchKernelGGL(init_kernel<Prec::F32C>, dim3(grid), dim3(256), 0,
+                               stream_, a);
+            break;
+        case Prec::F32C16:
+            hipLaunchKernelGGL(init_kernel<Prec::F32C16>, dim3(grid), dim3(256),
+                               0, stream_, a);
+            break;
+    }
     SHM_HIP(hipGetLastError());
     ls.ghosts_ok[ls.cur] = false;
 }
@@ -1305,9 +1348,7 @@ void Runtime::autotune(int lev) {
     a.prm = prm;
     if (pol == kSsprk1) {
         a.x = ls.U[old].get();
-        a.kout = ls.K.get();
-        a.y = ls.S1.get();
-        a.c[1] = dt;
+        a.kout = ls.K[0].get();
     } else if (pol == kRkn0) {
         if (nodes_.size() < 2) return;
         a.u0 = ls.U[old].get();
@@ -1439,7 +1480,7 @@ void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
         for (size_t b = 0; b < l0.L.boxes.size(); ++b) {
             PackArgs p{};
             p.arr = l0.U[l0.cur].get();
-            p.arrlo = prec_ == Prec::F32C ? l0.Ulo[l0.cur].get() : nullptr;
+            p.arrlo = comp() ? l0.Ulo[l0.cur].get() : nullptr;
             p.cstride = l0.L.cstride;
             p.box = l0.L.hbox[b];
             p.comp = comps[c];
@@ -1447,12 +1488,20 @@ void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
             p.N = N;
             const long long nb = l0.L.boxes[b].vol();
             const int grid = (int)std::min<long long>((nb + 255) / 256, 8LL * ncu_);
-            if (prec_ == Prec::F64)
-                hipLaunchKernelGGL(pack_kernel<Prec::F64>, dim3(grid), dim3(256),
-                                   0, stream_, p);
-            else
-                hipLaunchKernelGGL(pack_kernel<Prec::F32C>, dim3(grid), dim3(256),
-                                   0, stream_, p);
+            switch (prec_) {
+                case Prec::F64:
+                    hipLaunchKernelGGL(pack_kernel<Prec::F64>, dim3(grid),
+                                       dim3(256), 0, stream_, p);
+                    break;
+                case Prec::F32C:
+                    hipLaunchKernelGGL(pack_kernel<Prec::F32C>, dim3(grid),
+                                       dim3(256), 0, stream_, p);
+                    break;
+                case Prec::F32C16:
+                    hipLaunchKernelGGL(pack_kernel<Prec::F32C16>, dim3(grid),
+                                       dim3(256), 0, stream_, p);
+                    break;
+            }
             SHM_HIP(hipGetLastError());
         }
         if (hipfftExecD2Z(plan, dense.as<hipfftDoubleReal>(),
@@ -1559,8 +1608,8 @@ void Runtime::synchronize_levels(int lev) {
             mirror.L.build(coarsen(fine.L.boxes), 0, S.dimN[lev], true);
             mirror.ncomp = ncomp_;
             mirror.U[0].alloc((size_t)mirror.L.cstride * ncomp_ * esize());
-            if (prec_ == Prec::F32C)
-                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * 4);
+            if (comp())
+                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * losize());
             avg_down(fine, fine.cur, mirror, 0);
             long long tot = 0;
             std::vector<HaloSeg> plan = copy_plan(ls.L, mirror.L, &tot);
@@ -1578,19 +1627,12 @@ void Runtime::synchronize_levels(int lev) {
                 c.nseg = (int)plan.size();
                 c.total = tot;
                 c.ncomp = ncomp_;
-                const int grid = (int)std::min<long long>((tot + 255) / 256, 8LL * ncu_);
-                if (prec_ == Prec::F64) {
-                    hipLaunchKernelGGL(copy_kernel<double>, dim3(grid), dim3(256),
-                                       0, stream_, c);
-                } else {
-                    hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256),
-                                       0, stream_, c);
+                launch_copy(c, esize(), ncu_, stream_);
+                if (comp()) {
                     c.dst = ls.Ulo[ls.cur].get();
                     c.src = mirror.Ulo[0].get();
-                    hipLaunchKernelGGL(copy_kernel<float>, dim3(grid), dim3(256),
-                                       0, stream_, c);
+                    launch_copy(c, losize(), ncu_, stream_);
                 }
-                SHM_HIP(hipGetLastError());
             }
             sync();
             ls.ghosts_ok[ls.cur] = false;

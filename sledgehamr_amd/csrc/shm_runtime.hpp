@@ -92,7 +92,7 @@ struct LevelState {
     DevMem U[2], Ulo[2];
     int cur = 0;               // index of grid_new in U
     bool ghosts_ok[2] = {false, false};
-    DevMem S1, S2, K;          // SSPRK3 stage states and k
+    DevMem K[2];               // SSPRK3 k accumulators (k1, k2)
     DevMem F[5];               // RKN forces
     DevMem VH;                 // leapfrog half-step momenta (FP64)
     DevMem RHS;                // FillRhs target
@@ -167,6 +167,8 @@ class Runtime {
     void fill_params(double *blk, double time, double dt, double dx, int lev);
     void ensure_arrays(LevelState &ls, int integrator);
     size_t esize() const { return prec_ == Prec::F64 ? 8 : 4; }
+    size_t losize() const { return prec_ == Prec::F32C16 ? 2 : 4; }
+    bool comp() const { return prec_ != Prec::F64; }
     void launch_stage(int pol, LevelState &ls, StageArgs &a, int lev);
     void halo(LevelState &ls, void *arr, const std::vector<int> &comps,
               bool full = false);

@@ -11,8 +11,8 @@
 // staged in LDS (double buffered, one barrier per plane).  The project's
 // Rhs() reads the tile through amrex::Array4<const double> in tile mode and
 // writes into a register sink; point-wise components (momenta) are read at
-// the centre only.  Plane z+R+1, its halo ring and the point-wise values of
-// plane z+1 are prefetched one plane ahead.
+// the centre only.  Plane z+R+1, its halo ring, the point-wise values and the
+// store-time inputs of plane z+1 are prefetched one plane ahead.
 //
 // Reference behaviour reproduced: the Rhs cell loop with the optional GW
 // source/backreaction and Kreiss-Oliger term (source/macros.h:301-485), and
@@ -22,24 +22,49 @@
 
 #include <sledgehamr.h>
 
+#include <cstdint>
 #include <type_traits>
 
 #include "shm_layout.hpp"
 
 namespace shm {
 
-enum class Prec : int { F64 = 0, F32C = 1 };
+/** Storage precision of a level's state.
+ *   F64    : FP64 state and stage arrays.
+ *   F32C   : FP32 state + FP32 remainder (u = hi + lo, ~48 significant bits).
+ *   F32C16 : FP32 state + bf16 remainder (~32 significant bits).
+ *  Stage arrays (k, RKN forces, stage states) are FP32 in both FP32 modes. */
+enum class Prec : int { F64 = 0, F32C = 1, F32C16 = 2 };
+constexpr int kNPrec = 3;
 
 template <Prec P>
 struct PrecTraits;
 template <>
 struct PrecTraits<Prec::F64> {
-    using S = double;  // stored state
+    using S = double;
+    using L = double;  // unused
+    static constexpr bool comp = false;
 };
 template <>
 struct PrecTraits<Prec::F32C> {
-    using S = float;  // stored state; a second float array holds the remainder
+    using S = float;
+    using L = float;
+    static constexpr bool comp = true;
 };
+template <>
+struct PrecTraits<Prec::F32C16> {
+    using S = float;
+    using L = uint16_t;  // bf16 bit pattern
+    static constexpr bool comp = true;
+};
+
+__device__ AMREX_FORCE_INLINE uint16_t f32_to_bf16_rne(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ AMREX_FORCE_INLINE float bf16_to_f32(uint16_t h) {
+    return __uint_as_float((uint32_t)h << 16);
+}
 
 /** Per-stage parameter record in device memory (written by the host before
  *  the stage runs, so a captured graph replays with fresh values). */
@@ -163,119 +188,141 @@ __device__ AMREX_FORCE_INLINE void st(void *p, long long base, unsigned off, T v
     __builtin_nontemporal_store(v, reinterpret_cast<T *>(b + (unsigned)(off * (unsigned)sizeof(T))));
 }
 
-/** Write a full-precision value as FP64, or as an FP32 pair hi + lo with
- *  hi = fl32(u) and lo = fl32(u - hi) (compensated storage). */
-template <class S>
-__device__ AMREX_FORCE_INLINE void store_state(void *hi, void *lo,
-                                               long long base, unsigned off,
-                                               double u) {
-    if constexpr (std::is_same_v<S, float>) {
+/** Split a full-precision value into the stored pair (or keep it FP64):
+ *  hi = fl32(u), lo = u - hi rounded to the remainder type. */
+template <class T>
+__device__ AMREX_FORCE_INLINE void store_state(void *hi, void *lo, long long base,
+                                               unsigned off, double u) {
+    using S = typename T::S;
+    if constexpr (T::comp) {
         const float h = (float)u;
         st<float>(hi, base, off, h);
-        st<float>(lo, base, off, (float)(u - (double)h));
+        const float r = (float)(u - (double)h);
+        if constexpr (std::is_same_v<typename T::L, float>)
+            st<float>(lo, base, off, r);
+        else
+            st<uint16_t>(lo, base, off, f32_to_bf16_rne(r));
     } else {
-        st<double>(hi, base, off, u);
+        st<S>(hi, base, off, (S)u);
     }
 }
-template <class S>
+/** Remainder of the stored pair as FP64 (0 without compensation). */
+template <class T>
+__device__ AMREX_FORCE_INLINE double load_lo(const void *lo, long long base,
+                                             unsigned off) {
+    if constexpr (!T::comp)
+        return 0.;
+    else if constexpr (std::is_same_v<typename T::L, float>)
+        return (double)ld<float>(lo, base, off);
+    else
+        return (double)bf16_to_f32(ld<uint16_t>(lo, base, off));
+}
+template <class T>
 __device__ AMREX_FORCE_INLINE double load_state(const void *hi, const void *lo,
                                                 long long base, unsigned off) {
-    if constexpr (std::is_same_v<S, float>)
-        return (double)ld<float>(hi, base, off) + (double)ld<float>(lo, base, off);
-    else
-        return ld<double>(hi, base, off);
+    return (double)ld<typename T::S>(hi, base, off) + load_lo<T>(lo, base, off);
 }
-
 /** Same, addressed by a 64-bit element index (auxiliary kernels). */
-template <class S>
+template <class T>
 __device__ AMREX_FORCE_INLINE void store_state_at(void *hi, void *lo,
                                                   long long e, double u) {
-    store_state<S>(hi, lo, e, 0u, u);
+    store_state<T>(hi, lo, e, 0u, u);
 }
-template <class S>
+template <class T>
 __device__ AMREX_FORCE_INLINE double load_state_at(const void *hi,
                                                    const void *lo, long long e) {
-    return load_state<S>(hi, lo, e, 0u);
+    return load_state<T>(hi, lo, e, 0u);
 }
 
-/** Low-storage SSPRK3 stage / FillRhs / FillAddRhs, flags fixed at compile
- *  time.  k = f(X) + c0 * k_in;  then either y = u0 + c1 * k, or at the
- *  final stage u_new = u0 + c1 * k with compensated storage, or (fill mode)
- *  rhs_out = k. */
-enum SsprkFlags : int { kHasKin = 1, kWriteK = 2, kFinal = 4, kXisU0 = 8, kFillOnly = 16 };
+/** Low-storage SSPRK3 (lsssprk3.cpp:14-31) and FillRhs / FillAddRhs.
+ *
+ *  The stage states y1 = u0 + dt k1 and y2 = u0 + dt/4 k2 are not stored:
+ *  stages 2 and 3 rebuild them at every stencil point from u0 and the k
+ *  accumulator (X = u0 + c2 * k_in), which removes one write and one read
+ *  per component and stage.  k_in and k_out are different arrays (k is read
+ *  at neighbouring cells while it is being rewritten).
+ *    stage 1: k1 = f(u0)                        -> k_out
+ *    stage 2: k2 = f(u0 + dt k1) + k1           -> k_out
+ *    stage 3: k3 = f(u0 + dt/4 k2) + k2/4;  u = u0 + 2dt/3 k3  (compensated)
+ *    fill   : rhs = f(X)  /  rhs = f(X) + w rhs
+ *  Coefficients: c0 weight of k_in, c1 update coefficient, c2 stage-state
+ *  coefficient. */
+enum SsprkKind : int { kSsStage1 = 1, kSsStage2 = 2, kSsStage3 = 3, kSsFill = 4, kSsFillAdd = 5 };
 
-template <int F>
+template <int K>
 struct PolSsprk {
-    template <class S>
-    using Ext = S;
-    // Per component: k_in, u0, u0 remainder -- whichever this stage reads.
-    template <class CS, class S>
-    static constexpr int per_comp() {
-        constexpr bool fill = (F & kFillOnly) != 0;
-        return ((F & kHasKin) ? 1 : 0) + ((!fill && !(F & kXisU0)) ? 1 : 0) +
-               ((!fill && (F & kFinal) && std::is_same_v<S, float>) ? 1 : 0);
+    static constexpr bool composed = (K == kSsStage2 || K == kSsStage3);
+    template <class T>
+    using Ext = typename T::S;
+    // Store-time inputs per component: stage 2: k_in; stage 3: k_in, u0,
+    // remainder; fill-add: previous rhs.
+    template <class CS, class T>
+    static constexpr int next() {
+        return CS::NC * (K == kSsStage2 ? 1
+                         : K == kSsStage3 ? (T::comp ? 3 : 2)
+                         : K == kSsFillAdd ? 1 : 0);
     }
-    template <class CS, class S>
-    static constexpr int next() { return CS::NC * per_comp<CS, S>(); }
-
-    template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
-                                                   int n, unsigned off) {
-        return ld<S>(a.x, org + n * a.cstride, off);
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        using S = typename T::S;
+        const long long b = org + n * a.cstride;
+        if constexpr (composed)
+            return (S)((double)ld<S>(a.u0, b, off) +
+                       a.c[2] * (double)ld<S>(a.kin, b, off));
+        else
+            return ld<S>(a.x, b, off);
     }
-    template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_pt(const StageArgs &a,
-                                                   long long org, int n,
-                                                   unsigned off) {
-        return ld<S>(a.x, org + n * a.cstride, off);
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        return load_st<CS, T>(a, org, n, off);
     }
-    template <class CS, class S>
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
                                                        long long org, unsigned off,
-                                                       S *e) {
-        constexpr bool fill = (F & kFillOnly) != 0;
+                                                       typename T::S *e) {
+        using S = typename T::S;
         int i = 0;
 #pragma unroll
         for (int n = 0; n < CS::NC; ++n) {
             const long long b = org + n * a.cstride;
-            if constexpr ((F & kHasKin) != 0) e[i++] = ld<S>(a.kin, b, off);
-            if constexpr (!fill && !(F & kXisU0)) e[i++] = ld<S>(a.u0, b, off);
-            if constexpr (!fill && (F & kFinal) && std::is_same_v<S, float>)
-                e[i++] = ld<float>(a.u0lo, b, off);
+            if constexpr (K == kSsStage2 || K == kSsStage3 || K == kSsFillAdd)
+                e[i++] = ld<S>(a.kin, b, off);
+            if constexpr (K == kSsStage3) {
+                e[i++] = ld<S>(a.u0, b, off);
+                if constexpr (T::comp) e[i++] = (S)load_lo<T>(a.u0lo, b, off);
+            }
         }
     }
-    template <class CS, class S>
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
-                                                    const double *ctr, const S *e) {
+                                                    const double *ctr,
+                                                    const typename T::S *e) {
+        using S = typename T::S;
         int i = 0;
 #pragma unroll
         for (int n = 0; n < CS::NC; ++n) {
             const long long b = org + n * a.cstride;
             double k = rhs[n];
-            if constexpr ((F & kHasKin) != 0) k += a.c[0] * (double)e[i++];
-            if constexpr ((F & kFillOnly) != 0) {
-                st<S>(a.kout, b, off, (S)k);
+            if constexpr (K == kSsStage2 || K == kSsStage3 || K == kSsFillAdd)
+                k += a.c[0] * (double)e[i++];
+            if constexpr (K == kSsStage3) {
+                double u0 = (double)e[i++];
+                if constexpr (T::comp) u0 += (double)e[i++];
+                store_state<T>(a.unew, a.unewlo, b, off, u0 + a.c[1] * k);
             } else {
-                if constexpr ((F & kWriteK) != 0) st<S>(a.kout, b, off, (S)k);
-                double u0;
-                if constexpr ((F & kXisU0) != 0)
-                    u0 = ctr[n];
-                else
-                    u0 = (double)e[i++];
-                if constexpr ((F & kFinal) == 0) {
-                    st<S>(a.y, b, off, (S)(u0 + a.c[1] * k));
-                } else {
-                    if constexpr (std::is_same_v<S, float>) u0 += (double)e[i++];
-                    store_state<S>(a.unew, a.unewlo, b, off, u0 + a.c[1] * k);
-                }
+                st<S>(a.kout, b, off, (S)k);
             }
         }
     }
 };
 
-/** Runge-Kutta-Nystroem stage with NP earlier stages (rkn.cpp:52-119).
+/** Runge-Kutta-Nystroem stage with NPREV earlier stages (rkn.cpp:52-119).
  *  Stage state of a field: y = phi0 + c0*pi0 + sum_j c[1+j] F_j; momenta
  *  stay at pi0.  Non-final stages write F (the momentum right-hand sides);
  *  the final stage writes phi = phi0 + c6*pi0 + sum_j c[7+j] F_j and
@@ -284,19 +331,19 @@ struct PolSsprk {
 template <int NPREV, bool FINAL>
 struct PolRkn {
     static_assert(NPREV >= 0 && NPREV <= 4, "RKN supports up to 5 stages");
-    template <class S>
-    using Ext = S;
+    template <class T>
+    using Ext = typename T::S;
     // Final stage, per pair: phi0 and pi0 (with remainders) and F_j.
-    template <class CS, class S>
+    template <class CS, class T>
     static constexpr int next() {
-        return FINAL ? CS::NPAIR * (2 * (std::is_same_v<S, float> ? 2 : 1) + NPREV)
-                     : 0;
+        return FINAL ? CS::NPAIR * (2 * (T::comp ? 2 : 1) + NPREV) : 0;
     }
 
-    template <class CS, class S>
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static double state(const StageArgs &a,
                                                       long long org, int n,
                                                       unsigned off) {
+        using S = typename T::S;
         const int p = CS::pair_of_field(n);
         if (p < 0) return (double)ld<S>(a.u0, org + n * a.cstride, off);
         double y = (double)ld<S>(a.u0, org + n * a.cstride, off);
@@ -306,21 +353,23 @@ struct PolRkn {
             y += a.c[1 + j] * (double)ld<S>(a.f[j], org + p * a.cstride, off);
         return y;
     }
-    template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
-                                                   int n, unsigned off) {
-        return (S)state<CS, S>(a, org, n, off);
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        return (typename T::S)state<CS, T>(a, org, n, off);
     }
-    template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_pt(const StageArgs &a,
-                                                   long long org, int n,
-                                                   unsigned off) {
-        return (S)state<CS, S>(a, org, n, off);
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        return (typename T::S)state<CS, T>(a, org, n, off);
     }
-    template <class CS, class S>
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
                                                        long long org, unsigned off,
-                                                       S *e) {
+                                                       typename T::S *e) {
+        using S = typename T::S;
         if constexpr (FINAL) {
             int i = 0;
 #pragma unroll
@@ -329,9 +378,9 @@ struct PolRkn {
                 const long long bm = org + CS::pair_mom(p) * a.cstride;
                 e[i++] = ld<S>(a.u0, bf, off);
                 e[i++] = ld<S>(a.u0, bm, off);
-                if constexpr (std::is_same_v<S, float>) {
-                    e[i++] = ld<float>(a.u0lo, bf, off);
-                    e[i++] = ld<float>(a.u0lo, bm, off);
+                if constexpr (T::comp) {
+                    e[i++] = (S)load_lo<T>(a.u0lo, bf, off);
+                    e[i++] = (S)load_lo<T>(a.u0lo, bm, off);
                 }
 #pragma unroll
                 for (int j = 0; j < NPREV; ++j)
@@ -339,11 +388,13 @@ struct PolRkn {
             }
         }
     }
-    template <class CS, class S>
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
-                                                    const double *ctr, const S *e) {
+                                                    const double *ctr,
+                                                    const typename T::S *e) {
+        using S = typename T::S;
         int i = 0;
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
@@ -355,7 +406,7 @@ struct PolRkn {
                 const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
                 double phi = (double)e[i++];
                 double pi0 = (double)e[i++];
-                if constexpr (std::is_same_v<S, float>) {
+                if constexpr (T::comp) {
                     phi += (double)e[i++];
                     pi0 += (double)e[i++];
                 }
@@ -369,8 +420,8 @@ struct PolRkn {
                 }
                 phi += a.c[11] * fi;
                 pi += a.c[16] * fi;
-                store_state<S>(a.unew, a.unewlo, bf, off, phi);
-                store_state<S>(a.unew, a.unewlo, bm, off, pi);
+                store_state<T>(a.unew, a.unewlo, bf, off, phi);
+                store_state<T>(a.unew, a.unewlo, bm, off, pi);
             }
         }
     }
@@ -382,24 +433,24 @@ struct PolRkn {
  *  exactly as the reference's second Rhs sees them): pi1 = vh + c0*a(pi). */
 template <bool KICK2>
 struct PolLeapfrog {
-    template <class S>
+    template <class T>
     using Ext = double;
-    // Kick 1 per pair: pi0 and phi0 (full precision); kick 2: vh.
-    template <class CS, class S>
+    template <class CS, class T>
     static constexpr int next() { return CS::NPAIR * (KICK2 ? 1 : 2); }
 
-    template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_st(const StageArgs &a, long long org,
-                                                   int n, unsigned off) {
-        return ld<S>(a.x, org + n * a.cstride, off);
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        return ld<typename T::S>(a.x, org + n * a.cstride, off);
     }
-    template <class CS, class S>
-    __device__ AMREX_FORCE_INLINE static S load_pt(const StageArgs &a,
-                                                   long long org, int n,
-                                                   unsigned off) {
-        return ld<S>(a.x, org + n * a.cstride, off);
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        return ld<typename T::S>(a.x, org + n * a.cstride, off);
     }
-    template <class CS, class S>
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
                                                        long long org, unsigned off,
                                                        double *e) {
@@ -407,14 +458,14 @@ struct PolLeapfrog {
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
             if constexpr (!KICK2) {
-                e[i++] = load_state<S>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
-                e[i++] = load_state<S>(a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
+                e[i++] = load_state<T>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+                e[i++] = load_state<T>(a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
             } else {
                 e[i++] = ld<double>(a.vh, org + p * a.cstride, off);
             }
         }
     }
-    template <class CS, class S>
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
@@ -431,10 +482,10 @@ struct PolLeapfrog {
                 const double phi0 = e[i++];
                 const double vh = pi0 + a.c[0] * rhs[nm];
                 st<double>(a.vh, bp, off, vh);
-                store_state<S>(a.unew, a.unewlo, bf, off, phi0 + a.c[1] * vh);
+                store_state<T>(a.unew, a.unewlo, bf, off, phi0 + a.c[1] * vh);
             } else {
                 const double vh = e[i++];
-                store_state<S>(a.unew, a.unewlo, bm, off, vh + a.c[0] * rhs[nm]);
+                store_state<T>(a.unew, a.unewlo, bm, off, vh + a.c[0] * rhs[nm]);
             }
         }
     }
@@ -468,7 +519,8 @@ __device__ AMREX_FORCE_INLINE void ring_xy(int h, int &x, int &y) {
 template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int NW, int RY>
 __global__ void __launch_bounds__(64 * NW)
     stage_kernel(const StageArgs a) {
-    using S = typename PrecTraits<P>::S;
+    using T = PrecTraits<P>;
+    using S = typename T::S;
     using CS = CompSet<Ops, GW, KO>;
     constexpr int NC = CS::NC;
     constexpr int NS = CS::NS;
@@ -570,8 +622,8 @@ __global__ void __launch_bounds__(64 * NW)
         for (int r = 0; r < RY; ++r) qoff[r] = (ldx && ldy[r]) ? coff[r] : 0u;
         auto zoff = [&](int z) { return (unsigned)(wz(z) + g) * ks; };
 
-        using E = typename Pol::template Ext<S>;
-        constexpr int NE = Pol::template next<CS, S>();
+        using E = typename Pol::template Ext<T>;
+        constexpr int NE = Pol::template next<CS, T>();
         S q[NS][RY][Q];
         S nq[NS][RY];
         S hv[NS][HPT];
@@ -585,10 +637,10 @@ __global__ void __launch_bounds__(64 * NW)
             for (int r = 0; r < RY; ++r) {
 #pragma unroll
                 for (int m = 1; m < Q; ++m)
-                    q[s][r][m] = Pol::template load_st<CS, S>(
+                    q[s][r][m] = Pol::template load_st<CS, T>(
                         a, org, CS::comp_of_slot(s),
                         qoff[r] + zoff(td.z0 - R + m - 1));
-                nq[s][r] = Pol::template load_st<CS, S>(
+                nq[s][r] = Pol::template load_st<CS, T>(
                     a, org, CS::comp_of_slot(s), qoff[r] + zoff(td.z0 + R));
             }
         auto load_halo = [&](int z) {
@@ -597,7 +649,7 @@ __global__ void __launch_bounds__(64 * NW)
             for (int m = 0; m < HPT; ++m)
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    hv[s][m] = Pol::template load_st<CS, S>(
+                    hv[s][m] = Pol::template load_st<CS, T>(
                         a, org, CS::comp_of_slot(s), hoff[m] + zo);
         };
         auto load_point = [&](int z) {
@@ -606,12 +658,12 @@ __global__ void __launch_bounds__(64 * NW)
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int r = 0; r < RY; ++r)
-                    pv[p][r] = Pol::template load_pt<CS, S>(
+                    pv[p][r] = Pol::template load_pt<CS, T>(
                         a, org, CS::comp_of_point(p), qoff[r] + zo);
             if constexpr (NE > 0) {
 #pragma unroll
                 for (int r = 0; r < RY; ++r)
-                    Pol::template load_ext<CS, S>(a, org, qoff[r] + zo, ev[r]);
+                    Pol::template load_ext<CS, T>(a, org, qoff[r] + zo, ev[r]);
             }
         };
         load_halo(td.z0);
@@ -657,7 +709,7 @@ __global__ void __launch_bounds__(64 * NW)
                 for (int s = 0; s < NS; ++s)
 #pragma unroll
                     for (int r = 0; r < RY; ++r)
-                        nq[s][r] = Pol::template load_st<CS, S>(
+                        nq[s][r] = Pol::template load_st<CS, T>(
                             a, org, CS::comp_of_slot(s), qoff[r] + zq);
                 load_halo(z + 1);
                 load_point(z + 1);
@@ -724,7 +776,7 @@ __global__ void __launch_bounds__(64 * NW)
                                 sv, gi, gj, gk, n, dx, a.ko_strength[n]);
                     }
                 }
-                Pol::template store<CS, S>(a, org, coff[r] + zc, rhs, ctr, ec[r]);
+                Pol::template store<CS, T>(a, org, coff[r] + zc, rhs, ctr, ec[r]);
             }
         }
     }

@@ -15,9 +15,9 @@ struct ProbeOut;
 /** One instantiated stage program per kind: the integrator stage is a
  *  compile-time property of the kernel, so its store path has no branches. */
 enum StageKind : int {
-    kSsprk1 = 0,   // k = f(u0); y = u0 + dt k
-    kSsprk2,       // k = f(y1) + k; y = u0 + dt/4 k
-    kSsprk3,       // k = f(y2) + k/4; u = u0 + 2dt/3 k (compensated)
+    kSsprk1 = 0,   // k1 = f(u0)
+    kSsprk2,       // k2 = f(u0 + dt k1) + k1
+    kSsprk3,       // k3 = f(u0 + dt/4 k2) + k2/4; u = u0 + 2dt/3 k3
     kFillRhs,      // rhs = f(u)
     kFillAddRhs,   // rhs = f(u) + w rhs
     kRkn0,         // RKN non-final stage with 0..3 earlier stages
@@ -61,8 +61,8 @@ struct KernelTable {
     bool built_gw = false, built_ko = false;
     int r_mask = 0;
     // [kind][precision][R][gw][ko][variant]
-    StageEntry stage[kNKind][2][4][2][2][kNVar];
-    TeTagFn te_tag[2][2] = {};  // [precision][gw]
+    StageEntry stage[kNKind][3][4][2][2][kNVar];
+    TeTagFn te_tag[3][2] = {};  // [precision][gw]
     ProbeFn probe[2] = {};      // [gw]
 };
 

@@ -17,7 +17,10 @@ from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
 
-PRECS = [(shm.PREC_FP64, 1e-12), (shm.PREC_FP32C, 1e-6)]
+PRECS = [(shm.PREC_FP64, 1e-12), (shm.PREC_FP32C, 1e-6), (shm.PREC_FP32C16, 1e-6)]
+ALLP = [shm.PREC_FP64, shm.PREC_FP32C, shm.PREC_FP32C16]
+# Representation error of the stored pair hi + lo.
+PAIR_TOL = {shm.PREC_FP64: 1e-15, shm.PREC_FP32C: 1e-13, shm.PREC_FP32C16: 1e-9}
 
 
 def _load_same(sim, orc, lev, fields):
@@ -27,7 +30,7 @@ def _load_same(sim, orc, lev, fields):
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C])
+@pytest.mark.parametrize("prec", ALLP)
 def test_fill_rhs_real_scalar(order, prec):
     inp = real_scalar(N=32, order=order, integrator=21)
     sim = shm.Simulation("RealScalar", inp, precision=prec)
@@ -43,7 +46,7 @@ def test_fill_rhs_real_scalar(order, prec):
             assert err < (1e-13 if prec == shm.PREC_FP64 else 1e-5), (c, add, err)
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C])
+@pytest.mark.parametrize("prec", ALLP)
 def test_fill_rhs_complex_scalar(prec):
     inp = complex_scalar(N=32)
     sim = shm.Simulation("ComplexScalar", inp, precision=prec)
@@ -100,7 +103,7 @@ def test_device_initial_state_matches_oracle():
         assert np.max(np.abs(sim.download(0, c) - orc.download(0, c))) < 1e-13
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C])
+@pytest.mark.parametrize("prec", ALLP)
 def test_multibox_equals_single_box(prec):
     """A level chopped into 8 boxes (box-to-box halos) gives the single-box
     result bit-for-bit."""
@@ -131,7 +134,7 @@ def test_multibox_equals_single_box(prec):
             off += nb
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C])
+@pytest.mark.parametrize("prec", ALLP)
 def test_te_tag_bit_exact(prec):
     """Tags and counters from identical fine and shadow data: the GPU mask
     equals the oracle's literal CPU tag loop bit-for-bit."""
@@ -141,12 +144,12 @@ def test_te_tag_bit_exact(prec):
     n = sim.ncells(0)
     rng = np.random.default_rng(99)
     fine = [1.0 + 0.01 * rng.standard_normal(n) for _ in range(4)]
-    if prec == shm.PREC_FP32C:  # the TE reads the FP32 values of the state
+    if prec != shm.PREC_FP64:  # the TE reads the FP32 values of the state
         fine = [f.astype(np.float32).astype(np.float64) for f in fine]
     _load_same(sim, orc, 0, fine)
     nc = n // 8
     crse = [1.0 + 0.01 * rng.standard_normal(nc) for _ in range(4)]
-    if prec == shm.PREC_FP32C:
+    if prec != shm.PREC_FP64:
         crse = [f.astype(np.float32).astype(np.float64) for f in crse]
     _load_same(sim, orc, -1, crse)
     gm, gc, gte = sim.te_tag(0, want_te=True)
@@ -157,7 +160,7 @@ def test_te_tag_bit_exact(prec):
     assert gm.sum() > 0 and gm.sum() < n
     # Averaged-down coarse data.
     for c in range(4):
-        assert rel_linf(sim.download(-1, c), orc.download(-1, c)) < 1e-13
+        assert rel_linf(sim.download(-1, c), orc.download(-1, c)) < PAIR_TOL[prec]
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
