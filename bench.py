@@ -163,6 +163,8 @@ def main():
                           "avg_launch_ms": avg_ms,
                           "algorithmic_bytes_per_launch": bpl}
     result["halo_ms_per_step"] = tm32["halo_ms"] / a.steps
+    # Achievable bandwidth on this GPU (plain float4 kernels over 4 GiB).
+    result["measured_bandwidth"] = sim32.stream_probe(4 << 30, 10)
 
     if not a.no_fp64:
         sim64, sec64, tm64 = run_gpu(shm, inp, shm.PREC_FP64, a.steps, a.warmup,

@@ -126,6 +126,13 @@ int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
                     int *nout);
 
 int shm_sync(shm_ctx *ctx);
+
+/* Achievable HBM bandwidth: plain float4 read, write and copy kernels over
+ * buffers of `bytes` bytes, mean of `reps` launches (GB/s; copy counts read
+ * plus write).  No counterpart in the reference: the measured-copy reference
+ * printed beside the HBM peak. */
+int shm_stream_probe(shm_ctx *ctx, unsigned long long bytes, int reps,
+                     double *read_gbs, double *write_gbs, double *copy_gbs);
 int shm_autotune(shm_ctx *ctx, int lev);
 int shm_set_tile(shm_ctx *ctx, int variant, int tile_z);
 

@@ -29,8 +29,8 @@ EXPORTS = (
     "shm_level_info", "shm_set_level_time", "shm_upload", "shm_download",
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
-    "shm_sync", "shm_autotune", "shm_set_tile", "shm_set_timing",
-    "shm_get_timing",
+    "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_tile",
+    "shm_set_timing", "shm_get_timing",
 )
 
 
@@ -92,6 +92,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "shm_sync": (i, [vp]),
+        "shm_stream_probe": (i, [vp, C.c_ulonglong, i, dp, dp, dp]),
         "shm_autotune": (i, [vp, i]),
         "shm_set_tile": (i, [vp, i, i]),
         "shm_set_timing": (i, [vp, i]),
@@ -262,6 +263,12 @@ class Simulation:
 
     def sync(self) -> None:
         self._check(self.lib.shm_sync(self._ctx))
+
+    def stream_probe(self, nbytes: int = 4 << 30, reps: int = 10) -> dict:
+        r, w, c = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.lib.shm_stream_probe(self._ctx, nbytes, reps, C.byref(r),
+                                              C.byref(w), C.byref(c)))
+        return {"read_gbs": r.value, "write_gbs": w.value, "copy_gbs": c.value}
 
     def autotune(self, lev: int = 0) -> None:
         self._check(self.lib.shm_autotune(self._ctx, lev))

@@ -286,6 +286,17 @@ __attribute__((visibility("default"))) int shm_sync(shm_ctx *ctx) {
     });
 }
 
+__attribute__((visibility("default"))) int shm_stream_probe(
+    shm_ctx *ctx, unsigned long long bytes, int reps, double *read_gbs,
+    double *write_gbs, double *copy_gbs) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (bytes < (1u << 20) || reps < 1)
+            throw shm::Error(SHM_ERR_ARG, "stream probe needs >= 1 MiB and reps >= 1");
+        ctx->rt->stream_probe(bytes, reps, read_gbs, write_gbs, copy_gbs);
+    });
+}
+
 __attribute__((visibility("default"))) int shm_autotune(shm_ctx *ctx, int lev) {
     return guard(ctx, [&] {
         SHM_NEED_RT;

@@ -75,6 +75,43 @@ const std::vector<std::string> *ParmParse::find(const char *name) const {
 
 namespace shm {
 
+// ---------------------------------------------------------------------------
+// Stream probe: plain read / write / copy kernels over large buffers, the
+// achievable-bandwidth reference printed beside the HBM peak.
+// ---------------------------------------------------------------------------
+
+typedef float sf4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) stream_read_kernel(const sf4 *a, long long n,
+                                                          float *out) {
+    float s = 0.f;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        const sf4 v = a[i];
+        s += v.x + v.y + v.z + v.w;
+    }
+    if (s == 12345.f) out[blockIdx.x] = s;  // keeps the loads alive
+}
+
+__global__ void __launch_bounds__(256) stream_write_kernel(sf4 *a, long long n,
+                                                           float v) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const sf4 w = {v, v, v, v};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride)
+        __builtin_nontemporal_store(w, &a[i]);
+}
+
+__global__ void __launch_bounds__(256) stream_copy_kernel(const sf4 *a, sf4 *b,
+                                                          long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride)
+        __builtin_nontemporal_store(a[i], &b[i]);
+}
+
+
 #define SHM_HIP(call)                                                          \
     do {                                                                       \
         hipError_t e_ = (call);                                                \
@@ -1296,6 +1333,43 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
 }
 
 void Runtime::sync() { SHM_HIP(hipStreamSynchronize(stream_)); }
+
+void Runtime::stream_probe(size_t bytes, int reps, double *read_gbs,
+                           double *write_gbs, double *copy_gbs) {
+    bytes = bytes / 16 * 16;
+    const long long n = (long long)(bytes / 16);
+    DevMem A, B, O;
+    A.alloc(bytes);
+    B.alloc(bytes);
+    O.alloc(4096 * sizeof(float));
+    const int grid = 8 * ncu_ * 4;
+    auto timed = [&](auto &&launch) {
+        launch();
+        SHM_HIP(hipEventRecord(ev0_, stream_));
+        for (int r = 0; r < reps; ++r) launch();
+        SHM_HIP(hipEventRecord(ev1_, stream_));
+        SHM_HIP(hipEventSynchronize(ev1_));
+        float ms = 0;
+        SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        return ms / reps;
+    };
+    const double r_ms = timed([&] {
+        hipLaunchKernelGGL(stream_read_kernel, dim3(grid), dim3(256), 0, stream_,
+                           A.as<sf4>(), n, O.as<float>());
+    });
+    const double w_ms = timed([&] {
+        hipLaunchKernelGGL(stream_write_kernel, dim3(grid), dim3(256), 0, stream_,
+                           B.as<sf4>(), n, 1.f);
+    });
+    const double c_ms = timed([&] {
+        hipLaunchKernelGGL(stream_copy_kernel, dim3(grid), dim3(256), 0, stream_,
+                           A.as<const sf4>(), B.as<sf4>(), n);
+    });
+    SHM_HIP(hipGetLastError());
+    if (read_gbs) *read_gbs = bytes / (r_ms * 1e-3) / 1e9;
+    if (write_gbs) *write_gbs = bytes / (w_ms * 1e-3) / 1e9;
+    if (copy_gbs) *copy_gbs = 2. * bytes / (c_ms * 1e-3) / 1e9;
+}
 
 /** Event pair for one timed launch, from a pool that grows on demand. */
 Runtime::EventPair *Runtime::timed_pair(int kind) {

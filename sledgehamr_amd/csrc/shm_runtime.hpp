@@ -135,6 +135,9 @@ class Runtime {
 
     void sync();
     void autotune(int lev);
+    /** Plain read / write / copy bandwidth over buffers of `bytes` (GB/s). */
+    void stream_probe(size_t bytes, int reps, double *read_gbs,
+                      double *write_gbs, double *copy_gbs);
     void collect_timing();
 
     // State queries.

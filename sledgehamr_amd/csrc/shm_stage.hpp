@@ -268,9 +268,8 @@ struct PolSsprk {
                                                                unsigned off) {
         using S = typename T::S;
         const long long b = org + n * a.cstride;
-        if constexpr (composed)
-            return (S)((double)ld<S>(a.u0, b, off) +
-                       a.c[2] * (double)ld<S>(a.kin, b, off));
+        if constexpr (composed)  // stage state in the storage precision
+            return ld<S>(a.u0, b, off) + (S)a.c[2] * ld<S>(a.kin, b, off);
         else
             return ld<S>(a.x, b, off);
     }
@@ -339,31 +338,33 @@ struct PolRkn {
         return FINAL ? CS::NPAIR * (2 * (T::comp ? 2 : 1) + NPREV) : 0;
     }
 
+    /** Stage state in the storage precision (FP32 states are FP32 values,
+     *  so their sums are formed in FP32). */
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static double state(const StageArgs &a,
-                                                      long long org, int n,
-                                                      unsigned off) {
+    __device__ AMREX_FORCE_INLINE static typename T::S state(const StageArgs &a,
+                                                             long long org, int n,
+                                                             unsigned off) {
         using S = typename T::S;
         const int p = CS::pair_of_field(n);
-        if (p < 0) return (double)ld<S>(a.u0, org + n * a.cstride, off);
-        double y = (double)ld<S>(a.u0, org + n * a.cstride, off);
-        y += a.c[0] * (double)ld<S>(a.u0, org + CS::pair_mom(p) * a.cstride, off);
+        if (p < 0) return ld<S>(a.u0, org + n * a.cstride, off);
+        S y = ld<S>(a.u0, org + n * a.cstride, off);
+        y += (S)a.c[0] * ld<S>(a.u0, org + CS::pair_mom(p) * a.cstride, off);
 #pragma unroll
         for (int j = 0; j < NPREV; ++j)
-            y += a.c[1 + j] * (double)ld<S>(a.f[j], org + p * a.cstride, off);
+            y += (S)a.c[1 + j] * ld<S>(a.f[j], org + p * a.cstride, off);
         return y;
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        return (typename T::S)state<CS, T>(a, org, n, off);
+        return state<CS, T>(a, org, n, off);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static typename T::S load_pt(const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        return (typename T::S)state<CS, T>(a, org, n, off);
+        return state<CS, T>(a, org, n, off);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
