@@ -133,8 +133,25 @@ int shm_sync(shm_ctx *ctx);
  * printed beside the HBM peak. */
 int shm_stream_probe(shm_ctx *ctx, unsigned long long bytes, int reps,
                      double *read_gbs, double *write_gbs, double *copy_gbs);
+
+/* Launch shape of the fused stage kernels.  No counterpart in the reference
+ * (AMReX picks its own launch bounds): the tile shape and work order of the
+ * persistent tile-table launch.  A field of -1 (tile_z: 0) is not set: the
+ * tuned value of the level, else the built-in default, applies. */
+typedef struct shm_launch_t {
+    int variant;      /* tile shape: 0..2 = 64 x {4, 8, 16} cells per z plane */
+    int tile_z;       /* z-chunk length of a tile (0: automatic) */
+    int xcd_split;    /* 1: each XCD walks one contiguous range of tiles */
+    int persistent;   /* 1: grid = resident capacity; 0: one tile per workgroup */
+    int zc_minor;     /* tile order: 0 = z chunk outermost, 1 = between y and x */
+} shm_launch_t;
+
+/* Times the stage kernels of a level over every launch shape not fixed by
+ * shm_set_launch and keeps the fastest for that level. */
 int shm_autotune(shm_ctx *ctx, int lev);
-int shm_set_tile(shm_ctx *ctx, int variant, int tile_z);
+int shm_set_launch(shm_ctx *ctx, const shm_launch_t *cfg);
+/* The launch shape a level's stage kernels use now. */
+int shm_get_launch(shm_ctx *ctx, int lev, shm_launch_t *cfg);
 
 /* Kernel timing with HIP events on the library's stream. */
 int shm_set_timing(shm_ctx *ctx, int on);

@@ -29,8 +29,8 @@ EXPORTS = (
     "shm_level_info", "shm_set_level_time", "shm_upload", "shm_download",
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
-    "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_tile",
-    "shm_set_timing", "shm_get_timing",
+    "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
+    "shm_get_launch", "shm_set_timing", "shm_get_timing",
 )
 
 
@@ -43,6 +43,11 @@ class ShmError(RuntimeError):
 class _Config(C.Structure):
     _fields_ = [("inputs", C.c_char_p), ("precision", C.c_int),
                 ("device", C.c_int)]
+
+
+class _Launch(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "variant", "tile_z", "xcd_split", "persistent", "zc_minor")]
 
 
 class _Info(C.Structure):
@@ -94,7 +99,8 @@ def load_library(project: str) -> C.CDLL:
         "shm_sync": (i, [vp]),
         "shm_stream_probe": (i, [vp, C.c_ulonglong, i, dp, dp, dp]),
         "shm_autotune": (i, [vp, i]),
-        "shm_set_tile": (i, [vp, i, i]),
+        "shm_set_launch": (i, [vp, C.POINTER(_Launch)]),
+        "shm_get_launch": (i, [vp, i, C.POINTER(_Launch)]),
         "shm_set_timing": (i, [vp, i]),
         "shm_get_timing": (i, [vp, dp, llp, llp, dp, llp]),
     }
@@ -273,8 +279,17 @@ class Simulation:
     def autotune(self, lev: int = 0) -> None:
         self._check(self.lib.shm_autotune(self._ctx, lev))
 
-    def set_tile(self, variant: int, tile_z: int = 0) -> None:
-        self._check(self.lib.shm_set_tile(self._ctx, variant, tile_z))
+    def set_launch(self, variant: int = -1, tile_z: int = 0, xcd_split: int = -1,
+                   persistent: int = -1, zc_minor: int = -1) -> None:
+        """Fixes fields of the stage-kernel launch shape (-1 / 0: not set)."""
+        cfg = _Launch(variant, tile_z, xcd_split, persistent, zc_minor)
+        self._check(self.lib.shm_set_launch(self._ctx, C.byref(cfg)))
+
+    def get_launch(self, lev: int = 0) -> dict:
+        """Launch shape the stage kernels of level lev use now."""
+        cfg = _Launch()
+        self._check(self.lib.shm_get_launch(self._ctx, lev, C.byref(cfg)))
+        return {n: getattr(cfg, n) for n, _ in _Launch._fields_}
 
     def set_timing(self, on: bool) -> None:
         self._check(self.lib.shm_set_timing(self._ctx, int(on)))

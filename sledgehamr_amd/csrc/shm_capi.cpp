@@ -304,14 +304,42 @@ __attribute__((visibility("default"))) int shm_autotune(shm_ctx *ctx, int lev) {
     });
 }
 
-__attribute__((visibility("default"))) int shm_set_tile(shm_ctx *ctx,
-                                                        int variant,
-                                                        int tile_z) {
+__attribute__((visibility("default"))) int shm_set_launch(shm_ctx *ctx,
+                                                          const shm_launch_t *cfg) {
     return guard(ctx, [&] {
         SHM_NEED_RT;
-        if (variant >= shm::kNVar) throw shm::Error(SHM_ERR_ARG, "bad tile variant");
-        ctx->rt->variant = variant;
-        ctx->rt->tz = tile_z;
+        if (!cfg) throw shm::Error(SHM_ERR_ARG, "null launch shape");
+        if (cfg->variant >= shm::kNVar || cfg->variant < -1)
+            throw shm::Error(SHM_ERR_ARG, "bad tile variant");
+        if (cfg->tile_z < 0) throw shm::Error(SHM_ERR_ARG, "bad tile_z");
+        auto flag = [](int v, const char *what) {
+            if (v < -1 || v > 1)
+                throw shm::Error(SHM_ERR_ARG, std::string("bad ") + what);
+            return v;
+        };
+        shm::LaunchCfg c;
+        c.variant = cfg->variant;
+        c.tz = cfg->tile_z;
+        c.xcd_split = flag(cfg->xcd_split, "xcd_split");
+        c.persistent = flag(cfg->persistent, "persistent");
+        c.zc_minor = flag(cfg->zc_minor, "zc_minor");
+        ctx->rt->launch = c;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_get_launch(shm_ctx *ctx, int lev,
+                                                          shm_launch_t *cfg) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!cfg) throw shm::Error(SHM_ERR_ARG, "null launch shape");
+        if (lev < -1 || lev >= ctx->rt->nlevels())
+            throw shm::Error(SHM_ERR_ARG, "bad level");
+        const shm::LaunchCfg c = ctx->rt->resolved_launch(lev);
+        cfg->variant = c.variant;
+        cfg->tile_z = c.tz;
+        cfg->xcd_split = c.xcd_split;
+        cfg->persistent = c.persistent;
+        cfg->zc_minor = c.zc_minor;
     });
 }
 

@@ -168,10 +168,13 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
             bd.lo[d] = bx.lo[d];
             bd.n[d] = bx.n(d);
         }
-        bd.jstride = (int)round_up(bx.n(0) + 2 * g, 32);
+        // Rows of valid cells start on a 64-element boundary (one 128-byte
+        // line holds a tile row of a bf16 remainder, two of an FP32 array)
+        // for boxes at least one tile wide, on 32 elements otherwise.
+        const int al = bx.n(0) >= 64 ? 64 : 32;
+        bd.jstride = (int)round_up(bx.n(0) + 2 * g, al);
         bd.kstride = bd.jstride * (bx.n(1) + 2 * g);
-        // Place the brick so that the first valid cell is 32-element aligned.
-        const long long start = round_up(end + g, 32) - g;
+        const long long start = round_up(end + g, al) - g;
         bd.base = start + (long long)g * bd.kstride + (long long)g * bd.jstride + g;
         bd.org = start;
         bd.wrap = 0;
@@ -251,24 +254,34 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
     tiles.clear();
 }
 
-const TileDesc *Layout::get_tiles(int ty, int tz, int *ntiles) {
-    auto key = std::make_pair(ty, tz);
+const TileDesc *Layout::get_tiles(int ty, int tz, int zc_minor, int *ntiles) {
+    auto key = std::make_tuple(ty, tz, zc_minor);
     auto it = tiles.find(key);
     if (it == tiles.end()) {
         std::vector<TileDesc> tv;
         for (int b = 0; b < (int)boxes.size(); ++b) {
             const Box3 &bx = boxes[b];
-            for (int z0 = 0; z0 < bx.n(2); z0 += tz)
-                for (int y0 = 0; y0 < bx.n(1); y0 += ty)
-                    for (int x0 = 0; x0 < bx.n(0); x0 += 64) {
-                        TileDesc t;
-                        t.box = b;
-                        t.x0 = x0;
-                        t.y0 = y0;
-                        t.z0 = z0;
-                        t.z1 = std::min(z0 + tz, bx.n(2));
-                        tv.push_back(t);
-                    }
+            const int nzc = (bx.n(2) + tz - 1) / tz;
+            const int ny = (bx.n(1) + ty - 1) / ty, nx = (bx.n(0) + 63) / 64;
+            auto push = [&](int x, int y, int zc) {
+                TileDesc t;
+                t.box = b;
+                t.x0 = x * 64;
+                t.y0 = y * ty;
+                t.z0 = zc * tz;
+                t.z1 = std::min(t.z0 + tz, bx.n(2));
+                tv.push_back(t);
+            };
+            // x fastest; the z chunk outermost or between y and x.
+            if (zc_minor) {
+                for (int y = 0; y < ny; ++y)
+                    for (int zc = 0; zc < nzc; ++zc)
+                        for (int x = 0; x < nx; ++x) push(x, y, zc);
+            } else {
+                for (int zc = 0; zc < nzc; ++zc)
+                    for (int y = 0; y < ny; ++y)
+                        for (int x = 0; x < nx; ++x) push(x, y, zc);
+            }
         }
         auto mem = std::make_shared<DevMem>();
         upload_vec(*mem, tv);
@@ -562,9 +575,12 @@ void Runtime::init_from_inputs(const std::string &inputs,
     if (precision_override == 0) prec_ = Prec::F64;
     if (precision_override == 1) prec_ = Prec::F32C;
     if (precision_override == 2) prec_ = Prec::F32C16;
-    pp.query("shm.tile_variant", variant);
-    pp.query("shm.tile_z", tz);
-    pp.query("shm.xcd_split", xcd_split);
+    pp.query("shm.tile_variant", launch.variant);
+    pp.query("shm.tile_z", launch.tz);
+    pp.query("shm.xcd_split", launch.xcd_split);
+    pp.query("shm.persistent", launch.persistent);
+    pp.query("shm.tile_order_zc_minor", launch.zc_minor);
+    if (launch.variant >= kNVar) throw Error(1, "shm.tile_variant out of range");
 
     // Project initialisation, then the reach of its cell functions.
     sim_->Init();
@@ -587,8 +603,7 @@ void Runtime::init_from_inputs(const std::string &inputs,
     levels_.resize(S.max_level + 1);
     set_level(0, boxes);
     S.finest_level = 0;
-    tuned_variant_.assign(S.max_level + 2, -1);
-    tuned_tz_.assign(S.max_level + 2, 0);
+    tuned_.assign(S.max_level + 2, LaunchCfg());
     regrid_dt_.clear();
     last_regrid_.clear();
     double reg_dt = DBL_MAX;
@@ -700,13 +715,25 @@ void Runtime::fill_params(double *blk, double time, double dt, double dx,
 // Launch helpers.
 // ---------------------------------------------------------------------------
 
+LaunchCfg Runtime::resolved_launch(int lev) const {
+    const LaunchCfg t = lev + 1 >= 0 && lev + 1 < (int)tuned_.size() ? tuned_[lev + 1]
+                                                                     : LaunchCfg();
+    auto pick = [](int user, int tuned, int dflt) {
+        return user >= 0 ? user : (tuned >= 0 ? tuned : dflt);
+    };
+    LaunchCfg c;
+    c.variant = pick(launch.variant, t.variant, kDefaultVariant);
+    c.tz = launch.tz > 0 ? launch.tz : t.tz;
+    c.xcd_split = pick(launch.xcd_split, t.xcd_split, 1);
+    c.persistent = pick(launch.persistent, t.persistent, 1);
+    c.zc_minor = pick(launch.zc_minor, t.zc_minor, 0);
+    return c;
+}
+
 void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
     const int pi = (int)prec_;
-    int v = variant >= 0 ? variant
-                         : (lev + 1 < (int)tuned_variant_.size() && lev >= -1 &&
-                                    tuned_variant_[lev + 1] >= 0
-                                ? tuned_variant_[lev + 1]
-                                : kDefaultVariant);
+    const LaunchCfg lc = resolved_launch(lev);
+    int v = lc.variant;
     const auto &row = kt_.stage[kind][pi][R_][gw_ ? 1 : 0][ko_ ? 1 : 0];
     // Variants over the LDS budget (and non-default dissipation variants)
     // are not instantiated: take the nearest built tile shape.
@@ -720,11 +747,7 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
     }
     const StageEntry &e = row[v];
     if (!e.fn) throw Error(4, "stage kernel not built for this configuration");
-    int z = tz > 0 ? tz
-                   : (lev + 1 < (int)tuned_tz_.size() && lev >= -1 &&
-                              tuned_tz_[lev + 1] > 0
-                          ? tuned_tz_[lev + 1]
-                          : 0);
+    int z = lc.tz;
     if (z <= 0) {
         // Enough tiles to fill every CU several times, long z runs otherwise.
         int maxn = 1;
@@ -737,7 +760,7 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
         while (z > 16 && xy * ((maxn + z - 1) / z) < 8LL * ncu_) z = (z + 1) / 2;
     }
     int ntiles = 0;
-    a.tiles = ls.L.get_tiles(e.ty, z, &ntiles);
+    a.tiles = ls.L.get_tiles(e.ty, z, lc.zc_minor, &ntiles);
     a.boxes = ls.L.dbox.as<BoxDesc>();
     a.ntiles = ntiles;
     a.cstride = ls.L.cstride;
@@ -747,8 +770,9 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
     a.ghost = ls.L.g;
     if (ntiles == 0) return;
     const int occ = e.occupancy();
-    int grid = std::min<long long>(ntiles, (long long)occ * ncu_);
-    a.xcd_split = xcd_split ? 1 : 0;
+    int grid = lc.persistent ? (int)std::min<long long>(ntiles, (long long)occ * ncu_)
+                             : ntiles;
+    a.xcd_split = lc.xcd_split ? 1 : 0;
     if (a.xcd_split) grid = (int)round_up(grid, 8);
     EventPair *ep = timing_on ? timed_pair(0) : nullptr;
     if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
@@ -1407,6 +1431,11 @@ void Runtime::collect_timing() {
 // only stage scratch) for each tile variant and a few z-chunk lengths.
 // ---------------------------------------------------------------------------
 
+/** Times the stage kernels of a level over the launch shapes (tile variant,
+ *  z-chunk length, persistent or one tile per workgroup, tile order, XCD
+ *  split) and keeps the fastest for the level.  Only kernels that write
+ *  stage scratch run (SSPRK3 stages 1 and 2, the first RKN stage), so the
+ *  level's state is unchanged.  Fields of `launch` that are set are kept. */
 void Runtime::autotune(int lev) {
     LevelState &ls = level(lev);
     const int old = ls.cur;
@@ -1416,61 +1445,87 @@ void Runtime::autotune(int lev) {
     fill_params(blk.data(), ls.t, dt, dx, lev);
     const double *prm = push_params(prm_host_, prm_ev_, prm_slot_,
                                     prm_dev_.as<double>(), stream_, blk);
-    const int pol = integrator_ == 10 ? kSsprk1
-                                      : (integrator_ == 11 ? kLeap1 : kRkn0);
+    std::vector<std::pair<int, StageArgs>> runs;
     StageArgs a{};
     a.prm = prm;
-    if (pol == kSsprk1) {
-        a.x = ls.U[old].get();
-        a.kout = ls.K[0].get();
-    } else if (pol == kRkn0) {
+    if (integrator_ == 10) {
+        StageArgs s1 = a;
+        s1.x = ls.U[old].get();
+        s1.kout = ls.K[0].get();
+        runs.push_back({kSsprk1, s1});
+        StageArgs s2 = a;
+        s2.u0 = ls.U[old].get();
+        s2.kin = ls.K[0].get();
+        s2.kout = ls.K[1].get();
+        s2.c[0] = 1.;
+        s2.c[2] = dt;
+        runs.push_back({kSsprk2, s2});
+    } else if (integrator_ >= 20) {
         if (nodes_.size() < 2) return;
-        a.u0 = ls.U[old].get();
-        a.fout = ls.F[0].get();
+        StageArgs s1 = a;
+        s1.u0 = ls.U[old].get();
+        s1.fout = ls.F[0].get();
+        runs.push_back({kRkn0, s1});
     } else {
         return;  // leapfrog kick 1 writes u_new fields; keep the default
     }
     int maxn = 1;
     for (const Box3 &b : ls.L.boxes) maxn = std::max(maxn, b.n(2));
+    auto choices = [](int user, std::vector<int> all) {
+        return user >= 0 ? std::vector<int>{user} : all;
+    };
     std::vector<int> zs = {0};
-    for (int z = 16; z <= maxn; z *= 2) zs.push_back(z);
-    const int saved_v = variant, saved_tz = tz;
+    for (int z = 8; z < maxn; z *= 2) zs.push_back(z);
+    zs.push_back(maxn);
+    if (launch.tz > 0) zs = {launch.tz};
+    std::vector<int> vs;
+    for (int v = 0; v < kNVar; ++v)
+        if (kt_.stage[runs[0].first][(int)prec_][R_][gw_ ? 1 : 0][ko_ ? 1 : 0][v].fn)
+            vs.push_back(v);
+    vs = choices(launch.variant, vs);
+    const std::vector<int> ps = choices(launch.persistent, {1, 0});
+    const std::vector<int> os = choices(launch.zc_minor, {0, 1});
+    const std::vector<int> xs = choices(launch.xcd_split, {1, 0});
+
+    const LaunchCfg saved = launch;
     const bool saved_t = timing_on;
     timing_on = false;
     double best = 1e30;
-    int bv = 1, bz = 0;
-    for (int v = 0; v < kNVar; ++v) {
-        const StageEntry &e = kt_.stage[pol][(int)prec_][R_][gw_ ? 1 : 0][ko_ ? 1 : 0][v];
-        if (!e.fn) continue;
-        for (int z : zs) {
-            variant = v;
-            tz = z;
-            for (int w = 0; w < 2; ++w) {
-                StageArgs c = a;
-                launch_stage(pol, ls, c, lev);
-            }
-            SHM_HIP(hipEventRecord(ev0_, stream_));
-            const int reps = 5;
-            for (int r = 0; r < reps; ++r) {
-                StageArgs c = a;
-                launch_stage(pol, ls, c, lev);
-            }
-            SHM_HIP(hipEventRecord(ev1_, stream_));
-            SHM_HIP(hipEventSynchronize(ev1_));
-            float ms = 0;
-            SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-            if (ms < best) {
-                best = ms;
-                bv = v;
-                bz = z;
-            }
-        }
-    }
-    variant = saved_v;
-    tz = saved_tz;
+    LaunchCfg bc;
+    for (int v : vs)
+        for (int z : zs)
+            for (int p : ps)
+                for (int o : os)
+                    for (int x : xs) {
+                        if (o == 1 && (z == 0 || z >= maxn)) continue;  // one chunk
+                        launch.variant = v;
+                        launch.tz = z;
+                        launch.persistent = p;
+                        launch.zc_minor = o;
+                        launch.xcd_split = x;
+                        for (auto &r : runs) {
+                            StageArgs c = r.second;
+                            launch_stage(r.first, ls, c, lev);
+                        }
+                        SHM_HIP(hipEventRecord(ev0_, stream_));
+                        const int reps = 3;
+                        for (int k = 0; k < reps; ++k)
+                            for (auto &r : runs) {
+                                StageArgs c = r.second;
+                                launch_stage(r.first, ls, c, lev);
+                            }
+                        SHM_HIP(hipEventRecord(ev1_, stream_));
+                        SHM_HIP(hipEventSynchronize(ev1_));
+                        float ms = 0;
+                        SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+                        if (ms < best) {
+                            best = ms;
+                            bc = launch;
+                        }
+                    }
+    launch = saved;
     timing_on = saved_t;
-    tuned_variant_[lev + 1] = bv;
-    tuned_tz_[lev + 1] = bz;
+    tuned_[lev + 1] = bc;
 }
 
 // ---------------------------------------------------------------------------

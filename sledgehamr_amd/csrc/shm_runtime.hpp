@@ -6,6 +6,7 @@
 
 #include <cfloat>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <string>
 #include <utility>
@@ -77,11 +78,11 @@ struct Layout {
     std::vector<HaloSeg> halo_full;  // every ghost cell
     DevMem dhalo_full;
     long long halo_full_total = 0;
-    std::map<std::pair<int, int>, std::pair<std::shared_ptr<DevMem>, int>> tiles;
+    std::map<std::tuple<int, int, int>, std::pair<std::shared_ptr<DevMem>, int>> tiles;
 
     void build(const std::vector<Box3> &b, int ghost, int domain_n,
                bool is_periodic, bool is_independent = false);
-    const TileDesc *get_tiles(int ty, int tz, int *ntiles);
+    const TileDesc *get_tiles(int ty, int tz, int zc_minor, int *ntiles);
     long long nvalid() const { return vprefix.empty() ? 0 : vprefix.back(); }
 };
 
@@ -100,6 +101,16 @@ struct LevelState {
     double t_old = -DBL_MAX;   // time of grid_old
     int istep = 0;
     bool defined() const { return !L.boxes.empty(); }
+};
+
+/** Launch shape of the stage kernels.  A field of -1 (tz: 0) is not set:
+ *  the level's tuned value, else the default, applies. */
+struct LaunchCfg {
+    int variant = -1;     // tile shape index into kVariants
+    int tz = 0;           // z-chunk length of a tile (0: automatic)
+    int xcd_split = -1;   // each XCD walks one contiguous range of tiles
+    int persistent = -1;  // grid = resident capacity (1) or one tile per workgroup (0)
+    int zc_minor = -1;    // tile order: z-chunk outermost (0) or innermost (1)
 };
 
 struct Timing {
@@ -154,9 +165,9 @@ class Runtime {
 
     Timing timing;
     bool timing_on = false;
-    int variant = -1;   // tile variant, -1 = autotune/default
-    int tz = 0;         // z-chunk length, 0 = automatic
-    bool xcd_split = true;
+    LaunchCfg launch;   // user overrides of the launch shape
+    /** Launch shape used for level lev (overrides, tuned values, defaults). */
+    LaunchCfg resolved_launch(int lev) const;
 
   private:
     struct EventPair {
@@ -205,8 +216,7 @@ class Runtime {
     std::vector<hipEvent_t> prm_ev_;
     int prm_slot_ = 0;
     DevMem ko_dev_;
-    std::vector<int> tuned_variant_;
-    std::vector<int> tuned_tz_;
+    std::vector<LaunchCfg> tuned_;  // per level + 1 (index 0: shadow level)
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     // regrid/TE schedule of the time stepper
     std::vector<std::pair<int, double>> schedule_;

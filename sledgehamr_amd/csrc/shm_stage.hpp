@@ -249,18 +249,28 @@ __device__ AMREX_FORCE_INLINE double load_state_at(const void *hi,
  *  coefficient. */
 enum SsprkKind : int { kSsStage1 = 1, kSsStage2 = 2, kSsStage3 = 3, kSsFill = 4, kSsFillAdd = 5 };
 
+// Policy hooks used by the stage kernel:
+//   load_st(a, org, n, off)          stage-state value of component n
+//   load_keep(a, org, n, off, keep)  the same, and the nkeep() raw values
+//                                    the store reads at the centre cell
+//   load_ext(a, org, off, e)         next() further store-time inputs
+//   store(a, org, off, rhs, ctr, e, kc)  combine and write; kc[n] holds the
+//                                    kept raw values of component n
 template <int K>
 struct PolSsprk {
     static constexpr bool composed = (K == kSsStage2 || K == kSsStage3);
     template <class T>
     using Ext = typename T::S;
-    // Store-time inputs per component: stage 2: k_in; stage 3: k_in, u0,
-    // remainder; fill-add: previous rhs.
+    // Kept per component: stage 2: k_in; stage 3: k_in, u0.
+    template <class CS, class T>
+    static constexpr int nkeep() {
+        return K == kSsStage2 ? 1 : K == kSsStage3 ? 2 : 0;
+    }
+    // Loaded at store time per component: stage 3 remainder of u0; fill-add
+    // previous rhs.
     template <class CS, class T>
     static constexpr int next() {
-        return CS::NC * (K == kSsStage2 ? 1
-                         : K == kSsStage3 ? (T::comp ? 3 : 2)
-                         : K == kSsFillAdd ? 1 : 0);
+        return CS::NC * ((K == kSsStage3 && T::comp) || K == kSsFillAdd ? 1 : 0);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
@@ -274,10 +284,18 @@ struct PolSsprk {
             return ld<S>(a.x, b, off);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(const StageArgs &a,
-                                                               long long org, int n,
-                                                               unsigned off) {
-        return load_st<CS, T>(a, org, n, off);
+    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+        using S = typename T::S;
+        if constexpr (composed) {
+            const long long b = org + n * a.cstride;
+            const S k = ld<S>(a.kin, b, off), u = ld<S>(a.u0, b, off);
+            keep[0] = k;
+            if constexpr (K == kSsStage3) keep[1] = u;
+            return u + (S)a.c[2] * k;
+        } else {
+            return load_st<CS, T>(a, org, n, off);
+        }
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
@@ -288,30 +306,28 @@ struct PolSsprk {
 #pragma unroll
         for (int n = 0; n < CS::NC; ++n) {
             const long long b = org + n * a.cstride;
-            if constexpr (K == kSsStage2 || K == kSsStage3 || K == kSsFillAdd)
-                e[i++] = ld<S>(a.kin, b, off);
-            if constexpr (K == kSsStage3) {
-                e[i++] = ld<S>(a.u0, b, off);
-                if constexpr (T::comp) e[i++] = (S)load_lo<T>(a.u0lo, b, off);
-            }
+            if constexpr (K == kSsStage3 && T::comp)
+                e[i++] = (S)load_lo<T>(a.u0lo, b, off);
+            if constexpr (K == kSsFillAdd) e[i++] = ld<S>(a.kin, b, off);
         }
     }
-    template <class CS, class T>
+    template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
                                                     const double *ctr,
-                                                    const typename T::S *e) {
+                                                    const typename T::S *e,
+                                                    const typename T::S (*kc)[NK]) {
         using S = typename T::S;
         int i = 0;
 #pragma unroll
         for (int n = 0; n < CS::NC; ++n) {
             const long long b = org + n * a.cstride;
             double k = rhs[n];
-            if constexpr (K == kSsStage2 || K == kSsStage3 || K == kSsFillAdd)
-                k += a.c[0] * (double)e[i++];
+            if constexpr (composed) k += a.c[0] * (double)kc[n][0];
+            if constexpr (K == kSsFillAdd) k += a.c[0] * (double)e[i++];
             if constexpr (K == kSsStage3) {
-                double u0 = (double)e[i++];
+                double u0 = (double)kc[n][1];
                 if constexpr (T::comp) u0 += (double)e[i++];
                 store_state<T>(a.unew, a.unewlo, b, off, u0 + a.c[1] * k);
             } else {
@@ -326,75 +342,75 @@ struct PolSsprk {
  *  stay at pi0.  Non-final stages write F (the momentum right-hand sides);
  *  the final stage writes phi = phi0 + c6*pi0 + sum_j c[7+j] F_j and
  *  pi = pi0 + sum_j c[12+j] F_j, with this stage's F (weights c[11], c[16])
- *  taken from registers. */
+ *  taken from registers.  The final stage keeps phi0, pi0 and the F_j that
+ *  formed a field's stage state for its own update. */
 template <int NPREV, bool FINAL>
 struct PolRkn {
     static_assert(NPREV >= 0 && NPREV <= 4, "RKN supports up to 5 stages");
     template <class T>
     using Ext = typename T::S;
-    // Final stage, per pair: phi0 and pi0 (with remainders) and F_j.
+    template <class CS, class T>
+    static constexpr int nkeep() {
+        return FINAL ? 2 + NPREV : 0;
+    }
+    // Final stage, per pair: the remainders of phi0 and pi0.
     template <class CS, class T>
     static constexpr int next() {
-        return FINAL ? CS::NPAIR * (2 * (T::comp ? 2 : 1) + NPREV) : 0;
+        return FINAL && T::comp ? CS::NPAIR * 2 : 0;
     }
 
     /** Stage state in the storage precision (FP32 states are FP32 values,
      *  so their sums are formed in FP32). */
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S state(const StageArgs &a,
-                                                             long long org, int n,
-                                                             unsigned off) {
+    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
         using S = typename T::S;
         const int p = CS::pair_of_field(n);
-        if (p < 0) return ld<S>(a.u0, org + n * a.cstride, off);
-        S y = ld<S>(a.u0, org + n * a.cstride, off);
-        y += (S)a.c[0] * ld<S>(a.u0, org + CS::pair_mom(p) * a.cstride, off);
+        const S phi0 = ld<S>(a.u0, org + n * a.cstride, off);
+        if (p < 0) return phi0;
+        const S pi0 = ld<S>(a.u0, org + CS::pair_mom(p) * a.cstride, off);
+        S y = phi0;
+        y += (S)a.c[0] * pi0;
+        if constexpr (FINAL) {
+            keep[0] = phi0;
+            keep[1] = pi0;
+        }
 #pragma unroll
-        for (int j = 0; j < NPREV; ++j)
-            y += (S)a.c[1 + j] * ld<S>(a.f[j], org + p * a.cstride, off);
+        for (int j = 0; j < NPREV; ++j) {
+            const S fj = ld<S>(a.f[j], org + p * a.cstride, off);
+            if constexpr (FINAL) keep[2 + j] = fj;
+            y += (S)a.c[1 + j] * fj;
+        }
         return y;
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        return state<CS, T>(a, org, n, off);
-    }
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(const StageArgs &a,
-                                                               long long org, int n,
-                                                               unsigned off) {
-        return state<CS, T>(a, org, n, off);
+        typename T::S keep[FINAL ? 2 + NPREV : 1];
+        return load_keep<CS, T>(a, org, n, off, keep);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
                                                        long long org, unsigned off,
                                                        typename T::S *e) {
         using S = typename T::S;
-        if constexpr (FINAL) {
+        if constexpr (FINAL && T::comp) {
             int i = 0;
 #pragma unroll
             for (int p = 0; p < CS::NPAIR; ++p) {
-                const long long bf = org + CS::pair_field(p) * a.cstride;
-                const long long bm = org + CS::pair_mom(p) * a.cstride;
-                e[i++] = ld<S>(a.u0, bf, off);
-                e[i++] = ld<S>(a.u0, bm, off);
-                if constexpr (T::comp) {
-                    e[i++] = (S)load_lo<T>(a.u0lo, bf, off);
-                    e[i++] = (S)load_lo<T>(a.u0lo, bm, off);
-                }
-#pragma unroll
-                for (int j = 0; j < NPREV; ++j)
-                    e[i++] = ld<S>(a.f[j], org + p * a.cstride, off);
+                e[i++] = (S)load_lo<T>(a.u0lo, org + CS::pair_field(p) * a.cstride, off);
+                e[i++] = (S)load_lo<T>(a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
             }
         }
     }
-    template <class CS, class T>
+    template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
                                                     const double *ctr,
-                                                    const typename T::S *e) {
+                                                    const typename T::S *e,
+                                                    const typename T::S (*kc)[NK]) {
         using S = typename T::S;
         int i = 0;
 #pragma unroll
@@ -405,8 +421,8 @@ struct PolRkn {
                 st<S>(a.fout, org + p * a.cstride, off, (S)fi);
             } else {
                 const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
-                double phi = (double)e[i++];
-                double pi0 = (double)e[i++];
+                double phi = (double)kc[nf][0];
+                double pi0 = (double)kc[nf][1];
                 if constexpr (T::comp) {
                     phi += (double)e[i++];
                     pi0 += (double)e[i++];
@@ -415,7 +431,7 @@ struct PolRkn {
                 double pi = pi0;
 #pragma unroll
                 for (int j = 0; j < NPREV; ++j) {
-                    const double fj = (double)e[i++];
+                    const double fj = (double)kc[nf][2 + j];
                     phi += a.c[7 + j] * fj;
                     pi += a.c[12 + j] * fj;
                 }
@@ -437,6 +453,8 @@ struct PolLeapfrog {
     template <class T>
     using Ext = double;
     template <class CS, class T>
+    static constexpr int nkeep() { return 0; }
+    template <class CS, class T>
     static constexpr int next() { return CS::NPAIR * (KICK2 ? 1 : 2); }
 
     template <class CS, class T>
@@ -446,10 +464,9 @@ struct PolLeapfrog {
         return ld<typename T::S>(a.x, org + n * a.cstride, off);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(const StageArgs &a,
-                                                               long long org, int n,
-                                                               unsigned off) {
-        return ld<typename T::S>(a.x, org + n * a.cstride, off);
+    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *) {
+        return load_st<CS, T>(a, org, n, off);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
@@ -466,12 +483,13 @@ struct PolLeapfrog {
             }
         }
     }
-    template <class CS, class T>
+    template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
                                                     const double *ctr,
-                                                    const double *e) {
+                                                    const double *e,
+                                                    const typename T::S (*)[NK]) {
         int i = 0;
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
@@ -625,11 +643,23 @@ __global__ void __launch_bounds__(64 * NW)
 
         using E = typename Pol::template Ext<T>;
         constexpr int NE = Pol::template next<CS, T>();
+        constexpr int NK = Pol::template nkeep<CS, T>();
+        constexpr int NKA = NK > 0 ? NK : 1;
+        // Raw values a stencil component's store reads at the centre (k_in,
+        // u0, earlier RKN forces) were loaded R+1 planes earlier for the
+        // queue head; when the registers allow they ride along in a delay
+        // line instead of being read again at store time, when they have
+        // left L2.  Otherwise they are reloaded with the point values.
+        constexpr bool DELAY =
+            NK > 0 && NS * RY * (R + 2) * NK * (int)(sizeof(S) / 4) <= 64;
         S q[NS][RY][Q];
         S nq[NS][RY];
         S hv[NS][HPT];
         S pv[NP > 0 ? NP : 1][RY];
         E ev[RY][NE > 0 ? NE : 1];  // store-time inputs of the next plane
+        S kq[NS][RY][DELAY ? R + 1 : 1][NKA];  // kept values, planes z..z+R
+        S nk[NS][RY][NKA];  // kept values of the queue head / of plane z+1
+        S pk[NP > 0 ? NP : 1][RY][NKA];  // kept values of point components
 
         // Prologue: planes z0-R .. z0+R-1 into q[..][1..2R], plane z0+R into nq.
 #pragma unroll
@@ -637,12 +667,22 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
 #pragma unroll
-                for (int m = 1; m < Q; ++m)
-                    q[s][r][m] = Pol::template load_st<CS, T>(
-                        a, org, CS::comp_of_slot(s),
-                        qoff[r] + zoff(td.z0 - R + m - 1));
-                nq[s][r] = Pol::template load_st<CS, T>(
-                    a, org, CS::comp_of_slot(s), qoff[r] + zoff(td.z0 + R));
+                for (int m = 1; m < Q; ++m) {
+                    const unsigned o = qoff[r] + zoff(td.z0 - R + m - 1);
+                    if (DELAY && m > R)
+                        q[s][r][m] = Pol::template load_keep<CS, T>(
+                            a, org, CS::comp_of_slot(s), o, kq[s][r][DELAY ? m - R : 0]);
+                    else
+                        q[s][r][m] = Pol::template load_st<CS, T>(
+                            a, org, CS::comp_of_slot(s), o);
+                }
+                const unsigned o = qoff[r] + zoff(td.z0 + R);
+                if constexpr (DELAY)
+                    nq[s][r] = Pol::template load_keep<CS, T>(
+                        a, org, CS::comp_of_slot(s), o, nk[s][r]);
+                else
+                    nq[s][r] = Pol::template load_st<CS, T>(
+                        a, org, CS::comp_of_slot(s), o);
             }
         auto load_halo = [&](int z) {
             const unsigned zo = zoff(z);
@@ -659,8 +699,16 @@ __global__ void __launch_bounds__(64 * NW)
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int r = 0; r < RY; ++r)
-                    pv[p][r] = Pol::template load_pt<CS, T>(
-                        a, org, CS::comp_of_point(p), qoff[r] + zo);
+                    pv[p][r] = Pol::template load_keep<CS, T>(
+                        a, org, CS::comp_of_point(p), qoff[r] + zo, pk[p][r]);
+            if constexpr (NK > 0 && !DELAY) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int r = 0; r < RY; ++r)
+                        (void)Pol::template load_keep<CS, T>(
+                            a, org, CS::comp_of_slot(s), qoff[r] + zo, nk[s][r]);
+            }
             if constexpr (NE > 0) {
 #pragma unroll
                 for (int r = 0; r < RY; ++r)
@@ -680,6 +728,16 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
                     for (int m = 0; m < Q - 1; ++m) q[s][r][m] = q[s][r][m + 1];
                     q[s][r][Q - 1] = nq[s][r];
+                    if constexpr (NK > 0) {
+                        constexpr int D = DELAY ? R : 0;
+#pragma unroll
+                        for (int m = 0; m < D; ++m)
+#pragma unroll
+                            for (int j = 0; j < NK; ++j)
+                                kq[s][r][m][j] = kq[s][r][m + 1][j];
+#pragma unroll
+                        for (int j = 0; j < NK; ++j) kq[s][r][D][j] = nk[s][r][j];
+                    }
                 }
             // Stage plane z: own centres and the halo ring.
 #pragma unroll
@@ -693,10 +751,15 @@ __global__ void __launch_bounds__(64 * NW)
                         (&lds[buf][s][0][0])[hl[m]] = hv[s][m];
             }
             S pc[NP > 0 ? NP : 1][RY];
+            S pkc[NP > 0 ? NP : 1][RY][NKA];
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int r = 0; r < RY; ++r) pc[p][r] = pv[p][r];
+                for (int r = 0; r < RY; ++r) {
+                    pc[p][r] = pv[p][r];
+#pragma unroll
+                    for (int j = 0; j < NK; ++j) pkc[p][r][j] = pk[p][r][j];
+                }
             E ec[RY][NE > 0 ? NE : 1];
 #pragma unroll
             for (int r = 0; r < RY; ++r)
@@ -709,9 +772,14 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
 #pragma unroll
-                    for (int r = 0; r < RY; ++r)
-                        nq[s][r] = Pol::template load_st<CS, T>(
-                            a, org, CS::comp_of_slot(s), qoff[r] + zq);
+                    for (int r = 0; r < RY; ++r) {
+                        if constexpr (DELAY)
+                            nq[s][r] = Pol::template load_keep<CS, T>(
+                                a, org, CS::comp_of_slot(s), qoff[r] + zq, nk[s][r]);
+                        else
+                            nq[s][r] = Pol::template load_st<CS, T>(
+                                a, org, CS::comp_of_slot(s), qoff[r] + zq);
+                    }
                 load_halo(z + 1);
                 load_point(z + 1);
             }
@@ -777,7 +845,19 @@ __global__ void __launch_bounds__(64 * NW)
                                 sv, gi, gj, gk, n, dx, a.ko_strength[n]);
                     }
                 }
-                Pol::template store<CS, T>(a, org, coff[r] + zc, rhs, ctr, ec[r]);
+                S kc[NC][NKA];
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int j = 0; j < NK; ++j)
+                        kc[CS::comp_of_slot(s)][j] = kq[s][r][0][j];
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+#pragma unroll
+                    for (int j = 0; j < NK; ++j)
+                        kc[CS::comp_of_point(p)][j] = pkc[p][r][j];
+                Pol::template store<CS, T, NKA>(a, org, coff[r] + zc, rhs, ctr,
+                                                ec[r], kc);
             }
         }
     }
