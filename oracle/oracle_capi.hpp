@@ -175,4 +175,24 @@
             else                                                               \
                 throw std::runtime_error("unknown initial-state kind");        \
         });                                                                    \
+    }                                                                          \
+    /* Spectral building blocks, exposed for their own identity tests. */     \
+    ORC_EXPORT void orc_r2c(int N, const double *in, double *out) {            \
+        std::vector<double> v(in, in + (size_t)N * N * N);                     \
+        std::vector<std::complex<double>> c;                                   \
+        orc::r2c(v, N, c);                                                     \
+        std::memcpy(out, c.data(), c.size() * sizeof(c[0]));                   \
+    }                                                                          \
+    ORC_EXPORT double orc_gw_lambda(int i, int j, int l, int m, int a, int b,  \
+                                    int c, int N, int type) {                  \
+        std::vector<double> itk(N);                                            \
+        for (int q = 0; q < N; ++q) itk[q] = orc::index_to_k(q, N, type);      \
+        const int abc[3] = {a, b, c};                                          \
+        return orc::gw_lambda(i, j, l, m, abc, itk.data());                    \
+    }                                                                          \
+    ORC_EXPORT int orc_unique_k2(int N, long long *out, int maxn) {            \
+        const std::vector<long long> v = orc::unique_k2(N);                    \
+        if (out && maxn >= (int)v.size())                                      \
+            std::memcpy(out, v.data(), v.size() * sizeof(long long));          \
+        return (int)v.size();                                                  \
     }

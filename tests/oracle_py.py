@@ -49,6 +49,9 @@ def _load(project: str) -> C.CDLL:
         "orc_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "orc_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "orc_init_state": (i, [vp, i, i, C.c_ulonglong, dp, i]),
+        "orc_r2c": (None, [i, dp, dp]),
+        "orc_gw_lambda": (d, [i, i, i, i, i, i, i, i, i]),
+        "orc_unique_k2": (i, [i, llp, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -158,3 +161,32 @@ class Oracle:
 
     def index_to_k(self, a, N, ptype):
         return self.lib.orc_index_to_k(a, N, ptype)
+
+
+def spectral_lib(project: str = "PhaseTransitionGW") -> C.CDLL:
+    """The oracle library of a project, for its project-independent spectral
+    helpers (r2c, Lambda, unique |k|^2)."""
+    return _load(project)
+
+
+def oracle_r2c(a: np.ndarray) -> np.ndarray:
+    """The oracle's R2C DFT of a real N^3 array indexed [z, y, x]; returns
+    complex [kz, ky, kx] with kx = 0..N/2."""
+    lib = spectral_lib()
+    N = a.shape[0]
+    src = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.zeros(N * N * (N // 2 + 1) * 2)
+    lib.orc_r2c(N, _dp(src), _dp(out))
+    return out.view(np.complex128).reshape(N, N, N // 2 + 1)
+
+
+def oracle_lambda(i, j, l, m, abc, N, ptype) -> float:
+    return spectral_lib().orc_gw_lambda(i, j, l, m, abc[0], abc[1], abc[2], N, ptype)
+
+
+def oracle_unique_k2(N: int) -> np.ndarray:
+    lib = spectral_lib()
+    n = lib.orc_unique_k2(N, None, 0)
+    out = np.zeros(n, dtype=np.int64)
+    lib.orc_unique_k2(N, out.ctypes.data_as(C.POINTER(C.c_longlong)), n)
+    return out

@@ -1,0 +1,408 @@
+"""Pins of the FP64 CPU oracle (no GPU).
+
+Every check derives its expected value independently of the oracle's code:
+polynomial exactness and plane-wave symbols of the central difference
+stencils, closed-form amplification matrices and convergence orders of the
+integrators, hand-built truncation-error blocks, numpy's FFT, the algebraic
+identities of the transverse-traceless projector, plane-wave spectra, and
+the reference's own k-bin tables (tests/golden/spectra_ks.json, extracted
+from data/spectra_ks.hdf5).
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from cases import phase_transition, real_scalar
+from oracle_py import (Oracle, oracle_lambda, oracle_r2c, oracle_unique_k2,
+                       spectral_lib)
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------------------
+# Stencils (sledgehamr_utils.h:50-188)
+# ---------------------------------------------------------------------------
+
+def _grid(N, dx, shift):
+    """Cell coordinates (x fastest, flattened [z][y][x])."""
+    c = (np.arange(N) - shift) * dx
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    return x, y, z
+
+
+def _lap_probe(N, order, phi, dx=1.0):
+    """rhs(dPhi) of RealScalar with lambda = 0 is exactly Laplacian<order>."""
+    orc = Oracle("RealScalar", real_scalar(N=N, order=order, lam=0.0, L=N * dx,
+                                           nghost=order))
+    orc.upload(0, 0, phi.ravel())
+    orc.upload(0, 1, np.zeros(N ** 3))
+    orc.fill_rhs(0, 0.0)
+    return orc.rhs(0, 1).reshape(N, N, N)
+
+
+def _interior(N, r):
+    s = slice(r, N - r)
+    return (s, s, s)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_laplacian_polynomial_exact(order):
+    """A central 2p-order second difference is exact on polynomials whose
+    restriction to each axis has degree <= 2p + 1; checked away from the
+    periodic seam, with an analytic Laplacian."""
+    N, deg = 20, 2 * order + 1
+    x, y, z = _grid(N, 0.5, N / 2)
+    rng = np.random.default_rng(order)
+    phi = np.zeros_like(x)
+    lap = np.zeros_like(x)
+    for a, b, c in itertools.product(range(deg + 1), repeat=3):
+        if max(a, b, c) > deg or a + b + c > deg + 2:
+            continue
+        w = rng.uniform(-1, 1)
+        phi += w * x ** a * y ** b * z ** c
+        if a >= 2:
+            lap += w * a * (a - 1) * x ** (a - 2) * y ** b * z ** c
+        if b >= 2:
+            lap += w * b * (b - 1) * x ** a * y ** (b - 2) * z ** c
+        if c >= 2:
+            lap += w * c * (c - 1) * x ** a * y ** b * z ** (c - 2)
+    got = _lap_probe(N, order, phi, dx=0.5)
+    sl = _interior(N, order)
+    err = np.max(np.abs(got[sl] - lap[sl])) / np.max(np.abs(lap[sl]))
+    assert err < 1e-10, err
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_laplacian_polynomial_negative_control(order):
+    """One degree higher the stencil is no longer exact (the check above
+    would catch a stencil of lower order)."""
+    N, deg = 20, 2 * order + 2
+    x, y, z = _grid(N, 0.5, N / 2)
+    phi = x ** deg
+    lap = deg * (deg - 1) * x ** (deg - 2)
+    got = _lap_probe(N, order, phi, dx=0.5)
+    sl = _interior(N, order)
+    assert np.max(np.abs(got[sl] - lap[sl])) > 1e-3
+
+
+def _lap_symbol(order, th):
+    """Per-axis eigenvalue (times dx^2, negated) of the central second
+    difference of order 2p on exp(i th n)."""
+    c = np.cos
+    if order == 1:
+        return 2 - 2 * c(th)
+    if order == 2:
+        return (30 - 32 * c(th) + 2 * c(2 * th)) / 12
+    return (490 - 540 * c(th) + 54 * c(2 * th) - 4 * c(3 * th)) / 180
+
+
+def _grad_symbol(order, th):
+    """Eigenvalue (times dx, over i) of the central first difference."""
+    s = np.sin
+    if order == 1:
+        return s(th)
+    if order == 2:
+        return (8 * s(th) - s(2 * th)) / 6
+    return (45 * s(th) - 9 * s(2 * th) + s(3 * th)) / 30
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("m", [(1, 0, 0), (2, 3, 1), (5, 7, 3)])
+def test_laplacian_plane_wave_symbol(order, m):
+    N, dx = 16, 0.25
+    n = np.arange(N)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    arg = 2 * np.pi * (m[0] * i + m[1] * j + m[2] * k) / N + 0.3
+    phi = np.cos(arg)
+    th = [2 * np.pi * q / N for q in m]
+    lam = -sum(_lap_symbol(order, t) for t in th) / dx ** 2
+    got = _lap_probe(N, order, phi, dx=dx)
+    assert np.max(np.abs(got - lam * phi)) < 1e-11 * max(abs(lam), 1)
+
+
+def test_gradient_plane_wave_symbol():
+    """du_ij source of PhaseTransitionGW with u = 0 is d_i Phi d_j Phi
+    through Gradient<3>: a plane wave gives the first-difference symbol."""
+    N, dx = 16, 0.5
+    m = (3, 1, 2)
+    orc = Oracle("PhaseTransitionGW", phase_transition(N=N, L=N * dx))
+    n = np.arange(N)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    arg = 2 * np.pi * (m[0] * i + m[1] * j + m[2] * k) / N + 0.1
+    orc.upload(0, 0, np.sin(arg).ravel())
+    for c in range(1, orc.ncomp):
+        orc.upload(0, c, np.zeros(N ** 3))
+    orc.fill_rhs(0, 0.0)
+    g = [_grad_symbol(3, 2 * np.pi * q / N) / dx * np.cos(arg) for q in m]
+    # du_xx, du_yy, du_zz, du_xy, du_xz, du_yz follow the six u_ij.
+    want = {8: g[0] * g[0], 9: g[1] * g[1], 10: g[2] * g[2], 11: g[0] * g[1],
+            12: g[0] * g[2], 13: g[1] * g[2]}
+    for comp, w in want.items():
+        got = orc.rhs(0, comp).reshape(N, N, N)
+        assert np.max(np.abs(got - w)) < 1e-12, comp
+
+
+def test_gradient_polynomial_exact():
+    """Gradient<3> is exact on polynomials of degree <= 6 per axis."""
+    N, dx = 20, 0.5
+    x, y, z = _grid(N, dx, N / 2)
+    orc = Oracle("PhaseTransitionGW", phase_transition(N=N, L=N * dx))
+    phi = 0.01 * (x ** 6 - 2 * x ** 3 * y ** 2 + y ** 5 * z + z ** 4)
+    gx = 0.01 * (6 * x ** 5 - 6 * x ** 2 * y ** 2)
+    gy = 0.01 * (-4 * x ** 3 * y + 5 * y ** 4 * z)
+    gz = 0.01 * (y ** 5 + 4 * z ** 3)
+    orc.upload(0, 0, phi.ravel())
+    for c in range(1, orc.ncomp):
+        orc.upload(0, c, np.zeros(N ** 3))
+    orc.fill_rhs(0, 0.0)
+    sl = _interior(N, 3)
+    for comp, w in {8: gx * gx, 11: gx * gy, 13: gy * gz, 10: gz * gz}.items():
+        got = orc.rhs(0, comp).reshape(N, N, N)
+        assert np.max(np.abs(got[sl] - w[sl])) < 1e-9 * np.max(np.abs(w[sl]))
+
+
+# ---------------------------------------------------------------------------
+# Integrators (lsssprk3.cpp:14-31, rkn.cpp:25-205, leapfrog.cpp:13-57)
+# ---------------------------------------------------------------------------
+
+def _mode_case(integrator, cfl, N=16, m=2, order=1):
+    """RealScalar with lambda = 0 and one Fourier mode along x: each cell
+    obeys phi'' = -w^2 phi with w^2 the discrete Laplacian eigenvalue."""
+    dx = 1.0
+    orc = Oracle("RealScalar", real_scalar(N=N, order=order, lam=0.0, L=N * dx,
+                                           cfl=cfl, integrator=integrator))
+    n = np.arange(N)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    shape = np.cos(2 * np.pi * m * i / N + 0.2).ravel()
+    w2 = _lap_symbol(order, 2 * np.pi * m / N) / dx ** 2
+    return orc, shape, w2, cfl * dx
+
+
+def _one_step(integrator, phi0, pi0, cfl):
+    orc, shape, w2, h = _mode_case(integrator, cfl)
+    orc.upload(0, 0, phi0 * shape)
+    orc.upload(0, 1, pi0 * shape)
+    orc.advance(0, 1)
+    i0 = int(np.argmax(np.abs(shape)))
+    return (orc.download(0, 0)[i0] / shape[i0], orc.download(0, 1)[i0] / shape[i0],
+            w2, h)
+
+
+def test_ssprk3_amplification_matrix():
+    """Any three-stage third-order RK on y' = A y gives
+    I + hA + (hA)^2/2 + (hA)^3/6."""
+    for phi0, pi0 in ((1.0, 0.0), (0.0, 1.0)):
+        p, q, w2, h = _one_step(10, phi0, pi0, cfl=0.3)
+        A = np.array([[0, 1], [-w2, 0]]) * h
+        M = np.eye(2) + A + A @ A / 2 + A @ A @ A / 6
+        want = M @ np.array([phi0, pi0])
+        assert abs(p - want[0]) < 1e-14 and abs(q - want[1]) < 1e-14
+
+
+def test_leapfrog_amplification_matrix():
+    """Kick-drift-kick on phi'' = -w^2 phi."""
+    for phi0, pi0 in ((1.0, 0.0), (0.0, 1.0)):
+        p, q, w2, h = _one_step(11, phi0, pi0, cfl=0.3)
+        ph = pi0 - h / 2 * w2 * phi0
+        p1 = phi0 + h * ph
+        q1 = ph - h / 2 * w2 * p1
+        assert abs(p - p1) < 1e-14 and abs(q - q1) < 1e-14
+
+
+@pytest.mark.parametrize("integrator,order", [(10, 3), (11, 2), (21, 4), (22, 5)])
+def test_convergence_order(integrator, order):
+    """Global error at a fixed time against the exact cos/sin solution
+    falls as h^order."""
+    errs = []
+    for cfl in (0.2, 0.1):
+        orc, shape, w2, h = _mode_case(integrator, cfl)
+        w = np.sqrt(w2)
+        orc.upload(0, 0, shape)
+        orc.upload(0, 1, np.zeros_like(shape))
+        steps = int(round(4.0 / h))
+        orc.advance(0, steps)
+        t = steps * h
+        i0 = int(np.argmax(np.abs(shape)))
+        e = abs(orc.download(0, 0)[i0] / shape[i0] - np.cos(w * t))
+        e = max(e, abs(orc.download(0, 1)[i0] / shape[i0] + w * np.sin(w * t)))
+        errs.append(e)
+    rate = np.log2(errs[0] / errs[1])
+    assert abs(rate - order) < 0.35, (errs, rate)
+
+
+def test_ssprk3_matches_shu_osher_form():
+    """Low-storage SSPRK3 equals u0 + dt/6 (k1 + k2 + 4 k3) with
+    k1 = f(u0), k2 = f(u0 + dt k1), k3 = f(u0 + dt/4 (k1 + k2)), on a
+    nonlinear problem (lambda = 1)."""
+    N = 8
+    inp = real_scalar(N=N, order=2, lam=1.0, L=N * 0.5, cfl=0.4, integrator=10)
+    rng = np.random.default_rng(5)
+    u0 = [rng.uniform(-1, 1, N ** 3), rng.uniform(-1, 1, N ** 3)]
+    probe = Oracle("RealScalar", inp)
+
+    def f(u):
+        for c in range(2):
+            probe.upload(0, c, u[c])
+        probe.fill_rhs(0, 0.0)
+        return [probe.rhs(0, c) for c in range(2)]
+
+    dt = 0.4 * 0.5
+    k1 = f(u0)
+    k2 = f([u0[c] + dt * k1[c] for c in range(2)])
+    k3 = f([u0[c] + dt / 4 * (k1[c] + k2[c]) for c in range(2)])
+    want = [u0[c] + dt / 6 * (k1[c] + k2[c] + 4 * k3[c]) for c in range(2)]
+    orc = Oracle("RealScalar", inp)
+    for c in range(2):
+        orc.upload(0, c, u0[c])
+    orc.advance(0, 1)
+    for c in range(2):
+        assert np.max(np.abs(orc.download(0, c) - want[c])) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# Truncation errors and tagging (kernels.h:18-42, macros.h:489-566)
+# ---------------------------------------------------------------------------
+
+def test_te_tag_hand_built_blocks():
+    """te = |coarse - mean of the 8 children|; a block passes when
+    te >= te_crit for any component.  The reference's CPU loop SETs the 2x2x2
+    block at the even cell and re-CLEARs the 7 odd cells when it visits them
+    later (macros.h:504-527), so the mask holds the even corner only, while
+    the counters add 8 per passing component and 8 once to the total."""
+    N, crit = 8, 0.5
+    inp = real_scalar(N=N, order=1, lam=0.0, L=float(N), **{"amr.te_crit": crit})
+    orc = Oracle("RealScalar", inp)
+    nc3 = (N // 2) ** 3
+    crse_phi = np.zeros(nc3)
+    crse_pi = np.zeros(nc3)
+    eps = 2.0 ** -50
+    crse_phi[0] = crit                      # at the threshold: tagged
+    crse_phi[1] = crit * (1 - eps)          # just below: not tagged
+    crse_phi[2] = -crit * (1 + eps)         # just above (negative): tagged
+    crse_pi[2] = 2 * crit                   # second component, same block
+    crse_pi[5] = crit                       # second component only
+    crse_phi[7] = 0.75                      # children average 0.25 -> te 0.5
+    # coarse cell 7 = (ci, cj, ck) = (3, 1, 0): fine children x 6..7,
+    # y 2..3, z 0..1 (arrays are [z, y, x]).
+    fine = np.zeros((N, N, N))
+    fine[0:2, 2:4, 6:8] = 0.25
+    orc.upload(0, 0, fine.ravel())
+    orc.upload(0, 1, np.zeros(N ** 3))
+    orc.upload(-1, 0, crse_phi)
+    orc.upload(-1, 1, crse_pi)
+    mask, counts, te = orc.te_tag(0, flags=1 | 4, want_te=True)
+
+    assert te[0][7] == 0.5 and te[0][0] == crit and te[1][5] == crit
+    tagged = {0, 2, 5, 7}
+    want = np.zeros((N, N, N), dtype=np.uint8)
+    for c in tagged:
+        ci, cj, ck = c % 4, (c // 4) % 4, c // 16
+        want[2 * ck, 2 * cj, 2 * ci] = 1
+    assert np.array_equal(mask.reshape(N, N, N), want)
+    assert counts[0] == 8 * len(tagged)            # total
+    assert counts[1] == 0                          # user tags
+    assert counts[2] == 8 * 3                      # Phi: blocks 0, 2, 7
+    assert counts[3] == 8 * 2                      # dPhi: blocks 2, 5
+
+
+# ---------------------------------------------------------------------------
+# GW spectrum (gravitational_waves.cpp:52-322, gravitational_waves.h:101-133)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("N", [6, 8, 12])
+def test_r2c_matches_numpy(N):
+    rng = np.random.default_rng(N)
+    a = rng.standard_normal((N, N, N))
+    got = oracle_r2c(a)
+    want = np.fft.rfftn(a)
+    assert np.max(np.abs(got - want)) < 1e-11 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("ptype", [1, 2, 3])
+def test_lambda_projector_identities(ptype):
+    N = 16
+    rng = np.random.default_rng(ptype)
+    for _ in range(12):
+        abc = tuple(int(v) for v in rng.integers(0, N, 3))
+        if abc == (0, 0, 0):
+            continue
+        Lm = np.zeros((3, 3, 3, 3))
+        for i, j, l, m in itertools.product(range(3), repeat=4):
+            Lm[i, j, l, m] = oracle_lambda(i, j, l, m, abc, N, ptype)
+        d = np.array([_grad_symbol(ptype, 2 * np.pi * (a - N if a > N // 2 - 1 else a) / N)
+                      for a in abc])
+        if np.linalg.norm(d) < 1e-12:
+            continue
+        tol = 1e-13
+        assert np.max(np.abs(np.einsum("i,ijlm->jlm", d, Lm))) < tol   # transverse
+        assert np.max(np.abs(np.einsum("iilm->lm", Lm))) < tol         # traceless
+        assert np.max(np.abs(Lm - Lm.transpose(2, 3, 0, 1))) < tol     # symmetric
+        P = np.einsum("ijlm,lmpq->ijpq", Lm, Lm)
+        assert np.max(np.abs(P - Lm)) < 1e-12                          # projector
+    assert oracle_lambda(0, 1, 0, 1, (0, 0, 0), N, ptype) == 0.0
+
+
+def _plane_wave_spectrum(comp, axis, m, N=16, mode=1, nbins=8):
+    orc = Oracle("PhaseTransitionGW", phase_transition(N=N, L=float(N)))
+    n = np.arange(N)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    x = (i, j, k)[axis]
+    for c in range(orc.ncomp):
+        orc.upload(0, c, np.zeros(N ** 3))
+    orc.upload(0, comp, np.cos(2 * np.pi * m * x / N).ravel())
+    return orc.gw_spectrum(mode=mode, nbins=nbins, zero_padding=1, projection_type=3)
+
+
+@pytest.mark.parametrize("comp,axis,want", [
+    (11, 2, 1.0),    # du_xy, k along z: fully transverse-traceless
+    (8, 2, 0.25),    # du_xx, k along z: TT part diag(1/2, -1/2, 0)
+    (10, 2, 0.0),    # du_zz, k along z: longitudinal
+    (13, 0, 1.0),    # du_yz, k along x (halved axis, multiplicity 2)
+    (12, 1, 1.0),    # du_xz, k along y: transverse-traceless
+    (11, 1, 0.0),    # du_xy, k along y: mixed longitudinal
+])
+def test_plane_wave_spectrum(comp, axis, want):
+    m = 3
+    spec, kv = _plane_wave_spectrum(comp, axis, m)
+    assert abs(spec[m] - want) < 1e-12
+    others = np.delete(spec, m)
+    assert np.max(np.abs(others)) < 1e-12
+    assert kv[m] == m * m
+
+
+def test_spectrum_modes_conserve_total():
+    spec1, _ = _plane_wave_spectrum(12, 1, 5, mode=1)
+    spec0, k0 = _plane_wave_spectrum(12, 1, 5, mode=0)
+    spec2, _ = _plane_wave_spectrum(12, 1, 5, mode=2, nbins=16)
+    assert abs(spec0.sum() - 1.0) < 1e-12 and abs(spec2.sum() - 1.0) < 1e-12
+    assert abs(spec0[int(np.searchsorted(k0, 25))] - 1.0) < 1e-12
+    assert abs(spec1.sum() - 1.0) < 1e-12
+
+
+def _golden():
+    with open(os.path.join(GOLDEN, "spectra_ks.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("N", [32, 64, 128])
+def test_index_to_k_matches_reference_tables(N):
+    g = _golden()[str(N)]
+    lib = spectral_lib()
+    for ptype, key in ((1, "k1"), (2, "k2")):
+        got = np.array([lib.orc_index_to_k(a, N, ptype) for a in range(N)])
+        assert np.max(np.abs(got - np.array(g[key]))) < 1e-15, key
+
+
+@pytest.mark.parametrize("N", [32, 64, 128, 256])
+def test_unique_k2_matches_reference_tables(N):
+    g = _golden()[str(N)]
+    got = oracle_unique_k2(N)
+    assert len(got) == g["nks"] == g["nbins"]
+    assert int(got.sum()) == g["bins_sum"] and int(got[-1]) == g["bins_last"]
+    if "bins" in g:
+        assert np.array_equal(got, np.array(g["bins"], dtype=np.int64))
