@@ -15,15 +15,19 @@ HBM.  The same seeded run on the FP64 path gives `fp64` and the field error;
 
 roofline: the fused stage kernel's algorithmic bytes per launch over its mean
 launch time, measured with HIP events on the library's stream inside the
-timed region.  Algorithmic bytes: every stored array read once and written
-at most once per stage, stage states rebuilt from u0 and k (8 words per
-component and step, plus one read and one write of the remainder): 48 B per
-cell-stage for FP32 + bf16 remainder, 53.3 B with an FP32 remainder, 85.3 B
-for FP64 (C = 4).
+timed region.  Algorithmic bytes per cell-stage are those of the minimal
+fused low-storage schedule (every stored array read once and written at most
+once per stage: 12 words per component and step, plus one read and one write
+of the remainder per step): 69.3 B for FP32 + bf16 remainder, 74.7 B with an
+FP32 remainder, 128 B for FP64 (C = 4).  This schedule rebuilds the SSPRK3
+stage states from u0 and k instead of storing them (8 words per component and
+step): the bytes it actually moves, 48 / 53.3 / 85.3 B per cell-stage, give
+`moved_frac`.  `traffic` is the HBM bytes per launch measured by rocprofv3
+counters (DRAM read and write requests of the same stage launches).
 
 cpu_baseline: the FP64 oracle in the reference's unfused structure (Rhs pass,
-LinComb passes, ghost fill per stage; OpenMP), one step of the same 512^3
-workload on the host cores.
+LinComb passes, ghost fill per stage; OpenMP), whole SSPRK3 steps of the same
+512^3 workload on the host cores after one warm-up step.
 """
 import argparse
 import json
@@ -40,14 +44,24 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 METRIC = ("cell-updates/s per RK stage + % HBM roofline; FP32-comp vs FP64 "
           "at ≤1e-6 rel err")
-# Algorithmic bytes per cell-stage of the fused low-storage SSPRK3 schedule
-# with C = 4 components.  Per component and step: stage 1 reads u0, writes k1;
-# stage 2 reads u0, k1, writes k2; stage 3 reads u0, k2, writes u (8 words),
-# plus one read and one write of the remainder for the compensated paths.
+# Algorithmic bytes per cell-stage, C = 4 components, SSPRK3: 12 words per
+# component and step (each stored array read once, written at most once per
+# stage), plus one read and one write of the remainder per step.
 C = 4
-BYTES_PER_CELL_STAGE = {"fp32c16": (8 * 4 + 2 * 2) * C / 3.0,
-                        "fp32c": (8 * 4 + 2 * 4) * C / 3.0,
-                        "fp64": 8 * 8 * C / 3.0}
+BYTES_PER_CELL_STAGE = {"fp32c16": (12 * 4 + 2 * 2) * C / 3.0,
+                        "fp32c": (12 * 4 + 2 * 4) * C / 3.0,
+                        "fp64": 12 * 8 * C / 3.0}
+# Bytes this schedule moves: stage states rebuilt from u0 and k, so per
+# component and step stage 1 reads u0, writes k1; stage 2 reads u0, k1, writes
+# k2; stage 3 reads u0, k2, writes u (8 words), plus the remainder.
+MOVED_BYTES_PER_CELL_STAGE = {"fp32c16": (8 * 4 + 2 * 2) * C / 3.0,
+                              "fp32c": (8 * 4 + 2 * 4) * C / 3.0,
+                              "fp64": 8 * 8 * C / 3.0}
+# HBM bytes per stage launch (mean over the three SSPRK3 stages) at 512^3,
+# FP32 + bf16 remainder, autotuned launch: rocprofv3 --pmc
+# TCC_EA0_RDREQ_DRAM_32B TCC_EA0_WRREQ_WRITE_DRAM_32B (32-byte units), same
+# command as profiles/stage_kernel_traffic_complex512.md.
+TRAFFIC_BYTES_PER_LAUNCH_512 = None
 
 
 def inputs(N, L, cfl, t0):
@@ -64,10 +78,13 @@ def inputs(N, L, cfl, t0):
     }
 
 
-def run_gpu(shm, inp, prec, steps, warmup, seed):
+def run_gpu(shm, inp, prec, steps, warmup, seed, launch=None):
     sim = shm.Simulation("ComplexScalar", inp, precision=prec)
     sim.init_state(0, 1, seed, [0.05])
-    sim.autotune(0)
+    if launch:
+        sim.set_launch(*launch)
+    else:
+        sim.autotune(0)
     sim.advance(0, warmup)
     sim.sync()
     sim.set_timing(True)
@@ -80,24 +97,39 @@ def run_gpu(shm, inp, prec, steps, warmup, seed):
     return sim, t1 - t0, tm
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(inp, seed):
+    """FP64 oracle, whole SSPRK3 steps after one warm-up step, until about
+    10 s of work (at most 8 steps)."""
     from oracle_py import Oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     orc = Oracle("ComplexScalar", inp)
     orc.init_state(0, 1, seed, [0.05])
     n = orc.ncells(0)
+    orc.advance(0, 1)
     steps = 0
     t0 = time.perf_counter()
-    while steps < 5 and (steps == 0 or time.perf_counter() - t0 < 10.0):
+    while steps < 8 and (steps == 0 or time.perf_counter() - t0 < 10.0):
         orc.advance(0, 1)
         steps += 1
     dt = time.perf_counter() - t0
     return {"value": n * 3 * steps / dt, "unit": "cell-updates/s",
             "cores": threads, "kind": "port",
             "sample": f"{steps} SSPRK3 step(s) (3 stages each) of the same "
-                      f"{round(n ** (1/3))}^3 workload, FP64 oracle in the "
-                      "reference's unfused structure, OpenMP "
-                      f"{threads} threads, {dt:.1f} s"}
+                      f"{round(n ** (1/3))}^3 workload after 1 warm-up step, FP64 "
+                      "oracle in the reference's unfused structure (Rhs pass, "
+                      "LinComb passes, ghost fill per stage), OpenMP "
+                      f"{threads} threads on {cpu_model()}, {dt:.1f} s"}
 
 
 def main():
@@ -109,7 +141,13 @@ def main():
     ap.add_argument("--seed", type=int, default=20240401)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="fixed stage launch shape 'variant,tile_z,xcd_split,"
+                         "persistent,zc_minor' instead of the autotuner")
     a = ap.parse_args()
+    launch = [int(v) for v in a.launch.split(",")] if a.launch else None
+    if launch is not None and len(launch) != 5:
+        ap.error("--launch takes five integers")
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs each level on a "
               "single MI355X (multi-GPU sharding is design-only); use --gpus 1",
@@ -122,7 +160,7 @@ def main():
     inp = inputs(N, L=N * 0.25, cfl=0.2, t0=0.1)
     stages = 3
     sim32, sec32, tm32 = run_gpu(shm, inp, shm.PREC_FP32C16, a.steps, a.warmup,
-                                 a.seed)
+                                 a.seed, launch)
     cells = sim32.ncells(0)
     rate32 = cells * stages * a.steps / sec32
 
@@ -149,26 +187,29 @@ def main():
     # Roofline of the dominant (fused stage) kernel.
     launches = tm32["stage_launches"]
     avg_ms = tm32["stage_ms"] / max(launches, 1)
-    bpl = BYTES_PER_CELL_STAGE["fp32c16"] * tm32["stage_cells"] / max(launches, 1)
+    cells_pl = tm32["stage_cells"] / max(launches, 1)
+    bpl = BYTES_PER_CELL_STAGE["fp32c16"] * cells_pl
     achieved = bpl / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_ssprk3_complex512.json")
-    if os.path.exists(tfile) and N == 512:
-        with open(tfile) as f:
-            traffic = json.load(f).get("fp32c_bytes_per_launch")
+    moved = MOVED_BYTES_PER_CELL_STAGE["fp32c16"] * cells_pl / (avg_ms * 1e-3) / 1e9
+    traffic = TRAFFIC_BYTES_PER_LAUNCH_512 if N == 512 else None
     result["roofline"] = {"bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                           "kernel": "shm::stage_kernel<ComplexScalar, PolSsprk<1..3>, F32C16>",
                           "avg_launch_ms": avg_ms,
-                          "algorithmic_bytes_per_launch": bpl}
+                          "algorithmic_bytes_per_launch": bpl,
+                          "moved_bytes_per_launch":
+                              MOVED_BYTES_PER_CELL_STAGE["fp32c16"] * cells_pl,
+                          "moved_achieved": moved,
+                          "moved_frac": moved / HBM_PEAK_GBS}
+    result["launch"] = sim32.get_launch(0)
     result["halo_ms_per_step"] = tm32["halo_ms"] / a.steps
     # Achievable bandwidth on this GPU (plain float4 kernels over 4 GiB).
     result["measured_bandwidth"] = sim32.stream_probe(4 << 30, 10)
 
     if not a.no_fp64:
         sim64, sec64, tm64 = run_gpu(shm, inp, shm.PREC_FP64, a.steps, a.warmup,
-                                     a.seed)
+                                     a.seed, launch)
         rate64 = cells * stages * a.steps / sec64
         l64 = tm64["stage_launches"]
         avg64 = tm64["stage_ms"] / max(l64, 1)
@@ -180,12 +221,14 @@ def main():
             err = max(err, float(np.max(np.abs(x32 - x64)) /
                                  max(np.max(np.abs(x64)), 1e-300)))
             del x32, x64
+        m64 = MOVED_BYTES_PER_CELL_STAGE["fp64"] * tm64["stage_cells"] / max(l64, 1)
         result["fp64"] = {"value": rate64, "ms_per_step": sec64 / a.steps * 1e3,
-                          "roofline_frac": b64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                          "roofline_frac": b64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "moved_frac": m64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS}
         result["speedup_fp32c_vs_fp64"] = rate32 / rate64
         result["fp32c_rel_err_vs_fp64"] = err
         simc, secc, tmc = run_gpu(shm, inp, shm.PREC_FP32C, a.steps, a.warmup,
-                                  a.seed)
+                                  a.seed, launch)
         lc = tmc["stage_launches"]
         bc = BYTES_PER_CELL_STAGE["fp32c"] * tmc["stage_cells"] / max(lc, 1)
         errc = 0.0
