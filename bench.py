@@ -58,10 +58,10 @@ MOVED_BYTES_PER_CELL_STAGE = {"fp32c16": (8 * 4 + 2 * 2) * C / 3.0,
                               "fp32c": (8 * 4 + 2 * 4) * C / 3.0,
                               "fp64": 8 * 8 * C / 3.0}
 # HBM bytes per stage launch (mean over the three SSPRK3 stages) at 512^3,
-# FP32 + bf16 remainder, autotuned launch: rocprofv3 --pmc
-# TCC_EA0_RDREQ_DRAM_32B TCC_EA0_WRREQ_WRITE_DRAM_32B (32-byte units), same
-# command as profiles/stage_kernel_traffic_complex512.md.
-TRAFFIC_BYTES_PER_LAUNCH_512 = None
+# FP32 + bf16 remainder, 64 x 16 tiles: DRAM read + write requests counted by
+# rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B TCC_EA0_WRREQ_WRITE_DRAM_32B (x 32 B),
+# profiles/stage_kernel_traffic_complex512.md.
+TRAFFIC_BYTES_PER_LAUNCH_512 = 7.097e9
 
 
 def inputs(N, L, cfl, t0):
