@@ -107,6 +107,12 @@ int shm_advance_level(shm_ctx *ctx, int lev);
 /* TimeStepper::Advance(0) repeated ncoarse times (time_stepper.cpp:90-133). */
 int shm_evolve(shm_ctx *ctx, int ncoarse);
 
+/* Counters of the most recent truncation-error tagging that shm_evolve ran
+ * on lev at a scheduled regrid (Sledgehamr::ErrorEst's ntags_total,
+ * ntags_user, ntags_trunc[ncomp], sledgehamr.cpp:280-348); the layout
+ * itself stays static.  counts holds 2 + ncomp values. */
+int shm_level_tag_counts(shm_ctx *ctx, int lev, long long *counts);
+
 /* Sledgehamr::CreateShadowLevel (sledgehamr.cpp:234-251). */
 int shm_create_shadow(shm_ctx *ctx);
 

@@ -104,10 +104,20 @@ struct Array4 {
  *  to double on read), or the LDS/register tile of a fused stage kernel. */
 template <>
 struct Array4<const double> {
-    enum Mode : int { kPlain64 = 0, kPlain32 = 1, kTile = 2, kProbe = 3 };
+    // kPlain32C / kPlain32C16: FP32 value plus an FP32 / bf16 remainder,
+    // read back as their FP64 sum.
+    enum Mode : int {
+        kPlain64 = 0,
+        kPlain32 = 1,
+        kTile = 2,
+        kProbe = 3,
+        kPlain32C = 4,
+        kPlain32C16 = 5
+    };
 
     const double *p = nullptr;
     const float *pf = nullptr;
+    const void *plo = nullptr;  // remainder plane (kPlain32C, kPlain32C16)
     Long jstride = 0, kstride = 0, nstride = 0;
     Dim3 begin{0, 0, 0}, end{0, 0, 0};
     int ncomp = 1;
@@ -325,6 +335,13 @@ Array4<const double>::operator()(int i, int j, int k, int n) const {
         return static_cast<const shm::ProbeCtx *>(ext)->get(i, j, k, n);
     const Long off = (i - begin.x) + (j - begin.y) * jstride +
                      (k - begin.z) * kstride + n * nstride;
+    if (mode == kPlain32C)
+        return (double)pf[off] + (double)static_cast<const float *>(plo)[off];
+    if (mode == kPlain32C16) {
+        const unsigned int bits =
+            (unsigned int)static_cast<const unsigned short *>(plo)[off] << 16;
+        return (double)pf[off] + (double)__builtin_bit_cast(float, bits);
+    }
     return mode == kPlain32 ? (double)pf[off] : p[off];
 }
 }  // namespace amrex

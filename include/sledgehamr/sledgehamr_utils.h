@@ -51,14 +51,15 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE constexpr GradWeights grad_weights(int 
 
 }  // namespace detail
 
-/** FP32 ring sums on an FP32 tile (fused stage kernels): x/y neighbours
- *  from the staged LDS plane, z neighbours from the register queue.  Returns
- *  false when the request is not the tile's own cell of a stencil slot, in
- *  which case the caller takes the generic path. */
+/** Ring sums on an FP32 tile (fused stage kernels): x/y neighbours from the
+ *  staged LDS plane, z neighbours from the register queue, accumulated in
+ *  FP64 (sums of FP32 values are then exact, so only the stored values
+ *  carry rounding).  Returns false when the request is not the tile's own
+ *  cell of a stencil slot, in which case the caller takes the generic path. */
 template <int P>
 AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool tile_lap_f32(
     amrex::Array4<amrex::Real const> const &state, int i, int j, int k, int c,
-    float &sum) {
+    double &sum) {
     if (state.mode != amrex::Array4<const double>::kTile) return false;
     const shm::TileCtx &tc = *static_cast<const shm::TileCtx *>(state.ext);
     if (!tc.f32 || i != tc.ci || j != tc.cj || k != tc.ck) return false;
@@ -67,11 +68,12 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool tile_lap_f32(
     constexpr detail::LapWeights W = detail::lap_weights(P);
     const float *L = static_cast<const float *>(tc.lds) + s * tc.lds_cstride;
     const float *Z = static_cast<const float *>(tc.zq) + s * tc.zq_cstride + tc.R;
-    float acc = (float)W.w[0] * Z[0];
+    double acc = W.w[0] * (double)Z[0];
     constexpr_for<1, P + 1, 1>([&](auto m) {
-        const float ring = L[m] + L[-m] + L[m * tc.lds_pitch] +
-                           L[-m * tc.lds_pitch] + Z[m] + Z[-m];
-        acc += (float)W.w[m] * ring;
+        const double ring = ((double)L[m] + (double)L[-m]) +
+                            ((double)L[m * tc.lds_pitch] + (double)L[-m * tc.lds_pitch]) +
+                            ((double)Z[m] + (double)Z[-m]);
+        acc += W.w[m] * ring;
     });
     sum = acc;
     return true;
@@ -88,8 +90,8 @@ Laplacian(amrex::Array4<amrex::Real const> const &state, const int i,
     } else {
         constexpr detail::LapWeights W = detail::lap_weights(P);
         const double scale = 1. / (W.den * dx2);  // loop invariant
-        float fs;
-        if (tile_lap_f32<P>(state, i, j, k, c, fs)) return (double)fs * scale;
+        double fs;
+        if (tile_lap_f32<P>(state, i, j, k, c, fs)) return fs * scale;
         double acc = W.w[0] * state(i, j, k, c);
         constexpr_for<1, P + 1, 1>([&](auto m) {
             const double ring = state(i + m, j, k, c) + state(i - m, j, k, c) +
@@ -120,21 +122,23 @@ Gradient(amrex::Array4<amrex::Real const> const &state, const int i,
             const shm::TileCtx &tc = *static_cast<const shm::TileCtx *>(state.ext);
             const int s = tc.slot[c];
             if (tc.f32 && s >= 0 && i == tc.ci && j == tc.cj && k == tc.ck) {
-                float acc = 0.f;
+                // Differences of FP32 values are exact in FP64.
+                double acc = 0.;
                 if (az) {
                     const float *Z = static_cast<const float *>(tc.zq) +
                                      s * tc.zq_cstride + tc.R;
-                    constexpr_for<1, P + 1, 1>(
-                        [&](auto m) { acc += (float)W.g[m] * (Z[m] - Z[-m]); });
+                    constexpr_for<1, P + 1, 1>([&](auto m) {
+                        acc += W.g[m] * ((double)Z[m] - (double)Z[-m]);
+                    });
                 } else {
                     const float *L =
                         static_cast<const float *>(tc.lds) + s * tc.lds_cstride;
                     const int st = ax ? 1 : tc.lds_pitch;
                     constexpr_for<1, P + 1, 1>([&](auto m) {
-                        acc += (float)W.g[m] * (L[m * st] - L[-m * st]);
+                        acc += W.g[m] * ((double)L[m * st] - (double)L[-m * st]);
                     });
                 }
-                return (double)acc * scale;
+                return acc * scale;
             }
         }
         double acc = 0.;

@@ -152,6 +152,27 @@ inline void LinComb(MultiFab &dst, double a, MultiFab &x, int xc, double bb,
     }
 }
 
+/** Ghost fill of boxes that are each their own periodic domain (an
+ *  ensemble of independent realizations): images wrap inside the box. */
+inline void fill_boundary_independent(MultiFab &m) {
+    if (m.g == 0) return;
+    for (size_t b = 0; b < m.fabs.size(); ++b) {
+        Fab &D = m.fabs[b];
+        const int g = D.g;
+        auto wrap = [](int v, int lo, int n) { return lo + (((v - lo) % n) + n) % n; };
+#pragma omp parallel for schedule(static)
+        for (int k = D.box.lo[2] - g; k <= D.box.hi[2] + g; ++k)
+            for (int j = D.box.lo[1] - g; j <= D.box.hi[1] + g; ++j)
+                for (int i = D.box.lo[0] - g; i <= D.box.hi[0] + g; ++i) {
+                    const int si = wrap(i, D.box.lo[0], D.box.n(0));
+                    const int sj = wrap(j, D.box.lo[1], D.box.n(1));
+                    const int sk = wrap(k, D.box.lo[2], D.box.n(2));
+                    if (si == i && sj == j && sk == k) continue;
+                    for (int n = 0; n < m.nc; ++n) D.at(i, j, k, n) = D.at(si, sj, sk, n);
+                }
+    }
+}
+
 /** Same-level ghost fill with periodic wrap (FillBoundary, int_dir BCs).
  *  Ghost cells whose periodic image lies in no box keep their values. */
 inline void fill_boundary(MultiFab &m, int N) {
@@ -201,6 +222,36 @@ inline void average_down(MultiFab &fine, MultiFab &crse) {
                                     c += F.at(2 * i + di, 2 * j + dj, 2 * k + dk, n);
                         Cf.at(i, j, k, n) = c * 0.125;
                     }
+    }
+}
+
+/** AverageDownTo (level_synchronizer.cpp:219-233): the mean of the 8
+ *  children of every coarse cell under a fine box, written into whichever
+ *  coarse box holds that cell. */
+inline void average_down_to(MultiFab &fine, MultiFab &crse) {
+    for (size_t b = 0; b < fine.fabs.size(); ++b) {
+        Fab &F = fine.fabs[b];
+#pragma omp parallel for schedule(static)
+        for (int k = F.box.lo[2] / 2; k <= (F.box.hi[2] + 1) / 2 - 1; ++k)
+            for (int j = F.box.lo[1] / 2; j <= (F.box.hi[1] + 1) / 2 - 1; ++j)
+                for (int i = F.box.lo[0] / 2; i <= (F.box.hi[0] + 1) / 2 - 1; ++i) {
+                    Fab *Cf = nullptr;
+                    for (auto &c : crse.fabs)
+                        if (i >= c.box.lo[0] && i <= c.box.hi[0] && j >= c.box.lo[1] &&
+                            j <= c.box.hi[1] && k >= c.box.lo[2] && k <= c.box.hi[2]) {
+                            Cf = &c;
+                            break;
+                        }
+                    if (!Cf) continue;  // not properly nested: nothing to update
+                    for (int n = 0; n < fine.nc; ++n) {
+                        double c = 0;
+                        for (int dk = 0; dk < 2; ++dk)
+                            for (int dj = 0; dj < 2; ++dj)
+                                for (int di = 0; di < 2; ++di)
+                                    c += F.at(2 * i + di, 2 * j + dj, 2 * k + dk, n);
+                        Cf->at(i, j, k, n) = c * 0.125;
+                    }
+                }
     }
 }
 
@@ -313,6 +364,8 @@ struct Config {
     bool ko_on = false;
     int ko_order = 0;
     bool shadow_hierarchy = false;
+    bool independent = false;  // level-0 boxes are separate periodic domains
+    double regrid_dt = DBL_MAX;
 };
 
 template <class Proj>
@@ -357,6 +410,8 @@ class Sim {
         pp.get("sim.t_start", cfg.t_start);
         pp.get("sim.t_end", cfg.t_end);
         pp.query("sim.gravitational_waves", cfg.gw);
+        pp.query("shm.independent_boxes", cfg.independent);
+        pp.query("amr.regrid_dt", cfg.regrid_dt);
         pp.get("integrator.type", cfg.integrator);
         nc = cfg.gw ? Ops::kNGw : Ops::kNScalars;
         std::vector<std::string> names;
@@ -512,6 +567,10 @@ class Sim {
     }
 
     void fill_patch(MultiFab &m, int lev) {
+        if (cfg.independent && lev == 0) {
+            fill_boundary_independent(m);
+            return;
+        }
         fill_boundary(m, lev < 0 ? cfg.N0 / 2 : dimN[lev]);
     }
 
@@ -836,6 +895,173 @@ class Sim {
                 }
         const double n6 = std::pow((double)N, 6);
         for (double &v : spec) v /= n6;
+    }
+
+    // -----------------------------------------------------------------------
+    // Sub-cycled stepping on a static layout: TimeStepper::Advance with
+    // ScheduleRegrid, SynchronizeLevels and DoRegridIfScheduled
+    // (time_stepper.cpp:90-171, 221-311, 347-397) and RegridScheduler
+    // (regrid_scheduler.cpp:9-74).  A scheduled regrid computes truncation
+    // errors and tags as the reference does; the box layout stays.
+    // -----------------------------------------------------------------------
+    struct Sched { int lowest; double t; };
+    std::vector<Sched> sched;
+    std::vector<double> last_regrid;
+
+    int find_sched(double t) const {
+        for (size_t i = 0; i < sched.size(); ++i)
+            if (sched[i].t == t) return (int)i;
+        return -1;
+    }
+    bool do_regrid(int lev, double t) const {
+        const int id = find_sched(t);
+        return id >= 0 && lev == sched[id].lowest;
+    }
+    bool need_te(int lev, double t) const {
+        const int id = find_sched(t);
+        return id >= 0 && lev >= sched[id].lowest;
+    }
+    double regrid_dt_of(int lev) const { return cfg.regrid_dt / std::pow(2., lev); }
+
+    void schedule_regrid(int lev) {
+        const double time = grid_new[lev].t;
+        const int istep = grid_new[lev].istep;
+        if (do_regrid(lev, time + dt[lev])) return;
+        if (grid_old[lev].t == time) return;
+        if (lev >= cfg.max_level) return;
+        if (istep % 2 == 0 && lev > 0) return;
+        const double next = lev > 0 ? time + 3. * dt[lev] : time + 2. * dt[lev];
+        if (lev + 1 > 0 && !prj.CreateLevelIf(lev + 1, next)) return;
+        if (next <= last_regrid[lev] + regrid_dt_of(lev)) return;
+        if (lev == 0 && grid_old[lev].t == -DBL_MAX) return;
+        const int id = find_sched(time + dt[lev]);
+        if (id < 0)
+            sched.push_back({lev, time + dt[lev]});
+        else
+            sched[id].lowest = std::min(lev, sched[id].lowest);
+        if (lev == 0) create_shadow();
+    }
+
+    void synchronize_levels(int lev) {
+        const bool te = need_te(lev, grid_new[lev].t);
+        const int finest = this->finest();
+        if (lev < finest && !(cfg.shadow_hierarchy && te))
+            average_down_to(grid_new[lev + 1], grid_new[lev]);
+        if (lev >= 1 - (cfg.shadow_hierarchy ? 1 : 0) && te) {
+            std::vector<double> t;
+            te_avgdown(lev, t, true);
+            std::vector<uint8_t> m;
+            std::vector<long long> c;
+            tag(lev, t, true, m, c);
+            last_tags[lev] = c;
+            if (lev == 0) {
+                shadow = MultiFab();
+                shadow_tmp = MultiFab();
+            }
+        }
+    }
+
+    void do_regrid_if_scheduled(int lev) {
+        const double time = grid_new[lev].t;
+        if (!do_regrid(lev, time)) return;
+        for (int l = lev; l <= finest(); ++l) last_regrid[l] = time;
+        const int id = find_sched(time);
+        if (id >= 0) sched.erase(sched.begin() + id);
+    }
+
+    void step(int lev) {
+        if (cfg.shadow_hierarchy) schedule_regrid(lev);
+        if (lev == 0 && !shadow.defined()) prj.BeforeTimestep(grid_new[lev].t);
+        advance(lev);
+        if (lev != finest()) {
+            step(lev + 1);
+            step(lev + 1);
+        }
+        synchronize_levels(lev);
+        do_regrid_if_scheduled(lev);
+        if (lev == 0)
+            for (int l = 1; l <= finest(); ++l) grid_new[l].t = grid_new[0].t;
+    }
+
+    void evolve(int ncoarse) {
+        if (last_regrid.empty()) last_regrid.assign(cfg.max_level + 1, cfg.t_start);
+        for (int n = 0; n < ncoarse; ++n) step(0);
+    }
+
+    std::map<int, std::vector<long long>> last_tags;  // counters of the last TE per level
+
+    /** Seeded bubbles (kind 3: over vacuum, nucleated by p[7]; kind 4: added
+     *  to the current field, nucleated in (p[6], p[7]]) and closed string
+     *  loops (kind 5), as the device generator defines them: objects drawn
+     *  from streams 100+ / 200+ in level-0 cell units, periodic minimum
+     *  image, bubble wall phi_in (1 - tanh((r - R0)/w))/2 combined by max,
+     *  loops psi = prod tanh(d/w) exp(i atan2(h, rho - R)). */
+    void init_objects(int lev, int kind, uint64_t seed, const double *p, int np) {
+        MultiFab &m = grid_new[lev];
+        const double N0 = (double)cfg.N0;
+        const double cell = N0 / (double)dimN[lev];
+        struct Obj { double v[8]; };
+        std::vector<Obj> obj;
+        const int n = np > 0 ? (int)p[0] : 0;
+        for (int i = 0; i < n; ++i) {
+            Obj o{};
+            if (kind == 3 || kind == 4) {
+                o.v[0] = N0 * rng_uniform(seed, 100, i);
+                o.v[1] = N0 * rng_uniform(seed, 101, i);
+                o.v[2] = N0 * rng_uniform(seed, 102, i);
+                o.v[3] = p[4] + (p[5] - p[4]) * rng_uniform(seed, 103, i);
+            } else {
+                const double u = 2. * rng_uniform(seed, 204, i) - 1.;
+                const double ph = 2. * M_PI * rng_uniform(seed, 205, i);
+                const double st = std::sqrt(std::max(0., 1. - u * u));
+                o.v[0] = N0 * rng_uniform(seed, 200, i);
+                o.v[1] = N0 * rng_uniform(seed, 201, i);
+                o.v[2] = N0 * rng_uniform(seed, 202, i);
+                o.v[3] = p[1] + (p[2] - p[1]) * rng_uniform(seed, 203, i);
+                o.v[4] = st * std::cos(ph);
+                o.v[5] = st * std::sin(ph);
+                o.v[6] = u;
+            }
+            obj.push_back(o);
+        }
+        auto mi = [N0](double d) { return d - N0 * std::floor(d / N0 + 0.5); };
+        for (auto &f : m.fabs)
+            for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
+                for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
+                    for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i) {
+                        const double x = (i + 0.5) * cell, y = (j + 0.5) * cell,
+                                     z = (k + 0.5) * cell;
+                        if (kind == 3 || kind == 4) {
+                            double phi = kind == 3 ? 0. : f.at(i, j, k, 0);
+                            for (const Obj &o : obj) {
+                                const bool now = kind == 3 ? o.v[3] <= p[7]
+                                                           : (o.v[3] > p[6] && o.v[3] <= p[7]);
+                                if (!now) continue;
+                                const double dx = mi(x - o.v[0]), dy = mi(y - o.v[1]),
+                                             dz = mi(z - o.v[2]);
+                                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                                phi = std::max(phi, p[1] * 0.5 * (1. - std::tanh((r - p[2]) / p[3])));
+                            }
+                            if (kind == 3)
+                                for (int c = 0; c < nc; ++c) f.at(i, j, k, c) = 0.;
+                            f.at(i, j, k, 0) = phi;
+                        } else {
+                            double mod = 1., th = 0.;
+                            for (const Obj &o : obj) {
+                                const double dx = mi(x - o.v[0]), dy = mi(y - o.v[1]),
+                                             dz = mi(z - o.v[2]);
+                                const double h = dx * o.v[4] + dy * o.v[5] + dz * o.v[6];
+                                const double px = dx - h * o.v[4], py = dy - h * o.v[5],
+                                             pz = dz - h * o.v[6];
+                                const double sr = std::sqrt(px * px + py * py + pz * pz) - o.v[3];
+                                mod *= std::tanh(std::sqrt(sr * sr + h * h) / p[3]);
+                                th += std::atan2(h, sr);
+                            }
+                            for (int c = 0; c < nc; ++c) f.at(i, j, k, c) = 0.;
+                            f.at(i, j, k, 0) = mod * std::cos(th);
+                            f.at(i, j, k, 1) = mod * std::sin(th);
+                        }
+                    }
     }
 
     /** Seeded complex random phase, identical to the device generator. */

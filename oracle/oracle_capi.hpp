@@ -132,6 +132,19 @@
     ORC_EXPORT int orc_advance(void *h, int lev) {                             \
         return orc_guard([&] { static_cast<OrcSim *>(h)->advance(lev); });    \
     }                                                                          \
+    ORC_EXPORT int orc_evolve(void *h, int ncoarse) {                          \
+        return orc_guard([&] { static_cast<OrcSim *>(h)->evolve(ncoarse); });  \
+    }                                                                          \
+    ORC_EXPORT int orc_last_tags(void *h, int lev, long long *counts, int n) { \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            auto it = s->last_tags.find(lev);                                  \
+            if (it == s->last_tags.end())                                      \
+                throw std::runtime_error("no truncation errors at this level"); \
+            for (int i = 0; i < n && i < (int)it->second.size(); ++i)          \
+                counts[i] = it->second[i];                                     \
+        });                                                                    \
+    }                                                                          \
     ORC_EXPORT int orc_create_shadow(void *h) {                                \
         return orc_guard([&] { static_cast<OrcSim *>(h)->create_shadow(); });  \
     }                                                                          \
@@ -172,6 +185,8 @@
             auto *s = static_cast<OrcSim *>(h);                                \
             if (kind == 1)                                                     \
                 s->init_complex_random_phase(lev, seed, np > 0 ? p[0] : 0.);   \
+            else if (kind >= 3 && kind <= 5)                                   \
+                s->init_objects(lev, kind, seed, p, np);                       \
             else                                                               \
                 throw std::runtime_error("unknown initial-state kind");        \
         });                                                                    \

@@ -110,6 +110,8 @@ class Sledgehamr {
                                                const int) {}
     virtual void SetParamsTruncationModifier(std::vector<double> &, const double,
                                              const int) {}
+    virtual bool CreateLevelIf(const int, const double) { return true; }
+    virtual void BeforeTimestep(const double) {}
     std::vector<ScalarField *> scalar_fields;
 };
 

@@ -16,7 +16,7 @@ __all__ = ["Simulation", "library_path", "PROJECTS", "ShmError",
            "TE_AVGDOWN", "TE_TAGS"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBDIR = os.path.join(_HERE, "lib")
+LIBDIR = os.environ.get("SHM_LIBDIR") or os.path.join(_HERE, "lib")
 PROJECTS = ("RealScalar", "ComplexScalar", "PhaseTransitionGW")
 
 PREC_INPUTS, PREC_FP64, PREC_FP32C, PREC_FP32C16 = -1, 0, 1, 2
@@ -28,7 +28,7 @@ EXPORTS = (
     "shm_get_info", "shm_set_box_table", "shm_get_box_table",
     "shm_level_info", "shm_set_level_time", "shm_upload", "shm_download",
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
-    "shm_evolve", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
+    "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
     "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
     "shm_get_launch", "shm_set_timing", "shm_get_timing",
 )
@@ -94,6 +94,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_advance_level": (i, [vp, i]),
         "shm_evolve": (i, [vp, i]),
         "shm_create_shadow": (i, [vp]),
+        "shm_level_tag_counts": (i, [vp, i, llp]),
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "shm_sync": (i, [vp]),
@@ -236,6 +237,13 @@ class Simulation:
 
     def evolve(self, ncoarse: int) -> None:
         self._check(self.lib.shm_evolve(self._ctx, ncoarse))
+
+    def level_tag_counts(self, lev: int) -> np.ndarray:
+        """Counters [total, user, trunc...] of the last scheduled tagging."""
+        c = np.zeros(2 + self.info["ncomp"], dtype=np.int64)
+        self._check(self.lib.shm_level_tag_counts(
+            self._ctx, lev, c.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return c
 
     def create_shadow(self) -> None:
         self._check(self.lib.shm_create_shadow(self._ctx))

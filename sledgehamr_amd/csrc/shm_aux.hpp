@@ -38,6 +38,7 @@ struct TeTagArgs {
     const BoxDesc *cboxes;   // coarse box holding each fine box's parent cells
     long long fcstride, ccstride;
     const void *fine;        // fine state (hi part for FP32C)
+    const void *finelo;      // fine remainder (FP32C)
     void *crse;              // coarse state, averaged in place when requested
     void *crselo;            // coarse remainder (FP32C)
     const long long *cprefix;  // coarse cells per fine box, prefix (nbox+1)
@@ -62,13 +63,20 @@ enum TeTagFlags : int {
     kTeTags = 4,      // write the tag mask and counters
 };
 
-template <class S>
+/** Plain view of a box of a stored state; with a remainder the view reads
+ *  hi + lo in FP64. */
+template <Prec P>
 __device__ AMREX_FORCE_INLINE amrex::Array4<const double> plain_view(
-    const void *arr, const BoxDesc &b, long long cstride, int ncomp) {
+    const void *arr, const void *arrlo, const BoxDesc &b, long long cstride,
+    int ncomp) {
+    using T = PrecTraits<P>;
     amrex::Array4<const double> v;
-    if constexpr (std::is_same_v<S, float>) {
-        v.mode = amrex::Array4<const double>::kPlain32;
+    if constexpr (T::comp) {
+        v.mode = std::is_same_v<typename T::L, float>
+                     ? amrex::Array4<const double>::kPlain32C
+                     : amrex::Array4<const double>::kPlain32C16;
         v.pf = static_cast<const float *>(arr) + b.base;
+        v.plo = static_cast<const typename T::L *>(arrlo) + b.base;
     } else {
         v.mode = amrex::Array4<const double>::kPlain64;
         v.p = static_cast<const double *>(arr) + b.base;
@@ -127,7 +135,7 @@ __global__ void __launch_bounds__(256) te_tag_kernel(const TeTagArgs a) {
         const int ci = (fb.lo[0] >> 1) + cx, cj = (fb.lo[1] >> 1) + cy,
                   ck = (fb.lo[2] >> 1) + cz;
         const amrex::Array4<const double> fv =
-            plain_view<S>(a.fine, fb, a.fcstride, NC);
+            plain_view<P>(a.fine, a.finelo, fb, a.fcstride, NC);
 
         bool te_pass = false;
         if (a.flags & kTeCompute) {
@@ -138,11 +146,8 @@ __global__ void __launch_bounds__(256) te_tag_kernel(const TeTagArgs a) {
                 auto f = [&](int x, int y, int z) { return fv(x, y, z, n); };
                 const double sum = child_sum(f, ii, jj, kk);
                 const long long cen = (long long)n * a.ccstride + ce;
-                double crse;
-                if constexpr (std::is_same_v<S, float>)
-                    crse = (double)static_cast<const float *>(a.crse)[cen];
-                else
-                    crse = static_cast<const double *>(a.crse)[cen];
+                const double crse =
+                    load_state_at<PrecTraits<P>>(a.crse, a.crselo, cen);
                 double avg;
                 const double te = te_cell(crse, sum, &avg);
                 if (a.flags & kTeAvgDown) store_state_at<PrecTraits<P>>(a.crse, a.crselo, cen, avg);
@@ -451,12 +456,50 @@ struct InitArgs {
     int nfields;
     int ncomp;
     double p[8];
+    double cell;               // level-0 cells per cell of this level
+    double N0;                 // level-0 cells per axis (periodic length)
+    const double *obj;         // object table (bubbles / loops), 8 per object
+    int nobj;
 };
 
 enum InitKind : int {
     kInitComplexRandomPhase = 1,  // psi = (1 + p0 N(0,1)) exp(i 2 pi U), pi = 0
     kInitZero = 2,
+    kInitBubbles = 3,        // vacuum plus the bubbles nucleated by p[7]
+    kInitAddBubbles = 4,     // inject the bubbles nucleated in (p[6], p[7]]
+    kInitStringLoops = 5,    // psi = prod tanh(d/w) exp(i theta), unit winding
 };
+
+/** Minimum periodic image of a displacement along one axis. */
+__host__ __device__ AMREX_FORCE_INLINE double min_image(double d, double L) {
+    return d - L * floor(d / L + 0.5);
+}
+
+/** Bubble wall: phi_in (1 - tanh((r - R0) / w)) / 2 at distance r (all in
+ *  level-0 cells).  Object row: centre x, y, z, nucleation time. */
+__host__ __device__ AMREX_FORCE_INLINE double bubble_profile(
+    const double *o, double x, double y, double z, double L, double phi_in,
+    double R0, double w) {
+    const double dx = min_image(x - o[0], L), dy = min_image(y - o[1], L),
+                 dz = min_image(z - o[2], L);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    return phi_in * 0.5 * (1. - tanh((r - R0) / w));
+}
+
+/** One closed string loop: object row centre x, y, z, radius, unit normal.
+ *  Returns the modulus factor tanh(d / w) (d = distance to the string) and
+ *  the winding angle atan2(h, rho - R) of the point around the string. */
+__host__ __device__ AMREX_FORCE_INLINE void loop_factor(
+    const double *o, double x, double y, double z, double L, double w,
+    double &mod, double &theta) {
+    const double dx = min_image(x - o[0], L), dy = min_image(y - o[1], L),
+                 dz = min_image(z - o[2], L);
+    const double h = dx * o[4] + dy * o[5] + dz * o[6];
+    const double px = dx - h * o[4], py = dy - h * o[5], pz = dz - h * o[6];
+    const double s = sqrt(px * px + py * py + pz * pz) - o[3];
+    mod = tanh(sqrt(s * s + h * h) / w);
+    theta = atan2(h, s);
+}
 
 template <Prec P>
 __global__ void __launch_bounds__(256) init_kernel(const InitArgs a) {
@@ -485,6 +528,38 @@ __global__ void __launch_bounds__(256) init_kernel(const InitArgs a) {
                                                (uint64_t)a.N * (uint64_t)(b.lo[2] + z));
         double v[32];
         for (int n = 0; n < a.ncomp && n < 32; ++n) v[n] = 0.;
+        // Cell centre in level-0 cell units.
+        const double px = (b.lo[0] + x + 0.5) * a.cell, py = (b.lo[1] + y + 0.5) * a.cell,
+                     pz = (b.lo[2] + z + 0.5) * a.cell;
+        if (a.kind == kInitBubbles || a.kind == kInitAddBubbles) {
+            double phi = a.kind == kInitBubbles
+                             ? 0.
+                             : load_state_at<PrecTraits<P>>(a.arr, a.arrlo, e);
+            for (int o = 0; o < a.nobj; ++o) {
+                const double *ob = a.obj + 8 * o;
+                const bool now = a.kind == kInitBubbles
+                                     ? ob[3] <= a.p[7]
+                                     : (ob[3] > a.p[6] && ob[3] <= a.p[7]);
+                if (now)
+                    phi = fmax(phi, bubble_profile(ob, px, py, pz, a.N0, a.p[1],
+                                                   a.p[2], a.p[3]));
+            }
+            if (a.kind == kInitAddBubbles) {
+                store_state_at<PrecTraits<P>>(a.arr, a.arrlo, e, phi);
+                continue;
+            }
+            v[0] = phi;
+        } else if (a.kind == kInitStringLoops) {
+            double mod = 1., th = 0.;
+            for (int o = 0; o < a.nobj; ++o) {
+                double m, t;
+                loop_factor(a.obj + 8 * o, px, py, pz, a.N0, a.p[3], m, t);
+                mod *= m;
+                th += t;
+            }
+            v[0] = mod * cos(th);
+            v[1] = mod * sin(th);
+        }
         if (a.kind == kInitComplexRandomPhase) {
             const double u1 = rng_uniform(a.seed, 1, gidx);
             const double u2 = rng_uniform(a.seed, 2, gidx);

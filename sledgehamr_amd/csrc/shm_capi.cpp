@@ -1,6 +1,7 @@
 // extern "C" entry points (include/shm.h) over shm::Runtime.
 #include "../../include/shm.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -294,6 +295,16 @@ __attribute__((visibility("default"))) int shm_stream_probe(
         if (bytes < (1u << 20) || reps < 1)
             throw shm::Error(SHM_ERR_ARG, "stream probe needs >= 1 MiB and reps >= 1");
         ctx->rt->stream_probe(bytes, reps, read_gbs, write_gbs, copy_gbs);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_level_tag_counts(shm_ctx *ctx, int lev,
+                                                                long long *counts) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!counts) throw shm::Error(SHM_ERR_ARG, "null counts");
+        const std::vector<long long> &c = ctx->rt->last_tags(lev);
+        std::copy(c.begin(), c.end(), counts);
     });
 }
 

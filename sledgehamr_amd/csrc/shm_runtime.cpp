@@ -574,6 +574,11 @@ void Runtime::init_from_inputs(const std::string &inputs,
     pp.query("shm.tile_z", launch.tz);
     pp.query("shm.xcd_split", launch.xcd_split);
     pp.query("shm.persistent", launch.persistent);
+    // Ensemble of independent realizations: every level-0 box is its own
+    // periodic domain (boxes may share index ranges).
+    pp.query("shm.independent_boxes", independent_);
+    if (independent_ && S.max_level > 0)
+        throw Error(1, "shm.independent_boxes needs a single level");
     pp.query("shm.tile_order_zc_minor", launch.zc_minor);
     if (launch.variant >= kNVar) throw Error(1, "shm.tile_variant out of range");
 
@@ -659,7 +664,8 @@ void Runtime::set_level(int lev, const std::vector<Box3> &boxes) {
         for (int a = 0; a < 3; ++a)
             if (b.lo[a] < 0 || b.hi[a] >= N || b.hi[a] < b.lo[a])
                 throw Error(1, "box outside the level domain");
-    ls.L.build(boxes, sim_->nghost, N, /*periodic=*/true);
+    const bool indep = independent_ && lev == 0;
+    ls.L.build(boxes, sim_->nghost, N, /*periodic=*/!indep, indep);
     ls.ncomp = ncomp_;
     ensure_arrays(ls, integrator_);
     ls.t = sim_->t_start;
@@ -1009,6 +1015,8 @@ void Runtime::avg_down(LevelState &fine, int fw, LevelState &crse, int cw) {
 }
 
 void Runtime::define_shadow() {
+    if (independent_)
+        throw Error(4, "no shadow level for independent boxes");
     LevelState &l0 = levels_.at(0);
     if (!shadow_.defined() || shadow_.L.boxes.size() != l0.L.boxes.size()) {
         shadow_ = LevelState();
@@ -1117,6 +1125,7 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     a.fcstride = fine.L.cstride;
     a.ccstride = crse ? crse->L.cstride : 0;
     a.fine = fine.U[fine.cur].get();
+    a.finelo = comp() ? fine.Ulo[fine.cur].get() : nullptr;
     a.crse = crse ? crse->U[crse->cur].get() : nullptr;
     a.crselo = (crse && comp()) ? crse->Ulo[crse->cur].get() : nullptr;
     a.cprefix = fine.L.dcprefix.as<long long>();
@@ -1332,6 +1341,47 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
     a.ncomp = ncomp_;
     for (int i = 0; i < np && i < 8; ++i) a.p[i] = p[i];
     if (ncomp_ > 32) throw Error(4, "initial-state generator supports <= 32 components");
+    const double N0 = (double)sim_->coarse_level_grid_size;
+    a.N0 = N0;
+    a.cell = N0 / (double)ls.L.N;
+    // Object tables, 8 doubles per object, drawn from the counter-based
+    // generator (streams 100+: bubbles, 200+: loops), positions in level-0
+    // cells so every level sees the same objects.
+    std::vector<double> obj;
+    if (kind == kInitBubbles || kind == kInitAddBubbles) {
+        // p: count, phi_in, R0, wall width w, nucleation times in
+        // [t_lo, t_hi], injection window (t_from, t_to].
+        if (np < 8 || p[0] < 0 || p[0] > 4096 || p[3] <= 0)
+            throw Error(1, "bubbles take 8 parameters: n, phi_in, R0, w, t_lo, t_hi, t_from, t_to");
+        for (int i = 0; i < (int)p[0]; ++i) {
+            const double row[8] = {
+                N0 * rng_uniform(seed, 100, i), N0 * rng_uniform(seed, 101, i),
+                N0 * rng_uniform(seed, 102, i),
+                p[4] + (p[5] - p[4]) * rng_uniform(seed, 103, i), 0., 0., 0., 0.};
+            obj.insert(obj.end(), row, row + 8);
+        }
+    } else if (kind == kInitStringLoops) {
+        // p: count, r_min, r_max (level-0 cells), core width w.
+        if (np < 4 || p[0] < 0 || p[0] > 4096 || p[3] <= 0 || ncomp_ < 2)
+            throw Error(1, "string loops take 4 parameters: n, r_min, r_max, w");
+        for (int i = 0; i < (int)p[0]; ++i) {
+            const double u = 2. * rng_uniform(seed, 204, i) - 1.;
+            const double ph = 2. * M_PI * rng_uniform(seed, 205, i);
+            const double st = std::sqrt(std::max(0., 1. - u * u));
+            const double row[8] = {
+                N0 * rng_uniform(seed, 200, i), N0 * rng_uniform(seed, 201, i),
+                N0 * rng_uniform(seed, 202, i),
+                p[1] + (p[2] - p[1]) * rng_uniform(seed, 203, i),
+                st * std::cos(ph), st * std::sin(ph), u, 0.};
+            obj.insert(obj.end(), row, row + 8);
+        }
+    } else if (kind != kInitComplexRandomPhase && kind != kInitZero) {
+        throw Error(1, "unknown initial-state kind");
+    }
+    DevMem dobj;
+    if (!obj.empty()) upload_vec(dobj, obj);
+    a.obj = obj.empty() ? nullptr : dobj.as<double>();
+    a.nobj = (int)(obj.size() / 8);
     const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
     switch (prec_) {
         case Prec::F64:
@@ -1348,6 +1398,7 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
             break;
     }
     SHM_HIP(hipGetLastError());
+    if (dobj) SHM_HIP(hipStreamSynchronize(stream_));  // table freed on return
     ls.ghosts_ok[ls.cur] = false;
 }
 
@@ -1547,6 +1598,8 @@ static double index_to_k(int a, int N, int type) {
 void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
                           std::vector<double> &bins,
                           std::vector<double> &kvals) {
+    if (independent_)
+        throw Error(4, "the GW spectrum needs one periodic domain, not independent boxes");
     if (!gw_) throw Error(3, "gravitational waves are not enabled");
     if (proj < 1 || proj > 3) throw Error(1, "projection_type must be 1, 2 or 3");
     if (zp < 1 || !sledgehamr::utils::IsPowerOfTwo(zp))
@@ -1668,6 +1721,13 @@ void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
 // regrid itself is replaced by tagging (ErrorEst) on the current layout.
 // ---------------------------------------------------------------------------
 
+const std::vector<long long> &Runtime::last_tags(int lev) const {
+    auto it = last_tags_.find(lev);
+    if (it == last_tags_.end())
+        throw Error(3, "no scheduled truncation-error tagging at this level yet");
+    return it->second;
+}
+
 void Runtime::step(int ncoarse) {
     for (int n = 0; n < ncoarse; ++n) scheduled_step(0);
 }
@@ -1710,7 +1770,7 @@ void Runtime::scheduled_step(int lev) {
     // DoRegridIfScheduled: tag on the static layout.
     for (size_t i = 0; i < schedule_.size(); ++i) {
         if (schedule_[i].second == ls.t && schedule_[i].first == lev) {
-            for (int l = lev; l < S.finest_level; ++l) last_regrid_[l] = ls.t;
+            for (int l = lev; l <= S.finest_level; ++l) last_regrid_[l] = ls.t;
             schedule_.erase(schedule_.begin() + i);
             break;
         }
@@ -1765,7 +1825,9 @@ void Runtime::synchronize_levels(int lev) {
         }
     }
     if (lev >= 1 - (S.shadow_hierarchy ? 1 : 0) && need_te) {
-        te_tag(lev, kTeCompute | kTeAvgDown | kTeTags, nullptr, nullptr, nullptr);
+        std::vector<long long> counts(2 + ncomp_, 0);
+        te_tag(lev, kTeCompute | kTeAvgDown | kTeTags, nullptr, counts.data(), nullptr);
+        last_tags_[lev] = counts;
         if (lev == 0) shadow_ = LevelState();
     }
 }

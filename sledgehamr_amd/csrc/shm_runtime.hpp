@@ -143,6 +143,9 @@ class Runtime {
     void gw_spectrum(int mode, int nbins, int zero_padding, int proj_type,
                      std::vector<double> &bins, std::vector<double> &kvals);
     void step(int ncoarse);              // TimeStepper::Advance(0)
+    /** Tag counters of the most recent scheduled truncation-error tagging
+     *  of lev: [total, user, trunc[ncomp]]. */
+    const std::vector<long long> &last_tags(int lev) const;
 
     void sync();
     void autotune(int lev);
@@ -204,6 +207,8 @@ class Runtime {
     int ncomp_ = 0;
     int R_ = 1;
     bool gw_ = false, ko_ = false;
+    bool independent_ = false;  // level-0 boxes are separate periodic domains
+    std::map<int, std::vector<long long>> last_tags_;
     int integrator_ = 10;
     std::vector<double> nodes_, bbar_, b_;
     std::vector<std::vector<double>> tableau_;

@@ -78,3 +78,53 @@ def random_fields(n, ncomp, seed, scale=0.1, offset=0.0):
 
 def rel_linf(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def ensemble_boxes(nbox, lo, hi, seed, mult=8):
+    """Cubic boxes of seeded edges in [lo, hi] (multiples of mult), all at the
+    index origin: an ensemble of independent periodic realizations."""
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(lo // mult, hi // mult + 1, size=nbox) * mult
+    return [[0, 0, 0, int(n) - 1, int(n) - 1, int(n) - 1] for n in edges]
+
+
+def ensemble_inputs(boxes, integrator=21, order=2, cfl=0.2, dx=1.0 / 64):
+    """configs[0] physics on every box, common dx (1/64), 4th-order
+    Laplacian, ghost width 2."""
+    # Index domain: the next power of two holding the largest box (the
+    # coarse grid size must be a power of two).
+    N = 1 << int(np.ceil(np.log2(max(b[3] + 1 for b in boxes))))
+    return real_scalar(N=N, order=order, integrator=integrator, L=N * dx,
+                       cfl=cfl, nghost=order, **{"shm.independent_boxes": 1})
+
+
+def ensemble_fields(boxes, seed, std=0.1, kcut=16):
+    """Per box a Gaussian random field (box-major, x fastest), pi = 0."""
+    phi = np.concatenate([gaussian_random_field(b[3] + 1, seed + i, kcut=kcut, std=std)
+                          for i, b in enumerate(boxes)])
+    return [phi, np.zeros_like(phi)]
+
+
+def bubble_params(n, t_from, t_to, t_lo=-2.0, t_hi=8.0, phi_in=1.0, R0=3.0, w=2.0):
+    """Initial-state parameters of seeded bubbles (kinds 3 and 4): count,
+    interior value, radius and wall width in level-0 cells, nucleation times
+    uniform in [t_lo, t_hi], injection window (t_from, t_to]."""
+    return [n, phi_in, R0, w, t_lo, t_hi, t_from, t_to]
+
+
+def amr_boxes(seed, nbox=6):
+    """Nested static layout on a 32^3 base: level-1 boxes of edge 8/12/16 at
+    16-cell slots of the 64^3 level-1 domain, each holding one level-2 box."""
+    rng = np.random.default_rng(seed)
+    slots = [(x, y, z) for z in range(4) for y in range(4) for x in range(4)]
+    lev1, lev2 = [], []
+    for s in rng.choice(len(slots), nbox, replace=False):
+        x, y, z = slots[s]
+        e = int(rng.choice([8, 12, 16]))
+        lo = (16 * x, 16 * y, 16 * z)
+        lev1.append([*lo, *(v + e - 1 for v in lo)])
+        e2 = int(rng.choice([8, 12, 16]))
+        if 4 + e2 <= 2 * e:
+            lo2 = tuple(2 * v + 4 for v in lo)
+            lev2.append([*lo2, *(v + e2 - 1 for v in lo2)])
+    return lev1, lev2

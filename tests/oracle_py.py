@@ -23,7 +23,10 @@ def build_oracle() -> None:
 def _load(project: str) -> C.CDLL:
     if project in _LIBS:
         return _LIBS[project]
-    path = os.path.join(ORACLE_DIR, "_build", f"liboracle_{project}.so")
+    # ORACLE_LIB_OVERRIDE points at another build of the same library (for
+    # example one compiled with a host sanitizer).
+    path = os.environ.get("ORACLE_LIB_OVERRIDE") or os.path.join(
+        ORACLE_DIR, "_build", f"liboracle_{project}.so")
     if not os.path.exists(path):
         build_oracle()
     lib = C.CDLL(path, mode=C.RTLD_LOCAL)
@@ -50,6 +53,8 @@ def _load(project: str) -> C.CDLL:
         "orc_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "orc_init_state": (i, [vp, i, i, C.c_ulonglong, dp, i]),
         "orc_r2c": (None, [i, dp, dp]),
+        "orc_evolve": (i, [vp, i]),
+        "orc_last_tags": (i, [vp, i, llp, i]),
         "orc_gw_lambda": (d, [i, i, i, i, i, i, i, i, i]),
         "orc_unique_k2": (i, [i, llp, i]),
     }
@@ -130,6 +135,17 @@ class Oracle:
     def advance(self, lev=0, nsteps=1):
         for _ in range(nsteps):
             self._ck(self.lib.orc_advance(self.h, lev))
+
+    def evolve(self, ncoarse=1):
+        """TimeStepper::Advance(0) repeated on the static layout."""
+        self._ck(self.lib.orc_evolve(self.h, ncoarse))
+
+    def last_tags(self, lev):
+        """Tag counters [total, user, trunc...] of the last scheduled TE."""
+        c = np.zeros(2 + self.ncomp, dtype=np.int64)
+        self._ck(self.lib.orc_last_tags(self.h, lev,
+                                        c.ctypes.data_as(C.POINTER(C.c_longlong)), c.size))
+        return c
 
     def create_shadow(self):
         self._ck(self.lib.orc_create_shadow(self.h))

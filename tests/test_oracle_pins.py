@@ -406,3 +406,118 @@ def test_unique_k2_matches_reference_tables(N):
     assert int(got.sum()) == g["bins_sum"] and int(got[-1]) == g["bins_last"]
     if "bins" in g:
         assert np.array_equal(got, np.array(g["bins"], dtype=np.int64))
+
+
+# ---------------------------------------------------------------------------
+# Independent boxes (ensemble of periodic realizations)
+# ---------------------------------------------------------------------------
+
+def test_ensemble_member_equals_single_periodic_box():
+    """With shm.independent_boxes every box wraps onto itself: each member
+    evolves bit-for-bit as the same box run alone as an ordinary periodic
+    domain (whose ghost fill is the reference FillBoundary)."""
+    from cases import ensemble_boxes, ensemble_fields, ensemble_inputs
+    boxes = ensemble_boxes(4, 8, 24, seed=3)
+    inp = ensemble_inputs(boxes)
+    orc = Oracle("RealScalar", inp)
+    orc.set_boxes(0, boxes)
+    fields = ensemble_fields(boxes, seed=17)
+    for c, f in enumerate(fields):
+        orc.upload(0, c, f)
+    orc.advance(0, 5)
+    out = [orc.download(0, c) for c in range(2)]
+    offs = np.cumsum([0] + [(b[3] + 1) ** 3 for b in boxes])
+    for b, bx in enumerate(boxes):
+        n = bx[3] + 1
+        alone = real_scalar(N=n, order=2, integrator=21, L=n / 64.0, cfl=0.2, nghost=2)
+        one = Oracle("RealScalar", alone)
+        for c in range(2):
+            one.upload(0, c, fields[c][offs[b]:offs[b + 1]])
+        one.advance(0, 5)
+        for c in range(2):
+            assert np.array_equal(one.download(0, c), out[c][offs[b]:offs[b + 1]]), (b, c)
+
+
+# ---------------------------------------------------------------------------
+# Seeded object initial states (bubbles, string loops)
+# ---------------------------------------------------------------------------
+
+def _bubble_params(n, t_from, t_to):
+    # count, phi_in, R0, wall width, nucleation times in [0, 1], window
+    return [n, 1.0, 3.0, 2.0, 0.0, 1.0, t_from, t_to]
+
+
+def test_bubble_injection_composes():
+    """Vacuum plus the bubbles nucleated by t_mid, then the ones nucleated in
+    (t_mid, 1], equals vacuum plus all bubbles nucleated by 1, bit for bit;
+    the field stays within [0, phi_in] and reaches the bubble interior."""
+    N = 24
+    a = Oracle("PhaseTransitionGW", phase_transition(N=N, L=float(N)))
+    b = Oracle("PhaseTransitionGW", phase_transition(N=N, L=float(N)))
+    a.init_state(0, 3, 42, _bubble_params(10, -1.0, 1.0))
+    b.init_state(0, 3, 42, _bubble_params(10, -1.0, 0.4))
+    b.init_state(0, 4, 42, _bubble_params(10, 0.4, 1.0))
+    pa, pb = a.download(0, 0), b.download(0, 0)
+    assert np.array_equal(pa, pb)
+    assert pa.min() >= 0.0 and pa.max() <= 1.0 and pa.max() > 0.9
+    # Fewer bubbles nucleated by an earlier time: pointwise smaller field.
+    c = Oracle("PhaseTransitionGW", phase_transition(N=N, L=float(N)))
+    c.init_state(0, 3, 42, _bubble_params(10, -1.0, 0.4))
+    assert np.all(c.download(0, 0) <= pa) and c.download(0, 0).sum() < pa.sum()
+
+
+def _plaquette_winding(psi, axes):
+    """Winding of arg(psi) around every unit plaquette spanned by two axes
+    (periodic), in units of 2 pi."""
+    a, b = axes
+    p00 = psi
+    p10 = np.roll(psi, -1, axis=a)
+    p11 = np.roll(p10, -1, axis=b)
+    p01 = np.roll(psi, -1, axis=b)
+    d = lambda u, v: np.angle(v / u)
+    w = d(p00, p10) + d(p10, p11) + d(p11, p01) + d(p01, p00)
+    return np.rint(w / (2 * np.pi)).astype(int), w
+
+
+def test_string_loops_carry_conserved_unit_winding():
+    """psi = prod tanh(d/w) exp(i theta): every plaquette winds by 0 or
+    +-1, strings are present, and the winding flux out of every cell cube
+    vanishes (strings neither end nor branch)."""
+    N = 32
+    from cases import complex_scalar
+    orc = Oracle("ComplexScalar", complex_scalar(N=N))
+    orc.init_state(0, 5, 11, [3, 5.0, 9.0, 1.5])
+    psi = (orc.download(0, 0) + 1j * orc.download(0, 1)).reshape(N, N, N)  # [z, y, x]
+    assert np.max(np.abs(psi)) <= 1.0
+    # Plaquette normals: x (axes y, z), y (axes z, x), z (axes x, y).
+    wx, rx = _plaquette_winding(psi, (1, 0))
+    wy, ry = _plaquette_winding(psi, (0, 2))
+    wz, rz = _plaquette_winding(psi, (2, 1))
+    for w, r in ((wx, rx), (wy, ry), (wz, rz)):
+        assert np.max(np.abs(r / (2 * np.pi) - w)) < 1e-9
+        assert set(np.unique(w)) <= {-1, 0, 1}
+    assert np.count_nonzero(wx) + np.count_nonzero(wy) + np.count_nonzero(wz) > 20
+    div = (np.roll(wx, -1, axis=2) - wx) + (np.roll(wy, -1, axis=1) - wy) + \
+          (np.roll(wz, -1, axis=0) - wz)
+    assert np.count_nonzero(div) == 0
+
+
+def test_oracle_subcycled_schedule():
+    """Sub-cycling on a static 3-level layout: level l advances 2^l times
+    per coarse step, times stay synchronised, and a regrid scheduled on
+    level 0 runs truncation errors on every level."""
+    from cases import amr_boxes, complex_scalar
+    lev1, lev2 = amr_boxes(seed=2024)
+    inp = complex_scalar(N=32, integrator=10, nghost=2, L=8.0, cfl=0.25,
+                         t0=0.125, **{"amr.max_refinement_levels": 2,
+                                      "amr.te_crit": 1e-3, "amr.regrid_dt": 0.01})
+    orc = Oracle("ComplexScalar", inp)
+    orc.set_boxes(1, lev1)
+    orc.set_boxes(2, lev2)
+    for lev in range(3):
+        orc.init_state(lev, 5, 77, [4, 5.0, 10.0, 1.5])
+    orc.evolve(3)
+    dt0 = 0.25 * 8.0 / 32   # binary-exact times: the scheduler compares them exactly
+    for lev in range(3):
+        assert orc.time(lev) == 0.125 + 3 * dt0
+        assert orc.last_tags(lev)[0] >= 0
