@@ -97,6 +97,20 @@ def run_gpu(shm, inp, prec, steps, warmup, seed, launch=None):
     return sim, t1 - t0, tm
 
 
+def graph_ms_per_step(sim, steps):
+    """Wall time per step with each level step replayed from a captured
+    graph (the first step after enabling captures it)."""
+    sim.set_graphs(True)
+    sim.advance(0, 2)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.advance(0, steps)
+    sim.sync()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    sim.set_graphs(False)
+    return ms
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -132,6 +146,58 @@ def cpu_baseline(inp, seed):
                       f"{threads} threads on {cpu_model()}, {dt:.1f} s"}
 
 
+def ensemble(shm, seed, steps=200, warmup=10):
+    """Ensemble of 64 independent real-scalar realizations (edges 32..160,
+    seeded), RKN4, 4th-order Laplacian, one launch per stage over the box
+    table: p50 / p99 wall time per step (each step synchronised), aggregate
+    cell-updates/s, and the FP32-compensated field error against FP64."""
+    from cases import ensemble_boxes, ensemble_fields, ensemble_inputs
+    boxes = ensemble_boxes(64, 32, 160, seed)
+    inp = ensemble_inputs(boxes)
+    fields = ensemble_fields(boxes, seed)
+    out = {}
+    for prec in (shm.PREC_FP32C16, shm.PREC_FP64):
+        sim = shm.Simulation("RealScalar", inp, precision=prec)
+        sim.set_boxes(0, boxes)
+        for c, f in enumerate(fields):
+            sim.upload(0, c, f)
+        sim.autotune(0)
+        sim.set_graphs(True)
+        sim.advance(0, warmup)
+        sim.sync()
+        lat = []
+        n = steps if prec == shm.PREC_FP32C16 else 0
+        for _ in range(n):
+            t0 = time.perf_counter()
+            sim.advance(0, 1)
+            sim.sync()
+            lat.append(time.perf_counter() - t0)
+        if n:
+            lat = np.array(lat) * 1e3
+            cells = sim.ncells(0)
+            out = {"workload": "64 independent real-scalar boxes, edges 32..160 "
+                               "(seeded), dx 1/64, RKN4, 4th-order Laplacian, "
+                               "FP32 + bf16 remainder, steps replayed from "
+                               "captured graphs",
+                   "boxes": len(boxes), "cells": cells, "steps": n,
+                   "p50_ms": float(np.percentile(lat, 50)),
+                   "p99_ms": float(np.percentile(lat, 99)),
+                   "value": cells * 3 * n / (lat.sum() * 1e-3),
+                   "unit": "cell-updates/s"}
+            u32 = [sim.download(0, c) for c in range(2)]
+            ref_steps = warmup + n
+        else:
+            sim.advance(0, ref_steps - warmup)
+            err = 0.0
+            for c in range(2):
+                x64 = sim.download(0, c)
+                err = max(err, float(np.max(np.abs(u32[c] - x64)) /
+                                     max(np.max(np.abs(x64)), 1e-300)))
+            out["rel_err_vs_fp64"] = err
+        sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +207,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20240401)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--launch", default="",
                     help="fixed stage launch shape 'variant,tile_z,xcd_split,"
                          "persistent,zc_minor' instead of the autotuner")
@@ -245,7 +312,14 @@ def main():
                            "remainder": "fp32"}
         simc.close()
         sim64.close()
+    # After the comparisons above (this advances the FP32 run further).
+    result["graphs"] = {"ms_per_step": graph_ms_per_step(sim32, a.steps),
+                        "note": "same workload, level steps replayed from "
+                                "captured HIP graphs, no kernel timing"}
     sim32.close()
+
+    if not a.no_ensemble:
+        result["ensemble"] = ensemble(shm, a.seed)
 
     if not a.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(inp, a.seed)

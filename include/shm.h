@@ -67,6 +67,7 @@ typedef struct shm_info_t {
     int with_gravitational_waves;
     int shadow_hierarchy;
     double L, cfl, t_start, t_end;
+    long long graph_replays;  /* level steps run from captured graphs */
 } shm_info_t;
 
 /* Sledgehamr::Sledgehamr + InitSledgehamr (sledgehamr.cpp:15-77): build the
@@ -158,6 +159,14 @@ int shm_autotune(shm_ctx *ctx, int lev);
 int shm_set_launch(shm_ctx *ctx, const shm_launch_t *cfg);
 /* The launch shape a level's stage kernels use now. */
 int shm_get_launch(shm_ctx *ctx, int lev, shm_launch_t *cfg);
+
+/* Level steps as captured HIP graphs (also shm.graphs = 1 in the inputs
+ * file).  No counterpart in the reference: the stage launches and ghost
+ * fills of one Integrator::Integrate are captured per (level, state slot)
+ * on the first step after an eager one and replayed with fresh stage
+ * records afterwards; layout or launch-shape changes recapture.  Steps run
+ * eagerly while kernel timing is on. */
+int shm_set_graphs(shm_ctx *ctx, int on);
 
 /* Kernel timing with HIP events on the library's stream. */
 int shm_set_timing(shm_ctx *ctx, int on);

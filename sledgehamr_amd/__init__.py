@@ -30,7 +30,7 @@ EXPORTS = (
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
     "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
-    "shm_get_launch", "shm_set_timing", "shm_get_timing",
+    "shm_get_launch", "shm_set_graphs", "shm_set_timing", "shm_get_timing",
 )
 
 
@@ -55,7 +55,8 @@ class _Info(C.Structure):
         "ncomp", "nscalars", "nfields", "nghost", "stencil_radius",
         "precision", "max_level", "finest_level", "coarse_level_grid_size",
         "integrator_type", "with_gravitational_waves", "shadow_hierarchy")] + \
-        [(n, C.c_double) for n in ("L", "cfl", "t_start", "t_end")]
+        [(n, C.c_double) for n in ("L", "cfl", "t_start", "t_end")] + \
+        [("graph_replays", C.c_longlong)]
 
 
 _LIBS: dict[str, C.CDLL] = {}
@@ -102,6 +103,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_autotune": (i, [vp, i]),
         "shm_set_launch": (i, [vp, C.POINTER(_Launch)]),
         "shm_get_launch": (i, [vp, i, C.POINTER(_Launch)]),
+        "shm_set_graphs": (i, [vp, i]),
         "shm_set_timing": (i, [vp, i]),
         "shm_get_timing": (i, [vp, dp, llp, llp, dp, llp]),
     }
@@ -298,6 +300,10 @@ class Simulation:
         cfg = _Launch()
         self._check(self.lib.shm_get_launch(self._ctx, lev, C.byref(cfg)))
         return {n: getattr(cfg, n) for n, _ in _Launch._fields_}
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay captured level steps (HIP graphs)."""
+        self._check(self.lib.shm_set_graphs(self._ctx, int(on)))
 
     def set_timing(self, on: bool) -> None:
         self._check(self.lib.shm_set_timing(self._ctx, int(on)))

@@ -121,6 +121,7 @@ __attribute__((visibility("default"))) int shm_get_info(shm_ctx *ctx,
         double crit = DBL_MAX;
         pp.query("amr.te_crit", crit);
         info->shadow_hierarchy = crit != DBL_MAX;
+        info->graph_replays = ctx->rt->graph_replays;
     });
 }
 
@@ -335,6 +336,7 @@ __attribute__((visibility("default"))) int shm_set_launch(shm_ctx *ctx,
         c.persistent = flag(cfg->persistent, "persistent");
         c.zc_minor = flag(cfg->zc_minor, "zc_minor");
         ctx->rt->launch = c;
+        ctx->rt->bump_epoch();
     });
 }
 
@@ -351,6 +353,14 @@ __attribute__((visibility("default"))) int shm_get_launch(shm_ctx *ctx, int lev,
         cfg->xcd_split = c.xcd_split;
         cfg->persistent = c.persistent;
         cfg->zc_minor = c.zc_minor;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_set_graphs(shm_ctx *ctx, int on) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->sync();
+        ctx->rt->graphs = on != 0;
     });
 }
 

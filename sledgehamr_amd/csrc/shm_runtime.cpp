@@ -370,6 +370,7 @@ Runtime::Runtime(sledgehamr::Sledgehamr *sim, const KernelTable &kt)
 
 Runtime::~Runtime() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    release_graphs();
     for (EventPair &p : ev_pool_) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -580,6 +581,7 @@ void Runtime::init_from_inputs(const std::string &inputs,
     if (independent_ && S.max_level > 0)
         throw Error(1, "shm.independent_boxes needs a single level");
     pp.query("shm.tile_order_zc_minor", launch.zc_minor);
+    pp.query("shm.graphs", graphs);
     if (launch.variant >= kNVar) throw Error(1, "shm.tile_variant out of range");
 
     // Project initialisation, then the reach of its cell functions.
@@ -655,6 +657,7 @@ void Runtime::probe() {
 }
 
 void Runtime::set_level(int lev, const std::vector<Box3> &boxes) {
+    bump_epoch();
     if (lev < 0 || lev > sim_->max_level) throw Error(1, "level out of range");
     if ((int)levels_.size() <= lev) levels_.resize(lev + 1);
     LevelState &ls = levels_[lev];
@@ -845,11 +848,23 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                         double t0) {
     const int old = ls.cur ^ 1, nw = ls.cur;
     std::vector<double> blk(kPrmBlock);
-    auto prm = [&](double time) {
+    auto prm = [&](double time) -> const double * {
+        if (capture_) {
+            // Captured step: a copy node from the graph's pinned record into
+            // its device record; the host fills the pinned record per replay.
+            const int i = capture_stage_++;
+            if (i >= capture_->nstage) throw Error(5, "stage count changed in capture");
+            double *h = capture_->host + (size_t)i * kPrmBlock;
+            double *d = capture_->dev.as<double>() + (size_t)i * kPrmBlock;
+            SHM_HIP(hipMemcpyAsync(d, h, sizeof(double) * kPrmBlock,
+                                   hipMemcpyHostToDevice, stream_));
+            return d;
+        }
         fill_params(blk.data(), time, dt, dx, lev);
         return push_params(prm_host_, prm_ev_, prm_slot_,
                            prm_dev_.as<double>(), stream_, blk);
     };
+    const std::vector<double> st = stage_times(t0, dt);
     const int npair = kt_.nfields + (gw_ ? 6 : 0);
     std::vector<int> pairs(npair);
     for (int p = 0; p < npair; ++p) pairs[p] = p;
@@ -860,16 +875,15 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
     if (integrator_ == 10) {
         // Low-storage SSPRK3 with the stage states rebuilt from u0 and k:
         // k1 -> K[0], k2 -> K[1] (k is read at neighbours, so not in place).
-        const double t1 = t0 + dt;
         StageArgs a{};
-        a.prm = prm(t0);
+        a.prm = prm(st[0]);
         a.x = ls.U[old].get();
         a.kout = ls.K[0].get();
         launch_stage(kSsprk1, ls, a, lev);
         halo(ls, ls.K[0].get(), stencil_comps());
 
         StageArgs b{};
-        b.prm = prm(t1);
+        b.prm = prm(st[1]);
         b.u0 = ls.U[old].get();
         b.kin = ls.K[0].get();
         b.kout = ls.K[1].get();
@@ -879,7 +893,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         halo(ls, ls.K[1].get(), stencil_comps());
 
         StageArgs c{};
-        c.prm = prm(t0 + dt / 2.);
+        c.prm = prm(st[2]);
         c.u0 = ls.U[old].get();
         c.u0lo = ulo_old;
         c.kin = ls.K[1].get();
@@ -893,7 +907,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         const int s = (int)nodes_.size();
         for (int i = 0; i < s; ++i) {
             StageArgs a{};
-            a.prm = prm(t0 + dt * nodes_[i]);
+            a.prm = prm(st[i]);
             a.u0 = ls.U[old].get();
             a.u0lo = ulo_old;
             a.c[0] = dt * nodes_[i];
@@ -920,9 +934,8 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             }
         }
     } else if (integrator_ == 11) {
-        const double t1 = t0 + dt;
         StageArgs a{};
-        a.prm = prm(t0);
+        a.prm = prm(st[0]);
         a.x = ls.U[old].get();
         a.u0 = ls.U[old].get();
         a.u0lo = ulo_old;
@@ -935,7 +948,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         halo(ls, ls.U[nw].get(), stencil_comps());
 
         StageArgs b{};
-        b.prm = prm(t1);
+        b.prm = prm(st[1]);
         b.x = ls.U[nw].get();
         b.vh = ls.VH.get();
         b.unew = ls.U[nw].get();
@@ -948,6 +961,79 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
     ls.ghosts_ok[nw] = true;
 }
 
+std::vector<double> Runtime::stage_times(double t0, double dt) const {
+    if (integrator_ == 10) return {t0, t0 + dt, t0 + dt / 2.};
+    if (integrator_ == 11) return {t0, t0 + dt};
+    std::vector<double> t;
+    for (double c : nodes_) t.push_back(t0 + dt * c);
+    return t;
+}
+
+void Runtime::release_graphs() {
+    for (auto &kv : graphs_) {
+        StepGraph &g = kv.second;
+        if (g.done) (void)hipEventSynchronize(g.done);
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.done) (void)hipEventDestroy(g.done);
+        if (g.host) (void)hipHostFree(g.host);
+    }
+    graphs_.clear();
+}
+
+/** Runs Integrate for a level from a captured graph: captured on the first
+ *  step after an eager one with the same layout and launch shape (so every
+ *  lazily built table exists), replayed afterwards.  Returns false when the
+ *  step has to run eagerly. */
+bool Runtime::replay_step(LevelState &ls, int lev, double dt, double dx, double t0) {
+    if (!graphs || timing_on || lev < 0) return false;
+    auto w = warm_.find(lev);
+    if (w == warm_.end() || w->second != epoch_) return false;
+    const std::vector<double> st = stage_times(t0, dt);
+    StepGraph &g = graphs_[{lev, ls.cur ^ 1}];
+    if (g.epoch != epoch_) {
+        if (g.exec) {
+            if (g.done) SHM_HIP(hipEventSynchronize(g.done));
+            SHM_HIP(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+        }
+        if (!g.host) {
+            g.nstage = (int)st.size();
+            SHM_HIP(hipHostMalloc((void **)&g.host, sizeof(double) * kPrmBlock * g.nstage,
+                                  hipHostMallocDefault));
+            g.dev.alloc(sizeof(double) * kPrmBlock * g.nstage);
+            SHM_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+        }
+        hipGraph_t graph = nullptr;
+        capture_ = &g;
+        capture_stage_ = 0;
+        SHM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            integrate(ls, lev, dt, dx, t0);
+        } catch (...) {
+            capture_ = nullptr;
+            (void)hipStreamEndCapture(stream_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        capture_ = nullptr;
+        SHM_HIP(hipStreamEndCapture(stream_, &graph));
+        const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        SHM_HIP(ie);
+        g.epoch = epoch_;
+    } else {
+        ls.ghosts_ok[ls.cur] = true;  // what integrate() records eagerly
+    }
+    // The previous replay of this graph has read its pinned records.
+    SHM_HIP(hipEventSynchronize(g.done));
+    for (int i = 0; i < g.nstage; ++i)
+        fill_params(g.host + (size_t)i * kPrmBlock, st[i], dt, dx, lev);
+    SHM_HIP(hipGraphLaunch(g.exec, stream_));
+    SHM_HIP(hipEventRecord(g.done, stream_));
+    ++graph_replays;
+    return true;
+}
+
 void Runtime::advance(int lev) {
     LevelState &ls = level(lev);
     ls.cur ^= 1;  // swap grid_old / grid_new
@@ -955,7 +1041,10 @@ void Runtime::advance(int lev) {
     ls.t_old = ls.t;
     const double dt = sim_->dt[lev], dx = sim_->dx[lev];
     fill_patch(ls, old);
-    integrate(ls, lev, dt, dx, ls.t_old);
+    if (!replay_step(ls, lev, dt, dx, ls.t_old)) {
+        integrate(ls, lev, dt, dx, ls.t_old);
+        if (lev >= 0) warm_[lev] = epoch_;
+    }
     ls.t = ls.t_old + dt;
     ls.istep += 1;
 }
@@ -1574,6 +1663,7 @@ void Runtime::autotune(int lev) {
     launch = saved;
     timing_on = saved_t;
     tuned_[lev + 1] = bc;
+    bump_epoch();
 }
 
 // ---------------------------------------------------------------------------

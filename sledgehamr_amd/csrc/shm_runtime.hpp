@@ -113,6 +113,19 @@ struct LaunchCfg {
     int zc_minor = -1;    // tile order: z-chunk outermost (0) or innermost (1)
 };
 
+/** A captured level step (swap excluded): the stage launches and ghost
+ *  fills of Integrator::Integrate for one (level, old-state slot), with its
+ *  own pinned / device parameter records that the host refreshes before
+ *  every replay. */
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;   // recorded after the last replay
+    double *host = nullptr;      // pinned, nstage parameter records
+    DevMem dev;                  // device copies read by the stage kernels
+    int nstage = 0;
+    long epoch = -1;             // layout / launch-shape generation captured
+};
+
 struct Timing {
     double stage_ms = 0, halo_ms = 0;
     long long stage_launches = 0, halo_launches = 0;
@@ -169,6 +182,10 @@ class Runtime {
     Timing timing;
     bool timing_on = false;
     LaunchCfg launch;   // user overrides of the launch shape
+    bool graphs = false;   // replay captured level steps (shm.graphs)
+    long long graph_replays = 0;
+    /** Invalidates captured steps (layout or launch shape changed). */
+    void bump_epoch() { ++epoch_; }
     /** Launch shape used for level lev (overrides, tuned values, defaults). */
     LaunchCfg resolved_launch(int lev) const;
 
@@ -192,6 +209,17 @@ class Runtime {
     void fill_patch(LevelState &ls, int which);
     void integrate(LevelState &ls, int lev, double dt, double dx,
                    double t_old);
+    /** Times at which the integrator evaluates its stages. */
+    std::vector<double> stage_times(double t0, double dt) const;
+    bool replay_step(LevelState &ls, int lev, double dt, double dx, double t0);
+    void release_graphs();
+    // Parameter source while a step is being captured: stage i reads the
+    // graph's device record i, copied from its pinned record inside the graph.
+    StepGraph *capture_ = nullptr;
+    int capture_stage_ = 0;
+    std::map<std::pair<int, int>, StepGraph> graphs_;  // (level, old slot)
+    std::map<int, long> warm_;   // level -> epoch of its last eager step
+    long epoch_ = 0;
     void probe();
     std::vector<int> stencil_comps() const;
     std::vector<int> all_comps() const;
