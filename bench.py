@@ -13,17 +13,20 @@ state + bf16 remainder; valid cells x stages / wall time), inputs resident in
 HBM.  The same seeded run on the FP64 path gives `fp64` and the field error;
 `fp32c` is the FP32 + FP32-remainder variant.
 
-roofline: the fused stage kernel's algorithmic bytes per launch over its mean
+roofline: the stage kernels' algorithmic bytes per launch over their mean
 launch time, measured with HIP events on the library's stream inside the
-timed region.  Algorithmic bytes per cell-stage are those of the minimal
-fused low-storage schedule (every stored array read once and written at most
-once per stage: 12 words per component and step, plus one read and one write
-of the remainder per step): 69.3 B for FP32 + bf16 remainder, 74.7 B with an
-FP32 remainder, 128 B for FP64 (C = 4).  This schedule rebuilds the SSPRK3
-stage states from u0 and k instead of storing them (8 words per component and
-step): the bytes it actually moves, 48 / 53.3 / 85.3 B per cell-stage, give
-`moved_frac`.  `traffic` is the HBM bytes per launch measured by rocprofv3
-counters (DRAM read and write requests of the same stage launches).
+timed region (cells x stages per step / launches per step per launch).
+Algorithmic bytes per cell-stage are those of the minimal low-storage
+schedule that processes one stage per pass (every stored array read once and
+written at most once per stage: 12 words per component and step, plus one
+read and one write of the remainder per step): 69.3 B for FP32 + bf16
+remainder, 74.7 B with an FP32 remainder, 128 B for FP64 (C = 4).  This
+library moves fewer: stage states are rebuilt from u0 and k (8 words per
+component and step: 48 / 53.3 / 85.3 B per cell-stage), and on a level of
+periodic boxes stages one and two run in one pass that keeps k1 on chip (5
+words: 32 / 37.3 / 53.3 B); those bytes give `moved_frac`.  `traffic` is the
+HBM bytes per launch measured by rocprofv3 counters (DRAM read and write
+requests of the same stage launches).
 
 cpu_baseline: the FP64 oracle in the reference's unfused structure (Rhs pass,
 LinComb passes, ghost fill per stage; OpenMP), whole SSPRK3 steps of the same
@@ -57,6 +60,15 @@ BYTES_PER_CELL_STAGE = {"fp32c16": (12 * 4 + 2 * 2) * C / 3.0,
 MOVED_BYTES_PER_CELL_STAGE = {"fp32c16": (8 * 4 + 2 * 2) * C / 3.0,
                               "fp32c": (8 * 4 + 2 * 4) * C / 3.0,
                               "fp64": 8 * 8 * C / 3.0}
+# With stages one and two fused (k1 kept on chip): stage 1+2 reads u0 and
+# writes k2, stage 3 reads u0, k2 and writes u (5 words), plus the remainder.
+MOVED_BYTES_FUSED = {"fp32c16": (5 * 4 + 2 * 2) * C / 3.0,
+                     "fp32c": (5 * 4 + 2 * 4) * C / 3.0,
+                     "fp64": 5 * 8 * C / 3.0}
+
+
+def moved_bytes(prec_key, fused):
+    return (MOVED_BYTES_FUSED if fused else MOVED_BYTES_PER_CELL_STAGE)[prec_key]
 # HBM bytes per stage launch (mean over the three SSPRK3 stages) at 512^3,
 # FP32 + bf16 remainder, 64 x 16 tiles: DRAM read + write requests counted by
 # rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B TCC_EA0_WRREQ_WRITE_DRAM_32B (x 32 B),
@@ -257,7 +269,9 @@ def main():
     cells_pl = tm32["stage_cells"] / max(launches, 1)
     bpl = BYTES_PER_CELL_STAGE["fp32c16"] * cells_pl
     achieved = bpl / (avg_ms * 1e-3) / 1e9
-    moved = MOVED_BYTES_PER_CELL_STAGE["fp32c16"] * cells_pl / (avg_ms * 1e-3) / 1e9
+    fused32 = bool(sim32.get_launch(0)["fused"])
+    mpl = moved_bytes("fp32c16", fused32) * cells_pl
+    moved = mpl / (avg_ms * 1e-3) / 1e9
     traffic = TRAFFIC_BYTES_PER_LAUNCH_512 if N == 512 else None
     result["roofline"] = {"bound": "hbm", "achieved": achieved,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -265,8 +279,8 @@ def main():
                           "kernel": "shm::stage_kernel<ComplexScalar, PolSsprk<1..3>, F32C16>",
                           "avg_launch_ms": avg_ms,
                           "algorithmic_bytes_per_launch": bpl,
-                          "moved_bytes_per_launch":
-                              MOVED_BYTES_PER_CELL_STAGE["fp32c16"] * cells_pl,
+                          "moved_bytes_per_launch": mpl,
+                          "stages_one_two_fused": fused32,
                           "moved_achieved": moved,
                           "moved_frac": moved / HBM_PEAK_GBS}
     result["launch"] = sim32.get_launch(0)
@@ -288,10 +302,12 @@ def main():
             err = max(err, float(np.max(np.abs(x32 - x64)) /
                                  max(np.max(np.abs(x64)), 1e-300)))
             del x32, x64
-        m64 = MOVED_BYTES_PER_CELL_STAGE["fp64"] * tm64["stage_cells"] / max(l64, 1)
+        fused64 = bool(sim64.get_launch(0)["fused"])
+        m64 = moved_bytes("fp64", fused64) * tm64["stage_cells"] / max(l64, 1)
         result["fp64"] = {"value": rate64, "ms_per_step": sec64 / a.steps * 1e3,
                           "roofline_frac": b64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                          "moved_frac": m64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                          "moved_frac": m64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "launch": sim64.get_launch(0)}
         result["speedup_fp32c_vs_fp64"] = rate32 / rate64
         result["fp32c_rel_err_vs_fp64"] = err
         simc, secc, tmc = run_gpu(shm, inp, shm.PREC_FP32C, a.steps, a.warmup,

@@ -151,6 +151,9 @@ typedef struct shm_launch_t {
     int xcd_split;    /* 1: each XCD walks one contiguous range of tiles */
     int persistent;   /* 1: grid = resident capacity; 0: one tile per workgroup */
     int zc_minor;     /* tile order: 0 = z chunk outermost, 1 = between y and x */
+    int fused;        /* SSPRK3 on levels of periodic boxes: stages one and two
+                         in one launch (k1 kept on chip) */
+    int fused_variant;/* tile shape of the fused launch (0..2 as variant) */
 } shm_launch_t;
 
 /* Times the stage kernels of a level over every launch shape not fixed by

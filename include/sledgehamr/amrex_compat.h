@@ -277,6 +277,9 @@ struct TileCtx {
     const void *zq;
     const double *ctr;
     const signed char *slot;
+    // LDS-window mode (zq unused): element offset, relative to the centre
+    // plane, of plane ck + dk for dk = -R..R.
+    const int *zplane = nullptr;
 
     AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double get(int i, int j, int k,
                                                         int n) const {
@@ -284,6 +287,11 @@ struct TileCtx {
         if (di == 0 && dj == 0 && dk == 0) return ctr[n];
         const int s = slot[n];
         if (s < 0) return __builtin_nan("");
+        if (dk != 0 && zplane) {
+            const int l = s * lds_cstride + zplane[dk + R] + dj * lds_pitch + di;
+            return f32 ? (double)static_cast<const float *>(lds)[l]
+                       : static_cast<const double *>(lds)[l];
+        }
         if (dk != 0) {
             if (di != 0 || dj != 0) return __builtin_nan("");
             const int q = s * zq_cstride + dk + R;

@@ -67,12 +67,17 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool tile_lap_f32(
     if (s < 0) return false;
     constexpr detail::LapWeights W = detail::lap_weights(P);
     const float *L = static_cast<const float *>(tc.lds) + s * tc.lds_cstride;
-    const float *Z = static_cast<const float *>(tc.zq) + s * tc.zq_cstride + tc.R;
-    double acc = W.w[0] * (double)Z[0];
+    // z neighbours: the register queue, or the LDS window's other planes.
+    const float *Z = tc.zplane ? nullptr
+                               : static_cast<const float *>(tc.zq) + s * tc.zq_cstride + tc.R;
+    auto zv = [&](int m) -> double {
+        return tc.zplane ? (double)L[tc.zplane[tc.R + m]] : (double)Z[m];
+    };
+    double acc = W.w[0] * zv(0);
     constexpr_for<1, P + 1, 1>([&](auto m) {
         const double ring = ((double)L[m] + (double)L[-m]) +
                             ((double)L[m * tc.lds_pitch] + (double)L[-m * tc.lds_pitch]) +
-                            ((double)Z[m] + (double)Z[-m]);
+                            (zv(m) + zv(-m));
         acc += W.w[m] * ring;
     });
     sum = acc;
@@ -124,7 +129,14 @@ Gradient(amrex::Array4<amrex::Real const> const &state, const int i,
             if (tc.f32 && s >= 0 && i == tc.ci && j == tc.cj && k == tc.ck) {
                 // Differences of FP32 values are exact in FP64.
                 double acc = 0.;
-                if (az) {
+                if (az && tc.zplane) {
+                    const float *L =
+                        static_cast<const float *>(tc.lds) + s * tc.lds_cstride;
+                    constexpr_for<1, P + 1, 1>([&](auto m) {
+                        acc += W.g[m] * ((double)L[tc.zplane[tc.R + m]] -
+                                         (double)L[tc.zplane[tc.R - m]]);
+                    });
+                } else if (az) {
                     const float *Z = static_cast<const float *>(tc.zq) +
                                      s * tc.zq_cstride + tc.R;
                     constexpr_for<1, P + 1, 1>([&](auto m) {

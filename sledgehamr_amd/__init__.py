@@ -47,7 +47,8 @@ class _Config(C.Structure):
 
 class _Launch(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
-        "variant", "tile_z", "xcd_split", "persistent", "zc_minor")]
+        "variant", "tile_z", "xcd_split", "persistent", "zc_minor", "fused",
+        "fused_variant")]
 
 
 class _Info(C.Structure):
@@ -290,9 +291,11 @@ class Simulation:
         self._check(self.lib.shm_autotune(self._ctx, lev))
 
     def set_launch(self, variant: int = -1, tile_z: int = 0, xcd_split: int = -1,
-                   persistent: int = -1, zc_minor: int = -1) -> None:
+                   persistent: int = -1, zc_minor: int = -1, fused: int = -1,
+                   fused_variant: int = -1) -> None:
         """Fixes fields of the stage-kernel launch shape (-1 / 0: not set)."""
-        cfg = _Launch(variant, tile_z, xcd_split, persistent, zc_minor)
+        cfg = _Launch(variant, tile_z, xcd_split, persistent, zc_minor, fused,
+                      fused_variant)
         self._check(self.lib.shm_set_launch(self._ctx, C.byref(cfg)))
 
     def get_launch(self, lev: int = 0) -> dict:

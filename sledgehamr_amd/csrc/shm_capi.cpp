@@ -335,6 +335,10 @@ __attribute__((visibility("default"))) int shm_set_launch(shm_ctx *ctx,
         c.xcd_split = flag(cfg->xcd_split, "xcd_split");
         c.persistent = flag(cfg->persistent, "persistent");
         c.zc_minor = flag(cfg->zc_minor, "zc_minor");
+        c.fused = flag(cfg->fused, "fused");
+        if (cfg->fused_variant >= shm::kNVar || cfg->fused_variant < -1)
+            throw shm::Error(SHM_ERR_ARG, "bad fused tile variant");
+        c.fused_variant = cfg->fused_variant;
         ctx->rt->launch = c;
         ctx->rt->bump_epoch();
     });
@@ -353,6 +357,8 @@ __attribute__((visibility("default"))) int shm_get_launch(shm_ctx *ctx, int lev,
         cfg->xcd_split = c.xcd_split;
         cfg->persistent = c.persistent;
         cfg->zc_minor = c.zc_minor;
+        cfg->fused = c.fused && ctx->rt->fused_possible(ctx->rt->level(lev)) ? 1 : 0;
+        cfg->fused_variant = c.fused_variant;
     });
 }
 

@@ -1,0 +1,352 @@
+// Fused first and second stages of the low-storage SSPRK3 step
+// (lsssprk3.cpp:20-25): k2 = f(u0 + dt k1) + k1 with k1 = f(u0), in one pass
+// over the level, so k1 is neither written nor read back.
+//
+// The second stage needs u1 = u0 + dt k1 within R cells of each tile cell,
+// so the first stage is evaluated on the tile plus a ring of R cells (the
+// ring values are what the neighbouring tiles compute for themselves, so
+// the result equals the two-launch path bit for bit).  Per tile and plane:
+//   * u0 of the stencil components over the tile plus 2R cells sits in a
+//     circular LDS window of 2R+1 planes, u0 of the point-wise components of
+//     the current plane over the tile plus R cells in one LDS plane;
+//   * stage one at plane zz writes u1 of the stencil components into a
+//     second window (tile plus R) and keeps k1 and the point-wise u1 of the
+//     tile's own cells in a register delay line of R+1 planes;
+//   * stage two at plane zz - R reads the u1 window and the delay line and
+//     writes k2.
+// Planes are loaded one plane ahead; two barriers per plane.  Applies to
+// boxes that wrap periodically along every axis (a single box spanning the
+// domain, or an ensemble of independent boxes), where the extra reach needs
+// no ghost cells.
+#pragma once
+
+#include "shm_stage.hpp"
+
+namespace shm {
+
+struct FusedArgs {
+    const BoxDesc *boxes;
+    const TileDesc *tiles;
+    int ntiles;
+    int lev;
+    const double *prm1;  // stage record of stage one (t0)
+    const double *prm2;  // stage record of stage two (t0 + dt)
+    int xcd_split;
+    int ghost;
+    long long cstride;
+    const void *u0;
+    void *kout;
+    double dt;           // u1 = u0 + dt k1
+};
+
+/** Static shape of a fused tile: 64 x NW cells, radius R. */
+template <class Ops, Prec P, int R, bool GW, bool KO, int NW>
+struct FusedShape {
+    using S = typename PrecTraits<P>::S;
+    using CS = CompSet<Ops, GW, KO>;
+    static constexpr int NC = CS::NC, NS = CS::NS, NP = CS::NP;
+    static constexpr int TX = 64, TY = NW, NT = 64 * NW, W = 2 * R + 1;
+    static constexpr int E1X = TX + 2 * R, E1Y = TY + 2 * R, E1 = E1X * E1Y;
+    static constexpr int E0X = TX + 4 * R, E0Y = TY + 4 * R, E0 = E0X * E0Y;
+    static constexpr int L0 = (E0 + NT - 1) / NT;  // window points per thread
+    static constexpr int L1 = (E1 + NT - 1) / NT;  // stage-one points per thread
+    static constexpr long lds_bytes =
+        (long)sizeof(S) * ((long)NS * W * E0 + (long)NS * W * E1 + (long)(NP > 0 ? NP : 1) * E1);
+    // Instantiated when the LDS fits one workgroup per CU and the delay
+    // line stays small.
+    static constexpr bool fits =
+        !KO && lds_bytes <= 160 * 1024 && L1 <= 3 &&
+        (NC + NP) * (R + 1) * (int)(sizeof(S) / 4) <= 48;
+};
+
+template <class Ops, Prec P, int R, bool GW, bool KO, int NW>
+__global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
+    using F = FusedShape<Ops, P, R, GW, KO, NW>;
+    using S = typename F::S;
+    using CS = typename F::CS;
+    constexpr int NC = F::NC, NS = F::NS, NP = F::NP, NPA = NP > 0 ? NP : 1;
+    constexpr int W = F::W, NT = F::NT;
+    constexpr int E0X = F::E0X, E0 = F::E0, E1X = F::E1X, E1 = F::E1;
+    constexpr int L0 = F::L0, L1 = F::L1;
+    static_assert(F::fits, "fused tile exceeds LDS or register budget");
+
+    __shared__ S w0[NS][W][E0];   // u0, stencil components, tile + 2R
+    __shared__ S w1[NS][W][E1];   // u1, stencil components, tile + R
+    __shared__ S p0[NPA][E1];     // u0, point components, tile + R
+
+    signed char slot[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n) slot[n] = (signed char)CS::slot(n);
+
+    const int tid = threadIdx.x;
+    const int tx = tid & 63, w = tid >> 6;
+    const int g = a.ghost;
+    const double t1 = a.prm1[kPrmTime], t2 = a.prm2[kPrmTime];
+    const double dt = a.prm1[kPrmDt], dx = a.prm1[kPrmDx];
+    const double *par1 = a.prm1 + kPrmUser, *par2 = a.prm2 + kPrmUser;
+    const double *gpar1 = a.prm1 + kPrmGw, *gpar2 = a.prm2 + kPrmGw;
+    const S cdt = (S)a.dt;
+
+    // Stage-one points of this thread: its own cell (x = tx, y = w), then a
+    // ring cell of the tile + R region.
+    int e1x[L1], e1y[L1];
+    bool e1ok[L1];
+#pragma unroll
+    for (int m = 0; m < L1; ++m) {
+        e1x[m] = 0;
+        e1y[m] = 0;
+        e1ok[m] = m == 0 || tid + NT * m < E1;
+        if (m == 0) {
+            e1x[m] = tx;
+            e1y[m] = w;
+        } else if (e1ok[m]) {
+            ring_xy<R, NW>(tid + NT * (m - 1), e1x[m], e1y[m]);
+        }
+    }
+
+    int t, tstep, tend;
+    if (a.xcd_split) {
+        const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+        const int per = (a.ntiles + 7) >> 3;
+        t = xcd * per + l;
+        tend = min(a.ntiles, (xcd + 1) * per);
+        tstep = gridDim.x >> 3;
+    } else {
+        t = blockIdx.x;
+        tend = a.ntiles;
+        tstep = gridDim.x;
+    }
+
+    for (; t < tend; t += tstep) {
+        const TileDesc td = a.tiles[t];
+        const BoxDesc bd = a.boxes[td.box];
+        const int nx = bd.n[0], ny = bd.n[1], nz = bd.n[2];
+        const unsigned js = bd.jstride, ks = bd.kstride;
+        const long long org = bd.org;
+        auto wrapi = [](int v, int n) { return ((v % n) + n) % n; };
+        auto zoff = [&](int z) { return (unsigned)(wrapi(z, nz) + g) * ks; };
+        auto col = [&](int x, int y) {
+            return (unsigned)(wrapi(td.x0 + x, nx) + g) +
+                   (unsigned)(wrapi(td.y0 + y, ny) + g) * js;
+        };
+        // Window points (tile + 2R) of this thread.
+        unsigned c0[L0];
+        int l0[L0];
+        bool ok0[L0];
+#pragma unroll
+        for (int m = 0; m < L0; ++m) {
+            const int e = tid + NT * m;
+            ok0[m] = e < E0;
+            const int ee = ok0[m] ? e : 0;
+            l0[m] = ee;
+            c0[m] = col(ee % E0X - 2 * R, ee / E0X - 2 * R);
+        }
+        unsigned c1[L1];
+        int l1[L1];
+#pragma unroll
+        for (int m = 0; m < L1; ++m) {
+            c1[m] = col(e1x[m], e1y[m]);
+            l1[m] = (e1y[m] + R) * E1X + (e1x[m] + R);
+        }
+        const bool own = td.x0 + tx < nx && td.y0 + w < ny;
+        const unsigned cown = (unsigned)(td.x0 + tx + g) + (unsigned)(td.y0 + w + g) * js;
+
+        const int zbase = td.z0 - 2 * R;  // lowest plane loaded
+        auto slotz = [&](int z) { return (z - zbase) % W; };
+
+        // Next plane's loads (registers).
+        S v0[NS][L0];
+        S vp[NPA][L1];
+        auto load_window = [&](int z) {
+            const unsigned zo = zoff(z);
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int m = 0; m < L0; ++m)
+                    v0[s][m] = ld<S>(a.u0, org + CS::comp_of_slot(s) * a.cstride, c0[m] + zo);
+        };
+        auto load_point = [&](int z) {
+            const unsigned zo = zoff(z);
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int m = 0; m < L1; ++m)
+                    vp[p][m] = ld<S>(a.u0, org + CS::comp_of_point(p) * a.cstride, c1[m] + zo);
+        };
+        auto store_window = [&](int z) {
+            const int sl = slotz(z);
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int m = 0; m < L0; ++m)
+                    if (ok0[m]) w0[s][sl][l0[m]] = v0[s][m];
+        };
+
+        // Prologue: planes z0-2R .. z0-1 of the window; prefetch plane z0
+        // of the window and plane z0-R of the point components.
+        for (int z = td.z0 - 2 * R; z < td.z0; ++z) {
+            load_window(z);
+            store_window(z);
+        }
+        load_window(td.z0);
+        load_point(td.z0 - R);
+
+        S dk1[R + 1][NC];    // k1 of the own cell, planes zz-R .. zz
+        S du1[R + 1][NPA];   // u1 of the point components, same planes
+
+        for (int zz = td.z0 - R; zz < td.z1 + R; ++zz) {
+            // Stage one runs on plane zz: window planes zz-R .. zz+R.
+            store_window(zz + R);
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int m = 0; m < L1; ++m)
+                    if (e1ok[m]) p0[p][l1[m]] = vp[p][m];
+            if (zz + 1 < td.z1 + R) {
+                load_window(zz + 1 + R);
+                load_point(zz + 1);
+            }
+            __syncthreads();
+
+            // ---- stage one at plane zz over the tile + R ----
+            int zpl0[W];
+#pragma unroll
+            for (int d = 0; d < W; ++d) zpl0[d] = (slotz(zz + d - R) - slotz(zz)) * E0;
+            const int sz = slotz(zz);
+#pragma unroll
+            for (int s = 0; s < R; ++s)
+#pragma unroll
+                for (int n = 0; n < NC; ++n) dk1[s][n] = dk1[s + 1][n];
+#pragma unroll
+            for (int s = 0; s < R; ++s)
+#pragma unroll
+                for (int p = 0; p < NP; ++p) du1[s][p] = du1[s + 1][p];
+#pragma unroll
+            for (int m = 0; m < L1; ++m) {
+                if (!e1ok[m]) continue;
+                const int lc = (e1y[m] + 2 * R) * E0X + (e1x[m] + 2 * R);
+                const int gi = bd.lo[0] + td.x0 + e1x[m], gj = bd.lo[1] + td.y0 + e1y[m],
+                          gk = bd.lo[2] + zz;
+                S u0v[NC];
+                double ctr[NC];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    u0v[CS::comp_of_slot(s)] = w0[s][sz][lc];
+                    ctr[CS::comp_of_slot(s)] = (double)w0[s][sz][lc];
+                }
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    u0v[CS::comp_of_point(p)] = p0[p][l1[m]];
+                    ctr[CS::comp_of_point(p)] = (double)p0[p][l1[m]];
+                }
+                TileCtx tc;
+                tc.ci = gi;
+                tc.cj = gj;
+                tc.ck = gk;
+                tc.R = R;
+                tc.f32 = std::is_same_v<S, float> ? 1 : 0;
+                tc.lds_pitch = E0X;
+                tc.lds_cstride = W * E0;
+                tc.zq_cstride = 0;
+                tc.lds = &w0[0][sz][lc];
+                tc.zq = nullptr;
+                tc.zplane = zpl0;
+                tc.ctr = ctr;
+                tc.slot = slot;
+                amrex::Array4<const double> sv;
+                sv.mode = amrex::Array4<const double>::kTile;
+                sv.ext = &tc;
+                sv.ncomp = NC;
+                double rhs[NC];
+#pragma unroll
+                for (int n = 0; n < NC; ++n) rhs[n] = 0.;
+                amrex::Array4<double> rv;
+                rv.mode = amrex::Array4<double>::kSink;
+                rv.sink = rhs;
+                rv.ncomp = NC;
+                Ops::rhs(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, par1);
+                if constexpr (GW) {
+                    Ops::gw_rhs(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, gpar1);
+                    Ops::gw_backreaction(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, par1, gpar1);
+                }
+                // k1 in the storage type, as the two-launch path stores it;
+                // u1 = u0 + dt k1 in the storage type.
+                S k1[NC];
+#pragma unroll
+                for (int n = 0; n < NC; ++n) k1[n] = (S)rhs[n];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const int n = CS::comp_of_slot(s);
+                    w1[s][sz][l1[m]] = fma_st<S>(cdt, k1[n], u0v[n]);
+                }
+                if (m == 0) {
+#pragma unroll
+                    for (int n = 0; n < NC; ++n) dk1[R][n] = k1[n];
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        const int n = CS::comp_of_point(p);
+                        du1[R][p] = fma_st<S>(cdt, k1[n], u0v[n]);
+                    }
+                }
+            }
+            __syncthreads();
+
+            // ---- stage two at plane zc = zz - R on the tile ----
+            const int zc = zz - R;
+            if (zc < td.z0) continue;
+            int zpl1[W];
+#pragma unroll
+            for (int d = 0; d < W; ++d) zpl1[d] = (slotz(zc + d - R) - slotz(zc)) * E1;
+            const int sc = slotz(zc);
+            const int lc = l1[0];
+            const int gi = bd.lo[0] + td.x0 + tx, gj = bd.lo[1] + td.y0 + w,
+                      gk = bd.lo[2] + zc;
+            double ctr[NC];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) ctr[CS::comp_of_slot(s)] = (double)w1[s][sc][lc];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) ctr[CS::comp_of_point(p)] = (double)du1[0][p];
+            TileCtx tc;
+            tc.ci = gi;
+            tc.cj = gj;
+            tc.ck = gk;
+            tc.R = R;
+            tc.f32 = std::is_same_v<S, float> ? 1 : 0;
+            tc.lds_pitch = E1X;
+            tc.lds_cstride = W * E1;
+            tc.zq_cstride = 0;
+            tc.lds = &w1[0][sc][lc];
+            tc.zq = nullptr;
+            tc.zplane = zpl1;
+            tc.ctr = ctr;
+            tc.slot = slot;
+            amrex::Array4<const double> sv;
+            sv.mode = amrex::Array4<const double>::kTile;
+            sv.ext = &tc;
+            sv.ncomp = NC;
+            double rhs[NC];
+#pragma unroll
+            for (int n = 0; n < NC; ++n) rhs[n] = 0.;
+            amrex::Array4<double> rv;
+            rv.mode = amrex::Array4<double>::kSink;
+            rv.sink = rhs;
+            rv.ncomp = NC;
+            Ops::rhs(rv, sv, gi, gj, gk, a.lev, t2, dt, dx, par2);
+            if constexpr (GW) {
+                Ops::gw_rhs(rv, sv, gi, gj, gk, a.lev, t2, dt, dx, gpar2);
+                Ops::gw_backreaction(rv, sv, gi, gj, gk, a.lev, t2, dt, dx, par2, gpar2);
+            }
+            if (own) {
+                const unsigned o = cown + zoff(zc);
+#pragma unroll
+                for (int n = 0; n < NC; ++n) {
+                    const double k = rhs[n] + (double)dk1[0][n];
+                    st<S>(a.kout, org + n * a.cstride, o, (S)k);
+                }
+            }
+        }
+        __syncthreads();  // the next tile's prologue rewrites the windows
+    }
+}
+
+}  // namespace shm

@@ -13,6 +13,7 @@
 #pragma once
 
 #include "shm_aux.hpp"
+#include "shm_fused.hpp"
 #include "shm_stage.hpp"
 #include "shm_table.hpp"
 
@@ -158,10 +159,47 @@ void fill_kinds(KernelTable &kt) {
     }
 }
 
+template <class Ops, Prec P, int R, bool GW, int V>
+hipError_t launch_fused(const FusedArgs &a, int grid, hipStream_t s) {
+    constexpr int NW = kVariants[V].nw;
+    hipLaunchKernelGGL((fused12_kernel<Ops, P, R, GW, false, NW>), dim3(grid),
+                       dim3(64 * NW), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class Ops, Prec P, int R, bool GW, int V>
+int occupancy_fused() {
+    constexpr int NW = kVariants[V].nw;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, fused12_kernel<Ops, P, R, GW, false, NW>, 64 * NW, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+template <class Ops, Prec P, int R, bool GW>
+void fill_fused(KernelTable &kt) {
+    auto put = [&](auto vc) {
+        constexpr int V = decltype(vc)::value;
+        if constexpr (kVariants[V].ry == 1 &&
+                      FusedShape<Ops, P, R, GW, false, kVariants[V].nw>::fits) {
+            FusedEntry &e = kt.fused12[(int)P][R][GW ? 1 : 0][V];
+            e.fn = &launch_fused<Ops, P, R, GW, V>;
+            e.occupancy = &occupancy_fused<Ops, P, R, GW, V>;
+            e.threads = 64 * kVariants[V].nw;
+            e.ty = kVariants[V].nw;
+        }
+    };
+    put(std::integral_constant<int, 0>{});
+    put(std::integral_constant<int, 1>{});
+    put(std::integral_constant<int, 2>{});
+}
+
 template <class Ops, Prec P, int R>
 void fill_radius(KernelTable &kt) {
     if constexpr (SHM_GW_MODES & 1) {
         fill_kinds<Ops, P, R, false, false>(kt);
+        fill_fused<Ops, P, R, false>(kt);
         if constexpr (SHM_WITH_KO) fill_kinds<Ops, P, R, false, true>(kt);
     }
     if constexpr (SHM_GW_MODES & 2) {

@@ -22,6 +22,7 @@
 #include <sstream>
 
 #include "shm_aux.hpp"
+#include "shm_fused.hpp"
 
 // ---------------------------------------------------------------------------
 // ParmParse table.
@@ -582,6 +583,7 @@ void Runtime::init_from_inputs(const std::string &inputs,
         throw Error(1, "shm.independent_boxes needs a single level");
     pp.query("shm.tile_order_zc_minor", launch.zc_minor);
     pp.query("shm.graphs", graphs);
+    pp.query("shm.fused", launch.fused);
     if (launch.variant >= kNVar) throw Error(1, "shm.tile_variant out of range");
 
     // Project initialisation, then the reach of its cell functions.
@@ -731,7 +733,67 @@ LaunchCfg Runtime::resolved_launch(int lev) const {
     c.xcd_split = pick(launch.xcd_split, t.xcd_split, 1);
     c.persistent = pick(launch.persistent, t.persistent, 1);
     c.zc_minor = pick(launch.zc_minor, t.zc_minor, 0);
+    c.fused = pick(launch.fused, t.fused, 0);
+    int dv = -1;
+    for (int v = kNVar - 1; v >= 0 && dv < 0; --v)
+        if (kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0][v].fn) dv = v;
+    c.fused_variant = pick(launch.fused_variant, t.fused_variant, dv);
     return c;
+}
+
+bool Runtime::fused_possible(const LevelState &ls) const {
+    if (integrator_ != 10 || ko_) return false;
+    bool any = false;
+    for (int v = 0; v < kNVar; ++v)
+        any = any || kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0][v].fn != nullptr;
+    if (!any) return false;
+    for (const BoxDesc &b : ls.L.hbox)
+        if (b.wrap != 7) return false;
+    return !ls.L.hbox.empty();
+}
+
+void Runtime::launch_fused(LevelState &ls, const double *prm1, const double *prm2,
+                           double dt, int lev) {
+    const LaunchCfg lc = resolved_launch(lev);
+    int v = lc.fused_variant;
+    const auto &row = kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0];
+    if (v < 0 || v >= kNVar || !row[v].fn) {
+        v = -1;
+        for (int d = kNVar - 1; d >= 0 && v < 0; --d)
+            if (row[d].fn) v = d;
+    }
+    if (v < 0) throw Error(4, "fused stage kernel not built for this configuration");
+    const FusedEntry &e = row[v];
+    int maxn = 1;
+    for (const Box3 &b : ls.L.boxes) maxn = std::max(maxn, b.n(2));
+    // Whole z columns: every chunk boundary costs 2R extra stage-one planes.
+    const int z = lc.tz > 0 ? lc.tz : maxn;
+    int ntiles = 0;
+    FusedArgs a{};
+    a.tiles = ls.L.get_tiles(e.ty, z, 0, &ntiles);
+    a.boxes = ls.L.dbox.as<BoxDesc>();
+    a.ntiles = ntiles;
+    a.lev = lev;
+    a.prm1 = prm1;
+    a.prm2 = prm2;
+    a.ghost = ls.L.g;
+    a.cstride = ls.L.cstride;
+    a.u0 = ls.U[ls.cur ^ 1].get();
+    a.kout = ls.K[1].get();
+    a.dt = dt;
+    if (ntiles == 0) return;
+    const int occ = e.occupancy();
+    int grid = lc.persistent ? (int)std::min<long long>(ntiles, (long long)occ * ncu_)
+                             : ntiles;
+    a.xcd_split = lc.xcd_split ? 1 : 0;
+    if (a.xcd_split) grid = (int)round_up(grid, 8);
+    EventPair *ep = timing_on ? timed_pair(0) : nullptr;
+    if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
+    SHM_HIP(e.fn(a, grid, stream_));
+    if (ep) {
+        SHM_HIP(hipEventRecord(ep->b, stream_));
+        timing.stage_cells += 2 * ls.L.nvalid();  // two stages
+    }
 }
 
 void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
@@ -875,22 +937,29 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
     if (integrator_ == 10) {
         // Low-storage SSPRK3 with the stage states rebuilt from u0 and k:
         // k1 -> K[0], k2 -> K[1] (k is read at neighbours, so not in place).
-        StageArgs a{};
-        a.prm = prm(st[0]);
-        a.x = ls.U[old].get();
-        a.kout = ls.K[0].get();
-        launch_stage(kSsprk1, ls, a, lev);
-        halo(ls, ls.K[0].get(), stencil_comps());
+        if (resolved_launch(lev).fused && fused_possible(ls)) {
+            // Stages one and two in one pass: k1 stays on chip.
+            const double *p1 = prm(st[0]);
+            const double *p2 = prm(st[1]);
+            launch_fused(ls, p1, p2, dt, lev);
+        } else {
+            StageArgs a{};
+            a.prm = prm(st[0]);
+            a.x = ls.U[old].get();
+            a.kout = ls.K[0].get();
+            launch_stage(kSsprk1, ls, a, lev);
+            halo(ls, ls.K[0].get(), stencil_comps());
 
-        StageArgs b{};
-        b.prm = prm(st[1]);
-        b.u0 = ls.U[old].get();
-        b.kin = ls.K[0].get();
-        b.kout = ls.K[1].get();
-        b.c[0] = 1.;
-        b.c[2] = dt;
-        launch_stage(kSsprk2, ls, b, lev);
-        halo(ls, ls.K[1].get(), stencil_comps());
+            StageArgs b{};
+            b.prm = prm(st[1]);
+            b.u0 = ls.U[old].get();
+            b.kin = ls.K[0].get();
+            b.kout = ls.K[1].get();
+            b.c[0] = 1.;
+            b.c[2] = dt;
+            launch_stage(kSsprk2, ls, b, lev);
+            halo(ls, ls.K[1].get(), stencil_comps());
+        }
 
         StageArgs c{};
         c.prm = prm(st[2]);
@@ -1660,6 +1729,34 @@ void Runtime::autotune(int lev) {
                             bc = launch;
                         }
                     }
+    // SSPRK3 on a level of periodic boxes: one fused launch for stages one
+    // and two against the best pair of launches found above.
+    bc.fused = 0;
+    bc.fused_variant = -1;
+    if (integrator_ == 10 && fused_possible(ls) && saved.fused != 0) {
+        const auto &row = kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0];
+        for (int v = 0; v < kNVar; ++v) {
+            if (!row[v].fn) continue;
+            if (saved.fused_variant >= 0 && v != saved.fused_variant) continue;
+            launch = bc;
+            launch.fused = 1;
+            launch.fused_variant = v;
+            launch.tz = 0;
+            launch_fused(ls, prm, prm, dt, lev);
+            SHM_HIP(hipEventRecord(ev0_, stream_));
+            const int reps = 3;
+            for (int k = 0; k < reps; ++k) launch_fused(ls, prm, prm, dt, lev);
+            SHM_HIP(hipEventRecord(ev1_, stream_));
+            SHM_HIP(hipEventSynchronize(ev1_));
+            float ms = 0;
+            SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            if (ms < best) {
+                best = ms;
+                bc.fused = 1;
+                bc.fused_variant = v;
+            }
+        }
+    }
     launch = saved;
     timing_on = saved_t;
     tuned_[lev + 1] = bc;

@@ -111,6 +111,8 @@ struct LaunchCfg {
     int xcd_split = -1;   // each XCD walks one contiguous range of tiles
     int persistent = -1;  // grid = resident capacity (1) or one tile per workgroup (0)
     int zc_minor = -1;    // tile order: z-chunk outermost (0) or innermost (1)
+    int fused = -1;       // SSPRK3 stages one and two in one launch (0/1)
+    int fused_variant = -1;  // tile shape of that launch
 };
 
 /** A captured level step (swap excluded): the stage launches and ghost
@@ -188,6 +190,9 @@ class Runtime {
     void bump_epoch() { ++epoch_; }
     /** Launch shape used for level lev (overrides, tuned values, defaults). */
     LaunchCfg resolved_launch(int lev) const;
+    /** Fused SSPRK3 stages one and two possible at this level (every box
+     *  periodic on itself, kernel built for the precision and radius). */
+    bool fused_possible(const LevelState &ls) const;
 
   private:
     struct EventPair {
@@ -204,6 +209,10 @@ class Runtime {
     size_t losize() const { return prec_ == Prec::F32C16 ? 2 : 4; }
     bool comp() const { return prec_ != Prec::F64; }
     void launch_stage(int pol, LevelState &ls, StageArgs &a, int lev);
+    /** Fused SSPRK3 stages one and two possible at this level (every box
+     *  periodic on itself, kernel built for the precision and radius). */
+    void launch_fused(LevelState &ls, const double *prm1, const double *prm2,
+                      double dt, int lev);
     void halo(LevelState &ls, void *arr, const std::vector<int> &comps,
               bool full = false);
     void fill_patch(LevelState &ls, int which);

@@ -188,6 +188,15 @@ __device__ AMREX_FORCE_INLINE void st(void *p, long long base, unsigned off, T v
     __builtin_nontemporal_store(v, reinterpret_cast<T *>(b + (unsigned)(off * (unsigned)sizeof(T))));
 }
 
+/** a * b + c with one rounding in the storage type (stage states). */
+template <class S>
+__device__ AMREX_FORCE_INLINE S fma_st(S a, S b, S c) {
+    if constexpr (std::is_same_v<S, float>)
+        return __builtin_fmaf(a, b, c);
+    else
+        return __builtin_fma(a, b, c);
+}
+
 /** Split a full-precision value into the stored pair (or keep it FP64):
  *  hi = fl32(u), lo = u - hi rounded to the remainder type. */
 template <class T>
@@ -279,7 +288,7 @@ struct PolSsprk {
         using S = typename T::S;
         const long long b = org + n * a.cstride;
         if constexpr (composed)  // stage state in the storage precision
-            return ld<S>(a.u0, b, off) + (S)a.c[2] * ld<S>(a.kin, b, off);
+            return fma_st<S>((S)a.c[2], ld<S>(a.kin, b, off), ld<S>(a.u0, b, off));
         else
             return ld<S>(a.x, b, off);
     }
@@ -292,7 +301,7 @@ struct PolSsprk {
             const S k = ld<S>(a.kin, b, off), u = ld<S>(a.u0, b, off);
             keep[0] = k;
             if constexpr (K == kSsStage3) keep[1] = u;
-            return u + (S)a.c[2] * k;
+            return fma_st<S>((S)a.c[2], k, u);
         } else {
             return load_st<CS, T>(a, org, n, off);
         }

@@ -9,6 +9,7 @@
 namespace shm {
 
 struct StageArgs;
+struct FusedArgs;
 struct TeTagArgs;
 struct ProbeOut;
 
@@ -44,6 +45,7 @@ constexpr TileVariant kVariants[kNVar] = {{4, 1}, {8, 1}, {16, 1}};
 constexpr int kDefaultVariant = 1;
 
 using StageFn = hipError_t (*)(const StageArgs &, int grid, hipStream_t);
+using FusedFn = hipError_t (*)(const FusedArgs &, int grid, hipStream_t);
 using OccFn = int (*)();
 using TeTagFn = hipError_t (*)(const TeTagArgs &, int grid, hipStream_t);
 using ProbeFn = hipError_t (*)(ProbeOut *, const double *, const double *,
@@ -56,12 +58,21 @@ struct StageEntry {
     int ty = 0;
 };
 
+/** Fused SSPRK3 stages one and two (tile 64 x nw). */
+struct FusedEntry {
+    FusedFn fn = nullptr;
+    int (*occupancy)() = nullptr;
+    int threads = 0;
+    int ty = 0;
+};
+
 struct KernelTable {
     int nscalars = 0, nfields = 0, ngw = 0;
     bool built_gw = false, built_ko = false;
     int r_mask = 0;
     // [kind][precision][R][gw][ko][variant]
     StageEntry stage[kNKind][3][4][2][2][kNVar];
+    FusedEntry fused12[3][4][2][kNVar];  // [precision][R][gw][variant]
     TeTagFn te_tag[3][2] = {};  // [precision][gw]
     ProbeFn probe[2] = {};      // [gw]
 };
