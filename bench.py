@@ -13,9 +13,9 @@ state + bf16 remainder; valid cells x stages / wall time), inputs resident in
 HBM.  The same seeded run on the FP64 path gives `fp64` and the field error;
 `fp32c` is the FP32 + FP32-remainder variant.
 
-roofline: the stage kernels' algorithmic bytes per launch over their mean
-launch time, measured with HIP events on the library's stream inside the
-timed region (cells x stages per step / launches per step per launch).
+roofline: the dominant stage kernel (longest mean launch; the other one in
+`other_kernel`), its algorithmic bytes per launch over its mean launch time,
+measured with HIP events on the library's stream inside the timed region.
 Algorithmic bytes per cell-stage are those of the minimal low-storage
 schedule that processes one stage per pass (every stored array read once and
 written at most once per stage: 12 words per component and step, plus one
@@ -69,11 +69,76 @@ MOVED_BYTES_FUSED = {"fp32c16": (5 * 4 + 2 * 2) * C / 3.0,
 
 def moved_bytes(prec_key, fused):
     return (MOVED_BYTES_FUSED if fused else MOVED_BYTES_PER_CELL_STAGE)[prec_key]
-# HBM bytes per stage launch (mean over the three SSPRK3 stages) at 512^3,
-# FP32 + bf16 remainder, 64 x 16 tiles: DRAM read + write requests counted by
-# rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B TCC_EA0_WRREQ_WRITE_DRAM_32B (x 32 B),
-# profiles/stage_kernel_traffic_complex512.md.
-TRAFFIC_BYTES_PER_LAUNCH_512 = 7.097e9
+
+
+# Bytes per valid cell each kernel of the fused schedule moves: the fused
+# stage-one-and-two launch reads u0 (leading words only) and writes k2; the
+# stage-three launch reads u0 with its remainder and k2 and writes u with its
+# remainder.
+WORD = {"fp32c16": (4, 2), "fp32c": (4, 4), "fp64": (8, 0)}
+
+
+def moved_per_cell(prec_key, kernel):
+    w, r = WORD[prec_key]
+    return C * (2 * w if kernel == "fused12" else 4 * w + 2 * r)
+
+
+# HBM bytes per launch at 512^3, FP32 + bf16 remainder, fused schedule: DRAM
+# read + write requests counted by rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B
+# TCC_EA0_WRREQ_WRITE_DRAM_32B (x 32 B), profiles/stage_kernel_traffic_complex512.md.
+TRAFFIC_BYTES_512 = {"fused12": None, "stage3": None}
+
+
+def kernel_record(name, avg_ms, alg_bytes, moved, traffic):
+    rec = {"kernel": name, "avg_launch_ms": avg_ms,
+           "algorithmic_bytes_per_launch": alg_bytes,
+           "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9,
+           "moved_bytes_per_launch": moved,
+           "moved_achieved": moved / (avg_ms * 1e-3) / 1e9, "traffic": traffic}
+    rec["frac"] = rec["achieved"] / HBM_PEAK_GBS
+    rec["moved_frac"] = rec["moved_achieved"] / HBM_PEAK_GBS
+    if traffic:
+        rec["traffic_achieved"] = traffic / (avg_ms * 1e-3) / 1e9
+    return rec
+
+
+def kernel_roofline(tm, prec_key, fused, N):
+    """Roofline of the dominant stage kernel (longest mean launch) from the
+    HIP-event timings.  `achieved` prices a cell-stage at the algorithmic
+    bytes; `moved_*` at what this schedule must read and write."""
+    launches = max(tm["stage_launches"], 1)
+    contract = BYTES_PER_CELL_STAGE[prec_key]
+    if fused and tm["fused_launches"]:
+        fl = tm["fused_launches"]
+        # Each fused launch updates ncell cells twice, each stage-3 launch once.
+        ncell = tm["stage_cells"] / (2 * fl + (launches - fl))
+        t = TRAFFIC_BYTES_512 if N == 512 and prec_key == "fp32c16" else {}
+        recs = [kernel_record("shm::fused12_kernel (SSPRK3 stages 1+2)",
+                              tm["fused_ms"] / fl, 2 * contract * ncell,
+                              moved_per_cell(prec_key, "fused12") * ncell,
+                              t.get("fused12")),
+                kernel_record("shm::stage_kernel (SSPRK3 stage 3)",
+                              (tm["stage_ms"] - tm["fused_ms"]) / max(launches - fl, 1),
+                              contract * ncell,
+                              moved_per_cell(prec_key, "stage3") * ncell,
+                              t.get("stage3"))]
+        recs.sort(key=lambda r: -r["avg_launch_ms"])
+        main, other = recs
+        res = {"bound": "hbm", "achieved": main["achieved"], "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": main["frac"], "traffic": main["traffic"]}
+        res.update(main)
+        res["other_kernel"] = other
+        res["note"] = ("per step: one fused stage-1+2 launch (two cell-stages per "
+                       "cell, k1 and u1 kept on chip) and one stage-3 launch")
+        return res
+    cells_pl = tm["stage_cells"] / launches
+    main = kernel_record("shm::stage_kernel (SSPRK3 stages 1..3, mean)",
+                         tm["stage_ms"] / launches, contract * cells_pl,
+                         moved_bytes(prec_key, False) * cells_pl, None)
+    res = {"bound": "hbm", "achieved": main["achieved"], "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": main["frac"], "traffic": None}
+    res.update(main)
+    return res
 
 
 def inputs(N, L, cfl, t0):
@@ -222,11 +287,12 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--launch", default="",
                     help="fixed stage launch shape 'variant,tile_z,xcd_split,"
-                         "persistent,zc_minor' instead of the autotuner")
+                         "persistent,zc_minor[,fused,fused_variant]' instead "
+                         "of the autotuner")
     a = ap.parse_args()
     launch = [int(v) for v in a.launch.split(",")] if a.launch else None
-    if launch is not None and len(launch) != 5:
-        ap.error("--launch takes five integers")
+    if launch is not None and len(launch) not in (5, 7):
+        ap.error("--launch takes five or seven integers")
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs each level on a "
               "single MI355X (multi-GPU sharding is design-only); use --gpus 1",
@@ -263,26 +329,13 @@ def main():
                    "L": N * 0.25, "cfl": 0.2, "seed": a.seed},
     }
 
-    # Roofline of the dominant (fused stage) kernel.
-    launches = tm32["stage_launches"]
-    avg_ms = tm32["stage_ms"] / max(launches, 1)
-    cells_pl = tm32["stage_cells"] / max(launches, 1)
-    bpl = BYTES_PER_CELL_STAGE["fp32c16"] * cells_pl
-    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    # Roofline of the dominant stage kernel.
     fused32 = bool(sim32.get_launch(0)["fused"])
-    mpl = moved_bytes("fp32c16", fused32) * cells_pl
-    moved = mpl / (avg_ms * 1e-3) / 1e9
-    traffic = TRAFFIC_BYTES_PER_LAUNCH_512 if N == 512 else None
-    result["roofline"] = {"bound": "hbm", "achieved": achieved,
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                          "kernel": "shm::stage_kernel<ComplexScalar, PolSsprk<1..3>, F32C16>",
-                          "avg_launch_ms": avg_ms,
-                          "algorithmic_bytes_per_launch": bpl,
-                          "moved_bytes_per_launch": mpl,
-                          "stages_one_two_fused": fused32,
-                          "moved_achieved": moved,
-                          "moved_frac": moved / HBM_PEAK_GBS}
+    result["roofline"] = kernel_roofline(tm32, "fp32c16", fused32, N)
+    result["roofline"]["stages_one_two_fused"] = fused32
+    # Whole-step rate priced at the algorithmic bytes.
+    result["step_frac"] = (BYTES_PER_CELL_STAGE["fp32c16"] * rate32 / 1e9
+                           / HBM_PEAK_GBS)
     result["launch"] = sim32.get_launch(0)
     result["halo_ms_per_step"] = tm32["halo_ms"] / a.steps
     # Achievable bandwidth on this GPU (plain float4 kernels over 4 GiB).
@@ -292,9 +345,6 @@ def main():
         sim64, sec64, tm64 = run_gpu(shm, inp, shm.PREC_FP64, a.steps, a.warmup,
                                      a.seed, launch)
         rate64 = cells * stages * a.steps / sec64
-        l64 = tm64["stage_launches"]
-        avg64 = tm64["stage_ms"] / max(l64, 1)
-        b64 = BYTES_PER_CELL_STAGE["fp64"] * tm64["stage_cells"] / max(l64, 1)
         err = 0.0
         for c in range(4):
             x32 = sim32.download(0, c)
@@ -303,17 +353,18 @@ def main():
                                  max(np.max(np.abs(x64)), 1e-300)))
             del x32, x64
         fused64 = bool(sim64.get_launch(0)["fused"])
-        m64 = moved_bytes("fp64", fused64) * tm64["stage_cells"] / max(l64, 1)
+        r64 = kernel_roofline(tm64, "fp64", fused64, 0)
         result["fp64"] = {"value": rate64, "ms_per_step": sec64 / a.steps * 1e3,
-                          "roofline_frac": b64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                          "moved_frac": m64 / (avg64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "step_frac": BYTES_PER_CELL_STAGE["fp64"] * rate64
+                          / 1e9 / HBM_PEAK_GBS,
+                          "kernel_frac": r64["frac"],
+                          "kernel_moved_frac": r64["moved_frac"],
                           "launch": sim64.get_launch(0)}
         result["speedup_fp32c_vs_fp64"] = rate32 / rate64
         result["fp32c_rel_err_vs_fp64"] = err
         simc, secc, tmc = run_gpu(shm, inp, shm.PREC_FP32C, a.steps, a.warmup,
                                   a.seed, launch)
-        lc = tmc["stage_launches"]
-        bc = BYTES_PER_CELL_STAGE["fp32c"] * tmc["stage_cells"] / max(lc, 1)
+        ratec = cells * stages * a.steps / secc
         errc = 0.0
         for c in range(4):
             xc = simc.download(0, c)
@@ -321,8 +372,8 @@ def main():
             errc = max(errc, float(np.max(np.abs(xc - x64)) /
                                    max(np.max(np.abs(x64)), 1e-300)))
             del xc, x64
-        result["fp32c"] = {"value": cells * stages * a.steps / secc,
-                           "roofline_frac": bc / (tmc["stage_ms"] / max(lc, 1) * 1e-3)
+        result["fp32c"] = {"value": ratec,
+                           "step_frac": BYTES_PER_CELL_STAGE["fp32c"] * ratec
                            / 1e9 / HBM_PEAK_GBS,
                            "rel_err_vs_fp64": errc,
                            "remainder": "fp32"}

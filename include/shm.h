@@ -171,11 +171,14 @@ int shm_get_launch(shm_ctx *ctx, int lev, shm_launch_t *cfg);
  * eagerly while kernel timing is on. */
 int shm_set_graphs(shm_ctx *ctx, int on);
 
-/* Kernel timing with HIP events on the library's stream. */
+/* Kernel timing with HIP events on the library's stream.  stage_* count every
+ * stage launch (a fused stage-one-and-two launch updates its cells twice);
+ * fused_* count the fused launches alone.  Any pointer may be NULL. */
 int shm_set_timing(shm_ctx *ctx, int on);
 int shm_get_timing(shm_ctx *ctx, double *stage_ms, long long *stage_launches,
                    long long *stage_cells, double *halo_ms,
-                   long long *halo_launches);
+                   long long *halo_launches, double *fused_ms,
+                   long long *fused_launches);
 
 #ifdef __cplusplus
 }

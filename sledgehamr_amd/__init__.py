@@ -106,7 +106,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_get_launch": (i, [vp, i, C.POINTER(_Launch)]),
         "shm_set_graphs": (i, [vp, i]),
         "shm_set_timing": (i, [vp, i]),
-        "shm_get_timing": (i, [vp, dp, llp, llp, dp, llp]),
+        "shm_get_timing": (i, [vp, dp, llp, llp, dp, llp, dp, llp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -312,10 +312,13 @@ class Simulation:
         self._check(self.lib.shm_set_timing(self._ctx, int(on)))
 
     def timing(self) -> dict:
-        sm, hm = C.c_double(), C.c_double()
-        sl, sc, hl = C.c_longlong(), C.c_longlong(), C.c_longlong()
-        self._check(self.lib.shm_get_timing(self._ctx, C.byref(sm), C.byref(sl),
-                                            C.byref(sc), C.byref(hm), C.byref(hl)))
+        sm, hm, fm = C.c_double(), C.c_double(), C.c_double()
+        sl, sc, hl, fl = (C.c_longlong(), C.c_longlong(), C.c_longlong(),
+                          C.c_longlong())
+        self._check(self.lib.shm_get_timing(
+            self._ctx, C.byref(sm), C.byref(sl), C.byref(sc), C.byref(hm),
+            C.byref(hl), C.byref(fm), C.byref(fl)))
         return {"stage_ms": sm.value, "stage_launches": sl.value,
                 "stage_cells": sc.value, "halo_ms": hm.value,
-                "halo_launches": hl.value}
+                "halo_launches": hl.value, "fused_ms": fm.value,
+                "fused_launches": fl.value}

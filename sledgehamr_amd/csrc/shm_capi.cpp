@@ -381,7 +381,8 @@ __attribute__((visibility("default"))) int shm_set_timing(shm_ctx *ctx, int on) 
 
 __attribute__((visibility("default"))) int shm_get_timing(
     shm_ctx *ctx, double *stage_ms, long long *stage_launches,
-    long long *stage_cells, double *halo_ms, long long *halo_launches) {
+    long long *stage_cells, double *halo_ms, long long *halo_launches,
+    double *fused_ms, long long *fused_launches) {
     return guard(ctx, [&] {
         SHM_NEED_RT;
         ctx->rt->collect_timing();
@@ -391,6 +392,8 @@ __attribute__((visibility("default"))) int shm_get_timing(
         if (stage_cells) *stage_cells = t.stage_cells;
         if (halo_ms) *halo_ms = t.halo_ms;
         if (halo_launches) *halo_launches = t.halo_launches;
+        if (fused_ms) *fused_ms = t.fused_ms;
+        if (fused_launches) *fused_launches = t.fused_launches;
     });
 }
 

@@ -151,30 +151,41 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         const bool own = td.x0 + tx < nx && td.y0 + w < ny;
         const unsigned cown = (unsigned)(td.x0 + tx + g) + (unsigned)(td.y0 + w + g) * js;
 
-        const int zbase = td.z0 - 2 * R;  // lowest plane loaded
-        auto slotz = [&](int z) { return (z - zbase) % W; };
+        // Plane bookkeeping without divisions in the z loop: wrapped plane
+        // indices advance by one per plane, and window slots are those of
+        // consecutive planes modulo W (the lowest loaded plane z0 - 2R has
+        // slot 0).
+        auto next = [](int v, int n) { return v + 1 == n ? 0 : v + 1; };
+        auto cyc = [](int v) { return v >= W ? v - W : v; };  // v < 2W
+        int zwin = wrapi(td.z0 - 2 * R, nz);  // next window plane to load
+        int zpt = wrapi(td.z0 - R, nz);       // next point plane to load
+        int zst = wrapi(td.z0, nz);           // next plane to store
+        int swin = 0;                         // slot of the next window plane
 
         // Next plane's loads (registers).
         S v0[NS][L0];
         S vp[NPA][L1];
-        auto load_window = [&](int z) {
-            const unsigned zo = zoff(z);
+        auto load_window = [&]() {
+            const unsigned zo = (unsigned)(zwin + g) * ks;
+            zwin = next(zwin, nz);
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
                 for (int m = 0; m < L0; ++m)
                     v0[s][m] = ld<S>(a.u0, org + CS::comp_of_slot(s) * a.cstride, c0[m] + zo);
         };
-        auto load_point = [&](int z) {
-            const unsigned zo = zoff(z);
+        auto load_point = [&]() {
+            const unsigned zo = (unsigned)(zpt + g) * ks;
+            zpt = next(zpt, nz);
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int m = 0; m < L1; ++m)
                     vp[p][m] = ld<S>(a.u0, org + CS::comp_of_point(p) * a.cstride, c1[m] + zo);
         };
-        auto store_window = [&](int z) {
-            const int sl = slotz(z);
+        auto store_window = [&]() {
+            const int sl = swin;
+            swin = next(swin, W);
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -185,34 +196,35 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         // Prologue: planes z0-2R .. z0-1 of the window; prefetch plane z0
         // of the window and plane z0-R of the point components.
         for (int z = td.z0 - 2 * R; z < td.z0; ++z) {
-            load_window(z);
-            store_window(z);
+            load_window();
+            store_window();
         }
-        load_window(td.z0);
-        load_point(td.z0 - R);
+        load_window();
+        load_point();
+        int slo = 0;  // slot of plane zz - R (lowest plane of stage one's window)
 
         S dk1[R + 1][NC];    // k1 of the own cell, planes zz-R .. zz
         S du1[R + 1][NPA];   // u1 of the point components, same planes
 
         for (int zz = td.z0 - R; zz < td.z1 + R; ++zz) {
             // Stage one runs on plane zz: window planes zz-R .. zz+R.
-            store_window(zz + R);
+            store_window();
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int m = 0; m < L1; ++m)
                     if (e1ok[m]) p0[p][l1[m]] = vp[p][m];
             if (zz + 1 < td.z1 + R) {
-                load_window(zz + 1 + R);
-                load_point(zz + 1);
+                load_window();
+                load_point();
             }
             __syncthreads();
 
             // ---- stage one at plane zz over the tile + R ----
+            const int sz = cyc(slo + R);
             int zpl0[W];
 #pragma unroll
-            for (int d = 0; d < W; ++d) zpl0[d] = (slotz(zz + d - R) - slotz(zz)) * E0;
-            const int sz = slotz(zz);
+            for (int d = 0; d < W; ++d) zpl0[d] = (cyc(slo + d) - sz) * E0;
 #pragma unroll
             for (int s = 0; s < R; ++s)
 #pragma unroll
@@ -293,11 +305,13 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 
             // ---- stage two at plane zc = zz - R on the tile ----
             const int zc = zz - R;
+            const int s2 = cyc(slo + W - R);  // slot of plane zz - 2R
+            slo = next(slo, W);
             if (zc < td.z0) continue;
+            const int sc = slo == 0 ? W - 1 : slo - 1;  // slot of plane zc
             int zpl1[W];
 #pragma unroll
-            for (int d = 0; d < W; ++d) zpl1[d] = (slotz(zc + d - R) - slotz(zc)) * E1;
-            const int sc = slotz(zc);
+            for (int d = 0; d < W; ++d) zpl1[d] = (cyc(s2 + d) - sc) * E1;
             const int lc = l1[0];
             const int gi = bd.lo[0] + td.x0 + tx, gj = bd.lo[1] + td.y0 + w,
                       gk = bd.lo[2] + zc;
@@ -336,8 +350,10 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                 Ops::gw_rhs(rv, sv, gi, gj, gk, a.lev, t2, dt, dx, gpar2);
                 Ops::gw_backreaction(rv, sv, gi, gj, gk, a.lev, t2, dt, dx, par2, gpar2);
             }
+            const unsigned zso = (unsigned)(zst + g) * ks;
+            zst = next(zst, nz);
             if (own) {
-                const unsigned o = cown + zoff(zc);
+                const unsigned o = cown + zso;
 #pragma unroll
                 for (int n = 0; n < NC; ++n) {
                     const double k = rhs[n] + (double)dk1[0][n];

@@ -787,7 +787,7 @@ void Runtime::launch_fused(LevelState &ls, const double *prm1, const double *prm
                              : ntiles;
     a.xcd_split = lc.xcd_split ? 1 : 0;
     if (a.xcd_split) grid = (int)round_up(grid, 8);
-    EventPair *ep = timing_on ? timed_pair(0) : nullptr;
+    EventPair *ep = timing_on ? timed_pair(2) : nullptr;
     if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
     SHM_HIP(e.fn(a, grid, stream_));
     if (ep) {
@@ -1621,9 +1621,13 @@ void Runtime::collect_timing() {
     for (size_t i = 0; i < ev_used_; ++i) {
         float ms = 0;
         SHM_HIP(hipEventElapsedTime(&ms, ev_pool_[i].a, ev_pool_[i].b));
-        if (ev_pool_[i].kind == 0) {
+        if (ev_pool_[i].kind != 1) {  // 0: stage, 2: fused stages one and two
             timing.stage_ms += ms;
             timing.stage_launches += 1;
+            if (ev_pool_[i].kind == 2) {
+                timing.fused_ms += ms;
+                timing.fused_launches += 1;
+            }
         } else {
             timing.halo_ms += ms;
             timing.halo_launches += 1;

@@ -132,6 +132,9 @@ struct Timing {
     double stage_ms = 0, halo_ms = 0;
     long long stage_launches = 0, halo_launches = 0;
     long long stage_cells = 0;  // valid cells updated by stage launches
+    // Launches of the fused stage-one-and-two kernel (also counted above).
+    double fused_ms = 0;
+    long long fused_launches = 0;
 };
 
 class Runtime {
@@ -197,7 +200,7 @@ class Runtime {
   private:
     struct EventPair {
         hipEvent_t a, b;
-        int kind;  // 0 stage kernel, 1 halo kernel
+        int kind;  // 0 stage kernel, 1 halo kernel, 2 fused stages one and two
     };
     EventPair *timed_pair(int kind);
     std::vector<EventPair> ev_pool_;
