@@ -80,13 +80,14 @@ WORD = {"fp32c16": (4, 2), "fp32c": (4, 4), "fp64": (8, 0)}
 
 def moved_per_cell(prec_key, kernel):
     w, r = WORD[prec_key]
-    return C * (2 * w if kernel == "fused12" else 4 * w + 2 * r)
+    return C * (2 * w if kernel == "fused12" else 3 * w + 2 * r)
 
 
 # HBM bytes per launch at 512^3, FP32 + bf16 remainder, fused schedule: DRAM
 # read + write requests counted by rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B
 # TCC_EA0_WRREQ_WRITE_DRAM_32B (x 32 B), profiles/stage_kernel_traffic_complex512.md.
-TRAFFIC_BYTES_512 = {"fused12": None, "stage3": None}
+TRAFFIC_BYTES_512 = {"fused12": (74270667 + 67108864) * 32.0,
+                     "stage3": (197784618 + 100663296) * 32.0}
 
 
 def kernel_record(name, avg_ms, alg_bytes, moved, traffic):
