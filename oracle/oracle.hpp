@@ -380,6 +380,7 @@ class Sim {
     std::vector<double> dx, dt;
     std::vector<MultiFab> grid_new, grid_old;
     MultiFab shadow, shadow_tmp;
+    MultiFab rhs_out;  // FillRhs target of the C interface, per instance
     std::vector<double> rhs_te;  // last truncation errors (per coarse cell)
 
     explicit Sim(const std::string &inputs) {

@@ -55,7 +55,6 @@
         }                                                                      \
         return s->shadow;                                                      \
     }                                                                          \
-    static orc::MultiFab orc_rhs;                                              \
     ORC_EXPORT const char *orc_last_error() { return orc_err.c_str(); }        \
     ORC_EXPORT void *orc_create(const char *inputs) {                          \
         try {                                                                  \
@@ -115,18 +114,19 @@
                                 int add) {                                     \
         return orc_guard([&] {                                                 \
             auto *s = static_cast<OrcSim *>(h);                                \
-            if (!add || !orc_rhs.defined())                                    \
-                orc_rhs.define(OrcSim::boxes_of(s->grid_new[lev]), s->nc,      \
-                               s->cfg.nghost);                                 \
+            if (!add || !s->rhs_out.defined())                                 \
+                s->rhs_out.define(OrcSim::boxes_of(s->grid_new[lev]), s->nc,   \
+                                  s->cfg.nghost);                              \
             s->fill_patch(s->grid_new[lev], lev);                              \
-            s->fill_rhs(orc_rhs, s->grid_new[lev], time, lev, s->dt[lev],      \
+            s->fill_rhs(s->rhs_out, s->grid_new[lev], time, lev, s->dt[lev],   \
                         s->dx[lev], add != 0, w);                              \
         });                                                                    \
     }                                                                          \
     ORC_EXPORT int orc_download_rhs(void *h, int comp, double *d) {            \
         return orc_guard([&] {                                                 \
-            orc_valid(orc_rhs, [&](orc::Fab &f, int i, int j, int k,           \
-                                   long long o) { d[o] = f.at(i, j, k, comp); }); \
+            orc_valid(static_cast<OrcSim *>(h)->rhs_out,                       \
+                      [&](orc::Fab &f, int i, int j, int k,                    \
+                          long long o) { d[o] = f.at(i, j, k, comp); });       \
         });                                                                    \
     }                                                                          \
     ORC_EXPORT int orc_advance(void *h, int lev) {                             \
