@@ -366,6 +366,7 @@ struct Config {
     bool shadow_hierarchy = false;
     bool independent = false;  // level-0 boxes are separate periodic domains
     double regrid_dt = DBL_MAX;
+    int interp_type = 2;  // amr.interpolation_type: 0 piecewise constant, 2 quadratic
 };
 
 template <class Proj>
@@ -438,6 +439,12 @@ class Sim {
             cfg.ko_order = cfg.nghost;
             pp.query("sim.dissipation_order", cfg.ko_order);
         }
+        // LevelSynchronizer::LevelSynchronizer (level_synchronizer.cpp:27-47).
+        pp.query("amr.interpolation_type", cfg.interp_type);
+        if (cfg.interp_type != 0 && cfg.interp_type != 2)
+            throw std::runtime_error(
+                "amr.interpolation_type: 0 (piecewise constant) and 2 (quadratic) "
+                "are supported");
         switch (cfg.integrator) {
             case 20: {
                 amrex::ParmParse pr("integrator.rkn");
@@ -567,12 +574,130 @@ class Sim {
         }
     }
 
-    void fill_patch(MultiFab &m, int lev) {
+    /** FillPatch / FillIntermediatePatch (level_synchronizer.cpp:92-209) at
+     *  `time`: on a fine level every ghost cell first takes level lev-1
+     *  interpolated in time and space, then the same-level fill overwrites
+     *  the ghost cells that boxes of the level cover (FillPatchTwoLevels). */
+    void fill_patch(MultiFab &m, int lev, double time) {
         if (cfg.independent && lev == 0) {
             fill_boundary_independent(m);
             return;
         }
+        if (lev > 0) fill_coarse_ghosts(m, lev, time);
         fill_boundary(m, lev < 0 ? cfg.N0 / 2 : dimN[lev]);
+    }
+
+    /** Weights of the level-lev states at `time`: GetLevelData picks the new
+     *  or the old state within 1e-3 of their time difference, else both
+     *  (level_synchronizer.cpp:613-632), combined linearly in time. */
+    void time_weights(int lev, double time, double &wo, double &wn) const {
+        const MultiFab &New = grid_new[lev], &Old = grid_old[lev];
+        const double teps = std::fabs(New.t - Old.t) * 1.e-3;
+        if (time > New.t - teps && time < New.t + teps) {
+            wo = 0.;
+            wn = 1.;
+        } else if (time > Old.t - teps && time < Old.t + teps) {
+            wo = 1.;
+            wn = 0.;
+        } else {
+            wo = (New.t - time) / (New.t - Old.t);
+            wn = (time - Old.t) / (New.t - Old.t);
+        }
+    }
+
+    /** Coarse part of FillPatchTwoLevels for every ghost cell of level lev.
+     *  Spatial interpolation, ratio 2, at the fine cell centre x, y, z =
+     *  +-1/4 coarse cells from the parent's centre:
+     *    type 0: the parent value;
+     *    type 2: c + sx x + sy y + sz z + sxx x^2 + syy y^2 + szz z^2
+     *            + sxy x y + sxz x z + syz y z, with central differences
+     *            sx = (c[+1] - c[-1]) / 2, sxx = (c[+1] - 2 c + c[-1]) / 2,
+     *            sxy = (c[+1,+1] - c[-1,+1] - c[+1,-1] + c[-1,-1]) / 4:
+     *            exact for every polynomial of degree <= 2.
+     *  Coarse values are the time-weighted states of level lev-1 at the
+     *  periodic image of each coarse cell; a coarse cell that no box of
+     *  level lev-1 holds is an error (the level is not properly nested). */
+    void fill_coarse_ghosts(MultiFab &m, int lev, double time) {
+        MultiFab &New = grid_new[lev - 1], &Old = grid_old[lev - 1];
+        double wo, wn;
+        time_weights(lev - 1, time, wo, wn);
+        const int Nc = dimN[lev - 1];
+        auto wrap = [Nc](int v) { return ((v % Nc) + Nc) % Nc; };
+        // 19 points: centre, 6 faces, 12 edges (i, j, k offsets).
+        static const int P[19][3] = {
+            {0, 0, 0},  {-1, 0, 0}, {1, 0, 0},   {0, -1, 0},  {0, 1, 0},
+            {0, 0, -1}, {0, 0, 1},  {1, 1, 0},   {-1, 1, 0},  {1, -1, 0},
+            {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1},  {1, 0, -1},  {-1, 0, -1},
+            {0, 1, 1},  {0, -1, 1}, {0, 1, -1},  {0, -1, -1}};
+        const int np = cfg.interp_type == 0 ? 1 : 19;
+        bool nested = true;
+        for (Fab &F : m.fabs) {
+            const int g = F.g;
+#pragma omp parallel for schedule(dynamic)
+            for (int k = F.box.lo[2] - g; k <= F.box.hi[2] + g; ++k)
+                for (int j = F.box.lo[1] - g; j <= F.box.hi[1] + g; ++j)
+                    for (int i = F.box.lo[0] - g; i <= F.box.hi[0] + g; ++i) {
+                        if (i >= F.box.lo[0] && i <= F.box.hi[0] && j >= F.box.lo[1] &&
+                            j <= F.box.hi[1] && k >= F.box.lo[2] && k <= F.box.hi[2])
+                            continue;
+                        const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+                        int fb[19], ci[19], cj[19], ck[19];
+                        bool ok = true;
+                        for (int p = 0; p < np; ++p) {
+                            ci[p] = wrap(ic + P[p][0]);
+                            cj[p] = wrap(jc + P[p][1]);
+                            ck[p] = wrap(kc + P[p][2]);
+                            fb[p] = -1;
+                            for (size_t b = 0; b < New.fabs.size() && fb[p] < 0; ++b) {
+                                const Box &B = New.fabs[b].box;
+                                if (ci[p] >= B.lo[0] && ci[p] <= B.hi[0] && cj[p] >= B.lo[1] &&
+                                    cj[p] <= B.hi[1] && ck[p] >= B.lo[2] && ck[p] <= B.hi[2])
+                                    fb[p] = (int)b;
+                            }
+                            if (fb[p] < 0) ok = false;
+                        }
+                        if (!ok) {
+#pragma omp atomic write
+                            nested = false;
+                            continue;
+                        }
+                        for (int n = 0; n < m.nc; ++n) {
+                            double c[19];
+                            for (int p = 0; p < np; ++p) {
+                                const double vn = New.fabs[fb[p]].at(ci[p], cj[p], ck[p], n);
+                                if (wo == 0.)
+                                    c[p] = vn;
+                                else {
+                                    const double vo = Old.fabs[fb[p]].at(ci[p], cj[p], ck[p], n);
+                                    c[p] = wn == 0. ? vo : wo * vo + wn * vn;
+                                }
+                            }
+                            if (np == 1) {
+                                F.at(i, j, k, n) = c[0];
+                                continue;
+                            }
+                            const double x = (i & 1) ? 0.25 : -0.25;
+                            const double y = (j & 1) ? 0.25 : -0.25;
+                            const double z = (k & 1) ? 0.25 : -0.25;
+                            const double sx = 0.5 * (c[2] - c[1]), sy = 0.5 * (c[4] - c[3]),
+                                         sz = 0.5 * (c[6] - c[5]);
+                            const double sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
+                            const double syy = 0.5 * (c[4] - 2. * c[0] + c[3]);
+                            const double szz = 0.5 * (c[6] - 2. * c[0] + c[5]);
+                            const double sxy = 0.25 * (c[7] - c[8] - c[9] + c[10]);
+                            const double sxz = 0.25 * (c[11] - c[12] - c[13] + c[14]);
+                            const double syz = 0.25 * (c[15] - c[16] - c[17] + c[18]);
+                            F.at(i, j, k, n) = c[0] + x * sx + y * sy + z * sz +
+                                               x * x * sxx + y * y * syy + z * z * szz +
+                                               x * y * sxy + x * z * sxz + y * z * syz;
+                        }
+                    }
+        }
+        if (!nested)
+            throw std::runtime_error("level " + std::to_string(lev) +
+                                     " is not properly nested: a ghost cell needs coarse "
+                                     "cells that no box of level " +
+                                     std::to_string(lev - 1) + " holds");
     }
 
     /** IntegratorLsssprk3::Integrate (lsssprk3.cpp:14-31). */
@@ -582,13 +707,13 @@ class Sim {
         k1.define(boxes_of(mo), nc, cfg.nghost);
         fill_rhs(k1, mo, t0, lev, dtl, dxl, false, 0.);
         LinComb(mn, 1, mo, 0, dtl, k1, 0, 0, nc, 0);
-        fill_patch(mn, lev);
+        fill_patch(mn, lev, t1);
         fill_rhs(k1, mn, t1, lev, dtl, dxl, true, 1.);
         LinComb(mn, 1, mo, 0, dtl / 4., k1, 0, 0, nc, 0);
-        fill_patch(mn, lev);
+        fill_patch(mn, lev, t0 + dtl / 2.);
         fill_rhs(k1, mn, t0 + dtl / 2., lev, dtl, dxl, true, 0.25);
         LinComb(mn, 1, mo, 0, dtl * 2. / 3., k1, 0, 0, nc, 0);
-        fill_patch(mn, lev);
+        fill_patch(mn, lev, t1);
     }
 
     /** IntegratorRkn::Integrate (rkn.cpp:91-186). */
@@ -611,7 +736,7 @@ class Sim {
                 if (Ngrav > 0)
                     Saxpy(mn, dtl * dtl * cfg.tableau[i][j], F[j], gN1, gN0, gN, ng);
             }
-            fill_patch(mn, lev);
+            fill_patch(mn, lev, st);
             fill_rhs(F[i], mn, st, lev, dtl, dxl, false, 0.);
         }
         Copy(mn, mo, 0, 0, N, ng);
@@ -624,7 +749,7 @@ class Sim {
             Saxpy(mn, dtl * dtl * cfg.bbar[i], F[i], gN1, gN0, gN, ng);
             Saxpy(mn, dtl * cfg.b[i], F[i], gN1, gN1, gN, ng);
         }
-        fill_patch(mn, lev);
+        fill_patch(mn, lev, t + dtl);
     }
 
     /** IntegratorLeapfrog::Integrate (leapfrog.cpp:13-57). */
@@ -640,11 +765,11 @@ class Sim {
         LinComb(vh, 1, mo, 0, dtl / 2., a, 0, 0, N, 0);
         LinComb(mn, 1, mo, uN0, dtl, vh, uN1, uN0, uN, 0);
         if (Ngrav > 0) LinComb(mn, 1, mo, gN0, dtl, vh, gN1, gN0, gN, 0);
-        fill_patch(mn, lev);
+        fill_patch(mn, lev, t1);
         fill_rhs(a, mn, t1, lev, dtl, dxl, false, 0.);
         LinComb(mn, 1, vh, uN1, dtl / 2., a, uN1, uN1, uN, 0);
         if (Ngrav > 0) LinComb(mn, 1, vh, gN1, dtl / 2., a, gN1, gN1, gN, 0);
-        fill_patch(mn, lev);
+        fill_patch(mn, lev, t1);
     }
 
     static std::vector<Box> boxes_of(const MultiFab &m) {
@@ -670,7 +795,7 @@ class Sim {
         MultiFab &mo = lev < 0 ? shadow_tmp : grid_old[lev];
         MultiFab &mn = lev < 0 ? shadow : grid_new[lev];
         if (!mn.defined()) mn.define(boxes_of(mo), nc, cfg.nghost);
-        fill_patch(mo, lev);
+        fill_patch(mo, lev, mo.t);
         integrate(mo, mn, lev, dtl, dxl);
         mn.t = mo.t + dtl;
         mn.istep = mo.istep + 1;

@@ -100,6 +100,30 @@
                       });                                                      \
         });                                                                    \
     }                                                                          \
+    ORC_EXPORT int orc_upload_old(void *h, int lev, int comp, const double *d, \
+                                  double t) {                                  \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            orc_valid(s->grid_old[lev],                                        \
+                      [&](orc::Fab &f, int i, int j, int k, long long o) {     \
+                          f.at(i, j, k, comp) = d[o];                          \
+                      });                                                      \
+            s->grid_old[lev].t = t;                                            \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_fill_patch(void *h, int lev, double time) {             \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            s->fill_patch(s->grid_new[lev], lev, time);                        \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_download_fab(void *h, int lev, int box, int comp,       \
+                                    double *d) {                               \
+        return orc_guard([&] {                                                 \
+            orc::Fab &f = static_cast<OrcSim *>(h)->grid_new[lev].fabs.at(box);\
+            std::memcpy(d, f.d.data() + comp * f.nst, sizeof(double) * f.nst); \
+        });                                                                    \
+    }                                                                          \
     ORC_EXPORT int orc_download(void *h, int lev, int comp, double *d,         \
                                 int old) {                                     \
         return orc_guard([&] {                                                 \
@@ -117,7 +141,7 @@
             if (!add || !s->rhs_out.defined())                                 \
                 s->rhs_out.define(OrcSim::boxes_of(s->grid_new[lev]), s->nc,   \
                                   s->cfg.nghost);                              \
-            s->fill_patch(s->grid_new[lev], lev);                              \
+            s->fill_patch(s->grid_new[lev], lev, s->grid_new[lev].t);          \
             s->fill_rhs(s->rhs_out, s->grid_new[lev], time, lev, s->dt[lev],   \
                         s->dx[lev], add != 0, w);                              \
         });                                                                    \

@@ -309,6 +309,135 @@ __global__ void __launch_bounds__(256) avgdown_kernel(const AvgArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Coarse/fine ghost fill (FillPatchTwoLevels, level_synchronizer.cpp:92-209).
+// Every ghost cell of every fine box takes the coarse level interpolated in
+// time and space; the same-level copy plan, run afterwards, overwrites the
+// ghost cells other fine boxes cover.  Spatial interpolation, ratio 2, at
+// the fine cell centre x, y, z = +-1/4 coarse cells from the parent centre:
+//   piecewise constant: the parent value;
+//   quadratic: c + sx x + sy y + sz z + sxx x^2 + syy y^2 + szz z^2
+//              + sxy x y + sxz x z + syz y z from central differences of
+//              the 19-point coarse neighbourhood (exact for polynomials of
+//              degree <= 2).
+// What a ghost cell receives depends on how the stage kernels form the
+// state they read there:
+//   kCfState  the state itself (value and remainder);
+//   kCfK      k of a composed SSPRK3 stage state u0 + c k: (v - u0) / c;
+//   kCfForce  the last force F_m of a composed RKN stage state
+//             phi0 + c0 pi0 + sum_{j<=m} c_j F_j:
+//             (v - phi0 - c0 pi0 - sum_{j<m} c_j F_j) / c_m.
+// ---------------------------------------------------------------------------
+
+__device__ AMREX_FORCE_INLINE int cf_wrap(int v, int n) {
+    const int r = v % n;
+    return r < 0 ? r + n : r;
+}
+
+template <Prec P, int MODE, bool QUAD>
+__global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
+    using T = PrecTraits<P>;
+    using S = typename T::S;
+    constexpr int NPT = QUAD ? 19 : 1;
+    // Centre, 6 faces, then the 12 edges as (+,+) (-,+) (+,-) (-,-) pairs.
+    constexpr int PO[19][3] = {
+        {0, 0, 0},   {-1, 0, 0}, {1, 0, 0},  {0, -1, 0}, {0, 1, 0},
+        {0, 0, -1},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0}, {1, -1, 0},
+        {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
+        {0, 1, 1},   {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < a.total; t += stride) {
+        int lo = 0, hi = a.nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.segs[mid].start <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const HaloSeg &sg = a.segs[lo];
+        long long r = t - sg.start;
+        const int x = (int)(r % sg.n[0]);
+        r /= sg.n[0];
+        const int y = (int)(r % sg.n[1]);
+        const int z = (int)(r / sg.n[1]);
+        const int gi = sg.lo[0] + x, gj = sg.lo[1] + y, gk = sg.lo[2] + z;
+        const int fb = sg.dst_box;
+        const BoxDesc &db = a.fboxes[fb];
+        const long long fe = db.base + (gi - db.lo[0]) +
+                             (long long)(gj - db.lo[1]) * db.jstride +
+                             (long long)(gk - db.lo[2]) * db.kstride;
+        const int ic = gi >> 1, jc = gj >> 1, kc = gk >> 1;
+        long long ce[NPT];
+        bool ok = true;
+#pragma unroll
+        for (int p = 0; p < NPT; ++p) {
+            const int ci = cf_wrap(ic + PO[p][0], a.Nc), cj = cf_wrap(jc + PO[p][1], a.Nc),
+                      ck = cf_wrap(kc + PO[p][2], a.Nc);
+            ce[p] = -1;
+            for (int q = a.cand_start[fb]; q < a.cand_start[fb + 1]; ++q) {
+                const BoxDesc &cb = a.cboxes[a.cand[q]];
+                if (ci >= cb.lo[0] && ci < cb.lo[0] + cb.n[0] && cj >= cb.lo[1] &&
+                    cj < cb.lo[1] + cb.n[1] && ck >= cb.lo[2] && ck < cb.lo[2] + cb.n[2]) {
+                    ce[p] = cb.base + (ci - cb.lo[0]) + (long long)(cj - cb.lo[1]) * cb.jstride +
+                            (long long)(ck - cb.lo[2]) * cb.kstride;
+                    break;
+                }
+            }
+            ok = ok && ce[p] >= 0;
+        }
+        const double xo = (gi & 1) ? 0.25 : -0.25, yo = (gj & 1) ? 0.25 : -0.25,
+                     zo = (gk & 1) ? 0.25 : -0.25;
+        for (int n = 0; n < a.ncomp; ++n) {
+            const int comp = a.comp[n];
+            double c[NPT];
+#pragma unroll
+            for (int p = 0; p < NPT; ++p) {
+                const long long e = (long long)comp * a.ccstride + (ok ? ce[p] : 0);
+                double v = 0.;
+                if (a.cnew) v = load_state_at<T>(a.cnew, a.cnewlo, e);
+                if (a.cold) {
+                    const double vo = load_state_at<T>(a.cold, a.coldlo, e);
+                    v = a.cnew ? a.wo * vo + a.wn * v : vo;
+                }
+                c[p] = v;
+            }
+            double v;
+            if constexpr (QUAD) {
+                const double sx = 0.5 * (c[2] - c[1]), sy = 0.5 * (c[4] - c[3]),
+                             sz = 0.5 * (c[6] - c[5]);
+                const double sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
+                const double syy = 0.5 * (c[4] - 2. * c[0] + c[3]);
+                const double szz = 0.5 * (c[6] - 2. * c[0] + c[5]);
+                const double sxy = 0.25 * (c[7] - c[8] - c[9] + c[10]);
+                const double sxz = 0.25 * (c[11] - c[12] - c[13] + c[14]);
+                const double syz = 0.25 * (c[15] - c[16] - c[17] + c[18]);
+                v = c[0] + xo * sx + yo * sy + zo * sz + xo * xo * sxx + yo * yo * syy +
+                    zo * zo * szz + xo * yo * sxy + xo * zo * sxz + yo * zo * syz;
+            } else {
+                v = c[0];
+            }
+            if (!ok) v = __builtin_nan("");
+            if constexpr (MODE == kCfState) {
+                store_state_at<T>(a.dst, a.dstlo, (long long)comp * a.fcstride + fe, v);
+            } else if constexpr (MODE == kCfK) {
+                const long long e = (long long)comp * a.fcstride + fe;
+                const double u0 = (double)ld<S>(a.u0, e, 0u);
+                st<S>(a.dst, e, 0u, (S)((v - u0) / (double)(S)a.c0));
+            } else {
+                double yv = (double)ld<S>(a.u0, (long long)comp * a.fcstride + fe, 0u) +
+                            (double)(S)a.c0 *
+                                (double)ld<S>(a.u0, (long long)a.mom[n] * a.fcstride + fe, 0u);
+                const long long pe = (long long)a.pair[n] * a.fcstride + fe;
+                for (int j = 0; j < a.m; ++j)
+                    yv += (double)(S)a.cf[j] * (double)ld<S>(a.f[j], pe, 0u);
+                st<S>(a.dst, pe, 0u, (S)((v - yv) / (double)(S)a.cf[a.m]));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Gravitational-wave spectrum: pack -> hipFFT D2Z -> projected binning.
 // ---------------------------------------------------------------------------
 

@@ -53,4 +53,32 @@ struct TePair {
     int crse_box;
 };
 
+/** Coarse/fine ghost fill: what a ghost cell receives (see cf_fill_kernel). */
+enum CfMode { kCfState = 0, kCfK = 1, kCfForce = 2 };
+
+struct CfArgs {
+    const BoxDesc *fboxes;
+    const BoxDesc *cboxes;
+    const HaloSeg *segs;       // ghost-shell slabs; dst_box = fine box
+    int nseg;
+    long long total;
+    const int *cand_start;     // per fine box: range in cand
+    const int *cand;           // coarse boxes the box's coarse stencil touches
+    int Nc;                    // coarse domain cells per axis
+    long long fcstride, ccstride;
+    const void *cold, *coldlo; // coarse state at t_old (null: weight 0)
+    const void *cnew, *cnewlo; // coarse state at t_new (null: weight 0)
+    double wo, wn;
+    int ncomp;
+    int comp[32];              // interpolated component (fine = coarse index)
+    void *dst, *dstlo;         // state (kCfState), k (kCfK), F_m (kCfForce)
+    const void *u0;            // fine u0 / phi0, pi0 (kCfK, kCfForce)
+    double c0;                 // kCfK: c; kCfForce: coefficient of pi0
+    int mom[32];               // kCfForce: momentum component of each pair
+    int pair[32];              // kCfForce: pair index (force component)
+    int m;                     // kCfForce: index of the force written
+    const void *f[4];          // kCfForce: F_0 .. F_{m-1}
+    double cf[5];              // kCfForce: coefficients of F_0 .. F_m
+};
+
 }  // namespace shm

@@ -485,6 +485,11 @@ void Runtime::init_from_inputs(const std::string &inputs,
         if (S.dissipation_strength[n] > 0) S.with_dissipation = true;
     }
     ko_ = S.with_dissipation;
+    // Coarse/fine interpolation (LevelSynchronizer, level_synchronizer.cpp:27-47).
+    pp.query("amr.interpolation_type", interp_type_);
+    if (interp_type_ != 0 && interp_type_ != 2)
+        throw Error(1, "amr.interpolation_type: 0 (piecewise constant) and 2 "
+                       "(quadratic) are supported");
     if (ko_) {
         S.dissipation_order = S.nghost;
         pp.query("sim.dissipation_order", S.dissipation_order);
@@ -676,6 +681,7 @@ void Runtime::set_level(int lev, const std::vector<Box3> &boxes) {
     ls.t = sim_->t_start;
     ls.t_old = -DBL_MAX;
     ls.istep = 0;
+    if (lev + 1 < (int)levels_.size()) levels_[lev + 1].cf_built = false;
     if (lev > sim_->finest_level) sim_->finest_level = lev;
 }
 
@@ -877,10 +883,180 @@ void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps,
     }
 }
 
-void Runtime::fill_patch(LevelState &ls, int which) {
-    if (ls.ghosts_ok[which]) return;
+void Runtime::fill_patch(LevelState &ls, int which, int lev, double time) {
+    if (has_cf(lev)) {
+        // The coarser level may have changed (its step, average-down).
+        cf_fill_state(lev, time, ls.U[which].get(),
+                      comp() ? ls.Ulo[which].get() : nullptr);
+    } else if (ls.ghosts_ok[which]) {
+        return;
+    }
     halo(ls, ls.U[which].get(), all_comps());
     ls.ghosts_ok[which] = true;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse/fine ghost fill (FillPatchTwoLevels, level_synchronizer.cpp:92-209).
+// ---------------------------------------------------------------------------
+
+static int floordiv2(int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+/** Ghost-shell slabs of every box of level lev, and per box the boxes of
+ *  level lev-1 its coarse stencil may touch.  Every coarse cell that stencil
+ *  can reach must lie in a box of level lev-1 (proper nesting), else the
+ *  ghost cells would have no coarse data. */
+void Runtime::build_cf(int lev) {
+    LevelState &ls = levels_[lev];
+    const LevelState &cs = levels_[lev - 1];
+    const int g = ls.L.g, Nc = cs.L.N;
+    const int s = interp_type_ == 2 ? 1 : 0;
+    ls.cf_segs.clear();
+    ls.cf_total = 0;
+    std::vector<int> start(1, 0), cand;
+    for (int b = 0; b < (int)ls.L.boxes.size(); ++b) {
+        const Box3 &bx = ls.L.boxes[b];
+        const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
+        const int slab[6][6] = {
+            {bx.lo[0] - g, bx.lo[1] - g, bx.lo[2] - g, nx + 2 * g, ny + 2 * g, g},
+            {bx.lo[0] - g, bx.lo[1] - g, bx.hi[2] + 1, nx + 2 * g, ny + 2 * g, g},
+            {bx.lo[0] - g, bx.lo[1] - g, bx.lo[2], nx + 2 * g, g, nz},
+            {bx.lo[0] - g, bx.hi[1] + 1, bx.lo[2], nx + 2 * g, g, nz},
+            {bx.lo[0] - g, bx.lo[1], bx.lo[2], g, ny, nz},
+            {bx.hi[0] + 1, bx.lo[1], bx.lo[2], g, ny, nz}};
+        for (int q = 0; g > 0 && q < 6; ++q) {
+            HaloSeg sg{};
+            sg.dst_box = b;
+            sg.src_box = -1;
+            for (int a = 0; a < 3; ++a) {
+                sg.lo[a] = slab[q][a];
+                sg.n[a] = slab[q][3 + a];
+            }
+            sg.start = ls.cf_total;
+            ls.cf_total += (long long)sg.n[0] * sg.n[1] * sg.n[2];
+            ls.cf_segs.push_back(sg);
+        }
+        // Coarse cells the interpolation of this box's ghost cells reads.
+        int glo[3], ghi[3];
+        long long need = 1;
+        for (int a = 0; a < 3; ++a) {
+            glo[a] = floordiv2(bx.lo[a] - g) - s;
+            ghi[a] = floordiv2(bx.hi[a] + g) + s;
+            need *= ghi[a] - glo[a] + 1;
+        }
+        long long have = 0;
+        for (int c = 0; c < (int)cs.L.boxes.size(); ++c) {
+            const Box3 &cb = cs.L.boxes[c];
+            bool hit = false;
+            for (int sz = -1; sz <= 1; ++sz)
+                for (int sy = -1; sy <= 1; ++sy)
+                    for (int sx = -1; sx <= 1; ++sx) {
+                        const int sh[3] = {sx * Nc, sy * Nc, sz * Nc};
+                        long long v = 1;
+                        for (int a = 0; a < 3 && v > 0; ++a) {
+                            const int lo = std::max(glo[a], cb.lo[a] + sh[a]);
+                            const int hi = std::min(ghi[a], cb.hi[a] + sh[a]);
+                            v *= std::max(0, hi - lo + 1);
+                        }
+                        if (v > 0) {
+                            have += v;
+                            hit = true;
+                        }
+                    }
+            if (hit) cand.push_back(c);
+        }
+        if (have != need)
+            throw Error(1, "level " + std::to_string(lev) +
+                               " is not properly nested: ghost cells of box " +
+                               std::to_string(b) + " need coarse cells that no box of level " +
+                               std::to_string(lev - 1) + " holds");
+        start.push_back((int)cand.size());
+    }
+    upload_vec(ls.dcf_segs, ls.cf_segs);
+    upload_vec(ls.dcf_start, start);
+    if (cand.empty()) cand.push_back(0);
+    upload_vec(ls.dcf_cand, cand);
+    ls.cf_built = true;
+}
+
+CfArgs Runtime::cf_args(int lev, double time) {
+    LevelState &ls = levels_[lev];
+    if (!ls.cf_built) build_cf(lev);
+    LevelState &cs = levels_[lev - 1];
+    CfArgs a{};
+    a.fboxes = ls.L.dbox.as<BoxDesc>();
+    a.cboxes = cs.L.dbox.as<BoxDesc>();
+    a.segs = ls.dcf_segs.as<HaloSeg>();
+    a.nseg = (int)ls.cf_segs.size();
+    a.total = ls.cf_total;
+    a.cand_start = ls.dcf_start.as<int>();
+    a.cand = ls.dcf_cand.as<int>();
+    a.Nc = cs.L.N;
+    a.fcstride = ls.L.cstride;
+    a.ccstride = cs.L.cstride;
+    // GetLevelData (level_synchronizer.cpp:613-632): the new or the old
+    // state within 1e-3 of their time difference, else both, linear in time.
+    const int nw = cs.cur, od = cs.cur ^ 1;
+    const double teps = std::fabs(cs.t - cs.t_old) * 1.e-3;
+    if (time > cs.t - teps && time < cs.t + teps) {
+        a.cnew = cs.U[nw].get();
+        a.cnewlo = comp() ? cs.Ulo[nw].get() : nullptr;
+    } else if (time > cs.t_old - teps && time < cs.t_old + teps) {
+        a.cold = cs.U[od].get();
+        a.coldlo = comp() ? cs.Ulo[od].get() : nullptr;
+    } else {
+        a.cnew = cs.U[nw].get();
+        a.cnewlo = comp() ? cs.Ulo[nw].get() : nullptr;
+        a.cold = cs.U[od].get();
+        a.coldlo = comp() ? cs.Ulo[od].get() : nullptr;
+        a.wo = (cs.t - time) / (cs.t - cs.t_old);
+        a.wn = (time - cs.t_old) / (cs.t - cs.t_old);
+    }
+    return a;
+}
+
+template <Prec P, int M>
+static void cf_dispatch(const CfArgs &a, bool quad, int grid, hipStream_t s) {
+    if (quad)
+        hipLaunchKernelGGL((cf_fill_kernel<P, M, true>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((cf_fill_kernel<P, M, false>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <Prec P>
+static void cf_dispatch_mode(const CfArgs &a, int mode, bool quad, int grid,
+                             hipStream_t s) {
+    if (mode == kCfState)
+        cf_dispatch<P, kCfState>(a, quad, grid, s);
+    else if (mode == kCfK)
+        cf_dispatch<P, kCfK>(a, quad, grid, s);
+    else
+        cf_dispatch<P, kCfForce>(a, quad, grid, s);
+}
+
+void Runtime::cf_launch(CfArgs &a, int mode) {
+    if (a.total == 0 || a.ncomp == 0) return;
+    const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
+    const bool quad = interp_type_ == 2;
+    EventPair *ep = timing_on ? timed_pair(1) : nullptr;
+    if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
+    switch (prec_) {
+        case Prec::F64: cf_dispatch_mode<Prec::F64>(a, mode, quad, grid, stream_); break;
+        case Prec::F32C: cf_dispatch_mode<Prec::F32C>(a, mode, quad, grid, stream_); break;
+        case Prec::F32C16: cf_dispatch_mode<Prec::F32C16>(a, mode, quad, grid, stream_); break;
+    }
+    SHM_HIP(hipGetLastError());
+    if (ep) SHM_HIP(hipEventRecord(ep->b, stream_));
+}
+
+void Runtime::cf_fill_state(int lev, double time, void *hi, void *lo) {
+    CfArgs a = cf_args(lev, time);
+    a.dst = hi;
+    a.dstlo = lo;
+    for (int c0 = 0; c0 < ncomp_; c0 += 32) {
+        a.ncomp = std::min(32, ncomp_ - c0);
+        for (int n = 0; n < a.ncomp; ++n) a.comp[n] = c0 + n;
+        cf_launch(a, kCfState);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -933,6 +1109,34 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
     void *ulo_old = comp() ? ls.Ulo[old].get() : nullptr;
     void *ulo_new = comp() ? ls.Ulo[nw].get() : nullptr;
     ls.ghosts_ok[nw] = false;
+    // Ghost cells at coarse/fine boundaries take level lev-1 at each stage
+    // time (FillIntermediatePatch).  Stage states are composed from u0 and
+    // the stage arrays, so the fill writes the k or force ghost values that
+    // compose to the interpolated state.
+    const bool cf = has_cf(lev);
+    if (cf && integrator_ >= 20) {
+        if (nodes_[0] != 0.)
+            throw Error(4, "coarse/fine ghost fill needs an RKN tableau with nodes[0] = 0");
+        for (size_t i = 1; i < nodes_.size(); ++i)
+            if (tableau_[i][i - 1] == 0.)
+                throw Error(4, "coarse/fine ghost fill needs a nonzero RKN tableau "
+                               "sub-diagonal");
+        if (ko_)
+            throw Error(4, "dissipation with RKN on levels with coarse/fine ghost "
+                           "cells is not supported (stage momenta at ghost cells)");
+    }
+    auto cf_fill_k = [&](void *k, double time, double c) {
+        CfArgs f = cf_args(lev, time);
+        f.dst = k;
+        f.u0 = ls.U[old].get();
+        f.c0 = c;
+        const std::vector<int> sc = stencil_comps();
+        for (size_t c0 = 0; c0 < sc.size(); c0 += 32) {
+            f.ncomp = (int)std::min<size_t>(32, sc.size() - c0);
+            for (int q = 0; q < f.ncomp; ++q) f.comp[q] = sc[c0 + q];
+            cf_launch(f, kCfK);
+        }
+    };
 
     if (integrator_ == 10) {
         // Low-storage SSPRK3 with the stage states rebuilt from u0 and k:
@@ -948,6 +1152,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             a.x = ls.U[old].get();
             a.kout = ls.K[0].get();
             launch_stage(kSsprk1, ls, a, lev);
+            if (cf) cf_fill_k(ls.K[0].get(), st[1], dt);
             halo(ls, ls.K[0].get(), stencil_comps());
 
             StageArgs b{};
@@ -958,6 +1163,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             b.c[0] = 1.;
             b.c[2] = dt;
             launch_stage(kSsprk2, ls, b, lev);
+            if (cf) cf_fill_k(ls.K[1].get(), st[2], dt / 4.);
             halo(ls, ls.K[1].get(), stencil_comps());
         }
 
@@ -988,6 +1194,25 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             if (i < s - 1) {
                 a.fout = ls.F[i].get();
                 launch_stage(kRkn0 + i, ls, a, lev);
+                if (cf) {
+                    // Stage i+1 reads phi0 + dt c_{i+1} pi0 + sum_j dt^2 a_{i+1,j} F_j.
+                    CfArgs f = cf_args(lev, st[i + 1]);
+                    f.dst = ls.F[i].get();
+                    f.u0 = ls.U[old].get();
+                    f.c0 = dt * nodes_[i + 1];
+                    f.m = i;
+                    for (int j = 0; j <= i; ++j) f.cf[j] = dt * dt * tableau_[i + 1][j];
+                    for (int j = 0; j < i; ++j) f.f[j] = ls.F[j].get();
+                    for (int p0 = 0; p0 < npair; p0 += 32) {
+                        f.ncomp = std::min(32, npair - p0);
+                        for (int q = 0; q < f.ncomp; ++q) {
+                            f.pair[q] = p0 + q;
+                            f.comp[q] = pair_field(p0 + q);
+                            f.mom[q] = pair_mom(p0 + q);
+                        }
+                        cf_launch(f, kCfForce);
+                    }
+                }
                 halo(ls, ls.F[i].get(), pairs);
             } else {
                 a.c[6] = dt;
@@ -1014,6 +1239,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         a.c[0] = dt / 2.;
         a.c[1] = dt;
         launch_stage(kLeap1, ls, a, lev);
+        if (cf) cf_fill_state(lev, st[1], ls.U[nw].get(), ulo_new);
         halo(ls, ls.U[nw].get(), stencil_comps());
 
         StageArgs b{};
@@ -1026,6 +1252,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         launch_stage(kLeap2, ls, b, lev);
     }
     // FillIntermediatePatch at the end of the step.
+    if (cf) cf_fill_state(lev, t0 + dt, ls.U[nw].get(), ulo_new);
     halo(ls, ls.U[nw].get(), all_comps());
     ls.ghosts_ok[nw] = true;
 }
@@ -1054,7 +1281,9 @@ void Runtime::release_graphs() {
  *  lazily built table exists), replayed afterwards.  Returns false when the
  *  step has to run eagerly. */
 bool Runtime::replay_step(LevelState &ls, int lev, double dt, double dx, double t0) {
-    if (!graphs || timing_on || lev < 0) return false;
+    // Coarse/fine fills take time weights and coarse states that change from
+    // step to step, so levels with coarse/fine ghost cells run eagerly.
+    if (!graphs || timing_on || lev < 0 || has_cf(lev)) return false;
     auto w = warm_.find(lev);
     if (w == warm_.end() || w->second != epoch_) return false;
     const std::vector<double> st = stage_times(t0, dt);
@@ -1109,7 +1338,7 @@ void Runtime::advance(int lev) {
     const int old = ls.cur ^ 1;
     ls.t_old = ls.t;
     const double dt = sim_->dt[lev], dx = sim_->dx[lev];
-    fill_patch(ls, old);
+    fill_patch(ls, old, lev, ls.t_old);
     if (!replay_step(ls, lev, dt, dx, ls.t_old)) {
         integrate(ls, lev, dt, dx, ls.t_old);
         if (lev >= 0) warm_[lev] = epoch_;
@@ -1121,7 +1350,7 @@ void Runtime::advance(int lev) {
 void Runtime::fill_rhs(int lev, double time, double weight, bool add) {
     LevelState &ls = level(lev);
     if (!ls.RHS) ls.RHS.alloc((size_t)ls.L.cstride * ls.ncomp * esize());
-    fill_patch(ls, ls.cur);
+    fill_patch(ls, ls.cur, lev, ls.t);
     std::vector<double> blk(kPrmBlock);
     fill_params(blk.data(), time, sim_->dt[std::max(lev, 0)],
                 sim_->dx[std::max(lev, 0)], lev);
@@ -1198,7 +1427,7 @@ void Runtime::create_shadow_level() {
     // Integrator::Advance(-1): one step of 2 dt on 2 dx from buffer 0 into 1.
     shadow_.cur = 1;
     shadow_.t_old = shadow_.t;
-    fill_patch(shadow_, 0);
+    fill_patch(shadow_, 0, -1, shadow_.t);
     const double dt = sim_->dt[0] * 2., dx = sim_->dx[0] * 2.;
     integrate(shadow_, -1, dt, dx, shadow_.t_old);
     shadow_.t = shadow_.t_old + dt;
@@ -1649,7 +1878,7 @@ void Runtime::collect_timing() {
 void Runtime::autotune(int lev) {
     LevelState &ls = level(lev);
     const int old = ls.cur;
-    fill_patch(ls, old);
+    fill_patch(ls, old, lev, ls.t);
     const double dt = sim_->dt[lev], dx = sim_->dx[lev];
     std::vector<double> blk(kPrmBlock);
     fill_params(blk.data(), ls.t, dt, dx, lev);

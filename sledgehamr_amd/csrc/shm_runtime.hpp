@@ -100,6 +100,13 @@ struct LevelState {
     double t = -DBL_MAX;       // time of grid_new
     double t_old = -DBL_MAX;   // time of grid_old
     int istep = 0;
+    // Coarse/fine ghost fill plan (levels above 0): the ghost-shell slabs of
+    // the boxes, and per box the boxes of the coarser level that its coarse
+    // interpolation stencil may touch.
+    std::vector<HaloSeg> cf_segs;
+    DevMem dcf_segs, dcf_start, dcf_cand;
+    long long cf_total = 0;
+    bool cf_built = false;
     bool defined() const { return !L.boxes.empty(); }
 };
 
@@ -218,7 +225,18 @@ class Runtime {
                       double dt, int lev);
     void halo(LevelState &ls, void *arr, const std::vector<int> &comps,
               bool full = false);
-    void fill_patch(LevelState &ls, int which);
+    /** FillPatch of state `which` of level lev at `time`: coarse/fine
+     *  ghosts (levels above 0), then the same-level copy plan. */
+    void fill_patch(LevelState &ls, int which, int lev, double time);
+    /** Level lev has ghost cells filled from level lev-1. */
+    bool has_cf(int lev) const { return lev >= 1 && has_level(lev - 1); }
+    void build_cf(int lev);
+    /** Arguments of a coarse/fine fill of level lev at `time` (layout,
+     *  coarse states and their time weights); the caller adds the target. */
+    CfArgs cf_args(int lev, double time);
+    void cf_launch(CfArgs &a, int mode);
+    /** kCfState fill of every component of a state of level lev. */
+    void cf_fill_state(int lev, double time, void *hi, void *lo);
     void integrate(LevelState &ls, int lev, double dt, double dx,
                    double t_old);
     /** Times at which the integrator evaluates its stages. */
@@ -234,6 +252,14 @@ class Runtime {
     long epoch_ = 0;
     void probe();
     std::vector<int> stencil_comps() const;
+    // (field, momentum) pairs of the RKN / leapfrog integrators: the
+    // project's fields, then the six GW tensor fields (rkn.cpp:32-44).
+    int pair_field(int p) const {
+        return p < kt_.nfields ? p : kt_.nscalars + (p - kt_.nfields);
+    }
+    int pair_mom(int p) const {
+        return p < kt_.nfields ? kt_.nfields + p : kt_.nscalars + 6 + (p - kt_.nfields);
+    }
     std::vector<int> all_comps() const;
     bool te_pass_fused(int lev) const;
     void avg_down(LevelState &fine, int fine_which, LevelState &crse,
@@ -247,6 +273,7 @@ class Runtime {
     int ncomp_ = 0;
     int R_ = 1;
     bool gw_ = false, ko_ = false;
+    int interp_type_ = 2;  // amr.interpolation_type (0 or 2)
     bool independent_ = false;  // level-0 boxes are separate periodic domains
     std::map<int, std::vector<long long>> last_tags_;
     int integrator_ = 10;

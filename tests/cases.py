@@ -112,9 +112,12 @@ def bubble_params(n, t_from, t_to, t_lo=-2.0, t_hi=8.0, phi_in=1.0, R0=3.0, w=2.
     return [n, phi_in, R0, w, t_lo, t_hi, t_from, t_to]
 
 
-def amr_boxes(seed, nbox=6):
+def amr_boxes(seed, nbox=6, margin=4):
     """Nested static layout on a 32^3 base: level-1 boxes of edge 8/12/16 at
-    16-cell slots of the 64^3 level-1 domain, each holding one level-2 box."""
+    16-cell slots of the 64^3 level-1 domain, each holding at most one
+    level-2 box `margin` level-2 cells inside it on every side.  Coarse/fine
+    ghost interpolation needs ceil(nghost / 2) + 1 coarse cells of margin:
+    margin 4 serves nghost 2, margin 6 nghost 3."""
     rng = np.random.default_rng(seed)
     slots = [(x, y, z) for z in range(4) for y in range(4) for x in range(4)]
     lev1, lev2 = [], []
@@ -124,7 +127,7 @@ def amr_boxes(seed, nbox=6):
         lo = (16 * x, 16 * y, 16 * z)
         lev1.append([*lo, *(v + e - 1 for v in lo)])
         e2 = int(rng.choice([8, 12, 16]))
-        if 4 + e2 <= 2 * e:
-            lo2 = tuple(2 * v + 4 for v in lo)
+        if 2 * margin + e2 <= 2 * e:
+            lo2 = tuple(2 * v + margin for v in lo)
             lev2.append([*lo2, *(v + e2 - 1 for v in lo2)])
     return lev1, lev2

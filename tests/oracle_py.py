@@ -44,6 +44,9 @@ def _load(project: str) -> C.CDLL:
         "orc_index_to_k": (d, [i, i, i]),
         "orc_upload": (i, [vp, i, i, dp]),
         "orc_download": (i, [vp, i, i, dp, i]),
+        "orc_upload_old": (i, [vp, i, i, dp, d]),
+        "orc_fill_patch": (i, [vp, i, d]),
+        "orc_download_fab": (i, [vp, i, i, i, dp]),
         "orc_fill_rhs": (i, [vp, i, d, d, i]),
         "orc_download_rhs": (i, [vp, i, dp]),
         "orc_advance": (i, [vp, i]),
@@ -123,6 +126,23 @@ class Oracle:
         a = np.empty(self.ncells(lev))
         self._ck(self.lib.orc_download(self.h, lev, comp, _dp(a), int(old)))
         return a
+
+    def upload_old(self, lev, comp, data, t):
+        """Valid cells of grid_old[lev] and its time."""
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        assert a.size == self.ncells(lev)
+        self._ck(self.lib.orc_upload_old(self.h, lev, comp, _dp(a), t))
+
+    def fill_patch(self, lev, time):
+        """FillPatch of grid_new[lev] at `time` (ghost cells included)."""
+        self._ck(self.lib.orc_fill_patch(self.h, lev, time))
+
+    def fab(self, lev, box, comp, shape):
+        """Ghosted box `box` of grid_new[lev], indexed [k, j, i]; `shape` is
+        its ghosted extent (nz, ny, nx)."""
+        a = np.empty(int(np.prod(shape)))
+        self._ck(self.lib.orc_download_fab(self.h, lev, box, comp, _dp(a)))
+        return a.reshape(shape)
 
     def fill_rhs(self, lev, time, weight=0.0, add=False):
         self._ck(self.lib.orc_fill_rhs(self.h, lev, time, weight, int(add)))

@@ -1,0 +1,149 @@
+"""Coarse/fine ghost fill on the GPU against the oracle (FillPatchTwoLevels
+and FillIntermediatePatch, level_synchronizer.cpp:92-209, GetLevelData at
+:613-632): the Rhs of a fine level whose stencils reach into interpolated
+ghost cells, and sub-cycled multi-level steps of every integrator, where
+stage states at coarse/fine ghost cells are the coarser level interpolated
+at the stage times.  The oracle's fill is pinned in test_oracle_pins.py
+(exact quadratics, parent values, time windows).
+
+Tolerances: relative L-infinity against max |oracle|; 1e-13 (FP64 Rhs),
+1e-5 (FP32-compensated Rhs), 1e-12 (FP64) / 1e-6 (FP32-compensated) for
+fields after steps.  FP32-compensated momenta on levels 1 and 2 get 2e-6:
+stencil inputs are FP32, and at coarse/fine ghost cells the stage states are
+rebuilt from the interpolated coarse data and rounded to FP32 at every stage
+(a valid cell carries one rounding of u0 through the step), so the Laplacian
+next to the boundary sees independent roundings of 3e-8 times weights / dx^2
+(dx = 1/4 on level 2) at each stage.  Measured with FP32 and bf16 remainders
+alike: 1.65e-6 in the three cells next to a box face, 7.7e-7 inside, fields
+1.1e-7.
+"""
+import numpy as np
+import pytest
+
+import sledgehamr_amd as shm
+from cases import real_scalar, rel_linf
+from oracle_py import Oracle
+
+pytestmark = pytest.mark.gpu
+
+ALLP = [shm.PREC_FP64, shm.PREC_FP32C, shm.PREC_FP32C16]
+PRECS = [(shm.PREC_FP64, 1e-12), (shm.PREC_FP32C16, 1e-6)]
+FINE_MOMENTUM_TOL = {shm.PREC_FP64: 1e-12, shm.PREC_FP32C16: 2e-6}
+# Level-1 boxes of a 32^3 level-1 domain: two touching boxes (ghosts covered
+# by the neighbour and interpolated next to it) and one at z = 0 whose
+# ghosts wrap through the periodic boundary.
+LEV1 = [[8, 8, 8, 15, 15, 15], [16, 8, 8, 23, 15, 15], [4, 20, 0, 11, 27, 7]]
+# Level-2 boxes (64^3 domain), 4 cells inside level-1 boxes 0 and 1 (the
+# second spans both); quadratic interpolation with 2 ghost cells needs 2
+# level-1 cells of margin.
+LEV2 = [[20, 20, 20, 27, 27, 27], [30, 20, 20, 43, 27, 27]]
+
+
+def _smooth(boxes, h, L, phase):
+    """Periodic smooth field at the cell centres of `boxes` (spacing h)."""
+    out = []
+    for b in boxes:
+        k, j, i = np.meshgrid(*(np.arange(b[a], b[a + 3] + 1) for a in (2, 1, 0)),
+                              indexing="ij")
+        x, y, z = ((i + 0.5) * h, (j + 0.5) * h, (k + 0.5) * h)
+        w = 2 * np.pi / L
+        f = (0.3 * np.sin(w * x + phase) * np.cos(2 * w * y) +
+             0.2 * np.cos(w * z - 0.5 * phase) + 0.1 * np.sin(3 * w * (x + y + z)))
+        out.append(f.ravel())
+    return np.concatenate(out)
+
+
+def _setup(inp, prec, levels):
+    sim = shm.Simulation("RealScalar", inp, precision=prec)
+    orc = Oracle("RealScalar", inp)
+    for lev, bx in levels.items():
+        sim.set_boxes(lev, bx)
+        orc.set_boxes(lev, bx)
+    return sim, orc
+
+
+@pytest.mark.parametrize("interp", [0, 2])
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("prec", ALLP)
+def test_fine_rhs_reads_interpolated_ghosts(order, interp, prec):
+    """Random coarse and fine data (coarse/fine ghosts differ strongly from
+    the fine data), so every wrong ghost value changes the Rhs."""
+    inp = real_scalar(N=16, order=order, nghost=order, lam=1.0, L=16.0,
+                      **{"amr.max_refinement_levels": 1,
+                         "amr.interpolation_type": interp})
+    sim, orc = _setup(inp, prec, {1: LEV1})
+    rng = np.random.default_rng(40 + order)
+    for lev in (0, 1):
+        for c in range(2):
+            d = rng.standard_normal(sim.ncells(lev)) * 0.3
+            sim.upload(lev, c, d)
+            orc.upload(lev, c, d)
+    sim.fill_rhs(1, 0.0)
+    orc.fill_rhs(1, 0.0)
+    for c in range(2):
+        err = rel_linf(sim.rhs(1, c), orc.rhs(1, c))
+        assert err < (1e-13 if prec == shm.PREC_FP64 else 1e-5), (c, err)
+
+
+@pytest.mark.parametrize("integrator", [10, 21, 22, 11])
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_subcycled_levels_match_oracle(integrator, prec, tol):
+    """Three levels, two coarse steps: level 1 sub-cycles twice and level 2
+    four times per coarse step, with coarse states interpolated in time at
+    every stage of the finer levels."""
+    inp = real_scalar(N=16, order=2, nghost=2, lam=1.0, L=16.0, cfl=0.25,
+                      integrator=integrator,
+                      **{"amr.max_refinement_levels": 2})
+    sim, orc = _setup(inp, prec, {1: LEV1, 2: LEV2})
+    for lev, bx in ((0, [[0, 0, 0, 15, 15, 15]]), (1, LEV1), (2, LEV2)):
+        h = 16.0 / (16 * 2 ** lev)
+        for c, ph in ((0, 0.3), (1, 1.1)):
+            d = _smooth(bx, h, 16.0, ph) * (1.0 if c == 0 else 0.2)
+            sim.upload(lev, c, d)
+            orc.upload(lev, c, d)
+    sim.evolve(2)
+    orc.evolve(2)
+    for lev in range(3):
+        assert sim.level_info(lev)[1] == orc.time(lev)
+        for c in range(2):
+            err = rel_linf(sim.download(lev, c), orc.download(lev, c))
+            t = FINE_MOMENTUM_TOL[prec] if (c == 1 and lev > 0) else tol
+            assert err < t, (integrator, lev, c, err)
+
+
+def test_ghost_fill_changes_the_fine_solution():
+    """Control: with the coarse level shifted by a constant, a fine level's
+    step changes near its boundary (the ghosts come from the coarse level)
+    and not far inside it."""
+    inp = real_scalar(N=16, order=2, nghost=2, lam=0.0, L=16.0, integrator=21,
+                      **{"amr.max_refinement_levels": 1})
+    box = [[8, 8, 8, 23, 23, 23]]
+    res = []
+    for shift in (0.0, 1.0):
+        sim = shm.Simulation("RealScalar", inp, precision=shm.PREC_FP64)
+        sim.set_boxes(1, box)
+        sim.upload(0, 0, _smooth([[0, 0, 0, 15, 15, 15]], 1.0, 16.0, 0.2) + shift)
+        sim.upload(0, 1, np.zeros(16 ** 3))
+        sim.upload(1, 0, _smooth(box, 0.5, 16.0, 0.2))
+        sim.upload(1, 1, np.zeros(16 ** 3))
+        sim.advance(1, 1)
+        res.append(sim.download(1, 1).reshape(16, 16, 16))
+    d = np.abs(res[1] - res[0])
+    assert d[0].max() > 1e-3 and d[6:10, 6:10, 6:10].max() == 0.0
+
+
+def test_unnested_level_is_refused():
+    inp = real_scalar(N=16, order=1, nghost=2, lam=1.0, L=16.0,
+                      **{"amr.max_refinement_levels": 2})
+    sim = shm.Simulation("RealScalar", inp, precision=shm.PREC_FP64)
+    sim.set_boxes(1, [[8, 8, 8, 15, 15, 15]])
+    sim.set_boxes(2, [[16, 16, 16, 23, 23, 23]])
+    with pytest.raises(shm.ShmError, match="not properly nested"):
+        sim.advance(2, 1)
+
+
+def test_interpolation_types_checked():
+    for t in (1, 4, 7):
+        with pytest.raises(shm.ShmError):
+            shm.Simulation("RealScalar", real_scalar(
+                N=16, **{"amr.interpolation_type": t}))

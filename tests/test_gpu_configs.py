@@ -139,8 +139,8 @@ def test_cfg3_bubbles_gw_spectrum(prec, tol, gtol, stol):
 # ---------------------------------------------------------------------------
 # cfg 2 (reduced): complex scalar with string loops on a static 3-level
 # layout, sub-cycled steps with the shadow level, scheduled truncation-error
-# tagging (te_crit 1e-3).  Ghost cells at coarse/fine boundaries are not
-# interpolated (they keep their zero initial value on both sides).
+# tagging (te_crit 1e-3).  Ghost cells at coarse/fine boundaries take the
+# coarser level interpolated in time and space (quadratic) on both sides.
 # ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("prec,tol", PRECS)

@@ -208,6 +208,138 @@ def test_kreiss_oliger_plane_wave_symbol(order):
 
 
 # ---------------------------------------------------------------------------
+# Coarse/fine ghost fill (FillPatchTwoLevels at level_synchronizer.cpp:92-209,
+# GetLevelData at :613-632).  AMReX's interpolaters are not in the reference
+# tree, so the semantics are defined here and pinned by what fixes them: exact
+# reproduction of quadratics, the parent value for type 0, the time windows.
+# ---------------------------------------------------------------------------
+
+def _cf_oracle(fine_boxes, interp=2, ng=2, N=16, levels=1):
+    inp = real_scalar(N=N, order=1, nghost=ng, lam=0.0, L=float(N),
+                      **{"amr.max_refinement_levels": levels,
+                         "amr.interpolation_type": interp})
+    orc = Oracle("RealScalar", inp)
+    orc.set_boxes(1, fine_boxes)
+    return orc
+
+
+def _box_cells(boxes, f):
+    """f(x, y, z) at the cell centres of each box, box after box, x fastest;
+    cell (i, j, k) of a level with spacing h has centre ((i + 1/2) h, ...)."""
+    out = []
+    for b, h in boxes:
+        k, j, i = np.meshgrid(*(np.arange(b[a], b[a + 3] + 1) for a in (2, 1, 0)),
+                              indexing="ij")
+        out.append(f((i + 0.5) * h, (j + 0.5) * h, (k + 0.5) * h).ravel())
+    return np.concatenate(out)
+
+
+def _quad(a):
+    return lambda x, y, z: (a[0] + a[1] * x + a[2] * y + a[3] * z + a[4] * x * x +
+                            a[5] * y * y + a[6] * z * z + a[7] * x * y + a[8] * x * z +
+                            a[9] * y * z)
+
+
+def _ghosted(orc, boxes, g, comp=0):
+    """Ghosted arrays of the level-1 boxes with their global index grids."""
+    res = []
+    for n, b in enumerate(boxes):
+        shape = tuple(b[a + 3] - b[a] + 1 + 2 * g for a in (2, 1, 0))
+        k, j, i = np.meshgrid(*(np.arange(b[a] - g, b[a + 3] + g + 1) for a in (2, 1, 0)),
+                              indexing="ij")
+        res.append((orc.fab(1, n, comp, shape), i, j, k))
+    return res
+
+
+def _inside(b, i, j, k):
+    return ((i >= b[0]) & (i <= b[3]) & (j >= b[1]) & (j <= b[4]) &
+            (k >= b[2]) & (k <= b[5]))
+
+
+@pytest.mark.parametrize("ng", [2, 3])
+def test_coarse_fine_fill_quadratic_exact(ng):
+    """Two touching level-1 boxes over quadratic coarse states at t = 0 and
+    t = 1, filled at t = 1/4: ghost cells another box covers hold that box's
+    (cubic) data, every other ghost cell 3/4 q_old + 1/4 q_new at its centre."""
+    boxes = [[8, 8, 8, 15, 15, 15], [16, 8, 8, 23, 15, 15]]
+    orc = _cf_oracle(boxes, ng=ng)
+    q_old = _quad([0.3, 0.2, -0.1, 0.05, 0.01, -0.02, 0.015, 0.007, -0.004, 0.003])
+    q_new = _quad([-0.1, 0.05, 0.1, -0.2, -0.012, 0.006, 0.02, -0.005, 0.009, -0.008])
+    cubic = lambda x, y, z: x ** 3 - y * z * z + 0.5
+    coarse = [([0, 0, 0, 15, 15, 15], 1.0)]
+    orc.upload_old(0, 0, _box_cells(coarse, q_old), 0.0)
+    orc.upload(0, 0, _box_cells(coarse, q_new))
+    orc.set_time(0, 1.0)
+    orc.upload(1, 0, _box_cells([(b, 0.5) for b in boxes], cubic))
+    orc.fill_patch(1, 0.25)
+    for n, (a, i, j, k) in enumerate(_ghosted(orc, boxes, ng)):
+        x, y, z = (i + 0.5) * 0.5, (j + 0.5) * 0.5, (k + 0.5) * 0.5
+        other = _inside(boxes[1 - n], i, j, k)
+        want = np.where(other, cubic(x, y, z),
+                        0.75 * q_old(x, y, z) + 0.25 * q_new(x, y, z))
+        ghost = ~_inside(boxes[n], i, j, k)
+        assert other.any() and (ghost & ~other).any()
+        assert np.max(np.abs(a - want)[ghost]) < 1e-12, n
+        # Valid cells are untouched.
+        assert np.array_equal(a[~ghost], cubic(x, y, z)[~ghost])
+
+
+def test_coarse_fine_fill_piecewise_constant_is_not_quadratic():
+    """Negative control of the quadratic check: type 0 misses the slope."""
+    boxes = [[8, 8, 8, 15, 15, 15]]
+    orc = _cf_oracle(boxes, interp=0)
+    q = _quad([0.3, 0.2, -0.1, 0.05, 0.01, -0.02, 0.015, 0.007, -0.004, 0.003])
+    orc.upload(0, 0, _box_cells([([0, 0, 0, 15, 15, 15], 1.0)], q))
+    orc.fill_patch(1, orc.time(0))
+    a, i, j, k = _ghosted(orc, boxes, 2)[0]
+    ghost = ~_inside(boxes[0], i, j, k)
+    err = np.abs(a - q((i + 0.5) * 0.5, (j + 0.5) * 0.5, (k + 0.5) * 0.5))[ghost]
+    assert err.max() > 1e-2
+
+
+def test_coarse_fine_fill_piecewise_constant_wraps():
+    """Type 0 gives each ghost cell its parent's value, through the periodic
+    wrap of the level-0 domain (the box touches z = 0)."""
+    boxes = [[8, 8, 0, 15, 15, 7]]
+    orc = _cf_oracle(boxes, interp=0, ng=3)
+    c = np.random.default_rng(5).standard_normal(16 ** 3)
+    orc.upload(0, 0, c)
+    orc.fill_patch(1, orc.time(0))
+    a, i, j, k = _ghosted(orc, boxes, 3)[0]
+    ghost = ~_inside(boxes[0], i, j, k)
+    par = c.reshape(16, 16, 16)[(k >> 1) % 16, (j >> 1) % 16, (i >> 1) % 16]
+    assert np.array_equal(a[ghost], par[ghost])
+    assert (k[ghost] < 0).any()
+
+
+@pytest.mark.parametrize("time,want", [(0.0005, 1.0), (0.9995, 3.0), (0.5, 2.0),
+                                       (0.25, 1.5)])
+def test_coarse_fine_fill_time_windows(time, want):
+    """GetLevelData: within 1e-3 |t_new - t_old| of a state's time that state
+    alone, else linear in time between them."""
+    boxes = [[8, 8, 8, 15, 15, 15]]
+    orc = _cf_oracle(boxes)
+    orc.upload_old(0, 0, np.full(16 ** 3, 1.0), 0.0)
+    orc.upload(0, 0, np.full(16 ** 3, 3.0))
+    orc.set_time(0, 1.0)
+    orc.fill_patch(1, time)
+    a, i, j, k = _ghosted(orc, boxes, 2)[0]
+    assert np.all(a[~_inside(boxes[0], i, j, k)] == want)
+
+
+def test_coarse_fine_fill_requires_nesting():
+    """A level-2 box at the corner of its level-1 box needs level-1 cells
+    outside every level-1 box for its ghost cells."""
+    inp = real_scalar(N=16, order=1, nghost=2, lam=0.0, L=16.0,
+                      **{"amr.max_refinement_levels": 2})
+    orc = Oracle("RealScalar", inp)
+    orc.set_boxes(1, [[8, 8, 8, 15, 15, 15]])
+    orc.set_boxes(2, [[16, 16, 16, 23, 23, 23]])
+    with pytest.raises(RuntimeError, match="not properly nested"):
+        orc.fill_patch(2, orc.time(2))
+
+
+# ---------------------------------------------------------------------------
 # Integrators (lsssprk3.cpp:14-31, rkn.cpp:25-205, leapfrog.cpp:13-57)
 # ---------------------------------------------------------------------------
 
