@@ -132,6 +132,26 @@ int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
                     int projection_type, double *bins, double *kvals,
                     int *nout);
 
+/* Output cell functions the project declares (SLEDGEHAMR_OUTPUT_FUNCTIONS;
+ * the device form of io_module->spectra / io_module->projections).  *count
+ * receives their number; shm_output_function_name copies the name of id
+ * into buf (len bytes, NUL-terminated). */
+int shm_output_functions(shm_ctx *ctx, int *count);
+int shm_output_function_name(shm_ctx *ctx, int id, char *buf, int len);
+
+/* Spectrum::Compute (output_types/spectrum.cpp:12-113) of output function fn
+ * on level 0: bins[b] = sum over modes with |k|^2 = k_sq[b] of
+ * mult t |F(k)|^2 / (dk N^6), dk = 2 pi / L.  bins / k_sq must hold *n
+ * values (call with bins = NULL to query *n). */
+int shm_spectrum(shm_ctx *ctx, int fn, double *bins, long long *k_sq, int *n);
+
+/* Projection::Compute (output_types/projection.cpp:11-118) of output function
+ * fn along z onto the finest level up to max_level (< 0: all levels).
+ * mode 0: sum of ratio * value, 1: maximum (from 0).  data / counts hold
+ * (*dimN)^2 values, x index slow (call with data = NULL to query *dimN). */
+int shm_projection(shm_ctx *ctx, int fn, int mode, int max_level, double *data,
+                   int *counts, int *dimN);
+
 int shm_sync(shm_ctx *ctx);
 
 /* Achievable HBM bandwidth: plain float4 read, write and copy kernels over

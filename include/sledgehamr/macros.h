@@ -210,6 +210,50 @@
     SLEDGEHAMR_TRUNCATION_ERROR_TAG_CPU                                        \
     SLEDGEHAMR_TRUNCATION_ERROR_TAG_GPU
 
+/** Output cell functions of a project, for spectra and projections (what
+ *  io_module->spectra / io_module->projections register in the reference,
+ *  output_types/spectrum.cpp:12-113, output_types/projection.cpp:11-118).
+ *  Each is double f(state, i, j, k, lev, time, dt, dx, params) with params
+ *  as const double * (SetParamsSpectra / SetParamsProjections), declared
+ *  AMREX_GPU_HOST_DEVICE so that it runs on the device.  A function's id is
+ *  its position in the list.  Place after the functions, before the project
+ *  class. */
+#define SHM_OUT_NAME(f) #f,
+#define SHM_OUT_CASE(f) \
+    if (id == c++) return f(s, i, j, k, lev, t, dt, dx, p);
+#define SLEDGEHAMR_OUTPUT_FUNCTIONS(...)                                       \
+    struct ShmOutputFns {                                                      \
+        static constexpr int count = SHM_PP_NARGS(__VA_ARGS__);                \
+        static const char *name(int id) {                                      \
+            static const char *const n[] = {                                   \
+                SHM_PP_FOR_EACH(SHM_OUT_NAME, __VA_ARGS__)};                   \
+            return id >= 0 && id < count ? n[id] : nullptr;                    \
+        }                                                                      \
+        AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE static double                 \
+        eval(int id, const amrex::Array4<const double> &s, int i, int j,      \
+             int k,                                                            \
+             int lev, double t, double dt, double dx, const double *p) {       \
+            int c = 0;                                                         \
+            SHM_PP_FOR_EACH(SHM_OUT_CASE, __VA_ARGS__)                         \
+            return 0.;                                                         \
+        }                                                                      \
+    };                                                                         \
+    inline ShmOutputFns shm_select_output(int) { return {}; }
+
+/** Projects without SLEDGEHAMR_OUTPUT_FUNCTIONS have no output functions. */
+namespace shm {
+struct NoOutputFns {
+    static constexpr int count = 0;
+    static const char *name(int) { return nullptr; }
+    AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE static double
+    eval(int, const amrex::Array4<const double> &, int, int, int, int, double,
+         double, double, const double *) {
+        return 0.;
+    }
+};
+}  // namespace shm
+inline shm::NoOutputFns shm_select_output(...) { return {}; }
+
 /** Binds the project's device functions to the MI355X kernel table.
  *  ShmOps is the compile-time description of the project that the fused
  *  stage, tagging and probe kernels are instantiated with. */
@@ -218,6 +262,7 @@
         static constexpr int kNScalars = Scalar::NScalars;                     \
         static constexpr int kNFields = _Scalar::NScalarFields;                \
         static constexpr int kNGw = Gw::NGwScalars;                            \
+        using Output = decltype(shm_select_output(0));                         \
         AMREX_GPU_DEVICE AMREX_FORCE_INLINE static void                        \
         rhs(const amrex::Array4<double> &r,                                    \
             const amrex::Array4<const double> &s, int i, int j, int k,         \

@@ -1024,6 +1024,110 @@ class Sim {
     }
 
     // -----------------------------------------------------------------------
+    // Output functions (the project's SLEDGEHAMR_OUTPUT_FUNCTIONS).  Ghost
+    // cells are filled at the level's time first (FillPatch), as they are
+    // at output time after a step.
+    // -----------------------------------------------------------------------
+
+    /** Spectrum::Compute (output_types/spectrum.cpp:12-113) on level 0:
+     *  spec[b] = sum over modes of the x-halved R2C transform with
+     *  |k|^2 = ks[b] of mult pre |F|^2, mult = 1 on the x = 0 and x = N/2
+     *  planes else 2, pre = t / (dk N^6), dk = 2 pi / L (:49-51); ks the
+     *  sorted unique |k|^2 (lower_bound, :80-84). */
+    void spectrum(int fn, std::vector<double> &spec, std::vector<long long> &ks) {
+        using Out = typename Ops::Output;
+        if (fn < 0 || fn >= Out::count) throw std::runtime_error("output function id");
+        MultiFab &S = grid_new[0];
+        const int N = dimN[0];
+        const double time = S.t;
+        fill_patch(S, 0, time);
+        std::vector<double> p;
+        prj.SetParamsSpectra(p, time);
+        p.resize(std::max<size_t>(p.size(), 64), 0.);
+        std::vector<double> dense((size_t)N * N * N, 0.);
+        for (Fab &f : S.fabs) {
+            const amrex::Array4<const double> sa = f.carr();
+            for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
+                for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
+                    for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i)
+                        dense[(size_t)(i + (long long)N * (j + (long long)N * k))] =
+                            Out::eval(fn, sa, i, j, k, 0, time, dt[0], dx[0], p.data());
+        }
+        std::vector<std::complex<double>> F;
+        r2c(dense, N, F);
+        ks = unique_k2(N);
+        spec.assign(ks.size(), 0.);
+        const double pre = std::pow(1. / N, 6) * time / (2. * M_PI / cfg.L);
+        const int NH = N / 2 + 1;
+        for (int c = 0; c < N; ++c)
+            for (int b = 0; b < N; ++b)
+                for (int a = 0; a < NH; ++a) {
+                    const double multpl = (a == 0 || a == N / 2) ? 1. : 2.;
+                    const int li = a >= N / 2 ? a - N : a;
+                    const int lj = b >= N / 2 ? b - N : b;
+                    const int lk = c >= N / 2 ? c - N : c;
+                    const long long sq = (long long)li * li + lj * lj + lk * lk;
+                    const size_t index =
+                        std::lower_bound(ks.begin(), ks.end(), sq) - ks.begin();
+                    spec[index] += multpl * pre *
+                                   std::norm(F[(size_t)(((long long)c * N + b) * NH + a)]);
+                }
+    }
+
+    /** Projection::Compute (output_types/projection.cpp:11-118) along z onto
+     *  the finest level up to max_level: every cell of level lev that the
+     *  next level does not hold (no finer box contains (2i, 2j, 2k)) adds
+     *  ratio * value (mode 0) or maximises (mode 1, from 0) the ratio^2
+     *  pixels (i ratio + a) D + (j ratio + b), counting each contribution. */
+    void projection(int fn, int mode, int max_level, std::vector<double> &data,
+                    std::vector<int> &n, int &D) {
+        using Out = typename Ops::Output;
+        if (fn < 0 || fn >= Out::count) throw std::runtime_error("output function id");
+        const int ml = std::min(max_level < 0 ? INT32_MAX : max_level, finest());
+        D = dimN[ml];
+        data.assign((size_t)D * D, 0.);
+        n.assign((size_t)D * D, 0);
+        std::vector<double> p;
+        prj.SetParamsProjections(p, grid_new[0].t);
+        p.resize(std::max<size_t>(p.size(), 64), 0.);
+        for (int lev = 0; lev <= ml; ++lev) {
+            MultiFab &S = grid_new[lev];
+            const double time = S.t;
+            fill_patch(S, lev, time);
+            const int ratio = D / dimN[lev];
+            for (Fab &f : S.fabs) {
+                const amrex::Array4<const double> sa = f.carr();
+                for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
+                    for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
+                        for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i) {
+                            bool covered = false;
+                            if (lev < ml)
+                                for (const Fab &g : grid_new[lev + 1].fabs) {
+                                    const Box &B = g.box;
+                                    if (2 * i >= B.lo[0] && 2 * i <= B.hi[0] &&
+                                        2 * j >= B.lo[1] && 2 * j <= B.hi[1] &&
+                                        2 * k >= B.lo[2] && 2 * k <= B.hi[2])
+                                        covered = true;
+                                }
+                            if (covered) continue;
+                            const double val =
+                                Out::eval(fn, sa, i, j, k, lev, time, dt[lev], dx[lev], p.data());
+                            for (int ia = 0; ia < ratio; ++ia)
+                                for (int ja = 0; ja < ratio; ++ja) {
+                                    const size_t ind =
+                                        (size_t)(i * ratio + ia) * D + (j * ratio + ja);
+                                    if (mode == 0)
+                                        data[ind] += ratio * val;
+                                    else
+                                        data[ind] = std::max(data[ind], val);
+                                    n[ind] += 1;
+                                }
+                        }
+            }
+        }
+    }
+
+    // -----------------------------------------------------------------------
     // Sub-cycled stepping on a static layout: TimeStepper::Advance with
     // ScheduleRegrid, SynchronizeLevels and DoRegridIfScheduled
     // (time_stepper.cpp:90-171, 221-311, 347-397) and RegridScheduler

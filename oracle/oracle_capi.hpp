@@ -124,6 +124,31 @@
             std::memcpy(d, f.d.data() + comp * f.nst, sizeof(double) * f.nst); \
         });                                                                    \
     }                                                                          \
+    ORC_EXPORT int orc_spectrum(void *h, int fn, double *bins, long long *ks,  \
+                                int *n) {                                      \
+        return orc_guard([&] {                                                 \
+            std::vector<double> b;                                             \
+            std::vector<long long> k;                                          \
+            static_cast<OrcSim *>(h)->spectrum(fn, b, k);                      \
+            if (bins) {                                                        \
+                std::memcpy(bins, b.data(), sizeof(double) * b.size());        \
+                std::memcpy(ks, k.data(), sizeof(long long) * k.size());       \
+            }                                                                  \
+            *n = (int)b.size();                                                \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_projection(void *h, int fn, int mode, int ml, double *d,\
+                                  int *c, int *D) {                            \
+        return orc_guard([&] {                                                 \
+            std::vector<double> dv;                                            \
+            std::vector<int> cv;                                               \
+            static_cast<OrcSim *>(h)->projection(fn, mode, ml, dv, cv, *D);    \
+            if (d) {                                                           \
+                std::memcpy(d, dv.data(), sizeof(double) * dv.size());         \
+                std::memcpy(c, cv.data(), sizeof(int) * cv.size());            \
+            }                                                                  \
+        });                                                                    \
+    }                                                                          \
     ORC_EXPORT int orc_download(void *h, int lev, int comp, double *d,         \
                                 int old) {                                     \
         return orc_guard([&] {                                                 \

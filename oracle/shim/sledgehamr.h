@@ -110,6 +110,8 @@ class Sledgehamr {
                                                const int) {}
     virtual void SetParamsTruncationModifier(std::vector<double> &, const double,
                                              const int) {}
+    virtual void SetParamsSpectra(std::vector<double> &, const double) {}
+    virtual void SetParamsProjections(std::vector<double> &, const double) {}
     virtual bool CreateLevelIf(const int, const double) { return true; }
     virtual void BeforeTimestep(const double) {}
     std::vector<ScalarField *> scalar_fields;
@@ -268,12 +270,41 @@ inline double KreissOligerDissipation<3>(const amrex::Array4<double const> &s, i
 
 #define SLEDGEHAMR_FINISH_SETUP
 
+/** Output cell functions for spectra and projections (the product header's
+ *  SLEDGEHAMR_OUTPUT_FUNCTIONS); a function's id is its list position. */
+#define ORC_OUT_CASE(f) \
+    if (id == c++) return f(s, i, j, k, lev, t, dt, dx, p);
+#define SLEDGEHAMR_OUTPUT_FUNCTIONS(...)                                       \
+    struct OrcOutputFns {                                                      \
+        static constexpr int count = ORC_NARGS(__VA_ARGS__);                   \
+        static double eval(int id, const amrex::Array4<const double> &s,       \
+                           int i, int j, int k, int lev, double t, double dt,  \
+                           double dx, const double *p) {                       \
+            int c = 0;                                                         \
+            ORC_FE(ORC_OUT_CASE, __VA_ARGS__)                                  \
+            return 0.;                                                         \
+        }                                                                      \
+    };                                                                         \
+    inline OrcOutputFns orc_select_output(int) { return {}; }
+
+namespace orc {
+struct NoOutputFns {
+    static constexpr int count = 0;
+    static double eval(int, const amrex::Array4<const double> &, int, int, int,
+                       int, double, double, double, const double *) {
+        return 0.;
+    }
+};
+}  // namespace orc
+inline orc::NoOutputFns orc_select_output(...) { return {}; }
+
 /** The oracle reaches the project's cell functions through OrcOps. */
 #define SLEDGEHAMR_INITIALIZE_PROJECT(prj)                                     \
     struct OrcOps {                                                            \
         static constexpr int kNScalars = Scalar::NScalars;                     \
         static constexpr int kNFields = _Scalar::NScalarFields;                \
         static constexpr int kNGw = Gw::NGwScalars;                            \
+        using Output = decltype(orc_select_output(0));                         \
         static void rhs(const amrex::Array4<double> &r,                        \
                         const amrex::Array4<const double> &s, int i, int j,    \
                         int k, int lev, double t, double dt, double dx,        \

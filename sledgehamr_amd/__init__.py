@@ -29,6 +29,7 @@ EXPORTS = (
     "shm_level_info", "shm_set_level_time", "shm_upload", "shm_download",
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
+    "shm_output_functions", "shm_output_function_name", "shm_spectrum", "shm_projection",
     "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
     "shm_get_launch", "shm_set_graphs", "shm_set_timing", "shm_get_timing",
 )
@@ -99,6 +100,10 @@ def load_library(project: str) -> C.CDLL:
         "shm_level_tag_counts": (i, [vp, i, llp]),
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
+        "shm_output_functions": (i, [vp, ip]),
+        "shm_output_function_name": (i, [vp, i, C.c_char_p, i]),
+        "shm_spectrum": (i, [vp, i, dp, llp, ip]),
+        "shm_projection": (i, [vp, i, i, i, dp, ip, ip]),
         "shm_sync": (i, [vp]),
         "shm_stream_probe": (i, [vp, C.c_ulonglong, i, dp, dp, dp]),
         "shm_autotune": (i, [vp, i]),
@@ -277,6 +282,40 @@ class Simulation:
                                              projection_type, _dptr(bins),
                                              _dptr(k), C.byref(n)))
         return bins, k
+
+    def output_functions(self) -> list:
+        """Names of the project's output functions, in id order."""
+        n = C.c_int(0)
+        self._check(self.lib.shm_output_functions(self._ctx, C.byref(n)))
+        names = []
+        for fid in range(n.value):
+            buf = C.create_string_buffer(256)
+            self._check(self.lib.shm_output_function_name(self._ctx, fid, buf, 256))
+            names.append(buf.value.decode())
+        return names
+
+    def spectrum(self, fn: int):
+        """Spectrum of output function fn on level 0: (bins, k_sq)."""
+        n = C.c_int(0)
+        self._check(self.lib.shm_spectrum(self._ctx, fn, None, None, C.byref(n)))
+        bins = np.zeros(n.value)
+        ks = np.zeros(n.value, dtype=np.int64)
+        self._check(self.lib.shm_spectrum(
+            self._ctx, fn, _dptr(bins), ks.ctypes.data_as(C.POINTER(C.c_longlong)),
+            C.byref(n)))
+        return bins, ks
+
+    def projection(self, fn: int, mode: int = 0, max_level: int = -1):
+        """Projection along z: (data, counts), both (dimN, dimN), x index first."""
+        D = C.c_int(0)
+        self._check(self.lib.shm_projection(self._ctx, fn, mode, max_level, None,
+                                            None, C.byref(D)))
+        data = np.zeros(D.value * D.value)
+        cnt = np.zeros(D.value * D.value, dtype=np.int32)
+        self._check(self.lib.shm_projection(
+            self._ctx, fn, mode, max_level, _dptr(data),
+            cnt.ctypes.data_as(C.POINTER(C.c_int)), C.byref(D)))
+        return data.reshape(D.value, D.value), cnt.reshape(D.value, D.value)
 
     def sync(self) -> None:
         self._check(self.lib.shm_sync(self._ctx))

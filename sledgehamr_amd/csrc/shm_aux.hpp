@@ -568,6 +568,122 @@ __global__ void __launch_bounds__(256) gw_bin_kernel(const BinArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Output functions: spectra (output_types/spectrum.cpp:12-113) and
+// projections (output_types/projection.cpp:11-118).
+// ---------------------------------------------------------------------------
+
+/** max with a CAS loop (values of either sign). */
+__device__ AMREX_FORCE_INLINE void atomic_max_double(double *p, double v) {
+    unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
+    unsigned long long old = *q;
+    while (__longlong_as_double((long long)old) < v) {
+        const unsigned long long prev =
+            atomicCAS(q, old, (unsigned long long)__double_as_longlong(v));
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+/** One thread per valid cell of a level: the output function on a full-
+ *  precision view of the state.  Projection: a cell contributes unless the
+ *  finer level holds fine cell (2i, 2j, 2k); it adds ratio * value to (or
+ *  maximises) the ratio x ratio pixels (i ratio + a) D + (j ratio + b). */
+template <class Ops, Prec P>
+__global__ void __launch_bounds__(256) output_kernel(const OutArgs a) {
+    using Out = typename Ops::Output;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < a.total; t += gstride) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.vprefix[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BoxDesc &b = a.boxes[lo];
+        long long r = t - a.vprefix[lo];
+        const int x = (int)(r % b.n[0]);
+        r /= b.n[0];
+        const int y = (int)(r % b.n[1]);
+        const int z = (int)(r / b.n[1]);
+        const int i = b.lo[0] + x, j = b.lo[1] + y, k = b.lo[2] + z;
+        if (a.target == 1) {
+            bool covered = false;
+            for (int q = a.cov_start[lo]; q < a.cov_start[lo + 1] && !covered; ++q) {
+                const int *f = a.cov + 6 * q;
+                covered = 2 * i >= f[0] && 2 * i <= f[3] && 2 * j >= f[1] &&
+                          2 * j <= f[4] && 2 * k >= f[2] && 2 * k <= f[5];
+            }
+            if (covered) continue;
+        }
+        const amrex::Array4<const double> v =
+            plain_view<P>(a.arr, a.arrlo, b, a.cstride, a.ncomp);
+        const double val =
+            Out::eval(a.fn, v, i, j, k, a.lev, a.time, a.dt, a.dx, a.params);
+        if (a.target == 0) {
+            a.dense[(long long)i + (long long)a.N * (j + (long long)a.N * k)] = val;
+        } else {
+            for (int ia = 0; ia < a.ratio; ++ia)
+                for (int ja = 0; ja < a.ratio; ++ja) {
+                    const long long ind =
+                        (long long)(i * a.ratio + ia) * a.D + (j * a.ratio + ja);
+                    if (a.mode == 0)
+                        atomicAdd(&a.proj[ind], a.ratio * val);
+                    else
+                        atomic_max_double(&a.proj[ind], val);
+                    atomicAdd(&a.nproj[ind], 1);
+                }
+        }
+    }
+}
+
+struct SpecBinArgs {
+    const double2 *F;      // R2C output [z][y][x], x halved (N/2 + 1)
+    int N;
+    const int *sq_to_bin;  // unique |k|^2 index, size 3 (N/2)^2 + 1
+    int nbins;
+    double *bins;
+};
+
+/** sum over modes of mult |F|^2 per unique |k|^2, mult = 1 on the x = 0 and
+ *  x = N/2 planes of the halved layout, else 2 (spectrum.cpp:73-90). */
+template <int LDSBINS>
+__global__ void __launch_bounds__(256) scalar_bin_kernel(const SpecBinArgs a) {
+    __shared__ double sb[LDSBINS];
+    const bool use_lds = a.nbins <= LDSBINS;
+    if (use_lds)
+        for (int i = threadIdx.x; i < a.nbins; i += blockDim.x) sb[i] = 0.;
+    __syncthreads();
+    const int N = a.N, NH = N / 2 + 1;
+    const long long total = (long long)NH * N * N;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         t < total; t += gstride) {
+        const int ax = (int)(t % NH);
+        const int by = (int)((t / NH) % N);
+        const int cz = (int)(t / ((long long)NH * N));
+        const int li = ax >= N / 2 ? ax - N : ax;
+        const int lj = by >= N / 2 ? by - N : by;
+        const int lk = cz >= N / 2 ? cz - N : cz;
+        const int bin = a.sq_to_bin[li * li + lj * lj + lk * lk];
+        if (bin < 0 || bin >= a.nbins) continue;
+        const double2 f = a.F[t];
+        const double val = ((ax == 0 || ax == N / 2) ? 1. : 2.) * (f.x * f.x + f.y * f.y);
+        if (use_lds)
+            atomicAdd(&sb[bin], val);
+        else
+            atomicAdd(&a.bins[bin], val);
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < a.nbins; i += blockDim.x)
+            if (sb[i] != 0.) atomicAdd(&a.bins[i], sb[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Seeded initial states (identical formulas in the oracle).
 // ---------------------------------------------------------------------------
 

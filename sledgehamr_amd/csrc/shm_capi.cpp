@@ -281,6 +281,64 @@ __attribute__((visibility("default"))) int shm_gw_spectrum(
     });
 }
 
+__attribute__((visibility("default"))) int shm_output_functions(shm_ctx *ctx,
+                                                               int *count) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!count) throw shm::Error(SHM_ERR_ARG, "null count");
+        *count = ctx->rt->output_count();
+    });
+}
+
+__attribute__((visibility("default"))) int shm_output_function_name(
+    shm_ctx *ctx, int id, char *buf, int len) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        const char *n = ctx->rt->output_name(id);
+        if (!n) throw shm::Error(SHM_ERR_ARG, "output function id out of range");
+        if (!buf || len <= (int)std::strlen(n))
+            throw shm::Error(SHM_ERR_ARG, "name buffer too small");
+        std::strcpy(buf, n);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_spectrum(shm_ctx *ctx, int fn,
+                                                       double *bins,
+                                                       long long *k_sq, int *n) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!n) throw shm::Error(SHM_ERR_ARG, "null n");
+        std::vector<double> b;
+        std::vector<long long> k;
+        ctx->rt->spectrum(fn, b, k);
+        if (bins) {
+            if (*n < (int)b.size()) throw shm::Error(SHM_ERR_ARG, "output arrays too small");
+            std::memcpy(bins, b.data(), b.size() * sizeof(double));
+            if (k_sq) std::memcpy(k_sq, k.data(), k.size() * sizeof(long long));
+        }
+        *n = (int)b.size();
+    });
+}
+
+__attribute__((visibility("default"))) int shm_projection(
+    shm_ctx *ctx, int fn, int mode, int max_level, double *data, int *counts,
+    int *dimN) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!dimN) throw shm::Error(SHM_ERR_ARG, "null dimN");
+        std::vector<double> d;
+        std::vector<int> c;
+        int D = 0;
+        ctx->rt->projection(fn, mode, max_level, d, c, D);
+        if (data) {
+            if (*dimN < D) throw shm::Error(SHM_ERR_ARG, "output arrays too small");
+            std::memcpy(data, d.data(), d.size() * sizeof(double));
+            if (counts) std::memcpy(counts, c.data(), c.size() * sizeof(int));
+        }
+        *dimN = D;
+    });
+}
+
 __attribute__((visibility("default"))) int shm_sync(shm_ctx *ctx) {
     return guard(ctx, [&] {
         SHM_NEED_RT;

@@ -81,4 +81,29 @@ struct CfArgs {
     double cf[5];              // kCfForce: coefficients of F_0 .. F_m
 };
 
+/** Evaluation of a project output function over the valid cells of a level
+ *  (spectra: into a dense N^3 array; projections: summed or maximised along
+ *  z onto the D x D grid of the finest projected level). */
+struct OutArgs {
+    const BoxDesc *boxes;
+    const long long *vprefix;  // valid cells per box, prefix (nbox + 1)
+    int nbox;
+    long long total;
+    long long cstride;
+    int ncomp;
+    const void *arr, *arrlo;   // state (with its remainder)
+    int fn;                    // output function id
+    const double *params;
+    int lev;
+    double time, dt, dx;
+    int target;                // 0 dense, 1 projection
+    double *dense;             // target 0: N^3, x fastest
+    int N;
+    const int *cov_start;      // target 1, per box: range of cov
+    const int *cov;            // finer-level boxes, lo[3] hi[3] each
+    int ratio, D, mode;        // mode 0 sum of ratio * value, 1 maximum
+    double *proj;              // D * D, x index slow
+    int *nproj;
+};
+
 }  // namespace shm

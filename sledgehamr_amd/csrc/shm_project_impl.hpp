@@ -215,6 +215,12 @@ hipError_t launch_te_tag(const TeTagArgs &a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <class Ops, Prec P>
+hipError_t launch_output(const OutArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((output_kernel<Ops, P>), dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <class Ops, bool GW>
 hipError_t launch_probe(ProbeOut *out, const double *p, const double *pg,
                         double t, double dt, double dx, hipStream_t s) {
@@ -238,6 +244,14 @@ void SHM_PP_CAT(SHM_PP_CAT(fill_table_p, SHM_TU_PREC),
 #else
 namespace shm {
 void fill_table_aux(KernelTable &kt) {
+    using Out = typename ShmProjectOps::Output;
+    kt.nout = Out::count;
+    kt.out_name = &Out::name;
+    if constexpr (Out::count > 0) {
+        kt.output[0] = &launch_output<ShmProjectOps, Prec::F64>;
+        kt.output[1] = &launch_output<ShmProjectOps, Prec::F32C>;
+        kt.output[2] = &launch_output<ShmProjectOps, Prec::F32C16>;
+    }
     if constexpr (SHM_GW_MODES & 1) {
         kt.te_tag[0][0] = &launch_te_tag<ShmProjectOps, Prec::F64, false>;
         kt.te_tag[1][0] = &launch_te_tag<ShmProjectOps, Prec::F32C, false>;

@@ -856,7 +856,8 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
 }
 
 void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps,
-                   bool full) {
+                   bool full, int elem_bytes) {
+    const int eb = elem_bytes > 0 ? elem_bytes : (int)esize();
     const long long total = full ? ls.L.halo_full_total : ls.L.halo_total;
     if (total == 0 || comps.empty()) return;
     HaloArgs h;
@@ -872,11 +873,14 @@ void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps,
         const int grid = (int)std::min<long long>((h.total + 255) / 256, 8LL * ncu_);
         EventPair *ep = timing_on ? timed_pair(1) : nullptr;
         if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
-        if (prec_ == Prec::F64)
+        if (eb == 8)
             hipLaunchKernelGGL(halo_kernel<double>, dim3(grid), dim3(256), 0,
                                stream_, h);
-        else
+        else if (eb == 4)
             hipLaunchKernelGGL(halo_kernel<float>, dim3(grid), dim3(256), 0,
+                               stream_, h);
+        else
+            hipLaunchKernelGGL(halo_kernel<uint16_t>, dim3(grid), dim3(256), 0,
                                stream_, h);
         SHM_HIP(hipGetLastError());
         if (ep) SHM_HIP(hipEventRecord(ep->b, stream_));
@@ -2133,6 +2137,188 @@ void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
     SHM_HIP(hipStreamSynchronize(stream_));
     const double n6 = std::pow((double)N, 6);
     for (double &v : bins) v /= n6;
+}
+
+// ---------------------------------------------------------------------------
+// Output functions: spectra (output_types/spectrum.cpp:12-113) and
+// projections (output_types/projection.cpp:11-118).
+// ---------------------------------------------------------------------------
+
+void Runtime::check_output_fn(int fn) const {
+    if (kt_.nout == 0)
+        throw Error(4, "the project declares no output functions "
+                       "(SLEDGEHAMR_OUTPUT_FUNCTIONS)");
+    if (fn < 0 || fn >= kt_.nout) throw Error(1, "output function id out of range");
+    if (independent_)
+        throw Error(4, "spectra and projections need one periodic domain, not "
+                       "independent boxes");
+}
+
+void Runtime::fill_output_ghosts(int lev) {
+    // FillPatch at the level's time, with every ghost cell written (boxes
+    // spanning the domain included, whose stage kernels wrap instead) and
+    // the remainders copied too.
+    LevelState &ls = level(lev);
+    void *lo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
+    if (has_cf(lev)) cf_fill_state(lev, ls.t, ls.U[ls.cur].get(), lo);
+    halo(ls, ls.U[ls.cur].get(), all_comps(), true);
+    if (comp()) halo(ls, lo, all_comps(), true, (int)losize());
+}
+
+void Runtime::spectrum(int fn, std::vector<double> &bins, std::vector<long long> &k_sq) {
+    check_output_fn(fn);
+    LevelState &l0 = level(0);
+    const int N = l0.L.N, NH = N / 2 + 1;
+    fill_output_ghosts(0);
+    std::vector<double> p;
+    sim_->SetParamsSpectra(p, l0.t);
+    p.resize(std::max<size_t>(p.size(), 1), 0.);
+    DevMem dp, dense, F, dsq, dbins;
+    upload_vec(dp, p);
+    dense.alloc(sizeof(double) * (size_t)N * N * N);
+    SHM_HIP(hipMemsetAsync(dense.get(), 0, dense.bytes(), stream_));
+    OutArgs a{};
+    a.boxes = l0.L.dbox.as<BoxDesc>();
+    a.vprefix = l0.L.dvprefix.as<long long>();
+    a.nbox = (int)l0.L.boxes.size();
+    a.total = l0.L.nvalid();
+    a.cstride = l0.L.cstride;
+    a.ncomp = ncomp_;
+    a.arr = l0.U[l0.cur].get();
+    a.arrlo = comp() ? l0.Ulo[l0.cur].get() : nullptr;
+    a.fn = fn;
+    a.params = dp.as<double>();
+    a.lev = 0;
+    a.time = l0.t;
+    a.dt = sim_->dt[0];
+    a.dx = sim_->dx[0];
+    a.target = 0;
+    a.dense = dense.as<double>();
+    a.N = N;
+    const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
+    SHM_HIP(kt_.output[(int)prec_](a, grid, stream_));
+    F.alloc(sizeof(double2) * (size_t)NH * N * N);
+    hipfftHandle plan;
+    if (hipfftPlan3d(&plan, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS)
+        throw Error(2, "hipfftPlan3d failed");
+    hipfftSetStream(plan, stream_);
+    const hipfftResult fr =
+        hipfftExecD2Z(plan, dense.as<hipfftDoubleReal>(), F.as<hipfftDoubleComplex>());
+    hipfftDestroy(plan);
+    if (fr != HIPFFT_SUCCESS) throw Error(2, "hipfftExecD2Z failed");
+    // Unique |k|^2 over the signed frequencies (the k-table of ReadSpectrumKs,
+    // sledgehamr.cpp:554-601).
+    std::set<long long> vals;
+    {
+        std::set<long long> u;
+        for (int q = 0; q < N; ++q) {
+            const long long f = q >= N / 2 ? q - N : q;
+            u.insert(f * f);
+        }
+        for (long long x : u)
+            for (long long y : u)
+                for (long long z : u) vals.insert(x + y + z);
+    }
+    const long long maxsq = 3LL * (N / 2) * (N / 2);
+    std::vector<int> sq2bin(maxsq + 1, -1);
+    k_sq.assign(vals.begin(), vals.end());
+    for (size_t b = 0; b < k_sq.size(); ++b)
+        if (k_sq[b] <= maxsq) sq2bin[k_sq[b]] = (int)b;
+    upload_vec(dsq, sq2bin);
+    const int nbins = (int)k_sq.size();
+    dbins.alloc(sizeof(double) * nbins);
+    SHM_HIP(hipMemsetAsync(dbins.get(), 0, dbins.bytes(), stream_));
+    SpecBinArgs sb{};
+    sb.F = F.as<double2>();
+    sb.N = N;
+    sb.sq_to_bin = dsq.as<int>();
+    sb.nbins = nbins;
+    sb.bins = dbins.as<double>();
+    const long long modes = (long long)NH * N * N;
+    const int bgrid = (int)std::min<long long>((modes + 255) / 256, 4LL * ncu_);
+    hipLaunchKernelGGL(scalar_bin_kernel<2048>, dim3(bgrid), dim3(256), 0, stream_, sb);
+    SHM_HIP(hipGetLastError());
+    bins.assign(nbins, 0.);
+    SHM_HIP(hipMemcpyAsync(bins.data(), dbins.get(), sizeof(double) * nbins,
+                           hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+    // pre = t / (dk N^6), dk = 2 pi / L (spectrum.cpp:49-51).
+    const double pre = std::pow(1. / N, 6) * l0.t / (2. * M_PI / sim_->L);
+    for (double &v : bins) v *= pre;
+}
+
+void Runtime::projection(int fn, int mode, int max_level, std::vector<double> &data,
+                         std::vector<int> &count, int &D) {
+    check_output_fn(fn);
+    if (mode != 0 && mode != 1) throw Error(1, "projection mode must be 0 (sum) or 1 (max)");
+    int ml = max_level < 0 ? INT32_MAX : max_level;
+    int finest = 0;
+    while (has_level(finest + 1)) ++finest;
+    ml = std::min(ml, finest);
+    D = sim_->dimN[ml];
+    std::vector<double> p;
+    sim_->SetParamsProjections(p, level(0).t);
+    p.resize(std::max<size_t>(p.size(), 1), 0.);
+    DevMem dp, dproj, dn;
+    upload_vec(dp, p);
+    dproj.alloc(sizeof(double) * (size_t)D * D);
+    dn.alloc(sizeof(int) * (size_t)D * D);
+    SHM_HIP(hipMemsetAsync(dproj.get(), 0, dproj.bytes(), stream_));
+    SHM_HIP(hipMemsetAsync(dn.get(), 0, dn.bytes(), stream_));
+    std::vector<DevMem> keep;
+    for (int lev = 0; lev <= ml; ++lev) {
+        LevelState &ls = level(lev);
+        fill_output_ghosts(lev);
+        // Per box, the finer boxes that may hold fine cell (2i, 2j, 2k).
+        std::vector<int> start(1, 0), cov;
+        for (const Box3 &b : ls.L.boxes) {
+            if (lev < ml)
+                for (const Box3 &f : level(lev + 1).L.boxes) {
+                    bool hit = true;
+                    for (int d = 0; d < 3; ++d)
+                        hit = hit && 2 * b.lo[d] <= f.hi[d] && 2 * b.hi[d] >= f.lo[d];
+                    if (hit)
+                        cov.insert(cov.end(), {f.lo[0], f.lo[1], f.lo[2], f.hi[0], f.hi[1], f.hi[2]});
+                }
+            start.push_back((int)(cov.size() / 6));
+        }
+        if (cov.empty()) cov.assign(6, 0);
+        keep.emplace_back();
+        upload_vec(keep.back(), start);
+        const int *dstart = keep.back().as<int>();
+        keep.emplace_back();
+        upload_vec(keep.back(), cov);
+        OutArgs a{};
+        a.boxes = ls.L.dbox.as<BoxDesc>();
+        a.vprefix = ls.L.dvprefix.as<long long>();
+        a.nbox = (int)ls.L.boxes.size();
+        a.total = ls.L.nvalid();
+        a.cstride = ls.L.cstride;
+        a.ncomp = ncomp_;
+        a.arr = ls.U[ls.cur].get();
+        a.arrlo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
+        a.fn = fn;
+        a.params = dp.as<double>();
+        a.lev = lev;
+        a.time = ls.t;
+        a.dt = sim_->dt[lev];
+        a.dx = sim_->dx[lev];
+        a.target = 1;
+        a.cov_start = dstart;
+        a.cov = keep.back().as<int>();
+        a.ratio = D / sim_->dimN[lev];
+        a.D = D;
+        a.mode = mode;
+        a.proj = dproj.as<double>();
+        a.nproj = dn.as<int>();
+        const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
+        SHM_HIP(kt_.output[(int)prec_](a, grid, stream_));
+    }
+    data.assign((size_t)D * D, 0.);
+    count.assign((size_t)D * D, 0);
+    SHM_HIP(hipMemcpyAsync(data.data(), dproj.get(), dproj.bytes(), hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipMemcpyAsync(count.data(), dn.get(), dn.bytes(), hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
 }
 
 // ---------------------------------------------------------------------------

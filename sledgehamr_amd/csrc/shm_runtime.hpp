@@ -168,6 +168,17 @@ class Runtime {
     void gw_spectrum(int mode, int nbins, int zero_padding, int proj_type,
                      std::vector<double> &bins, std::vector<double> &kvals);
     void step(int ncoarse);              // TimeStepper::Advance(0)
+    /** Spectrum::Compute of output function fn on level 0: bins per unique
+     *  |k|^2 (k_sq). */
+    void spectrum(int fn, std::vector<double> &bins, std::vector<long long> &k_sq);
+    /** Projection::Compute of output function fn along z onto the finest
+     *  level up to max_level (< 0: all levels); D x D pixels, x index slow. */
+    void projection(int fn, int mode, int max_level, std::vector<double> &data,
+                    std::vector<int> &count, int &D);
+    int output_count() const { return kt_.nout; }
+    const char *output_name(int id) const {
+        return kt_.out_name ? kt_.out_name(id) : nullptr;
+    }
     /** Tag counters of the most recent scheduled truncation-error tagging
      *  of lev: [total, user, trunc[ncomp]]. */
     const std::vector<long long> &last_tags(int lev) const;
@@ -224,7 +235,11 @@ class Runtime {
     void launch_fused(LevelState &ls, const double *prm1, const double *prm2,
                       double dt, int lev);
     void halo(LevelState &ls, void *arr, const std::vector<int> &comps,
-              bool full = false);
+              bool full = false, int elem_bytes = 0);
+    /** Ghost cells of grid_new of lev with their remainders, for functions
+     *  that read the full-precision state at neighbours. */
+    void fill_output_ghosts(int lev);
+    void check_output_fn(int fn) const;
     /** FillPatch of state `which` of level lev at `time`: coarse/fine
      *  ghosts (levels above 0), then the same-level copy plan. */
     void fill_patch(LevelState &ls, int which, int lev, double time);

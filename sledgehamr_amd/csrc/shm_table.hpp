@@ -50,6 +50,7 @@ using OccFn = int (*)();
 using TeTagFn = hipError_t (*)(const TeTagArgs &, int grid, hipStream_t);
 using ProbeFn = hipError_t (*)(ProbeOut *, const double *, const double *,
                                double, double, double, hipStream_t);
+using OutFn = hipError_t (*)(const OutArgs &, int grid, hipStream_t);
 
 struct StageEntry {
     StageFn fn = nullptr;
@@ -75,6 +76,10 @@ struct KernelTable {
     FusedEntry fused12[3][4][2][kNVar];  // [precision][R][gw][variant]
     TeTagFn te_tag[3][2] = {};  // [precision][gw]
     ProbeFn probe[2] = {};      // [gw]
+    // Output functions for spectra and projections (SLEDGEHAMR_OUTPUT_FUNCTIONS).
+    int nout = 0;
+    const char *(*out_name)(int) = nullptr;
+    OutFn output[3] = {};       // [precision]
 };
 
 }  // namespace shm

@@ -42,6 +42,35 @@ void Rhs(const amrex::Array4<double>& rhs,
     rhs(i, j, k, Scalar::dPhi) = laplacian_Phi - lambda*Phi*(Phi*Phi - 1.);
 }
 
+// Output functions for spectra and projections.
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE
+double ScaledPhi(const amrex::Array4<const double>& state, const int i,
+                 const int j, const int k, const int lev, const double time,
+                 const double dt, const double dx, const double* params) {
+    return params[0] * state(i, j, k, Scalar::Phi);
+}
+
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE
+double dPhi2(const amrex::Array4<const double>& state, const int i,
+             const int j, const int k, const int lev, const double time,
+             const double dt, const double dx, const double* params) {
+    const double dPhi = state(i, j, k, Scalar::dPhi);
+    return dPhi * dPhi;
+}
+
+AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE
+double GradientEnergy(const amrex::Array4<const double>& state, const int i,
+                      const int j, const int k, const int lev,
+                      const double time, const double dt, const double dx,
+                      const double* params) {
+    const double gx = sledgehamr::utils::Gradient<2>(state, i, j, k, Scalar::Phi, dx, 'x');
+    const double gy = sledgehamr::utils::Gradient<2>(state, i, j, k, Scalar::Phi, dx, 'y');
+    const double gz = sledgehamr::utils::Gradient<2>(state, i, j, k, Scalar::Phi, dx, 'z');
+    return 0.5 * (gx * gx + gy * gy + gz * gz);
+}
+
+SLEDGEHAMR_OUTPUT_FUNCTIONS(ScaledPhi, dPhi2, GradientEnergy)
+
 SLEDGEHAMR_FINISH_SETUP
 
 class RealScalar : public sledgehamr::Sledgehamr {
@@ -52,6 +81,17 @@ class RealScalar : public sledgehamr::Sledgehamr {
         amrex::ParmParse pp("project");
         pp.query("laplacian_order", laplacian_order);
         pp.query("lambda", lambda);
+        pp.query("output_scale", output_scale);
+    }
+
+    void SetParamsSpectra(std::vector<double>& params,
+                          const double time) override {
+        params = {output_scale};
+    }
+
+    void SetParamsProjections(std::vector<double>& params,
+                              const double time) override {
+        params = {output_scale};
     }
 
     void SetParamsRhs(std::vector<double>& params, const double time,
@@ -62,6 +102,7 @@ class RealScalar : public sledgehamr::Sledgehamr {
   private:
     int laplacian_order = 1;
     double lambda = 1.;
+    double output_scale = 1.;
 };
 
 }; // namespace RealScalar

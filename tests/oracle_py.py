@@ -47,6 +47,8 @@ def _load(project: str) -> C.CDLL:
         "orc_upload_old": (i, [vp, i, i, dp, d]),
         "orc_fill_patch": (i, [vp, i, d]),
         "orc_download_fab": (i, [vp, i, i, i, dp]),
+        "orc_spectrum": (i, [vp, i, dp, llp, ip]),
+        "orc_projection": (i, [vp, i, i, i, dp, ip, ip]),
         "orc_fill_rhs": (i, [vp, i, d, d, i]),
         "orc_download_rhs": (i, [vp, i, dp]),
         "orc_advance": (i, [vp, i]),
@@ -143,6 +145,29 @@ class Oracle:
         a = np.empty(int(np.prod(shape)))
         self._ck(self.lib.orc_download_fab(self.h, lev, box, comp, _dp(a)))
         return a.reshape(shape)
+
+    def spectrum(self, fn):
+        """Spectrum of output function fn on level 0: (bins, k_sq)."""
+        n = C.c_int(0)
+        self._ck(self.lib.orc_spectrum(self.h, fn, None, None, C.byref(n)))
+        b = np.zeros(n.value)
+        k = np.zeros(n.value, dtype=np.int64)
+        self._ck(self.lib.orc_spectrum(self.h, fn, _dp(b),
+                                       k.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                       C.byref(n)))
+        return b, k
+
+    def projection(self, fn, mode=0, max_level=-1):
+        """Projection along z: (data, counts) as (dimN, dimN), x index first."""
+        D = C.c_int(0)
+        self._ck(self.lib.orc_projection(self.h, fn, mode, max_level, None, None,
+                                         C.byref(D)))
+        d = np.zeros(D.value ** 2)
+        c = np.zeros(D.value ** 2, dtype=np.int32)
+        self._ck(self.lib.orc_projection(self.h, fn, mode, max_level, _dp(d),
+                                         c.ctypes.data_as(C.POINTER(C.c_int)),
+                                         C.byref(D)))
+        return d.reshape(D.value, D.value), c.reshape(D.value, D.value)
 
     def fill_rhs(self, lev, time, weight=0.0, add=False):
         self._ck(self.lib.orc_fill_rhs(self.h, lev, time, weight, int(add)))

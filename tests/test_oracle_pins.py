@@ -694,3 +694,98 @@ def test_oracle_subcycled_schedule():
     for lev in range(3):
         assert orc.time(lev) == 0.125 + 3 * dt0
         assert orc.last_tags(lev)[0] >= 0
+
+
+# ---------------------------------------------------------------------------
+# Spectra and projections of output functions (output_types/spectrum.cpp:12-113,
+# output_types/projection.cpp:11-118); RealScalar declares ScaledPhi (params[0]
+# times Phi, params = {project.output_scale}), dPhi2 and GradientEnergy.
+# ---------------------------------------------------------------------------
+
+def _out_oracle(N=16, L=8.0, t0=0.75, scale=1.5, levels=0):
+    inp = real_scalar(N=N, order=1, nghost=2, lam=0.0, L=L,
+                      **{"sim.t_start": t0, "project.output_scale": scale,
+                         "amr.max_refinement_levels": levels})
+    return Oracle("RealScalar", inp)
+
+
+@pytest.mark.parametrize("m", [(3, 1, 0), (0, 2, 5), (8, 0, 0)])
+def test_spectrum_plane_wave(m):
+    """cos(2 pi m.n / N + p) has |F| = N^3 / 2 at +-m, one of which the
+    x-halved layout holds (mult 2) unless m_x = 0 (both, mult 1): the bin of
+    |m|^2 holds s^2 t / (2 dk).  At m_x = N/2 the two coincide: |F| = N^3 cos p,
+    mult 1, s^2 t cos^2 p / dk."""
+    N, L, t0, s, p = 16, 8.0, 0.75, 1.5, 0.3
+    orc = _out_oracle(N, L, t0, s)
+    n = np.arange(N)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    orc.upload(0, 0, np.cos(2 * np.pi * (m[0] * i + m[1] * j + m[2] * k) / N + p).ravel())
+    orc.upload(0, 1, np.zeros(N ** 3))
+    bins, ks = orc.spectrum(0)
+    dk = 2 * np.pi / L
+    want = s * s * t0 * (np.cos(p) ** 2 / dk if m[0] == N // 2 else 0.5 / dk)
+    b = int(np.searchsorted(ks, sum(q * q for q in m)))
+    assert ks[b] == sum(q * q for q in m)
+    assert abs(bins[b] - want) < 1e-12 * want
+    bins[b] = 0.
+    assert np.max(np.abs(bins)) < 1e-12 * want
+
+
+def test_spectrum_parseval():
+    """Sum over bins = t / (dk N^3) sum_x f^2 (the halved layout with its
+    multiplicities counts every mode once)."""
+    N, L, t0, s = 16, 8.0, 0.75, 1.5
+    orc = _out_oracle(N, L, t0, s)
+    f = np.random.default_rng(3).standard_normal(N ** 3)
+    orc.upload(0, 0, f)
+    orc.upload(0, 1, f[::-1].copy())
+    for fn, vals in ((0, s * f), (1, f[::-1] ** 2)):
+        bins, _ = orc.spectrum(fn)
+        want = t0 / (2 * np.pi / L) / N ** 3 * np.sum(vals ** 2)
+        assert abs(bins.sum() - want) < 1e-11 * want, fn
+
+
+def test_projection_orientation_single_level():
+    """Single level, field = x: pixel [I, J] sums N cells of x_I (x index
+    slow); field = y puts the variation on the second index."""
+    N = 16
+    orc = _out_oracle(N, L=16.0, scale=1.0)
+    n = np.arange(N)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    for axis, g in ((0, i), (1, j)):
+        orc.upload(0, 0, (g + 0.5).ravel())
+        orc.upload(0, 1, np.zeros(N ** 3))
+        d, c = orc.projection(0)
+        want = N * (n + 0.5)
+        assert np.allclose(d, want[:, None] if axis == 0 else want[None, :],
+                           rtol=0, atol=1e-12)
+        assert np.all(c == N)
+
+
+def test_projection_two_levels_linear_in_z():
+    """Field = z on both levels (cell centres): a coarse cell adds
+    2 z_coarse = the sum of its two fine cells, so every pixel holds the
+    fine column sum whatever the refinement; counts are 8 fine cells plus
+    the 12 coarse cells the fine box leaves uncovered under it, else 16."""
+    orc = _out_oracle(16, L=16.0, scale=1.0, levels=1)
+    box = [8, 8, 4, 15, 15, 11]
+    orc.set_boxes(1, [box])
+    n = np.arange(16)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    orc.upload(0, 0, ((k + 0.5) * 1.0).ravel())
+    orc.upload(0, 1, np.zeros(16 ** 3))
+    kk = np.arange(4, 12)
+    orc.upload(1, 0, np.repeat((kk + 0.5) * 0.5, 64))
+    orc.upload(1, 1, np.zeros(512))
+    d, c = orc.projection(0, mode=0)
+    assert d.shape == (32, 32)
+    assert np.allclose(d, np.sum((np.arange(32) + 0.5) * 0.5), rtol=0, atol=1e-12)
+    inside = np.zeros((32, 32), bool)
+    inside[8:16, 8:16] = True
+    assert np.all(c[inside] == 20) and np.all(c[~inside] == 16)
+    dm, _ = orc.projection(0, mode=1)
+    assert np.all(dm == 15.5)
+    # Coarse level only.
+    d0, c0 = orc.projection(0, mode=0, max_level=0)
+    assert d0.shape == (16, 16) and np.all(c0 == 16)
+    assert np.allclose(d0, np.sum(np.arange(16) + 0.5), rtol=0, atol=1e-12)
