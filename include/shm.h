@@ -132,6 +132,14 @@ int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
                     int projection_type, double *bins, double *kvals,
                     int *nout);
 
+/* Initial-state arrays in the reference's orders (the HDF5 reader's data):
+ * layout 1: the whole level, ind = i N^2 + j N + k (FillLevel::FromArray,
+ * fill_level.cpp:205-223); layout 2: one chunk per box in box order,
+ * box-relative ind = i ly lz + j lz + k (FillLevel::FromArrayChunks,
+ * fill_level.cpp:225-251); layout 0: as shm_upload. */
+int shm_upload_array(shm_ctx *ctx, int lev, int comp, const double *data,
+                     int layout);
+
 /* Output cell functions the project declares (SLEDGEHAMR_OUTPUT_FUNCTIONS;
  * the device form of io_module->spectra / io_module->projections).  *count
  * receives their number; shm_output_function_name copies the name of id

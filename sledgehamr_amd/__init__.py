@@ -30,6 +30,7 @@ EXPORTS = (
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
     "shm_output_functions", "shm_output_function_name", "shm_spectrum", "shm_projection",
+    "shm_upload_array",
     "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
     "shm_get_launch", "shm_set_graphs", "shm_set_timing", "shm_get_timing",
 )
@@ -100,6 +101,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_level_tag_counts": (i, [vp, i, llp]),
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
+        "shm_upload_array": (i, [vp, i, i, dp, i]),
         "shm_output_functions": (i, [vp, ip]),
         "shm_output_function_name": (i, [vp, i, C.c_char_p, i]),
         "shm_spectrum": (i, [vp, i, dp, llp, ip]),
@@ -282,6 +284,12 @@ class Simulation:
                                              projection_type, _dptr(bins),
                                              _dptr(k), C.byref(n)))
         return bins, k
+
+    def upload_array(self, lev: int, comp: int, data, layout: int) -> None:
+        """Upload from the reference's array orders (1: whole level,
+        i N^2 + j N + k; 2: per-box chunks, box-relative i ly lz + j lz + k)."""
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        self._check(self.lib.shm_upload_array(self._ctx, lev, comp, _dptr(a), layout))
 
     def output_functions(self) -> list:
         """Names of the project's output functions, in id order."""

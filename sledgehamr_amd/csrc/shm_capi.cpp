@@ -281,6 +281,17 @@ __attribute__((visibility("default"))) int shm_gw_spectrum(
     });
 }
 
+__attribute__((visibility("default"))) int shm_upload_array(shm_ctx *ctx, int lev,
+                                                           int comp,
+                                                           const double *data,
+                                                           int layout) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!data) throw shm::Error(SHM_ERR_ARG, "null data");
+        ctx->rt->upload_array(lev, comp, data, layout);
+    });
+}
+
 __attribute__((visibility("default"))) int shm_output_functions(shm_ctx *ctx,
                                                                int *count) {
     return guard(ctx, [&] {

@@ -1590,6 +1590,31 @@ static float host_bf16_round(float r) {
     return o;
 }
 
+void Runtime::upload_array(int lev, int comp_id, const double *host, int layout) {
+    if (layout == 0) return upload(lev, comp_id, host);
+    if (layout != 1 && layout != 2) throw Error(1, "array layout must be 0, 1 or 2");
+    if (lev == -1) define_shadow();
+    LevelState &ls = level(lev);
+    const long long N = ls.L.N;
+    std::vector<double> v((size_t)ls.L.nvalid());
+    long long chunk = 0;
+    for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
+        const Box3 &bx = ls.L.boxes[b];
+        const long long nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
+        double *dst = v.data() + ls.L.vprefix[b];
+        for (long long z = 0; z < nz; ++z)
+            for (long long y = 0; y < ny; ++y)
+                for (long long x = 0; x < nx; ++x) {
+                    const long long i = bx.lo[0] + x, j = bx.lo[1] + y, k = bx.lo[2] + z;
+                    const long long ind = layout == 1 ? (i * N + j) * N + k
+                                                      : chunk + (x * ny + y) * nz + z;
+                    dst[(z * ny + y) * nx + x] = host[ind];
+                }
+        chunk += nx * ny * nz;
+    }
+    upload(lev, comp_id, v.data());
+}
+
 void Runtime::upload(int lev, int comp_id, const double *host) {
     if (lev == -1) define_shadow();
     LevelState &ls = level(lev);
@@ -1877,8 +1902,9 @@ void Runtime::collect_timing() {
 /** Times the stage kernels of a level over the launch shapes (tile variant,
  *  z-chunk length, persistent or one tile per workgroup, tile order, XCD
  *  split) and keeps the fastest for the level.  Only kernels that write
- *  stage scratch run (SSPRK3 stages 1 and 2, the first RKN stage), so the
- *  level's state is unchanged.  Fields of `launch` that are set are kept. */
+ *  stage scratch run (SSPRK3 stages 1 and 2, stage 3 into a scratch state,
+ *  the first RKN stage), so the level's state is unchanged.  Fields of
+ *  `launch` that are set are kept. */
 void Runtime::autotune(int lev) {
     LevelState &ls = level(lev);
     const int old = ls.cur;
@@ -1930,11 +1956,42 @@ void Runtime::autotune(int lev) {
     const std::vector<int> os = choices(launch.zc_minor, {0, 1});
     const std::vector<int> xs = choices(launch.xcd_split, {1, 0});
 
+    // SSPRK3: stage three (the new state into a scratch target, so the
+    // level's states are untouched) is timed per shape too; it is the
+    // stage kernel that remains when stages one and two run fused.
+    DevMem t3, t3lo;
+    StageArgs s3{};
+    const bool tune3 = integrator_ == 10;
+    if (tune3) {
+        t3.alloc((size_t)ls.L.cstride * ls.ncomp * esize());
+        if (comp()) t3lo.alloc((size_t)ls.L.cstride * ls.ncomp * losize());
+        s3 = a;
+        s3.u0 = ls.U[old].get();
+        s3.u0lo = comp() ? ls.Ulo[old].get() : nullptr;
+        s3.kin = ls.K[1].get();
+        s3.unew = t3.get();
+        s3.unewlo = comp() ? t3lo.get() : nullptr;
+        s3.c[0] = 0.25;
+        s3.c[1] = dt * 2. / 3.;
+        s3.c[2] = dt / 4.;
+    }
+    auto time_ms = [&](auto &&fn) {
+        fn();  // warm-up
+        SHM_HIP(hipEventRecord(ev0_, stream_));
+        const int reps = 3;
+        for (int k = 0; k < reps; ++k) fn();
+        SHM_HIP(hipEventRecord(ev1_, stream_));
+        SHM_HIP(hipEventSynchronize(ev1_));
+        float ms = 0;
+        SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        return (double)ms;
+    };
+
     const LaunchCfg saved = launch;
     const bool saved_t = timing_on;
     timing_on = false;
-    double best = 1e30;
-    LaunchCfg bc;
+    double best = 1e30, best3 = 1e30;
+    LaunchCfg bc, bc3;
     for (int v : vs)
         for (int z : zs)
             for (int p : ps)
@@ -1946,28 +2003,31 @@ void Runtime::autotune(int lev) {
                         launch.persistent = p;
                         launch.zc_minor = o;
                         launch.xcd_split = x;
-                        for (auto &r : runs) {
-                            StageArgs c = r.second;
-                            launch_stage(r.first, ls, c, lev);
-                        }
-                        SHM_HIP(hipEventRecord(ev0_, stream_));
-                        const int reps = 3;
-                        for (int k = 0; k < reps; ++k)
+                        double ms = time_ms([&] {
                             for (auto &r : runs) {
                                 StageArgs c = r.second;
                                 launch_stage(r.first, ls, c, lev);
                             }
-                        SHM_HIP(hipEventRecord(ev1_, stream_));
-                        SHM_HIP(hipEventSynchronize(ev1_));
-                        float ms = 0;
-                        SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+                        });
+                        if (tune3) {
+                            const double m3 = time_ms([&] {
+                                StageArgs c = s3;
+                                launch_stage(kSsprk3, ls, c, lev);
+                            });
+                            if (m3 < best3) {
+                                best3 = m3;
+                                bc3 = launch;
+                            }
+                            ms += m3;
+                        }
                         if (ms < best) {
                             best = ms;
                             bc = launch;
                         }
                     }
     // SSPRK3 on a level of periodic boxes: one fused launch for stages one
-    // and two against the best pair of launches found above.
+    // and two plus stage three at its best shape, against the three
+    // unfused launches at their best common shape.
     bc.fused = 0;
     bc.fused_variant = -1;
     if (integrator_ == 10 && fused_possible(ls) && saved.fused != 0) {
@@ -1975,20 +2035,15 @@ void Runtime::autotune(int lev) {
         for (int v = 0; v < kNVar; ++v) {
             if (!row[v].fn) continue;
             if (saved.fused_variant >= 0 && v != saved.fused_variant) continue;
-            launch = bc;
+            launch = bc3;
             launch.fused = 1;
             launch.fused_variant = v;
             launch.tz = 0;
-            launch_fused(ls, prm, prm, dt, lev);
-            SHM_HIP(hipEventRecord(ev0_, stream_));
-            const int reps = 3;
-            for (int k = 0; k < reps; ++k) launch_fused(ls, prm, prm, dt, lev);
-            SHM_HIP(hipEventRecord(ev1_, stream_));
-            SHM_HIP(hipEventSynchronize(ev1_));
-            float ms = 0;
-            SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            const double ms =
+                time_ms([&] { launch_fused(ls, prm, prm, dt, lev); }) + best3;
             if (ms < best) {
                 best = ms;
+                bc = bc3;
                 bc.fused = 1;
                 bc.fused_variant = v;
             }

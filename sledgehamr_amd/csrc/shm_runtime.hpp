@@ -155,6 +155,11 @@ class Runtime {
 
     // Data movement (valid cells, box-major, x fastest).
     void upload(int lev, int comp, const double *host);
+    /** Upload from the reference's array orders: 1 the whole level with
+     *  ind = i N^2 + j N + k (FillLevel::FromArray), 2 one chunk per box,
+     *  box-relative ind = i ly lz + j lz + k (FillLevel::FromArrayChunks),
+     *  chunks in box order. */
+    void upload_array(int lev, int comp, const double *host, int layout);
     void download(int lev, int comp, double *host, bool old_state = false);
     void download_rhs(int lev, int comp, double *host);
     void init_state(int lev, int kind, uint64_t seed, const double *p, int np);
