@@ -132,6 +132,12 @@ int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
                     int projection_type, double *bins, double *kvals,
                     int *nout);
 
+/* LevelSynchronizer::FillCoarsePatch (level_synchronizer.cpp:56-88): every
+ * cell of level lev (lev >= 1), ghosts included, interpolated from level
+ * lev-1 at the level's time (amr.interpolation_type); between the coarse
+ * level's two times the old state is used, as the reference does. */
+int shm_fill_coarse_patch(shm_ctx *ctx, int lev);
+
 /* Initial-state arrays in the reference's orders (the HDF5 reader's data):
  * layout 1: the whole level, ind = i N^2 + j N + k (FillLevel::FromArray,
  * fill_level.cpp:205-223); layout 2: one chunk per box in box order,

@@ -366,7 +366,8 @@ struct Config {
     bool shadow_hierarchy = false;
     bool independent = false;  // level-0 boxes are separate periodic domains
     double regrid_dt = DBL_MAX;
-    int interp_type = 2;  // amr.interpolation_type: 0 piecewise constant, 2 quadratic
+    int interp_type = 2;  // amr.interpolation_type: 0 piecewise constant,
+                          // 1 limited linear, 2 quadratic, 4 quartic
 };
 
 template <class Proj>
@@ -441,10 +442,9 @@ class Sim {
         }
         // LevelSynchronizer::LevelSynchronizer (level_synchronizer.cpp:27-47).
         pp.query("amr.interpolation_type", cfg.interp_type);
-        if (cfg.interp_type != 0 && cfg.interp_type != 2)
-            throw std::runtime_error(
-                "amr.interpolation_type: 0 (piecewise constant) and 2 (quadratic) "
-                "are supported");
+        if (cfg.interp_type != 0 && cfg.interp_type != 1 && cfg.interp_type != 2 &&
+            cfg.interp_type != 4)
+            throw std::runtime_error("amr.interpolation_type must be 0, 1, 2 or 4");
         switch (cfg.integrator) {
             case 20: {
                 amrex::ParmParse pr("integrator.rkn");
@@ -587,6 +587,14 @@ class Sim {
         fill_boundary(m, lev < 0 ? cfg.N0 / 2 : dimN[lev]);
     }
 
+    /** LevelSynchronizer::FillCoarsePatch (level_synchronizer.cpp:56-88):
+     *  every cell of grid_new[lev], ghosts included, interpolated from level
+     *  lev-1 at the level's time (InterpFromCoarseLevel). */
+    void fill_coarse_patch(int lev) {
+        if (lev < 1) throw std::runtime_error("FillCoarsePatch needs a level above 0");
+        fill_coarse_ghosts(grid_new[lev], lev, grid_new[lev].t, true);
+    }
+
     /** Weights of the level-lev states at `time`: GetLevelData picks the new
      *  or the old state within 1e-3 of their time difference, else both
      *  (level_synchronizer.cpp:613-632), combined linearly in time. */
@@ -609,18 +617,32 @@ class Sim {
      *  Spatial interpolation, ratio 2, at the fine cell centre x, y, z =
      *  +-1/4 coarse cells from the parent's centre:
      *    type 0: the parent value;
+     *    type 1: c + sx x + sy y + sz z with monotonized-central slopes
+     *            (sl = c - c[-1], sr = c[+1] - c, sc = (sl + sr) / 2;
+     *            s = sign(sc) min(|sc|, 2|sl|, 2|sr|) if sl sr > 0, else 0):
+     *            conservative, exact for linear data, no new extrema;
      *    type 2: c + sx x + sy y + sz z + sxx x^2 + syy y^2 + szz z^2
      *            + sxy x y + sxz x z + syz y z, with central differences
      *            sx = (c[+1] - c[-1]) / 2, sxx = (c[+1] - 2 c + c[-1]) / 2,
      *            sxy = (c[+1,+1] - c[-1,+1] - c[+1,-1] + c[-1,-1]) / 4:
-     *            exact for every polynomial of degree <= 2.
+     *            exact for every polynomial of degree <= 2;
+     *    type 4: the tensor product of the conservative quartic weights
+     *            (3, -22, 128, 22, -3) / 128 over coarse offsets -2..2 for
+     *            an upper child, mirrored for a lower one: the child's
+     *            average of the quartic with the 5 coarse averages.
      *  Coarse values are the time-weighted states of level lev-1 at the
      *  periodic image of each coarse cell; a coarse cell that no box of
      *  level lev-1 holds is an error (the level is not properly nested). */
-    void fill_coarse_ghosts(MultiFab &m, int lev, double time) {
+    void fill_coarse_ghosts(MultiFab &m, int lev, double time, bool all = false) {
         MultiFab &New = grid_new[lev - 1], &Old = grid_old[lev - 1];
         double wo, wn;
         time_weights(lev - 1, time, wo, wn);
+        if (all && wo != 0. && wn != 0.) {
+            // FillCoarsePatch hands InterpFromCoarseLevel only the first state
+            // of GetLevelData (level_synchronizer.cpp:64, 83): the old one.
+            wo = 1.;
+            wn = 0.;
+        }
         const int Nc = dimN[lev - 1];
         auto wrap = [Nc](int v) { return ((v % Nc) + Nc) % Nc; };
         // 19 points: centre, 6 faces, 12 edges (i, j, k offsets).
@@ -629,7 +651,11 @@ class Sim {
             {0, 0, -1}, {0, 0, 1},  {1, 1, 0},   {-1, 1, 0},  {1, -1, 0},
             {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1},  {1, 0, -1},  {-1, 0, -1},
             {0, 1, 1},  {0, -1, 1}, {0, 1, -1},  {0, -1, -1}};
-        const int np = cfg.interp_type == 0 ? 1 : 19;
+        const int np = cfg.interp_type == 0 ? 1 : cfg.interp_type == 1 ? 7 : 19;
+        if (cfg.interp_type == 4) {
+            fill_coarse_ghosts_quartic(m, lev, wo, wn, all);
+            return;
+        }
         bool nested = true;
         for (Fab &F : m.fabs) {
             const int g = F.g;
@@ -637,8 +663,9 @@ class Sim {
             for (int k = F.box.lo[2] - g; k <= F.box.hi[2] + g; ++k)
                 for (int j = F.box.lo[1] - g; j <= F.box.hi[1] + g; ++j)
                     for (int i = F.box.lo[0] - g; i <= F.box.hi[0] + g; ++i) {
-                        if (i >= F.box.lo[0] && i <= F.box.hi[0] && j >= F.box.lo[1] &&
-                            j <= F.box.hi[1] && k >= F.box.lo[2] && k <= F.box.hi[2])
+                        if (!all && i >= F.box.lo[0] && i <= F.box.hi[0] &&
+                            j >= F.box.lo[1] && j <= F.box.hi[1] && k >= F.box.lo[2] &&
+                            k <= F.box.hi[2])
                             continue;
                         const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
                         int fb[19], ci[19], cj[19], ck[19];
@@ -679,6 +706,21 @@ class Sim {
                             const double x = (i & 1) ? 0.25 : -0.25;
                             const double y = (j & 1) ? 0.25 : -0.25;
                             const double z = (k & 1) ? 0.25 : -0.25;
+                            if (np == 7) {
+                                auto mc = [](double cm, double c0, double cp) {
+                                    const double sl = c0 - cm, sr = cp - c0;
+                                    const double sc = 0.5 * (sl + sr);
+                                    if (!(sl * sr > 0.)) return 0.;
+                                    const double a = std::min(std::fabs(sc),
+                                                              std::min(2. * std::fabs(sl),
+                                                                       2. * std::fabs(sr)));
+                                    return sc > 0. ? a : -a;
+                                };
+                                F.at(i, j, k, n) = c[0] + x * mc(c[1], c[0], c[2]) +
+                                                   y * mc(c[3], c[0], c[4]) +
+                                                   z * mc(c[5], c[0], c[6]);
+                                continue;
+                            }
                             const double sx = 0.5 * (c[2] - c[1]), sy = 0.5 * (c[4] - c[3]),
                                          sz = 0.5 * (c[6] - c[5]);
                             const double sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
@@ -690,6 +732,77 @@ class Sim {
                             F.at(i, j, k, n) = c[0] + x * sx + y * sy + z * sz +
                                                x * x * sxx + y * y * syy + z * z * szz +
                                                x * y * sxy + x * z * sxz + y * z * syz;
+                        }
+                    }
+        }
+        if (!nested)
+            throw std::runtime_error("level " + std::to_string(lev) +
+                                     " is not properly nested: a ghost cell needs coarse "
+                                     "cells that no box of level " +
+                                     std::to_string(lev - 1) + " holds");
+    }
+
+    /** Type 4 of fill_coarse_ghosts: sum over coarse offsets a, b, c in
+     *  -2..2 (c outermost, a innermost) of wz[c] wy[b] wx[a] C. */
+    void fill_coarse_ghosts_quartic(MultiFab &m, int lev, double wo, double wn,
+                                    bool all) {
+        MultiFab &New = grid_new[lev - 1], &Old = grid_old[lev - 1];
+        const int Nc = dimN[lev - 1];
+        auto wrap = [Nc](int v) { return ((v % Nc) + Nc) % Nc; };
+        static const double WU[5] = {3. / 128., -22. / 128., 1., 22. / 128., -3. / 128.};
+        bool nested = true;
+        for (Fab &F : m.fabs) {
+            const int g = F.g;
+#pragma omp parallel for schedule(dynamic)
+            for (int k = F.box.lo[2] - g; k <= F.box.hi[2] + g; ++k)
+                for (int j = F.box.lo[1] - g; j <= F.box.hi[1] + g; ++j)
+                    for (int i = F.box.lo[0] - g; i <= F.box.hi[0] + g; ++i) {
+                        if (!all && i >= F.box.lo[0] && i <= F.box.hi[0] &&
+                            j >= F.box.lo[1] && j <= F.box.hi[1] && k >= F.box.lo[2] &&
+                            k <= F.box.hi[2])
+                            continue;
+                        const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+                        double wx[5], wy[5], wz[5];
+                        for (int q = 0; q < 5; ++q) {
+                            wx[q] = (i & 1) ? WU[q] : WU[4 - q];
+                            wy[q] = (j & 1) ? WU[q] : WU[4 - q];
+                            wz[q] = (k & 1) ? WU[q] : WU[4 - q];
+                        }
+                        for (int n = 0; n < m.nc; ++n) {
+                            double acc = 0.;
+                            bool ok = true;
+                            for (int cz = 0; cz < 5 && ok; ++cz)
+                                for (int by = 0; by < 5 && ok; ++by)
+                                    for (int ax = 0; ax < 5 && ok; ++ax) {
+                                        const int ci = wrap(ic + ax - 2), cj = wrap(jc + by - 2),
+                                                  ck = wrap(kc + cz - 2);
+                                        int fb = -1;
+                                        for (size_t b = 0; b < New.fabs.size() && fb < 0; ++b) {
+                                            const Box &B = New.fabs[b].box;
+                                            if (ci >= B.lo[0] && ci <= B.hi[0] && cj >= B.lo[1] &&
+                                                cj <= B.hi[1] && ck >= B.lo[2] && ck <= B.hi[2])
+                                                fb = (int)b;
+                                        }
+                                        if (fb < 0) {
+                                            ok = false;
+                                            break;
+                                        }
+                                        const double vn = New.fabs[fb].at(ci, cj, ck, n);
+                                        double v;
+                                        if (wo == 0.)
+                                            v = vn;
+                                        else {
+                                            const double vo = Old.fabs[fb].at(ci, cj, ck, n);
+                                            v = wn == 0. ? vo : wo * vo + wn * vn;
+                                        }
+                                        acc += wz[cz] * wy[by] * wx[ax] * v;
+                                    }
+                            if (!ok) {
+#pragma omp atomic write
+                                nested = false;
+                                continue;
+                            }
+                            F.at(i, j, k, n) = acc;
                         }
                     }
         }

@@ -111,6 +111,10 @@
             s->grid_old[lev].t = t;                                            \
         });                                                                    \
     }                                                                          \
+    ORC_EXPORT int orc_fill_coarse_patch(void *h, int lev) {                   \
+        return orc_guard(                                                      \
+            [&] { static_cast<OrcSim *>(h)->fill_coarse_patch(lev); });        \
+    }                                                                          \
     ORC_EXPORT int orc_fill_patch(void *h, int lev, double time) {             \
         return orc_guard([&] {                                                 \
             auto *s = static_cast<OrcSim *>(h);                                \

@@ -30,7 +30,7 @@ EXPORTS = (
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
     "shm_output_functions", "shm_output_function_name", "shm_spectrum", "shm_projection",
-    "shm_upload_array",
+    "shm_upload_array", "shm_fill_coarse_patch",
     "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
     "shm_get_launch", "shm_set_graphs", "shm_set_timing", "shm_get_timing",
 )
@@ -102,6 +102,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "shm_upload_array": (i, [vp, i, i, dp, i]),
+        "shm_fill_coarse_patch": (i, [vp, i]),
         "shm_output_functions": (i, [vp, ip]),
         "shm_output_function_name": (i, [vp, i, C.c_char_p, i]),
         "shm_spectrum": (i, [vp, i, dp, llp, ip]),
@@ -290,6 +291,10 @@ class Simulation:
         i N^2 + j N + k; 2: per-box chunks, box-relative i ly lz + j lz + k)."""
         a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
         self._check(self.lib.shm_upload_array(self._ctx, lev, comp, _dptr(a), layout))
+
+    def fill_coarse_patch(self, lev: int) -> None:
+        """FillCoarsePatch: level lev from level lev-1 at the level's time."""
+        self._check(self.lib.shm_fill_coarse_patch(self._ctx, lev))
 
     def output_functions(self) -> list:
         """Names of the project's output functions, in id order."""

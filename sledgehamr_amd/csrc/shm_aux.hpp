@@ -314,11 +314,15 @@ __global__ void __launch_bounds__(256) avgdown_kernel(const AvgArgs a) {
 // time and space; the same-level copy plan, run afterwards, overwrites the
 // ghost cells other fine boxes cover.  Spatial interpolation, ratio 2, at
 // the fine cell centre x, y, z = +-1/4 coarse cells from the parent centre:
-//   piecewise constant: the parent value;
-//   quadratic: c + sx x + sy y + sz z + sxx x^2 + syy y^2 + szz z^2
+//   0 piecewise constant: the parent value;
+//   1 limited linear: c + sx x + sy y + sz z with monotonized-central
+//              slopes (conservative, exact for linear data);
+//   2 quadratic: c + sx x + sy y + sz z + sxx x^2 + syy y^2 + szz z^2
 //              + sxy x y + sxz x z + syz y z from central differences of
 //              the 19-point coarse neighbourhood (exact for polynomials of
-//              degree <= 2).
+//              degree <= 2);
+//   4 quartic: tensor product of the conservative child weights
+//              (3, -22, 128, 22, -3) / 128 over the 5^3 coarse neighbourhood.
 // What a ghost cell receives depends on how the stage kernels form the
 // state they read there:
 //   kCfState  the state itself (value and remainder);
@@ -333,12 +337,13 @@ __device__ AMREX_FORCE_INLINE int cf_wrap(int v, int n) {
     return r < 0 ? r + n : r;
 }
 
-template <Prec P, int MODE, bool QUAD>
+template <Prec P, int MODE, int INTERP>
 __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
     using T = PrecTraits<P>;
     using S = typename T::S;
-    constexpr int NPT = QUAD ? 19 : 1;
-    // Centre, 6 faces, then the 12 edges as (+,+) (-,+) (+,-) (-,-) pairs.
+    // Points of the type 0 / 1 / 2 stencils: centre, 6 faces, then the 12
+    // edges as (+,+) (-,+) (+,-) (-,-) pairs.
+    constexpr int NPT = INTERP == 0 ? 1 : INTERP == 1 ? 7 : 19;
     constexpr int PO[19][3] = {
         {0, 0, 0},   {-1, 0, 0}, {1, 0, 0},  {0, -1, 0}, {0, 1, 0},
         {0, 0, -1},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0}, {1, -1, 0},
@@ -368,56 +373,34 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
                              (long long)(gj - db.lo[1]) * db.jstride +
                              (long long)(gk - db.lo[2]) * db.kstride;
         const int ic = gi >> 1, jc = gj >> 1, kc = gk >> 1;
-        long long ce[NPT];
-        bool ok = true;
-#pragma unroll
-        for (int p = 0; p < NPT; ++p) {
-            const int ci = cf_wrap(ic + PO[p][0], a.Nc), cj = cf_wrap(jc + PO[p][1], a.Nc),
-                      ck = cf_wrap(kc + PO[p][2], a.Nc);
-            ce[p] = -1;
+        // Element of coarse cell (ic + di, jc + dj, kc + dk), periodic image,
+        // among the candidate boxes; -1 if none holds it.
+        auto locate = [&](int di, int dj, int dk) -> long long {
+            const int ci = cf_wrap(ic + di, a.Nc), cj = cf_wrap(jc + dj, a.Nc),
+                      ck = cf_wrap(kc + dk, a.Nc);
             for (int q = a.cand_start[fb]; q < a.cand_start[fb + 1]; ++q) {
                 const BoxDesc &cb = a.cboxes[a.cand[q]];
                 if (ci >= cb.lo[0] && ci < cb.lo[0] + cb.n[0] && cj >= cb.lo[1] &&
-                    cj < cb.lo[1] + cb.n[1] && ck >= cb.lo[2] && ck < cb.lo[2] + cb.n[2]) {
-                    ce[p] = cb.base + (ci - cb.lo[0]) + (long long)(cj - cb.lo[1]) * cb.jstride +
-                            (long long)(ck - cb.lo[2]) * cb.kstride;
-                    break;
-                }
+                    cj < cb.lo[1] + cb.n[1] && ck >= cb.lo[2] && ck < cb.lo[2] + cb.n[2])
+                    return cb.base + (ci - cb.lo[0]) + (long long)(cj - cb.lo[1]) * cb.jstride +
+                           (long long)(ck - cb.lo[2]) * cb.kstride;
             }
-            ok = ok && ce[p] >= 0;
-        }
-        const double xo = (gi & 1) ? 0.25 : -0.25, yo = (gj & 1) ? 0.25 : -0.25,
-                     zo = (gk & 1) ? 0.25 : -0.25;
-        for (int n = 0; n < a.ncomp; ++n) {
+            return -1;
+        };
+        // Coarse value at element ce of component comp, weighted in time.
+        auto cval = [&](int comp, long long ce) -> double {
+            const long long e = (long long)comp * a.ccstride + ce;
+            double v = 0.;
+            if (a.cnew) v = load_state_at<T>(a.cnew, a.cnewlo, e);
+            if (a.cold) {
+                const double vo = load_state_at<T>(a.cold, a.coldlo, e);
+                v = a.cnew ? a.wo * vo + a.wn * v : vo;
+            }
+            return v;
+        };
+        // What the ghost cell receives for component slot n.
+        auto put = [&](int n, double v) {
             const int comp = a.comp[n];
-            double c[NPT];
-#pragma unroll
-            for (int p = 0; p < NPT; ++p) {
-                const long long e = (long long)comp * a.ccstride + (ok ? ce[p] : 0);
-                double v = 0.;
-                if (a.cnew) v = load_state_at<T>(a.cnew, a.cnewlo, e);
-                if (a.cold) {
-                    const double vo = load_state_at<T>(a.cold, a.coldlo, e);
-                    v = a.cnew ? a.wo * vo + a.wn * v : vo;
-                }
-                c[p] = v;
-            }
-            double v;
-            if constexpr (QUAD) {
-                const double sx = 0.5 * (c[2] - c[1]), sy = 0.5 * (c[4] - c[3]),
-                             sz = 0.5 * (c[6] - c[5]);
-                const double sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
-                const double syy = 0.5 * (c[4] - 2. * c[0] + c[3]);
-                const double szz = 0.5 * (c[6] - 2. * c[0] + c[5]);
-                const double sxy = 0.25 * (c[7] - c[8] - c[9] + c[10]);
-                const double sxz = 0.25 * (c[11] - c[12] - c[13] + c[14]);
-                const double syz = 0.25 * (c[15] - c[16] - c[17] + c[18]);
-                v = c[0] + xo * sx + yo * sy + zo * sz + xo * xo * sxx + yo * yo * syy +
-                    zo * zo * szz + xo * yo * sxy + xo * zo * sxz + yo * zo * syz;
-            } else {
-                v = c[0];
-            }
-            if (!ok) v = __builtin_nan("");
             if constexpr (MODE == kCfState) {
                 store_state_at<T>(a.dst, a.dstlo, (long long)comp * a.fcstride + fe, v);
             } else if constexpr (MODE == kCfK) {
@@ -432,6 +415,71 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
                 for (int j = 0; j < a.m; ++j)
                     yv += (double)(S)a.cf[j] * (double)ld<S>(a.f[j], pe, 0u);
                 st<S>(a.dst, pe, 0u, (S)((v - yv) / (double)(S)a.cf[a.m]));
+            }
+        };
+        if constexpr (INTERP == 4) {
+            // Tensor product of the conservative quartic child weights.
+            constexpr double WU[5] = {3. / 128., -22. / 128., 1., 22. / 128., -3. / 128.};
+            double wx[5], wy[5], wz[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                wx[q] = (gi & 1) ? WU[q] : WU[4 - q];
+                wy[q] = (gj & 1) ? WU[q] : WU[4 - q];
+                wz[q] = (gk & 1) ? WU[q] : WU[4 - q];
+            }
+            for (int n = 0; n < a.ncomp; ++n) {
+                double acc = 0.;
+                bool ok = true;
+                for (int cz = 0; cz < 5; ++cz)
+                    for (int by = 0; by < 5; ++by)
+                        for (int ax = 0; ax < 5; ++ax) {
+                            const long long ce = locate(ax - 2, by - 2, cz - 2);
+                            ok = ok && ce >= 0;
+                            acc += wz[cz] * wy[by] * wx[ax] * cval(a.comp[n], ce >= 0 ? ce : 0);
+                        }
+                put(n, ok ? acc : __builtin_nan(""));
+            }
+        } else {
+            long long ce[NPT];
+            bool ok = true;
+#pragma unroll
+            for (int p = 0; p < NPT; ++p) {
+                ce[p] = locate(PO[p][0], PO[p][1], PO[p][2]);
+                ok = ok && ce[p] >= 0;
+            }
+            const double xo = (gi & 1) ? 0.25 : -0.25, yo = (gj & 1) ? 0.25 : -0.25,
+                         zo = (gk & 1) ? 0.25 : -0.25;
+            for (int n = 0; n < a.ncomp; ++n) {
+                double c[NPT];
+#pragma unroll
+                for (int p = 0; p < NPT; ++p) c[p] = cval(a.comp[n], ok ? ce[p] : 0);
+                double v;
+                if constexpr (INTERP == 2) {
+                    const double sx = 0.5 * (c[2] - c[1]), sy = 0.5 * (c[4] - c[3]),
+                                 sz = 0.5 * (c[6] - c[5]);
+                    const double sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
+                    const double syy = 0.5 * (c[4] - 2. * c[0] + c[3]);
+                    const double szz = 0.5 * (c[6] - 2. * c[0] + c[5]);
+                    const double sxy = 0.25 * (c[7] - c[8] - c[9] + c[10]);
+                    const double sxz = 0.25 * (c[11] - c[12] - c[13] + c[14]);
+                    const double syz = 0.25 * (c[15] - c[16] - c[17] + c[18]);
+                    v = c[0] + xo * sx + yo * sy + zo * sz + xo * xo * sxx + yo * yo * syy +
+                        zo * zo * szz + xo * yo * sxy + xo * zo * sxz + yo * zo * syz;
+                } else if constexpr (INTERP == 1) {
+                    // Monotonized-central slopes per direction.
+                    auto mc = [](double cm, double c0, double cp) {
+                        const double sl = c0 - cm, sr = cp - c0;
+                        const double sc = 0.5 * (sl + sr);
+                        if (!(sl * sr > 0.)) return 0.;
+                        const double m = fmin(fabs(sc), fmin(2. * fabs(sl), 2. * fabs(sr)));
+                        return sc > 0. ? m : -m;
+                    };
+                    v = c[0] + xo * mc(c[1], c[0], c[2]) + yo * mc(c[3], c[0], c[4]) +
+                        zo * mc(c[5], c[0], c[6]);
+                } else {
+                    v = c[0];
+                }
+                put(n, ok ? v : __builtin_nan(""));
             }
         }
     }

@@ -281,6 +281,14 @@ __attribute__((visibility("default"))) int shm_gw_spectrum(
     });
 }
 
+__attribute__((visibility("default"))) int shm_fill_coarse_patch(shm_ctx *ctx,
+                                                                int lev) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->fill_coarse_patch(lev);
+    });
+}
+
 __attribute__((visibility("default"))) int shm_upload_array(shm_ctx *ctx, int lev,
                                                            int comp,
                                                            const double *data,

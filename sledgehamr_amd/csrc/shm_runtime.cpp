@@ -487,9 +487,8 @@ void Runtime::init_from_inputs(const std::string &inputs,
     ko_ = S.with_dissipation;
     // Coarse/fine interpolation (LevelSynchronizer, level_synchronizer.cpp:27-47).
     pp.query("amr.interpolation_type", interp_type_);
-    if (interp_type_ != 0 && interp_type_ != 2)
-        throw Error(1, "amr.interpolation_type: 0 (piecewise constant) and 2 "
-                       "(quadratic) are supported");
+    if (interp_type_ != 0 && interp_type_ != 1 && interp_type_ != 2 && interp_type_ != 4)
+        throw Error(1, "amr.interpolation_type must be 0, 1, 2 or 4");
     if (ko_) {
         S.dissipation_order = S.nghost;
         pp.query("sim.dissipation_order", S.dissipation_order);
@@ -913,9 +912,11 @@ void Runtime::build_cf(int lev) {
     LevelState &ls = levels_[lev];
     const LevelState &cs = levels_[lev - 1];
     const int g = ls.L.g, Nc = cs.L.N;
-    const int s = interp_type_ == 2 ? 1 : 0;
+    const int s = interp_type_ == 0 ? 0 : interp_type_ == 4 ? 2 : 1;  // stencil reach
     ls.cf_segs.clear();
     ls.cf_total = 0;
+    ls.cf_full.clear();
+    ls.cf_full_total = 0;
     std::vector<int> start(1, 0), cand;
     for (int b = 0; b < (int)ls.L.boxes.size(); ++b) {
         const Box3 &bx = ls.L.boxes[b];
@@ -938,6 +939,18 @@ void Runtime::build_cf(int lev) {
             sg.start = ls.cf_total;
             ls.cf_total += (long long)sg.n[0] * sg.n[1] * sg.n[2];
             ls.cf_segs.push_back(sg);
+        }
+        {
+            HaloSeg sg{};
+            sg.dst_box = b;
+            sg.src_box = -1;
+            for (int a = 0; a < 3; ++a) {
+                sg.lo[a] = bx.lo[a] - g;
+                sg.n[a] = bx.n(a) + 2 * g;
+            }
+            sg.start = ls.cf_full_total;
+            ls.cf_full_total += (long long)sg.n[0] * sg.n[1] * sg.n[2];
+            ls.cf_full.push_back(sg);
         }
         // Coarse cells the interpolation of this box's ghost cells reads.
         int glo[3], ghi[3];
@@ -976,6 +989,7 @@ void Runtime::build_cf(int lev) {
         start.push_back((int)cand.size());
     }
     upload_vec(ls.dcf_segs, ls.cf_segs);
+    upload_vec(ls.dcf_full, ls.cf_full);
     upload_vec(ls.dcf_start, start);
     if (cand.empty()) cand.push_back(0);
     upload_vec(ls.dcf_cand, cand);
@@ -1019,37 +1033,71 @@ CfArgs Runtime::cf_args(int lev, double time) {
 }
 
 template <Prec P, int M>
-static void cf_dispatch(const CfArgs &a, bool quad, int grid, hipStream_t s) {
-    if (quad)
-        hipLaunchKernelGGL((cf_fill_kernel<P, M, true>), dim3(grid), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((cf_fill_kernel<P, M, false>), dim3(grid), dim3(256), 0, s, a);
+static void cf_dispatch(const CfArgs &a, int interp, int grid, hipStream_t s) {
+    switch (interp) {
+        case 0:
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 0>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        case 1:
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 1>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        case 2:
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 2>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        default:
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 4>), dim3(grid), dim3(256), 0, s, a);
+            break;
+    }
 }
 
 template <Prec P>
-static void cf_dispatch_mode(const CfArgs &a, int mode, bool quad, int grid,
+static void cf_dispatch_mode(const CfArgs &a, int mode, int interp, int grid,
                              hipStream_t s) {
     if (mode == kCfState)
-        cf_dispatch<P, kCfState>(a, quad, grid, s);
+        cf_dispatch<P, kCfState>(a, interp, grid, s);
     else if (mode == kCfK)
-        cf_dispatch<P, kCfK>(a, quad, grid, s);
+        cf_dispatch<P, kCfK>(a, interp, grid, s);
     else
-        cf_dispatch<P, kCfForce>(a, quad, grid, s);
+        cf_dispatch<P, kCfForce>(a, interp, grid, s);
 }
 
 void Runtime::cf_launch(CfArgs &a, int mode) {
     if (a.total == 0 || a.ncomp == 0) return;
     const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
-    const bool quad = interp_type_ == 2;
+    const int ip = interp_type_;
     EventPair *ep = timing_on ? timed_pair(1) : nullptr;
     if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
     switch (prec_) {
-        case Prec::F64: cf_dispatch_mode<Prec::F64>(a, mode, quad, grid, stream_); break;
-        case Prec::F32C: cf_dispatch_mode<Prec::F32C>(a, mode, quad, grid, stream_); break;
-        case Prec::F32C16: cf_dispatch_mode<Prec::F32C16>(a, mode, quad, grid, stream_); break;
+        case Prec::F64: cf_dispatch_mode<Prec::F64>(a, mode, ip, grid, stream_); break;
+        case Prec::F32C: cf_dispatch_mode<Prec::F32C>(a, mode, ip, grid, stream_); break;
+        case Prec::F32C16: cf_dispatch_mode<Prec::F32C16>(a, mode, ip, grid, stream_); break;
     }
     SHM_HIP(hipGetLastError());
     if (ep) SHM_HIP(hipEventRecord(ep->b, stream_));
+}
+
+void Runtime::fill_coarse_patch(int lev) {
+    if (!has_cf(lev)) throw Error(1, "FillCoarsePatch needs a defined level above 0");
+    LevelState &ls = level(lev);
+    CfArgs a = cf_args(lev, ls.t);
+    // InterpFromCoarseLevel receives only the first state GetLevelData
+    // returns (level_synchronizer.cpp:64, 83): between the two times, the
+    // old one, not a time interpolation.
+    if (a.cold && a.cnew) {
+        a.cnew = nullptr;
+        a.cnewlo = nullptr;
+    }
+    a.segs = ls.dcf_full.as<HaloSeg>();
+    a.nseg = (int)ls.cf_full.size();
+    a.total = ls.cf_full_total;
+    a.dst = ls.U[ls.cur].get();
+    a.dstlo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
+    for (int c0 = 0; c0 < ncomp_; c0 += 32) {
+        a.ncomp = std::min(32, ncomp_ - c0);
+        for (int n = 0; n < a.ncomp; ++n) a.comp[n] = c0 + n;
+        cf_launch(a, kCfState);
+    }
+    ls.ghosts_ok[ls.cur] = false;
 }
 
 void Runtime::cf_fill_state(int lev, double time, void *hi, void *lo) {

@@ -106,6 +106,10 @@ struct LevelState {
     std::vector<HaloSeg> cf_segs;
     DevMem dcf_segs, dcf_start, dcf_cand;
     long long cf_total = 0;
+    // Every cell of each ghosted box (FillCoarsePatch).
+    std::vector<HaloSeg> cf_full;
+    DevMem dcf_full;
+    long long cf_full_total = 0;
     bool cf_built = false;
     bool defined() const { return !L.boxes.empty(); }
 };
@@ -166,6 +170,9 @@ class Runtime {
 
     // Reference operations.
     void fill_rhs(int lev, double time, double weight, bool add);
+    /** LevelSynchronizer::FillCoarsePatch: every cell of grid_new[lev],
+     *  ghosts included, interpolated from level lev-1 at the level's time. */
+    void fill_coarse_patch(int lev);
     void advance(int lev);               // Integrator::Advance(lev)
     void create_shadow_level();          // Sledgehamr::CreateShadowLevel
     void te_tag(int lev, int flags, uint8_t *mask, long long *counts,

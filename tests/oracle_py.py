@@ -46,6 +46,7 @@ def _load(project: str) -> C.CDLL:
         "orc_download": (i, [vp, i, i, dp, i]),
         "orc_upload_old": (i, [vp, i, i, dp, d]),
         "orc_fill_patch": (i, [vp, i, d]),
+        "orc_fill_coarse_patch": (i, [vp, i]),
         "orc_download_fab": (i, [vp, i, i, i, dp]),
         "orc_spectrum": (i, [vp, i, dp, llp, ip]),
         "orc_projection": (i, [vp, i, i, i, dp, ip, ip]),
@@ -138,6 +139,10 @@ class Oracle:
     def fill_patch(self, lev, time):
         """FillPatch of grid_new[lev] at `time` (ghost cells included)."""
         self._ck(self.lib.orc_fill_patch(self.h, lev, time))
+
+    def fill_coarse_patch(self, lev):
+        """FillCoarsePatch: every cell of level lev from level lev-1."""
+        self._ck(self.lib.orc_fill_coarse_patch(self.h, lev))
 
     def fab(self, lev, box, comp, shape):
         """Ghosted box `box` of grid_new[lev], indexed [k, j, i]; `shape` is

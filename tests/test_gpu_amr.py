@@ -62,7 +62,7 @@ def _setup(inp, prec, levels):
     return sim, orc
 
 
-@pytest.mark.parametrize("interp", [0, 2])
+@pytest.mark.parametrize("interp", [0, 1, 2, 4])
 @pytest.mark.parametrize("order", [2, 3])
 @pytest.mark.parametrize("prec", ALLP)
 def test_fine_rhs_reads_interpolated_ghosts(order, interp, prec):
@@ -111,6 +111,60 @@ def test_subcycled_levels_match_oracle(integrator, prec, tol):
             assert err < t, (integrator, lev, c, err)
 
 
+# Level-2 boxes 6 level-2 cells inside level 1: the quartic reaches 2 coarse
+# cells beyond the parent, so it needs 3 level-1 cells of margin.
+LEV2_Q = [[22, 22, 22, 25, 25, 25], [28, 22, 22, 41, 25, 25]]
+
+
+@pytest.mark.parametrize("interp", [1, 4])
+@pytest.mark.parametrize("prec,tol", PRECS)
+def test_subcycled_levels_other_interpolations(interp, prec, tol):
+    inp = real_scalar(N=16, order=2, nghost=2, lam=1.0, L=16.0, cfl=0.25,
+                      integrator=10, **{"amr.max_refinement_levels": 2,
+                                        "amr.interpolation_type": interp})
+    sim, orc = _setup(inp, prec, {1: LEV1, 2: LEV2_Q})
+    for lev, bx in ((0, [[0, 0, 0, 15, 15, 15]]), (1, LEV1), (2, LEV2_Q)):
+        h = 16.0 / (16 * 2 ** lev)
+        for c, ph in ((0, 0.3), (1, 1.1)):
+            d = _smooth(bx, h, 16.0, ph) * (1.0 if c == 0 else 0.2)
+            sim.upload(lev, c, d)
+            orc.upload(lev, c, d)
+    sim.evolve(2)
+    orc.evolve(2)
+    for lev in range(3):
+        for c in range(2):
+            err = rel_linf(sim.download(lev, c), orc.download(lev, c))
+            t = FINE_MOMENTUM_TOL[prec] if (c == 1 and lev > 0) else tol
+            assert err < t, (interp, lev, c, err)
+
+
+@pytest.mark.parametrize("interp", [0, 1, 2, 4])
+@pytest.mark.parametrize("prec", ALLP)
+def test_fill_coarse_patch_matches_oracle(interp, prec):
+    """FillCoarsePatch of level 1 from random level-0 data at the level's
+    time; FP32-compensated states hold 32 significant bits (1e-8)."""
+    inp = real_scalar(N=16, order=2, nghost=2, lam=1.0, L=16.0,
+                      **{"amr.max_refinement_levels": 1,
+                         "amr.interpolation_type": interp})
+    sim, orc = _setup(inp, prec, {1: LEV1})
+    rng = np.random.default_rng(7)
+    for c in range(2):
+        d = rng.standard_normal(sim.ncells(0))
+        sim.upload(0, c, d)
+        orc.upload(0, c, d)
+    sim.fill_coarse_patch(1)
+    orc.fill_coarse_patch(1)
+    for c in range(2):
+        err = rel_linf(sim.download(1, c), orc.download(1, c))
+        assert err < (1e-13 if prec == shm.PREC_FP64 else 1e-8), (c, err)
+    # The level steps from the interpolated state like the oracle.
+    sim.advance(1, 1)
+    orc.advance(1, 1)
+    for c in range(2):
+        err = rel_linf(sim.download(1, c), orc.download(1, c))
+        assert err < (1e-12 if prec == shm.PREC_FP64 else 1e-6), (c, err)
+
+
 def test_ghost_fill_changes_the_fine_solution():
     """Control: with the coarse level shifted by a constant, a fine level's
     step changes near its boundary (the ghosts come from the coarse level)
@@ -143,7 +197,7 @@ def test_unnested_level_is_refused():
 
 
 def test_interpolation_types_checked():
-    for t in (1, 4, 7):
+    for t in (3, 5, -1):
         with pytest.raises(shm.ShmError):
             shm.Simulation("RealScalar", real_scalar(
                 N=16, **{"amr.interpolation_type": t}))

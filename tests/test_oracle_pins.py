@@ -327,6 +327,122 @@ def test_coarse_fine_fill_time_windows(time, want):
     assert np.all(a[~_inside(boxes[0], i, j, k)] == want)
 
 
+def _children_in_ghosts(box, g):
+    """Coarse cells whose 8 children are all ghost cells of the fine box."""
+    lo = [(box[a] - g) // 2 for a in range(3)]
+    hi = [(box[a + 3] + g + 1) // 2 - 1 for a in range(3)]
+    out = []
+    for c in itertools.product(*(range(lo[a], hi[a] + 1) for a in range(3))):
+        kids = [(2 * c[0] + a, 2 * c[1] + b, 2 * c[2] + d)
+                for a in (0, 1) for b in (0, 1) for d in (0, 1)]
+        if all(not (box[0] <= x <= box[3] and box[1] <= y <= box[4] and box[2] <= z <= box[5])
+               for x, y, z in kids):
+            out.append(c)
+    return out
+
+
+def _fab_at(a, box, g, i, j, k):
+    return a[k - box[2] + g, j - box[1] + g, i - box[0] + g]
+
+
+@pytest.mark.parametrize("interp", [1, 4])
+def test_coarse_fine_fill_conservative(interp):
+    """Types 1 and 4 are conservative: the 8 children of a coarse cell
+    average to it (random coarse data)."""
+    box = [8, 8, 8, 15, 15, 15]
+    orc = _cf_oracle([box], interp=interp)
+    c = np.random.default_rng(9).standard_normal(16 ** 3)
+    orc.upload(0, 0, c)
+    orc.fill_patch(1, orc.time(0))
+    a = _ghosted(orc, [box], 2)[0][0]
+    cc = c.reshape(16, 16, 16)
+    cells = _children_in_ghosts(box, 2)
+    assert len(cells) > 100
+    for ci, cj, ck in cells:
+        kids = [_fab_at(a, box, 2, 2 * ci + p, 2 * cj + q, 2 * ck + r)
+                for p in (0, 1) for q in (0, 1) for r in (0, 1)]
+        assert abs(np.mean(kids) - cc[ck, cj, ci]) < 1e-14
+
+
+def test_coarse_fine_fill_limited_linear():
+    """Type 1 is exact on linear data and keeps a one-cell spike without
+    overshoot: slopes vanish at the extremum and next to it."""
+    box = [8, 8, 8, 15, 15, 15]
+    orc = _cf_oracle([box], interp=1)
+    lin = lambda x, y, z: 0.4 + 0.1 * x - 0.05 * y + 0.02 * z
+    orc.upload(0, 0, _box_cells([([0, 0, 0, 15, 15, 15], 1.0)], lin))
+    orc.fill_patch(1, orc.time(0))
+    a, i, j, k = _ghosted(orc, [box], 2)[0]
+    ghost = ~_inside(box, i, j, k)
+    want = lin((i + 0.5) * 0.5, (j + 0.5) * 0.5, (k + 0.5) * 0.5)
+    assert np.max(np.abs(a - want)[ghost]) < 1e-14
+    spike = np.zeros((16, 16, 16))
+    spike[5, 5, 3] = 1.0   # coarse (3, 5, 5): its children are ghost cells
+    orc.upload(0, 0, spike.ravel())
+    orc.fill_patch(1, orc.time(0))
+    a = _ghosted(orc, [box], 2)[0][0]
+    assert a[~_inside(box, i, j, k)].max() == 1.0
+    for p, q, r in itertools.product((0, 1), repeat=3):
+        assert _fab_at(a, box, 2, 6 + p, 10 + q, 10 + r) == 1.0
+    assert _fab_at(a, box, 2, 6, 12, 12) == 0.0
+
+
+def test_coarse_fine_fill_quartic_exact():
+    """Type 4 maps the coarse cell averages of a tensor-product quartic to
+    the fine cell averages of the same quartic."""
+    box = [8, 8, 8, 15, 15, 15]
+    orc = _cf_oracle([box], interp=4)
+    coef = [(0.3, -0.2, 0.05, 0.01, -0.002), (1.0, 0.1, -0.03, 0.004, 0.0005),
+            (0.5, 0.2, 0.02, -0.003, 0.0002)]
+
+    def cell_avg(cf, lo, h):
+        # average of sum cf[m] x^m over [lo, lo + h]
+        return sum(c * ((lo + h) ** (m + 1) - lo ** (m + 1)) / ((m + 1) * h)
+                   for m, c in enumerate(cf))
+
+    n = np.arange(16)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    orc.upload(0, 0, (cell_avg(coef[0], i * 1.0, 1.0) * cell_avg(coef[1], j * 1.0, 1.0) *
+                      cell_avg(coef[2], k * 1.0, 1.0)).ravel())
+    orc.fill_patch(1, orc.time(0))
+    a, i, j, k = _ghosted(orc, [box], 2)[0]
+    ghost = ~_inside(box, i, j, k)
+    want = (cell_avg(coef[0], i * 0.5, 0.5) * cell_avg(coef[1], j * 0.5, 0.5) *
+            cell_avg(coef[2], k * 0.5, 0.5))
+    assert np.max(np.abs(a - want)[ghost]) < 1e-12 * np.max(np.abs(want))
+    # The quadratic type is not exact here (negative control).
+    orc2 = _cf_oracle([box], interp=2)
+    orc2.upload(0, 0, orc.download(0, 0))
+    orc2.fill_patch(1, orc2.time(0))
+    a2 = _ghosted(orc2, [box], 2)[0][0]
+    assert np.max(np.abs(a2 - want)[ghost]) > 1e-6
+
+
+def test_fill_coarse_patch():
+    """FillCoarsePatch (level_synchronizer.cpp:56-88) interpolates every cell
+    of the level, valid cells included (exact for a quadratic), and between
+    the coarse level's two times takes the old state alone (the first of
+    GetLevelData), where FillPatch would interpolate in time."""
+    box = [8, 8, 8, 15, 15, 15]
+    orc = _cf_oracle([box])
+    q = _quad([0.3, 0.2, -0.1, 0.05, 0.01, -0.02, 0.015, 0.007, -0.004, 0.003])
+    orc.upload(0, 0, _box_cells([([0, 0, 0, 15, 15, 15], 1.0)], q))
+    orc.fill_coarse_patch(1)
+    a, i, j, k = _ghosted(orc, [box], 2)[0]
+    assert np.max(np.abs(a - q((i + 0.5) * 0.5, (j + 0.5) * 0.5, (k + 0.5) * 0.5))) < 1e-12
+    orc.upload_old(0, 0, np.full(16 ** 3, 1.0), 0.0)
+    orc.upload(0, 0, np.full(16 ** 3, 3.0))
+    orc.set_time(0, 1.0)
+    for t1, want in ((0.5, 1.0), (1.0, 3.0), (0.0, 1.0)):
+        orc.set_time(1, t1)
+        orc.fill_coarse_patch(1)
+        assert np.all(_ghosted(orc, [box], 2)[0][0] == want), t1
+    orc.set_time(1, 0.5)
+    orc.fill_patch(1, 0.5)   # FillPatch interpolates in time
+    a = _ghosted(orc, [box], 2)[0][0]
+    assert np.all(a[~_inside(box, i, j, k)] == 2.0)
+
+
 def test_coarse_fine_fill_requires_nesting():
     """A level-2 box at the corner of its level-1 box needs level-1 cells
     outside every level-1 box for its ghost cells."""
