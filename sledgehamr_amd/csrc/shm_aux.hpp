@@ -388,13 +388,18 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
             return -1;
         };
         // Coarse value at element ce of component comp, weighted in time.
+        const double wo = a.rec[0], wn = a.rec[1];
+        const bool use_old = a.rec[2] != 0., use_new = a.rec[3] != 0.;
+        const int ns = a.rec[4] != 0. ? 1 : 0;
+        const void *cnew = a.cs[ns], *cnewlo = a.cslo[ns];
+        const void *cold = a.cs[ns ^ 1], *coldlo = a.cslo[ns ^ 1];
         auto cval = [&](int comp, long long ce) -> double {
             const long long e = (long long)comp * a.ccstride + ce;
             double v = 0.;
-            if (a.cnew) v = load_state_at<T>(a.cnew, a.cnewlo, e);
-            if (a.cold) {
-                const double vo = load_state_at<T>(a.cold, a.coldlo, e);
-                v = a.cnew ? a.wo * vo + a.wn * v : vo;
+            if (use_new) v = load_state_at<T>(cnew, cnewlo, e);
+            if (use_old) {
+                const double vo = load_state_at<T>(cold, coldlo, e);
+                v = use_new ? wo * vo + wn * v : vo;
             }
             return v;
         };

@@ -66,9 +66,11 @@ struct CfArgs {
     const int *cand;           // coarse boxes the box's coarse stencil touches
     int Nc;                    // coarse domain cells per axis
     long long fcstride, ccstride;
-    const void *cold, *coldlo; // coarse state at t_old (null: weight 0)
-    const void *cnew, *cnewlo; // coarse state at t_new (null: weight 0)
-    double wo, wn;
+    const void *cs[2], *cslo[2];  // the coarse level's two state slots
+    // Record read on the device (so captured steps take fresh values):
+    // [0] weight of the old state, [1] of the new one, [2] old state used,
+    // [3] new state used, [4] slot of the new state.
+    const double *rec;
     int ncomp;
     int comp[32];              // interpolated component (fine = coarse index)
     void *dst, *dstlo;         // state (kCfState), k (kCfK), F_m (kCfForce)

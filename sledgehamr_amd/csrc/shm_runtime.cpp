@@ -996,7 +996,54 @@ void Runtime::build_cf(int lev) {
     ls.cf_built = true;
 }
 
-CfArgs Runtime::cf_args(int lev, double time) {
+void Runtime::cf_record(double *rec, int lev, double time, bool first_state_only) const {
+    const LevelState &cs = levels_[lev - 1];
+    std::fill(rec, rec + 8, 0.);
+    rec[4] = cs.cur;
+    // GetLevelData (level_synchronizer.cpp:613-632): the new or the old
+    // state within 1e-3 of their time difference, else both, linear in time.
+    const double teps = std::fabs(cs.t - cs.t_old) * 1.e-3;
+    if (time > cs.t - teps && time < cs.t + teps) {
+        rec[3] = 1.;
+    } else if (time > cs.t_old - teps && time < cs.t_old + teps) {
+        rec[2] = 1.;
+    } else if (first_state_only) {
+        // InterpFromCoarseLevel in FillCoarsePatch receives only the first
+        // state GetLevelData returns (level_synchronizer.cpp:64, 83).
+        rec[2] = 1.;
+    } else {
+        rec[2] = rec[3] = 1.;
+        rec[0] = (cs.t - time) / (cs.t - cs.t_old);
+        rec[1] = (time - cs.t_old) / (cs.t - cs.t_old);
+    }
+}
+
+static const double *push_params(double *host_ring, std::vector<hipEvent_t> &evs,
+                                 int &slot, double *dev_ring, hipStream_t s,
+                                 const std::vector<double> &blk);
+
+const double *Runtime::push_record(const std::vector<double> &blk, int kind, double time) {
+    if (capture_) {
+        // A copy node from the graph's pinned record into its device record;
+        // the host fills the pinned record before every replay.
+        const int i = (int)capture_->recs.size();
+        if (i >= capture_->nrec) throw Error(5, "too many parameter records in a captured step");
+        int tidx = -1;
+        for (size_t q = 0; q < step_times_.size() && tidx < 0; ++q)
+            if (step_times_[q] == time) tidx = (int)q;
+        if (tidx < 0) throw Error(5, "record time outside the captured step's time table");
+        capture_->recs.push_back({kind, tidx});
+        double *h = capture_->host + (size_t)i * kPrmBlock;
+        double *d = capture_->dev.as<double>() + (size_t)i * kPrmBlock;
+        std::copy(blk.begin(), blk.end(), h);
+        SHM_HIP(hipMemcpyAsync(d, h, sizeof(double) * kPrmBlock, hipMemcpyHostToDevice,
+                               stream_));
+        return d;
+    }
+    return push_params(prm_host_, prm_ev_, prm_slot_, prm_dev_.as<double>(), stream_, blk);
+}
+
+CfArgs Runtime::cf_args(int lev, double time, bool first_state_only) {
     LevelState &ls = levels_[lev];
     if (!ls.cf_built) build_cf(lev);
     LevelState &cs = levels_[lev - 1];
@@ -1011,24 +1058,13 @@ CfArgs Runtime::cf_args(int lev, double time) {
     a.Nc = cs.L.N;
     a.fcstride = ls.L.cstride;
     a.ccstride = cs.L.cstride;
-    // GetLevelData (level_synchronizer.cpp:613-632): the new or the old
-    // state within 1e-3 of their time difference, else both, linear in time.
-    const int nw = cs.cur, od = cs.cur ^ 1;
-    const double teps = std::fabs(cs.t - cs.t_old) * 1.e-3;
-    if (time > cs.t - teps && time < cs.t + teps) {
-        a.cnew = cs.U[nw].get();
-        a.cnewlo = comp() ? cs.Ulo[nw].get() : nullptr;
-    } else if (time > cs.t_old - teps && time < cs.t_old + teps) {
-        a.cold = cs.U[od].get();
-        a.coldlo = comp() ? cs.Ulo[od].get() : nullptr;
-    } else {
-        a.cnew = cs.U[nw].get();
-        a.cnewlo = comp() ? cs.Ulo[nw].get() : nullptr;
-        a.cold = cs.U[od].get();
-        a.coldlo = comp() ? cs.Ulo[od].get() : nullptr;
-        a.wo = (cs.t - time) / (cs.t - cs.t_old);
-        a.wn = (time - cs.t_old) / (cs.t - cs.t_old);
+    for (int q = 0; q < 2; ++q) {
+        a.cs[q] = cs.U[q].get();
+        a.cslo[q] = comp() ? cs.Ulo[q].get() : nullptr;
     }
+    std::vector<double> rec(kPrmBlock, 0.);
+    cf_record(rec.data(), lev, time, first_state_only);
+    a.rec = push_record(rec, first_state_only ? 2 : 1, time);
     return a;
 }
 
@@ -1079,14 +1115,9 @@ void Runtime::cf_launch(CfArgs &a, int mode) {
 void Runtime::fill_coarse_patch(int lev) {
     if (!has_cf(lev)) throw Error(1, "FillCoarsePatch needs a defined level above 0");
     LevelState &ls = level(lev);
-    CfArgs a = cf_args(lev, ls.t);
-    // InterpFromCoarseLevel receives only the first state GetLevelData
-    // returns (level_synchronizer.cpp:64, 83): between the two times, the
-    // old one, not a time interpolation.
-    if (a.cold && a.cnew) {
-        a.cnew = nullptr;
-        a.cnewlo = nullptr;
-    }
+    // Between the coarse level's two times the old state alone, not a time
+    // interpolation (the first state GetLevelData returns).
+    CfArgs a = cf_args(lev, ls.t, true);
     a.segs = ls.dcf_full.as<HaloSeg>();
     a.nseg = (int)ls.cf_full.size();
     a.total = ls.cf_full_total;
@@ -1138,23 +1169,13 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                         double t0) {
     const int old = ls.cur ^ 1, nw = ls.cur;
     std::vector<double> blk(kPrmBlock);
-    auto prm = [&](double time) -> const double * {
-        if (capture_) {
-            // Captured step: a copy node from the graph's pinned record into
-            // its device record; the host fills the pinned record per replay.
-            const int i = capture_stage_++;
-            if (i >= capture_->nstage) throw Error(5, "stage count changed in capture");
-            double *h = capture_->host + (size_t)i * kPrmBlock;
-            double *d = capture_->dev.as<double>() + (size_t)i * kPrmBlock;
-            SHM_HIP(hipMemcpyAsync(d, h, sizeof(double) * kPrmBlock,
-                                   hipMemcpyHostToDevice, stream_));
-            return d;
-        }
-        fill_params(blk.data(), time, dt, dx, lev);
-        return push_params(prm_host_, prm_ev_, prm_slot_,
-                           prm_dev_.as<double>(), stream_, blk);
-    };
     const std::vector<double> st = stage_times(t0, dt);
+    step_times_ = st;
+    step_times_.push_back(t0 + dt);
+    auto prm = [&](double time) -> const double * {
+        fill_params(blk.data(), time, dt, dx, lev);
+        return push_record(blk, 0, time);
+    };
     const int npair = kt_.nfields + (gw_ ? 6 : 0);
     std::vector<int> pairs(npair);
     for (int p = 0; p < npair; ++p) pairs[p] = p;
@@ -1333,9 +1354,7 @@ void Runtime::release_graphs() {
  *  lazily built table exists), replayed afterwards.  Returns false when the
  *  step has to run eagerly. */
 bool Runtime::replay_step(LevelState &ls, int lev, double dt, double dx, double t0) {
-    // Coarse/fine fills take time weights and coarse states that change from
-    // step to step, so levels with coarse/fine ghost cells run eagerly.
-    if (!graphs || timing_on || lev < 0 || has_cf(lev)) return false;
+    if (!graphs || timing_on || lev < 0) return false;
     auto w = warm_.find(lev);
     if (w == warm_.end() || w->second != epoch_) return false;
     const std::vector<double> st = stage_times(t0, dt);
@@ -1347,15 +1366,16 @@ bool Runtime::replay_step(LevelState &ls, int lev, double dt, double dx, double 
             g.exec = nullptr;
         }
         if (!g.host) {
-            g.nstage = (int)st.size();
-            SHM_HIP(hipHostMalloc((void **)&g.host, sizeof(double) * kPrmBlock * g.nstage,
+            // Stage parameters plus the coarse/fine fills of a step.
+            g.nrec = 4 * (int)st.size() + 8;
+            SHM_HIP(hipHostMalloc((void **)&g.host, sizeof(double) * kPrmBlock * g.nrec,
                                   hipHostMallocDefault));
-            g.dev.alloc(sizeof(double) * kPrmBlock * g.nstage);
+            g.dev.alloc(sizeof(double) * kPrmBlock * g.nrec);
             SHM_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
         }
         hipGraph_t graph = nullptr;
         capture_ = &g;
-        capture_stage_ = 0;
+        g.recs.clear();
         SHM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
             integrate(ls, lev, dt, dx, t0);
@@ -1376,8 +1396,16 @@ bool Runtime::replay_step(LevelState &ls, int lev, double dt, double dx, double 
     }
     // The previous replay of this graph has read its pinned records.
     SHM_HIP(hipEventSynchronize(g.done));
-    for (int i = 0; i < g.nstage; ++i)
-        fill_params(g.host + (size_t)i * kPrmBlock, st[i], dt, dx, lev);
+    std::vector<double> times = st;
+    times.push_back(t0 + dt);
+    for (size_t i = 0; i < g.recs.size(); ++i) {
+        double *h = g.host + i * kPrmBlock;
+        const double time = times[g.recs[i].second];
+        if (g.recs[i].first == 0)
+            fill_params(h, time, dt, dx, lev);
+        else
+            cf_record(h, lev, time, g.recs[i].first == 2);
+    }
     SHM_HIP(hipGraphLaunch(g.exec, stream_));
     SHM_HIP(hipEventRecord(g.done, stream_));
     ++graph_replays;

@@ -133,9 +133,13 @@ struct LaunchCfg {
 struct StepGraph {
     hipGraphExec_t exec = nullptr;
     hipEvent_t done = nullptr;   // recorded after the last replay
-    double *host = nullptr;      // pinned, nstage parameter records
-    DevMem dev;                  // device copies read by the stage kernels
-    int nstage = 0;
+    double *host = nullptr;      // pinned parameter records (capacity nrec)
+    DevMem dev;                  // device copies read by the kernels
+    int nrec = 0;
+    // What fills record i before a replay: kind 0 stage parameters, kind 1
+    // coarse/fine time weights (2: first state only); at entry tidx of the
+    // step's time table.
+    std::vector<std::pair<int, int>> recs;
     long epoch = -1;             // layout / launch-shape generation captured
 };
 
@@ -260,7 +264,14 @@ class Runtime {
     void build_cf(int lev);
     /** Arguments of a coarse/fine fill of level lev at `time` (layout,
      *  coarse states and their time weights); the caller adds the target. */
-    CfArgs cf_args(int lev, double time);
+    CfArgs cf_args(int lev, double time, bool first_state_only = false);
+    /** Coarse/fine record of level lev at `time` (see CfArgs::rec). */
+    void cf_record(double *rec, int lev, double time, bool first_state_only) const;
+    /** A parameter record for the device: the next slot of the ring, or,
+     *  while a step is captured, the graph's next record (filled again by
+     *  the host before every replay from (kind, time index)). */
+    const double *push_record(const std::vector<double> &blk, int kind, double time);
+    std::vector<double> step_times_;  // stage times and t0 + dt of the step
     void cf_launch(CfArgs &a, int mode);
     /** kCfState fill of every component of a state of level lev. */
     void cf_fill_state(int lev, double time, void *hi, void *lo);
@@ -273,7 +284,6 @@ class Runtime {
     // Parameter source while a step is being captured: stage i reads the
     // graph's device record i, copied from its pinned record inside the graph.
     StepGraph *capture_ = nullptr;
-    int capture_stage_ = 0;
     std::map<std::pair<int, int>, StepGraph> graphs_;  // (level, old slot)
     std::map<int, long> warm_;   // level -> epoch of its last eager step
     long epoch_ = 0;

@@ -131,3 +131,31 @@ def amr_boxes(seed, nbox=6, margin=4):
             lo2 = tuple(2 * v + margin for v in lo)
             lev2.append([*lo2, *(v + e2 - 1 for v in lo2)])
     return lev1, lev2
+
+
+def smooth_boxes(boxes, h, L, phase):
+    """Periodic smooth field at the cell centres of `boxes` (spacing h),
+    box by box in the order of `boxes`, x fastest."""
+    out = []
+    for b in boxes:
+        k, j, i = np.meshgrid(*(np.arange(b[a], b[a + 3] + 1) for a in (2, 1, 0)),
+                              indexing="ij")
+        x, y, z = ((i + 0.5) * h, (j + 0.5) * h, (k + 0.5) * h)
+        w = 2 * np.pi / L
+        f = (0.3 * np.sin(w * x + phase) * np.cos(2 * w * y) +
+             0.2 * np.cos(w * z - 0.5 * phase) + 0.1 * np.sin(3 * w * (x + y + z)))
+        out.append(f.ravel())
+    return np.concatenate(out)
+
+
+# Level-1 boxes of a 32^3 level-1 domain (coarse 16^3): two touching boxes
+# (ghosts covered by the neighbour and interpolated next to it) and one at
+# z = 0 whose ghosts wrap through the periodic boundary.
+AMR_LEV1 = [[8, 8, 8, 15, 15, 15], [16, 8, 8, 23, 15, 15], [4, 20, 0, 11, 27, 7]]
+# Level-2 boxes (64^3 domain), 4 cells inside level-1 boxes 0 and 1 (the
+# second spans both); quadratic interpolation with 2 ghost cells needs 2
+# level-1 cells of margin.
+AMR_LEV2 = [[20, 20, 20, 27, 27, 27], [30, 20, 20, 43, 27, 27]]
+# Level-2 boxes 6 level-2 cells inside level 1: the quartic reaches 2 coarse
+# cells beyond the parent, so it needs 3 level-1 cells of margin.
+AMR_LEV2_Q = [[22, 22, 22, 25, 25, 25], [28, 22, 22, 41, 25, 25]]

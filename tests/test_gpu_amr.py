@@ -21,7 +21,11 @@ import numpy as np
 import pytest
 
 import sledgehamr_amd as shm
+from cases import AMR_LEV1 as LEV1
+from cases import AMR_LEV2 as LEV2
+from cases import AMR_LEV2_Q as LEV2_Q
 from cases import real_scalar, rel_linf
+from cases import smooth_boxes as _smooth
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -29,29 +33,6 @@ pytestmark = pytest.mark.gpu
 ALLP = [shm.PREC_FP64, shm.PREC_FP32C, shm.PREC_FP32C16]
 PRECS = [(shm.PREC_FP64, 1e-12), (shm.PREC_FP32C16, 1e-6)]
 FINE_MOMENTUM_TOL = {shm.PREC_FP64: 1e-12, shm.PREC_FP32C16: 2e-6}
-# Level-1 boxes of a 32^3 level-1 domain: two touching boxes (ghosts covered
-# by the neighbour and interpolated next to it) and one at z = 0 whose
-# ghosts wrap through the periodic boundary.
-LEV1 = [[8, 8, 8, 15, 15, 15], [16, 8, 8, 23, 15, 15], [4, 20, 0, 11, 27, 7]]
-# Level-2 boxes (64^3 domain), 4 cells inside level-1 boxes 0 and 1 (the
-# second spans both); quadratic interpolation with 2 ghost cells needs 2
-# level-1 cells of margin.
-LEV2 = [[20, 20, 20, 27, 27, 27], [30, 20, 20, 43, 27, 27]]
-
-
-def _smooth(boxes, h, L, phase):
-    """Periodic smooth field at the cell centres of `boxes` (spacing h)."""
-    out = []
-    for b in boxes:
-        k, j, i = np.meshgrid(*(np.arange(b[a], b[a + 3] + 1) for a in (2, 1, 0)),
-                              indexing="ij")
-        x, y, z = ((i + 0.5) * h, (j + 0.5) * h, (k + 0.5) * h)
-        w = 2 * np.pi / L
-        f = (0.3 * np.sin(w * x + phase) * np.cos(2 * w * y) +
-             0.2 * np.cos(w * z - 0.5 * phase) + 0.1 * np.sin(3 * w * (x + y + z)))
-        out.append(f.ravel())
-    return np.concatenate(out)
-
 
 def _setup(inp, prec, levels):
     sim = shm.Simulation("RealScalar", inp, precision=prec)
@@ -109,11 +90,6 @@ def test_subcycled_levels_match_oracle(integrator, prec, tol):
             err = rel_linf(sim.download(lev, c), orc.download(lev, c))
             t = FINE_MOMENTUM_TOL[prec] if (c == 1 and lev > 0) else tol
             assert err < t, (integrator, lev, c, err)
-
-
-# Level-2 boxes 6 level-2 cells inside level 1: the quartic reaches 2 coarse
-# cells beyond the parent, so it needs 3 level-1 cells of margin.
-LEV2_Q = [[22, 22, 22, 25, 25, 25], [28, 22, 22, 41, 25, 25]]
 
 
 @pytest.mark.parametrize("interp", [1, 4])

@@ -1,11 +1,13 @@
 """Captured level steps (HIP graphs) against eager steps: every replay,
 with host work (downloads) between replays, gives the eager result bit for
-bit; so does a sub-cycled multi-level evolve."""
+bit; so does a sub-cycled multi-level evolve, with and without coarse/fine
+ghost cells (whose fill weights and coarse slot are refreshed per replay)."""
 import numpy as np
 import pytest
 
 import sledgehamr_amd as shm
-from cases import amr_boxes, complex_scalar, gaussian_random_field, real_scalar
+from cases import (AMR_LEV1, AMR_LEV2, AMR_LEV2_Q, amr_boxes, complex_scalar,
+                   gaussian_random_field, real_scalar, smooth_boxes)
 
 pytestmark = pytest.mark.gpu
 
@@ -79,3 +81,34 @@ def test_graph_subcycled_evolve_matches_eager():
     for lev in range(3):
         assert np.array_equal(a.level_tag_counts(lev), b.level_tag_counts(lev))
     assert b._get_info()["graph_replays"] > 0
+
+
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("integrator,interp", [(10, 2), (21, 2), (22, 2), (11, 2),
+                                               (10, 4), (10, 1), (21, 0)])
+def test_graph_coarse_fine_levels_match_eager(integrator, interp, prec):
+    """Three levels whose fine levels fill coarse/fine ghost cells at every
+    stage: three coarse steps (level 1 steps 6 times, level 2 12 times), all
+    but each level's first step replayed from graphs, equal to eager steps
+    bit for bit."""
+    inp = real_scalar(N=16, order=2, nghost=2, lam=1.0, L=16.0, cfl=0.25,
+                      integrator=integrator,
+                      **{"amr.max_refinement_levels": 2,
+                         "amr.interpolation_type": interp})
+    lev2 = AMR_LEV2_Q if interp == 4 else AMR_LEV2
+    a, b = _pair("RealScalar", inp, prec)
+    for s in (a, b):
+        s.set_boxes(1, AMR_LEV1)
+        s.set_boxes(2, lev2)
+        for lev, bx in ((0, [[0, 0, 0, 15, 15, 15]]), (1, AMR_LEV1), (2, lev2)):
+            for c, ph in ((0, 0.3), (1, 1.1)):
+                s.upload(lev, c, smooth_boxes(bx, 16.0 / (16 * 2 ** lev), 16.0, ph)
+                         * (1.0 if c == 0 else 0.2))
+    for n in range(3):
+        a.evolve(1)
+        b.evolve(1)
+        for lev in range(3):
+            for c in range(2):
+                assert np.array_equal(a.download(lev, c), b.download(lev, c)), (n, lev, c)
+    assert a._get_info()["graph_replays"] == 0
+    assert b._get_info()["graph_replays"] == 2 + 5 + 11
