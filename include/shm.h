@@ -138,6 +138,43 @@ int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
  * level's two times the old state is used, as the reference does. */
 int shm_fill_coarse_patch(shm_ctx *ctx, int lev);
 
+/* LocalRegrid::DetermineNewBoxArray with TagAndMeasure / CheckBorders
+ * (local_regrid/local_regrid.cpp:511-680) for tagging level lev, on the
+ * device: the blocks (amr.blocking_factor cubes) of level lev+1 that tagged
+ * cells within amr.n_error_buf of a coarse/fine boundary ask for, and the
+ * shortest distance of a tagged cell to such a boundary (-1: none measured).
+ * mask: one byte per valid cell of lev (box-major, x fastest), or NULL for
+ * the mask of the level's last tagging at its time.  blocks receives 3 ints
+ * (i, j, k) per block in (i, j, k) order; *nblock their number. */
+int shm_regrid_select(shm_ctx *ctx, int lev, const unsigned char *mask,
+                      int *blocks, int maxblock, int *nblock,
+                      double *min_distance);
+
+/* LocalRegrid::AddBoxes (local_regrid.cpp:763-828): level lev keeps its
+ * boxes and data and gains nbox boxes after them (lo[3], hi[3] each, no
+ * overlap), filled as FillPatch does at the level's new and old times. */
+int shm_add_boxes(shm_ctx *ctx, int lev, const int *boxes, int nbox);
+
+/* Sledgehamr::RemakeLevel (sledgehamr.cpp:173-190) on a new box list, state
+ * FillPatch-ed at the level's time from its previous data and level lev-1;
+ * the local regrid's counters reset as after a global regrid
+ * (LocalRegrid::DidGlobalRegrid, local_regrid.cpp:34-46). */
+int shm_remake_level(shm_ctx *ctx, int lev, const int *boxes, int nbox);
+
+/* LocalRegrid::DoAttemptRegrid (local_regrid.cpp:407-440) from tagging level
+ * lev, using the masks of each level's last tagging: new blocks, nesting
+ * fixes, volume thresholds and vetoes, then AddBoxes.  *outcome: 0 a global
+ * regrid is wanted (not done here), 1 boxes added (possibly none), 2 the
+ * regrid is delayed.  shm_evolve calls it at scheduled regrids when the
+ * input shm.local_regrid = 1. */
+int shm_local_regrid(shm_ctx *ctx, int lev, int *outcome);
+
+/* Counters of the local regrid: stats[4] = attempts, global regrids wanted,
+ * consecutive local regrids (nregrids), veto level; min_distance[nlev]: the
+ * shortest tagged-cell distance per level of the last attempt (-1: none). */
+int shm_regrid_stats(shm_ctx *ctx, long long *stats, double *min_distance,
+                     int nlev);
+
 /* Initial-state arrays in the reference's orders (the HDF5 reader's data):
  * layout 1: the whole level, ind = i N^2 + j N + k (FillLevel::FromArray,
  * fill_level.cpp:205-223); layout 2: one chunk per box in box order,

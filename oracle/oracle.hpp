@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -41,6 +42,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #ifdef _OPENMP
@@ -366,6 +368,12 @@ struct Config {
     bool shadow_hierarchy = false;
     bool independent = false;  // level-0 boxes are separate periodic domains
     double regrid_dt = DBL_MAX;
+    // Local regrid (local_regrid.h:97-122; blocking factor per level, the
+    // last value repeated).
+    std::vector<int> bf;
+    int n_error_buf = 1, max_local_regrids = 10;
+    double vt_strong = 1.05, vt_weak = 1.1;
+    bool local_regrid = false;  // shm.local_regrid: regrid at scheduled times
     int interp_type = 2;  // amr.interpolation_type: 0 piecewise constant,
                           // 1 limited linear, 2 quadratic, 4 quartic
 };
@@ -477,6 +485,21 @@ class Sim {
                 cfg.b = {14. / 336., 125. / 336., 162. / 336., 35. / 336.};
                 break;
         }
+        {
+            // Checked here when given, else where the local regrid uses it.
+            std::vector<int> b{8};
+            const bool given = pp.queryarr("amr.blocking_factor", b);
+            for (int lev = 0; lev <= cfg.max_level; ++lev) {
+                const int v = b[std::min<size_t>(lev, b.size() - 1)];
+                if (given) check_bf(v, cfg.N0 << lev);
+                cfg.bf.push_back(v);
+            }
+        }
+        pp.query("amr.n_error_buf", cfg.n_error_buf);
+        pp.query("amr.max_local_regrids", cfg.max_local_regrids);
+        pp.query("amr.volume_threshold_strong", cfg.vt_strong);
+        pp.query("amr.volume_threshold_weak", cfg.vt_weak);
+        pp.query("shm.local_regrid", cfg.local_regrid);
         for (int lev = 0; lev <= cfg.max_level; ++lev) {
             dimN.push_back(cfg.N0 * (1 << lev));
             dx.push_back(cfg.L / (double)dimN[lev]);
@@ -484,6 +507,8 @@ class Sim {
         }
         grid_new.resize(cfg.max_level + 1);
         grid_old.resize(cfg.max_level + 1);
+        lr.no_local.assign(cfg.max_level + 1, 0);
+        lr.do_global.assign(cfg.max_level + 1, 0);
         prj.Init();
         int mgs = cfg.N0;
         pp.query("amr.max_grid_size", mgs);
@@ -592,7 +617,7 @@ class Sim {
      *  lev-1 at the level's time (InterpFromCoarseLevel). */
     void fill_coarse_patch(int lev) {
         if (lev < 1) throw std::runtime_error("FillCoarsePatch needs a level above 0");
-        fill_coarse_ghosts(grid_new[lev], lev, grid_new[lev].t, true);
+        fill_coarse_ghosts(grid_new[lev], lev, grid_new[lev].t, true, true);
     }
 
     /** Weights of the level-lev states at `time`: GetLevelData picks the new
@@ -633,11 +658,12 @@ class Sim {
      *  Coarse values are the time-weighted states of level lev-1 at the
      *  periodic image of each coarse cell; a coarse cell that no box of
      *  level lev-1 holds is an error (the level is not properly nested). */
-    void fill_coarse_ghosts(MultiFab &m, int lev, double time, bool all = false) {
+    void fill_coarse_ghosts(MultiFab &m, int lev, double time, bool all = false,
+                            bool first_only = false) {
         MultiFab &New = grid_new[lev - 1], &Old = grid_old[lev - 1];
         double wo, wn;
         time_weights(lev - 1, time, wo, wn);
-        if (all && wo != 0. && wn != 0.) {
+        if (first_only && wo != 0. && wn != 0.) {
             // FillCoarsePatch hands InterpFromCoarseLevel only the first state
             // of GetLevelData (level_synchronizer.cpp:64, 83): the old one.
             wo = 1.;
@@ -1275,7 +1301,7 @@ class Sim {
         if (istep % 2 == 0 && lev > 0) return;
         const double next = lev > 0 ? time + 3. * dt[lev] : time + 2. * dt[lev];
         if (lev + 1 > 0 && !prj.CreateLevelIf(lev + 1, next)) return;
-        if (next <= last_regrid[lev] + regrid_dt_of(lev)) return;
+        if (next <= last_regrid[lev] + regrid_dt_of(lev) && !lr.do_global[lev]) return;
         if (lev == 0 && grid_old[lev].t == -DBL_MAX) return;
         const int id = find_sched(time + dt[lev]);
         if (id < 0)
@@ -1297,6 +1323,7 @@ class Sim {
             std::vector<long long> c;
             tag(lev, t, true, m, c);
             last_tags[lev] = c;
+            last_mask[lev] = {grid_new[lev].t, m};
             if (lev == 0) {
                 shadow = MultiFab();
                 shadow_tmp = MultiFab();
@@ -1304,9 +1331,405 @@ class Sim {
         }
     }
 
+    // ---------------------------------------------------------------------
+    // Local regrid: local_regrid/local_regrid.cpp, unique_layout.cpp.
+    // Blocks are blocking-factor cubes of a level, (i, j, k) block indices.
+    // ---------------------------------------------------------------------
+    using Block = std::array<int, 3>;
+    using BlockSet = std::set<Block>;
+    struct LocalRegridState {
+        int nregrids = 0, veto_level = -1;
+        std::vector<long long> last_numPts;
+        std::vector<char> no_local, do_global;
+        std::vector<double> min_distance, latest;
+        long long global_wanted = 0, attempts = 0;
+    } lr;
+    // Tag mask of the last tagging per level (time, mask box-major x fastest).
+    std::map<int, std::pair<double, std::vector<uint8_t>>> last_mask;
+
+    long long num_pts(const std::vector<Box> &bl) const {
+        long long n = 0;
+        for (const Box &b : bl) n += b.vol();
+        return n;
+    }
+    std::vector<Box> boxes_of(int lev) const {
+        std::vector<Box> v;
+        for (auto &f : grid_new[lev].fabs) v.push_back(f.box);
+        return v;
+    }
+    static bool covers(const MultiFab &m, int i, int j, int k) {
+        for (auto &f : m.fabs)
+            if (i >= f.box.lo[0] && i <= f.box.hi[0] && j >= f.box.lo[1] &&
+                j <= f.box.hi[1] && k >= f.box.lo[2] && k <= f.box.hi[2])
+                return true;
+        return false;
+    }
+
+    /** LocalRegrid::WrapIndices (local_regrid.cpp:474-490): entry c in 1..N
+     *  is cell (c - 1/2) bf, inside block c-1; entries 0 and N+1 are cells of
+     *  the periodic images of the last and the first block. */
+    static void check_bf(int v, int n) {
+        if (v < 2 || (v & (v - 1)) || n % v)
+            throw std::runtime_error("amr.blocking_factor must be a power of 2 dividing "
+                                     "the level size");
+    }
+    std::vector<int> wrapped_index(int lev) const {
+        check_bf(cfg.bf[lev], dimN[lev]);
+        const int bf = cfg.bf[lev], N = dimN[lev] / bf;
+        std::vector<int> w(N + 2);
+        w[0] = dimN[lev] - bf / 2;
+        w[N + 1] = bf / 2;
+        for (int i = 1; i <= N; ++i) w[i] = (int)(((double)i - 0.5) * (double)bf);
+        return w;
+    }
+
+    /** LocalRegrid::DetermineNewBoxArray, TagAndMeasure and CheckBorders
+     *  (local_regrid.cpp:511-680) over the tag mask of level lev (box-major,
+     *  x fastest), boxes in order and cells in k, j, i order.  Per box, a
+     *  border map over the 1-based wrapped block coordinates c0-1 .. c1+1 of
+     *  level lev+1 (c = 2 lo / bf + 1) marks blocks whose wrapped-index cell
+     *  no box of lev+1 holds.  A tagged cell ci (fine index f = 2 ci, own
+     *  block cf = f / bf + 1) measures, for each of the 27 blocks cf + d that
+     *  is still marked, d^2 = sum over axes of 0 if lo < f < hi, else
+     *  min((f - lo)^2, (f - hi)^2), lo = bf (cf + d - 1), hi = bf (cf + d) - 1;
+     *  below (n_error_buf + 1)^2 the block (wrapped) is added and unmarked
+     *  for the rest of this box.  Returns the smallest measured d^2, INT_MAX
+     *  when nothing was measured. */
+    int select_blocks(int lev, const std::vector<uint8_t> &mask, BlockSet &add) const {
+        const double threshold = (double)((cfg.n_error_buf + 1) * (cfg.n_error_buf + 1));
+        const int ibff = cfg.bf[lev + 1];
+        const double bff = (double)ibff;
+        const std::vector<int> wi = wrapped_index(lev + 1);
+        const MultiFab &fine = grid_new[lev + 1];
+        int closest = INT_MAX;
+        long long moff = 0;
+        for (const Fab &F : grid_new[lev].fabs) {
+            const Box &bx = F.box;
+            int c0[3], xs[3];
+            for (int a = 0; a < 3; ++a) {
+                c0[a] = (int)((double)(bx.lo[a] * 2) / bff) + 1;
+                xs[a] = (int)((double)(bx.hi[a] * 2) / bff) + 1 - c0[a] + 1;
+            }
+            const size_t sx = (size_t)xs[0] + 2, sy = (size_t)xs[1] + 2;
+            std::vector<char> border(sx * sy * (size_t)(xs[2] + 2));
+            auto B = [&](int i, int j, int k) -> char & {
+                return border[(size_t)i + sx * ((size_t)j + sy * (size_t)k)];
+            };
+            int remaining = 0;
+            for (int k = -1; k <= xs[2]; ++k)
+                for (int j = -1; j <= xs[1]; ++j)
+                    for (int i = -1; i <= xs[0]; ++i) {
+                        B(i + 1, j + 1, k + 1) =
+                            !covers(fine, wi[c0[0] + i], wi[c0[1] + j], wi[c0[2] + k]);
+                        remaining += B(i + 1, j + 1, k + 1);
+                    }
+            const int nx = bx.n(0), ny = bx.n(1);
+            for (int k = bx.lo[2]; k <= bx.hi[2] && remaining > 0; ++k)
+                for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+                    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+                        if (!mask[(size_t)(moff + (i - bx.lo[0]) +
+                                           (long long)nx * ((j - bx.lo[1]) +
+                                                            (long long)ny * (k - bx.lo[2])))])
+                            continue;
+                        const int fi[3] = {2 * i, 2 * j, 2 * k};
+                        int cfi[3];
+                        for (int a = 0; a < 3; ++a) cfi[a] = (int)((double)fi[a] / bff) + 1;
+                        for (int kk = -1; kk <= 1; ++kk)
+                            for (int jj = -1; jj <= 1; ++jj)
+                                for (int ii = -1; ii <= 1; ++ii) {
+                                    const int d[3] = {ii, jj, kk};
+                                    int ref[3];
+                                    for (int a = 0; a < 3; ++a) ref[a] = cfi[a] - c0[a] + d[a] + 1;
+                                    char &b = B(ref[0], ref[1], ref[2]);
+                                    if (!b) continue;
+                                    int d_sq = 0;
+                                    for (int a = 0; a < 3; ++a) {
+                                        const int lo = ibff * (cfi[a] + d[a] - 1);
+                                        const int hi = ibff * (cfi[a] + d[a]) - 1;
+                                        d_sq += (lo < fi[a] && fi[a] < hi)
+                                                    ? 0
+                                                    : std::min((fi[a] - lo) * (fi[a] - lo),
+                                                               (fi[a] - hi) * (fi[a] - hi));
+                                    }
+                                    closest = std::min(closest, d_sq);
+                                    if (d_sq < threshold) {
+                                        b = 0;
+                                        Block nb;
+                                        for (int a = 0; a < 3; ++a)
+                                            nb[a] = (int)((double)wi[cfi[a] + d[a]] / bff - 0.5);
+                                        add.insert(nb);
+                                    }
+                                }
+                    }
+            moff += bx.vol();
+        }
+        return closest;
+    }
+
+    /** UniqueLayout::BoxList (unique_layout.cpp:133-170): per block row
+     *  (i, j), ascending, one box per run of consecutive k. */
+    static std::vector<Box> block_boxes(const BlockSet &s, int bf) {
+        std::vector<Box> out;
+        for (auto it = s.begin(); it != s.end();) {
+            const Block b0 = *it;
+            int km = b0[2];
+            ++it;
+            while (it != s.end() && (*it)[0] == b0[0] && (*it)[1] == b0[1] &&
+                   (*it)[2] == km + 1) {
+                km = (*it)[2];
+                ++it;
+            }
+            out.push_back(Box{{b0[0] * bf, b0[1] * bf, b0[2] * bf},
+                              {(b0[0] + 1) * bf - 1, (b0[1] + 1) * bf - 1, (km + 1) * bf - 1}});
+        }
+        return out;
+    }
+
+    /** Joins runs into larger boxes (the reference leaves this to AMReX's
+     *  BoxList::simplify, local_regrid.cpp:67-76): boxes with the same x and
+     *  z ranges that touch along y merge, then boxes with the same y and z
+     *  ranges that touch along x; the result is sorted by (z, y, x) of lo. */
+    static std::vector<Box> merge_boxes(std::vector<Box> v) {
+        auto merge = [&](int a, int o1, int o2) {
+            std::sort(v.begin(), v.end(), [&](const Box &p, const Box &q) {
+                return std::make_tuple(p.lo[o1], p.hi[o1], p.lo[o2], p.hi[o2], p.lo[a]) <
+                       std::make_tuple(q.lo[o1], q.hi[o1], q.lo[o2], q.hi[o2], q.lo[a]);
+            });
+            std::vector<Box> out;
+            for (const Box &b : v) {
+                if (!out.empty()) {
+                    Box &p = out.back();
+                    if (p.lo[o1] == b.lo[o1] && p.hi[o1] == b.hi[o1] && p.lo[o2] == b.lo[o2] &&
+                        p.hi[o2] == b.hi[o2] && p.hi[a] + 1 == b.lo[a]) {
+                        p.hi[a] = b.hi[a];
+                        continue;
+                    }
+                }
+                out.push_back(b);
+            }
+            v = out;
+        };
+        merge(1, 0, 2);
+        merge(0, 1, 2);
+        std::sort(v.begin(), v.end(), [](const Box &p, const Box &q) {
+            return std::make_tuple(p.lo[2], p.lo[1], p.lo[0]) <
+                   std::make_tuple(q.lo[2], q.lo[1], q.lo[0]);
+        });
+        return v;
+    }
+
+    /** LocalRegrid::FixNesting (local_regrid.cpp:687-729): the runs added to
+     *  level l, grown by nghost + 4 cells, coarsened by 2 (floor), cut at
+     *  the periodic images of the domain (WrapBoxArray, :735-757); for each
+     *  piece, wrapped block coordinates lo / bf + 1 .. hi / bf + 2 of level
+     *  l-1 whose cell no box of l-1 holds are added to l-1. */
+    void fix_nesting(int l, std::vector<BlockSet> &add) const {
+        const std::vector<Box> runs = block_boxes(add[l], cfg.bf[l]);
+        const int g = cfg.nghost + 4, Nc = dimN[l - 1];
+        auto floordiv2 = [](int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); };
+        std::vector<Box> pieces;
+        for (Box b : runs) {
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = floordiv2(b.lo[a] - g);
+                b.hi[a] = floordiv2(b.hi[a] + g);
+            }
+            for (int i = -1; i <= 1; ++i)
+                for (int j = -1; j <= 1; ++j)
+                    for (int k = -1; k <= 1; ++k) {
+                        const int sh[3] = {i * Nc, j * Nc, k * Nc};
+                        Box p;
+                        bool empty = false;
+                        for (int a = 0; a < 3; ++a) {
+                            p.lo[a] = std::max(b.lo[a], sh[a]) - sh[a];
+                            p.hi[a] = std::min(b.hi[a], sh[a] + Nc - 1) - sh[a];
+                            if (p.hi[a] < p.lo[a]) empty = true;
+                        }
+                        if (!empty) pieces.push_back(p);
+                    }
+        }
+        const double bfc = (double)cfg.bf[l - 1];
+        const std::vector<int> wi = wrapped_index(l - 1);
+        for (const Box &p : pieces) {
+            int cl[3], ch[3];
+            for (int a = 0; a < 3; ++a) {
+                cl[a] = (int)((double)p.lo[a] / bfc + 1.);
+                ch[a] = (int)((double)p.hi[a] / bfc + 2.);
+            }
+            for (int cx = cl[0]; cx <= ch[0]; ++cx)
+                for (int cy = cl[1]; cy <= ch[1]; ++cy)
+                    for (int cz = cl[2]; cz <= ch[2]; ++cz)
+                        if (!covers(grid_new[l - 1], wi[cx], wi[cy], wi[cz]))
+                            add[l - 1].insert(Block{(int)((double)wi[cx] / bfc - 0.5),
+                                                    (int)((double)wi[cy] / bfc - 0.5),
+                                                    (int)((double)wi[cz] / bfc - 0.5)});
+        }
+    }
+
+    /** Valid cells of `boxes` as FillPatch (level_synchronizer.cpp:92-209)
+     *  fills them at `time` from level lev's data `src` and level lev-1:
+     *  coarse data interpolated in time and space, then src where it holds
+     *  the cell. */
+    MultiFab fill_patch_boxes(int lev, const std::vector<Box> &boxes, const MultiFab &src,
+                              double time) {
+        MultiFab m;
+        m.define(boxes, nc, cfg.nghost);
+        m.t = time;
+        fill_coarse_ghosts(m, lev, time, true, false);
+        for (Fab &D : m.fabs)
+            for (const Fab &S : src.fabs) {
+                int lo[3], hi[3];
+                bool empty = false;
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = std::max(D.box.lo[a], S.box.lo[a]);
+                    hi[a] = std::min(D.box.hi[a], S.box.hi[a]);
+                    if (hi[a] < lo[a]) empty = true;
+                }
+                if (empty) continue;
+                for (int n = 0; n < nc; ++n)
+                    for (int k = lo[2]; k <= hi[2]; ++k)
+                        for (int j = lo[1]; j <= hi[1]; ++j)
+                            for (int i = lo[0]; i <= hi[0]; ++i)
+                                D.at(i, j, k, n) = const_cast<Fab &>(S).at(i, j, k, n);
+            }
+        return m;
+    }
+
+    /** LocalRegrid::AddBoxes (local_regrid.cpp:763-828): the level keeps its
+     *  boxes and data and gains `nb` after them, filled by FillPatch at the
+     *  new and at the old time (the old state from the new one when the
+     *  level has no old time yet). */
+    void add_boxes(int lev, const std::vector<Box> &nb) {
+        if (lev < 1) throw std::runtime_error("boxes are added to levels above 0");
+        if (nb.empty()) return;
+        MultiFab &New = grid_new[lev], &Old = grid_old[lev];
+        const MultiFab fn = fill_patch_boxes(lev, nb, New, New.t);
+        const MultiFab fo =
+            Old.t == -DBL_MAX ? fn : fill_patch_boxes(lev, nb, Old, Old.t);
+        for (size_t b = 0; b < nb.size(); ++b) {
+            New.fabs.push_back(fn.fabs[b]);
+            Old.fabs.push_back(fo.fabs[b]);
+        }
+    }
+
+    /** Sledgehamr::RemakeLevel (sledgehamr.cpp:173-190): level lev on new
+     *  boxes, its state FillPatch-ed at the level's time from the level's
+     *  data and level lev-1.  The old state, which the reference leaves
+     *  undefined, is a copy of the new one. */
+    void remake_level(int lev, const std::vector<Box> &boxes) {
+        if (lev < 1) throw std::runtime_error("levels above 0 are remade");
+        MultiFab &New = grid_new[lev], &Old = grid_old[lev];
+        MultiFab fn = fill_patch_boxes(lev, boxes, New, New.t);
+        fn.istep = New.istep;
+        MultiFab fo = fn;
+        fo.t = Old.t;
+        fo.istep = Old.istep;
+        New = fn;
+        Old = fo;
+        did_global_regrid(lev - 1);
+    }
+
+    /** LocalRegrid::DidGlobalRegrid (local_regrid.cpp:34-46). */
+    void did_global_regrid(int lev) {
+        std::fill(lr.no_local.begin(), lr.no_local.end(), 0);
+        std::fill(lr.do_global.begin(), lr.do_global.end(), 0);
+        lr.nregrids = 0;
+        for (int l = lev + 1; l < (int)lr.last_numPts.size(); ++l)
+            lr.last_numPts[l] = num_pts(boxes_of(l));
+    }
+
+    /** Tags ErrorEst would hand DetermineNewBoxArray for level l: none at
+     *  t_start with a shadow hierarchy or when the project declines level
+     *  l+1 (sledgehamr.cpp:207-230), else the mask of the level's tagging at
+     *  its current time. */
+    std::vector<uint8_t> regrid_mask(int l) {
+        const double t = grid_new[l].t;
+        long long n = num_pts(boxes_of(l));
+        if ((t == cfg.t_start && cfg.shadow_hierarchy) || !prj.CreateLevelIf(l + 1, t))
+            return std::vector<uint8_t>((size_t)n, 0);
+        auto it = last_mask.find(l);
+        if (it == last_mask.end() || it->second.first != t || (long long)it->second.second.size() != n)
+            throw std::runtime_error("no tags of level " + std::to_string(l) + " at its time");
+        return it->second.second;
+    }
+
+    /** LocalRegrid::DoAttemptRegrid (local_regrid.cpp:407-440) with its
+     *  Prechecks (:106-145), DetermineAllBoxArrays (:163-172), FixAllNesting
+     *  (:176-182), CheckForVeto / CheckThresholds / ComputeLatestPossible-
+     *  RegridTime (:214-322) and DealWithVeto (:331-400).  Returns 0 when a
+     *  global regrid is wanted, 1 after adding boxes (possibly none), 2 when
+     *  the regrid is delayed. */
+    int local_regrid(int lev) {
+        ++lr.attempts;
+        const int finest = this->finest();
+        if (lr.nregrids++ >= cfg.max_local_regrids) return 0;
+        if (cfg.vt_weak <= 1.) {
+            lr.veto_level = lev - 1;
+            return 0;
+        }
+        lr.veto_level = -1;
+        if (lr.do_global[lev]) return 0;
+        if (lev == finest) return 0;
+        while ((int)lr.last_numPts.size() <= finest)
+            lr.last_numPts.push_back(num_pts(boxes_of((int)lr.last_numPts.size())));
+        std::vector<BlockSet> add(finest + 1);
+        lr.min_distance.assign(finest + 1, -1.);
+        for (int l = lev; l < finest && !lr.no_local[l]; ++l) {
+            const int d = select_blocks(l, regrid_mask(l), add[l + 1]);
+            lr.min_distance[l + 1] = d == INT_MAX ? -1. : std::sqrt((double)d);
+        }
+        for (int l = finest; l > 1; --l) fix_nesting(l, add);
+        std::vector<std::vector<Box>> ba(finest + 1);
+        for (int l = 1; l <= finest; ++l) ba[l] = merge_boxes(block_boxes(add[l], cfg.bf[l]));
+        bool veto = false;
+        lr.latest.assign(finest + 1, -1.);
+        for (int l = 1; l <= finest; ++l) {
+            if (!veto) {
+                const double Nb = (double)num_pts(ba[l]);
+                const double Nc = (double)num_pts(boxes_of(l));
+                double Nr = (double)lr.last_numPts[l];
+                if (Nr == 0) Nr = Nc;
+                const double fV = (Nb + Nc) / Nr;
+                veto = fV > cfg.vt_weak;
+                if (fV > cfg.vt_strong && lr.veto_level == -1) lr.veto_level = l - 1;
+            }
+            if (l > lev) {
+                double delay = DBL_MAX;
+                if (lr.min_distance[l] >= 0)
+                    delay = lr.min_distance[l] / (double)cfg.n_error_buf * regrid_dt_of(l);
+                lr.latest[l] = grid_new[l].t + delay;
+            }
+        }
+        if (veto) {
+            const int vl = lr.veto_level;
+            if (vl >= lev || vl <= 0) return 0;
+            lr.do_global[vl] = 1;
+            double nsteps = grid_new[vl].istep % 2 == 0 ? 1 : 2;
+            if (cfg.shadow_hierarchy) nsteps = 0;
+            const double target = grid_new[vl].t + nsteps * dt[vl];
+            bool wait = true;
+            for (int l = lev + 1; l <= finest; ++l)
+                if (lr.latest[l] < target) {
+                    wait = false;
+                    break;
+                }
+            if (wait) {
+                for (int l = lev; l <= finest; ++l) lr.no_local[l] = 1;
+                return 2;
+            }
+        }
+        for (int l = 1; l <= finest; ++l)
+            if (!ba[l].empty()) add_boxes(l, ba[l]);
+        return 1;
+    }
+
     void do_regrid_if_scheduled(int lev) {
         const double time = grid_new[lev].t;
         if (!do_regrid(lev, time)) return;
+        if (cfg.local_regrid && local_regrid(lev) == 0) {
+            // The global regrid (AMReX clustering) keeps the layout here.
+            ++lr.global_wanted;
+            did_global_regrid(lev);
+        }
         for (int l = lev; l <= finest(); ++l) last_regrid[l] = time;
         const int id = find_sched(time);
         if (id >= 0) sched.erase(sched.begin() + id);

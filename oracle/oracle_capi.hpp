@@ -218,6 +218,117 @@
             if (te && !t.empty()) std::memcpy(te, t.data(), t.size() * 8);     \
         });                                                                    \
     }                                                                          \
+    ORC_EXPORT int orc_boxes(void *h, int lev, int *b, int maxbox, int *nbox) { \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            const auto &f = s->grid_new[lev].fabs;                             \
+            *nbox = (int)f.size();                                             \
+            if (b && (int)f.size() <= maxbox)                                  \
+                for (size_t i = 0; i < f.size(); ++i)                          \
+                    for (int a = 0; a < 3; ++a) {                              \
+                        b[6 * i + a] = f[i].box.lo[a];                         \
+                        b[6 * i + 3 + a] = f[i].box.hi[a];                     \
+                    }                                                          \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_regrid_select(void *h, int lev, const unsigned char *mask, \
+                                     int *blocks, int maxblock, int *nblock,   \
+                                     double *dist) {                           \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            std::vector<uint8_t> m;                                            \
+            if (mask)                                                          \
+                m.assign(mask, mask + orc_cells(s, lev));                      \
+            else                                                               \
+                m = s->regrid_mask(lev);                                       \
+            typename OrcSim::BlockSet add;                                     \
+            const int d = s->select_blocks(lev, m, add);                       \
+            *dist = d == INT_MAX ? -1. : std::sqrt((double)d);                 \
+            *nblock = (int)add.size();                                         \
+            int i = 0;                                                         \
+            if (blocks && (int)add.size() <= maxblock)                         \
+                for (const auto &b : add) {                                    \
+                    for (int a = 0; a < 3; ++a) blocks[3 * i + a] = b[a];      \
+                    ++i;                                                       \
+                }                                                              \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_fix_nesting(void *h, int l, const int *blocks, int nb,  \
+                                   int *out, int maxout, int *nout) {          \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            std::vector<typename OrcSim::BlockSet> add(l + 1);                 \
+            for (int i = 0; i < nb; ++i)                                       \
+                add[l].insert({blocks[3 * i], blocks[3 * i + 1], blocks[3 * i + 2]}); \
+            s->fix_nesting(l, add);                                            \
+            *nout = (int)add[l - 1].size();                                    \
+            int i = 0;                                                         \
+            if (out && (int)add[l - 1].size() <= maxout)                       \
+                for (const auto &b : add[l - 1]) {                             \
+                    for (int a = 0; a < 3; ++a) out[3 * i + a] = b[a];         \
+                    ++i;                                                       \
+                }                                                              \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_block_layout(const int *blocks, int nb, int bf, int *out, \
+                                    int maxout, int *nout) {                   \
+        return orc_guard([&] {                                                 \
+            typename OrcSim::BlockSet set;                                     \
+            for (int i = 0; i < nb; ++i)                                       \
+                set.insert({blocks[3 * i], blocks[3 * i + 1], blocks[3 * i + 2]}); \
+            const auto v = OrcSim::merge_boxes(OrcSim::block_boxes(set, bf));  \
+            *nout = (int)v.size();                                             \
+            if (out && (int)v.size() <= maxout)                                \
+                for (size_t i = 0; i < v.size(); ++i)                          \
+                    for (int a = 0; a < 3; ++a) {                              \
+                        out[6 * i + a] = v[i].lo[a];                           \
+                        out[6 * i + 3 + a] = v[i].hi[a];                       \
+                    }                                                          \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_add_boxes(void *h, int lev, const int *b, int nb) {     \
+        return orc_guard([&] {                                                 \
+            std::vector<orc::Box> v(nb);                                       \
+            for (int i = 0; i < nb; ++i)                                       \
+                for (int a = 0; a < 3; ++a) {                                  \
+                    v[i].lo[a] = b[6 * i + a];                                 \
+                    v[i].hi[a] = b[6 * i + 3 + a];                             \
+                }                                                              \
+            static_cast<OrcSim *>(h)->add_boxes(lev, v);                       \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_remake_level(void *h, int lev, const int *b, int nb) {  \
+        return orc_guard([&] {                                                 \
+            std::vector<orc::Box> v(nb);                                       \
+            for (int i = 0; i < nb; ++i)                                       \
+                for (int a = 0; a < 3; ++a) {                                  \
+                    v[i].lo[a] = b[6 * i + a];                                 \
+                    v[i].hi[a] = b[6 * i + 3 + a];                             \
+                }                                                              \
+            static_cast<OrcSim *>(h)->remake_level(lev, v);                    \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_local_regrid(void *h, int lev, int *outcome) {          \
+        return orc_guard([&] { *outcome = static_cast<OrcSim *>(h)->local_regrid(lev); }); \
+    }                                                                          \
+    ORC_EXPORT int orc_regrid_stats(void *h, long long *st, double *md, int nlev) { \
+        return orc_guard([&] {                                                 \
+            auto &r = static_cast<OrcSim *>(h)->lr;                            \
+            st[0] = r.attempts;                                                \
+            st[1] = r.global_wanted;                                           \
+            st[2] = r.nregrids;                                                \
+            st[3] = r.veto_level;                                              \
+            for (int l = 0; l < nlev; ++l)                                     \
+                md[l] = l < (int)r.min_distance.size() ? r.min_distance[l] : -1.; \
+        });                                                                    \
+    }                                                                          \
+    ORC_EXPORT int orc_set_tags(void *h, int lev, const unsigned char *mask) { \
+        return orc_guard([&] {                                                 \
+            auto *s = static_cast<OrcSim *>(h);                                \
+            s->last_mask[lev] = {s->grid_new[lev].t,                           \
+                                 std::vector<uint8_t>(mask, mask + orc_cells(s, lev))}; \
+        });                                                                    \
+    }                                                                          \
     ORC_EXPORT int orc_gw_spectrum(void *h, int mode, int nbins, int zp,       \
                                    int proj, double *bins, double *k,          \
                                    int *nout) {                                \

@@ -31,6 +31,8 @@ EXPORTS = (
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
     "shm_output_functions", "shm_output_function_name", "shm_spectrum", "shm_projection",
     "shm_upload_array", "shm_fill_coarse_patch",
+    "shm_regrid_select", "shm_add_boxes", "shm_remake_level", "shm_local_regrid",
+    "shm_regrid_stats",
     "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
     "shm_get_launch", "shm_set_graphs", "shm_set_timing", "shm_get_timing",
 )
@@ -103,6 +105,11 @@ def load_library(project: str) -> C.CDLL:
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "shm_upload_array": (i, [vp, i, i, dp, i]),
         "shm_fill_coarse_patch": (i, [vp, i]),
+        "shm_regrid_select": (i, [vp, i, C.POINTER(C.c_ubyte), ip, i, ip, dp]),
+        "shm_add_boxes": (i, [vp, i, ip, i]),
+        "shm_remake_level": (i, [vp, i, ip, i]),
+        "shm_local_regrid": (i, [vp, i, ip]),
+        "shm_regrid_stats": (i, [vp, llp, dp, i]),
         "shm_output_functions": (i, [vp, ip]),
         "shm_output_function_name": (i, [vp, i, C.c_char_p, i]),
         "shm_spectrum": (i, [vp, i, dp, llp, ip]),
@@ -295,6 +302,53 @@ class Simulation:
     def fill_coarse_patch(self, lev: int) -> None:
         """FillCoarsePatch: level lev from level lev-1 at the level's time."""
         self._check(self.lib.shm_fill_coarse_patch(self._ctx, lev))
+
+    def regrid_select(self, lev: int, mask=None):
+        """DetermineNewBoxArray of tagging level lev: (blocks of level lev+1
+        as an (n, 3) array in (i, j, k) order, shortest distance or -1).
+        mask: one byte per valid cell, or None for the last tagging's."""
+        mp = None
+        if mask is not None:
+            m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).ravel())
+            if m.size != self.ncells(lev):
+                raise ValueError("mask needs one byte per valid cell")
+            mp = m.ctypes.data_as(C.POINTER(C.c_ubyte))
+        n, dist = C.c_int(), C.c_double()
+        self._check(self.lib.shm_regrid_select(self._ctx, lev, mp, None, 0, C.byref(n),
+                                               C.byref(dist)))
+        b = np.zeros((n.value, 3), dtype=np.int32)
+        self._check(self.lib.shm_regrid_select(
+            self._ctx, lev, mp, b.ctypes.data_as(C.POINTER(C.c_int)), n.value,
+            C.byref(n), C.byref(dist)))
+        return b, dist.value
+
+    def add_boxes(self, lev: int, boxes) -> None:
+        """AddBoxes: the level gains `boxes` (lo, hi rows) after its own."""
+        b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        self._check(self.lib.shm_add_boxes(
+            self._ctx, lev, b.ctypes.data_as(C.POINTER(C.c_int)), len(b)))
+
+    def remake_level(self, lev: int, boxes) -> None:
+        """RemakeLevel on a new box list."""
+        b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        self._check(self.lib.shm_remake_level(
+            self._ctx, lev, b.ctypes.data_as(C.POINTER(C.c_int)), len(b)))
+
+    def local_regrid(self, lev: int) -> int:
+        """DoAttemptRegrid from tagging level lev: 0 a global regrid is
+        wanted, 1 boxes added, 2 delayed."""
+        o = C.c_int()
+        self._check(self.lib.shm_local_regrid(self._ctx, lev, C.byref(o)))
+        return o.value
+
+    def regrid_stats(self) -> dict:
+        s = np.zeros(4, dtype=np.int64)
+        nl = self.info["max_level"] + 1
+        d = np.zeros(nl)
+        self._check(self.lib.shm_regrid_stats(
+            self._ctx, s.ctypes.data_as(C.POINTER(C.c_longlong)), _dptr(d), nl))
+        return {"attempts": int(s[0]), "global_wanted": int(s[1]),
+                "nregrids": int(s[2]), "veto_level": int(s[3]), "min_distance": d}
 
     def output_functions(self) -> list:
         """Names of the project's output functions, in id order."""

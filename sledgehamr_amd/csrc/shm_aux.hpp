@@ -4,6 +4,7 @@
 
 #include <sledgehamr.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "shm_layout.hpp"
@@ -870,6 +871,79 @@ __global__ void __launch_bounds__(256) init_kernel(const InitArgs a) {
         }
         for (int n = 0; n < a.ncomp && n < 32; ++n)
             store_state_at<PrecTraits<P>>(a.arr, a.arrlo, (long long)n * a.cstride + e, v[n]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Local regrid: blocks of the finer level a tagged cell asks for.
+// ---------------------------------------------------------------------------
+
+/** LocalRegrid::CheckBorders (local_regrid.cpp:563-620) for every tagged
+ *  cell.  A cell at fine index f = 2 ci, in 1-based block cf = f / bf + 1,
+ *  measures for each of the 27 blocks cf + d marked in its box's border map
+ *  d^2 = sum over axes of 0 inside (lo < f < hi), else min((f - lo)^2,
+ *  (f - hi)^2), lo = bf (cf + d - 1), hi = bf (cf + d) - 1.
+ *  Pass 0: below thr the block (wrapped) is added, and the box-local index
+ *  of the first such cell in k, j, i order is kept per border entry.
+ *  Pass 1: the reference scans a box's cells in that order and unmarks a
+ *  block once added, so a cell measures a block iff its index is at most
+ *  that first index; the smallest measured d^2 is kept. */
+template <int PASS>
+__global__ void __launch_bounds__(256) regrid_select_kernel(const RegridArgs a) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < a.nvalid;
+         idx += stride) {
+        if (!a.mask[idx]) continue;
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.boxes[mid].voff <= idx) lo = mid;
+            else hi = mid - 1;
+        }
+        const RegridBox &B = a.boxes[lo];
+        const int local = (int)(idx - B.voff);
+        const int r = local / B.n[0];
+        const int ci[3] = {B.lo[0] + local % B.n[0], B.lo[1] + r % B.n[1],
+                           B.lo[2] + r / B.n[1]};
+        int fi[3], cf[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            fi[d] = 2 * ci[d];
+            cf[d] = fi[d] / a.bf + 1;
+        }
+        const long long sx = B.xs[0] + 2, sy = B.xs[1] + 2;
+        int best = INT_MAX;
+        for (int kk = -1; kk <= 1; ++kk)
+            for (int jj = -1; jj <= 1; ++jj)
+                for (int ii = -1; ii <= 1; ++ii) {
+                    const int off[3] = {ii, jj, kk};
+                    int ref[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) ref[d] = cf[d] - B.c0[d] + off[d] + 1;
+                    const long long e = B.boff + ref[0] + sx * (ref[1] + sy * ref[2]);
+                    if (!a.border[e]) continue;
+                    int d_sq = 0;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const int blo = a.bf * (cf[d] + off[d] - 1);
+                        const int bhi = a.bf * (cf[d] + off[d]) - 1;
+                        const int dl = fi[d] - blo, dh = fi[d] - bhi;
+                        d_sq += (blo < fi[d] && fi[d] < bhi) ? 0 : min(dl * dl, dh * dh);
+                    }
+                    if (PASS == 0) {
+                        if (d_sq < a.thr) {
+                            atomicMin(&a.first[e], local);
+                            int b[3];
+#pragma unroll
+                            for (int d = 0; d < 3; ++d)
+                                b[d] = (cf[d] + off[d] - 1 + a.Np) % a.Np;
+                            a.added[b[0] + (long long)a.Np * (b[1] + (long long)a.Np * b[2])] = 1;
+                        }
+                    } else if (local <= a.first[e]) {
+                        best = min(best, d_sq);
+                    }
+                }
+        if (PASS == 1 && best != INT_MAX) atomicMin(a.min_dsq, best);
     }
 }
 

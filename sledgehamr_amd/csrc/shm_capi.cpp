@@ -2,6 +2,8 @@
 #include "../../include/shm.h"
 
 #include <algorithm>
+#include <climits>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -286,6 +288,83 @@ __attribute__((visibility("default"))) int shm_fill_coarse_patch(shm_ctx *ctx,
     return guard(ctx, [&] {
         SHM_NEED_RT;
         ctx->rt->fill_coarse_patch(lev);
+    });
+}
+
+static std::vector<shm::Box3> box_list(const int *boxes, int nbox) {
+    if (nbox < 0 || (nbox > 0 && !boxes)) throw shm::Error(SHM_ERR_ARG, "bad box list");
+    std::vector<shm::Box3> b(nbox);
+    for (int i = 0; i < nbox; ++i)
+        for (int a = 0; a < 3; ++a) {
+            b[i].lo[a] = boxes[6 * i + a];
+            b[i].hi[a] = boxes[6 * i + 3 + a];
+        }
+    return b;
+}
+
+__attribute__((visibility("default"))) int shm_regrid_select(
+    shm_ctx *ctx, int lev, const unsigned char *mask, int *blocks, int maxblock,
+    int *nblock, double *min_distance) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        shm::Runtime::BlockSet s;
+        int d = INT_MAX;
+        ctx->rt->regrid_select(lev, mask, s, d);
+        if (nblock) *nblock = (int)s.size();
+        if (min_distance) *min_distance = d == INT_MAX ? -1. : std::sqrt((double)d);
+        if (blocks) {
+            if ((int)s.size() > maxblock)
+                throw shm::Error(SHM_ERR_ARG, "block buffer too small");
+            int i = 0;
+            for (const auto &b : s) {
+                for (int a = 0; a < 3; ++a) blocks[3 * i + a] = b[a];
+                ++i;
+            }
+        }
+    });
+}
+
+__attribute__((visibility("default"))) int shm_add_boxes(shm_ctx *ctx, int lev,
+                                                         const int *boxes, int nbox) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->add_boxes(lev, box_list(boxes, nbox));
+    });
+}
+
+__attribute__((visibility("default"))) int shm_remake_level(shm_ctx *ctx, int lev,
+                                                            const int *boxes, int nbox) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->remake_level(lev, box_list(boxes, nbox));
+    });
+}
+
+__attribute__((visibility("default"))) int shm_local_regrid(shm_ctx *ctx, int lev,
+                                                            int *outcome) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        const int r = ctx->rt->local_regrid(lev);
+        if (outcome) *outcome = r;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_regrid_stats(shm_ctx *ctx,
+                                                            long long *stats,
+                                                            double *min_distance,
+                                                            int nlev) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        const shm::RegridState &r = ctx->rt->regrid_state();
+        if (stats) {
+            stats[0] = r.attempts;
+            stats[1] = r.global_wanted;
+            stats[2] = r.nregrids;
+            stats[3] = r.veto_level;
+        }
+        if (min_distance)
+            for (int l = 0; l < nlev; ++l)
+                min_distance[l] = l < (int)r.min_distance.size() ? r.min_distance[l] : -1.;
     });
 }
 

@@ -108,4 +108,27 @@ struct OutArgs {
     int *nproj;
 };
 
+/** A box of a tagging level for the local regrid's block selection: its
+ *  valid cells, the offset of its first mask byte, and its border map, the
+ *  1-based wrapped block coordinates c0 - 1 .. c0 + xs of the finer level
+ *  (x fastest), from entry boff on. */
+struct RegridBox {
+    int lo[3], n[3];
+    int c0[3], xs[3];
+    long long voff;
+    long long boff;
+};
+
+struct RegridArgs {
+    const RegridBox *boxes;
+    int nbox;
+    long long nvalid;
+    const uint8_t *mask;    // one byte per valid cell, box-major, x fastest
+    const uint8_t *border;  // 1: the block holds no finer cell at its centre
+    int *first;             // per border entry: first cell adding it (box-local)
+    uint8_t *added;         // Np^3 blocks of the finer level, x fastest
+    int *min_dsq;
+    int bf, Np, thr;        // blocking factor, blocks per axis, (n_error_buf+1)^2
+};
+
 }  // namespace shm

@@ -612,6 +612,32 @@ void Runtime::init_from_inputs(const std::string &inputs,
     set_level(0, boxes);
     S.finest_level = 0;
     tuned_.assign(S.max_level + 2, LaunchCfg());
+    // Local regrid (local_regrid.cpp:449-457, local_regrid.h:93-122); the
+    // blocking factor per level, the last value repeated.
+    {
+        // Checked here when given, else where the local regrid uses it.
+        std::vector<int> b{8};
+        const bool given = pp.queryarr("amr.blocking_factor", b);
+        if (b.empty()) throw Error(1, "amr.blocking_factor needs a value");
+        bf_.clear();
+        for (int lev = 0; lev <= S.max_level; ++lev) {
+            bf_.push_back(b[std::min<size_t>(lev, b.size() - 1)]);
+            if (given) check_bf(lev);
+        }
+    }
+    pp.query("amr.n_error_buf", n_error_buf_);
+    pp.query("amr.max_local_regrids", max_local_regrids_);
+    pp.query("amr.volume_threshold_strong", vt_strong_);
+    pp.query("amr.volume_threshold_weak", vt_weak_);
+    if (n_error_buf_ < 0) throw Error(1, "amr.n_error_buf must be >= 0");
+    {
+        int on = 0;
+        pp.query("shm.local_regrid", on);
+        local_regrid_on_ = on != 0;
+    }
+    lr_ = RegridState();
+    lr_.no_local.assign(S.max_level + 1, 0);
+    lr_.do_global.assign(S.max_level + 1, 0);
     regrid_dt_.clear();
     last_regrid_.clear();
     double reg_dt = DBL_MAX;
@@ -1117,18 +1143,23 @@ void Runtime::fill_coarse_patch(int lev) {
     LevelState &ls = level(lev);
     // Between the coarse level's two times the old state alone, not a time
     // interpolation (the first state GetLevelData returns).
-    CfArgs a = cf_args(lev, ls.t, true);
+    fill_coarse_full(lev, ls.cur, ls.t, true);
+}
+
+void Runtime::fill_coarse_full(int lev, int which, double time, bool first_state_only) {
+    LevelState &ls = level(lev);
+    CfArgs a = cf_args(lev, time, first_state_only);
     a.segs = ls.dcf_full.as<HaloSeg>();
     a.nseg = (int)ls.cf_full.size();
     a.total = ls.cf_full_total;
-    a.dst = ls.U[ls.cur].get();
-    a.dstlo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
+    a.dst = ls.U[which].get();
+    a.dstlo = comp() ? ls.Ulo[which].get() : nullptr;
     for (int c0 = 0; c0 < ncomp_; c0 += 32) {
         a.ncomp = std::min(32, ncomp_ - c0);
         for (int n = 0; n < a.ncomp; ++n) a.comp[n] = c0 + n;
         cf_launch(a, kCfState);
     }
-    ls.ghosts_ok[ls.cur] = false;
+    ls.ghosts_ok[which] = false;
 }
 
 void Runtime::cf_fill_state(int lev, double time, void *hi, void *lo) {
@@ -1581,7 +1612,15 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     pmod.resize(std::max<size_t>(pmod.size(), 1));
     upload_vec(dptag, ptag);
     upload_vec(dpmod, pmod);
-    dmask.alloc(std::max<long long>(1, fine.L.nvalid()));
+    // The mask stays with the level for the local regrid.
+    if (flags & kTeTags) {
+        if (fine.tags.bytes() != (size_t)std::max<long long>(1, fine.L.nvalid()))
+            fine.tags.alloc(std::max<long long>(1, fine.L.nvalid()));
+        fine.tags_t = -DBL_MAX;
+    } else {
+        dmask.alloc(std::max<long long>(1, fine.L.nvalid()));
+    }
+    DevMem &mask_mem = (flags & kTeTags) ? fine.tags : dmask;
     dcounts.alloc(sizeof(unsigned long long) * (2 + ncomp_));
     const long long total = fine.L.cprefix.back();
     if (te_host && (flags & kTeCompute)) dte.alloc(sizeof(double) * total * ncomp_);
@@ -1601,7 +1640,7 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     a.total = total;
     a.ncomp = ncomp_;
     a.te_out = dte ? dte.as<double>() : nullptr;
-    a.mask = dmask.as<uint8_t>();
+    a.mask = mask_mem.as<uint8_t>();
     a.counts = dcounts.as<unsigned long long>();
     a.te_crit = dcrit.as<double>();
     a.params_tag = dptag.as<double>();
@@ -1638,8 +1677,9 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
         }
         cl.ghosts_ok[cl.cur] = false;
     }
-    if (mask)
-        SHM_HIP(hipMemcpyAsync(mask, dmask.get(), fine.L.nvalid(),
+    if (flags & kTeTags) fine.tags_t = time;
+    if (mask && (flags & kTeTags))
+        SHM_HIP(hipMemcpyAsync(mask, mask_mem.get(), fine.L.nvalid(),
                                hipMemcpyDeviceToHost, stream_));
     std::vector<unsigned long long> hc(2 + ncomp_);
     SHM_HIP(hipMemcpyAsync(hc.data(), dcounts.get(), hc.size() * 8,
@@ -2487,7 +2527,8 @@ void Runtime::scheduled_step(int lev) {
                   !(istep % 2 == 0 && lev > 0);
         const double opp = lev > 0 ? time + 3. * S.dt[lev] : time + 2. * S.dt[lev];
         if (ok && !S.DoCreateLevelIf(lev + 1, opp)) ok = false;
-        if (ok && opp <= last_regrid_[lev] + regrid_dt_[lev]) ok = false;
+        if (ok && opp <= last_regrid_[lev] + regrid_dt_[lev] && !lr_.do_global[lev])
+            ok = false;
         if (ok && lev == 0 && ls.t_old == -DBL_MAX) ok = false;
         if (ok) {
             if (id < 0)
@@ -2504,9 +2545,15 @@ void Runtime::scheduled_step(int lev) {
         scheduled_step(lev + 1);
     }
     synchronize_levels(lev);
-    // DoRegridIfScheduled: tag on the static layout.
+    // DoRegridIfScheduled (time_stepper.cpp:287-296, DoRegrid :346-390): the
+    // local regrid when shm.local_regrid is set; where the reference falls
+    // back to a global regrid (AMReX clustering) the layout stays as it is.
     for (size_t i = 0; i < schedule_.size(); ++i) {
         if (schedule_[i].second == ls.t && schedule_[i].first == lev) {
+            if (local_regrid_on_ && local_regrid(lev) == 0) {
+                ++lr_.global_wanted;
+                did_global_regrid(lev);
+            }
             for (int l = lev; l <= S.finest_level; ++l) last_regrid_[l] = ls.t;
             schedule_.erase(schedule_.begin() + i);
             break;
@@ -2567,6 +2614,410 @@ void Runtime::synchronize_levels(int lev) {
         last_tags_[lev] = counts;
         if (lev == 0) shadow_ = LevelState();
     }
+}
+
+// ---------------------------------------------------------------------------
+// Local regrid (local_regrid/local_regrid.cpp, unique_layout.cpp).  Blocks
+// are blocking-factor cubes of a level, (i, j, k) block indices.
+// ---------------------------------------------------------------------------
+
+static int fdiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+void Runtime::check_bf(int lev) const {
+    const int v = bf_[lev];
+    if (v < 2 || (v & (v - 1)) || sim_->dimN[lev] % v)
+        throw Error(1, "amr.blocking_factor must be a power of 2 dividing the level size");
+}
+
+std::vector<uint8_t> Runtime::block_cover(int lev) const {
+    check_bf(lev);
+    // LocalRegrid::WrapIndices (local_regrid.cpp:474-490) tests the cell bf/2
+    // into each block: blocks q with lo <= q bf + bf/2 <= hi per axis.
+    const int bf = bf_[lev], Np = sim_->dimN[lev] / bf;
+    std::vector<uint8_t> cov((size_t)Np * Np * Np, 0);
+    if (!has_level(lev)) return cov;
+    for (const Box3 &b : levels_[lev].L.boxes) {
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = -fdiv(-(b.lo[a] - bf / 2), bf);
+            hi[a] = fdiv(b.hi[a] - bf / 2, bf);
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x)
+                    cov[(size_t)x + (size_t)Np * ((size_t)y + (size_t)Np * z)] = 1;
+    }
+    return cov;
+}
+
+void Runtime::regrid_select(int lev, const uint8_t *host_mask, BlockSet &blocks,
+                            int &min_dsq) {
+    blocks.clear();
+    min_dsq = INT_MAX;
+    if (lev < 0 || !has_level(lev) || !has_level(lev + 1))
+        throw Error(1, "block selection needs the tagging level and the level above it");
+    LevelState &ls = level(lev);
+    const long long nv = ls.L.nvalid();
+    DevMem dmask;
+    const uint8_t *mask = nullptr;
+    if (host_mask) {
+        dmask.alloc((size_t)std::max<long long>(1, nv));
+        SHM_HIP(hipMemcpy(dmask.get(), host_mask, (size_t)nv, hipMemcpyHostToDevice));
+        mask = dmask.as<uint8_t>();
+    } else {
+        if (!ls.tags || ls.tags_t != ls.t)
+            throw Error(3, "level " + std::to_string(lev) + " has no tags at its time");
+        mask = ls.tags.as<uint8_t>();
+    }
+    const int bf = bf_[lev + 1], Np = sim_->dimN[lev + 1] / bf;
+    const std::vector<uint8_t> cov = block_cover(lev + 1);
+    std::vector<RegridBox> rb;
+    std::vector<uint8_t> border;
+    long long voff = 0;
+    for (const Box3 &b : ls.L.boxes) {
+        if (b.vol() > INT_MAX) throw Error(1, "box too large for the block selection");
+        RegridBox r{};
+        for (int a = 0; a < 3; ++a) {
+            r.lo[a] = b.lo[a];
+            r.n[a] = b.n(a);
+            r.c0[a] = 2 * b.lo[a] / bf + 1;
+            r.xs[a] = 2 * b.hi[a] / bf + 1 - r.c0[a] + 1;
+        }
+        r.voff = voff;
+        r.boff = (long long)border.size();
+        // Border map of the box (GetBoxCoarseFineBorders, local_regrid.cpp:
+        // 500-530): wrapped blocks with no cell of the finer level.
+        for (int k = -1; k <= r.xs[2]; ++k)
+            for (int j = -1; j <= r.xs[1]; ++j)
+                for (int i = -1; i <= r.xs[0]; ++i) {
+                    const int q[3] = {(r.c0[0] + i - 1 + Np) % Np, (r.c0[1] + j - 1 + Np) % Np,
+                                      (r.c0[2] + k - 1 + Np) % Np};
+                    border.push_back(
+                        cov[(size_t)q[0] + (size_t)Np * ((size_t)q[1] + (size_t)Np * q[2])] ? 0
+                                                                                           : 1);
+                }
+        rb.push_back(r);
+        voff += b.vol();
+    }
+    if (nv == 0) return;
+    DevMem dboxes, dborder, dfirst, dadded, dmin;
+    upload_vec(dboxes, rb);
+    upload_vec(dborder, border);
+    upload_vec(dfirst, std::vector<int>(border.size(), INT_MAX));
+    upload_vec(dmin, std::vector<int>(1, INT_MAX));
+    dadded.alloc((size_t)Np * Np * Np);
+    RegridArgs a{};
+    a.boxes = dboxes.as<RegridBox>();
+    a.nbox = (int)rb.size();
+    a.nvalid = nv;
+    a.mask = mask;
+    a.border = dborder.as<uint8_t>();
+    a.first = dfirst.as<int>();
+    a.added = dadded.as<uint8_t>();
+    a.min_dsq = dmin.as<int>();
+    a.bf = bf;
+    a.Np = Np;
+    a.thr = (n_error_buf_ + 1) * (n_error_buf_ + 1);
+    const int grid = (int)std::min<long long>((nv + 255) / 256, 8LL * ncu_);
+    hipLaunchKernelGGL(regrid_select_kernel<0>, dim3(grid), dim3(256), 0, stream_, a);
+    SHM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(regrid_select_kernel<1>, dim3(grid), dim3(256), 0, stream_, a);
+    SHM_HIP(hipGetLastError());
+    std::vector<uint8_t> added((size_t)Np * Np * Np);
+    SHM_HIP(hipMemcpyAsync(added.data(), dadded.get(), added.size(), hipMemcpyDeviceToHost,
+                           stream_));
+    SHM_HIP(hipMemcpyAsync(&min_dsq, dmin.get(), sizeof(int), hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+    for (int x = 0; x < Np; ++x)
+        for (int y = 0; y < Np; ++y)
+            for (int z = 0; z < Np; ++z)
+                if (added[(size_t)x + (size_t)Np * ((size_t)y + (size_t)Np * z)])
+                    blocks.insert(Block{x, y, z});
+}
+
+std::vector<Box3> Runtime::block_boxes(const BlockSet &s, int bf) {
+    std::vector<Box3> out;
+    for (auto it = s.begin(); it != s.end();) {
+        const Block b0 = *it;
+        int km = b0[2];
+        for (++it; it != s.end() && (*it)[0] == b0[0] && (*it)[1] == b0[1] &&
+                   (*it)[2] == km + 1;
+             ++it)
+            km = (*it)[2];
+        out.push_back(Box3{{b0[0] * bf, b0[1] * bf, b0[2] * bf},
+                           {(b0[0] + 1) * bf - 1, (b0[1] + 1) * bf - 1, (km + 1) * bf - 1}});
+    }
+    return out;
+}
+
+std::vector<Box3> Runtime::merge_boxes(std::vector<Box3> v) {
+    // Join along axis a boxes whose extents along o1 and o2 agree.
+    auto join = [&v](int a, int o1, int o2) {
+        std::sort(v.begin(), v.end(), [&](const Box3 &p, const Box3 &q) {
+            return std::make_tuple(p.lo[o1], p.hi[o1], p.lo[o2], p.hi[o2], p.lo[a]) <
+                   std::make_tuple(q.lo[o1], q.hi[o1], q.lo[o2], q.hi[o2], q.lo[a]);
+        });
+        std::vector<Box3> out;
+        for (const Box3 &b : v) {
+            if (!out.empty()) {
+                Box3 &p = out.back();
+                if (p.lo[o1] == b.lo[o1] && p.hi[o1] == b.hi[o1] && p.lo[o2] == b.lo[o2] &&
+                    p.hi[o2] == b.hi[o2] && p.hi[a] + 1 == b.lo[a]) {
+                    p.hi[a] = b.hi[a];
+                    continue;
+                }
+            }
+            out.push_back(b);
+        }
+        v.swap(out);
+    };
+    join(1, 0, 2);
+    join(0, 1, 2);
+    std::sort(v.begin(), v.end(), [](const Box3 &p, const Box3 &q) {
+        return std::make_tuple(p.lo[2], p.lo[1], p.lo[0]) <
+               std::make_tuple(q.lo[2], q.lo[1], q.lo[0]);
+    });
+    return v;
+}
+
+void Runtime::fix_nesting(int l, std::vector<BlockSet> &add) const {
+    // The runs added to level l grown by nghost + 4 and coarsened by 2
+    // (floor), cut at the periodic images of the domain (WrapBoxArray,
+    // local_regrid.cpp:735-757); per piece the wrapped block coordinates
+    // lo / bf + 1 .. hi / bf + 2 of level l-1 (local_regrid.cpp:687-729).
+    const std::vector<Box3> runs = block_boxes(add[l], bf_[l]);
+    const int g = sim_->nghost + 4, Nc = sim_->dimN[l - 1];
+    const int bfc = bf_[l - 1], Npc = Nc / bfc;
+    const std::vector<uint8_t> cov = block_cover(l - 1);
+    for (Box3 b : runs) {
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = floordiv2(b.lo[a] - g);
+            b.hi[a] = floordiv2(b.hi[a] + g);
+        }
+        for (int i = -1; i <= 1; ++i)
+            for (int j = -1; j <= 1; ++j)
+                for (int k = -1; k <= 1; ++k) {
+                    const int sh[3] = {i * Nc, j * Nc, k * Nc};
+                    int cl[3], ch[3];
+                    bool empty = false;
+                    for (int a = 0; a < 3; ++a) {
+                        const int lo = std::max(b.lo[a], sh[a]) - sh[a];
+                        const int hi = std::min(b.hi[a], sh[a] + Nc - 1) - sh[a];
+                        if (hi < lo) empty = true;
+                        cl[a] = lo / bfc + 1;
+                        ch[a] = hi / bfc + 2;
+                    }
+                    if (empty) continue;
+                    for (int cx = cl[0]; cx <= ch[0]; ++cx)
+                        for (int cy = cl[1]; cy <= ch[1]; ++cy)
+                            for (int cz = cl[2]; cz <= ch[2]; ++cz) {
+                                const Block q{(cx - 1) % Npc, (cy - 1) % Npc, (cz - 1) % Npc};
+                                if (!cov[(size_t)q[0] +
+                                         (size_t)Npc * ((size_t)q[1] + (size_t)Npc * q[2])])
+                                    add[l - 1].insert(q);
+                            }
+                }
+    }
+}
+
+void Runtime::check_new_boxes(int lev, const std::vector<Box3> &boxes,
+                              const std::vector<Box3> &keep) const {
+    const int N = sim_->dimN[lev];
+    auto overlap = [](const Box3 &p, const Box3 &q) {
+        for (int a = 0; a < 3; ++a)
+            if (p.hi[a] < q.lo[a] || q.hi[a] < p.lo[a]) return false;
+        return true;
+    };
+    for (size_t i = 0; i < boxes.size(); ++i) {
+        const Box3 &b = boxes[i];
+        for (int a = 0; a < 3; ++a) {
+            if (b.lo[a] < 0 || b.hi[a] >= N || b.hi[a] < b.lo[a])
+                throw Error(1, "box outside the level domain");
+            if ((b.lo[a] | (b.hi[a] + 1)) & 1)
+                throw Error(1, "boxes above level 0 need even corners");
+        }
+        for (size_t j = 0; j < i; ++j)
+            if (overlap(b, boxes[j])) throw Error(1, "new boxes overlap");
+        for (const Box3 &k : keep)
+            if (overlap(b, k)) throw Error(1, "a new box overlaps a box of the level");
+    }
+}
+
+void Runtime::relayout(int lev, const std::vector<Box3> &boxes, bool both) {
+    LevelState prev = std::move(levels_[lev]);
+    LevelState &ls = levels_[lev];
+    ls = LevelState();
+    bump_epoch();
+    if (lev + 1 < (int)levels_.size()) levels_[lev + 1].cf_built = false;
+    try {
+        ls.L.build(boxes, sim_->nghost, sim_->dimN[lev], /*periodic=*/true);
+        ls.ncomp = ncomp_;
+        ensure_arrays(ls, integrator_);
+        ls.t = prev.t;
+        ls.t_old = prev.t_old;
+        ls.istep = prev.istep;
+        ls.cur = prev.cur;
+        long long tot = 0;
+        const std::vector<HaloSeg> plan = copy_plan(ls.L, prev.L, &tot);
+        DevMem dplan;
+        upload_vec(dplan, plan);
+        // FillPatch: coarse data everywhere, then the level's own valid data.
+        auto fill = [&](int s, double time) {
+            fill_coarse_full(lev, s, time, false);
+            if (tot == 0) return;
+            CopyArgs c{};
+            c.dst = ls.U[s].get();
+            c.src = prev.U[s].get();
+            c.dcstride = ls.L.cstride;
+            c.scstride = prev.L.cstride;
+            c.dboxes = ls.L.dbox.as<BoxDesc>();
+            c.sboxes = prev.L.dbox.as<BoxDesc>();
+            c.segs = dplan.as<HaloSeg>();
+            c.nseg = (int)plan.size();
+            c.total = tot;
+            c.ncomp = ncomp_;
+            launch_copy(c, esize(), ncu_, stream_);
+            if (comp()) {
+                c.dst = ls.Ulo[s].get();
+                c.src = prev.Ulo[s].get();
+                launch_copy(c, losize(), ncu_, stream_);
+            }
+        };
+        fill(ls.cur, ls.t);
+        const int od = ls.cur ^ 1;
+        if (both && ls.t_old != -DBL_MAX) {
+            fill(od, ls.t_old);
+        } else {
+            const size_t n = (size_t)ls.L.cstride * ncomp_;
+            SHM_HIP(hipMemcpyAsync(ls.U[od].get(), ls.U[ls.cur].get(), n * esize(),
+                                   hipMemcpyDeviceToDevice, stream_));
+            if (comp())
+                SHM_HIP(hipMemcpyAsync(ls.Ulo[od].get(), ls.Ulo[ls.cur].get(),
+                                       n * losize(), hipMemcpyDeviceToDevice, stream_));
+        }
+        ls.ghosts_ok[0] = ls.ghosts_ok[1] = false;
+        SHM_HIP(hipStreamSynchronize(stream_));
+    } catch (...) {
+        (void)hipStreamSynchronize(stream_);
+        levels_[lev] = std::move(prev);
+        levels_[lev].cf_built = false;
+        bump_epoch();
+        throw;
+    }
+}
+
+void Runtime::add_boxes(int lev, const std::vector<Box3> &boxes) {
+    if (lev < 1 || !has_level(lev) || !has_level(lev - 1))
+        throw Error(1, "boxes are added to a defined level above 0");
+    check_new_boxes(lev, boxes, level(lev).L.boxes);
+    if (boxes.empty()) return;
+    std::vector<Box3> all = level(lev).L.boxes;
+    all.insert(all.end(), boxes.begin(), boxes.end());
+    relayout(lev, all, true);
+}
+
+void Runtime::remake_level(int lev, const std::vector<Box3> &boxes) {
+    if (lev < 1 || !has_level(lev) || !has_level(lev - 1))
+        throw Error(1, "a defined level above 0 is remade");
+    if (boxes.empty()) throw Error(1, "a remade level needs boxes");
+    check_new_boxes(lev, boxes, {});
+    relayout(lev, boxes, false);
+    did_global_regrid(lev - 1);
+}
+
+void Runtime::did_global_regrid(int lev) {
+    // LocalRegrid::DidGlobalRegrid (local_regrid.cpp:34-46).
+    std::fill(lr_.no_local.begin(), lr_.no_local.end(), 0);
+    std::fill(lr_.do_global.begin(), lr_.do_global.end(), 0);
+    lr_.nregrids = 0;
+    for (int l = lev + 1; l < (int)lr_.last_numPts.size(); ++l)
+        lr_.last_numPts[l] = has_level(l) ? levels_[l].L.nvalid() : 0;
+}
+
+int Runtime::local_regrid(int lev) {
+    sledgehamr::Sledgehamr &S = *sim_;
+    if (lev < 0 || !has_level(lev)) throw Error(1, "no such level");
+    ++lr_.attempts;
+    int finest = 0;
+    for (int l = 0; l < (int)levels_.size(); ++l)
+        if (has_level(l)) finest = l;
+    // Prechecks (local_regrid.cpp:106-145); no restarts here.
+    if (lr_.nregrids++ >= max_local_regrids_) return 0;
+    if (vt_weak_ <= 1.) {
+        lr_.veto_level = lev - 1;
+        return 0;
+    }
+    lr_.veto_level = -1;
+    if (lr_.do_global[lev]) return 0;
+    if (lev == finest) return 0;
+    // InitializeLocalRegrid (:151-160).
+    while ((int)lr_.last_numPts.size() <= finest) {
+        const int l = (int)lr_.last_numPts.size();
+        lr_.last_numPts.push_back(has_level(l) ? levels_[l].L.nvalid() : 0);
+    }
+    // DetermineAllBoxArrays (:163-172); ErrorEst leaves the tags clear at
+    // t_start with a shadow hierarchy and where the project declines the
+    // finer level (sledgehamr.cpp:207-230).
+    std::vector<BlockSet> add(finest + 1);
+    lr_.min_distance.assign(finest + 1, -1.);
+    for (int l = lev; l < finest && !lr_.no_local[l]; ++l) {
+        const double t = levels_[l].t;
+        if ((t == S.t_start && S.shadow_hierarchy) || !S.DoCreateLevelIf(l + 1, t)) continue;
+        int d = INT_MAX;
+        BlockSet sel;
+        regrid_select(l, nullptr, sel, d);
+        add[l + 1].insert(sel.begin(), sel.end());
+        lr_.min_distance[l + 1] = d == INT_MAX ? -1. : std::sqrt((double)d);
+    }
+    // FixAllNesting (:176-182), JoinAllBoxArrays (:186-193).
+    for (int l = finest; l > 1; --l) fix_nesting(l, add);
+    std::vector<std::vector<Box3>> ba(finest + 1);
+    for (int l = 1; l <= finest; ++l) ba[l] = merge_boxes(block_boxes(add[l], bf_[l]));
+    // CheckForVeto (:304-322): CheckThresholds until the first veto,
+    // ComputeLatestPossibleRegridTime for every level above lev.
+    bool veto = false;
+    lr_.latest.assign(finest + 1, -1.);
+    for (int l = 1; l <= finest; ++l) {
+        if (!veto) {
+            double nb = 0;
+            for (const Box3 &b : ba[l]) nb += (double)b.vol();
+            const double nc = (double)levels_[l].L.nvalid();
+            double nr = (double)lr_.last_numPts[l];
+            if (nr == 0) nr = nc;
+            const double fv = (nb + nc) / nr;
+            veto = fv > vt_weak_;
+            if (fv > vt_strong_ && lr_.veto_level == -1) lr_.veto_level = l - 1;
+        }
+        if (l > lev) {
+            double delay = DBL_MAX;
+            if (lr_.min_distance[l] >= 0)
+                delay = lr_.min_distance[l] / (double)n_error_buf_ * regrid_dt_[l];
+            lr_.latest[l] = levels_[l].t + delay;
+        }
+    }
+    // DealWithVeto (:331-400).
+    if (veto) {
+        const int vl = lr_.veto_level;
+        if (vl >= lev || vl <= 0) return 0;
+        lr_.do_global[vl] = 1;
+        double nsteps = levels_[vl].istep % 2 == 0 ? 1 : 2;
+        if (S.shadow_hierarchy) nsteps = 0;
+        const double target = levels_[vl].t + nsteps * S.dt[vl];
+        bool wait = true;
+        for (int l = lev + 1; l <= finest; ++l)
+            if (lr_.latest[l] < target) {
+                wait = false;
+                break;
+            }
+        if (wait) {
+            for (int l = lev; l <= finest; ++l) lr_.no_local[l] = 1;
+            return 2;
+        }
+    }
+    // AddAllBoxes (:196-204).
+    for (int l = 1; l <= finest; ++l)
+        if (!ba[l].empty()) add_boxes(l, ba[l]);
+    return 1;
 }
 
 }  // namespace shm

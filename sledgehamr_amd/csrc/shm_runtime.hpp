@@ -4,8 +4,10 @@
 
 #include <sledgehamr.h>
 
+#include <array>
 #include <cfloat>
 #include <map>
+#include <set>
 #include <tuple>
 #include <memory>
 #include <string>
@@ -111,7 +113,20 @@ struct LevelState {
     DevMem dcf_full;
     long long cf_full_total = 0;
     bool cf_built = false;
+    // Mask of the last tagging (one byte per valid cell) and its time.
+    DevMem tags;
+    double tags_t = -DBL_MAX;
     bool defined() const { return !L.boxes.empty(); }
+};
+
+/** Counters and per-level values of the local regrid (LocalRegrid's
+ *  members, local_regrid.h:93-150). */
+struct RegridState {
+    int nregrids = 0, veto_level = -1;
+    std::vector<long long> last_numPts;
+    std::vector<char> no_local, do_global;
+    std::vector<double> min_distance, latest;
+    long long attempts = 0, global_wanted = 0;
 };
 
 /** Launch shape of the stage kernels.  A field of -1 (tz: 0) is not set:
@@ -199,6 +214,33 @@ class Runtime {
      *  of lev: [total, user, trunc[ncomp]]. */
     const std::vector<long long> &last_tags(int lev) const;
 
+    // Local regrid (local_regrid/local_regrid.cpp).
+    using Block = std::array<int, 3>;
+    using BlockSet = std::set<Block>;
+    /** LocalRegrid::DetermineNewBoxArray of tagging level lev from `mask`
+     *  (host bytes, one per valid cell; null: the level's last tagging at
+     *  its time): the blocks of level lev+1 to add, in (i, j, k) order, and
+     *  the smallest measured squared distance (INT_MAX: none measured). */
+    void regrid_select(int lev, const uint8_t *mask, BlockSet &blocks, int &min_dsq);
+    /** LocalRegrid::AddBoxes: the level keeps its boxes and data and gains
+     *  `boxes` after them, FillPatch-ed at the new and the old time. */
+    void add_boxes(int lev, const std::vector<Box3> &boxes);
+    /** Sledgehamr::RemakeLevel on `boxes`, then LocalRegrid::DidGlobalRegrid. */
+    void remake_level(int lev, const std::vector<Box3> &boxes);
+    /** LocalRegrid::DoAttemptRegrid from tagging level lev: 0 a global
+     *  regrid is wanted, 1 boxes added (possibly none), 2 delayed. */
+    int local_regrid(int lev);
+    /** LocalRegrid::FixNesting of level l: blocks of level l-1 that the
+     *  blocks added to level l need. */
+    void fix_nesting(int l, std::vector<BlockSet> &add) const;
+    /** UniqueLayout::BoxList: one box per run of consecutive k blocks. */
+    static std::vector<Box3> block_boxes(const BlockSet &s, int bf);
+    /** Runs joined along y, then x, where their other extents agree; sorted
+     *  by (z, y, x) of their low corner. */
+    static std::vector<Box3> merge_boxes(std::vector<Box3> v);
+    const RegridState &regrid_state() const { return lr_; }
+    int blocking_factor(int lev) const { return bf_[lev]; }
+
     void sync();
     void autotune(int lev);
     /** Plain read / write / copy bandwidth over buffers of `bytes` (GB/s). */
@@ -275,6 +317,19 @@ class Runtime {
     void cf_launch(CfArgs &a, int mode);
     /** kCfState fill of every component of a state of level lev. */
     void cf_fill_state(int lev, double time, void *hi, void *lo);
+    /** Every cell of state `which` of level lev, ghosts included, from level
+     *  lev-1 at `time` (first_state_only: FillCoarsePatch's choice). */
+    void fill_coarse_full(int lev, int which, double time, bool first_state_only);
+    /** Level lev on `boxes`: every cell from level lev-1, then the valid
+     *  data of the level's previous layout; state grid_old too when `both`
+     *  (else a copy of grid_new). */
+    void relayout(int lev, const std::vector<Box3> &boxes, bool both);
+    /** Blocks of level lev whose wrapped-index cell a box holds. */
+    std::vector<uint8_t> block_cover(int lev) const;
+    void did_global_regrid(int lev);
+    void check_bf(int lev) const;
+    void check_new_boxes(int lev, const std::vector<Box3> &boxes,
+                         const std::vector<Box3> &keep) const;
     void integrate(LevelState &ls, int lev, double dt, double dx,
                    double t_old);
     /** Times at which the integrator evaluates its stages. */
@@ -331,6 +386,13 @@ class Runtime {
     std::vector<std::pair<int, double>> schedule_;
     std::vector<double> regrid_dt_, last_regrid_;
     double te_lowest_time_ = -DBL_MAX;
+    // Local regrid inputs (amr.blocking_factor per level, amr.n_error_buf,
+    // amr.max_local_regrids, amr.volume_threshold_strong / _weak) and state.
+    std::vector<int> bf_;
+    int n_error_buf_ = 1, max_local_regrids_ = 10;
+    double vt_strong_ = 1.05, vt_weak_ = 1.1;
+    bool local_regrid_on_ = false;  // shm.local_regrid
+    RegridState lr_;
 };
 
 /** Last error text for the C ABI (per context). */

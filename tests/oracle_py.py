@@ -63,6 +63,15 @@ def _load(project: str) -> C.CDLL:
         "orc_last_tags": (i, [vp, i, llp, i]),
         "orc_gw_lambda": (d, [i, i, i, i, i, i, i, i, i]),
         "orc_unique_k2": (i, [i, llp, i]),
+        "orc_boxes": (i, [vp, i, ip, i, ip]),
+        "orc_regrid_select": (i, [vp, i, C.POINTER(C.c_ubyte), ip, i, ip, dp]),
+        "orc_fix_nesting": (i, [vp, i, ip, i, ip, i, ip]),
+        "orc_block_layout": (i, [ip, i, i, ip, i, ip]),
+        "orc_add_boxes": (i, [vp, i, ip, i]),
+        "orc_remake_level": (i, [vp, i, ip, i]),
+        "orc_local_regrid": (i, [vp, i, ip]),
+        "orc_regrid_stats": (i, [vp, llp, dp, i]),
+        "orc_set_tags": (i, [vp, i, C.POINTER(C.c_ubyte)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -256,3 +265,85 @@ def oracle_unique_k2(N: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.int64)
     lib.orc_unique_k2(N, out.ctypes.data_as(C.POINTER(C.c_longlong)), n)
     return out
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def block_layout(project, blocks, bf):
+    """Boxes of a block set: z-runs joined along y then x (oracle)."""
+    lib = _load(project)
+    b = np.ascontiguousarray(np.asarray(blocks, dtype=np.int32).reshape(-1, 3))
+    n = C.c_int()
+    if lib.orc_block_layout(_ip(b), len(b), bf, None, 0, C.byref(n)) != 0:
+        raise RuntimeError(lib.orc_last_error().decode())
+    out = np.zeros((n.value, 6), dtype=np.int32)
+    if lib.orc_block_layout(_ip(b), len(b), bf, _ip(out), n.value, C.byref(n)) != 0:
+        raise RuntimeError(lib.orc_last_error().decode())
+    return out
+
+
+def _oracle_regrid_methods():
+    def boxes(self, lev):
+        n = C.c_int()
+        self._ck(self.lib.orc_boxes(self.h, lev, None, 0, C.byref(n)))
+        b = np.zeros((n.value, 6), dtype=np.int32)
+        self._ck(self.lib.orc_boxes(self.h, lev, _ip(b), n.value, C.byref(n)))
+        return b
+
+    def regrid_select(self, lev, mask=None):
+        mp = None
+        if mask is not None:
+            m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).ravel())
+            assert m.size == self.ncells(lev)
+            mp = m.ctypes.data_as(C.POINTER(C.c_ubyte))
+        n, d = C.c_int(), C.c_double()
+        self._ck(self.lib.orc_regrid_select(self.h, lev, mp, None, 0, C.byref(n), C.byref(d)))
+        b = np.zeros((n.value, 3), dtype=np.int32)
+        self._ck(self.lib.orc_regrid_select(self.h, lev, mp, _ip(b), n.value, C.byref(n),
+                                            C.byref(d)))
+        return b, d.value
+
+    def fix_nesting(self, l, blocks):
+        b = np.ascontiguousarray(np.asarray(blocks, dtype=np.int32).reshape(-1, 3))
+        n = C.c_int()
+        self._ck(self.lib.orc_fix_nesting(self.h, l, _ip(b), len(b), None, 0, C.byref(n)))
+        out = np.zeros((n.value, 3), dtype=np.int32)
+        self._ck(self.lib.orc_fix_nesting(self.h, l, _ip(b), len(b), _ip(out), n.value,
+                                          C.byref(n)))
+        return out
+
+    def add_boxes(self, lev, boxes):
+        b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        self._ck(self.lib.orc_add_boxes(self.h, lev, _ip(b), len(b)))
+
+    def remake_level(self, lev, boxes):
+        b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        self._ck(self.lib.orc_remake_level(self.h, lev, _ip(b), len(b)))
+
+    def local_regrid(self, lev):
+        o = C.c_int()
+        self._ck(self.lib.orc_local_regrid(self.h, lev, C.byref(o)))
+        return o.value
+
+    def regrid_stats(self, nlev):
+        s = np.zeros(4, dtype=np.int64)
+        d = np.zeros(nlev)
+        self._ck(self.lib.orc_regrid_stats(self.h, s.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                           _dp(d), nlev))
+        return {"attempts": int(s[0]), "global_wanted": int(s[1]),
+                "nregrids": int(s[2]), "veto_level": int(s[3]), "min_distance": d}
+
+    def set_tags(self, lev, mask):
+        """Tag mask of level lev at its current time, as a tagging leaves it."""
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).ravel())
+        assert m.size == self.ncells(lev)
+        self._ck(self.lib.orc_set_tags(self.h, lev, m.ctypes.data_as(C.POINTER(C.c_ubyte))))
+
+    for f in (set_tags, boxes, regrid_select, fix_nesting, add_boxes, remake_level, local_regrid,
+              regrid_stats):
+        setattr(Oracle, f.__name__, f)
+
+
+_oracle_regrid_methods()
