@@ -345,12 +345,21 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
     // Points of the type 0 / 1 / 2 stencils: centre, 6 faces, then the 12
     // edges as (+,+) (-,+) (+,-) (-,-) pairs.
     constexpr int NPT = INTERP == 0 ? 1 : INTERP == 1 ? 7 : 19;
+    constexpr int REACH = INTERP == 0 ? 0 : INTERP == 4 ? 2 : 1;
     constexpr int PO[19][3] = {
         {0, 0, 0},   {-1, 0, 0}, {1, 0, 0},  {0, -1, 0}, {0, 1, 0},
         {0, 0, -1},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0}, {1, -1, 0},
         {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
         {0, 1, 1},   {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+    // Coarse states and their time weights.
+    const double wo = a.rec[0], wn = a.rec[1];
+    const bool use_old = a.rec[2] != 0., use_new = a.rec[3] != 0.;
+    const int ns = a.rec[4] != 0. ? 1 : 0;
+    const void *cnew = a.cs[ns], *cnewlo = a.cslo[ns];
+    const void *cold = a.cs[ns ^ 1], *coldlo = a.cslo[ns ^ 1];
     const long long stride = (long long)gridDim.x * blockDim.x;
+    // One thread per coarse parent cell: its stencil is located and loaded
+    // once for the (up to 8) fine children inside the segment.
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
          t < a.total; t += stride) {
         int lo = 0, hi = a.nseg - 1;
@@ -361,19 +370,22 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
             else
                 hi = mid - 1;
         }
-        const HaloSeg &sg = a.segs[lo];
+        const CfPSeg &sg = a.segs[lo];
         long long r = t - sg.start;
-        const int x = (int)(r % sg.n[0]);
-        r /= sg.n[0];
-        const int y = (int)(r % sg.n[1]);
-        const int z = (int)(r / sg.n[1]);
-        const int gi = sg.lo[0] + x, gj = sg.lo[1] + y, gk = sg.lo[2] + z;
+        const int ic = sg.plo[0] + (int)(r % sg.pn[0]);
+        r /= sg.pn[0];
+        const int jc = sg.plo[1] + (int)(r % sg.pn[1]);
+        const int kc = sg.plo[2] + (int)(r / sg.pn[1]);
         const int fb = sg.dst_box;
         const BoxDesc &db = a.fboxes[fb];
-        const long long fe = db.base + (gi - db.lo[0]) +
-                             (long long)(gj - db.lo[1]) * db.jstride +
-                             (long long)(gk - db.lo[2]) * db.kstride;
-        const int ic = gi >> 1, jc = gj >> 1, kc = gk >> 1;
+        // Children 2 p + c of the parent inside the segment, per axis.
+        int c0[3], c1[3];
+        const int pc[3] = {ic, jc, kc};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            c0[d] = max(0, sg.flo[d] - 2 * pc[d]);
+            c1[d] = min(1, sg.flo[d] + sg.fn[d] - 1 - 2 * pc[d]);
+        }
         // Element of coarse cell (ic + di, jc + dj, kc + dk), periodic image,
         // among the candidate boxes; -1 if none holds it.
         auto locate = [&](int di, int dj, int dk) -> long long {
@@ -388,12 +400,34 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
             }
             return -1;
         };
-        // Coarse value at element ce of component comp, weighted in time.
-        const double wo = a.rec[0], wn = a.rec[1];
-        const bool use_old = a.rec[2] != 0., use_new = a.rec[3] != 0.;
-        const int ns = a.rec[4] != 0. ? 1 : 0;
-        const void *cnew = a.cs[ns], *cnewlo = a.cslo[ns];
-        const void *cold = a.cs[ns ^ 1], *coldlo = a.cslo[ns ^ 1];
+        // The box holding the parent usually holds its whole stencil: then
+        // every point is an offset from the parent's element (boxes do not
+        // overlap, so it is the element the search would find).
+        long long cc = -1;
+        int cjs = 0, cks = 0;
+        {
+            const int ci = cf_wrap(ic, a.Nc), cj = cf_wrap(jc, a.Nc), ck = cf_wrap(kc, a.Nc);
+            for (int q = a.cand_start[fb]; q < a.cand_start[fb + 1]; ++q) {
+                const BoxDesc &cb = a.cboxes[a.cand[q]];
+                if (ci >= cb.lo[0] && ci < cb.lo[0] + cb.n[0] && cj >= cb.lo[1] &&
+                    cj < cb.lo[1] + cb.n[1] && ck >= cb.lo[2] && ck < cb.lo[2] + cb.n[2]) {
+                    if (ci - REACH >= cb.lo[0] && ci + REACH < cb.lo[0] + cb.n[0] &&
+                        cj - REACH >= cb.lo[1] && cj + REACH < cb.lo[1] + cb.n[1] &&
+                        ck - REACH >= cb.lo[2] && ck + REACH < cb.lo[2] + cb.n[2]) {
+                        cc = cb.base + (ci - cb.lo[0]) +
+                             (long long)(cj - cb.lo[1]) * cb.jstride +
+                             (long long)(ck - cb.lo[2]) * cb.kstride;
+                        cjs = cb.jstride;
+                        cks = cb.kstride;
+                    }
+                    break;
+                }
+            }
+        }
+        auto elem = [&](int di, int dj, int dk) -> long long {
+            return cc >= 0 ? cc + di + (long long)dj * cjs + (long long)dk * cks
+                           : locate(di, dj, dk);
+        };
         auto cval = [&](int comp, long long ce) -> double {
             const long long e = (long long)comp * a.ccstride + ce;
             double v = 0.;
@@ -404,8 +438,14 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
             }
             return v;
         };
-        // What the ghost cell receives for component slot n.
-        auto put = [&](int n, double v) {
+        // Fine element of child (cx, cy, cz).
+        auto fine_elem = [&](int cx, int cy, int cz) -> long long {
+            return db.base + (2 * ic + cx - db.lo[0]) +
+                   (long long)(2 * jc + cy - db.lo[1]) * db.jstride +
+                   (long long)(2 * kc + cz - db.lo[2]) * db.kstride;
+        };
+        // What the ghost cell at fine element fe receives for component slot n.
+        auto put = [&](int n, long long fe, double v) {
             const int comp = a.comp[n];
             if constexpr (MODE == kCfState) {
                 store_state_at<T>(a.dst, a.dstlo, (long long)comp * a.fcstride + fe, v);
@@ -424,68 +464,88 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
             }
         };
         if constexpr (INTERP == 4) {
-            // Tensor product of the conservative quartic child weights.
+            // Tensor product of the conservative quartic child weights; an
+            // upper child (odd fine index) takes WU, a lower one its mirror.
             constexpr double WU[5] = {3. / 128., -22. / 128., 1., 22. / 128., -3. / 128.};
-            double wx[5], wy[5], wz[5];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                wx[q] = (gi & 1) ? WU[q] : WU[4 - q];
-                wy[q] = (gj & 1) ? WU[q] : WU[4 - q];
-                wz[q] = (gk & 1) ? WU[q] : WU[4 - q];
-            }
             for (int n = 0; n < a.ncomp; ++n) {
-                double acc = 0.;
+                double acc[2][2][2] = {{{0., 0.}, {0., 0.}}, {{0., 0.}, {0., 0.}}};
                 bool ok = true;
                 for (int cz = 0; cz < 5; ++cz)
                     for (int by = 0; by < 5; ++by)
                         for (int ax = 0; ax < 5; ++ax) {
-                            const long long ce = locate(ax - 2, by - 2, cz - 2);
+                            const long long ce = elem(ax - 2, by - 2, cz - 2);
                             ok = ok && ce >= 0;
-                            acc += wz[cz] * wy[by] * wx[ax] * cval(a.comp[n], ce >= 0 ? ce : 0);
+                            const double v = cval(a.comp[n], ce >= 0 ? ce : 0);
+#pragma unroll
+                            for (int z = 0; z < 2; ++z)
+#pragma unroll
+                                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                                    for (int x = 0; x < 2; ++x)
+                                        acc[z][y][x] += (z ? WU[cz] : WU[4 - cz]) *
+                                                        (y ? WU[by] : WU[4 - by]) *
+                                                        (x ? WU[ax] : WU[4 - ax]) * v;
                         }
-                put(n, ok ? acc : __builtin_nan(""));
+                for (int z = c0[2]; z <= c1[2]; ++z)
+                    for (int y = c0[1]; y <= c1[1]; ++y)
+                        for (int x = c0[0]; x <= c1[0]; ++x)
+                            put(n, fine_elem(x, y, z), ok ? acc[z][y][x] : __builtin_nan(""));
             }
         } else {
             long long ce[NPT];
             bool ok = true;
 #pragma unroll
             for (int p = 0; p < NPT; ++p) {
-                ce[p] = locate(PO[p][0], PO[p][1], PO[p][2]);
+                ce[p] = elem(PO[p][0], PO[p][1], PO[p][2]);
                 ok = ok && ce[p] >= 0;
             }
-            const double xo = (gi & 1) ? 0.25 : -0.25, yo = (gj & 1) ? 0.25 : -0.25,
-                         zo = (gk & 1) ? 0.25 : -0.25;
             for (int n = 0; n < a.ncomp; ++n) {
                 double c[NPT];
 #pragma unroll
                 for (int p = 0; p < NPT; ++p) c[p] = cval(a.comp[n], ok ? ce[p] : 0);
-                double v;
+                // Slopes of the parent, shared by its children at x, y, z =
+                // +-1/4 coarse cells.
+                double sx = 0., sy = 0., sz = 0., sxx = 0., syy = 0., szz = 0., sxy = 0.,
+                       sxz = 0., syz = 0.;
                 if constexpr (INTERP == 2) {
-                    const double sx = 0.5 * (c[2] - c[1]), sy = 0.5 * (c[4] - c[3]),
-                                 sz = 0.5 * (c[6] - c[5]);
-                    const double sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
-                    const double syy = 0.5 * (c[4] - 2. * c[0] + c[3]);
-                    const double szz = 0.5 * (c[6] - 2. * c[0] + c[5]);
-                    const double sxy = 0.25 * (c[7] - c[8] - c[9] + c[10]);
-                    const double sxz = 0.25 * (c[11] - c[12] - c[13] + c[14]);
-                    const double syz = 0.25 * (c[15] - c[16] - c[17] + c[18]);
-                    v = c[0] + xo * sx + yo * sy + zo * sz + xo * xo * sxx + yo * yo * syy +
-                        zo * zo * szz + xo * yo * sxy + xo * zo * sxz + yo * zo * syz;
+                    sx = 0.5 * (c[2] - c[1]);
+                    sy = 0.5 * (c[4] - c[3]);
+                    sz = 0.5 * (c[6] - c[5]);
+                    sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
+                    syy = 0.5 * (c[4] - 2. * c[0] + c[3]);
+                    szz = 0.5 * (c[6] - 2. * c[0] + c[5]);
+                    sxy = 0.25 * (c[7] - c[8] - c[9] + c[10]);
+                    sxz = 0.25 * (c[11] - c[12] - c[13] + c[14]);
+                    syz = 0.25 * (c[15] - c[16] - c[17] + c[18]);
                 } else if constexpr (INTERP == 1) {
                     // Monotonized-central slopes per direction.
-                    auto mc = [](double cm, double c0, double cp) {
-                        const double sl = c0 - cm, sr = cp - c0;
+                    auto mc = [](double cm, double c0v, double cp) {
+                        const double sl = c0v - cm, sr = cp - c0v;
                         const double sc = 0.5 * (sl + sr);
                         if (!(sl * sr > 0.)) return 0.;
                         const double m = fmin(fabs(sc), fmin(2. * fabs(sl), 2. * fabs(sr)));
                         return sc > 0. ? m : -m;
                     };
-                    v = c[0] + xo * mc(c[1], c[0], c[2]) + yo * mc(c[3], c[0], c[4]) +
-                        zo * mc(c[5], c[0], c[6]);
-                } else {
-                    v = c[0];
+                    sx = mc(c[1], c[0], c[2]);
+                    sy = mc(c[3], c[0], c[4]);
+                    sz = mc(c[5], c[0], c[6]);
                 }
-                put(n, ok ? v : __builtin_nan(""));
+                for (int z = c0[2]; z <= c1[2]; ++z)
+                    for (int y = c0[1]; y <= c1[1]; ++y)
+                        for (int x = c0[0]; x <= c1[0]; ++x) {
+                            const double xo = x ? 0.25 : -0.25, yo = y ? 0.25 : -0.25,
+                                         zo = z ? 0.25 : -0.25;
+                            double v;
+                            if constexpr (INTERP == 2)
+                                v = c[0] + xo * sx + yo * sy + zo * sz + xo * xo * sxx +
+                                    yo * yo * syy + zo * zo * szz + xo * yo * sxy +
+                                    xo * zo * sxz + yo * zo * syz;
+                            else if constexpr (INTERP == 1)
+                                v = c[0] + xo * sx + yo * sy + zo * sz;
+                            else
+                                v = c[0];
+                            put(n, fine_elem(x, y, z), ok ? v : __builtin_nan(""));
+                        }
             }
         }
     }

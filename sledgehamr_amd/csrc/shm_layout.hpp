@@ -56,12 +56,21 @@ struct TePair {
 /** Coarse/fine ghost fill: what a ghost cell receives (see cf_fill_kernel). */
 enum CfMode { kCfState = 0, kCfK = 1, kCfForce = 2 };
 
+/** Coarse parent cells of a region of fine cells [flo, flo + fn) of fine
+ *  box dst_box: parent p covers fine cells 2p and 2p + 1 along each axis. */
+struct CfPSeg {
+    int dst_box;
+    int plo[3], pn[3];
+    int flo[3], fn[3];
+    long long start;  // prefix sum of parent cells
+};
+
 struct CfArgs {
     const BoxDesc *fboxes;
     const BoxDesc *cboxes;
-    const HaloSeg *segs;       // ghost-shell slabs; dst_box = fine box
+    const CfPSeg *segs;        // parents of ghost-shell slabs (or whole boxes)
     int nseg;
-    long long total;
+    long long total;           // parent cells
     const int *cand_start;     // per fine box: range in cand
     const int *cand;           // coarse boxes the box's coarse stencil touches
     int Nc;                    // coarse domain cells per axis

@@ -944,6 +944,23 @@ void Runtime::build_cf(int lev) {
     ls.cf_full.clear();
     ls.cf_full_total = 0;
     std::vector<int> start(1, 0), cand;
+    // The parent cells of fine cells [flo, flo + fn) of box b.
+    auto parents = [](std::vector<CfPSeg> &v, long long &tot, int b, const int *flo,
+                      const int *fn) {
+        CfPSeg p{};
+        p.dst_box = b;
+        long long n = 1;
+        for (int a = 0; a < 3; ++a) {
+            p.flo[a] = flo[a];
+            p.fn[a] = fn[a];
+            p.plo[a] = floordiv2(flo[a]);
+            p.pn[a] = floordiv2(flo[a] + fn[a] - 1) - p.plo[a] + 1;
+            n *= p.pn[a];
+        }
+        p.start = tot;
+        tot += n;
+        v.push_back(p);
+    };
     for (int b = 0; b < (int)ls.L.boxes.size(); ++b) {
         const Box3 &bx = ls.L.boxes[b];
         const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
@@ -954,29 +971,12 @@ void Runtime::build_cf(int lev) {
             {bx.lo[0] - g, bx.hi[1] + 1, bx.lo[2], nx + 2 * g, g, nz},
             {bx.lo[0] - g, bx.lo[1], bx.lo[2], g, ny, nz},
             {bx.hi[0] + 1, bx.lo[1], bx.lo[2], g, ny, nz}};
-        for (int q = 0; g > 0 && q < 6; ++q) {
-            HaloSeg sg{};
-            sg.dst_box = b;
-            sg.src_box = -1;
-            for (int a = 0; a < 3; ++a) {
-                sg.lo[a] = slab[q][a];
-                sg.n[a] = slab[q][3 + a];
-            }
-            sg.start = ls.cf_total;
-            ls.cf_total += (long long)sg.n[0] * sg.n[1] * sg.n[2];
-            ls.cf_segs.push_back(sg);
-        }
+        for (int q = 0; g > 0 && q < 6; ++q)
+            parents(ls.cf_segs, ls.cf_total, b, slab[q], slab[q] + 3);
         {
-            HaloSeg sg{};
-            sg.dst_box = b;
-            sg.src_box = -1;
-            for (int a = 0; a < 3; ++a) {
-                sg.lo[a] = bx.lo[a] - g;
-                sg.n[a] = bx.n(a) + 2 * g;
-            }
-            sg.start = ls.cf_full_total;
-            ls.cf_full_total += (long long)sg.n[0] * sg.n[1] * sg.n[2];
-            ls.cf_full.push_back(sg);
+            const int flo[3] = {bx.lo[0] - g, bx.lo[1] - g, bx.lo[2] - g};
+            const int fn[3] = {nx + 2 * g, ny + 2 * g, nz + 2 * g};
+            parents(ls.cf_full, ls.cf_full_total, b, flo, fn);
         }
         // Coarse cells the interpolation of this box's ghost cells reads.
         int glo[3], ghi[3];
@@ -1076,7 +1076,7 @@ CfArgs Runtime::cf_args(int lev, double time, bool first_state_only) {
     CfArgs a{};
     a.fboxes = ls.L.dbox.as<BoxDesc>();
     a.cboxes = cs.L.dbox.as<BoxDesc>();
-    a.segs = ls.dcf_segs.as<HaloSeg>();
+    a.segs = ls.dcf_segs.as<CfPSeg>();
     a.nseg = (int)ls.cf_segs.size();
     a.total = ls.cf_total;
     a.cand_start = ls.dcf_start.as<int>();
@@ -1149,7 +1149,7 @@ void Runtime::fill_coarse_patch(int lev) {
 void Runtime::fill_coarse_full(int lev, int which, double time, bool first_state_only) {
     LevelState &ls = level(lev);
     CfArgs a = cf_args(lev, time, first_state_only);
-    a.segs = ls.dcf_full.as<HaloSeg>();
+    a.segs = ls.dcf_full.as<CfPSeg>();
     a.nseg = (int)ls.cf_full.size();
     a.total = ls.cf_full_total;
     a.dst = ls.U[which].get();
@@ -1544,15 +1544,56 @@ void Runtime::create_shadow_level() {
     shadow_.t = shadow_.t_old + dt;
 }
 
+LevelState &Runtime::coarse_mirror(int lev) {
+    LevelState &fine = level(lev);
+    if (!fine.mirror || fine.mirror_epoch != epoch_) {
+        LevelState &cl = level(lev - 1);
+        auto m = std::make_unique<LevelState>();
+        m->L.build(coarsen(fine.L.boxes), 0, sim_->dimN[lev - 1], true);
+        m->ncomp = ncomp_;
+        m->U[0].alloc((size_t)m->L.cstride * ncomp_ * esize());
+        if (comp()) m->Ulo[0].alloc((size_t)m->L.cstride * ncomp_ * losize());
+        m->cur = 0;
+        const std::vector<HaloSeg> in = copy_plan(m->L, cl.L, &fine.mtot_in);
+        const std::vector<HaloSeg> out = copy_plan(cl.L, m->L, &fine.mtot_out);
+        upload_vec(fine.mplan_in, in);
+        upload_vec(fine.mplan_out, out);
+        fine.mnseg_in = (int)in.size();
+        fine.mnseg_out = (int)out.size();
+        fine.mirror = std::move(m);
+        fine.mirror_epoch = epoch_;
+    }
+    return *fine.mirror;
+}
+
+void Runtime::copy_state(void *dst, void *dstlo, const Layout &dl, const void *src,
+                         const void *srclo, const Layout &sl, const DevMem &plan,
+                         int nseg, long long total) {
+    if (total == 0) return;
+    CopyArgs c{};
+    c.dst = dst;
+    c.src = src;
+    c.dcstride = dl.cstride;
+    c.scstride = sl.cstride;
+    c.dboxes = dl.dbox.as<BoxDesc>();
+    c.sboxes = sl.dbox.as<BoxDesc>();
+    c.segs = plan.as<HaloSeg>();
+    c.nseg = nseg;
+    c.total = total;
+    c.ncomp = ncomp_;
+    launch_copy(c, esize(), ncu_, stream_);
+    if (comp()) {
+        c.dst = dstlo;
+        c.src = srclo;
+        launch_copy(c, losize(), ncu_, stream_);
+    }
+}
+
 void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
                      double *te_host) {
     LevelState &fine = level(lev);
     const double time = fine.t;
     LevelState *crse = nullptr;
-    LevelState mirror;
-    std::vector<HaloSeg> plan_in, plan_out;
-    long long tot_in = 0, tot_out = 0;
-    DevMem dplan_in, dplan_out;
     if (flags & kTeCompute) {
         if (lev == 0) {
             if (!shadow_.defined())
@@ -1563,39 +1604,11 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
             crse = &shadow_;
         } else {
             LevelState &cl = level(lev - 1);
-            mirror.L.build(coarsen(fine.L.boxes), 0, sim_->dimN[lev - 1], true);
-            mirror.ncomp = ncomp_;
-            mirror.U[0].alloc((size_t)mirror.L.cstride * ncomp_ * esize());
-            if (comp())
-                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * losize());
-            mirror.cur = 0;
-            plan_in = copy_plan(mirror.L, cl.L, &tot_in);
-            plan_out = copy_plan(cl.L, mirror.L, &tot_out);
-            upload_vec(dplan_in, plan_in);
-            upload_vec(dplan_out, plan_out);
-            auto run_copy = [&](void *dst, const void *src, const Layout &dl,
-                                const Layout &sl, const DevMem &plan, int nseg,
-                                long long total, size_t es) {
-                if (total == 0) return;
-                CopyArgs c{};
-                c.dst = dst;
-                c.src = src;
-                c.dcstride = dl.cstride;
-                c.scstride = sl.cstride;
-                c.dboxes = dl.dbox.as<BoxDesc>();
-                c.sboxes = sl.dbox.as<BoxDesc>();
-                c.segs = plan.as<HaloSeg>();
-                c.nseg = nseg;
-                c.total = total;
-                c.ncomp = ncomp_;
-                launch_copy(c, es, ncu_, stream_);
-            };
-            run_copy(mirror.U[0].get(), cl.U[cl.cur].get(), mirror.L, cl.L,
-                     dplan_in, (int)plan_in.size(), tot_in, esize());
-            if (comp())
-                run_copy(mirror.Ulo[0].get(), cl.Ulo[cl.cur].get(), mirror.L,
-                         cl.L, dplan_in, (int)plan_in.size(), tot_in, losize());
-            crse = &mirror;
+            LevelState &m = coarse_mirror(lev);
+            copy_state(m.U[0].get(), m.Ulo[0].get(), m.L, cl.U[cl.cur].get(),
+                       cl.Ulo[cl.cur].get(), cl.L, fine.mplan_in, fine.mnseg_in,
+                       fine.mtot_in);
+            crse = &m;
         }
     }
     // User criteria may read neighbours of a cell: complete every ghost cell
@@ -1656,25 +1669,11 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
         const int grid = (int)std::min<long long>((total + 255) / 256, 8LL * ncu_);
         SHM_HIP(fn(a, grid, stream_));
     }
-    if (crse && crse == &mirror && (flags & kTeAvgDown) && tot_out > 0) {
+    if (crse && lev >= 1 && (flags & kTeAvgDown)) {
         LevelState &cl = level(lev - 1);
-        CopyArgs c{};
-        c.dst = cl.U[cl.cur].get();
-        c.src = mirror.U[0].get();
-        c.dcstride = cl.L.cstride;
-        c.scstride = mirror.L.cstride;
-        c.dboxes = cl.L.dbox.as<BoxDesc>();
-        c.sboxes = mirror.L.dbox.as<BoxDesc>();
-        c.segs = dplan_out.as<HaloSeg>();
-        c.nseg = (int)plan_out.size();
-        c.total = tot_out;
-        c.ncomp = ncomp_;
-        launch_copy(c, esize(), ncu_, stream_);
-        if (comp()) {
-            c.dst = cl.Ulo[cl.cur].get();
-            c.src = mirror.Ulo[0].get();
-            launch_copy(c, losize(), ncu_, stream_);
-        }
+        copy_state(cl.U[cl.cur].get(), cl.Ulo[cl.cur].get(), cl.L, crse->U[0].get(),
+                   crse->Ulo[0].get(), crse->L, fine.mplan_out, fine.mnseg_out,
+                   fine.mtot_out);
         cl.ghosts_ok[cl.cur] = false;
     }
     if (flags & kTeTags) fine.tags_t = time;
@@ -2574,37 +2573,10 @@ void Runtime::synchronize_levels(int lev) {
         if (!(S.shadow_hierarchy && need_te)) {
             LevelState &fine = level(lev + 1);
             // amrex::average_down onto the coarse boxes covering each fine box.
-            LevelState mirror;
-            mirror.L.build(coarsen(fine.L.boxes), 0, S.dimN[lev], true);
-            mirror.ncomp = ncomp_;
-            mirror.U[0].alloc((size_t)mirror.L.cstride * ncomp_ * esize());
-            if (comp())
-                mirror.Ulo[0].alloc((size_t)mirror.L.cstride * ncomp_ * losize());
-            avg_down(fine, fine.cur, mirror, 0);
-            long long tot = 0;
-            std::vector<HaloSeg> plan = copy_plan(ls.L, mirror.L, &tot);
-            DevMem dplan;
-            upload_vec(dplan, plan);
-            if (tot > 0) {
-                CopyArgs c{};
-                c.dst = ls.U[ls.cur].get();
-                c.src = mirror.U[0].get();
-                c.dcstride = ls.L.cstride;
-                c.scstride = mirror.L.cstride;
-                c.dboxes = ls.L.dbox.as<BoxDesc>();
-                c.sboxes = mirror.L.dbox.as<BoxDesc>();
-                c.segs = dplan.as<HaloSeg>();
-                c.nseg = (int)plan.size();
-                c.total = tot;
-                c.ncomp = ncomp_;
-                launch_copy(c, esize(), ncu_, stream_);
-                if (comp()) {
-                    c.dst = ls.Ulo[ls.cur].get();
-                    c.src = mirror.Ulo[0].get();
-                    launch_copy(c, losize(), ncu_, stream_);
-                }
-            }
-            sync();
+            LevelState &m = coarse_mirror(lev + 1);
+            avg_down(fine, fine.cur, m, 0);
+            copy_state(ls.U[ls.cur].get(), ls.Ulo[ls.cur].get(), ls.L, m.U[0].get(),
+                       m.Ulo[0].get(), m.L, fine.mplan_out, fine.mnseg_out, fine.mtot_out);
             ls.ghosts_ok[ls.cur] = false;
         }
     }

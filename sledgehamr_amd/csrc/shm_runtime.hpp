@@ -105,17 +105,25 @@ struct LevelState {
     // Coarse/fine ghost fill plan (levels above 0): the ghost-shell slabs of
     // the boxes, and per box the boxes of the coarser level that its coarse
     // interpolation stencil may touch.
-    std::vector<HaloSeg> cf_segs;
+    std::vector<CfPSeg> cf_segs;
     DevMem dcf_segs, dcf_start, dcf_cand;
     long long cf_total = 0;
     // Every cell of each ghosted box (FillCoarsePatch).
-    std::vector<HaloSeg> cf_full;
+    std::vector<CfPSeg> cf_full;
     DevMem dcf_full;
     long long cf_full_total = 0;
     bool cf_built = false;
     // Mask of the last tagging (one byte per valid cell) and its time.
     DevMem tags;
     double tags_t = -DBL_MAX;
+    // This level's boxes coarsened onto level lev-1 (average-down and
+    // truncation errors), with copy plans from and to level lev-1; rebuilt
+    // when the epoch changes.
+    std::unique_ptr<LevelState> mirror;
+    DevMem mplan_in, mplan_out;
+    int mnseg_in = 0, mnseg_out = 0;
+    long long mtot_in = 0, mtot_out = 0;
+    long mirror_epoch = -1;
     bool defined() const { return !L.boxes.empty(); }
 };
 
@@ -328,6 +336,14 @@ class Runtime {
     std::vector<uint8_t> block_cover(int lev) const;
     void did_global_regrid(int lev);
     void check_bf(int lev) const;
+    /** The coarsened copy of level lev (>= 1) on level lev-1, built once
+     *  per epoch with its copy plans. */
+    LevelState &coarse_mirror(int lev);
+    /** Copy of ncomp components along a plan between two layouts, for the
+     *  state (es = element size) and, compensated, its remainder. */
+    void copy_state(void *dst, void *dstlo, const Layout &dl, const void *src,
+                    const void *srclo, const Layout &sl, const DevMem &plan, int nseg,
+                    long long total);
     void check_new_boxes(int lev, const std::vector<Box3> &boxes,
                          const std::vector<Box3> &keep) const;
     void integrate(LevelState &ls, int lev, double dt, double dx,

@@ -159,3 +159,25 @@ AMR_LEV2 = [[20, 20, 20, 27, 27, 27], [30, 20, 20, 43, 27, 27]]
 # Level-2 boxes 6 level-2 cells inside level 1: the quartic reaches 2 coarse
 # cells beyond the parent, so it needs 3 level-1 cells of margin.
 AMR_LEV2_Q = [[22, 22, 22, 25, 25, 25], [28, 22, 22, 41, 25, 25]]
+
+
+def amr_batched_boxes(seed, n1=2000, base=256, margin=6):
+    """Static nested layout of the batched AMR workload.  Level 1 (2 base
+    cells per axis) holds n1 boxes, one in each of n1 seeded 32-cell slots,
+    even edges 16..32 at even offsets inside the slot; level 2 holds one box
+    per level-1 box, even edges 16 .. min(64, 2 e1 - 2 margin), at least
+    `margin` level-2 cells inside it (margin 6 serves 3 ghost cells)."""
+    rng = np.random.default_rng(seed)
+    ns = 2 * base // 32
+    lev1, lev2 = [], []
+    for s in sorted(rng.choice(ns ** 3, n1, replace=False)):
+        slot = (s % ns, (s // ns) % ns, s // (ns * ns))
+        e1 = int(rng.choice([16, 20, 24, 28, 32]))
+        lo = [32 * c + 2 * int(rng.integers(0, (32 - e1) // 2 + 1)) for c in slot]
+        lev1.append([*lo, *(v + e1 - 1 for v in lo)])
+        emax = min(64, 2 * e1 - 2 * margin)
+        e2 = 2 * int(rng.integers(8, emax // 2 + 1))
+        room = 2 * e1 - 2 * margin - e2
+        lo2 = [2 * v + margin + 2 * int(rng.integers(0, room // 2 + 1)) for v in lo]
+        lev2.append([*lo2, *(v + e2 - 1 for v in lo2)])
+    return lev1, lev2
