@@ -334,175 +334,192 @@ __global__ void __launch_bounds__(256) avgdown_kernel(const AvgArgs a) {
 // ---------------------------------------------------------------------------
 
 __device__ AMREX_FORCE_INLINE int cf_wrap(int v, int n) {
-    const int r = v % n;
-    return r < 0 ? r + n : r;
+    // Coarse indices lie within a period of the domain: no division.
+    while (v < 0) v += n;
+    while (v >= n) v -= n;
+    return v;
 }
+
+/** The 3D indices (x fastest, nx x ny columns) of first, first + step,
+ *  first + 2 step, ... without a division per step. */
+struct Walk3 {
+    int x, y, z, dx, dy, dz, nx, ny;
+    __device__ Walk3(int first, int step, int nx_, int ny_) : nx(nx_), ny(ny_) {
+        x = first % nx;
+        y = (first / nx) % ny;
+        z = first / (nx * ny);
+        dx = step % nx;
+        dy = (step / nx) % ny;
+        dz = step / (nx * ny);
+    }
+    __device__ void next() {
+        x += dx;
+        int c = x >= nx;
+        if (c) x -= nx;
+        y += dy + c;
+        c = y >= ny;
+        if (c) y -= ny;
+        z += dz + c;
+    }
+};
 
 template <Prec P, int MODE, int INTERP>
 __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
     using T = PrecTraits<P>;
     using S = typename T::S;
+    // Time-weighted coarse values of the tile's brick: its parents grown by
+    // the stencil reach, x fastest.
+    extern __shared__ double brick[];
     // Points of the type 0 / 1 / 2 stencils: centre, 6 faces, then the 12
     // edges as (+,+) (-,+) (+,-) (-,-) pairs.
     constexpr int NPT = INTERP == 0 ? 1 : INTERP == 1 ? 7 : 19;
-    constexpr int REACH = INTERP == 0 ? 0 : INTERP == 4 ? 2 : 1;
+    constexpr int R = INTERP == 0 ? 0 : INTERP == 4 ? 2 : 1;
     constexpr int PO[19][3] = {
         {0, 0, 0},   {-1, 0, 0}, {1, 0, 0},  {0, -1, 0}, {0, 1, 0},
         {0, 0, -1},  {0, 0, 1},  {1, 1, 0},  {-1, 1, 0}, {1, -1, 0},
         {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
         {0, 1, 1},   {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+    const CfTile tl = a.tiles[blockIdx.x];
+    const CfPSeg &sg = a.segs[tl.seg];
+    const int fb = sg.dst_box;
+    const BoxDesc &db = a.fboxes[fb];
+    const int bxn = tl.pn[0] + 2 * R, byn = tl.pn[1] + 2 * R;
+    const int nb = bxn * byn * (tl.pn[2] + 2 * R);
+    const int np = tl.pn[0] * tl.pn[1] * tl.pn[2];
+    const int cs0 = a.cand_start[fb], ncand = a.cand_start[fb + 1] - cs0;
     // Coarse states and their time weights.
     const double wo = a.rec[0], wn = a.rec[1];
     const bool use_old = a.rec[2] != 0., use_new = a.rec[3] != 0.;
     const int ns = a.rec[4] != 0. ? 1 : 0;
     const void *cnew = a.cs[ns], *cnewlo = a.cslo[ns];
     const void *cold = a.cs[ns ^ 1], *coldlo = a.cslo[ns ^ 1];
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    // One thread per coarse parent cell: its stencil is located and loaded
-    // once for the (up to 8) fine children inside the segment.
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-         t < a.total; t += stride) {
-        int lo = 0, hi = a.nseg - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.segs[mid].start <= t)
-                lo = mid;
-            else
-                hi = mid - 1;
+    auto cval = [&](int comp, long long ce) -> double {
+        const long long e = (long long)comp * a.ccstride + ce;
+        double v = 0.;
+        if (use_new) v = load_state_at<T>(cnew, cnewlo, e);
+        if (use_old) {
+            const double vo = load_state_at<T>(cold, coldlo, e);
+            v = use_new ? wo * vo + wn * v : vo;
         }
-        const CfPSeg &sg = a.segs[lo];
-        long long r = t - sg.start;
-        const int ic = sg.plo[0] + (int)(r % sg.pn[0]);
-        r /= sg.pn[0];
-        const int jc = sg.plo[1] + (int)(r % sg.pn[1]);
-        const int kc = sg.plo[2] + (int)(r / sg.pn[1]);
-        const int fb = sg.dst_box;
-        const BoxDesc &db = a.fboxes[fb];
-        // Children 2 p + c of the parent inside the segment, per axis.
-        int c0[3], c1[3];
-        const int pc[3] = {ic, jc, kc};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            c0[d] = max(0, sg.flo[d] - 2 * pc[d]);
-            c1[d] = min(1, sg.flo[d] + sg.fn[d] - 1 - 2 * pc[d]);
-        }
-        // Element of coarse cell (ic + di, jc + dj, kc + dk), periodic image,
-        // among the candidate boxes; -1 if none holds it.
-        auto locate = [&](int di, int dj, int dk) -> long long {
-            const int ci = cf_wrap(ic + di, a.Nc), cj = cf_wrap(jc + dj, a.Nc),
-                      ck = cf_wrap(kc + dk, a.Nc);
-            for (int q = a.cand_start[fb]; q < a.cand_start[fb + 1]; ++q) {
-                const BoxDesc &cb = a.cboxes[a.cand[q]];
-                if (ci >= cb.lo[0] && ci < cb.lo[0] + cb.n[0] && cj >= cb.lo[1] &&
-                    cj < cb.lo[1] + cb.n[1] && ck >= cb.lo[2] && ck < cb.lo[2] + cb.n[2])
-                    return cb.base + (ci - cb.lo[0]) + (long long)(cj - cb.lo[1]) * cb.jstride +
-                           (long long)(ck - cb.lo[2]) * cb.kstride;
+        return v;
+    };
+    // What the ghost cell at fine element fe receives for component slot n.
+    auto put = [&](int n, long long fe, double v) {
+        const int comp = a.comp[n];
+        if constexpr (MODE == kCfState) {
+            const long long e = (long long)comp * a.fcstride + fe;
+            // Without a remainder array (dstlo null) only the leading word.
+            if constexpr (T::comp) {
+                if (a.dstlo)
+                    store_state_at<T>(a.dst, a.dstlo, e, v);
+                else
+                    st<float>(a.dst, e, 0u, (float)v);
+            } else {
+                store_state_at<T>(a.dst, a.dstlo, e, v);
             }
-            return -1;
-        };
-        // The box holding the parent usually holds its whole stencil: then
-        // every point is an offset from the parent's element (boxes do not
-        // overlap, so it is the element the search would find).
-        long long cc = -1;
-        int cjs = 0, cks = 0;
-        {
-            const int ci = cf_wrap(ic, a.Nc), cj = cf_wrap(jc, a.Nc), ck = cf_wrap(kc, a.Nc);
-            for (int q = a.cand_start[fb]; q < a.cand_start[fb + 1]; ++q) {
-                const BoxDesc &cb = a.cboxes[a.cand[q]];
+        } else if constexpr (MODE == kCfK) {
+            const long long e = (long long)comp * a.fcstride + fe;
+            const double u0 = (double)ld<S>(a.u0, e, 0u);
+            st<S>(a.dst, e, 0u, (S)((v - u0) / (double)(S)a.c0));
+        } else {
+            double yv = (double)ld<S>(a.u0, (long long)comp * a.fcstride + fe, 0u) +
+                        (double)(S)a.c0 *
+                            (double)ld<S>(a.u0, (long long)a.mom[n] * a.fcstride + fe, 0u);
+            const long long pe = (long long)a.pair[n] * a.fcstride + fe;
+            for (int j = 0; j < a.m; ++j)
+                yv += (double)(S)a.cf[j] * (double)ld<S>(a.f[j], pe, 0u);
+            st<S>(a.dst, pe, 0u, (S)((v - yv) / (double)(S)a.cf[a.m]));
+        }
+    };
+    for (int n = 0; n < a.ncomp; ++n) {
+        const int comp = a.comp[n];
+        __syncthreads();  // the previous component's brick has been read
+        // Brick cells, coalesced along x: the periodic image of each coarse
+        // cell in the candidate boxes of the fine box (every one is held:
+        // nesting is checked when the plan is built).
+        int hint = 0;
+        Walk3 w(threadIdx.x, blockDim.x, bxn, byn);
+        for (int q = threadIdx.x; q < nb; q += blockDim.x, w.next()) {
+            const int ci = cf_wrap(tl.plo[0] - R + w.x, a.Nc);
+            const int cj = cf_wrap(tl.plo[1] - R + w.y, a.Nc);
+            const int ck = cf_wrap(tl.plo[2] - R + w.z, a.Nc);
+            long long e = -1;
+            for (int k = 0; k < ncand && e < 0; ++k) {
+                const int qq = hint + k < ncand ? hint + k : hint + k - ncand;
+                const BoxDesc &cb = a.cboxes[a.cand[cs0 + qq]];
                 if (ci >= cb.lo[0] && ci < cb.lo[0] + cb.n[0] && cj >= cb.lo[1] &&
                     cj < cb.lo[1] + cb.n[1] && ck >= cb.lo[2] && ck < cb.lo[2] + cb.n[2]) {
-                    if (ci - REACH >= cb.lo[0] && ci + REACH < cb.lo[0] + cb.n[0] &&
-                        cj - REACH >= cb.lo[1] && cj + REACH < cb.lo[1] + cb.n[1] &&
-                        ck - REACH >= cb.lo[2] && ck + REACH < cb.lo[2] + cb.n[2]) {
-                        cc = cb.base + (ci - cb.lo[0]) +
-                             (long long)(cj - cb.lo[1]) * cb.jstride +
-                             (long long)(ck - cb.lo[2]) * cb.kstride;
-                        cjs = cb.jstride;
-                        cks = cb.kstride;
-                    }
-                    break;
+                    e = cb.base + (ci - cb.lo[0]) + (long long)(cj - cb.lo[1]) * cb.jstride +
+                        (long long)(ck - cb.lo[2]) * cb.kstride;
+                    hint = qq;
                 }
             }
+            brick[q] = e >= 0 ? cval(comp, e) : __builtin_nan("");
         }
-        auto elem = [&](int di, int dj, int dk) -> long long {
-            return cc >= 0 ? cc + di + (long long)dj * cjs + (long long)dk * cks
-                           : locate(di, dj, dk);
-        };
-        auto cval = [&](int comp, long long ce) -> double {
-            const long long e = (long long)comp * a.ccstride + ce;
-            double v = 0.;
-            if (use_new) v = load_state_at<T>(cnew, cnewlo, e);
-            if (use_old) {
-                const double vo = load_state_at<T>(cold, coldlo, e);
-                v = use_new ? wo * vo + wn * v : vo;
+        __syncthreads();
+        Walk3 pw(threadIdx.x, blockDim.x, tl.pn[0], tl.pn[1]);
+        for (int p = threadIdx.x; p < np; p += blockDim.x, pw.next()) {
+            const int px = pw.x, py = pw.y, pz = pw.z;
+            const int pc[3] = {tl.plo[0] + px, tl.plo[1] + py, tl.plo[2] + pz};
+            // Children 2 p + c of the parent inside the segment, per axis.
+            int c0[3], c1[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                c0[d] = max(0, sg.flo[d] - 2 * pc[d]);
+                c1[d] = min(1, sg.flo[d] + sg.fn[d] - 1 - 2 * pc[d]);
             }
-            return v;
-        };
-        // Fine element of child (cx, cy, cz).
-        auto fine_elem = [&](int cx, int cy, int cz) -> long long {
-            return db.base + (2 * ic + cx - db.lo[0]) +
-                   (long long)(2 * jc + cy - db.lo[1]) * db.jstride +
-                   (long long)(2 * kc + cz - db.lo[2]) * db.kstride;
-        };
-        // What the ghost cell at fine element fe receives for component slot n.
-        auto put = [&](int n, long long fe, double v) {
-            const int comp = a.comp[n];
-            if constexpr (MODE == kCfState) {
-                store_state_at<T>(a.dst, a.dstlo, (long long)comp * a.fcstride + fe, v);
-            } else if constexpr (MODE == kCfK) {
-                const long long e = (long long)comp * a.fcstride + fe;
-                const double u0 = (double)ld<S>(a.u0, e, 0u);
-                st<S>(a.dst, e, 0u, (S)((v - u0) / (double)(S)a.c0));
-            } else {
-                double yv = (double)ld<S>(a.u0, (long long)comp * a.fcstride + fe, 0u) +
-                            (double)(S)a.c0 *
-                                (double)ld<S>(a.u0, (long long)a.mom[n] * a.fcstride + fe, 0u);
-                const long long pe = (long long)a.pair[n] * a.fcstride + fe;
-                for (int j = 0; j < a.m; ++j)
-                    yv += (double)(S)a.cf[j] * (double)ld<S>(a.f[j], pe, 0u);
-                st<S>(a.dst, pe, 0u, (S)((v - yv) / (double)(S)a.cf[a.m]));
-            }
-        };
-        if constexpr (INTERP == 4) {
-            // Tensor product of the conservative quartic child weights; an
-            // upper child (odd fine index) takes WU, a lower one its mirror.
-            constexpr double WU[5] = {3. / 128., -22. / 128., 1., 22. / 128., -3. / 128.};
-            for (int n = 0; n < a.ncomp; ++n) {
+            const int b0 = (px + R) + bxn * ((py + R) + byn * (pz + R));
+            auto C = [&](int di, int dj, int dk) -> double {
+                return brick[b0 + di + bxn * (dj + byn * dk)];
+            };
+            auto fine_elem = [&](int cx, int cy, int cz) -> long long {
+                return db.base + (2 * pc[0] + cx - db.lo[0]) +
+                       (long long)(2 * pc[1] + cy - db.lo[1]) * db.jstride +
+                       (long long)(2 * pc[2] + cz - db.lo[2]) * db.kstride;
+            };
+            if constexpr (INTERP == 4) {
+                // Tensor product of the conservative quartic child weights;
+                // an upper child (odd fine index) takes WU, a lower one its
+                // mirror.
+                // WU = (3, -22, 128, 22, -3) / 128 as selects (no indexed
+                // array, which would live in scratch memory).
+                auto wu = [](int q) -> double {
+                    return q == 0 ? 3. / 128. : q == 1 ? -22. / 128. : q == 2 ? 1.
+                                              : q == 3 ? 22. / 128. : -3. / 128.;
+                };
                 double acc[2][2][2] = {{{0., 0.}, {0., 0.}}, {{0., 0.}, {0., 0.}}};
-                bool ok = true;
+#pragma unroll 1
                 for (int cz = 0; cz < 5; ++cz)
+#pragma unroll 1
                     for (int by = 0; by < 5; ++by)
+#pragma unroll
                         for (int ax = 0; ax < 5; ++ax) {
-                            const long long ce = elem(ax - 2, by - 2, cz - 2);
-                            ok = ok && ce >= 0;
-                            const double v = cval(a.comp[n], ce >= 0 ? ce : 0);
+                            const double v = C(ax - 2, by - 2, cz - 2);
+                            const double wz[2] = {wu(4 - cz), wu(cz)};
+                            const double wy[2] = {wu(4 - by), wu(by)};
+                            const double wx[2] = {wu(4 - ax), wu(ax)};
 #pragma unroll
                             for (int z = 0; z < 2; ++z)
 #pragma unroll
                                 for (int y = 0; y < 2; ++y)
 #pragma unroll
                                     for (int x = 0; x < 2; ++x)
-                                        acc[z][y][x] += (z ? WU[cz] : WU[4 - cz]) *
-                                                        (y ? WU[by] : WU[4 - by]) *
-                                                        (x ? WU[ax] : WU[4 - ax]) * v;
+                                        acc[z][y][x] += wz[z] * wy[y] * wx[x] * v;
                         }
-                for (int z = c0[2]; z <= c1[2]; ++z)
-                    for (int y = c0[1]; y <= c1[1]; ++y)
-                        for (int x = c0[0]; x <= c1[0]; ++x)
-                            put(n, fine_elem(x, y, z), ok ? acc[z][y][x] : __builtin_nan(""));
-            }
-        } else {
-            long long ce[NPT];
-            bool ok = true;
+                // Compile-time child indices keep acc in registers.
 #pragma unroll
-            for (int p = 0; p < NPT; ++p) {
-                ce[p] = elem(PO[p][0], PO[p][1], PO[p][2]);
-                ok = ok && ce[p] >= 0;
-            }
-            for (int n = 0; n < a.ncomp; ++n) {
+                for (int z = 0; z < 2; ++z)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y)
+#pragma unroll
+                        for (int x = 0; x < 2; ++x)
+                            if (z >= c0[2] && z <= c1[2] && y >= c0[1] && y <= c1[1] &&
+                                x >= c0[0] && x <= c1[0])
+                                put(n, fine_elem(x, y, z), acc[z][y][x]);
+            } else {
                 double c[NPT];
 #pragma unroll
-                for (int p = 0; p < NPT; ++p) c[p] = cval(a.comp[n], ok ? ce[p] : 0);
+                for (int q = 0; q < NPT; ++q) c[q] = C(PO[q][0], PO[q][1], PO[q][2]);
                 // Slopes of the parent, shared by its children at x, y, z =
                 // +-1/4 coarse cells.
                 double sx = 0., sy = 0., sz = 0., sxx = 0., syy = 0., szz = 0., sxy = 0.,
@@ -544,7 +561,7 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
                                 v = c[0] + xo * sx + yo * sy + zo * sz;
                             else
                                 v = c[0];
-                            put(n, fine_elem(x, y, z), ok ? v : __builtin_nan(""));
+                            put(n, fine_elem(x, y, z), v);
                         }
             }
         }

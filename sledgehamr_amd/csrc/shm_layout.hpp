@@ -65,11 +65,20 @@ struct CfPSeg {
     long long start;  // prefix sum of parent cells
 };
 
+/** A workgroup's share of a CfPSeg: parents plo .. plo + pn - 1. */
+struct CfTile {
+    int seg;
+    int plo[3], pn[3];
+    int pad_;
+};
+
 struct CfArgs {
     const BoxDesc *fboxes;
     const BoxDesc *cboxes;
     const CfPSeg *segs;        // parents of ghost-shell slabs (or whole boxes)
-    int nseg;
+    const CfTile *tiles;       // one per workgroup
+    int ntile;
+    int lds_cells;             // largest coarse brick of a tile (doubles)
     long long total;           // parent cells
     const int *cand_start;     // per fine box: range in cand
     const int *cand;           // coarse boxes the box's coarse stencil touches
@@ -82,7 +91,8 @@ struct CfArgs {
     const double *rec;
     int ncomp;
     int comp[32];              // interpolated component (fine = coarse index)
-    void *dst, *dstlo;         // state (kCfState), k (kCfK), F_m (kCfForce)
+    void *dst, *dstlo;         // state (kCfState; dstlo null: leading words
+                               // only), k (kCfK), F_m (kCfForce)
     const void *u0;            // fine u0 / phi0, pi0 (kCfK, kCfForce)
     double c0;                 // kCfK: c; kCfForce: coefficient of pi0
     int mom[32];               // kCfForce: momentum component of each pair

@@ -912,9 +912,18 @@ void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps,
     }
 }
 
-void Runtime::fill_patch(LevelState &ls, int which, int lev, double time) {
+void Runtime::fill_patch(LevelState &ls, int which, int lev, double time, bool for_step) {
     if (has_cf(lev)) {
         // The coarser level may have changed (its step, average-down).
+        if (for_step) {
+            // A step reads the leading words of step_ghost_comps() at ghost
+            // cells; the remainders and other components stay as they are.
+            const std::vector<int> comps = step_ghost_comps();
+            cf_fill_state(lev, time, ls.U[which].get(), nullptr, comps);
+            halo(ls, ls.U[which].get(), comps);
+            ls.ghosts_ok[which] = false;
+            return;
+        }
         cf_fill_state(lev, time, ls.U[which].get(),
                       comp() ? ls.Ulo[which].get() : nullptr);
     } else if (ls.ghosts_ok[which]) {
@@ -922,6 +931,11 @@ void Runtime::fill_patch(LevelState &ls, int which, int lev, double time) {
     }
     halo(ls, ls.U[which].get(), all_comps());
     ls.ghosts_ok[which] = true;
+}
+
+std::vector<int> Runtime::step_ghost_comps() const {
+    if (ko_ || integrator_ >= 20) return all_comps();
+    return stencil_comps();
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,6 +1030,36 @@ void Runtime::build_cf(int lev) {
     }
     upload_vec(ls.dcf_segs, ls.cf_segs);
     upload_vec(ls.dcf_full, ls.cf_full);
+    // Workgroup tiles: at most 32 x 8 x 4 parents (16 x 8 x 4 for the
+    // quartic's reach 2), so the coarse brick of a tile, its parents grown
+    // by the reach, takes at most 16 KB of LDS as doubles and several
+    // workgroups share a CU.
+    auto tile = [s](const std::vector<CfPSeg> &v, DevMem &d, int &n, int &lds) {
+        const int lim[3] = {s > 1 ? 16 : 32, 8, 4};
+        std::vector<CfTile> t;
+        lds = 0;
+        for (int q = 0; q < (int)v.size(); ++q)
+            for (int z = 0; z < v[q].pn[2]; z += lim[2])
+                for (int y = 0; y < v[q].pn[1]; y += lim[1])
+                    for (int x = 0; x < v[q].pn[0]; x += lim[0]) {
+                        CfTile c{};
+                        c.seg = q;
+                        const int o[3] = {x, y, z};
+                        int cells = 1;
+                        for (int a = 0; a < 3; ++a) {
+                            c.plo[a] = v[q].plo[a] + o[a];
+                            c.pn[a] = std::min(lim[a], v[q].pn[a] - o[a]);
+                            cells *= c.pn[a] + 2 * s;
+                        }
+                        lds = std::max(lds, cells);
+                        t.push_back(c);
+                    }
+        if (t.size() > (size_t)INT_MAX) throw Error(1, "too many coarse/fine tiles");
+        n = (int)t.size();
+        upload_vec(d, t);
+    };
+    tile(ls.cf_segs, ls.dcf_tiles, ls.cf_ntile, ls.cf_lds);
+    tile(ls.cf_full, ls.dcf_full_tiles, ls.cf_full_ntile, ls.cf_full_lds);
     upload_vec(ls.dcf_start, start);
     if (cand.empty()) cand.push_back(0);
     upload_vec(ls.dcf_cand, cand);
@@ -1077,7 +1121,9 @@ CfArgs Runtime::cf_args(int lev, double time, bool first_state_only) {
     a.fboxes = ls.L.dbox.as<BoxDesc>();
     a.cboxes = cs.L.dbox.as<BoxDesc>();
     a.segs = ls.dcf_segs.as<CfPSeg>();
-    a.nseg = (int)ls.cf_segs.size();
+    a.tiles = ls.dcf_tiles.as<CfTile>();
+    a.ntile = ls.cf_ntile;
+    a.lds_cells = ls.cf_lds;
     a.total = ls.cf_total;
     a.cand_start = ls.dcf_start.as<int>();
     a.cand = ls.dcf_cand.as<int>();
@@ -1096,18 +1142,19 @@ CfArgs Runtime::cf_args(int lev, double time, bool first_state_only) {
 
 template <Prec P, int M>
 static void cf_dispatch(const CfArgs &a, int interp, int grid, hipStream_t s) {
+    const size_t lds = sizeof(double) * (size_t)a.lds_cells;
     switch (interp) {
         case 0:
-            hipLaunchKernelGGL((cf_fill_kernel<P, M, 0>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 0>), dim3(grid), dim3(256), lds, s, a);
             break;
         case 1:
-            hipLaunchKernelGGL((cf_fill_kernel<P, M, 1>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 1>), dim3(grid), dim3(256), lds, s, a);
             break;
         case 2:
-            hipLaunchKernelGGL((cf_fill_kernel<P, M, 2>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 2>), dim3(grid), dim3(256), lds, s, a);
             break;
         default:
-            hipLaunchKernelGGL((cf_fill_kernel<P, M, 4>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((cf_fill_kernel<P, M, 4>), dim3(grid), dim3(256), lds, s, a);
             break;
     }
 }
@@ -1125,7 +1172,8 @@ static void cf_dispatch_mode(const CfArgs &a, int mode, int interp, int grid,
 
 void Runtime::cf_launch(CfArgs &a, int mode) {
     if (a.total == 0 || a.ncomp == 0) return;
-    const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
+    const int grid = a.ntile;  // one workgroup per tile
+    if (a.lds_cells * sizeof(double) > 65536) throw Error(5, "coarse/fine brick exceeds LDS");
     const int ip = interp_type_;
     EventPair *ep = timing_on ? timed_pair(1) : nullptr;
     if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
@@ -1150,7 +1198,9 @@ void Runtime::fill_coarse_full(int lev, int which, double time, bool first_state
     LevelState &ls = level(lev);
     CfArgs a = cf_args(lev, time, first_state_only);
     a.segs = ls.dcf_full.as<CfPSeg>();
-    a.nseg = (int)ls.cf_full.size();
+    a.tiles = ls.dcf_full_tiles.as<CfTile>();
+    a.ntile = ls.cf_full_ntile;
+    a.lds_cells = ls.cf_full_lds;
     a.total = ls.cf_full_total;
     a.dst = ls.U[which].get();
     a.dstlo = comp() ? ls.Ulo[which].get() : nullptr;
@@ -1162,13 +1212,15 @@ void Runtime::fill_coarse_full(int lev, int which, double time, bool first_state
     ls.ghosts_ok[which] = false;
 }
 
-void Runtime::cf_fill_state(int lev, double time, void *hi, void *lo) {
+void Runtime::cf_fill_state(int lev, double time, void *hi, void *lo,
+                            const std::vector<int> &comps) {
+    const std::vector<int> cs = comps.empty() ? all_comps() : comps;
     CfArgs a = cf_args(lev, time);
     a.dst = hi;
     a.dstlo = lo;
-    for (int c0 = 0; c0 < ncomp_; c0 += 32) {
-        a.ncomp = std::min(32, ncomp_ - c0);
-        for (int n = 0; n < a.ncomp; ++n) a.comp[n] = c0 + n;
+    for (size_t c0 = 0; c0 < cs.size(); c0 += 32) {
+        a.ncomp = (int)std::min<size_t>(32, cs.size() - c0);
+        for (int n = 0; n < a.ncomp; ++n) a.comp[n] = cs[c0 + n];
         cf_launch(a, kCfState);
     }
 }
@@ -1343,7 +1395,8 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         a.c[0] = dt / 2.;
         a.c[1] = dt;
         launch_stage(kLeap1, ls, a, lev);
-        if (cf) cf_fill_state(lev, st[1], ls.U[nw].get(), ulo_new);
+        // The second kick reads the fields' leading words at ghost cells.
+        if (cf) cf_fill_state(lev, st[1], ls.U[nw].get(), nullptr, stencil_comps());
         halo(ls, ls.U[nw].get(), stencil_comps());
 
         StageArgs b{};
@@ -1355,10 +1408,12 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         b.c[0] = dt / 2.;
         launch_stage(kLeap2, ls, b, lev);
     }
-    // FillIntermediatePatch at the end of the step.
-    if (cf) cf_fill_state(lev, t0 + dt, ls.U[nw].get(), ulo_new);
-    halo(ls, ls.U[nw].get(), all_comps());
-    ls.ghosts_ok[nw] = true;
+    // FillIntermediatePatch at the end of the step.  On levels with
+    // coarse/fine ghost cells the next step fills them again, so the fill is
+    // left to what reads them in between (tagging, outputs), with the same
+    // coarse data and time.
+    if (!cf) halo(ls, ls.U[nw].get(), all_comps());
+    ls.ghosts_ok[nw] = !cf;
 }
 
 std::vector<double> Runtime::stage_times(double t0, double dt) const {
@@ -1423,7 +1478,7 @@ bool Runtime::replay_step(LevelState &ls, int lev, double dt, double dx, double 
         SHM_HIP(ie);
         g.epoch = epoch_;
     } else {
-        ls.ghosts_ok[ls.cur] = true;  // what integrate() records eagerly
+        ls.ghosts_ok[ls.cur] = !has_cf(lev);  // what integrate() records eagerly
     }
     // The previous replay of this graph has read its pinned records.
     SHM_HIP(hipEventSynchronize(g.done));
@@ -1449,7 +1504,7 @@ void Runtime::advance(int lev) {
     const int old = ls.cur ^ 1;
     ls.t_old = ls.t;
     const double dt = sim_->dt[lev], dx = sim_->dx[lev];
-    fill_patch(ls, old, lev, ls.t_old);
+    fill_patch(ls, old, lev, ls.t_old, /*for_step=*/true);
     if (!replay_step(ls, lev, dt, dx, ls.t_old)) {
         integrate(ls, lev, dt, dx, ls.t_old);
         if (lev >= 0) warm_[lev] = epoch_;
@@ -1613,6 +1668,11 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     }
     // User criteria may read neighbours of a cell: complete every ghost cell
     // of the fine state first (stage kernels wrap periodic boxes instead).
+    // The level's end-of-step coarse/fine ghost fill (FillIntermediatePatch
+    // at its time), then the same-level ghosts.
+    if (has_cf(lev))
+        cf_fill_state(lev, time, fine.U[fine.cur].get(),
+                      comp() ? fine.Ulo[fine.cur].get() : nullptr);
     halo(fine, fine.U[fine.cur].get(), all_comps(), /*full=*/true);
     fine.ghosts_ok[fine.cur] = false;
     const sledgehamr::Sledgehamr &S = *sim_;

@@ -108,6 +108,9 @@ struct LevelState {
     std::vector<CfPSeg> cf_segs;
     DevMem dcf_segs, dcf_start, dcf_cand;
     long long cf_total = 0;
+    // Workgroup tiles of the segments and their largest coarse brick.
+    DevMem dcf_tiles, dcf_full_tiles;
+    int cf_ntile = 0, cf_full_ntile = 0, cf_lds = 0, cf_full_lds = 0;
     // Every cell of each ghosted box (FillCoarsePatch).
     std::vector<CfPSeg> cf_full;
     DevMem dcf_full;
@@ -307,8 +310,10 @@ class Runtime {
     void fill_output_ghosts(int lev);
     void check_output_fn(int fn) const;
     /** FillPatch of state `which` of level lev at `time`: coarse/fine
-     *  ghosts (levels above 0), then the same-level copy plan. */
-    void fill_patch(LevelState &ls, int which, int lev, double time);
+     *  ghosts (levels above 0), then the same-level copy plan.  for_step:
+     *  only what a level step reads on levels above 0 (step_ghost_comps,
+     *  leading words). */
+    void fill_patch(LevelState &ls, int which, int lev, double time, bool for_step = false);
     /** Level lev has ghost cells filled from level lev-1. */
     bool has_cf(int lev) const { return lev >= 1 && has_level(lev - 1); }
     void build_cf(int lev);
@@ -323,8 +328,14 @@ class Runtime {
     const double *push_record(const std::vector<double> &blk, int kind, double time);
     std::vector<double> step_times_;  // stage times and t0 + dt of the step
     void cf_launch(CfArgs &a, int mode);
-    /** kCfState fill of every component of a state of level lev. */
-    void cf_fill_state(int lev, double time, void *hi, void *lo);
+    /** kCfState fill of the components `comps` (empty: all) of a state of
+     *  level lev; lo null: the leading words only. */
+    void cf_fill_state(int lev, double time, void *hi, void *lo,
+                       const std::vector<int> &comps = {});
+    /** Components whose coarse/fine ghost cells a level step reads: the
+     *  stencil components; every component with dissipation, or for RKN,
+     *  whose stage states at ghost cells compose phi0 + c pi0 + ... */
+    std::vector<int> step_ghost_comps() const;
     /** Every cell of state `which` of level lev, ghosts included, from level
      *  lev-1 at `time` (first_state_only: FillCoarsePatch's choice). */
     void fill_coarse_full(int lev, int which, double time, bool first_state_only);
