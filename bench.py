@@ -240,9 +240,11 @@ def ensemble(shm, seed, steps=200, warmup=10):
         for c, f in enumerate(fields):
             sim.upload(0, c, f)
         sim.autotune(0)
+        progress("ensemble: autotuned")
         sim.set_graphs(True)
         sim.advance(0, warmup)
         sim.sync()
+        progress("ensemble: warm-up steps")
         lat = []
         n = steps if prec == shm.PREC_FP32C16 else 0
         for _ in range(n):
@@ -250,6 +252,7 @@ def ensemble(shm, seed, steps=200, warmup=10):
             sim.advance(0, 1)
             sim.sync()
             lat.append(time.perf_counter() - t0)
+        progress("ensemble: timed steps")
         if n:
             lat = np.array(lat) * 1e3
             cells = sim.ncells(0)
@@ -276,6 +279,86 @@ def ensemble(shm, seed, steps=200, warmup=10):
     return out
 
 
+def amr_batched(shm, seed, steps=6):
+    """Batched AMR levels: complex scalar on a 256^3 base with two levels
+    of 2000 boxes each (edges 16..32 on level 1, 16..52 on level 2, seeded),
+    3 ghost cells, 40 seeded string loops, SSPRK3, truncation-error tagging
+    at te_crit 1e-3 on its schedule, autotuned launches, sub-cycled coarse
+    steps replayed from captured graphs.  Rate = valid cells x stages x level
+    steps per coarse step / wall time; the FP32-compensated fields are
+    compared with an FP64 run of the same steps."""
+    from cases import amr_batched_boxes
+    lev1, lev2 = amr_batched_boxes(seed % 100000)
+    inp = {"project.name": "ComplexScalar", "project.lambda": 1.0, "sim.t_start": 0.1,
+           "sim.t_end": 2.0, "sim.L": 64.0, "sim.cfl": 0.2,
+           "amr.coarse_level_grid_size": 256, "amr.nghost": 3, "integrator.type": 10,
+           "amr.max_refinement_levels": 2, "amr.te_crit": 1e-3, "amr.regrid_dt": 0.2}
+    out, fields = {}, {}
+    for prec in (shm.PREC_FP32C16, shm.PREC_FP64):
+        sim = shm.Simulation("ComplexScalar", inp, precision=prec)
+        sim.set_boxes(1, lev1)
+        sim.set_boxes(2, lev2)
+        for lev in range(3):
+            sim.init_state(lev, 5, seed, [40, 8.0, 40.0, 1.5])
+        sim.sync()
+        progress("batched AMR: layout and initial state")
+        for lev in range(3):
+            sim.autotune(lev)
+        progress("batched AMR: autotuned")
+        sim.set_graphs(True)
+        sim.evolve(2)
+        sim.sync()
+        progress("batched AMR: warm-up steps")
+        cells = [sim.ncells(lev) for lev in range(3)]
+        work = 3 * (cells[0] + 2 * cells[1] + 4 * cells[2])
+        t0 = time.perf_counter()
+        sim.evolve(steps)
+        sim.sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        sim.set_graphs(False)
+        sim.set_timing(True)
+        sim.evolve(2)
+        sim.sync()
+        tm = sim.timing()
+        sim.set_timing(False)
+        rate = work / (ms * 1e-3)
+        key = "fp32c16" if prec == shm.PREC_FP32C16 else "fp64"
+        rec = {"ms_per_coarse_step": ms, "value": rate, "unit": "cell-updates/s",
+               "step_frac": BYTES_PER_CELL_STAGE[key] * rate / 1e9 / HBM_PEAK_GBS,
+               "stage_kernel_ms_per_coarse_step": tm["stage_ms"] / 2,
+               "ghost_fill_ms_per_coarse_step": tm["halo_ms"] / 2,
+               "launch": [sim.get_launch(lev) for lev in range(3)]}
+        if prec == shm.PREC_FP32C16:
+            out = {"workload": "complex scalar, 256^3 base + 2 levels (ratio 2, "
+                               "sub-cycled), 2000 boxes per fine level (seeded "
+                               "edges 16..32 / 16..52), 3 ghost cells, 40 string "
+                               "loops, SSPRK3, TE tagging at 1e-3, graphs",
+                   "boxes": [1, len(lev1), len(lev2)], "cells": cells,
+                   "cell_stages_per_coarse_step": work, "steps": steps, **rec}
+        else:
+            out["fp64"] = rec
+            out["speedup_fp32c_vs_fp64"] = out["value"] / rate
+        fields[key] = [[sim.download(lev, c) for c in range(4)] for lev in range(3)]
+        sim.close()
+        progress(f"batched AMR {key}: {ms:.2f} ms per coarse step")
+    err = 0.0
+    for lev in range(3):
+        for c in range(4):
+            a32, a64 = fields["fp32c16"][lev][c], fields["fp64"][lev][c]
+            err = max(err, float(np.max(np.abs(a32 - a64)) /
+                                 max(np.max(np.abs(a64)), 1e-300)))
+    out["rel_err_vs_fp64"] = err
+    return out
+
+
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line per finished phase on stderr (stdout carries the JSON)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {what}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +369,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--launch", default="",
                     help="fixed stage launch shape 'variant,tile_z,xcd_split,"
                          "persistent,zc_minor[,fused,fused_variant]' instead "
@@ -309,6 +393,7 @@ def main():
                                  a.seed, launch)
     cells = sim32.ncells(0)
     rate32 = cells * stages * a.steps / sec32
+    progress(f"fp32c16 {N}^3: {sec32 / a.steps * 1e3:.3f} ms/step")
 
     result = {
         "metric": METRIC,
@@ -353,6 +438,7 @@ def main():
             err = max(err, float(np.max(np.abs(x32 - x64)) /
                                  max(np.max(np.abs(x64)), 1e-300)))
             del x32, x64
+        progress(f"fp64 {N}^3: {sec64 / a.steps * 1e3:.3f} ms/step")
         fused64 = bool(sim64.get_launch(0)["fused"])
         r64 = kernel_roofline(tm64, "fp64", fused64, 0)
         result["fp64"] = {"value": rate64, "ms_per_step": sec64 / a.steps * 1e3,
@@ -373,6 +459,7 @@ def main():
             errc = max(errc, float(np.max(np.abs(xc - x64)) /
                                    max(np.max(np.abs(x64)), 1e-300)))
             del xc, x64
+        progress(f"fp32c {N}^3: {secc / a.steps * 1e3:.3f} ms/step")
         result["fp32c"] = {"value": ratec,
                            "step_frac": BYTES_PER_CELL_STAGE["fp32c"] * ratec
                            / 1e9 / HBM_PEAK_GBS,
@@ -385,12 +472,18 @@ def main():
                         "note": "same workload, level steps replayed from "
                                 "captured HIP graphs, no kernel timing"}
     sim32.close()
+    progress("graph replays")
 
     if not a.no_ensemble:
         result["ensemble"] = ensemble(shm, a.seed)
+        progress("ensemble")
+    if not a.no_amr:
+        result["amr"] = amr_batched(shm, a.seed)
+        progress("batched AMR levels")
 
     if not a.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(inp, a.seed)
+        progress("cpu baseline")
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result))

@@ -1024,4 +1024,36 @@ __global__ void __launch_bounds__(256) regrid_select_kernel(const RegridArgs a) 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Uploads and downloads: valid cells <-> one dense FP64 array in one pass.
+// ---------------------------------------------------------------------------
+
+/** TO_STATE: dense -> state (leading word and remainder as the stage kernels
+ *  store them); else state -> dense (leading word plus remainder). */
+template <Prec P, bool TO_STATE>
+__global__ void __launch_bounds__(256) valid_io_kernel(const ValidIoArgs a) {
+    using T = PrecTraits<P>;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < a.total;
+         t += stride) {
+        int lo = 0, hi = a.nbox - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.vprefix[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BoxDesc &b = a.boxes[lo];
+        const long long r = t - a.vprefix[lo];
+        const long long row = r / b.n[0];
+        const long long e = a.comp_off + b.base + (r - row * b.n[0]) +
+                            (row % b.n[1]) * b.jstride + (row / b.n[1]) * b.kstride;
+        if constexpr (TO_STATE)
+            store_state_at<T>(a.hi, a.lo, e, a.dense[t]);
+        else
+            a.dense[t] = load_state_at<T>(a.hi, a.lo, e);
+    }
+}
+
 }  // namespace shm

@@ -150,4 +150,16 @@ struct RegridArgs {
     int bf, Np, thr;        // blocking factor, blocks per axis, (n_error_buf+1)^2
 };
 
+/** Valid cells of one component of a level and a dense FP64 array of them
+ *  (box-major, x fastest). */
+struct ValidIoArgs {
+    const BoxDesc *boxes;
+    const long long *vprefix;  // nbox + 1 prefix sums of valid cells
+    int nbox;
+    long long total;
+    void *hi, *lo;             // state arrays (lo null without compensation)
+    long long comp_off;        // component * plane stride
+    double *dense;
+};
+
 }  // namespace shm

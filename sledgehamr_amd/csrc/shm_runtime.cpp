@@ -1755,16 +1755,6 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
 // Data movement and initial states.
 // ---------------------------------------------------------------------------
 
-/** Host-side split of an FP64 value into the stored hi and remainder. */
-static float host_bf16_round(float r) {
-    uint32_t b;
-    std::memcpy(&b, &r, 4);
-    b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
-    float o;
-    std::memcpy(&o, &b, 4);
-    return o;
-}
-
 void Runtime::upload_array(int lev, int comp_id, const double *host, int layout) {
     if (layout == 0) return upload(lev, comp_id, host);
     if (layout != 1 && layout != 2) throw Error(1, "array layout must be 0, 1 or 2");
@@ -1790,102 +1780,71 @@ void Runtime::upload_array(int lev, int comp_id, const double *host, int layout)
     upload(lev, comp_id, v.data());
 }
 
+void Runtime::valid_io(LevelState &ls, int comp_id, int which, double *host, bool to_state) {
+    if (comp_id < 0 || comp_id >= ncomp_) throw Error(1, "component out of range");
+    const long long n = ls.L.nvalid();
+    if (n == 0) return;
+    DevMem dense;
+    dense.alloc(sizeof(double) * (size_t)n);
+    ValidIoArgs a{};
+    a.boxes = ls.L.dbox.as<BoxDesc>();
+    a.vprefix = ls.L.dvprefix.as<long long>();
+    a.nbox = (int)ls.L.boxes.size();
+    a.total = n;
+    a.hi = ls.U[which].get();
+    a.lo = comp() ? ls.Ulo[which].get() : nullptr;
+    a.comp_off = (long long)comp_id * ls.L.cstride;
+    a.dense = dense.as<double>();
+    const int grid = (int)std::min<long long>((n + 255) / 256, 8LL * ncu_);
+    if (to_state)
+        SHM_HIP(hipMemcpy(dense.get(), host, sizeof(double) * (size_t)n,
+                          hipMemcpyHostToDevice));
+    switch (prec_) {
+        case Prec::F64:
+            if (to_state)
+                hipLaunchKernelGGL((valid_io_kernel<Prec::F64, true>), dim3(grid), dim3(256), 0,
+                                   stream_, a);
+            else
+                hipLaunchKernelGGL((valid_io_kernel<Prec::F64, false>), dim3(grid), dim3(256),
+                                   0, stream_, a);
+            break;
+        case Prec::F32C:
+            if (to_state)
+                hipLaunchKernelGGL((valid_io_kernel<Prec::F32C, true>), dim3(grid), dim3(256),
+                                   0, stream_, a);
+            else
+                hipLaunchKernelGGL((valid_io_kernel<Prec::F32C, false>), dim3(grid),
+                                   dim3(256), 0, stream_, a);
+            break;
+        case Prec::F32C16:
+            if (to_state)
+                hipLaunchKernelGGL((valid_io_kernel<Prec::F32C16, true>), dim3(grid),
+                                   dim3(256), 0, stream_, a);
+            else
+                hipLaunchKernelGGL((valid_io_kernel<Prec::F32C16, false>), dim3(grid),
+                                   dim3(256), 0, stream_, a);
+            break;
+    }
+    SHM_HIP(hipGetLastError());
+    if (!to_state)
+        SHM_HIP(hipMemcpyAsync(host, dense.get(), sizeof(double) * (size_t)n,
+                               hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+}
+
 void Runtime::upload(int lev, int comp_id, const double *host) {
     if (lev == -1) define_shadow();
     LevelState &ls = level(lev);
-    if (comp_id < 0 || comp_id >= ncomp_) throw Error(1, "component out of range");
-    const int w = ls.cur;
-    const size_t es = esize(), ls_ = losize();
-    for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
-        const Box3 &bx = ls.L.boxes[b];
-        const BoxDesc &bd = ls.L.hbox[b];
-        const long long nb = bx.vol();
-        const double *src = host + ls.L.vprefix[b];
-        std::vector<char> hi((size_t)nb * es), lo;
-        if (comp()) lo.resize((size_t)nb * ls_);
-        for (long long i = 0; i < nb; ++i) {
-            if (prec_ == Prec::F64) {
-                reinterpret_cast<double *>(hi.data())[i] = src[i];
-            } else {
-                const float h = (float)src[i];
-                reinterpret_cast<float *>(hi.data())[i] = h;
-                const float r = (float)(src[i] - (double)h);
-                if (prec_ == Prec::F32C) {
-                    reinterpret_cast<float *>(lo.data())[i] = r;
-                } else {
-                    const float q = host_bf16_round(r);
-                    uint32_t u;
-                    std::memcpy(&u, &q, 4);
-                    reinterpret_cast<uint16_t *>(lo.data())[i] = (uint16_t)(u >> 16);
-                }
-            }
-        }
-        const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
-        // Rows of consecutive planes are not equally spaced (ghost rows in
-        // between), so copy plane by plane.
-        for (int z = 0; z < nz; ++z) {
-            const size_t e0 = (size_t)comp_id * ls.L.cstride + bd.base +
-                              (size_t)z * bd.kstride;
-            SHM_HIP(hipMemcpy2D(static_cast<char *>(ls.U[w].get()) + e0 * es,
-                                (size_t)bd.jstride * es,
-                                hi.data() + (size_t)z * nx * ny * es, (size_t)nx * es,
-                                (size_t)nx * es, ny, hipMemcpyHostToDevice));
-            if (comp())
-                SHM_HIP(hipMemcpy2D(static_cast<char *>(ls.Ulo[w].get()) + e0 * ls_,
-                                    (size_t)bd.jstride * ls_,
-                                    lo.data() + (size_t)z * nx * ny * ls_,
-                                    (size_t)nx * ls_, (size_t)nx * ls_, ny,
-                                    hipMemcpyHostToDevice));
-        }
-    }
-    ls.ghosts_ok[w] = false;
+    // The stream may still read the state (steps queued before the upload).
+    sync();
+    valid_io(ls, comp_id, ls.cur, const_cast<double *>(host), true);
+    ls.ghosts_ok[ls.cur] = false;
 }
 
 void Runtime::download(int lev, int comp_id, double *host, bool old_state) {
     LevelState &ls = level(lev);
-    if (comp_id < 0 || comp_id >= ncomp_) throw Error(1, "component out of range");
     sync();
-    const int w = old_state ? (ls.cur ^ 1) : ls.cur;
-    const size_t es = esize(), ls_ = losize();
-    for (size_t b = 0; b < ls.L.boxes.size(); ++b) {
-        const Box3 &bx = ls.L.boxes[b];
-        const BoxDesc &bd = ls.L.hbox[b];
-        const int nx = bx.n(0), ny = bx.n(1), nz = bx.n(2);
-        std::vector<char> hi((size_t)bx.vol() * es), lo;
-        if (comp()) lo.resize((size_t)bx.vol() * ls_);
-        for (int z = 0; z < nz; ++z) {
-            const size_t e0 = (size_t)comp_id * ls.L.cstride + bd.base +
-                              (size_t)z * bd.kstride;
-            SHM_HIP(hipMemcpy2D(hi.data() + (size_t)z * nx * ny * es,
-                                (size_t)nx * es,
-                                static_cast<const char *>(ls.U[w].get()) + e0 * es,
-                                (size_t)bd.jstride * es, (size_t)nx * es, ny,
-                                hipMemcpyDeviceToHost));
-            if (comp())
-                SHM_HIP(hipMemcpy2D(lo.data() + (size_t)z * nx * ny * ls_,
-                                    (size_t)nx * ls_,
-                                    static_cast<const char *>(ls.Ulo[w].get()) + e0 * ls_,
-                                    (size_t)bd.jstride * ls_, (size_t)nx * ls_, ny,
-                                    hipMemcpyDeviceToHost));
-        }
-        double *dst = host + ls.L.vprefix[b];
-        for (long long i = 0; i < bx.vol(); ++i) {
-            if (prec_ == Prec::F64) {
-                dst[i] = reinterpret_cast<const double *>(hi.data())[i];
-            } else {
-                double v = (double)reinterpret_cast<const float *>(hi.data())[i];
-                if (prec_ == Prec::F32C) {
-                    v += (double)reinterpret_cast<const float *>(lo.data())[i];
-                } else {
-                    const uint32_t u = (uint32_t)reinterpret_cast<const uint16_t *>(lo.data())[i] << 16;
-                    float f;
-                    std::memcpy(&f, &u, 4);
-                    v += (double)f;
-                }
-                dst[i] = v;
-            }
-        }
-    }
+    valid_io(ls, comp_id, old_state ? (ls.cur ^ 1) : ls.cur, host, false);
 }
 
 void Runtime::download_rhs(int lev, int comp, double *host) {

@@ -347,6 +347,9 @@ class Runtime {
     std::vector<uint8_t> block_cover(int lev) const;
     void did_global_regrid(int lev);
     void check_bf(int lev) const;
+    /** Valid cells of component comp_id of state `which`: from the host
+     *  array (to_state) or into it, in one kernel and one copy. */
+    void valid_io(LevelState &ls, int comp_id, int which, double *host, bool to_state);
     /** The coarsened copy of level lev (>= 1) on level lev-1, built once
      *  per epoch with its copy plans. */
     LevelState &coarse_mirror(int lev);
