@@ -919,7 +919,7 @@ void Runtime::fill_patch(LevelState &ls, int which, int lev, double time, bool f
             // A step reads the leading words of step_ghost_comps() at ghost
             // cells; the remainders and other components stay as they are.
             const std::vector<int> comps = step_ghost_comps();
-            cf_fill_state(lev, time, ls.U[which].get(), nullptr, comps);
+            cf_fill_state(lev, time, ls.U[which].get(), nullptr, comps, true);
             halo(ls, ls.U[which].get(), comps);
             ls.ghosts_ok[which] = false;
             return;
@@ -957,6 +957,9 @@ void Runtime::build_cf(int lev) {
     ls.cf_total = 0;
     ls.cf_full.clear();
     ls.cf_full_total = 0;
+    ls.cf_step.clear();
+    ls.cf_step_total = 0;
+    const int gs = std::min(g, R_);  // ghost layers a stencil reads
     std::vector<int> start(1, 0), cand;
     // The parent cells of fine cells [flo, flo + fn) of box b.
     auto parents = [](std::vector<CfPSeg> &v, long long &tot, int b, const int *flo,
@@ -987,6 +990,15 @@ void Runtime::build_cf(int lev) {
             {bx.hi[0] + 1, bx.lo[1], bx.lo[2], g, ny, nz}};
         for (int q = 0; g > 0 && q < 6; ++q)
             parents(ls.cf_segs, ls.cf_total, b, slab[q], slab[q] + 3);
+        const int sslab[6][6] = {
+            {bx.lo[0] - gs, bx.lo[1] - gs, bx.lo[2] - gs, nx + 2 * gs, ny + 2 * gs, gs},
+            {bx.lo[0] - gs, bx.lo[1] - gs, bx.hi[2] + 1, nx + 2 * gs, ny + 2 * gs, gs},
+            {bx.lo[0] - gs, bx.lo[1] - gs, bx.lo[2], nx + 2 * gs, gs, nz},
+            {bx.lo[0] - gs, bx.hi[1] + 1, bx.lo[2], nx + 2 * gs, gs, nz},
+            {bx.lo[0] - gs, bx.lo[1], bx.lo[2], gs, ny, nz},
+            {bx.hi[0] + 1, bx.lo[1], bx.lo[2], gs, ny, nz}};
+        for (int q = 0; gs > 0 && q < 6; ++q)
+            parents(ls.cf_step, ls.cf_step_total, b, sslab[q], sslab[q] + 3);
         {
             const int flo[3] = {bx.lo[0] - g, bx.lo[1] - g, bx.lo[2] - g};
             const int fn[3] = {nx + 2 * g, ny + 2 * g, nz + 2 * g};
@@ -1030,6 +1042,7 @@ void Runtime::build_cf(int lev) {
     }
     upload_vec(ls.dcf_segs, ls.cf_segs);
     upload_vec(ls.dcf_full, ls.cf_full);
+    upload_vec(ls.dcf_step, ls.cf_step);
     // Workgroup tiles: at most 32 x 8 x 4 parents (16 x 8 x 4 for the
     // quartic's reach 2), so the coarse brick of a tile, its parents grown
     // by the reach, takes at most 16 KB of LDS as doubles and several
@@ -1060,6 +1073,7 @@ void Runtime::build_cf(int lev) {
     };
     tile(ls.cf_segs, ls.dcf_tiles, ls.cf_ntile, ls.cf_lds);
     tile(ls.cf_full, ls.dcf_full_tiles, ls.cf_full_ntile, ls.cf_full_lds);
+    tile(ls.cf_step, ls.dcf_step_tiles, ls.cf_step_ntile, ls.cf_step_lds);
     upload_vec(ls.dcf_start, start);
     if (cand.empty()) cand.push_back(0);
     upload_vec(ls.dcf_cand, cand);
@@ -1113,18 +1127,18 @@ const double *Runtime::push_record(const std::vector<double> &blk, int kind, dou
     return push_params(prm_host_, prm_ev_, prm_slot_, prm_dev_.as<double>(), stream_, blk);
 }
 
-CfArgs Runtime::cf_args(int lev, double time, bool first_state_only) {
+CfArgs Runtime::cf_args(int lev, double time, bool first_state_only, bool step) {
     LevelState &ls = levels_[lev];
     if (!ls.cf_built) build_cf(lev);
     LevelState &cs = levels_[lev - 1];
     CfArgs a{};
     a.fboxes = ls.L.dbox.as<BoxDesc>();
     a.cboxes = cs.L.dbox.as<BoxDesc>();
-    a.segs = ls.dcf_segs.as<CfPSeg>();
-    a.tiles = ls.dcf_tiles.as<CfTile>();
-    a.ntile = ls.cf_ntile;
-    a.lds_cells = ls.cf_lds;
-    a.total = ls.cf_total;
+    a.segs = (step ? ls.dcf_step : ls.dcf_segs).as<CfPSeg>();
+    a.tiles = (step ? ls.dcf_step_tiles : ls.dcf_tiles).as<CfTile>();
+    a.ntile = step ? ls.cf_step_ntile : ls.cf_ntile;
+    a.lds_cells = step ? ls.cf_step_lds : ls.cf_lds;
+    a.total = step ? ls.cf_step_total : ls.cf_total;
     a.cand_start = ls.dcf_start.as<int>();
     a.cand = ls.dcf_cand.as<int>();
     a.Nc = cs.L.N;
@@ -1213,9 +1227,9 @@ void Runtime::fill_coarse_full(int lev, int which, double time, bool first_state
 }
 
 void Runtime::cf_fill_state(int lev, double time, void *hi, void *lo,
-                            const std::vector<int> &comps) {
+                            const std::vector<int> &comps, bool step) {
     const std::vector<int> cs = comps.empty() ? all_comps() : comps;
-    CfArgs a = cf_args(lev, time);
+    CfArgs a = cf_args(lev, time, false, step);
     a.dst = hi;
     a.dstlo = lo;
     for (size_t c0 = 0; c0 < cs.size(); c0 += 32) {
@@ -1282,7 +1296,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                            "cells is not supported (stage momenta at ghost cells)");
     }
     auto cf_fill_k = [&](void *k, double time, double c) {
-        CfArgs f = cf_args(lev, time);
+        CfArgs f = cf_args(lev, time, false, true);
         f.dst = k;
         f.u0 = ls.U[old].get();
         f.c0 = c;
@@ -1352,7 +1366,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                 launch_stage(kRkn0 + i, ls, a, lev);
                 if (cf) {
                     // Stage i+1 reads phi0 + dt c_{i+1} pi0 + sum_j dt^2 a_{i+1,j} F_j.
-                    CfArgs f = cf_args(lev, st[i + 1]);
+                    CfArgs f = cf_args(lev, st[i + 1], false, true);
                     f.dst = ls.F[i].get();
                     f.u0 = ls.U[old].get();
                     f.c0 = dt * nodes_[i + 1];
@@ -1396,7 +1410,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         a.c[1] = dt;
         launch_stage(kLeap1, ls, a, lev);
         // The second kick reads the fields' leading words at ghost cells.
-        if (cf) cf_fill_state(lev, st[1], ls.U[nw].get(), nullptr, stencil_comps());
+        if (cf) cf_fill_state(lev, st[1], ls.U[nw].get(), nullptr, stencil_comps(), true);
         halo(ls, ls.U[nw].get(), stencil_comps());
 
         StageArgs b{};

@@ -111,6 +111,12 @@ struct LevelState {
     // Workgroup tiles of the segments and their largest coarse brick.
     DevMem dcf_tiles, dcf_full_tiles;
     int cf_ntile = 0, cf_full_ntile = 0, cf_lds = 0, cf_full_lds = 0;
+    // The ghost shell a level step reads: the stencil radius instead of all
+    // ghost layers.
+    std::vector<CfPSeg> cf_step;
+    DevMem dcf_step, dcf_step_tiles;
+    long long cf_step_total = 0;
+    int cf_step_ntile = 0, cf_step_lds = 0;
     // Every cell of each ghosted box (FillCoarsePatch).
     std::vector<CfPSeg> cf_full;
     DevMem dcf_full;
@@ -319,7 +325,7 @@ class Runtime {
     void build_cf(int lev);
     /** Arguments of a coarse/fine fill of level lev at `time` (layout,
      *  coarse states and their time weights); the caller adds the target. */
-    CfArgs cf_args(int lev, double time, bool first_state_only = false);
+    CfArgs cf_args(int lev, double time, bool first_state_only = false, bool step = false);
     /** Coarse/fine record of level lev at `time` (see CfArgs::rec). */
     void cf_record(double *rec, int lev, double time, bool first_state_only) const;
     /** A parameter record for the device: the next slot of the ring, or,
@@ -331,7 +337,7 @@ class Runtime {
     /** kCfState fill of the components `comps` (empty: all) of a state of
      *  level lev; lo null: the leading words only. */
     void cf_fill_state(int lev, double time, void *hi, void *lo,
-                       const std::vector<int> &comps = {});
+                       const std::vector<int> &comps = {}, bool step = false);
     /** Components whose coarse/fine ghost cells a level step reads: the
      *  stencil components; every component with dissipation, or for RKN,
      *  whose stage states at ghost cells compose phi0 + c pi0 + ... */
