@@ -279,7 +279,7 @@ def ensemble(shm, seed, steps=200, warmup=10):
     return out
 
 
-def amr_batched(shm, seed, steps=6):
+def amr_batched(shm, seed, steps=6, autotune=True):
     """Batched AMR levels: complex scalar on a 256^3 base with two levels
     of 2000 boxes each (edges 16..32 on level 1, 16..52 on level 2, seeded),
     3 ghost cells, 40 seeded string loops, SSPRK3, truncation-error tagging
@@ -302,9 +302,10 @@ def amr_batched(shm, seed, steps=6):
             sim.init_state(lev, 5, seed, [40, 8.0, 40.0, 1.5])
         sim.sync()
         progress("batched AMR: layout and initial state")
-        for lev in range(3):
-            sim.autotune(lev)
-        progress("batched AMR: autotuned")
+        if autotune:
+            for lev in range(3):
+                sim.autotune(lev)
+        progress("batched AMR: autotuned" if autotune else "batched AMR: default launches")
         sim.set_graphs(True)
         sim.evolve(2)
         sim.sync()
@@ -370,6 +371,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--only-amr", action="store_true",
+                    help="run only the batched AMR block (for profiling) and print it")
+    ap.add_argument("--amr-no-autotune", action="store_true",
+                    help="default launch shapes in the batched AMR block (a profile "
+                         "then holds no autotuner launches)")
     ap.add_argument("--launch", default="",
                     help="fixed stage launch shape 'variant,tile_z,xcd_split,"
                          "persistent,zc_minor[,fused,fused_variant]' instead "
@@ -385,6 +391,11 @@ def main():
         sys.exit(2)
 
     import sledgehamr_amd as shm
+
+    if a.only_amr:
+        print(json.dumps({"amr": amr_batched(shm, a.seed,
+                                             autotune=not a.amr_no_autotune)}))
+        return
 
     N = a.n
     inp = inputs(N, L=N * 0.25, cfl=0.2, t0=0.1)
@@ -478,7 +489,7 @@ def main():
         result["ensemble"] = ensemble(shm, a.seed)
         progress("ensemble")
     if not a.no_amr:
-        result["amr"] = amr_batched(shm, a.seed)
+        result["amr"] = amr_batched(shm, a.seed, autotune=not a.amr_no_autotune)
         progress("batched AMR levels")
 
     if not a.no_cpu_baseline:
