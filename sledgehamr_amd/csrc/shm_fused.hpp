@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             e1x[m] = tx;
             e1y[m] = w;
         } else if (e1ok[m]) {
-            ring_xy<R, NW>(tid + NT * (m - 1), e1x[m], e1y[m]);
+            ring_xy<R, NW, 64>(tid + NT * (m - 1), e1x[m], e1y[m]);
         }
     }
 

@@ -94,7 +94,7 @@ template <class Ops, int K, Prec P, int R, bool GW, bool KO, int V>
 hipError_t launch_stage(const StageArgs &a, int grid, hipStream_t s) {
     constexpr TileVariant v = kVariants[V];
     hipLaunchKernelGGL((stage_kernel<Ops, typename KindPol<K>::T, P, R, GW, KO,
-                                     v.nw, v.ry>),
+                                     v.nw, v.ry, v.tx>),
                        dim3(grid), dim3(64 * v.nw), 0, s, a);
     return hipGetLastError();
 }
@@ -105,7 +105,7 @@ int occupancy_stage() {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb,
-            stage_kernel<Ops, typename KindPol<K>::T, P, R, GW, KO, v.nw, v.ry>,
+            stage_kernel<Ops, typename KindPol<K>::T, P, R, GW, KO, v.nw, v.ry, v.tx>,
             64 * v.nw, 0) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
@@ -116,9 +116,10 @@ void fill_variants(KernelTable &kt) {
     auto put = [&](auto vc) {
         constexpr int V = decltype(vc)::value;
         using S = typename PrecTraits<P>::S;
-        constexpr int TY = kVariants[V].nw * kVariants[V].ry;
+        constexpr int TX = kVariants[V].tx;
+        constexpr int TY = kVariants[V].nw * 64 / TX * kVariants[V].ry;
         constexpr long lds = 2L * CompSet<Ops, GW, KO>::NS * (TY + 2 * R) *
-                             (64 + 2 * R) * (long)sizeof(S);
+                             (TX + 2 * R) * (long)sizeof(S);
         // Keep at least two workgroups per CU (160 KB LDS); dissipation
         // variants only in the default tile shape.
         if constexpr (lds <= 80 * 1024 && (!KO || V == kDefaultVariant)) {
@@ -126,12 +127,14 @@ void fill_variants(KernelTable &kt) {
             e.fn = &launch_stage<Ops, K, P, R, GW, KO, V>;
             e.occupancy = &occupancy_stage<Ops, K, P, R, GW, KO, V>;
             e.threads = 64 * kVariants[V].nw;
-            e.ty = kVariants[V].nw * kVariants[V].ry;
+            e.ty = TY;
+            e.tx = TX;
         }
     };
     put(std::integral_constant<int, 0>{});
     put(std::integral_constant<int, 1>{});
     put(std::integral_constant<int, 2>{});
+    put(std::integral_constant<int, 3>{});
 }
 
 template <class Ops, Prec P, int R, bool GW, bool KO>
@@ -181,7 +184,7 @@ template <class Ops, Prec P, int R, bool GW>
 void fill_fused(KernelTable &kt) {
     auto put = [&](auto vc) {
         constexpr int V = decltype(vc)::value;
-        if constexpr (kVariants[V].ry == 1 &&
+        if constexpr (kVariants[V].ry == 1 && kVariants[V].tx == 64 &&
                       FusedShape<Ops, P, R, GW, false, kVariants[V].nw>::fits) {
             FusedEntry &e = kt.fused12[(int)P][R][GW ? 1 : 0][V];
             e.fn = &launch_fused<Ops, P, R, GW, V>;
@@ -193,6 +196,7 @@ void fill_fused(KernelTable &kt) {
     put(std::integral_constant<int, 0>{});
     put(std::integral_constant<int, 1>{});
     put(std::integral_constant<int, 2>{});
+    put(std::integral_constant<int, 3>{});
 }
 
 template <class Ops, Prec P, int R>

@@ -250,19 +250,19 @@ void Layout::build(const std::vector<Box3> &b, int ghost, int domain_n,
     tiles.clear();
 }
 
-const TileDesc *Layout::get_tiles(int ty, int tz, int zc_minor, int *ntiles) {
-    auto key = std::make_tuple(ty, tz, zc_minor);
+const TileDesc *Layout::get_tiles(int ty, int tz, int zc_minor, int *ntiles, int tx) {
+    auto key = std::make_tuple(ty, tz, zc_minor, tx);
     auto it = tiles.find(key);
     if (it == tiles.end()) {
         std::vector<TileDesc> tv;
         for (int b = 0; b < (int)boxes.size(); ++b) {
             const Box3 &bx = boxes[b];
             const int nzc = (bx.n(2) + tz - 1) / tz;
-            const int ny = (bx.n(1) + ty - 1) / ty, nx = (bx.n(0) + 63) / 64;
+            const int ny = (bx.n(1) + ty - 1) / ty, nx = (bx.n(0) + tx - 1) / tx;
             auto push = [&](int x, int y, int zc) {
                 TileDesc t;
                 t.box = b;
-                t.x0 = x * 64;
+                t.x0 = x * tx;
                 t.y0 = y * ty;
                 t.z0 = zc * tz;
                 t.z1 = std::min(t.z0 + tz, bx.n(2));
@@ -851,13 +851,13 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
         long long xy = 0;
         for (const Box3 &b : ls.L.boxes) {
             maxn = std::max(maxn, b.n(2));
-            xy += (long long)((b.n(0) + 63) / 64) * ((b.n(1) + e.ty - 1) / e.ty);
+            xy += (long long)((b.n(0) + e.tx - 1) / e.tx) * ((b.n(1) + e.ty - 1) / e.ty);
         }
         z = maxn;
         while (z > 16 && xy * ((maxn + z - 1) / z) < 8LL * ncu_) z = (z + 1) / 2;
     }
     int ntiles = 0;
-    a.tiles = ls.L.get_tiles(e.ty, z, lc.zc_minor, &ntiles);
+    a.tiles = ls.L.get_tiles(e.ty, z, lc.zc_minor, &ntiles, e.tx);
     a.boxes = ls.L.dbox.as<BoxDesc>();
     a.ntiles = ntiles;
     a.cstride = ls.L.cstride;

@@ -80,11 +80,11 @@ struct Layout {
     std::vector<HaloSeg> halo_full;  // every ghost cell
     DevMem dhalo_full;
     long long halo_full_total = 0;
-    std::map<std::tuple<int, int, int>, std::pair<std::shared_ptr<DevMem>, int>> tiles;
+    std::map<std::tuple<int, int, int, int>, std::pair<std::shared_ptr<DevMem>, int>> tiles;
 
     void build(const std::vector<Box3> &b, int ghost, int domain_n,
                bool is_periodic, bool is_independent = false);
-    const TileDesc *get_tiles(int ty, int tz, int zc_minor, int *ntiles);
+    const TileDesc *get_tiles(int ty, int tz, int zc_minor, int *ntiles, int tx = 64);
     long long nvalid() const { return vprefix.empty() ? 0 : vprefix.back(); }
 };
 

@@ -4,7 +4,8 @@
 // stage update in registers, so each stored array is read once and written at
 // most once per stage.  Work is a persistent loop over a device tile table
 // that spans every box of a level: a tile is 64 cells in x (one wavefront per
-// row, fully coalesced) by TY = NW*RY rows in y, marched along z.
+// row, fully coalesced; 32 and two rows per wavefront for narrow boxes) by
+// TY rows in y, marched along z.
 //
 // The z-march keeps, per thread and stencil component, a register queue of
 // the 2R+1 values of its column; the centre plane with its x/y ghost ring is
@@ -523,12 +524,12 @@ struct PolLeapfrog {
 // The fused stage kernel.
 // ---------------------------------------------------------------------------
 
-/** Halo-ring cell h of a (64 + 2R) x (TY + 2R) plane: rows above, rows
+/** Halo-ring cell h of a (TX + 2R) x (TY + 2R) plane: rows above, rows
  *  below, then the R columns left and right of each interior row.  Returns
  *  tile-local (x, y), both >= -R. */
-template <int R, int TY>
+template <int R, int TY, int TX>
 __device__ AMREX_FORCE_INLINE void ring_xy(int h, int &x, int &y) {
-    constexpr int PX = 64 + 2 * R;
+    constexpr int PX = TX + 2 * R;
     if (h < R * PX) {
         y = -R + h / PX;
         x = -R + h % PX;
@@ -540,11 +541,11 @@ __device__ AMREX_FORCE_INLINE void ring_xy(int h, int &x, int &y) {
         h -= 2 * R * PX;
         y = h / (2 * R);
         const int c = h % (2 * R);
-        x = c < R ? -R + c : 64 + (c - R);
+        x = c < R ? -R + c : TX + (c - R);
     }
 }
 
-template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int NW, int RY>
+template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int NW, int RY, int TX>
 __global__ void __launch_bounds__(64 * NW)
     stage_kernel(const StageArgs a) {
     using T = PrecTraits<P>;
@@ -553,11 +554,14 @@ __global__ void __launch_bounds__(64 * NW)
     constexpr int NC = CS::NC;
     constexpr int NS = CS::NS;
     constexpr int NP = CS::NP;
-    constexpr int TX = 64;
-    constexpr int TY = NW * RY;
+    // TX cells of a row per thread group (64: one wavefront per row; 32:
+    // two rows per wavefront, for narrow boxes); ROWS rows per workgroup.
+    static_assert(TX == 64 || TX == 32, "tile width 64 or 32");
+    constexpr int ROWS = NW * 64 / TX;
+    constexpr int TY = ROWS * RY;
     constexpr int PX = TX + 2 * R, PY = TY + 2 * R;
     constexpr int Q = 2 * R + 1;
-    constexpr int NT = TX * NW;
+    constexpr int NT = 64 * NW;
     constexpr int H = PX * PY - TX * TY;
     constexpr int HPT = (H + NT - 1) / NT;
     static_assert(R >= 1 && R <= 3, "stencil radius must be 1..3");
@@ -568,8 +572,8 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
     for (int n = 0; n < NC; ++n) slot[n] = (signed char)CS::slot(n);
 
-    const int tx = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int tx = threadIdx.x & (TX - 1);
+    const int w = threadIdx.x / TX;
     const int g = a.ghost;
     // Halo-ring cells of this thread (tile-local) and their LDS slots.
     int hx[HPT], hy[HPT], hl[HPT];
@@ -578,7 +582,7 @@ __global__ void __launch_bounds__(64 * NW)
         const int h = threadIdx.x + m * NT;
         hx[m] = 0;
         hy[m] = 0;
-        if (h < H) ring_xy<R, TY>(h, hx[m], hy[m]);
+        if (h < H) ring_xy<R, TY, TX>(h, hx[m], hy[m]);
         hl[m] = (hy[m] + R) * PX + (hx[m] + R);
     }
     const double time = a.prm[kPrmTime], dt = a.prm[kPrmDt], dx = a.prm[kPrmDx];
@@ -629,7 +633,7 @@ __global__ void __launch_bounds__(64 * NW)
         bool ldy[RY], cpy[RY];
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            const int ly = td.y0 + w + r * NW;
+            const int ly = td.y0 + w + r * ROWS;
             ldy[r] = ly < ny + R;
             cpy[r] = ly < ny;
             coff[r] = (unsigned)(wx(lx) + g) + (unsigned)(wy(ly) + g) * js;
@@ -753,7 +757,7 @@ __global__ void __launch_bounds__(64 * NW)
             for (int s = 0; s < NS; ++s) {
 #pragma unroll
                 for (int r = 0; r < RY; ++r)
-                    lds[buf][s][w + r * NW + R][tx + R] = q[s][r][R];
+                    lds[buf][s][w + r * ROWS + R][tx + R] = q[s][r][R];
 #pragma unroll
                 for (int m = 0; m < HPT; ++m)
                     if (m < HPT - 1 || threadIdx.x + m * NT < H)
@@ -798,7 +802,7 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
                 if (!(cpx && cpy[r])) continue;
-                const int gi = bd.lo[0] + lx, gj = bd.lo[1] + td.y0 + w + r * NW,
+                const int gi = bd.lo[0] + lx, gj = bd.lo[1] + td.y0 + w + r * ROWS,
                           gk = bd.lo[2] + z;
                 double ctr[NC];
 #pragma unroll
@@ -817,7 +821,7 @@ __global__ void __launch_bounds__(64 * NW)
                 tc.lds_pitch = PX;
                 tc.lds_cstride = PX * PY;
                 tc.zq_cstride = RY * Q;
-                tc.lds = &lds[buf][0][w + r * NW + R][tx + R];
+                tc.lds = &lds[buf][0][w + r * ROWS + R][tx + R];
                 tc.zq = &q[0][r][0];
                 tc.ctr = ctr;
                 tc.slot = slot;

@@ -35,13 +35,15 @@ enum StageKind : int {
     kNKind
 };
 
-/** Tile shape variants: NW wavefronts per workgroup, RY rows per thread
- *  (tile = 64 x NW*RY cells in x/y).  The autotuner picks one per level. */
+/** Tile shape variants: NW wavefronts per workgroup, RY rows per thread, TX
+ *  cells in x (tile = TX x (NW * 64 / TX) * RY cells in x/y).  The 32-wide
+ *  tile serves boxes narrower than 64 cells.  The autotuner picks one per
+ *  level. */
 struct TileVariant {
-    int nw, ry;
+    int nw, ry, tx;
 };
-constexpr int kNVar = 3;
-constexpr TileVariant kVariants[kNVar] = {{4, 1}, {8, 1}, {16, 1}};
+constexpr int kNVar = 4;
+constexpr TileVariant kVariants[kNVar] = {{4, 1, 64}, {8, 1, 64}, {16, 1, 64}, {4, 1, 32}};
 constexpr int kDefaultVariant = 1;
 
 using StageFn = hipError_t (*)(const StageArgs &, int grid, hipStream_t);
@@ -57,6 +59,7 @@ struct StageEntry {
     OccFn occupancy = nullptr;  // resident workgroups per CU
     int threads = 0;
     int ty = 0;
+    int tx = 64;
 };
 
 /** Fused SSPRK3 stages one and two (tile 64 x nw). */
