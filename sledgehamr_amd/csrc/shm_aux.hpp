@@ -387,7 +387,19 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
     const int nb = bxn * byn * (tl.pn[2] + 2 * R);
     const int np = tl.pn[0] * tl.pn[1] * tl.pn[2];
     const int cs0 = a.cand_start[fb], ncand = a.cand_start[fb + 1] - cs0;
-    // Coarse states and their time weights.
+    // Coarse states and their time weights (kCfSsprk: three times, one
+    // brick each).
+    constexpr int NTIME = MODE == kCfSsprk ? 3 : 1;
+    double twO[NTIME], twN[NTIME];
+    bool tuO[NTIME], tuN[NTIME];
+#pragma unroll
+    for (int ti = 0; ti < NTIME; ++ti) {
+        const double *rc = ti == 0 ? a.rec : a.rec3[ti - 1];
+        twO[ti] = rc[0];
+        twN[ti] = rc[1];
+        tuO[ti] = rc[2] != 0.;
+        tuN[ti] = rc[3] != 0.;
+    }
     const double wo = a.rec[0], wn = a.rec[1];
     const bool use_old = a.rec[2] != 0., use_new = a.rec[3] != 0.;
     const int ns = a.rec[4] != 0. ? 1 : 0;
@@ -454,7 +466,22 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
                     hint = qq;
                 }
             }
-            brick[q] = e >= 0 ? cval(comp, e) : __builtin_nan("");
+            if constexpr (MODE == kCfSsprk) {
+                // Both coarse states once, weighted per time as cval does.
+                double vn = 0., vo = 0.;
+                const long long ce = (long long)comp * a.ccstride + (e >= 0 ? e : 0);
+                if (tuN[0] || tuN[1] || tuN[2]) vn = load_state_at<T>(cnew, cnewlo, ce);
+                if (tuO[0] || tuO[1] || tuO[2]) vo = load_state_at<T>(cold, coldlo, ce);
+#pragma unroll
+                for (int ti = 0; ti < NTIME; ++ti) {
+                    double v = 0.;
+                    if (tuN[ti]) v = vn;
+                    if (tuO[ti]) v = tuN[ti] ? twO[ti] * vo + twN[ti] * v : vo;
+                    brick[ti * nb + q] = e >= 0 ? v : __builtin_nan("");
+                }
+            } else {
+                brick[q] = e >= 0 ? cval(comp, e) : __builtin_nan("");
+            }
         }
         __syncthreads();
         Walk3 pw(threadIdx.x, blockDim.x, tl.pn[0], tl.pn[1]);
@@ -477,7 +504,82 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
                        (long long)(2 * pc[1] + cy - db.lo[1]) * db.jstride +
                        (long long)(2 * pc[2] + cz - db.lo[2]) * db.kstride;
             };
-            if constexpr (INTERP == 4) {
+            if constexpr (MODE == kCfSsprk) {
+                // The interpolated value of child (x, y, z) from brick bb,
+                // by the same expressions as the per-parent forms below.
+                auto value = [&](const double *bb, int x, int y, int z) -> double {
+                    auto Cb = [&](int di, int dj, int dk) -> double {
+                        return bb[b0 + di + bxn * (dj + byn * dk)];
+                    };
+                    if constexpr (INTERP == 4) {
+                        auto wu = [](int q) -> double {
+                            return q == 0 ? 3. / 128. : q == 1 ? -22. / 128. : q == 2 ? 1.
+                                                      : q == 3 ? 22. / 128. : -3. / 128.;
+                        };
+                        double acc = 0.;
+#pragma unroll 1
+                        for (int cz = 0; cz < 5; ++cz)
+#pragma unroll 1
+                            for (int by = 0; by < 5; ++by)
+#pragma unroll
+                                for (int ax = 0; ax < 5; ++ax) {
+                                    const double wz = z ? wu(cz) : wu(4 - cz);
+                                    const double wy = y ? wu(by) : wu(4 - by);
+                                    const double wx = x ? wu(ax) : wu(4 - ax);
+                                    acc += wz * wy * wx * Cb(ax - 2, by - 2, cz - 2);
+                                }
+                        return acc;
+                    } else {
+                        double c[NPT];
+#pragma unroll
+                        for (int q = 0; q < NPT; ++q) c[q] = Cb(PO[q][0], PO[q][1], PO[q][2]);
+                        const double xo = x ? 0.25 : -0.25, yo = y ? 0.25 : -0.25,
+                                     zo = z ? 0.25 : -0.25;
+                        if constexpr (INTERP == 2) {
+                            const double sx = 0.5 * (c[2] - c[1]), sy = 0.5 * (c[4] - c[3]),
+                                         sz = 0.5 * (c[6] - c[5]);
+                            const double sxx = 0.5 * (c[2] - 2. * c[0] + c[1]);
+                            const double syy = 0.5 * (c[4] - 2. * c[0] + c[3]);
+                            const double szz = 0.5 * (c[6] - 2. * c[0] + c[5]);
+                            const double sxy = 0.25 * (c[7] - c[8] - c[9] + c[10]);
+                            const double sxz = 0.25 * (c[11] - c[12] - c[13] + c[14]);
+                            const double syz = 0.25 * (c[15] - c[16] - c[17] + c[18]);
+                            return c[0] + xo * sx + yo * sy + zo * sz + xo * xo * sxx +
+                                   yo * yo * syy + zo * zo * szz + xo * yo * sxy +
+                                   xo * zo * sxz + yo * zo * syz;
+                        } else if constexpr (INTERP == 1) {
+                            auto mc = [](double cm, double c0v, double cp) {
+                                const double sl = c0v - cm, sr = cp - c0v;
+                                const double sc = 0.5 * (sl + sr);
+                                if (!(sl * sr > 0.)) return 0.;
+                                const double m =
+                                    fmin(fabs(sc), fmin(2. * fabs(sl), 2. * fabs(sr)));
+                                return sc > 0. ? m : -m;
+                            };
+                            return c[0] + xo * mc(c[1], c[0], c[2]) +
+                                   yo * mc(c[3], c[0], c[4]) + zo * mc(c[5], c[0], c[6]);
+                        } else {
+                            return c[0];
+                        }
+                    }
+                };
+                // u0's leading word at t0, then k1 and k2 such that u0 + dt k1
+                // and u0 + dt/4 k2 are the states at t0 + dt and t0 + dt/2,
+                // as the separate state and k fills write them.
+                for (int z = c0[2]; z <= c1[2]; ++z)
+                    for (int y = c0[1]; y <= c1[1]; ++y)
+                        for (int x = c0[0]; x <= c1[0]; ++x) {
+                            const long long e = (long long)comp * a.fcstride + fine_elem(x, y, z);
+                            const S u0s = (S)value(brick, x, y, z);
+                            st<S>(a.dst, e, 0u, u0s);
+                            st<S>(a.kd[0], e, 0u,
+                                  (S)((value(brick + nb, x, y, z) - (double)u0s) /
+                                      (double)(S)a.cf[0]));
+                            st<S>(a.kd[1], e, 0u,
+                                  (S)((value(brick + 2 * nb, x, y, z) - (double)u0s) /
+                                      (double)(S)a.cf[1]));
+                        }
+            } else if constexpr (INTERP == 4) {
                 // Tensor product of the conservative quartic child weights;
                 // an upper child (odd fine index) takes WU, a lower one its
                 // mirror.

@@ -54,7 +54,7 @@ struct TePair {
 };
 
 /** Coarse/fine ghost fill: what a ghost cell receives (see cf_fill_kernel). */
-enum CfMode { kCfState = 0, kCfK = 1, kCfForce = 2 };
+enum CfMode { kCfState = 0, kCfK = 1, kCfForce = 2, kCfSsprk = 3 };
 
 /** Coarse parent cells of a region of fine cells [flo, flo + fn) of fine
  *  box dst_box: parent p covers fine cells 2p and 2p + 1 along each axis. */
@@ -89,6 +89,11 @@ struct CfArgs {
     // [0] weight of the old state, [1] of the new one, [2] old state used,
     // [3] new state used, [4] slot of the new state.
     const double *rec;
+    // kCfSsprk: records of t0 + dt and t0 + dt/2 (rec: t0); the state's
+    // leading words go to dst, k1 and k2 to kd[0] and kd[1], with c = cf[0]
+    // and cf[1] (dt, dt/4).
+    const double *rec3[2];
+    void *kd[2];
     int ncomp;
     int comp[32];              // interpolated component (fine = coarse index)
     void *dst, *dstlo;         // state (kCfState; dstlo null: leading words

@@ -933,6 +933,28 @@ void Runtime::fill_patch(LevelState &ls, int which, int lev, double time, bool f
     ls.ghosts_ok[which] = true;
 }
 
+void Runtime::cf_fill_ssprk(LevelState &ls, int lev, int old, double t0, double dt) {
+    CfArgs a = cf_args(lev, t0, false, true);
+    std::vector<double> rec(kPrmBlock, 0.);
+    const double tt[2] = {t0 + dt, t0 + dt / 2.};
+    for (int q = 0; q < 2; ++q) {
+        cf_record(rec.data(), lev, tt[q], false);
+        a.rec3[q] = push_record(rec, 1, tt[q]);
+    }
+    a.dst = ls.U[old].get();
+    a.dstlo = nullptr;
+    a.kd[0] = ls.K[0].get();
+    a.kd[1] = ls.K[1].get();
+    a.cf[0] = dt;
+    a.cf[1] = dt / 4.;
+    const std::vector<int> sc = stencil_comps();
+    for (size_t c0 = 0; c0 < sc.size(); c0 += 32) {
+        a.ncomp = (int)std::min<size_t>(32, sc.size() - c0);
+        for (int q = 0; q < a.ncomp; ++q) a.comp[q] = sc[c0 + q];
+        cf_launch(a, kCfSsprk);
+    }
+}
+
 std::vector<int> Runtime::step_ghost_comps() const {
     if (ko_ || integrator_ >= 20) return all_comps();
     return stencil_comps();
@@ -1156,7 +1178,7 @@ CfArgs Runtime::cf_args(int lev, double time, bool first_state_only, bool step) 
 
 template <Prec P, int M>
 static void cf_dispatch(const CfArgs &a, int interp, int grid, hipStream_t s) {
-    const size_t lds = sizeof(double) * (size_t)a.lds_cells;
+    const size_t lds = sizeof(double) * (size_t)a.lds_cells * (M == kCfSsprk ? 3 : 1);
     switch (interp) {
         case 0:
             hipLaunchKernelGGL((cf_fill_kernel<P, M, 0>), dim3(grid), dim3(256), lds, s, a);
@@ -1180,6 +1202,8 @@ static void cf_dispatch_mode(const CfArgs &a, int mode, int interp, int grid,
         cf_dispatch<P, kCfState>(a, interp, grid, s);
     else if (mode == kCfK)
         cf_dispatch<P, kCfK>(a, interp, grid, s);
+    else if (mode == kCfSsprk)
+        cf_dispatch<P, kCfSsprk>(a, interp, grid, s);
     else
         cf_dispatch<P, kCfForce>(a, interp, grid, s);
 }
@@ -1187,7 +1211,8 @@ static void cf_dispatch_mode(const CfArgs &a, int mode, int interp, int grid,
 void Runtime::cf_launch(CfArgs &a, int mode) {
     if (a.total == 0 || a.ncomp == 0) return;
     const int grid = a.ntile;  // one workgroup per tile
-    if (a.lds_cells * sizeof(double) > 65536) throw Error(5, "coarse/fine brick exceeds LDS");
+    if (a.lds_cells * sizeof(double) * (mode == kCfSsprk ? 3 : 1) > 65536)
+        throw Error(5, "coarse/fine brick exceeds LDS");
     const int ip = interp_type_;
     EventPair *ep = timing_on ? timed_pair(1) : nullptr;
     if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
@@ -1322,7 +1347,9 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             a.x = ls.U[old].get();
             a.kout = ls.K[0].get();
             launch_stage(kSsprk1, ls, a, lev);
-            if (cf) cf_fill_k(ls.K[0].get(), st[1], dt);
+            // Coarse/fine ghosts of k1 and k2 were written with the step's
+            // start state (cf_fill_ssprk); stage kernels write valid cells.
+            if (cf && !cf_merged(lev)) cf_fill_k(ls.K[0].get(), st[1], dt);
             halo(ls, ls.K[0].get(), stencil_comps());
 
             StageArgs b{};
@@ -1333,7 +1360,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             b.c[0] = 1.;
             b.c[2] = dt;
             launch_stage(kSsprk2, ls, b, lev);
-            if (cf) cf_fill_k(ls.K[1].get(), st[2], dt / 4.);
+            if (cf && !cf_merged(lev)) cf_fill_k(ls.K[1].get(), st[2], dt / 4.);
             halo(ls, ls.K[1].get(), stencil_comps());
         }
 
@@ -1518,7 +1545,13 @@ void Runtime::advance(int lev) {
     const int old = ls.cur ^ 1;
     ls.t_old = ls.t;
     const double dt = sim_->dt[lev], dx = sim_->dx[lev];
-    fill_patch(ls, old, lev, ls.t_old, /*for_step=*/true);
+    if (cf_merged(lev)) {
+        cf_fill_ssprk(ls, lev, old, ls.t_old, dt);
+        halo(ls, ls.U[old].get(), step_ghost_comps());
+        ls.ghosts_ok[old] = false;
+    } else {
+        fill_patch(ls, old, lev, ls.t_old, /*for_step=*/true);
+    }
     if (!replay_step(ls, lev, dt, dx, ls.t_old)) {
         integrate(ls, lev, dt, dx, ls.t_old);
         if (lev >= 0) warm_[lev] = epoch_;

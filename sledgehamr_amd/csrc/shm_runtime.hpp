@@ -342,6 +342,12 @@ class Runtime {
      *  stencil components; every component with dissipation, or for RKN,
      *  whose stage states at ghost cells compose phi0 + c pi0 + ... */
     std::vector<int> step_ghost_comps() const;
+    /** SSPRK3 step of a level with coarse/fine ghost cells: the step-start
+     *  state and both stage k arrays are filled at once, before stage one. */
+    bool cf_merged(int lev) const { return has_cf(lev) && integrator_ == 10; }
+    /** That one fill: u0 (leading words) at t0, k1 and k2 for t0 + dt and
+     *  t0 + dt/2, in the ghost layers the stencil reads. */
+    void cf_fill_ssprk(LevelState &ls, int lev, int old, double t0, double dt);
     /** Every cell of state `which` of level lev, ghosts included, from level
      *  lev-1 at `time` (first_state_only: FillCoarsePatch's choice). */
     void fill_coarse_full(int lev, int which, double time, bool first_state_only);
