@@ -181,6 +181,14 @@ __device__ AMREX_FORCE_INLINE T ld(const void *p, long long base, unsigned off) 
     const char *b = reinterpret_cast<const char *>(static_cast<const T *>(p) + base);
     return *reinterpret_cast<const T *>(b + (unsigned)(off * (unsigned)sizeof(T)));
 }
+// Read once, at the centre only (remainders): a non-temporal load keeps it
+// from displacing the stencil halos in L2.
+template <class T>
+__device__ AMREX_FORCE_INLINE T ld_nt(const void *p, long long base, unsigned off) {
+    const char *b = reinterpret_cast<const char *>(static_cast<const T *>(p) + base);
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const T *>(b + (unsigned)(off * (unsigned)sizeof(T))));
+}
 // Stage outputs are written once and not read again by the same kernel:
 // non-temporal stores keep them from displacing the stencil halos in L2.
 template <class T>
@@ -223,9 +231,9 @@ __device__ AMREX_FORCE_INLINE double load_lo(const void *lo, long long base,
     if constexpr (!T::comp)
         return 0.;
     else if constexpr (std::is_same_v<typename T::L, float>)
-        return (double)ld<float>(lo, base, off);
+        return (double)ld_nt<float>(lo, base, off);
     else
-        return (double)bf16_to_f32(ld<uint16_t>(lo, base, off));
+        return (double)bf16_to_f32(ld_nt<uint16_t>(lo, base, off));
 }
 template <class T>
 __device__ AMREX_FORCE_INLINE double load_state(const void *hi, const void *lo,
@@ -305,6 +313,22 @@ struct PolSsprk {
             return fma_st<S>((S)a.c[2], k, u);
         } else {
             return load_st<CS, T>(a, org, n, off);
+        }
+    }
+    /** load_keep for a point component: read at the centre only, so
+     *  non-temporal (the stencil components' halos keep L2). */
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+        using S = typename T::S;
+        const long long b = org + n * a.cstride;
+        if constexpr (composed) {
+            const S k = ld_nt<S>(a.kin, b, off), u = ld_nt<S>(a.u0, b, off);
+            keep[0] = k;
+            if constexpr (K == kSsStage3) keep[1] = u;
+            return fma_st<S>((S)a.c[2], k, u);
+        } else {
+            return ld_nt<S>(a.x, b, off);
         }
     }
     template <class CS, class T>
@@ -401,6 +425,11 @@ struct PolRkn {
         return load_keep<CS, T>(a, org, n, off, keep);
     }
     template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+        return load_keep<CS, T>(a, org, n, off, keep);
+    }
+    template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
                                                        long long org, unsigned off,
                                                        typename T::S *e) {
@@ -477,6 +506,11 @@ struct PolLeapfrog {
     __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
         const StageArgs &a, long long org, int n, unsigned off, typename T::S *) {
         return load_st<CS, T>(a, org, n, off);
+    }
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+        return load_keep<CS, T>(a, org, n, off, keep);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
@@ -712,7 +746,7 @@ __global__ void __launch_bounds__(64 * NW)
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int r = 0; r < RY; ++r)
-                    pv[p][r] = Pol::template load_keep<CS, T>(
+                    pv[p][r] = Pol::template load_pt<CS, T>(
                         a, org, CS::comp_of_point(p), qoff[r] + zo, pk[p][r]);
             if constexpr (NK > 0 && !DELAY) {
 #pragma unroll
