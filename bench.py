@@ -87,7 +87,7 @@ def moved_per_cell(prec_key, kernel):
 # read + write requests counted by rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B
 # TCC_EA0_WRREQ_WRITE_DRAM_32B (x 32 B), profiles/stage_kernel_traffic_complex512.md.
 TRAFFIC_BYTES_512 = {"fused12": (74270667 + 67108864) * 32.0,
-                     "stage3": (197784618 + 100663296) * 32.0}
+                     "stage3": (174360952 + 100663296) * 32.0}
 
 
 def kernel_record(name, avg_ms, alg_bytes, moved, traffic):
