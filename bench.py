@@ -224,58 +224,82 @@ def cpu_baseline(inp, seed):
                       f"{threads} threads on {cpu_model()}, {dt:.1f} s"}
 
 
-def ensemble(shm, seed, steps=200, warmup=10):
+# Bytes per cell and RKN4 step of the ensemble (real scalar: one field and
+# its momentum; three stages, each one pass): stage 1 reads phi0, pi0 and
+# writes F0; stage 2 reads phi0, pi0, F0 and writes F1; stage 3 reads phi0,
+# pi0, F0, F1 and the two remainders and writes phi, pi and their remainders.
+# No pass can do with less, so these are both the algorithmic and the moved
+# bytes.
+ENSEMBLE_BYTES_PER_CELL_STEP = {"fp32c16": 3 * 4 + 4 * 4 + 6 * 4 + 4 * 2,
+                                "fp64": 3 * 8 + 4 * 8 + 6 * 8}
+
+
+def ensemble(shm, seed, steps=200, warmup=10, timing_steps=5):
     """Ensemble of 64 independent real-scalar realizations (edges 32..160,
     seeded), RKN4, 4th-order Laplacian, one launch per stage over the box
-    table: p50 / p99 wall time per step (each step synchronised), aggregate
-    cell-updates/s, and the FP32-compensated field error against FP64."""
+    table, steps replayed from captured graphs: p50 / p99 wall time per step
+    (each step synchronised), aggregate cell-updates/s, the same for FP64,
+    the stage-kernel time per step (HIP events, eager steps) and the
+    FP32-compensated field error against FP64 after the same steps."""
     from cases import ensemble_boxes, ensemble_fields, ensemble_inputs
     boxes = ensemble_boxes(64, 32, 160, seed)
     inp = ensemble_inputs(boxes)
     fields = ensemble_fields(boxes, seed)
-    out = {}
+    out, final = {}, {}
     for prec in (shm.PREC_FP32C16, shm.PREC_FP64):
+        key = "fp32c16" if prec == shm.PREC_FP32C16 else "fp64"
         sim = shm.Simulation("RealScalar", inp, precision=prec)
         sim.set_boxes(0, boxes)
         for c, f in enumerate(fields):
             sim.upload(0, c, f)
         sim.autotune(0)
-        progress("ensemble: autotuned")
+        progress(f"ensemble {key}: autotuned")
         sim.set_graphs(True)
         sim.advance(0, warmup)
         sim.sync()
-        progress("ensemble: warm-up steps")
         lat = []
-        n = steps if prec == shm.PREC_FP32C16 else 0
-        for _ in range(n):
+        for _ in range(steps):
             t0 = time.perf_counter()
             sim.advance(0, 1)
             sim.sync()
             lat.append(time.perf_counter() - t0)
-        progress("ensemble: timed steps")
-        if n:
-            lat = np.array(lat) * 1e3
-            cells = sim.ncells(0)
+        progress(f"ensemble {key}: timed steps")
+        sim.set_graphs(False)
+        sim.set_timing(True)
+        sim.advance(0, timing_steps)
+        sim.sync()
+        tm = sim.timing()
+        sim.set_timing(False)
+        lat = np.array(lat) * 1e3
+        cells = sim.ncells(0)
+        bpc = ENSEMBLE_BYTES_PER_CELL_STEP[key]
+        kern_ms = tm["stage_ms"] / timing_steps
+        rec = {"p50_ms": float(np.percentile(lat, 50)),
+               "p99_ms": float(np.percentile(lat, 99)),
+               "value": cells * 3 * steps / (lat.sum() * 1e-3),
+               "unit": "cell-updates/s",
+               "bytes_per_cell_stage": bpc / 3.0,
+               "stage_kernel_ms_per_step": kern_ms,
+               "kernel_frac": bpc * cells / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               "launch": sim.get_launch(0)}
+        rec["step_frac"] = rec["value"] * bpc / 3.0 / 1e9 / HBM_PEAK_GBS
+        final[key] = [sim.download(0, c) for c in range(2)]
+        if prec == shm.PREC_FP32C16:
             out = {"workload": "64 independent real-scalar boxes, edges 32..160 "
                                "(seeded), dx 1/64, RKN4, 4th-order Laplacian, "
                                "FP32 + bf16 remainder, steps replayed from "
                                "captured graphs",
-                   "boxes": len(boxes), "cells": cells, "steps": n,
-                   "p50_ms": float(np.percentile(lat, 50)),
-                   "p99_ms": float(np.percentile(lat, 99)),
-                   "value": cells * 3 * n / (lat.sum() * 1e-3),
-                   "unit": "cell-updates/s"}
-            u32 = [sim.download(0, c) for c in range(2)]
-            ref_steps = warmup + n
+                   "boxes": len(boxes), "cells": cells, "steps": steps, **rec}
         else:
-            sim.advance(0, ref_steps - warmup)
-            err = 0.0
-            for c in range(2):
-                x64 = sim.download(0, c)
-                err = max(err, float(np.max(np.abs(u32[c] - x64)) /
-                                     max(np.max(np.abs(x64)), 1e-300)))
-            out["rel_err_vs_fp64"] = err
+            out["fp64"] = rec
+            out["speedup_fp32c_vs_fp64"] = out["value"] / rec["value"]
         sim.close()
+    err = 0.0
+    for c in range(2):
+        x32, x64 = final["fp32c16"][c], final["fp64"][c]
+        err = max(err, float(np.max(np.abs(x32 - x64)) /
+                             max(np.max(np.abs(x64)), 1e-300)))
+    out["rel_err_vs_fp64"] = err
     return out
 
 
@@ -373,6 +397,8 @@ def main():
     ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--only-amr", action="store_true",
                     help="run only the batched AMR block (for profiling) and print it")
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="run only the ensemble block (for profiling) and print it")
     ap.add_argument("--amr-no-autotune", action="store_true",
                     help="default launch shapes in the batched AMR block (a profile "
                          "then holds no autotuner launches)")
@@ -392,6 +418,9 @@ def main():
 
     import sledgehamr_amd as shm
 
+    if a.only_ensemble:
+        print(json.dumps({"ensemble": ensemble(shm, a.seed)}))
+        return
     if a.only_amr:
         print(json.dumps({"amr": amr_batched(shm, a.seed,
                                              autotune=not a.amr_no_autotune)}))

@@ -2111,11 +2111,24 @@ void Runtime::autotune(int lev) {
         s2.c[2] = dt;
         runs.push_back({kSsprk2, s2});
     } else if (integrator_ >= 20) {
-        if (nodes_.size() < 2) return;
-        StageArgs s1 = a;
-        s1.u0 = ls.U[old].get();
-        s1.fout = ls.F[0].get();
-        runs.push_back({kRkn0, s1});
+        // Every stage of a step: the stages differ in how many arrays they
+        // read (the last one composes phi0, pi0 and every earlier force at
+        // each stencil point), so the shape is chosen on their sum.  The
+        // final stage is added below, with a scratch target.
+        const int s = (int)nodes_.size();
+        if (s < 2) return;
+        for (int i = 0; i + 1 < s; ++i) {
+            StageArgs r = a;
+            r.u0 = ls.U[old].get();
+            r.c[0] = dt * nodes_[i];
+            r.nprev = i;
+            for (int j = 0; j < i; ++j) {
+                r.c[1 + j] = dt * dt * tableau_[i][j];
+                r.f[j] = ls.F[j].get();
+            }
+            r.fout = ls.F[i].get();
+            runs.push_back({kRkn0 + i, r});
+        }
     } else {
         return;  // leapfrog kick 1 writes u_new fields; keep the default
     }
@@ -2137,15 +2150,38 @@ void Runtime::autotune(int lev) {
     const std::vector<int> os = choices(launch.zc_minor, {0, 1});
     const std::vector<int> xs = choices(launch.xcd_split, {1, 0});
 
-    // SSPRK3: stage three (the new state into a scratch target, so the
-    // level's states are untouched) is timed per shape too; it is the
-    // stage kernel that remains when stages one and two run fused.
+    // Final stages write the new state into a scratch target, so the
+    // level's states are untouched.  SSPRK3: stage three is timed per shape
+    // too; it is the stage kernel that remains when stages one and two run
+    // fused.
     DevMem t3, t3lo;
     StageArgs s3{};
     const bool tune3 = integrator_ == 10;
-    if (tune3) {
+    if (tune3 || integrator_ >= 20) {
         t3.alloc((size_t)ls.L.cstride * ls.ncomp * esize());
         if (comp()) t3lo.alloc((size_t)ls.L.cstride * ls.ncomp * losize());
+    }
+    if (integrator_ >= 20) {
+        const int i = (int)nodes_.size() - 1;
+        StageArgs r = a;
+        r.u0 = ls.U[old].get();
+        r.u0lo = comp() ? ls.Ulo[old].get() : nullptr;
+        r.c[0] = dt * nodes_[i];
+        r.nprev = i;
+        for (int j = 0; j < i; ++j) {
+            r.c[1 + j] = dt * dt * tableau_[i][j];
+            r.f[j] = ls.F[j].get();
+            r.c[7 + j] = dt * dt * bbar_[j];
+            r.c[12 + j] = dt * b_[j];
+        }
+        r.c[6] = dt;
+        r.c[11] = dt * dt * bbar_[i];
+        r.c[16] = dt * b_[i];
+        r.unew = t3.get();
+        r.unewlo = comp() ? t3lo.get() : nullptr;
+        runs.push_back({kRknF0 + i, r});
+    }
+    if (tune3) {
         s3 = a;
         s3.u0 = ls.U[old].get();
         s3.u0lo = comp() ? ls.Ulo[old].get() : nullptr;

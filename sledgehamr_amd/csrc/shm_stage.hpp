@@ -402,6 +402,10 @@ struct PolRkn {
         const int p = CS::pair_of_field(n);
         const S phi0 = ld<S>(a.u0, org + n * a.cstride, off);
         if (p < 0) return phi0;
+        // A first stage at node 0 (RKN4, RKN5) has the state phi0: pi0 is
+        // then read at the centre only.
+        if constexpr (!FINAL && NPREV == 0)
+            if (a.c[0] == 0.) return phi0;
         const S pi0 = ld<S>(a.u0, org + CS::pair_mom(p) * a.cstride, off);
         S y = phi0;
         y += (S)a.c[0] * pi0;
