@@ -18,6 +18,13 @@
 // boxes that wrap periodically along every axis (a single box spanning the
 // domain, or an ensemble of independent boxes), where the extra reach needs
 // no ghost cells.
+//
+// With RKN set, the same pass runs the first two stages of a Runge-Kutta-
+// Nystroem step whose first node is 0 (rkn.cpp:52-119; RKN4 and RKN5): the
+// first stage's state is u0, its forces F0 (the momenta's right-hand sides)
+// are written for the tile's own cells, and the second stage's state of a
+// field is phi0 + c0 pi0 + c1 F0 (momenta stay at pi0), formed in the storage
+// type as the separate second stage forms it; that stage writes F1.
 #pragma once
 
 #include "shm_stage.hpp"
@@ -35,8 +42,10 @@ struct FusedArgs {
     int ghost;
     long long cstride;
     const void *u0;
-    void *kout;
+    void *kout;          // SSPRK3: k2; RKN: F1
     double dt;           // u1 = u0 + dt k1
+    void *f0out;         // RKN: F0
+    double c0, c1;       // RKN: second-stage state phi0 + c0 pi0 + c1 F0
 };
 
 /** Static shape of a fused tile: 64 x NW cells, radius R. */
@@ -59,7 +68,7 @@ struct FusedShape {
         (NC + NP) * (R + 1) * (int)(sizeof(S) / 4) <= 48;
 };
 
-template <class Ops, Prec P, int R, bool GW, bool KO, int NW>
+template <class Ops, Prec P, int R, bool GW, bool KO, int NW, bool RKN>
 __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     using F = FusedShape<Ops, P, R, GW, KO, NW>;
     using S = typename F::S;
@@ -281,23 +290,53 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                     Ops::gw_rhs(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, gpar1);
                     Ops::gw_backreaction(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, par1, gpar1);
                 }
-                // k1 in the storage type, as the two-launch path stores it;
-                // u1 = u0 + dt k1 in the storage type.
-                S k1[NC];
+                if constexpr (RKN) {
+                    // F0 per pair in the storage type, as the separate first
+                    // stage stores it.
+                    constexpr int NPR = CS::NPAIR > 0 ? CS::NPAIR : 1;
+                    S f0[NPR];
 #pragma unroll
-                for (int n = 0; n < NC; ++n) k1[n] = (S)rhs[n];
+                    for (int p = 0; p < CS::NPAIR; ++p) f0[p] = (S)rhs[CS::pair_mom(p)];
+                    auto state2 = [&](int n) -> S {
+                        const int p = CS::pair_of_field(n);
+                        S y = u0v[n];
+                        if (p < 0) return y;
+                        y += (S)a.c0 * u0v[CS::pair_mom(p)];
+                        y += (S)a.c1 * f0[p];
+                        return y;
+                    };
 #pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    const int n = CS::comp_of_slot(s);
-                    w1[s][sz][l1[m]] = fma_st<S>(cdt, k1[n], u0v[n]);
-                }
-                if (m == 0) {
+                    for (int s = 0; s < NS; ++s)
+                        w1[s][sz][l1[m]] = state2(CS::comp_of_slot(s));
+                    if (m == 0) {
 #pragma unroll
-                    for (int n = 0; n < NC; ++n) dk1[R][n] = k1[n];
+                        for (int p = 0; p < NP; ++p) du1[R][p] = state2(CS::comp_of_point(p));
+                        if (own && zz >= td.z0 && zz < td.z1) {
+                            const unsigned o = cown + (unsigned)(zz + g) * ks;
 #pragma unroll
-                    for (int p = 0; p < NP; ++p) {
-                        const int n = CS::comp_of_point(p);
-                        du1[R][p] = fma_st<S>(cdt, k1[n], u0v[n]);
+                            for (int p = 0; p < CS::NPAIR; ++p)
+                                st<S>(a.f0out, org + p * a.cstride, o, f0[p]);
+                        }
+                    }
+                } else {
+                    // k1 in the storage type, as the two-launch path stores
+                    // it; u1 = u0 + dt k1 in the storage type.
+                    S k1[NC];
+#pragma unroll
+                    for (int n = 0; n < NC; ++n) k1[n] = (S)rhs[n];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const int n = CS::comp_of_slot(s);
+                        w1[s][sz][l1[m]] = fma_st<S>(cdt, k1[n], u0v[n]);
+                    }
+                    if (m == 0) {
+#pragma unroll
+                        for (int n = 0; n < NC; ++n) dk1[R][n] = k1[n];
+#pragma unroll
+                        for (int p = 0; p < NP; ++p) {
+                            const int n = CS::comp_of_point(p);
+                            du1[R][p] = fma_st<S>(cdt, k1[n], u0v[n]);
+                        }
                     }
                 }
             }
@@ -354,10 +393,16 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             zst = next(zst, nz);
             if (own) {
                 const unsigned o = cown + zso;
+                if constexpr (RKN) {
 #pragma unroll
-                for (int n = 0; n < NC; ++n) {
-                    const double k = rhs[n] + (double)dk1[0][n];
-                    st<S>(a.kout, org + n * a.cstride, o, (S)k);
+                    for (int p = 0; p < CS::NPAIR; ++p)
+                        st<S>(a.kout, org + p * a.cstride, o, (S)rhs[CS::pair_mom(p)]);
+                } else {
+#pragma unroll
+                    for (int n = 0; n < NC; ++n) {
+                        const double k = rhs[n] + (double)dk1[0][n];
+                        st<S>(a.kout, org + n * a.cstride, o, (S)k);
+                    }
                 }
             }
         }

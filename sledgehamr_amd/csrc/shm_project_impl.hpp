@@ -162,24 +162,26 @@ void fill_kinds(KernelTable &kt) {
     }
 }
 
-template <class Ops, Prec P, int R, bool GW, int V>
+template <class Ops, Prec P, int R, bool GW, int V, bool RKN>
 hipError_t launch_fused(const FusedArgs &a, int grid, hipStream_t s) {
     constexpr int NW = kVariants[V].nw;
-    hipLaunchKernelGGL((fused12_kernel<Ops, P, R, GW, false, NW>), dim3(grid),
+    hipLaunchKernelGGL((fused12_kernel<Ops, P, R, GW, false, NW, RKN>), dim3(grid),
                        dim3(64 * NW), 0, s, a);
     return hipGetLastError();
 }
 
-template <class Ops, Prec P, int R, bool GW, int V>
+template <class Ops, Prec P, int R, bool GW, int V, bool RKN>
 int occupancy_fused() {
     constexpr int NW = kVariants[V].nw;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, fused12_kernel<Ops, P, R, GW, false, NW>, 64 * NW, 0) != hipSuccess)
+            &nb, fused12_kernel<Ops, P, R, GW, false, NW, RKN>, 64 * NW, 0) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }
 
+/** Fused first two stages: SSPRK3 (fused12) and RKN with a first node 0
+ *  (fusedrkn), on 64-wide tiles whose windows fit. */
 template <class Ops, Prec P, int R, bool GW>
 void fill_fused(KernelTable &kt) {
     auto put = [&](auto vc) {
@@ -187,10 +189,15 @@ void fill_fused(KernelTable &kt) {
         if constexpr (kVariants[V].ry == 1 && kVariants[V].tx == 64 &&
                       FusedShape<Ops, P, R, GW, false, kVariants[V].nw>::fits) {
             FusedEntry &e = kt.fused12[(int)P][R][GW ? 1 : 0][V];
-            e.fn = &launch_fused<Ops, P, R, GW, V>;
-            e.occupancy = &occupancy_fused<Ops, P, R, GW, V>;
+            e.fn = &launch_fused<Ops, P, R, GW, V, false>;
+            e.occupancy = &occupancy_fused<Ops, P, R, GW, V, false>;
             e.threads = 64 * kVariants[V].nw;
             e.ty = kVariants[V].nw;
+            FusedEntry &r = kt.fusedrkn[(int)P][R][GW ? 1 : 0][V];
+            r.fn = &launch_fused<Ops, P, R, GW, V, true>;
+            r.occupancy = &occupancy_fused<Ops, P, R, GW, V, true>;
+            r.threads = e.threads;
+            r.ty = e.ty;
         }
     };
     put(std::integral_constant<int, 0>{});

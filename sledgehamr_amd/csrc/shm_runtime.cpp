@@ -766,17 +766,31 @@ LaunchCfg Runtime::resolved_launch(int lev) const {
     c.zc_minor = pick(launch.zc_minor, t.zc_minor, 0);
     c.fused = pick(launch.fused, t.fused, 0);
     int dv = -1;
+    const FusedEntry *row = fused_row();
     for (int v = kNVar - 1; v >= 0 && dv < 0; --v)
-        if (kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0][v].fn) dv = v;
+        if (row[v].fn) dv = v;
     c.fused_variant = pick(launch.fused_variant, t.fused_variant, dv);
     return c;
 }
 
+const FusedEntry *Runtime::fused_row() const {
+    return integrator_ >= 20 ? kt_.fusedrkn[(int)prec_][R_][gw_ ? 1 : 0]
+                             : kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0];
+}
+
+/** The first two stages run in one pass: SSPRK3, or RKN whose first node is
+ *  0 and which has a third stage, without dissipation, on a level of boxes
+ *  that wrap on themselves. */
 bool Runtime::fused_possible(const LevelState &ls) const {
-    if (integrator_ != 10 || ko_) return false;
+    if (ko_) return false;
+    if (integrator_ >= 20) {
+        if (nodes_.size() < 3 || nodes_[0] != 0.) return false;
+    } else if (integrator_ != 10) {
+        return false;
+    }
+    const FusedEntry *row = fused_row();
     bool any = false;
-    for (int v = 0; v < kNVar; ++v)
-        any = any || kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0][v].fn != nullptr;
+    for (int v = 0; v < kNVar; ++v) any = any || row[v].fn != nullptr;
     if (!any) return false;
     for (const BoxDesc &b : ls.L.hbox)
         if (b.wrap != 7) return false;
@@ -787,7 +801,7 @@ void Runtime::launch_fused(LevelState &ls, const double *prm1, const double *prm
                            double dt, int lev) {
     const LaunchCfg lc = resolved_launch(lev);
     int v = lc.fused_variant;
-    const auto &row = kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0];
+    const FusedEntry *row = fused_row();
     if (v < 0 || v >= kNVar || !row[v].fn) {
         v = -1;
         for (int d = kNVar - 1; d >= 0 && v < 0; --d)
@@ -812,6 +826,13 @@ void Runtime::launch_fused(LevelState &ls, const double *prm1, const double *prm
     a.u0 = ls.U[ls.cur ^ 1].get();
     a.kout = ls.K[1].get();
     a.dt = dt;
+    if (integrator_ >= 20) {
+        // RKN stages one and two: F0, F1 and the second stage's state.
+        a.f0out = ls.F[0].get();
+        a.kout = ls.F[1].get();
+        a.c0 = dt * nodes_[1];
+        a.c1 = dt * dt * tableau_[1][0];
+    }
     if (ntiles == 0) return;
     const int occ = e.occupancy();
     int grid = lc.persistent ? (int)std::min<long long>(ntiles, (long long)occ * ncu_)
@@ -1377,7 +1398,16 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         launch_stage(kSsprk3, ls, c, lev);
     } else if (integrator_ >= 20) {
         const int s = (int)nodes_.size();
-        for (int i = 0; i < s; ++i) {
+        int i0 = 0;
+        if (resolved_launch(lev).fused && fused_possible(ls)) {
+            // Stages one and two in one pass (boxes wrap on themselves, so
+            // there are no coarse/fine ghosts); F0 and F1 as below.
+            launch_fused(ls, prm(st[0]), prm(st[1]), dt, lev);
+            halo(ls, ls.F[0].get(), pairs);
+            halo(ls, ls.F[1].get(), pairs);
+            i0 = 2;
+        }
+        for (int i = i0; i < s; ++i) {
             StageArgs a{};
             a.prm = prm(st[i]);
             a.u0 = ls.U[old].get();
@@ -2263,6 +2293,31 @@ void Runtime::autotune(int lev) {
                 bc = bc3;
                 bc.fused = 1;
                 bc.fused_variant = v;
+            }
+        }
+    }
+    // RKN on a level of periodic boxes: one fused launch for stages one and
+    // two plus the remaining stages at the best unfused shape, against the
+    // whole unfused step.
+    if (integrator_ >= 20 && fused_possible(ls) && saved.fused != 0) {
+        const FusedEntry *row = fused_row();
+        const LaunchCfg base = bc;
+        for (int v = 0; v < kNVar; ++v) {
+            if (!row[v].fn) continue;
+            if (saved.fused_variant >= 0 && v != saved.fused_variant) continue;
+            launch = base;
+            launch.fused = 1;
+            launch.fused_variant = v;
+            const double ms = time_ms([&] {
+                launch_fused(ls, prm, prm, dt, lev);
+                for (size_t k = 2; k < runs.size(); ++k) {
+                    StageArgs c = runs[k].second;
+                    launch_stage(runs[k].first, ls, c, lev);
+                }
+            });
+            if (ms < best) {
+                best = ms;
+                bc = launch;
             }
         }
     }

@@ -289,6 +289,7 @@ class Runtime {
     /** Fused SSPRK3 stages one and two possible at this level (every box
      *  periodic on itself, kernel built for the precision and radius). */
     bool fused_possible(const LevelState &ls) const;
+    const FusedEntry *fused_row() const;  // fused12 or fusedrkn, by integrator
 
   private:
     struct EventPair {

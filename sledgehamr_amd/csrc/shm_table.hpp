@@ -62,7 +62,7 @@ struct StageEntry {
     int tx = 64;
 };
 
-/** Fused SSPRK3 stages one and two (tile 64 x nw). */
+/** Fused first two stages of SSPRK3 or RKN (tile 64 x nw). */
 struct FusedEntry {
     FusedFn fn = nullptr;
     int (*occupancy)() = nullptr;
@@ -76,7 +76,8 @@ struct KernelTable {
     int r_mask = 0;
     // [kind][precision][R][gw][ko][variant]
     StageEntry stage[kNKind][3][4][2][2][kNVar];
-    FusedEntry fused12[3][4][2][kNVar];  // [precision][R][gw][variant]
+    FusedEntry fused12[3][4][2][kNVar];   // [precision][R][gw][variant]
+    FusedEntry fusedrkn[3][4][2][kNVar];  // the same for RKN stages one and two
     TeTagFn te_tag[3][2] = {};  // [precision][gw]
     ProbeFn probe[2] = {};      // [gw]
     // Output functions for spectra and projections (SLEDGEHAMR_OUTPUT_FUNCTIONS).

@@ -1,7 +1,9 @@
-"""Fused SSPRK3 stages one and two (k1 kept on chip) against the two-launch
-path: identical bits, for every precision, tile shape and stencil radius,
-on a periodic box, on ensembles of small independent boxes (tiles wider
-than the box), and inside captured level steps; and against the oracle."""
+"""Fused first two stages against separate launches: SSPRK3 (k1 kept on
+chip) and RKN with a first node 0 (F0 written, the second stage's state
+built on chip).  Identical bits, for every precision, tile shape and
+stencil radius, on a periodic box, on ensembles of small independent boxes
+(tiles wider than the box), and inside captured level steps; and against
+the oracle."""
 import numpy as np
 import pytest
 
@@ -97,3 +99,111 @@ def test_multibox_level_keeps_two_launches():
     sim = shm.Simulation("ComplexScalar", inp, precision=shm.PREC_FP32C16)
     sim.set_launch(fused=1)
     assert sim.get_launch(0)["fused"] == 0
+
+
+def _rkn_fields(N, seed):
+    phi = gaussian_random_field(N, seed=seed, kcut=8, std=0.2)
+    pi = gaussian_random_field(N, seed=seed + 1, kcut=8, std=0.05)
+    return phi, pi
+
+
+@pytest.mark.parametrize("fv", [0, 1, 2])
+@pytest.mark.parametrize("prec", ALLP)
+@pytest.mark.parametrize("integrator", [21, 22])
+def test_fused_rkn_equals_separate_stages(integrator, prec, fv):
+    """RKN4 (three stages: fused pair, then the final stage) and RKN5 (fused
+    pair, a third stage that reads F0 and F1, the final stage); momenta
+    nonzero, so the second stage's state carries c0 pi0."""
+    inp = real_scalar(N=32, order=2, integrator=integrator, lam=1.0)
+    a, b = _pair("RealScalar", inp, prec, fv)
+    phi, pi = _rkn_fields(32, 5)
+    for s in (a, b):
+        s.upload(0, 0, phi)
+        s.upload(0, 1, pi)
+    a.advance(0, 4)
+    b.advance(0, 4)
+    assert b.get_launch(0)["fused"] == 1 and a.get_launch(0)["fused"] == 0
+    _same(a, b, 0, 2, ("rkn", integrator, prec, fv))
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_fused_rkn_radius(order):
+    inp = real_scalar(N=32, order=order, integrator=21, lam=1.0)
+    a, b = _pair("RealScalar", inp, shm.PREC_FP32C16, 2)
+    phi, pi = _rkn_fields(32, order)
+    for s in (a, b):
+        s.upload(0, 0, phi)
+        s.upload(0, 1, pi)
+    a.advance(0, 3)
+    b.advance(0, 3)
+    _same(a, b, 0, 2, ("rkn radius", order))
+
+
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+def test_fused_rkn_ensemble_small_boxes(prec):
+    boxes = ensemble_boxes(6, 16, 48, seed=13)
+    inp = ensemble_inputs(boxes)
+    assert inp["integrator.type"] == 21
+    a, b = _pair("RealScalar", inp, prec, 2)
+    fields = ensemble_fields(boxes, seed=6)
+    for s in (a, b):
+        s.set_boxes(0, boxes)
+        for c, f in enumerate(fields):
+            s.upload(0, c, f)
+    a.advance(0, 3)
+    b.advance(0, 3)
+    _same(a, b, 0, 2, "rkn ensemble")
+    orc = Oracle("RealScalar", inp)
+    orc.set_boxes(0, boxes)
+    for c, f in enumerate(fields):
+        orc.upload(0, c, f)
+    orc.advance(0, 3)
+    tol = 1e-12 if prec == shm.PREC_FP64 else 1e-6
+    for c in range(2):
+        assert rel_linf(b.download(0, c), orc.download(0, c)) < tol
+
+
+def test_fused_rkn_in_captured_steps():
+    inp = real_scalar(N=32, order=2, integrator=22, lam=1.0)
+    a, b = _pair("RealScalar", inp, shm.PREC_FP32C16, 1, graphs=True)
+    phi, pi = _rkn_fields(32, 9)
+    for s in (a, b):
+        s.upload(0, 0, phi)
+        s.upload(0, 1, pi)
+    for step in range(5):
+        a.advance(0, 1)
+        b.advance(0, 1)
+        _same(a, b, 0, 2, ("rkn graph", step))
+    assert b._get_info()["graph_replays"] == 4
+
+
+def _user_tableau(nodes):
+    # RKN4's tableau and weights with the given nodes.
+    return {"integrator.rkn.nodes": nodes,
+            "integrator.rkn.tableau": [0.0, 1. / 8., 0.0, 0.0, 0.5, 0.0],
+            "integrator.rkn.weights_bar_b": [1. / 6., 1. / 3., 0.0],
+            "integrator.rkn.weights_b": [1. / 6., 4. / 6., 1. / 6.]}
+
+
+def test_fused_rkn_user_tableau():
+    """A user tableau whose first node is 0 fuses (and equals RKN4, whose
+    tableau it restates); one whose first node is not 0 keeps separate
+    stages, since its first stage's state reads pi0 at neighbours."""
+    inp = real_scalar(N=32, order=2, integrator=20, lam=1.0, **_user_tableau([0.0, 0.5, 1.0]))
+    a, b = _pair("RealScalar", inp, shm.PREC_FP32C16, 2)
+    ref = shm.Simulation("RealScalar", real_scalar(N=32, order=2, integrator=21, lam=1.0),
+                         precision=shm.PREC_FP32C16)
+    ref.set_launch(fused=0)
+    phi, pi = _rkn_fields(32, 11)
+    for s in (a, b, ref):
+        s.upload(0, 0, phi)
+        s.upload(0, 1, pi)
+        s.advance(0, 3)
+    assert b.get_launch(0)["fused"] == 1
+    _same(a, b, 0, 2, "user tableau")
+    _same(ref, b, 0, 2, "user tableau = RKN4")
+    inp2 = real_scalar(N=32, order=2, integrator=20, lam=1.0,
+                       **_user_tableau([0.1, 0.5, 1.0]))
+    c = shm.Simulation("RealScalar", inp2, precision=shm.PREC_FP32C16)
+    c.set_launch(fused=1)
+    assert c.get_launch(0)["fused"] == 0
