@@ -217,13 +217,15 @@ int shm_stream_probe(shm_ctx *ctx, unsigned long long bytes, int reps,
  * persistent tile-table launch.  A field of -1 (tile_z: 0) is not set: the
  * tuned value of the level, else the built-in default, applies. */
 typedef struct shm_launch_t {
-    int variant;      /* tile shape: 0..2 = 64 x {4, 8, 16} cells per z plane */
+    int variant;      /* tile shape: 0..2 = 64 x {4, 8, 16} cells per z plane,
+                         one row per thread; 3 = 32 x 8 (two rows per
+                         wavefront); 4 = 64 x 8, two rows per thread */
     int tile_z;       /* z-chunk length of a tile (0: automatic) */
     int xcd_split;    /* 1: each XCD walks one contiguous range of tiles */
     int persistent;   /* 1: grid = resident capacity; 0: one tile per workgroup */
     int zc_minor;     /* tile order: 0 = z chunk outermost, 1 = between y and x */
-    int fused;        /* SSPRK3 on levels of periodic boxes: stages one and two
-                         in one launch (k1 kept on chip) */
+    int fused;        /* SSPRK3, or RKN with a first node 0, on levels of
+                         periodic boxes: stages one and two in one launch */
     int fused_variant;/* tile shape of the fused launch (0..2 as variant) */
 } shm_launch_t;
 

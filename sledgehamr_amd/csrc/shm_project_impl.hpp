@@ -135,6 +135,7 @@ void fill_variants(KernelTable &kt) {
     put(std::integral_constant<int, 1>{});
     put(std::integral_constant<int, 2>{});
     put(std::integral_constant<int, 3>{});
+    put(std::integral_constant<int, 4>{});
 }
 
 template <class Ops, Prec P, int R, bool GW, bool KO>
@@ -204,6 +205,7 @@ void fill_fused(KernelTable &kt) {
     put(std::integral_constant<int, 1>{});
     put(std::integral_constant<int, 2>{});
     put(std::integral_constant<int, 3>{});
+    put(std::integral_constant<int, 4>{});
 }
 
 template <class Ops, Prec P, int R>

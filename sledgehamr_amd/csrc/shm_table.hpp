@@ -42,8 +42,9 @@ enum StageKind : int {
 struct TileVariant {
     int nw, ry, tx;
 };
-constexpr int kNVar = 4;
-constexpr TileVariant kVariants[kNVar] = {{4, 1, 64}, {8, 1, 64}, {16, 1, 64}, {4, 1, 32}};
+constexpr int kNVar = 5;
+constexpr TileVariant kVariants[kNVar] = {
+    {4, 1, 64}, {8, 1, 64}, {16, 1, 64}, {4, 1, 32}, {4, 2, 64}};
 constexpr int kDefaultVariant = 1;
 
 using StageFn = hipError_t (*)(const StageArgs &, int grid, hipStream_t);

@@ -36,7 +36,7 @@ def _run(integrator, prec, variant):
 @pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
 def test_tile_variants_agree(integrator, prec):
     ref = None
-    for v in (1, 0, 2, 3):
+    for v in (1, 0, 2, 3, 4):
         sim, out = _run(integrator, prec, v)
         assert sim.get_launch(0)["variant"] == v
         if ref is None:
