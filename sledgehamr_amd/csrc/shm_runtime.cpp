@@ -2222,16 +2222,15 @@ void Runtime::autotune(int lev) {
         s3.c[1] = dt * 2. / 3.;
         s3.c[2] = dt / 4.;
     }
-    auto time_ms = [&](auto &&fn) {
+    auto time_ms = [&](auto &&fn, int reps = 3) {
         fn();  // warm-up
         SHM_HIP(hipEventRecord(ev0_, stream_));
-        const int reps = 3;
         for (int k = 0; k < reps; ++k) fn();
         SHM_HIP(hipEventRecord(ev1_, stream_));
         SHM_HIP(hipEventSynchronize(ev1_));
         float ms = 0;
         SHM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
-        return (double)ms;
+        return (double)ms / reps;
     };
 
     const LaunchCfg saved = launch;
@@ -2239,6 +2238,20 @@ void Runtime::autotune(int lev) {
     timing_on = false;
     double best = 1e30, best3 = 1e30;
     LaunchCfg bc, bc3;
+    // Screening times every shape briefly; the few fastest are then timed
+    // again at length, since among hundreds of short, noisy timings the
+    // smallest is often a mediocre shape that was timed lucky.
+    std::vector<std::pair<double, LaunchCfg>> seen, seen3;
+    auto run_all = [&] {
+        for (auto &r : runs) {
+            StageArgs c = r.second;
+            launch_stage(r.first, ls, c, lev);
+        }
+    };
+    auto run3 = [&] {
+        StageArgs c = s3;
+        launch_stage(kSsprk3, ls, c, lev);
+    };
     for (int v : vs)
         for (int z : zs)
             for (int p : ps)
@@ -2250,74 +2263,73 @@ void Runtime::autotune(int lev) {
                         launch.persistent = p;
                         launch.zc_minor = o;
                         launch.xcd_split = x;
-                        double ms = time_ms([&] {
-                            for (auto &r : runs) {
-                                StageArgs c = r.second;
-                                launch_stage(r.first, ls, c, lev);
-                            }
-                        });
+                        double ms = time_ms(run_all);
                         if (tune3) {
-                            const double m3 = time_ms([&] {
-                                StageArgs c = s3;
-                                launch_stage(kSsprk3, ls, c, lev);
-                            });
-                            if (m3 < best3) {
-                                best3 = m3;
-                                bc3 = launch;
-                            }
+                            const double m3 = time_ms(run3);
+                            seen3.push_back({m3, launch});
                             ms += m3;
                         }
-                        if (ms < best) {
-                            best = ms;
-                            bc = launch;
-                        }
+                        seen.push_back({ms, launch});
                     }
-    // SSPRK3 on a level of periodic boxes: one fused launch for stages one
-    // and two plus stage three at its best shape, against the three
-    // unfused launches at their best common shape.
-    bc.fused = 0;
-    bc.fused_variant = -1;
-    if (integrator_ == 10 && fused_possible(ls) && saved.fused != 0) {
-        const auto &row = kt_.fused12[(int)prec_][R_][gw_ ? 1 : 0];
-        for (int v = 0; v < kNVar; ++v) {
-            if (!row[v].fn) continue;
-            if (saved.fused_variant >= 0 && v != saved.fused_variant) continue;
-            launch = bc3;
-            launch.fused = 1;
-            launch.fused_variant = v;
-            launch.tz = 0;
-            const double ms =
-                time_ms([&] { launch_fused(ls, prm, prm, dt, lev); }) + best3;
-            if (ms < best) {
-                best = ms;
-                bc = bc3;
-                bc.fused = 1;
-                bc.fused_variant = v;
-            }
+    auto by_time = [](const std::pair<double, LaunchCfg> &x,
+                      const std::pair<double, LaunchCfg> &y) { return x.first < y.first; };
+    constexpr int kFinalists = 4, kLongReps = 12;
+    std::stable_sort(seen.begin(), seen.end(), by_time);
+    std::stable_sort(seen3.begin(), seen3.end(), by_time);
+    for (int i = 0; i < (int)seen.size() && i < kFinalists; ++i) {
+        launch = seen[i].second;
+        double ms = time_ms(run_all, kLongReps);
+        if (tune3) ms += time_ms(run3, kLongReps);
+        if (ms < best) {
+            best = ms;
+            bc = launch;
         }
     }
-    // RKN on a level of periodic boxes: one fused launch for stages one and
-    // two plus the remaining stages at the best unfused shape, against the
-    // whole unfused step.
-    if (integrator_ >= 20 && fused_possible(ls) && saved.fused != 0) {
+    for (int i = 0; i < (int)seen3.size() && i < kFinalists; ++i) {
+        launch = seen3[i].second;
+        const double m3 = time_ms(run3, kLongReps);
+        if (m3 < best3) {
+            best3 = m3;
+            bc3 = launch;
+        }
+    }
+    // On a level of periodic boxes: one fused launch for the first two
+    // stages plus the remaining stages (SSPRK3: stage three at its best
+    // shape; RKN: the later stages at the best unfused shape), against the
+    // unfused step.  Each candidate is timed exactly as a step runs it (one
+    // z chunk length for all its launches), with that chunk length or with
+    // whole columns, which spare the fused pass its 2R extra planes a chunk.
+    bc.fused = 0;
+    bc.fused_variant = -1;
+    if (fused_possible(ls) && saved.fused != 0 && (integrator_ == 10 || integrator_ >= 20)) {
         const FusedEntry *row = fused_row();
-        const LaunchCfg base = bc;
+        const LaunchCfg base = integrator_ == 10 ? bc3 : bc;
         for (int v = 0; v < kNVar; ++v) {
             if (!row[v].fn) continue;
             if (saved.fused_variant >= 0 && v != saved.fused_variant) continue;
-            launch = base;
-            launch.fused = 1;
-            launch.fused_variant = v;
-            const double ms = time_ms([&] {
-                launch_fused(ls, prm, prm, dt, lev);
-                for (size_t k = 2; k < runs.size(); ++k) {
-                    StageArgs c = runs[k].second;
-                    launch_stage(runs[k].first, ls, c, lev);
+            for (int tzc : {base.tz, 0}) {
+                launch = base;
+                launch.tz = tzc;
+                launch.fused = 1;
+                launch.fused_variant = v;
+                const double ms = time_ms(
+                    [&] {
+                        launch_fused(ls, prm, prm, dt, lev);
+                        if (integrator_ == 10) {
+                            run3();
+                        } else {
+                            for (size_t k = 2; k < runs.size(); ++k) {
+                                StageArgs c = runs[k].second;
+                                launch_stage(runs[k].first, ls, c, lev);
+                            }
+                        }
+                    },
+                    kLongReps);
+                if (ms < best) {
+                    best = ms;
+                    bc = launch;
                 }
-            });
-            if (ms < best) {
-                best = ms;
-                bc = launch;
+                if (base.tz == 0) break;  // one chunk length to try
             }
         }
     }
