@@ -87,6 +87,8 @@ template <> struct KindPol<kRknF1> { using T = PolRkn<1, true>; };
 template <> struct KindPol<kRknF2> { using T = PolRkn<2, true>; };
 template <> struct KindPol<kRknF3> { using T = PolRkn<3, true>; };
 template <> struct KindPol<kRknF4> { using T = PolRkn<4, true>; };
+template <> struct KindPol<kRkn0Y> { using T = PolRkn0Y; };
+template <> struct KindPol<kRkn1Y> { using T = PolRkn1Y; };
 template <> struct KindPol<kLeap1> { using T = PolLeapfrog<false>; };
 template <> struct KindPol<kLeap2> { using T = PolLeapfrog<true>; };
 
@@ -154,6 +156,8 @@ void fill_kinds(KernelTable &kt) {
     fill_variants<Ops, kRknF2, P, R, GW, KO>(kt);
     fill_variants<Ops, kRknF3, P, R, GW, KO>(kt);
     fill_variants<Ops, kRknF4, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn0Y, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn1Y, P, R, GW, KO>(kt);
     // Leapfrog writes u_new momenta in place while the stage reads them
     // point-wise; with dissipation the momenta are differenced, so that
     // combination is not instantiated.

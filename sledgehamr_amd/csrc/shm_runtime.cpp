@@ -588,6 +588,7 @@ void Runtime::init_from_inputs(const std::string &inputs,
     pp.query("shm.tile_order_zc_minor", launch.zc_minor);
     pp.query("shm.graphs", graphs);
     pp.query("shm.fused", launch.fused);
+    pp.query("shm.rkn_state", rkn_state_);
     if (launch.variant >= kNVar) throw Error(1, "shm.tile_variant out of range");
 
     // Project initialisation, then the reach of its cell functions.
@@ -725,6 +726,8 @@ void Runtime::ensure_arrays(LevelState &ls, int integrator) {
     } else if (integrator >= 20) {
         for (size_t i = 0; i + 1 < nodes_.size(); ++i)
             if (!ls.F[i]) ls.F[i].alloc(plane * npair * es);
+        if (nodes_.size() >= 3 && nodes_[0] == 0. && !ls.Y2)
+            ls.Y2.alloc(plane * npair * es);
     } else if (integrator == 11) {
         if (!ls.VH) ls.VH.alloc(plane * npair * 8);
     }
@@ -771,6 +774,11 @@ LaunchCfg Runtime::resolved_launch(int lev) const {
         if (row[v].fn) dv = v;
     c.fused_variant = pick(launch.fused_variant, t.fused_variant, dv);
     return c;
+}
+
+bool Runtime::rkn_y(int lev) const {
+    return rkn_state_ != 0 && integrator_ >= 20 && nodes_.size() >= 3 && nodes_[0] == 0. &&
+           !has_cf(lev);
 }
 
 const FusedEntry *Runtime::fused_row() const {
@@ -1398,6 +1406,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         launch_stage(kSsprk3, ls, c, lev);
     } else if (integrator_ >= 20) {
         const int s = (int)nodes_.size();
+        const bool ry = rkn_y(lev);
         int i0 = 0;
         if (resolved_launch(lev).fused && fused_possible(ls)) {
             // Stages one and two in one pass (boxes wrap on themselves, so
@@ -1420,7 +1429,18 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             }
             if (i < s - 1) {
                 a.fout = ls.F[i].get();
-                launch_stage(kRkn0 + i, ls, a, lev);
+                int kind = kRkn0 + i;
+                if (ry && i == 0) {
+                    // Also the second stage's field states, at its node.
+                    kind = kRkn0Y;
+                    a.y = ls.Y2.get();
+                    a.c[18] = dt * nodes_[1];
+                    a.c[19] = dt * dt * tableau_[1][0];
+                } else if (ry && i == 1) {
+                    kind = kRkn1Y;
+                    a.y = ls.Y2.get();
+                }
+                launch_stage(kind, ls, a, lev);
                 if (cf) {
                     // Stage i+1 reads phi0 + dt c_{i+1} pi0 + sum_j dt^2 a_{i+1,j} F_j.
                     CfArgs f = cf_args(lev, st[i + 1], false, true);
@@ -1441,6 +1461,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                     }
                 }
                 halo(ls, ls.F[i].get(), pairs);
+                if (ry && i == 0) halo(ls, ls.Y2.get(), pairs);
             } else {
                 a.c[6] = dt;
                 for (int j = 0; j < i; ++j) {
@@ -2157,7 +2178,14 @@ void Runtime::autotune(int lev) {
                 r.f[j] = ls.F[j].get();
             }
             r.fout = ls.F[i].get();
-            runs.push_back({kRkn0 + i, r});
+            int kind = kRkn0 + i;
+            if (rkn_y(lev) && i < 2) {
+                kind = i == 0 ? kRkn0Y : kRkn1Y;
+                r.y = ls.Y2.get();
+                r.c[18] = dt * nodes_[1];
+                r.c[19] = dt * dt * tableau_[1][0];
+            }
+            runs.push_back({kind, r});
         }
     } else {
         return;  // leapfrog kick 1 writes u_new fields; keep the default

@@ -97,6 +97,7 @@ struct LevelState {
     bool ghosts_ok[2] = {false, false};
     DevMem K[2];               // SSPRK3 k accumulators (k1, k2)
     DevMem F[5];               // RKN forces
+    DevMem Y2;                 // RKN second-stage field states (rkn_state)
     DevMem VH;                 // leapfrog half-step momenta (FP64)
     DevMem RHS;                // FillRhs target
     double t = -DBL_MAX;       // time of grid_new
@@ -435,6 +436,11 @@ class Runtime {
     int n_error_buf_ = 1, max_local_regrids_ = 10;
     double vt_strong_ = 1.05, vt_weak_ = 1.1;
     bool local_regrid_on_ = false;  // shm.local_regrid
+    int rkn_state_ = 1;             // shm.rkn_state
+    /** RKN steps of level lev whose first stage also writes the second
+     *  stage's field states (first node 0, three or more stages, no
+     *  coarse/fine ghosts). */
+    bool rkn_y(int lev) const;
     RegridState lr_;
 };
 

@@ -487,6 +487,56 @@ struct PolRkn {
     }
 };
 
+/** First RKN stage at node 0 (its state is u0) that also writes, per pair,
+ *  the second stage's field state y1 = phi0 + c[18] pi0 + c[19] F0 into a.y,
+ *  formed in the storage type exactly as PolRkn<1, false> forms it from the
+ *  stored phi0, pi0 and F0.  The second stage (PolRkn1Y) then reads y1 at
+ *  its stencil points instead of composing it from three arrays there. */
+struct PolRkn0Y : PolRkn<0, false> {
+    template <class CS, class T, int NK>
+    __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
+                                                    long long org, unsigned off,
+                                                    const double *rhs,
+                                                    const double *ctr,
+                                                    const typename T::S *e,
+                                                    const typename T::S (*kc)[NK]) {
+        using S = typename T::S;
+        PolRkn<0, false>::template store<CS, T, NK>(a, org, off, rhs, ctr, e, kc);
+#pragma unroll
+        for (int p = 0; p < CS::NPAIR; ++p) {
+            const S f0 = (S)rhs[CS::pair_mom(p)];
+            S y = (S)ctr[CS::pair_field(p)];
+            y += (S)a.c[18] * (S)ctr[CS::pair_mom(p)];
+            y += (S)a.c[19] * f0;
+            st<S>(a.y, org + p * a.cstride, off, y);
+        }
+    }
+};
+
+/** Second RKN stage reading the field states PolRkn0Y wrote; momenta stay
+ *  at pi0.  Writes F1 as PolRkn<1, false> does. */
+struct PolRkn1Y : PolRkn<1, false> {
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        using S = typename T::S;
+        const int p = CS::pair_of_field(n);
+        if (p < 0) return ld<S>(a.u0, org + n * a.cstride, off);
+        return ld<S>(a.y, org + p * a.cstride, off);
+    }
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *) {
+        return load_st<CS, T>(a, org, n, off);
+    }
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+        return load_keep<CS, T>(a, org, n, off, keep);
+    }
+};
+
 /** Leapfrog kick-drift-kick (leapfrog.cpp:41-56).
  *  Kick 1 (X = u0): vh = pi0 + c0*a(pi) (stored FP64), phi1 = phi0 + c1*vh.
  *  Kick 2 (X = u_new with phi1 and the momenta u_new held before the step,

@@ -32,6 +32,9 @@ enum StageKind : int {
     kRknF4,
     kLeap1,        // leapfrog kick + drift
     kLeap2,        // leapfrog second kick
+    kRkn0Y,        // RKN first stage at node 0 that also writes the second
+                   // stage's field states
+    kRkn1Y,        // RKN second stage reading them
     kNKind
 };
 

@@ -207,3 +207,34 @@ def test_fused_rkn_user_tableau():
     c = shm.Simulation("RealScalar", inp2, precision=shm.PREC_FP32C16)
     c.set_launch(fused=1)
     assert c.get_launch(0)["fused"] == 0
+
+
+@pytest.mark.parametrize("prec", ALLP)
+@pytest.mark.parametrize("integrator", [21, 22])
+@pytest.mark.parametrize("layout", ["periodic", "boxes", "ensemble"])
+def test_rkn_second_stage_state_equals_composed(layout, integrator, prec):
+    """The first stage writing the second stage's field states (shm.rkn_state
+    1, the default) against the second stage composing them from phi0, pi0
+    and F0 at every stencil point (0): identical bits, on a periodic box, on
+    a level chopped into boxes (same-level ghost copies of the states), and on
+    an ensemble; separate stage launches throughout."""
+    if layout == "ensemble":
+        boxes = ensemble_boxes(5, 16, 40, seed=21)
+        inp = ensemble_inputs(boxes, integrator=integrator)
+        fields = ensemble_fields(boxes, seed=8)
+    else:
+        extra = {"amr.max_grid_size": 16} if layout == "boxes" else {}
+        inp = real_scalar(N=32, order=2, integrator=integrator, lam=1.0, **extra)
+        boxes, fields = None, _rkn_fields(32, 3)
+    sims = []
+    for state in (0, 1):
+        s = shm.Simulation("RealScalar", dict(inp, **{"shm.rkn_state": state}),
+                           precision=prec)
+        s.set_launch(fused=0)
+        if boxes is not None:
+            s.set_boxes(0, boxes)
+        for c, f in enumerate(fields):
+            s.upload(0, c, f)
+        s.advance(0, 3)
+        sims.append(s)
+    _same(sims[0], sims[1], 0, 2, ("rkn state", layout, integrator, prec))
