@@ -134,6 +134,15 @@ void DevMem::reset() {
     bytes_ = 0;
 }
 
+/** Copies v into d, growing d only when it is too small. */
+template <class T>
+static void upload_into(DevMem &d, const std::vector<T> &v) {
+    const size_t b = std::max<size_t>(1, v.size()) * sizeof(T);
+    if (d.bytes() < b) d.alloc(b);
+    if (!v.empty())
+        SHM_HIP(hipMemcpy(d.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
 template <class T>
 static void upload_vec(DevMem &d, const std::vector<T> &v) {
     d.alloc(std::max<size_t>(1, v.size()) * sizeof(T));
@@ -1777,24 +1786,28 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     std::vector<double> ptag, pmod;
     sim_->SetParamsTagCellForRefinement(ptag, time, lev);
     if (S.shadow_hierarchy) sim_->SetParamsTruncationModifier(pmod, time, lev);
-    DevMem dcrit, dptag, dpmod, dmask, dcounts, dte;
-    upload_vec(dcrit, S.te_crit);
+    DevMem &dcrit = te_crit_, &dptag = te_ptag_, &dpmod = te_pmod_, &dcounts = te_counts_;
+    upload_into(dcrit, S.te_crit);
     ptag.resize(std::max<size_t>(ptag.size(), 1));
     pmod.resize(std::max<size_t>(pmod.size(), 1));
-    upload_vec(dptag, ptag);
-    upload_vec(dpmod, pmod);
+    upload_into(dptag, ptag);
+    upload_into(dpmod, pmod);
     // The mask stays with the level for the local regrid.
     if (flags & kTeTags) {
         if (fine.tags.bytes() != (size_t)std::max<long long>(1, fine.L.nvalid()))
             fine.tags.alloc(std::max<long long>(1, fine.L.nvalid()));
         fine.tags_t = -DBL_MAX;
-    } else {
-        dmask.alloc(std::max<long long>(1, fine.L.nvalid()));
+    } else if (te_mask_.bytes() < (size_t)std::max<long long>(1, fine.L.nvalid())) {
+        te_mask_.alloc(std::max<long long>(1, fine.L.nvalid()));
     }
-    DevMem &mask_mem = (flags & kTeTags) ? fine.tags : dmask;
-    dcounts.alloc(sizeof(unsigned long long) * (2 + ncomp_));
+    // Every valid cell's mask byte is written; the counters are summed
+    // atomically, so they start at zero.
+    DevMem &mask_mem = (flags & kTeTags) ? fine.tags : te_mask_;
+    upload_into(dcounts, std::vector<unsigned long long>(2 + ncomp_, 0ull));
     const long long total = fine.L.cprefix.back();
-    if (te_host && (flags & kTeCompute)) dte.alloc(sizeof(double) * total * ncomp_);
+    const bool want_te = te_host && (flags & kTeCompute);
+    if (want_te && te_out_.bytes() < sizeof(double) * total * ncomp_)
+        te_out_.alloc(sizeof(double) * total * ncomp_);
 
     TeTagArgs a{};
     a.fboxes = fine.L.dbox.as<BoxDesc>();
@@ -1810,7 +1823,7 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     a.nbox = (int)fine.L.boxes.size();
     a.total = total;
     a.ncomp = ncomp_;
-    a.te_out = dte ? dte.as<double>() : nullptr;
+    a.te_out = want_te ? te_out_.as<double>() : nullptr;
     a.mask = mask_mem.as<uint8_t>();
     a.counts = dcounts.as<unsigned long long>();
     a.te_crit = dcrit.as<double>();
@@ -1841,8 +1854,8 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     std::vector<unsigned long long> hc(2 + ncomp_);
     SHM_HIP(hipMemcpyAsync(hc.data(), dcounts.get(), hc.size() * 8,
                            hipMemcpyDeviceToHost, stream_));
-    if (te_host && dte)
-        SHM_HIP(hipMemcpyAsync(te_host, dte.get(), sizeof(double) * total * ncomp_,
+    if (want_te)
+        SHM_HIP(hipMemcpyAsync(te_host, te_out_.get(), sizeof(double) * total * ncomp_,
                                hipMemcpyDeviceToHost, stream_));
     SHM_HIP(hipStreamSynchronize(stream_));
     if (counts)

@@ -437,6 +437,9 @@ class Runtime {
     double vt_strong_ = 1.05, vt_weak_ = 1.1;
     bool local_regrid_on_ = false;  // shm.local_regrid
     int rkn_state_ = 1;             // shm.rkn_state
+    // Tagging's inputs and outputs, kept between calls (each call ends with
+    // a stream synchronisation) so that tagging allocates nothing per call.
+    DevMem te_crit_, te_ptag_, te_pmod_, te_mask_, te_counts_, te_out_;
     /** RKN steps of level lev whose first stage also writes the second
      *  stage's field states (first node 0, three or more stages, no
      *  coarse/fine ghosts). */
