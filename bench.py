@@ -411,8 +411,8 @@ def main():
     if launch is not None and len(launch) not in (5, 7):
         ap.error("--launch takes five or seven integers")
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs each level on a "
-              "single MI355X (multi-GPU sharding is design-only); use --gpus 1",
+        print("bench.py measures one GPU: each level runs on a single MI355X "
+              "(sharding a level across GPUs is not implemented); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
