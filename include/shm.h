@@ -38,7 +38,10 @@ enum shm_precision {
     SHM_PREC_INPUTS = -1,     /* take shm.precision from the inputs file */
     SHM_PREC_FP64 = 0,        /* FP64 storage and arithmetic */
     SHM_PREC_FP32C = 1,       /* FP32 storage + FP32 remainder (compensated) */
-    SHM_PREC_FP32C16 = 2      /* FP32 storage + bf16 remainder (compensated) */
+    SHM_PREC_FP32C16 = 2,     /* FP32 storage + bf16 remainder (compensated) */
+    SHM_PREC_FP32CS = 3       /* stored as FP32C16; stage states and stencils
+                                 formed in FP64 from value + remainder
+                                 (compensated stencils) */
 };
 
 enum shm_te_flags {
@@ -105,7 +108,13 @@ int shm_download_rhs(shm_ctx *ctx, int lev, int comp, double *data);
  * integrator (integrator.type 10, 11, 20, 21, 22). */
 int shm_advance_level(shm_ctx *ctx, int lev);
 
-/* TimeStepper::Advance(0) repeated ncoarse times (time_stepper.cpp:90-133). */
+/* TimeStepper::Advance(0) repeated ncoarse times (time_stepper.cpp:90-133).
+ * With shm.local_regrid = 1, a scheduled regrid whose local regrid asks for
+ * a global one (LocalRegrid::DoAttemptRegrid returning false) ends the call
+ * with SHM_ERR_UNSUPPORTED right after that regrid: the caller runs its
+ * global regrid (AmrCore::regrid) and hands the new box lists to
+ * shm_remake_level, then continues.  With shm.keep_layout_on_global_regrid
+ * = 1 the layout stays instead and the evolve goes on. */
 int shm_evolve(shm_ctx *ctx, int ncoarse);
 
 /* Counters of the most recent truncation-error tagging that shm_evolve ran
@@ -131,6 +140,20 @@ int shm_te_tag(shm_ctx *ctx, int lev, int flags, unsigned char *mask,
 int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
                     int projection_type, double *bins, double *kvals,
                     int *nout);
+
+/* GravitationalWaves::ComputeSpectrum(file, modifier) with a project
+ * spectrum modifier (GravitationalWavesSpectrumModifier, gravitational_waves.h:
+ * 79-92, called at gravitational_waves.cpp:69 and :96): modifier is the
+ * position of the struct in the project's SLEDGEHAMR_GW_SPECTRUM_MODIFIERS
+ * list (-1: the default du_ij, unmodified, as shm_gw_spectrum); params /
+ * nparams are its per-call parameters (the reference modifier's constructor
+ * arguments).  Other arguments as shm_gw_spectrum. */
+int shm_gw_spectrum_mod(shm_ctx *ctx, int mode, int nbins, int zero_padding,
+                        int projection_type, int modifier, const double *params,
+                        int nparams, double *bins, double *kvals, int *nout);
+/* The project's GW spectrum modifiers: their number, and the name of id. */
+int shm_gw_modifiers(shm_ctx *ctx, int *count);
+int shm_gw_modifier_name(shm_ctx *ctx, int id, char *buf, int len);
 
 /* LevelSynchronizer::FillCoarsePatch (level_synchronizer.cpp:56-88): every
  * cell of level lev (lev >= 1), ghosts included, interpolated from level

@@ -254,6 +254,71 @@ struct NoOutputFns {
 }  // namespace shm
 inline shm::NoOutputFns shm_select_output(...) { return {}; }
 
+/** Gravitational-wave spectrum modifiers of a project (the reference's
+ *  GravitationalWavesSpectrumModifier, gravitational_waves.h:79-92).  Each
+ *  listed struct M provides
+ *    static void SelectComponents(int components[6]);   // host
+ *    AMREX_GPU_HOST_DEVICE static void FourierSpaceModification(
+ *        double re[6], double im[6], int kx, int ky, int kz, double dk,
+ *        int dimN, const double *params);
+ *  SelectComponents names the six tensor components transformed, in the
+ *  order xx, xy, xz, yy, yz, zz (default: du_xx, du_xy, du_xz, du_yy, du_yz,
+ *  du_zz).  FourierSpaceModification is the reference's
+ *  FourierSpaceModifications applied to one mode of the x-halved R2C
+ *  spectrum, where the reference loops over the spectral MultiFabs: the six
+ *  amplitudes of the mode, its signed wave numbers (k = a >= dimN/2 ? a -
+ *  dimN : a per axis, as the reference's modifiers form them), dk = 2 pi /
+ *  L of the (zero-padded) transform, and per-call parameters (the
+ *  reference's modifier constructor arguments).  A modifier's id is its
+ *  position in the list.  Place after the structs, before the project
+ *  class. */
+#define SHM_GWM_NAME(m) #m,
+#define SHM_GWM_SELECT(m) \
+    if (id == c++) {      \
+        m::SelectComponents(comps); \
+        return true;      \
+    }
+#define SHM_GWM_MODIFY(m) \
+    if (id == c++) {      \
+        m::FourierSpaceModification(re, im, kx, ky, kz, dk, dimN, p); \
+        return;           \
+    }
+#define SLEDGEHAMR_GW_SPECTRUM_MODIFIERS(...)                                  \
+    struct ShmGwModifiers {                                                    \
+        static constexpr int count = SHM_PP_NARGS(__VA_ARGS__);                \
+        static const char *name(int id) {                                      \
+            static const char *const n[] = {                                   \
+                SHM_PP_FOR_EACH(SHM_GWM_NAME, __VA_ARGS__)};                   \
+            return id >= 0 && id < count ? n[id] : nullptr;                    \
+        }                                                                      \
+        static bool select(int id, int comps[6]) {                             \
+            int c = 0;                                                         \
+            SHM_PP_FOR_EACH(SHM_GWM_SELECT, __VA_ARGS__)                       \
+            return false;                                                      \
+        }                                                                      \
+        AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE static void                   \
+        modify(int id, double re[6], double im[6], int kx, int ky, int kz,     \
+               double dk, int dimN, const double *p) {                         \
+            int c = 0;                                                         \
+            SHM_PP_FOR_EACH(SHM_GWM_MODIFY, __VA_ARGS__)                       \
+        }                                                                      \
+    };                                                                         \
+    inline ShmGwModifiers shm_select_gw_modifiers(int) { return {}; }
+
+/** Projects without SLEDGEHAMR_GW_SPECTRUM_MODIFIERS have only the default
+ *  (du_ij, unmodified). */
+namespace shm {
+struct NoGwModifiers {
+    static constexpr int count = 0;
+    static const char *name(int) { return nullptr; }
+    static bool select(int, int *) { return false; }
+    AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE static void modify(int, double *, double *,
+                                                                int, int, int, double,
+                                                                int, const double *) {}
+};
+}  // namespace shm
+inline shm::NoGwModifiers shm_select_gw_modifiers(...) { return {}; }
+
 /** Binds the project's device functions to the MI355X kernel table.
  *  ShmOps is the compile-time description of the project that the fused
  *  stage, tagging and probe kernels are instantiated with. */
@@ -263,6 +328,7 @@ inline shm::NoOutputFns shm_select_output(...) { return {}; }
         static constexpr int kNFields = _Scalar::NScalarFields;                \
         static constexpr int kNGw = Gw::NGwScalars;                            \
         using Output = decltype(shm_select_output(0));                         \
+        using GwModifiers = decltype(shm_select_gw_modifiers(0));              \
         AMREX_GPU_DEVICE AMREX_FORCE_INLINE static void                        \
         rhs(const amrex::Array4<double> &r,                                    \
             const amrex::Array4<const double> &s, int i, int j, int k,         \

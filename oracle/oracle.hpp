@@ -374,6 +374,7 @@ struct Config {
     int n_error_buf = 1, max_local_regrids = 10;
     double vt_strong = 1.05, vt_weak = 1.1;
     bool local_regrid = false;  // shm.local_regrid: regrid at scheduled times
+    bool keep_layout = false;   // shm.keep_layout_on_global_regrid
     int interp_type = 2;  // amr.interpolation_type: 0 piecewise constant,
                           // 1 limited linear, 2 quadratic, 4 quartic
 };
@@ -500,6 +501,7 @@ class Sim {
         pp.query("amr.volume_threshold_strong", cfg.vt_strong);
         pp.query("amr.volume_threshold_weak", cfg.vt_weak);
         pp.query("shm.local_regrid", cfg.local_regrid);
+        pp.query("shm.keep_layout_on_global_regrid", cfg.keep_layout);
         for (int lev = 0; lev <= cfg.max_level; ++lev) {
             dimN.push_back(cfg.N0 * (1 << lev));
             dx.push_back(cfg.L / (double)dimN[lev]);
@@ -509,6 +511,17 @@ class Sim {
         grid_old.resize(cfg.max_level + 1);
         lr.no_local.assign(cfg.max_level + 1, 0);
         lr.do_global.assign(cfg.max_level + 1, 0);
+        // MultiFab::norm0 of grid_new[lev] over every component (valid cells).
+        prj.level_norm0_ = [this](int lev) {
+            std::vector<double> m(nc, 0.);
+            for (auto &f : grid_new[lev].fabs)
+                for (int n = 0; n < nc; ++n)
+                    for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
+                        for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
+                            for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i)
+                                m[n] = std::max(m[n], std::fabs(f.at(i, j, k, n)));
+            return m;
+        };
         prj.Init();
         int mgs = cfg.N0;
         pp.query("amr.max_grid_size", mgs);
@@ -1013,11 +1026,15 @@ class Sim {
 
     /** TagWithTruncationCpu (macros.h:489-566) executed literally: per tile
      *  and cell in k, j, i order, CLEAR own cell, user tag, TE block SET.
-     *  mask holds one byte per fine valid cell (box-major, x fastest). */
+     *  mask holds one byte per fine valid cell (box-major, x fastest).
+     *  User criteria may read neighbours: the ghost cells are filled at the
+     *  level's time first (the reference reads whatever its ghosts hold from
+     *  the last fill; here that is defined as FillPatch at the tag time). */
     void tag(int lev, const std::vector<double> &te, bool with_te,
              std::vector<uint8_t> &mask, std::vector<long long> &counts) {
         MultiFab &st = grid_new[lev];
         const double time = st.t;
+        fill_patch(st, lev, time);
         std::vector<double> ptag, pmod;
         prj.SetParamsTagCellForRefinement(ptag, time, lev);
         if (cfg.shadow_hierarchy) prj.SetParamsTruncationModifier(pmod, time, lev);
@@ -1092,10 +1109,20 @@ class Sim {
     /** GravitationalWaves::ComputeSpectrum, CPU branch.  mode 0 unbinned
      *  (lower_bound over unique |k|^2), 1 rounded |k|, 2 log bins (new). */
     void gw_spectrum(int mode, int nbins, int zp, int proj, std::vector<double> &spec,
-                     std::vector<double> &kv) {
+                     std::vector<double> &kv, int modifier = -1,
+                     const std::vector<double> &mparams = {}) {
         const int N = cfg.N0 * zp;
         const int NS = Ops::kNScalars;
-        const int comps[6] = {NS + 6, NS + 9, NS + 10, NS + 7, NS + 11, NS + 8};
+        // GravitationalWavesSpectrumModifier::SelectComponents
+        // (gravitational_waves.h:79-92); default du_xx, du_xy, du_xz, du_yy,
+        // du_yz, du_zz.
+        int comps[6] = {NS + 6, NS + 9, NS + 10, NS + 7, NS + 11, NS + 8};
+        using Mods = typename Ops::GwModifiers;
+        if (modifier >= 0 && !Mods::select(modifier, comps))
+            throw std::runtime_error("no such GW spectrum modifier");
+        const double dk = 2. * M_PI / (cfg.L * zp);
+        std::vector<double> mp = mparams;
+        mp.resize(std::max<size_t>(mp.size(), 1), 0.);
         const int mat[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
         std::vector<std::vector<std::complex<double>>> u(6);
         for (int c = 0; c < 6; ++c) {
@@ -1148,14 +1175,22 @@ class Sim {
                         index = std::max(0L, std::min((long)nbins - 1, index));
                     }
                     const size_t e = (size_t)(((long long)c * N + b) * NH + a);
+                    double re[6], im[6];
+                    for (int q = 0; q < 6; ++q) {
+                        re[q] = u[q][e].real();
+                        im[q] = u[q][e].imag();
+                    }
+                    // FourierSpaceModifications (gravitational_waves.cpp:96),
+                    // mode by mode.
+                    if (modifier >= 0) Mods::modify(modifier, re, im, li, lj, lk, dk, N, mp.data());
                     double running = 0;
                     for (int i = 0; i < 3; ++i)
                         for (int j = 0; j < 3; ++j)
                             for (int l = 0; l < 3; ++l)
                                 for (int m = 0; m < 3; ++m)
                                     running += gw_lambda(i, j, l, m, abc, itk.data()) * multpl *
-                                               (u[mat[i][j]][e].real() * u[mat[l][m]][e].real() +
-                                                u[mat[i][j]][e].imag() * u[mat[l][m]][e].imag());
+                                               (re[mat[i][j]] * re[mat[l][m]] +
+                                                im[mat[i][j]] * im[mat[l][m]]);
                     if (index >= 0 && index < kmax) spec[index] += running;
                 }
         const double n6 = std::pow((double)N, 6);
@@ -1725,14 +1760,17 @@ class Sim {
     void do_regrid_if_scheduled(int lev) {
         const double time = grid_new[lev].t;
         if (!do_regrid(lev, time)) return;
-        if (cfg.local_regrid && local_regrid(lev) == 0) {
-            // The global regrid (AMReX clustering) keeps the layout here.
-            ++lr.global_wanted;
-            did_global_regrid(lev);
-        }
+        const bool global = cfg.local_regrid && local_regrid(lev) == 0;
         for (int l = lev; l <= finest(); ++l) last_regrid[l] = time;
         const int id = find_sched(time);
         if (id >= 0) sched.erase(sched.begin() + id);
+        if (global) {
+            // A global regrid (AMReX clustering) is not restated: stop, or
+            // keep the layout with the counters as they are when asked to.
+            ++lr.global_wanted;
+            if (!cfg.keep_layout)
+                throw std::runtime_error("a global regrid is wanted");
+        }
     }
 
     void step(int lev) {

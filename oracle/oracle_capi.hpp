@@ -342,6 +342,21 @@
             *nout = (int)b.size();                                             \
         });                                                                    \
     }                                                                          \
+    ORC_EXPORT int orc_gw_spectrum_mod(void *h, int mode, int nbins, int zp,   \
+                                       int proj, int modifier,                 \
+                                       const double *mp, int nmp,              \
+                                       double *bins, double *k, int *nout) {   \
+        return orc_guard([&] {                                                 \
+            std::vector<double> b, kv, pv(mp, mp + (mp ? nmp : 0));            \
+            static_cast<OrcSim *>(h)->gw_spectrum(mode, nbins, zp, proj, b, kv, \
+                                                  modifier, pv);               \
+            if (bins && *nout >= (int)b.size()) {                              \
+                std::memcpy(bins, b.data(), b.size() * 8);                     \
+                std::memcpy(k, kv.data(), kv.size() * 8);                      \
+            }                                                                  \
+            *nout = (int)b.size();                                             \
+        });                                                                    \
+    }                                                                          \
     ORC_EXPORT int orc_init_state(void *h, int lev, int kind,                  \
                                   unsigned long long seed, const double *p,    \
                                   int np) {                                    \

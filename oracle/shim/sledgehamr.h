@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <functional>
 #include <vector>
 
 #define AMREX_GPU_DEVICE
@@ -114,7 +115,12 @@ class Sledgehamr {
     virtual void SetParamsProjections(std::vector<double> &, const double) {}
     virtual bool CreateLevelIf(const int, const double) { return true; }
     virtual void BeforeTimestep(const double) {}
+    /** grid_new[lev].norm0 over every component (set by the oracle). */
+    std::vector<double> LevelNorm0(const int lev) const {
+        return level_norm0_ ? level_norm0_(lev) : std::vector<double>();
+    }
     std::vector<ScalarField *> scalar_fields;
+    std::function<std::vector<double>(int)> level_norm0_;
 };
 
 namespace utils {
@@ -298,6 +304,45 @@ struct NoOutputFns {
 }  // namespace orc
 inline orc::NoOutputFns orc_select_output(...) { return {}; }
 
+/** GW spectrum modifiers (the product header's
+ *  SLEDGEHAMR_GW_SPECTRUM_MODIFIERS): SelectComponents and the per-mode
+ *  FourierSpaceModification of each listed struct, by list position. */
+#define ORC_GWM_SELECT(m) \
+    if (id == c++) {      \
+        m::SelectComponents(comps); \
+        return true;      \
+    }
+#define ORC_GWM_MODIFY(m) \
+    if (id == c++) {      \
+        m::FourierSpaceModification(re, im, kx, ky, kz, dk, dimN, p); \
+        return;           \
+    }
+#define SLEDGEHAMR_GW_SPECTRUM_MODIFIERS(...)                                  \
+    struct OrcGwModifiers {                                                    \
+        static constexpr int count = ORC_NARGS(__VA_ARGS__);                   \
+        static bool select(int id, int comps[6]) {                             \
+            int c = 0;                                                         \
+            ORC_FE(ORC_GWM_SELECT, __VA_ARGS__)                                \
+            return false;                                                      \
+        }                                                                      \
+        static void modify(int id, double re[6], double im[6], int kx, int ky, \
+                           int kz, double dk, int dimN, const double *p) {     \
+            int c = 0;                                                         \
+            ORC_FE(ORC_GWM_MODIFY, __VA_ARGS__)                                \
+        }                                                                      \
+    };                                                                         \
+    inline OrcGwModifiers orc_select_gw_modifiers(int) { return {}; }
+
+namespace orc {
+struct NoGwModifiers {
+    static constexpr int count = 0;
+    static bool select(int, int *) { return false; }
+    static void modify(int, double *, double *, int, int, int, double, int,
+                       const double *) {}
+};
+}  // namespace orc
+inline orc::NoGwModifiers orc_select_gw_modifiers(...) { return {}; }
+
 /** The oracle reaches the project's cell functions through OrcOps. */
 #define SLEDGEHAMR_INITIALIZE_PROJECT(prj)                                     \
     struct OrcOps {                                                            \
@@ -305,6 +350,7 @@ inline orc::NoOutputFns orc_select_output(...) { return {}; }
         static constexpr int kNFields = _Scalar::NScalarFields;                \
         static constexpr int kNGw = Gw::NGwScalars;                            \
         using Output = decltype(orc_select_output(0));                         \
+        using GwModifiers = decltype(orc_select_gw_modifiers(0));              \
         static void rhs(const amrex::Array4<double> &r,                        \
                         const amrex::Array4<const double> &s, int i, int j,    \
                         int k, int lev, double t, double dt, double dx,        \

@@ -12,14 +12,14 @@ import os
 import numpy as np
 
 __all__ = ["Simulation", "library_path", "PROJECTS", "ShmError",
-           "PREC_FP64", "PREC_FP32C", "PREC_FP32C16", "TE_COMPUTE",
+           "PREC_FP64", "PREC_FP32C", "PREC_FP32C16", "PREC_FP32CS", "TE_COMPUTE",
            "TE_AVGDOWN", "TE_TAGS"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.environ.get("SHM_LIBDIR") or os.path.join(_HERE, "lib")
 PROJECTS = ("RealScalar", "ComplexScalar", "PhaseTransitionGW")
 
-PREC_INPUTS, PREC_FP64, PREC_FP32C, PREC_FP32C16 = -1, 0, 1, 2
+PREC_INPUTS, PREC_FP64, PREC_FP32C, PREC_FP32C16, PREC_FP32CS = -1, 0, 1, 2, 3
 TE_COMPUTE, TE_AVGDOWN, TE_TAGS = 1, 2, 4
 
 # Exported symbols declared in include/shm.h (checked by the CPU tests).
@@ -29,6 +29,7 @@ EXPORTS = (
     "shm_level_info", "shm_set_level_time", "shm_upload", "shm_download",
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
+    "shm_gw_spectrum_mod", "shm_gw_modifiers", "shm_gw_modifier_name",
     "shm_output_functions", "shm_output_function_name", "shm_spectrum", "shm_projection",
     "shm_upload_array", "shm_fill_coarse_patch",
     "shm_regrid_select", "shm_add_boxes", "shm_remake_level", "shm_local_regrid",
@@ -103,6 +104,9 @@ def load_library(project: str) -> C.CDLL:
         "shm_level_tag_counts": (i, [vp, i, llp]),
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
+        "shm_gw_spectrum_mod": (i, [vp, i, i, i, i, i, dp, i, dp, dp, ip]),
+        "shm_gw_modifiers": (i, [vp, ip]),
+        "shm_gw_modifier_name": (i, [vp, i, C.c_char_p, i]),
         "shm_upload_array": (i, [vp, i, i, dp, i]),
         "shm_fill_coarse_patch": (i, [vp, i]),
         "shm_regrid_select": (i, [vp, i, C.POINTER(C.c_ubyte), ip, i, ip, dp]),
@@ -281,17 +285,42 @@ class Simulation:
         return mask, counts, te
 
     def gw_spectrum(self, mode: int = 0, nbins: int = 256, zero_padding: int = 1,
-                    projection_type: int = 2):
+                    projection_type: int = 2, modifier: int = -1, params=()):
+        """GW spectrum (bins, k values); modifier: id of a project spectrum
+        modifier (-1: du_ij, unmodified) with its parameters."""
         n = C.c_int(0)
-        self._check(self.lib.shm_gw_spectrum(self._ctx, mode, nbins, zero_padding,
-                                             projection_type, None, None,
-                                             C.byref(n)))
+        if modifier < 0 and not len(params):
+            self._check(self.lib.shm_gw_spectrum(self._ctx, mode, nbins, zero_padding,
+                                                 projection_type, None, None,
+                                                 C.byref(n)))
+            bins = np.zeros(n.value)
+            k = np.zeros(n.value)
+            self._check(self.lib.shm_gw_spectrum(self._ctx, mode, nbins, zero_padding,
+                                                 projection_type, _dptr(bins),
+                                                 _dptr(k), C.byref(n)))
+            return bins, k
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        pp = _dptr(p) if p.size else None
+        self._check(self.lib.shm_gw_spectrum_mod(self._ctx, mode, nbins, zero_padding,
+                                                 projection_type, modifier, pp, p.size,
+                                                 None, None, C.byref(n)))
         bins = np.zeros(n.value)
         k = np.zeros(n.value)
-        self._check(self.lib.shm_gw_spectrum(self._ctx, mode, nbins, zero_padding,
-                                             projection_type, _dptr(bins),
-                                             _dptr(k), C.byref(n)))
+        self._check(self.lib.shm_gw_spectrum_mod(self._ctx, mode, nbins, zero_padding,
+                                                 projection_type, modifier, pp, p.size,
+                                                 _dptr(bins), _dptr(k), C.byref(n)))
         return bins, k
+
+    def gw_modifiers(self) -> list:
+        """Names of the project's GW spectrum modifiers, by id."""
+        n = C.c_int(0)
+        self._check(self.lib.shm_gw_modifiers(self._ctx, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            buf = C.create_string_buffer(256)
+            self._check(self.lib.shm_gw_modifier_name(self._ctx, i, buf, 256))
+            out.append(buf.value.decode())
+        return out
 
     def upload_array(self, lev: int, comp: int, data, layout: int) -> None:
         """Upload from the reference's array orders (1: whole level,

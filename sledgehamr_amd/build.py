@@ -79,7 +79,7 @@ def _project_units(p: Project) -> list[tuple[str, list[str]]]:
     unit with the tagging/probe kernels and the project registration."""
     os.makedirs(BUILD, exist_ok=True)
     units = [("main", [])]
-    for prec in (0, 1, 2):
+    for prec in (0, 1, 2, 3):
         for r in (1, 2, 3):
             if (p.r_mask >> r) & 1:
                 units.append((f"p{prec}_r{r}",

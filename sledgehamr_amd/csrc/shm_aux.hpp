@@ -367,6 +367,14 @@ template <Prec P, int MODE, int INTERP>
 __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
     using T = PrecTraits<P>;
     using S = typename T::S;
+    // Stage states are composed in the tile type V from u0's tile value (the
+    // leading word for FP32 tiles, hi + lo for FP64 tiles) and coefficients
+    // in V: the ghost values below compose to the interpolated state the
+    // same way.
+    using V = typename T::V;
+    auto u0_at = [&](long long e) -> double {
+        return (double)ld_tile<T>(a.u0, a.u0lo, e, 0u);
+    };
     // Time-weighted coarse values of the tile's brick: its parents grown by
     // the stencil reach, x fastest.
     extern __shared__ double brick[];
@@ -431,16 +439,15 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
             }
         } else if constexpr (MODE == kCfK) {
             const long long e = (long long)comp * a.fcstride + fe;
-            const double u0 = (double)ld<S>(a.u0, e, 0u);
-            st<S>(a.dst, e, 0u, (S)((v - u0) / (double)(S)a.c0));
+            const double u0 = u0_at(e);
+            st<S>(a.dst, e, 0u, (S)((v - u0) / (double)(V)a.c0));
         } else {
-            double yv = (double)ld<S>(a.u0, (long long)comp * a.fcstride + fe, 0u) +
-                        (double)(S)a.c0 *
-                            (double)ld<S>(a.u0, (long long)a.mom[n] * a.fcstride + fe, 0u);
+            double yv = u0_at((long long)comp * a.fcstride + fe) +
+                        (double)(V)a.c0 * u0_at((long long)a.mom[n] * a.fcstride + fe);
             const long long pe = (long long)a.pair[n] * a.fcstride + fe;
             for (int j = 0; j < a.m; ++j)
-                yv += (double)(S)a.cf[j] * (double)ld<S>(a.f[j], pe, 0u);
-            st<S>(a.dst, pe, 0u, (S)((v - yv) / (double)(S)a.cf[a.m]));
+                yv += (double)(V)a.cf[j] * (double)ld<S>(a.f[j], pe, 0u);
+            st<S>(a.dst, pe, 0u, (S)((v - yv) / (double)(V)a.cf[a.m]));
         }
     };
     for (int n = 0; n < a.ncomp; ++n) {
@@ -563,21 +570,30 @@ __global__ void __launch_bounds__(256) cf_fill_kernel(const CfArgs a) {
                         }
                     }
                 };
-                // u0's leading word at t0, then k1 and k2 such that u0 + dt k1
-                // and u0 + dt/4 k2 are the states at t0 + dt and t0 + dt/2,
-                // as the separate state and k fills write them.
+                // u0 at t0 (its leading word; with FP64 tiles of stored
+                // pairs hi and lo), then k1 and k2 such that u0 + dt k1 and
+                // u0 + dt/4 k2 are the states at t0 + dt and t0 + dt/2, as
+                // the separate state and k fills write them.
                 for (int z = c0[2]; z <= c1[2]; ++z)
                     for (int y = c0[1]; y <= c1[1]; ++y)
                         for (int x = c0[0]; x <= c1[0]; ++x) {
                             const long long e = (long long)comp * a.fcstride + fine_elem(x, y, z);
-                            const S u0s = (S)value(brick, x, y, z);
-                            st<S>(a.dst, e, 0u, u0s);
+                            const double u0v = value(brick, x, y, z);
+                            double u0t;  // u0's tile value as the stage kernels read it
+                            if constexpr (T::comp && T::exact) {
+                                store_state_at<T>(a.dst, a.dstlo, e, u0v);
+                                u0t = pair_value<T>(u0v);
+                            } else {
+                                const S u0s = (S)u0v;
+                                st<S>(a.dst, e, 0u, u0s);
+                                u0t = (double)u0s;
+                            }
                             st<S>(a.kd[0], e, 0u,
-                                  (S)((value(brick + nb, x, y, z) - (double)u0s) /
-                                      (double)(S)a.cf[0]));
+                                  (S)((value(brick + nb, x, y, z) - u0t) /
+                                      (double)(V)a.cf[0]));
                             st<S>(a.kd[1], e, 0u,
-                                  (S)((value(brick + 2 * nb, x, y, z) - (double)u0s) /
-                                      (double)(S)a.cf[1]));
+                                  (S)((value(brick + 2 * nb, x, y, z) - u0t) /
+                                      (double)(V)a.cf[1]));
                         }
             } else if constexpr (INTERP == 4) {
                 // Tensor product of the conservative quartic child weights;
@@ -714,6 +730,9 @@ struct BinArgs {
     int nbins;
     double log_kmax;         // mode 2: ln(sqrt(3) N / 2)
     double *bins;            // FP64 accumulators
+    int modifier;            // project spectrum modifier id, -1: none
+    const double *mparams;   // its parameters
+    double dk;               // 2 pi / L of the (zero-padded) transform
 };
 
 /** sum_{ijlm} Lambda_{ij,lm}(k) A_ij A_lm = Tr(APAP) - Tr(PA)^2 / 2 for a
@@ -736,7 +755,7 @@ __device__ AMREX_FORCE_INLINE double lambda_contract(const double P[3][3],
     return tr2 - 0.5 * tr * tr;
 }
 
-template <int LDSBINS>
+template <class Mods, int LDSBINS>
 __global__ void __launch_bounds__(256) gw_bin_kernel(const BinArgs a) {
     __shared__ double sb[LDSBINS > 0 ? LDSBINS : 1];
     const bool use_lds = LDSBINS > 0 && a.nbins <= LDSBINS;
@@ -778,7 +797,21 @@ __global__ void __launch_bounds__(256) gw_bin_kernel(const BinArgs a) {
         double2 v[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) v[c] = a.u[c][t];
-        // Component order du_xx, du_xy, du_xz, du_yy, du_yz, du_zz.
+        if constexpr (Mods::count > 0) {
+            // The project's FourierSpaceModification of this mode.
+            if (a.modifier >= 0) {
+                double re[6], im[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    re[c] = v[c].x;
+                    im[c] = v[c].y;
+                }
+                Mods::modify(a.modifier, re, im, li, lj, lk, a.dk, N, a.mparams);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) v[c] = make_double2(re[c], im[c]);
+            }
+        }
+        // Component order xx, xy, xz, yy, yz, zz.
         const double Ar[3][3] = {{v[0].x, v[1].x, v[2].x},
                                  {v[1].x, v[3].x, v[4].x},
                                  {v[2].x, v[4].x, v[5].x}};
@@ -1155,6 +1188,54 @@ __global__ void __launch_bounds__(256) valid_io_kernel(const ValidIoArgs a) {
             store_state_at<T>(a.hi, a.lo, e, a.dense[t]);
         else
             a.dense[t] = load_state_at<T>(a.hi, a.lo, e);
+    }
+}
+
+/** Maximum |value| per component over the valid cells of a level (the
+ *  reference's MultiFab::norm0 as projects call it on grid_new).  Non-negative
+ *  doubles order like their bit patterns, so each wave folds its maximum
+ *  and one lane takes an atomic maximum of the bits per component. */
+struct Norm0Args {
+    const BoxDesc *boxes;
+    const long long *vprefix;
+    int nbox;
+    long long total;
+    long long cstride;
+    int ncomp;
+    const void *hi, *lo;
+    unsigned long long *out;  // ncomp bit patterns, zero before the launch
+};
+
+template <Prec P>
+__global__ void __launch_bounds__(256) norm0_kernel(const Norm0Args a) {
+    using T = PrecTraits<P>;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (int c = 0; c < a.ncomp; ++c) {
+        double m = 0.;
+        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < a.total;
+             t += stride) {
+            int lo = 0, hi = a.nbox - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a.vprefix[mid] <= t)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const BoxDesc &b = a.boxes[lo];
+            const long long r = t - a.vprefix[lo];
+            const long long row = r / b.n[0];
+            const long long e = (long long)c * a.cstride + b.base + (r - row * b.n[0]) +
+                                (row % b.n[1]) * b.jstride + (row / b.n[1]) * b.kstride;
+            m = fmax(m, fabs(load_state_at<T>(a.hi, a.lo, e)));
+        }
+        unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long v = __shfl_xor(bits, o, 64);
+            bits = v > bits ? v : bits;
+        }
+        if ((threadIdx.x & 63) == 0) atomicMax(&a.out[c], bits);
     }
 }
 

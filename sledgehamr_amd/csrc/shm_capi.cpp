@@ -283,6 +283,47 @@ __attribute__((visibility("default"))) int shm_gw_spectrum(
     });
 }
 
+__attribute__((visibility("default"))) int shm_gw_spectrum_mod(
+    shm_ctx *ctx, int mode, int nbins, int zero_padding, int projection_type,
+    int modifier, const double *params, int nparams, double *bins, double *kvals,
+    int *nout) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!nout) throw shm::Error(SHM_ERR_ARG, "null nout");
+        if (nparams < 0 || (nparams > 0 && !params))
+            throw shm::Error(SHM_ERR_ARG, "bad modifier parameters");
+        std::vector<double> p(params, params + nparams), b, k;
+        ctx->rt->gw_spectrum(mode, nbins, zero_padding, projection_type, b, k, modifier, p);
+        if (bins) {
+            if (*nout < (int)b.size())
+                throw shm::Error(SHM_ERR_ARG, "output arrays too small");
+            std::memcpy(bins, b.data(), b.size() * sizeof(double));
+            if (kvals) std::memcpy(kvals, k.data(), k.size() * sizeof(double));
+        }
+        *nout = (int)b.size();
+    });
+}
+
+__attribute__((visibility("default"))) int shm_gw_modifiers(shm_ctx *ctx, int *count) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!count) throw shm::Error(SHM_ERR_ARG, "null count");
+        *count = ctx->rt->kernels().ngwmod;
+    });
+}
+
+__attribute__((visibility("default"))) int shm_gw_modifier_name(shm_ctx *ctx, int id,
+                                                                char *buf, int len) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        const shm::KernelTable &kt = ctx->rt->kernels();
+        if (id < 0 || id >= kt.ngwmod || !kt.gwmod_name)
+            throw shm::Error(SHM_ERR_ARG, "no such GW spectrum modifier");
+        if (!buf || len <= 0) throw shm::Error(SHM_ERR_ARG, "bad buffer");
+        std::snprintf(buf, (size_t)len, "%s", kt.gwmod_name(id));
+    });
+}
+
 __attribute__((visibility("default"))) int shm_fill_coarse_patch(shm_ctx *ctx,
                                                                 int lev) {
     return guard(ctx, [&] {

@@ -42,6 +42,7 @@ struct FusedArgs {
     int ghost;
     long long cstride;
     const void *u0;
+    const void *u0lo;    // remainder of u0 (FP64 tiles of stored pairs)
     void *kout;          // SSPRK3: k2; RKN: F1
     double dt;           // u1 = u0 + dt k1
     void *f0out;         // RKN: F0
@@ -51,7 +52,8 @@ struct FusedArgs {
 /** Static shape of a fused tile: 64 x NW cells, radius R. */
 template <class Ops, Prec P, int R, bool GW, bool KO, int NW>
 struct FusedShape {
-    using S = typename PrecTraits<P>::S;
+    using S = typename PrecTraits<P>::S;  // storage (k1, F0)
+    using V = typename PrecTraits<P>::V;  // tile values (u0, u1)
     using CS = CompSet<Ops, GW, KO>;
     static constexpr int NC = CS::NC, NS = CS::NS, NP = CS::NP;
     static constexpr int TX = 64, TY = NW, NT = 64 * NW, W = 2 * R + 1;
@@ -60,18 +62,20 @@ struct FusedShape {
     static constexpr int L0 = (E0 + NT - 1) / NT;  // window points per thread
     static constexpr int L1 = (E1 + NT - 1) / NT;  // stage-one points per thread
     static constexpr long lds_bytes =
-        (long)sizeof(S) * ((long)NS * W * E0 + (long)NS * W * E1 + (long)(NP > 0 ? NP : 1) * E1);
+        (long)sizeof(V) * ((long)NS * W * E0 + (long)NS * W * E1 + (long)(NP > 0 ? NP : 1) * E1);
     // Instantiated when the LDS fits one workgroup per CU and the delay
     // line stays small.
     static constexpr bool fits =
         !KO && lds_bytes <= 160 * 1024 && L1 <= 3 &&
-        (NC + NP) * (R + 1) * (int)(sizeof(S) / 4) <= 48;
+        (NC * (int)(sizeof(S) / 4) + NP * (int)(sizeof(V) / 4)) * (R + 1) <= 48;
 };
 
 template <class Ops, Prec P, int R, bool GW, bool KO, int NW, bool RKN>
 __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     using F = FusedShape<Ops, P, R, GW, KO, NW>;
+    using T = PrecTraits<P>;
     using S = typename F::S;
+    using V = typename F::V;
     using CS = typename F::CS;
     constexpr int NC = F::NC, NS = F::NS, NP = F::NP, NPA = NP > 0 ? NP : 1;
     constexpr int W = F::W, NT = F::NT;
@@ -79,9 +83,9 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     constexpr int L0 = F::L0, L1 = F::L1;
     static_assert(F::fits, "fused tile exceeds LDS or register budget");
 
-    __shared__ S w0[NS][W][E0];   // u0, stencil components, tile + 2R
-    __shared__ S w1[NS][W][E1];   // u1, stencil components, tile + R
-    __shared__ S p0[NPA][E1];     // u0, point components, tile + R
+    __shared__ V w0[NS][W][E0];   // u0, stencil components, tile + 2R
+    __shared__ V w1[NS][W][E1];   // u1, stencil components, tile + R
+    __shared__ V p0[NPA][E1];     // u0, point components, tile + R
 
     signed char slot[NC];
 #pragma unroll
@@ -94,7 +98,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     const double dt = a.prm1[kPrmDt], dx = a.prm1[kPrmDx];
     const double *par1 = a.prm1 + kPrmUser, *par2 = a.prm2 + kPrmUser;
     const double *gpar1 = a.prm1 + kPrmGw, *gpar2 = a.prm2 + kPrmGw;
-    const S cdt = (S)a.dt;
+    const V cdt = (V)a.dt;
 
     // Stage-one points of this thread: its own cell (x = tx, y = w), then a
     // ring cell of the tile + R region.
@@ -172,8 +176,8 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         int swin = 0;                         // slot of the next window plane
 
         // Next plane's loads (registers).
-        S v0[NS][L0];
-        S vp[NPA][L1];
+        V v0[NS][L0];
+        V vp[NPA][L1];
         auto load_window = [&]() {
             const unsigned zo = (unsigned)(zwin + g) * ks;
             zwin = next(zwin, nz);
@@ -181,7 +185,8 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             for (int s = 0; s < NS; ++s)
 #pragma unroll
                 for (int m = 0; m < L0; ++m)
-                    v0[s][m] = ld<S>(a.u0, org + CS::comp_of_slot(s) * a.cstride, c0[m] + zo);
+                    v0[s][m] = ld_tile<T>(a.u0, a.u0lo, org + CS::comp_of_slot(s) * a.cstride,
+                                          c0[m] + zo);
         };
         auto load_point = [&]() {
             const unsigned zo = (unsigned)(zpt + g) * ks;
@@ -190,7 +195,8 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int m = 0; m < L1; ++m)
-                    vp[p][m] = ld<S>(a.u0, org + CS::comp_of_point(p) * a.cstride, c1[m] + zo);
+                    vp[p][m] = ld_tile<T>(a.u0, a.u0lo, org + CS::comp_of_point(p) * a.cstride,
+                                          c1[m] + zo);
         };
         auto store_window = [&]() {
             const int sl = swin;
@@ -213,7 +219,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         int slo = 0;  // slot of plane zz - R (lowest plane of stage one's window)
 
         S dk1[R + 1][NC];    // k1 of the own cell, planes zz-R .. zz
-        S du1[R + 1][NPA];   // u1 of the point components, same planes
+        V du1[R + 1][NPA];   // u1 of the point components, same planes
 
         for (int zz = td.z0 - R; zz < td.z1 + R; ++zz) {
             // Stage one runs on plane zz: window planes zz-R .. zz+R.
@@ -248,7 +254,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                 const int lc = (e1y[m] + 2 * R) * E0X + (e1x[m] + 2 * R);
                 const int gi = bd.lo[0] + td.x0 + e1x[m], gj = bd.lo[1] + td.y0 + e1y[m],
                           gk = bd.lo[2] + zz;
-                S u0v[NC];
+                V u0v[NC];
                 double ctr[NC];
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
@@ -265,7 +271,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                 tc.cj = gj;
                 tc.ck = gk;
                 tc.R = R;
-                tc.f32 = std::is_same_v<S, float> ? 1 : 0;
+                tc.f32 = std::is_same_v<V, float> ? 1 : 0;
                 tc.lds_pitch = E0X;
                 tc.lds_cstride = W * E0;
                 tc.zq_cstride = 0;
@@ -297,12 +303,12 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                     S f0[NPR];
 #pragma unroll
                     for (int p = 0; p < CS::NPAIR; ++p) f0[p] = (S)rhs[CS::pair_mom(p)];
-                    auto state2 = [&](int n) -> S {
+                    auto state2 = [&](int n) -> V {
                         const int p = CS::pair_of_field(n);
-                        S y = u0v[n];
+                        V y = u0v[n];
                         if (p < 0) return y;
-                        y += (S)a.c0 * u0v[CS::pair_mom(p)];
-                        y += (S)a.c1 * f0[p];
+                        y += (V)a.c0 * u0v[CS::pair_mom(p)];
+                        y += (V)a.c1 * (V)f0[p];
                         return y;
                     };
 #pragma unroll
@@ -327,7 +333,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         const int n = CS::comp_of_slot(s);
-                        w1[s][sz][l1[m]] = fma_st<S>(cdt, k1[n], u0v[n]);
+                        w1[s][sz][l1[m]] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
                     }
                     if (m == 0) {
 #pragma unroll
@@ -335,7 +341,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 #pragma unroll
                         for (int p = 0; p < NP; ++p) {
                             const int n = CS::comp_of_point(p);
-                            du1[R][p] = fma_st<S>(cdt, k1[n], u0v[n]);
+                            du1[R][p] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
                         }
                     }
                 }
@@ -364,7 +370,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             tc.cj = gj;
             tc.ck = gk;
             tc.R = R;
-            tc.f32 = std::is_same_v<S, float> ? 1 : 0;
+            tc.f32 = std::is_same_v<V, float> ? 1 : 0;
             tc.lds_pitch = E1X;
             tc.lds_cstride = W * E1;
             tc.zq_cstride = 0;

@@ -99,6 +99,7 @@ struct CfArgs {
     void *dst, *dstlo;         // state (kCfState; dstlo null: leading words
                                // only), k (kCfK), F_m (kCfForce)
     const void *u0;            // fine u0 / phi0, pi0 (kCfK, kCfForce)
+    const void *u0lo;          // their remainders (FP64 tiles of stored pairs)
     double c0;                 // kCfK: c; kCfForce: coefficient of pi0
     int mom[32];               // kCfForce: momentum component of each pair
     int pair[32];              // kCfForce: pair index (force component)

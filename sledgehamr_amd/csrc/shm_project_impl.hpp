@@ -39,6 +39,9 @@ void fill_table_p1_r3(KernelTable &kt);
 void fill_table_p2_r1(KernelTable &kt);
 void fill_table_p2_r2(KernelTable &kt);
 void fill_table_p2_r3(KernelTable &kt);
+void fill_table_p3_r1(KernelTable &kt);
+void fill_table_p3_r2(KernelTable &kt);
+void fill_table_p3_r3(KernelTable &kt);
 
 template <class Ops>
 const KernelTable &project_kernel_table() {
@@ -54,16 +57,19 @@ const KernelTable &project_kernel_table() {
         fill_table_p0_r1(kt);
         fill_table_p1_r1(kt);
         fill_table_p2_r1(kt);
+        fill_table_p3_r1(kt);
 #endif
 #if SHM_R_MASK & 4
         fill_table_p0_r2(kt);
         fill_table_p1_r2(kt);
         fill_table_p2_r2(kt);
+        fill_table_p3_r2(kt);
 #endif
 #if SHM_R_MASK & 8
         fill_table_p0_r3(kt);
         fill_table_p1_r3(kt);
         fill_table_p2_r3(kt);
+        fill_table_p3_r3(kt);
 #endif
         fill_table_aux(kt);
         return kt;
@@ -117,11 +123,11 @@ template <class Ops, int K, Prec P, int R, bool GW, bool KO>
 void fill_variants(KernelTable &kt) {
     auto put = [&](auto vc) {
         constexpr int V = decltype(vc)::value;
-        using S = typename PrecTraits<P>::S;
+        using TV = typename PrecTraits<P>::V;  // tile values in LDS
         constexpr int TX = kVariants[V].tx;
         constexpr int TY = kVariants[V].nw * 64 / TX * kVariants[V].ry;
         constexpr long lds = 2L * CompSet<Ops, GW, KO>::NS * (TY + 2 * R) *
-                             (TX + 2 * R) * (long)sizeof(S);
+                             (TX + 2 * R) * (long)sizeof(TV);
         // Keep at least two workgroups per CU (160 KB LDS); dissipation
         // variants only in the default tile shape.
         if constexpr (lds <= 80 * 1024 && (!KO || V == kDefaultVariant)) {
@@ -156,8 +162,12 @@ void fill_kinds(KernelTable &kt) {
     fill_variants<Ops, kRknF2, P, R, GW, KO>(kt);
     fill_variants<Ops, kRknF3, P, R, GW, KO>(kt);
     fill_variants<Ops, kRknF4, P, R, GW, KO>(kt);
-    fill_variants<Ops, kRkn0Y, P, R, GW, KO>(kt);
-    fill_variants<Ops, kRkn1Y, P, R, GW, KO>(kt);
+    // The second-stage state is stored in the storage type: tiles of the
+    // storage type only.
+    if constexpr (std::is_same_v<typename PrecTraits<P>::V, typename PrecTraits<P>::S>) {
+        fill_variants<Ops, kRkn0Y, P, R, GW, KO>(kt);
+        fill_variants<Ops, kRkn1Y, P, R, GW, KO>(kt);
+    }
     // Leapfrog writes u_new momenta in place while the stage reads them
     // point-wise; with dissipation the momenta are differenced, so that
     // combination is not instantiated.
@@ -238,6 +248,13 @@ hipError_t launch_output(const OutArgs &a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <class Ops>
+hipError_t launch_gw_bin(const BinArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((gw_bin_kernel<typename Ops::GwModifiers, 2048>), dim3(grid),
+                       dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <class Ops, bool GW>
 hipError_t launch_probe(ProbeOut *out, const double *p, const double *pg,
                         double t, double dt, double dx, hipStream_t s) {
@@ -280,6 +297,11 @@ void fill_table_aux(KernelTable &kt) {
         kt.te_tag[1][1] = &launch_te_tag<ShmProjectOps, Prec::F32C, true>;
         kt.te_tag[2][1] = &launch_te_tag<ShmProjectOps, Prec::F32C16, true>;
         kt.probe[1] = &launch_probe<ShmProjectOps, true>;
+        using Mods = typename ShmProjectOps::GwModifiers;
+        kt.gw_bin = &launch_gw_bin<ShmProjectOps>;
+        kt.ngwmod = Mods::count;
+        kt.gwmod_name = &Mods::name;
+        kt.gwmod_select = &Mods::select;
     }
 }
 }  // namespace shm

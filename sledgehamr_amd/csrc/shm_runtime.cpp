@@ -376,11 +376,52 @@ Runtime::Runtime(sledgehamr::Sledgehamr *sim, const KernelTable &kt)
     prm_ev_.assign(kPrmSlots, nullptr);
     SHM_HIP(hipEventCreate(&ev0_));
     SHM_HIP(hipEventCreate(&ev1_));
+    sim_->level_norm0_ = [this](int lev) { return level_norm0(lev); };
+}
+
+std::vector<double> Runtime::level_norm0(int lev) {
+    LevelState &ls = level(lev);
+    std::vector<double> out(ncomp_, 0.);
+    const long long total = ls.L.nvalid();
+    if (total == 0 || !ls.U[ls.cur]) return out;
+    upload_into(norm0_out_, std::vector<unsigned long long>(ncomp_, 0ull));
+    Norm0Args a{};
+    a.boxes = ls.L.dbox.as<BoxDesc>();
+    a.vprefix = ls.L.dvprefix.as<long long>();
+    a.nbox = (int)ls.L.boxes.size();
+    a.total = total;
+    a.cstride = ls.L.cstride;
+    a.ncomp = ncomp_;
+    a.hi = ls.U[ls.cur].get();
+    a.lo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
+    a.out = norm0_out_.as<unsigned long long>();
+    const int grid = (int)std::min<long long>((total + 255) / 256, 8LL * ncu_);
+    switch (sprec()) {
+        case Prec::F64:
+            hipLaunchKernelGGL(norm0_kernel<Prec::F64>, dim3(grid), dim3(256), 0, stream_, a);
+            break;
+        case Prec::F32C:
+            hipLaunchKernelGGL(norm0_kernel<Prec::F32C>, dim3(grid), dim3(256), 0, stream_, a);
+            break;
+        default:
+            hipLaunchKernelGGL(norm0_kernel<Prec::F32C16>, dim3(grid), dim3(256), 0, stream_,
+                               a);
+            break;
+    }
+    SHM_HIP(hipGetLastError());
+    std::vector<unsigned long long> bits(ncomp_);
+    SHM_HIP(hipMemcpyAsync(bits.data(), norm0_out_.get(), bits.size() * 8,
+                           hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+    for (int c = 0; c < ncomp_; ++c) std::memcpy(&out[c], &bits[c], 8);
+    return out;
 }
 
 Runtime::~Runtime() {
+    sim_->level_norm0_ = nullptr;
     if (stream_) (void)hipStreamSynchronize(stream_);
     release_graphs();
+    release_gw_cache();
     for (EventPair &p : ev_pool_) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -579,12 +620,15 @@ void Runtime::init_from_inputs(const std::string &inputs,
             prec_ = Prec::F32C;
         else if (prec == "fp32c16" || prec == "FP32C16")
             prec_ = Prec::F32C16;
+        else if (prec == "fp32cs" || prec == "FP32CS")
+            prec_ = Prec::F32CS;
         else
-            throw Error(1, "shm.precision must be fp64, fp32c or fp32c16");
+            throw Error(1, "shm.precision must be fp64, fp32c, fp32c16 or fp32cs");
     }
     if (precision_override == 0) prec_ = Prec::F64;
     if (precision_override == 1) prec_ = Prec::F32C;
     if (precision_override == 2) prec_ = Prec::F32C16;
+    if (precision_override == 3) prec_ = Prec::F32CS;
     pp.query("shm.tile_variant", launch.variant);
     pp.query("shm.tile_z", launch.tz);
     pp.query("shm.xcd_split", launch.xcd_split);
@@ -644,6 +688,9 @@ void Runtime::init_from_inputs(const std::string &inputs,
         int on = 0;
         pp.query("shm.local_regrid", on);
         local_regrid_on_ = on != 0;
+        int keep = 0;
+        pp.query("shm.keep_layout_on_global_regrid", keep);
+        keep_layout_on_global_ = keep != 0;
     }
     lr_ = RegridState();
     lr_.no_local.assign(S.max_level + 1, 0);
@@ -786,8 +833,10 @@ LaunchCfg Runtime::resolved_launch(int lev) const {
 }
 
 bool Runtime::rkn_y(int lev) const {
+    // The second-stage state is stored in the storage type: not with FP64
+    // tiles of stored pairs.
     return rkn_state_ != 0 && integrator_ >= 20 && nodes_.size() >= 3 && nodes_[0] == 0. &&
-           !has_cf(lev);
+           !has_cf(lev) && !tile_lo();
 }
 
 const FusedEntry *Runtime::fused_row() const {
@@ -841,6 +890,7 @@ void Runtime::launch_fused(LevelState &ls, const double *prm1, const double *prm
     a.ghost = ls.L.g;
     a.cstride = ls.L.cstride;
     a.u0 = ls.U[ls.cur ^ 1].get();
+    a.u0lo = tile_lo_of(ls, ls.cur ^ 1);
     a.kout = ls.K[1].get();
     a.dt = dt;
     if (integrator_ >= 20) {
@@ -950,6 +1000,11 @@ void Runtime::halo(LevelState &ls, void *arr, const std::vector<int> &comps,
     }
 }
 
+void Runtime::halo_state(LevelState &ls, int which, const std::vector<int> &comps) {
+    halo(ls, ls.U[which].get(), comps);
+    if (tile_lo()) halo(ls, ls.Ulo[which].get(), comps, false, (int)losize());
+}
+
 void Runtime::fill_patch(LevelState &ls, int which, int lev, double time, bool for_step) {
     if (has_cf(lev)) {
         // The coarser level may have changed (its step, average-down).
@@ -957,8 +1012,8 @@ void Runtime::fill_patch(LevelState &ls, int which, int lev, double time, bool f
             // A step reads the leading words of step_ghost_comps() at ghost
             // cells; the remainders and other components stay as they are.
             const std::vector<int> comps = step_ghost_comps();
-            cf_fill_state(lev, time, ls.U[which].get(), nullptr, comps, true);
-            halo(ls, ls.U[which].get(), comps);
+            cf_fill_state(lev, time, ls.U[which].get(), tile_lo_of(ls, which), comps, true);
+            halo_state(ls, which, comps);
             ls.ghosts_ok[which] = false;
             return;
         }
@@ -967,7 +1022,7 @@ void Runtime::fill_patch(LevelState &ls, int which, int lev, double time, bool f
     } else if (ls.ghosts_ok[which]) {
         return;
     }
-    halo(ls, ls.U[which].get(), all_comps());
+    halo_state(ls, which, all_comps());
     ls.ghosts_ok[which] = true;
 }
 
@@ -980,7 +1035,7 @@ void Runtime::cf_fill_ssprk(LevelState &ls, int lev, int old, double t0, double 
         a.rec3[q] = push_record(rec, 1, tt[q]);
     }
     a.dst = ls.U[old].get();
-    a.dstlo = nullptr;
+    a.dstlo = tile_lo_of(ls, old);
     a.kd[0] = ls.K[0].get();
     a.kd[1] = ls.K[1].get();
     a.cf[0] = dt;
@@ -1258,6 +1313,7 @@ void Runtime::cf_launch(CfArgs &a, int mode) {
         case Prec::F64: cf_dispatch_mode<Prec::F64>(a, mode, ip, grid, stream_); break;
         case Prec::F32C: cf_dispatch_mode<Prec::F32C>(a, mode, ip, grid, stream_); break;
         case Prec::F32C16: cf_dispatch_mode<Prec::F32C16>(a, mode, ip, grid, stream_); break;
+        case Prec::F32CS: cf_dispatch_mode<Prec::F32CS>(a, mode, ip, grid, stream_); break;
     }
     SHM_HIP(hipGetLastError());
     if (ep) SHM_HIP(hipEventRecord(ep->b, stream_));
@@ -1362,6 +1418,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         CfArgs f = cf_args(lev, time, false, true);
         f.dst = k;
         f.u0 = ls.U[old].get();
+        f.u0lo = tile_lo_of(ls, old);
         f.c0 = c;
         const std::vector<int> sc = stencil_comps();
         for (size_t c0 = 0; c0 < sc.size(); c0 += 32) {
@@ -1383,6 +1440,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             StageArgs a{};
             a.prm = prm(st[0]);
             a.x = ls.U[old].get();
+            a.xlo = ulo_old;
             a.kout = ls.K[0].get();
             launch_stage(kSsprk1, ls, a, lev);
             // Coarse/fine ghosts of k1 and k2 were written with the step's
@@ -1393,6 +1451,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             StageArgs b{};
             b.prm = prm(st[1]);
             b.u0 = ls.U[old].get();
+            b.u0lo = ulo_old;
             b.kin = ls.K[0].get();
             b.kout = ls.K[1].get();
             b.c[0] = 1.;
@@ -1455,6 +1514,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                     CfArgs f = cf_args(lev, st[i + 1], false, true);
                     f.dst = ls.F[i].get();
                     f.u0 = ls.U[old].get();
+                    f.u0lo = tile_lo_of(ls, old);
                     f.c0 = dt * nodes_[i + 1];
                     f.m = i;
                     for (int j = 0; j <= i; ++j) f.cf[j] = dt * dt * tableau_[i + 1][j];
@@ -1488,6 +1548,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         StageArgs a{};
         a.prm = prm(st[0]);
         a.x = ls.U[old].get();
+        a.xlo = ulo_old;
         a.u0 = ls.U[old].get();
         a.u0lo = ulo_old;
         a.vh = ls.VH.get();
@@ -1496,13 +1557,17 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
         a.c[0] = dt / 2.;
         a.c[1] = dt;
         launch_stage(kLeap1, ls, a, lev);
-        // The second kick reads the fields' leading words at ghost cells.
-        if (cf) cf_fill_state(lev, st[1], ls.U[nw].get(), nullptr, stencil_comps(), true);
-        halo(ls, ls.U[nw].get(), stencil_comps());
+        // The second kick reads the fields' leading words at ghost cells
+        // (and their remainders with FP64 tiles).
+        if (cf)
+            cf_fill_state(lev, st[1], ls.U[nw].get(), tile_lo_of(ls, nw), stencil_comps(),
+                          true);
+        halo_state(ls, nw, stencil_comps());
 
         StageArgs b{};
         b.prm = prm(st[1]);
         b.x = ls.U[nw].get();
+        b.xlo = ulo_new;
         b.vh = ls.VH.get();
         b.unew = ls.U[nw].get();
         b.unewlo = ulo_new;
@@ -1513,7 +1578,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
     // coarse/fine ghost cells the next step fills them again, so the fill is
     // left to what reads them in between (tagging, outputs), with the same
     // coarse data and time.
-    if (!cf) halo(ls, ls.U[nw].get(), all_comps());
+    if (!cf) halo_state(ls, nw, all_comps());
     ls.ghosts_ok[nw] = !cf;
 }
 
@@ -1607,7 +1672,7 @@ void Runtime::advance(int lev) {
     const double dt = sim_->dt[lev], dx = sim_->dx[lev];
     if (cf_merged(lev)) {
         cf_fill_ssprk(ls, lev, old, ls.t_old, dt);
-        halo(ls, ls.U[old].get(), step_ghost_comps());
+        halo_state(ls, old, step_ghost_comps());
         ls.ghosts_ok[old] = false;
     } else {
         fill_patch(ls, old, lev, ls.t_old, /*for_step=*/true);
@@ -1631,6 +1696,7 @@ void Runtime::fill_rhs(int lev, double time, double weight, bool add) {
     a.prm = push_params(prm_host_, prm_ev_, prm_slot_,
                         prm_dev_.as<double>(), stream_, blk);
     a.x = ls.U[ls.cur].get();
+    a.xlo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
     a.kin = ls.RHS.get();
     a.kout = ls.RHS.get();
     a.c[0] = weight;
@@ -1656,7 +1722,7 @@ void Runtime::avg_down(LevelState &fine, int fw, LevelState &crse, int cw) {
     a.total = fine.L.cprefix.back();
     a.ncomp = ncomp_;
     const int grid = (int)std::max<long long>(1, std::min<long long>((a.total + 255) / 256, 8LL * ncu_));
-    switch (prec_) {
+    switch (sprec()) {
         case Prec::F64:
             hipLaunchKernelGGL(avgdown_kernel<Prec::F64>, dim3(grid), dim3(256),
                                0, stream_, a);
@@ -1781,6 +1847,7 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
         cf_fill_state(lev, time, fine.U[fine.cur].get(),
                       comp() ? fine.Ulo[fine.cur].get() : nullptr);
     halo(fine, fine.U[fine.cur].get(), all_comps(), /*full=*/true);
+    if (comp()) halo(fine, fine.Ulo[fine.cur].get(), all_comps(), true, (int)losize());
     fine.ghosts_ok[fine.cur] = false;
     const sledgehamr::Sledgehamr &S = *sim_;
     std::vector<double> ptag, pmod;
@@ -1834,7 +1901,7 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     a.dt = sim_->dt[lev];
     a.dx = sim_->dx[lev];
     a.flags = flags;
-    const TeTagFn fn = kt_.te_tag[(int)prec_][gw_ ? 1 : 0];
+    const TeTagFn fn = kt_.te_tag[(int)sprec()][gw_ ? 1 : 0];
     if (!fn) throw Error(4, "tagging kernel not built");
     if (total > 0) {
         const int grid = (int)std::min<long long>((total + 255) / 256, 8LL * ncu_);
@@ -1910,7 +1977,7 @@ void Runtime::valid_io(LevelState &ls, int comp_id, int which, double *host, boo
     if (to_state)
         SHM_HIP(hipMemcpy(dense.get(), host, sizeof(double) * (size_t)n,
                           hipMemcpyHostToDevice));
-    switch (prec_) {
+    switch (sprec()) {
         case Prec::F64:
             if (to_state)
                 hipLaunchKernelGGL((valid_io_kernel<Prec::F64, true>), dim3(grid), dim3(256), 0,
@@ -2044,7 +2111,7 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
     a.obj = obj.empty() ? nullptr : dobj.as<double>();
     a.nobj = (int)(obj.size() / 8);
     const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
-    switch (prec_) {
+    switch (sprec()) {
         case Prec::F64:
             hipLaunchKernelGGL(init_kernel<Prec::F64>, dim3(grid), dim3(256), 0,
                                stream_, a);
@@ -2165,10 +2232,12 @@ void Runtime::autotune(int lev) {
     if (integrator_ == 10) {
         StageArgs s1 = a;
         s1.x = ls.U[old].get();
+        s1.xlo = comp() ? ls.Ulo[old].get() : nullptr;
         s1.kout = ls.K[0].get();
         runs.push_back({kSsprk1, s1});
         StageArgs s2 = a;
         s2.u0 = ls.U[old].get();
+        s2.u0lo = comp() ? ls.Ulo[old].get() : nullptr;
         s2.kin = ls.K[0].get();
         s2.kout = ls.K[1].get();
         s2.c[0] = 1.;
@@ -2184,6 +2253,7 @@ void Runtime::autotune(int lev) {
         for (int i = 0; i + 1 < s; ++i) {
             StageArgs r = a;
             r.u0 = ls.U[old].get();
+            r.u0lo = comp() ? ls.Ulo[old].get() : nullptr;
             r.c[0] = dt * nodes_[i];
             r.nprev = i;
             for (int j = 0; j < i; ++j) {
@@ -2399,67 +2469,104 @@ static double index_to_k(int a, int N, int type) {
     return 0.;
 }
 
+void Runtime::release_gw_cache() {
+    if (gwc_.have_plan) (void)hipfftDestroy(gwc_.plan);
+    gwc_ = GwSpecCache();
+}
+
+/** GravitationalWaves::ComputeSpectrum (gravitational_waves.cpp:52-322):
+ *  the six selected components of level 0, zero padded, through hipFFT D2Z,
+ *  the project's Fourier-space modification per mode, and the Lambda
+ *  projected sum per bin, divided by N^6.  The plan, the buffers and the k
+ *  tables stay with the context for the next call of the same size. */
 void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
-                          std::vector<double> &bins,
-                          std::vector<double> &kvals) {
+                          std::vector<double> &bins, std::vector<double> &kvals,
+                          int modifier, const std::vector<double> &mparams) {
     if (independent_)
         throw Error(4, "the GW spectrum needs one periodic domain, not independent boxes");
     if (!gw_) throw Error(3, "gravitational waves are not enabled");
     if (proj < 1 || proj > 3) throw Error(1, "projection_type must be 1, 2 or 3");
     if (zp < 1 || !sledgehamr::utils::IsPowerOfTwo(zp))
         throw Error(1, "zero padding must be a power of 2");
+    if (mode < 0 || mode > 2) throw Error(1, "spectrum mode must be 0, 1 or 2");
+    if (!kt_.gw_bin) throw Error(4, "GW spectrum kernel not built");
     LevelState &l0 = level(0);
     const int N = sim_->coarse_level_grid_size * zp;
     const int NH = N / 2 + 1;
     const int NS = kt_.nscalars;
-    const int comps[6] = {NS + 6, NS + 9, NS + 10, NS + 7, NS + 11, NS + 8};
-
-    std::vector<double> itk(N);
-    for (int a = 0; a < N; ++a) itk[a] = index_to_k(a, N, proj);
-    std::vector<int> sq2bin;
-    kvals.clear();
-    if (mode == 0) {
-        std::set<long long> vals;
-        std::vector<long long> k2(N);
-        for (int a = 0; a < N; ++a) {
-            const long long f = a >= N / 2 ? a - N : a;
-            k2[a] = f * f;
-        }
-        std::set<long long> u(k2.begin(), k2.end());
-        std::vector<long long> uv(u.begin(), u.end());
-        for (long long x : uv)
-            for (long long y : uv)
-                for (long long z : uv) vals.insert(x + y + z);
-        const long long maxsq = 3LL * (N / 2) * (N / 2);
-        sq2bin.assign(maxsq + 1, -1);
-        int i = 0;
-        for (long long v : vals) {
-            if (v <= maxsq) sq2bin[v] = i;
-            kvals.push_back((double)v);
-            ++i;
-        }
-        nbins = (int)vals.size();
-    } else if (mode == 1) {
-        nbins = 0;
-        for (int k = 0; k <= std::sqrt(3.) / 2. * (double)N + .5; ++k) {
-            kvals.push_back((double)k * k);
-            ++nbins;
-        }
-    } else {
-        if (nbins <= 0) throw Error(1, "log binning needs nbins > 0");
-        const double lk = std::log(std::sqrt(3.) * N / 2.);
-        for (int i = 0; i < nbins; ++i) kvals.push_back(std::exp(lk * i / nbins));
+    // GravitationalWavesSpectrumModifier::SelectComponents
+    // (gravitational_waves.h:79-92): du_xx, du_xy, du_xz, du_yy, du_yz, du_zz
+    // unless the project's modifier names others.
+    int comps[6] = {NS + 6, NS + 9, NS + 10, NS + 7, NS + 11, NS + 8};
+    if (modifier >= 0) {
+        if (modifier >= kt_.ngwmod || !kt_.gwmod_select(modifier, comps))
+            throw Error(1, "no such GW spectrum modifier");
+        for (int c = 0; c < 6; ++c)
+            if (comps[c] < 0 || comps[c] >= ncomp_)
+                throw Error(1, "GW spectrum modifier selects a component out of range");
     }
+    GwSpecCache &w = gwc_;
+    if (w.N != N) {
+        release_gw_cache();
+        w.N = N;
+        w.dense.alloc(sizeof(double) * (size_t)N * N * N);
+        for (int c = 0; c < 6; ++c) w.out[c].alloc(sizeof(double2) * (size_t)NH * N * N);
+        if (hipfftPlan3d(&w.plan, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS)
+            throw Error(2, "hipfftPlan3d failed");
+        w.have_plan = true;
+    }
+    if (hipfftSetStream(w.plan, stream_) != HIPFFT_SUCCESS)
+        throw Error(2, "hipfftSetStream failed");
+    if (w.proj != proj) {
+        std::vector<double> itk(N);
+        for (int a = 0; a < N; ++a) itk[a] = index_to_k(a, N, proj);
+        upload_vec(w.ditk, itk);
+        w.proj = proj;
+    }
+    if (w.mode != mode || (mode == 2 && w.nbins != nbins)) {
+        w.kvals.clear();
+        if (mode == 0) {
+            // Unique |n|^2 over the signed frequencies: every |f| in 0..N/2
+            // occurs on each axis, so mark f1^2 + f2^2 + f3^2 for
+            // 0 <= f1 <= f2 <= f3 <= N/2.
+            const long long h = N / 2, maxsq = 3 * h * h;
+            std::vector<char> seen(maxsq + 1, 0);
+            for (long long a = 0; a <= h; ++a)
+                for (long long b = a; b <= h; ++b)
+                    for (long long c = b; c <= h; ++c) seen[a * a + b * b + c * c] = 1;
+            std::vector<int> sq2bin(maxsq + 1, -1);
+            int i = 0;
+            for (long long v = 0; v <= maxsq; ++v)
+                if (seen[v]) {
+                    sq2bin[v] = i++;
+                    w.kvals.push_back((double)v);
+                }
+            upload_vec(w.dsq, sq2bin);
+            w.nbin_eff = i;
+        } else if (mode == 1) {
+            int nb = 0;
+            for (int k = 0; k <= std::sqrt(3.) / 2. * (double)N + .5; ++k) {
+                w.kvals.push_back((double)k * k);
+                ++nb;
+            }
+            w.nbin_eff = nb;
+        } else {
+            if (nbins <= 0) throw Error(1, "log binning needs nbins > 0");
+            const double lk = std::log(std::sqrt(3.) * N / 2.);
+            for (int i = 0; i < nbins; ++i) w.kvals.push_back(std::exp(lk * i / nbins));
+            w.nbin_eff = nbins;
+        }
+        w.dbins.alloc(sizeof(double) * std::max(1, w.nbin_eff));
+        w.mode = mode;
+        w.nbins = nbins;
+    }
+    kvals = w.kvals;
+    const int nb_eff = w.nbin_eff;
 
-    DevMem dense, out[6], ditk, dsq, dbins;
-    dense.alloc(sizeof(double) * (size_t)N * N * N);
-    hipfftHandle plan;
-    if (hipfftPlan3d(&plan, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS)
-        throw Error(2, "hipfftPlan3d failed");
-    hipfftSetStream(plan, stream_);
+    // Zero padding: the level fills the corner [0, N/zp)^3 of the dense
+    // array (its boxes tile it completely); the rest keeps the zeros of its
+    // allocation.
     for (int c = 0; c < 6; ++c) {
-        out[c].alloc(sizeof(double2) * (size_t)NH * N * N);
-        SHM_HIP(hipMemsetAsync(dense.get(), 0, dense.bytes(), stream_));
         for (size_t b = 0; b < l0.L.boxes.size(); ++b) {
             PackArgs p{};
             p.arr = l0.U[l0.cur].get();
@@ -2467,11 +2574,11 @@ void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
             p.cstride = l0.L.cstride;
             p.box = l0.L.hbox[b];
             p.comp = comps[c];
-            p.out = dense.as<double>();
+            p.out = w.dense.as<double>();
             p.N = N;
             const long long nb = l0.L.boxes[b].vol();
             const int grid = (int)std::min<long long>((nb + 255) / 256, 8LL * ncu_);
-            switch (prec_) {
+            switch (sprec()) {
                 case Prec::F64:
                     hipLaunchKernelGGL(pack_kernel<Prec::F64>, dim3(grid),
                                        dim3(256), 0, stream_, p);
@@ -2480,39 +2587,40 @@ void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
                     hipLaunchKernelGGL(pack_kernel<Prec::F32C>, dim3(grid),
                                        dim3(256), 0, stream_, p);
                     break;
-                case Prec::F32C16:
+                default:
                     hipLaunchKernelGGL(pack_kernel<Prec::F32C16>, dim3(grid),
                                        dim3(256), 0, stream_, p);
                     break;
             }
             SHM_HIP(hipGetLastError());
         }
-        if (hipfftExecD2Z(plan, dense.as<hipfftDoubleReal>(),
-                          out[c].as<hipfftDoubleComplex>()) != HIPFFT_SUCCESS) {
-            hipfftDestroy(plan);
+        if (hipfftExecD2Z(w.plan, w.dense.as<hipfftDoubleReal>(),
+                          w.out[c].as<hipfftDoubleComplex>()) != HIPFFT_SUCCESS)
             throw Error(2, "hipfftExecD2Z failed");
-        }
     }
-    hipfftDestroy(plan);
-    upload_vec(ditk, itk);
-    if (mode == 0) upload_vec(dsq, sq2bin);
-    dbins.alloc(sizeof(double) * std::max(1, nbins));
+    std::vector<double> mp = mparams;
+    mp.resize(std::max<size_t>(mp.size(), 1), 0.);
+    upload_into(w.dparams, mp);
+    // The bins are summed with atomics: clear them with a kernel-free copy
+    // of zeros, outside any captured sequence.
+    upload_into(w.dbins, std::vector<double>(std::max(1, nb_eff), 0.));
     BinArgs ba{};
-    for (int c = 0; c < 6; ++c) ba.u[c] = out[c].as<double2>();
+    for (int c = 0; c < 6; ++c) ba.u[c] = w.out[c].as<double2>();
     ba.N = N;
-    ba.index_to_k = ditk.as<double>();
+    ba.index_to_k = w.ditk.as<double>();
     ba.mode = mode;
-    ba.sq_to_bin = mode == 0 ? dsq.as<int>() : nullptr;
-    ba.nbins = nbins;
+    ba.sq_to_bin = mode == 0 ? w.dsq.as<int>() : nullptr;
+    ba.nbins = nb_eff;
     ba.log_kmax = std::log(std::sqrt(3.) * N / 2.);
-    ba.bins = dbins.as<double>();
+    ba.bins = w.dbins.as<double>();
+    ba.modifier = modifier;
+    ba.mparams = w.dparams.as<double>();
+    ba.dk = 2. * M_PI / (sim_->L * zp);
     const long long total = (long long)NH * N * N;
     const int grid = (int)std::min<long long>((total + 255) / 256, 4LL * ncu_);
-    hipLaunchKernelGGL(gw_bin_kernel<2048>, dim3(grid), dim3(256), 0, stream_,
-                       ba);
-    SHM_HIP(hipGetLastError());
-    bins.assign(nbins, 0.);
-    SHM_HIP(hipMemcpyAsync(bins.data(), dbins.get(), sizeof(double) * nbins,
+    SHM_HIP(kt_.gw_bin(ba, grid, stream_));
+    bins.assign(nb_eff, 0.);
+    SHM_HIP(hipMemcpyAsync(bins.data(), w.dbins.get(), sizeof(double) * nb_eff,
                            hipMemcpyDeviceToHost, stream_));
     SHM_HIP(hipStreamSynchronize(stream_));
     const double n6 = std::pow((double)N, 6);
@@ -2576,7 +2684,7 @@ void Runtime::spectrum(int fn, std::vector<double> &bins, std::vector<long long>
     a.dense = dense.as<double>();
     a.N = N;
     const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
-    SHM_HIP(kt_.output[(int)prec_](a, grid, stream_));
+    SHM_HIP(kt_.output[(int)sprec()](a, grid, stream_));
     F.alloc(sizeof(double2) * (size_t)NH * N * N);
     hipfftHandle plan;
     if (hipfftPlan3d(&plan, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS)
@@ -2692,7 +2800,7 @@ void Runtime::projection(int fn, int mode, int max_level, std::vector<double> &d
         a.proj = dproj.as<double>();
         a.nproj = dn.as<int>();
         const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
-        SHM_HIP(kt_.output[(int)prec_](a, grid, stream_));
+        SHM_HIP(kt_.output[(int)sprec()](a, grid, stream_));
     }
     data.assign((size_t)D * D, 0.);
     count.assign((size_t)D * D, 0);
@@ -2755,16 +2863,25 @@ void Runtime::scheduled_step(int lev) {
     }
     synchronize_levels(lev);
     // DoRegridIfScheduled (time_stepper.cpp:287-296, DoRegrid :346-390): the
-    // local regrid when shm.local_regrid is set; where the reference falls
-    // back to a global regrid (AMReX clustering) the layout stays as it is.
+    // local regrid when shm.local_regrid is set.  Where the reference falls
+    // back to a global regrid (AMReX clustering, not part of this library)
+    // the evolve stops with a status, the regrid done with: the caller runs
+    // its global regrid and remakes the levels (shm_remake_level).  With
+    // shm.keep_layout_on_global_regrid = 1 the layout stays instead and the
+    // local regrid's counters are left as they are (no regrid ran).
     for (size_t i = 0; i < schedule_.size(); ++i) {
         if (schedule_[i].second == ls.t && schedule_[i].first == lev) {
-            if (local_regrid_on_ && local_regrid(lev) == 0) {
-                ++lr_.global_wanted;
-                did_global_regrid(lev);
-            }
+            const bool global = local_regrid_on_ && local_regrid(lev) == 0;
             for (int l = lev; l <= S.finest_level; ++l) last_regrid_[l] = ls.t;
             schedule_.erase(schedule_.begin() + i);
+            if (global) {
+                ++lr_.global_wanted;
+                if (!keep_layout_on_global_)
+                    throw Error(4, "a global regrid of level " + std::to_string(lev) +
+                                       " is wanted at t = " + std::to_string(ls.t) +
+                                       " (remake the levels with shm_remake_level, or set "
+                                       "shm.keep_layout_on_global_regrid = 1)");
+            }
             break;
         }
     }

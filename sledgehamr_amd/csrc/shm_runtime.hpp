@@ -2,6 +2,7 @@
 // stage sequences of the integrators for one project instance.
 #pragma once
 
+#include <hipfft/hipfft.h>
 #include <sledgehamr.h>
 
 #include <array>
@@ -212,10 +213,16 @@ class Runtime {
     void fill_coarse_patch(int lev);
     void advance(int lev);               // Integrator::Advance(lev)
     void create_shadow_level();          // Sledgehamr::CreateShadowLevel
+    /** Maximum |value| per component over the valid cells of the current
+     *  state of level lev (Sledgehamr::LevelNorm0). */
+    std::vector<double> level_norm0(int lev);
     void te_tag(int lev, int flags, uint8_t *mask, long long *counts,
                 double *te_host);
+    /** GravitationalWaves::ComputeSpectrum with the project's spectrum
+     *  modifier `modifier` (-1: default components, unmodified). */
     void gw_spectrum(int mode, int nbins, int zero_padding, int proj_type,
-                     std::vector<double> &bins, std::vector<double> &kvals);
+                     std::vector<double> &bins, std::vector<double> &kvals,
+                     int modifier = -1, const std::vector<double> &mparams = {});
     void step(int ncoarse);              // TimeStepper::Advance(0)
     /** Spectrum::Compute of output function fn on level 0: bins per unique
      *  |k|^2 (k_sq). */
@@ -225,6 +232,7 @@ class Runtime {
     void projection(int fn, int mode, int max_level, std::vector<double> &data,
                     std::vector<int> &count, int &D);
     int output_count() const { return kt_.nout; }
+    const KernelTable &kernels() const { return kt_; }
     const char *output_name(int id) const {
         return kt_.out_name ? kt_.out_name(id) : nullptr;
     }
@@ -304,13 +312,25 @@ class Runtime {
     void fill_params(double *blk, double time, double dt, double dx, int lev);
     void ensure_arrays(LevelState &ls, int integrator);
     size_t esize() const { return prec_ == Prec::F64 ? 8 : 4; }
-    size_t losize() const { return prec_ == Prec::F32C16 ? 2 : 4; }
+    size_t losize() const { return sprec() == Prec::F32C16 ? 2 : 4; }
     bool comp() const { return prec_ != Prec::F64; }
+    /** Storage precision: the auxiliary kernels see only the stored arrays. */
+    Prec sprec() const { return storage_prec(prec_); }
+    /** Stencils read hi + lo (FP64 tiles of stored pairs): the remainders
+     *  of the states a step reads must be valid at ghost cells too. */
+    bool tile_lo() const { return prec_ == Prec::F32CS; }
+    /** Remainder array of state slot `which` when stencils read it. */
+    void *tile_lo_of(LevelState &ls, int which) const {
+        return tile_lo() ? ls.Ulo[which].get() : nullptr;
+    }
     void launch_stage(int pol, LevelState &ls, StageArgs &a, int lev);
     /** Fused SSPRK3 stages one and two possible at this level (every box
      *  periodic on itself, kernel built for the precision and radius). */
     void launch_fused(LevelState &ls, const double *prm1, const double *prm2,
                       double dt, int lev);
+    /** Same-level ghosts of state slot `which`: the leading words, and the
+     *  remainders when the stencils read them. */
+    void halo_state(LevelState &ls, int which, const std::vector<int> &comps);
     void halo(LevelState &ls, void *arr, const std::vector<int> &comps,
               bool full = false, int elem_bytes = 0);
     /** Ghost cells of grid_new of lev with their remainders, for functions
@@ -436,10 +456,24 @@ class Runtime {
     int n_error_buf_ = 1, max_local_regrids_ = 10;
     double vt_strong_ = 1.05, vt_weak_ = 1.1;
     bool local_regrid_on_ = false;  // shm.local_regrid
+    bool keep_layout_on_global_ = false;  // shm.keep_layout_on_global_regrid
     int rkn_state_ = 1;             // shm.rkn_state
     // Tagging's inputs and outputs, kept between calls (each call ends with
     // a stream synchronisation) so that tagging allocates nothing per call.
     DevMem te_crit_, te_ptag_, te_pmod_, te_mask_, te_counts_, te_out_;
+    DevMem norm0_out_;
+    /** GW spectrum work space, kept between calls: the D2Z plan and buffers
+     *  for one padded size, the k tables for one (size, projection type,
+     *  binning). */
+    struct GwSpecCache {
+        int N = 0, proj = 0, mode = -1, nbins = 0;
+        hipfftHandle plan = 0;
+        bool have_plan = false;
+        DevMem dense, out[6], ditk, dsq, dbins, dparams;
+        std::vector<double> kvals;
+        int nbin_eff = 0;
+    } gwc_;
+    void release_gw_cache();
     /** RKN steps of level lev whose first stage also writes the second
      *  stage's field states (first node 0, three or more stages, no
      *  coarse/fine ghosts). */

@@ -34,9 +34,16 @@ namespace shm {
  *   F64    : FP64 state and stage arrays.
  *   F32C   : FP32 state + FP32 remainder (u = hi + lo, ~48 significant bits).
  *   F32C16 : FP32 state + bf16 remainder (~32 significant bits).
- *  Stage arrays (k, RKN forces, stage states) are FP32 in both FP32 modes. */
-enum class Prec : int { F64 = 0, F32C = 1, F32C16 = 2 };
-constexpr int kNPrec = 3;
+ *   F32CS  : stored as F32C16; stage states and stencil tiles are formed in
+ *            FP64 from hi + lo (compensated stencils).
+ *  Stage arrays (k, RKN forces) are FP32 in every FP32 mode.
+ *
+ *  S is the storage type of the state and the stage arrays, L that of the
+ *  remainder, V the type of the stage states the stencils read (tile
+ *  values).  `exact`: a tile value carries the whole stored value (hi + lo),
+ *  so the stage updates need no separate remainder at the centre. */
+enum class Prec : int { F64 = 0, F32C = 1, F32C16 = 2, F32CS = 3 };
+constexpr int kNPrec = 4;
 
 template <Prec P>
 struct PrecTraits;
@@ -44,20 +51,37 @@ template <>
 struct PrecTraits<Prec::F64> {
     using S = double;
     using L = double;  // unused
+    using V = double;
     static constexpr bool comp = false;
+    static constexpr bool exact = true;
 };
 template <>
 struct PrecTraits<Prec::F32C> {
     using S = float;
     using L = float;
+    using V = float;
     static constexpr bool comp = true;
+    static constexpr bool exact = false;
 };
 template <>
 struct PrecTraits<Prec::F32C16> {
     using S = float;
     using L = uint16_t;  // bf16 bit pattern
+    using V = float;
     static constexpr bool comp = true;
+    static constexpr bool exact = false;
 };
+template <>
+struct PrecTraits<Prec::F32CS> {
+    using S = float;
+    using L = uint16_t;  // bf16 bit pattern
+    using V = double;
+    static constexpr bool comp = true;
+    static constexpr bool exact = true;
+};
+/** Storage twin of a precision: the auxiliary kernels (tagging, outputs,
+ *  transfers, ghost copies) see only the stored arrays. */
+constexpr Prec storage_prec(Prec p) { return p == Prec::F32CS ? Prec::F32C16 : p; }
 
 __device__ AMREX_FORCE_INLINE uint16_t f32_to_bf16_rne(float f) {
     const uint32_t b = __float_as_uint(f);
@@ -90,6 +114,7 @@ struct StageArgs {
     int xcd_split;
     int ghost;          // ghost width g of the level's bricks
     const void *x;      // stage-state source
+    const void *xlo;    // its remainder (read by FP64 tiles of stored pairs)
     const void *u0;     // old state (hi part for FP32C)
     const void *u0lo;   // old state remainder (FP32C)
     const void *kin;    // SSPRK3 k in / FillAddRhs previous rhs
@@ -224,6 +249,20 @@ __device__ AMREX_FORCE_INLINE void store_state(void *hi, void *lo, long long bas
         st<S>(hi, base, off, (S)u);
     }
 }
+/** The value a stored pair of u reads back as (hi + lo after rounding). */
+template <class T>
+__device__ AMREX_FORCE_INLINE double pair_value(double u) {
+    if constexpr (!T::comp) {
+        return (double)(typename T::S)u;
+    } else {
+        const float h = (float)u;
+        const float r = (float)(u - (double)h);
+        if constexpr (std::is_same_v<typename T::L, float>)
+            return (double)h + (double)r;
+        else
+            return (double)h + (double)bf16_to_f32(f32_to_bf16_rne(r));
+    }
+}
 /** Remainder of the stored pair as FP64 (0 without compensation). */
 template <class T>
 __device__ AMREX_FORCE_INLINE double load_lo(const void *lo, long long base,
@@ -239,6 +278,24 @@ template <class T>
 __device__ AMREX_FORCE_INLINE double load_state(const void *hi, const void *lo,
                                                 long long base, unsigned off) {
     return (double)ld<typename T::S>(hi, base, off) + load_lo<T>(lo, base, off);
+}
+/** Tile value (type V) of a stored state: the stored word for FP32 and FP64
+ *  tiles, hi + lo for the FP64 tiles of a stored pair.  NT: the value is
+ *  read at the centre only (non-temporal loads). */
+template <class T, bool NT = false>
+__device__ AMREX_FORCE_INLINE typename T::V ld_tile(const void *hi, const void *lo,
+                                                    long long base, unsigned off) {
+    using S = typename T::S;
+    if constexpr (T::comp && T::exact) {
+        const double h = NT ? (double)ld_nt<S>(hi, base, off) : (double)ld<S>(hi, base, off);
+        if constexpr (std::is_same_v<typename T::L, float>)
+            return h + (double)(NT ? ld_nt<float>(lo, base, off) : ld<float>(lo, base, off));
+        else
+            return h + (double)bf16_to_f32(NT ? ld_nt<uint16_t>(lo, base, off)
+                                               : ld<uint16_t>(lo, base, off));
+    } else {
+        return (typename T::V)(NT ? ld_nt<S>(hi, base, off) : ld<S>(hi, base, off));
+    }
 }
 /** Same, addressed by a 64-bit element index (auxiliary kernels). */
 template <class T>
@@ -284,52 +341,51 @@ struct PolSsprk {
     static constexpr int nkeep() {
         return K == kSsStage2 ? 1 : K == kSsStage3 ? 2 : 0;
     }
-    // Loaded at store time per component: stage 3 remainder of u0; fill-add
-    // previous rhs.
+    // Loaded at store time per component: stage 3 remainder of u0 (unless
+    // the kept u0 is the whole stored value); fill-add previous rhs.
     template <class CS, class T>
     static constexpr int next() {
-        return CS::NC * ((K == kSsStage3 && T::comp) || K == kSsFillAdd ? 1 : 0);
+        return CS::NC * ((K == kSsStage3 && T::comp && !T::exact) || K == kSsFillAdd ? 1 : 0);
+    }
+    /** Stage state of component n: u0 + c2 k_in in the tile type (composed
+     *  stages), else the source state. */
+    template <class CS, class T, bool NT>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_x(const StageArgs &a,
+                                                              long long org, int n,
+                                                              unsigned off,
+                                                              typename T::V *keep) {
+        using S = typename T::S;
+        using V = typename T::V;
+        const long long b = org + n * a.cstride;
+        if constexpr (composed) {
+            const V k = (V)(NT ? ld_nt<S>(a.kin, b, off) : ld<S>(a.kin, b, off));
+            const V u = ld_tile<T, NT>(a.u0, a.u0lo, b, off);
+            if (keep) {
+                keep[0] = k;
+                if constexpr (K == kSsStage3) keep[1] = u;
+            }
+            return fma_st<V>((V)a.c[2], k, u);
+        } else {
+            return ld_tile<T, NT>(a.x, a.xlo, b, off);
+        }
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        using S = typename T::S;
-        const long long b = org + n * a.cstride;
-        if constexpr (composed)  // stage state in the storage precision
-            return fma_st<S>((S)a.c[2], ld<S>(a.kin, b, off), ld<S>(a.u0, b, off));
-        else
-            return ld<S>(a.x, b, off);
+        return load_x<CS, T, false>(a, org, n, off, nullptr);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
-        using S = typename T::S;
-        if constexpr (composed) {
-            const long long b = org + n * a.cstride;
-            const S k = ld<S>(a.kin, b, off), u = ld<S>(a.u0, b, off);
-            keep[0] = k;
-            if constexpr (K == kSsStage3) keep[1] = u;
-            return fma_st<S>((S)a.c[2], k, u);
-        } else {
-            return load_st<CS, T>(a, org, n, off);
-        }
+    __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_x<CS, T, false>(a, org, n, off, keep);
     }
     /** load_keep for a point component: read at the centre only, so
      *  non-temporal (the stencil components' halos keep L2). */
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
-        using S = typename T::S;
-        const long long b = org + n * a.cstride;
-        if constexpr (composed) {
-            const S k = ld_nt<S>(a.kin, b, off), u = ld_nt<S>(a.u0, b, off);
-            keep[0] = k;
-            if constexpr (K == kSsStage3) keep[1] = u;
-            return fma_st<S>((S)a.c[2], k, u);
-        } else {
-            return ld_nt<S>(a.x, b, off);
-        }
+    __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_x<CS, T, true>(a, org, n, off, keep);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
@@ -340,7 +396,7 @@ struct PolSsprk {
 #pragma unroll
         for (int n = 0; n < CS::NC; ++n) {
             const long long b = org + n * a.cstride;
-            if constexpr (K == kSsStage3 && T::comp)
+            if constexpr (K == kSsStage3 && T::comp && !T::exact)
                 e[i++] = (S)load_lo<T>(a.u0lo, b, off);
             if constexpr (K == kSsFillAdd) e[i++] = ld<S>(a.kin, b, off);
         }
@@ -351,7 +407,7 @@ struct PolSsprk {
                                                     const double *rhs,
                                                     const double *ctr,
                                                     const typename T::S *e,
-                                                    const typename T::S (*kc)[NK]) {
+                                                    const typename T::V (*kc)[NK]) {
         using S = typename T::S;
         int i = 0;
 #pragma unroll
@@ -362,7 +418,7 @@ struct PolSsprk {
             if constexpr (K == kSsFillAdd) k += a.c[0] * (double)e[i++];
             if constexpr (K == kSsStage3) {
                 double u0 = (double)kc[n][1];
-                if constexpr (T::comp) u0 += (double)e[i++];
+                if constexpr (T::comp && !T::exact) u0 += (double)e[i++];
                 store_state<T>(a.unew, a.unewlo, b, off, u0 + a.c[1] * k);
             } else {
                 st<S>(a.kout, b, off, (S)k);
@@ -387,58 +443,67 @@ struct PolRkn {
     static constexpr int nkeep() {
         return FINAL ? 2 + NPREV : 0;
     }
-    // Final stage, per pair: the remainders of phi0 and pi0.
+    // Final stage, per pair: the remainders of phi0 and pi0 (unless the
+    // kept values are the whole stored values).
     template <class CS, class T>
     static constexpr int next() {
-        return FINAL && T::comp ? CS::NPAIR * 2 : 0;
+        return FINAL && T::comp && !T::exact ? CS::NPAIR * 2 : 0;
     }
 
-    /** Stage state in the storage precision (FP32 states are FP32 values,
-     *  so their sums are formed in FP32). */
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+    /** Stage state in the tile type (FP32 tiles hold FP32 values, so their
+     *  sums are formed in FP32; FP64 tiles of stored pairs start from
+     *  hi + lo). */
+    template <class CS, class T, bool NT>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_y(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
         using S = typename T::S;
+        using V = typename T::V;
         const int p = CS::pair_of_field(n);
-        const S phi0 = ld<S>(a.u0, org + n * a.cstride, off);
+        const V phi0 = ld_tile<T, NT>(a.u0, a.u0lo, org + n * a.cstride, off);
         if (p < 0) return phi0;
         // A first stage at node 0 (RKN4, RKN5) has the state phi0: pi0 is
         // then read at the centre only.
         if constexpr (!FINAL && NPREV == 0)
             if (a.c[0] == 0.) return phi0;
-        const S pi0 = ld<S>(a.u0, org + CS::pair_mom(p) * a.cstride, off);
-        S y = phi0;
-        y += (S)a.c[0] * pi0;
+        const V pi0 = ld_tile<T, NT>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+        V y = phi0;
+        y += (V)a.c[0] * pi0;
         if constexpr (FINAL) {
             keep[0] = phi0;
             keep[1] = pi0;
         }
 #pragma unroll
         for (int j = 0; j < NPREV; ++j) {
-            const S fj = ld<S>(a.f[j], org + p * a.cstride, off);
+            const V fj = (V)(NT ? ld_nt<S>(a.f[j], org + p * a.cstride, off)
+                                : ld<S>(a.f[j], org + p * a.cstride, off));
             if constexpr (FINAL) keep[2 + j] = fj;
-            y += (S)a.c[1 + j] * fj;
+            y += (V)a.c[1 + j] * fj;
         }
         return y;
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
-                                                               long long org, int n,
-                                                               unsigned off) {
-        typename T::S keep[FINAL ? 2 + NPREV : 1];
-        return load_keep<CS, T>(a, org, n, off, keep);
+    __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_y<CS, T, false>(a, org, n, off, keep);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
-        return load_keep<CS, T>(a, org, n, off, keep);
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        typename T::V keep[FINAL ? 2 + NPREV : 1];
+        return load_y<CS, T, false>(a, org, n, off, keep);
+    }
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_y<CS, T, false>(a, org, n, off, keep);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
                                                        long long org, unsigned off,
                                                        typename T::S *e) {
         using S = typename T::S;
-        if constexpr (FINAL && T::comp) {
+        if constexpr (FINAL && T::comp && !T::exact) {
             int i = 0;
 #pragma unroll
             for (int p = 0; p < CS::NPAIR; ++p) {
@@ -453,7 +518,7 @@ struct PolRkn {
                                                     const double *rhs,
                                                     const double *ctr,
                                                     const typename T::S *e,
-                                                    const typename T::S (*kc)[NK]) {
+                                                    const typename T::V (*kc)[NK]) {
         using S = typename T::S;
         int i = 0;
 #pragma unroll
@@ -466,7 +531,7 @@ struct PolRkn {
                 const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
                 double phi = (double)kc[nf][0];
                 double pi0 = (double)kc[nf][1];
-                if constexpr (T::comp) {
+                if constexpr (T::comp && !T::exact) {
                     phi += (double)e[i++];
                     pi0 += (double)e[i++];
                 }
@@ -491,7 +556,8 @@ struct PolRkn {
  *  the second stage's field state y1 = phi0 + c[18] pi0 + c[19] F0 into a.y,
  *  formed in the storage type exactly as PolRkn<1, false> forms it from the
  *  stored phi0, pi0 and F0.  The second stage (PolRkn1Y) then reads y1 at
- *  its stencil points instead of composing it from three arrays there. */
+ *  its stencil points instead of composing it from three arrays there.
+ *  Tiles of the storage type only: y1 is stored in that type. */
 struct PolRkn0Y : PolRkn<0, false> {
     template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
@@ -499,7 +565,7 @@ struct PolRkn0Y : PolRkn<0, false> {
                                                     const double *rhs,
                                                     const double *ctr,
                                                     const typename T::S *e,
-                                                    const typename T::S (*kc)[NK]) {
+                                                    const typename T::V (*kc)[NK]) {
         using S = typename T::S;
         PolRkn<0, false>::template store<CS, T, NK>(a, org, off, rhs, ctr, e, kc);
 #pragma unroll
@@ -517,22 +583,23 @@ struct PolRkn0Y : PolRkn<0, false> {
  *  at pi0.  Writes F1 as PolRkn<1, false> does. */
 struct PolRkn1Y : PolRkn<1, false> {
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
         using S = typename T::S;
+        using V = typename T::V;
         const int p = CS::pair_of_field(n);
-        if (p < 0) return ld<S>(a.u0, org + n * a.cstride, off);
-        return ld<S>(a.y, org + p * a.cstride, off);
+        if (p < 0) return ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
+        return (V)ld<S>(a.y, org + p * a.cstride, off);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *) {
+    __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *) {
         return load_st<CS, T>(a, org, n, off);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+    __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
         return load_keep<CS, T>(a, org, n, off, keep);
     }
 };
@@ -551,19 +618,19 @@ struct PolLeapfrog {
     static constexpr int next() { return CS::NPAIR * (KICK2 ? 1 : 2); }
 
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_st(const StageArgs &a,
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        return ld<typename T::S>(a.x, org + n * a.cstride, off);
+        return ld_tile<T>(a.x, a.xlo, org + n * a.cstride, off);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *) {
+    __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *) {
         return load_st<CS, T>(a, org, n, off);
     }
     template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::S load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::S *keep) {
+    __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
         return load_keep<CS, T>(a, org, n, off, keep);
     }
     template <class CS, class T>
@@ -587,7 +654,7 @@ struct PolLeapfrog {
                                                     const double *rhs,
                                                     const double *ctr,
                                                     const double *e,
-                                                    const typename T::S (*)[NK]) {
+                                                    const typename T::V (*)[NK]) {
         int i = 0;
 #pragma unroll
         for (int p = 0; p < CS::NPAIR; ++p) {
@@ -637,7 +704,7 @@ template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int NW, int RY,
 __global__ void __launch_bounds__(64 * NW)
     stage_kernel(const StageArgs a) {
     using T = PrecTraits<P>;
-    using S = typename T::S;
+    using V = typename T::V;  // tile values
     using CS = CompSet<Ops, GW, KO>;
     constexpr int NC = CS::NC;
     constexpr int NS = CS::NS;
@@ -654,7 +721,7 @@ __global__ void __launch_bounds__(64 * NW)
     constexpr int HPT = (H + NT - 1) / NT;
     static_assert(R >= 1 && R <= 3, "stencil radius must be 1..3");
 
-    __shared__ S lds[2][NS][PY][PX];
+    __shared__ V lds[2][NS][PY][PX];
 
     signed char slot[NC];
 #pragma unroll
@@ -752,15 +819,15 @@ __global__ void __launch_bounds__(64 * NW)
         // line instead of being read again at store time, when they have
         // left L2.  Otherwise they are reloaded with the point values.
         constexpr bool DELAY =
-            NK > 0 && NS * RY * (R + 2) * NK * (int)(sizeof(S) / 4) <= 64;
-        S q[NS][RY][Q];
-        S nq[NS][RY];
-        S hv[NS][HPT];
-        S pv[NP > 0 ? NP : 1][RY];
+            NK > 0 && NS * RY * (R + 2) * NK * (int)(sizeof(V) / 4) <= 64;
+        V q[NS][RY][Q];
+        V nq[NS][RY];
+        V hv[NS][HPT];
+        V pv[NP > 0 ? NP : 1][RY];
         E ev[RY][NE > 0 ? NE : 1];  // store-time inputs of the next plane
-        S kq[NS][RY][DELAY ? R + 1 : 1][NKA];  // kept values, planes z..z+R
-        S nk[NS][RY][NKA];  // kept values of the queue head / of plane z+1
-        S pk[NP > 0 ? NP : 1][RY][NKA];  // kept values of point components
+        V kq[NS][RY][DELAY ? R + 1 : 1][NKA];  // kept values, planes z..z+R
+        V nk[NS][RY][NKA];  // kept values of the queue head / of plane z+1
+        V pk[NP > 0 ? NP : 1][RY][NKA];  // kept values of point components
 
         // Prologue: planes z0-R .. z0+R-1 into q[..][1..2R], plane z0+R into nq.
 #pragma unroll
@@ -851,8 +918,8 @@ __global__ void __launch_bounds__(64 * NW)
                     if (m < HPT - 1 || threadIdx.x + m * NT < H)
                         (&lds[buf][s][0][0])[hl[m]] = hv[s][m];
             }
-            S pc[NP > 0 ? NP : 1][RY];
-            S pkc[NP > 0 ? NP : 1][RY][NKA];
+            V pc[NP > 0 ? NP : 1][RY];
+            V pkc[NP > 0 ? NP : 1][RY][NKA];
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
@@ -905,7 +972,7 @@ __global__ void __launch_bounds__(64 * NW)
                 tc.cj = gj;
                 tc.ck = gk;
                 tc.R = R;
-                tc.f32 = std::is_same_v<S, float> ? 1 : 0;
+                tc.f32 = std::is_same_v<V, float> ? 1 : 0;
                 tc.lds_pitch = PX;
                 tc.lds_cstride = PX * PY;
                 tc.zq_cstride = RY * Q;
@@ -946,7 +1013,7 @@ __global__ void __launch_bounds__(64 * NW)
                                 sv, gi, gj, gk, n, dx, a.ko_strength[n]);
                     }
                 }
-                S kc[NC][NKA];
+                V kc[NC][NKA];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
 #pragma unroll

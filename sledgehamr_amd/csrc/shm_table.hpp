@@ -12,6 +12,7 @@ struct StageArgs;
 struct FusedArgs;
 struct TeTagArgs;
 struct ProbeOut;
+struct BinArgs;
 
 /** One instantiated stage program per kind: the integrator stage is a
  *  compile-time property of the kernel, so its store path has no branches. */
@@ -57,6 +58,7 @@ using TeTagFn = hipError_t (*)(const TeTagArgs &, int grid, hipStream_t);
 using ProbeFn = hipError_t (*)(ProbeOut *, const double *, const double *,
                                double, double, double, hipStream_t);
 using OutFn = hipError_t (*)(const OutArgs &, int grid, hipStream_t);
+using GwBinFn = hipError_t (*)(const BinArgs &, int grid, hipStream_t);
 
 struct StageEntry {
     StageFn fn = nullptr;
@@ -78,16 +80,23 @@ struct KernelTable {
     int nscalars = 0, nfields = 0, ngw = 0;
     bool built_gw = false, built_ko = false;
     int r_mask = 0;
-    // [kind][precision][R][gw][ko][variant]
-    StageEntry stage[kNKind][3][4][2][2][kNVar];
-    FusedEntry fused12[3][4][2][kNVar];   // [precision][R][gw][variant]
-    FusedEntry fusedrkn[3][4][2][kNVar];  // the same for RKN stages one and two
-    TeTagFn te_tag[3][2] = {};  // [precision][gw]
+    // [kind][precision][R][gw][ko][variant]; every precision (shm::Prec).
+    StageEntry stage[kNKind][4][4][2][2][kNVar];
+    FusedEntry fused12[4][4][2][kNVar];   // [precision][R][gw][variant]
+    FusedEntry fusedrkn[4][4][2][kNVar];  // the same for RKN stages one and two
+    // Auxiliary kernels per storage precision (F64, F32C, F32C16).
+    TeTagFn te_tag[3][2] = {};  // [storage precision][gw]
     ProbeFn probe[2] = {};      // [gw]
     // Output functions for spectra and projections (SLEDGEHAMR_OUTPUT_FUNCTIONS).
     int nout = 0;
     const char *(*out_name)(int) = nullptr;
-    OutFn output[3] = {};       // [precision]
+    OutFn output[3] = {};       // [storage precision]
+    // GW spectrum binning with the project's spectrum modifiers
+    // (SLEDGEHAMR_GW_SPECTRUM_MODIFIERS).
+    GwBinFn gw_bin = nullptr;
+    int ngwmod = 0;
+    const char *(*gwmod_name)(int) = nullptr;
+    bool (*gwmod_select)(int, int *) = nullptr;
 };
 
 }  // namespace shm

@@ -3,7 +3,8 @@
 // First-order phase transition with gravitational waves.
 //   V = phi^2/2 - delta phi^3/3 + phi^4/4  ->  Phi'' = Lap(Phi) - V'(Phi)
 //   u_ij'' = Lap(u_ij) + d_i Phi d_j Phi
-// 6th-order Laplacian and gradient (Laplacian<3>, Gradient<3>).
+// 6th-order Laplacian and gradient (Laplacian<3>, Gradient<3>).  Two GW
+// spectrum modifiers: |k| u_ij, and du_ij plus a translated copy.
 
 #include <sledgehamr.h>
 
@@ -69,6 +70,48 @@ AMREX_GPU_DEVICE AMREX_FORCE_INLINE void GravitationalWavesRhs<true>(
 }
 
 SLEDGEHAMR_FINISH_SETUP
+
+/** Spectrum of |k| u_ij instead of du_ij: the six u components, each mode
+ *  scaled by its wave number |n| dk (n the signed integer wave vector). */
+struct SpectrumUTimesK {
+    static void SelectComponents(int components[6]) {
+        const int c[6] = {Gw::u_xx, Gw::u_xy, Gw::u_xz, Gw::u_yy, Gw::u_yz, Gw::u_zz};
+        for (int n = 0; n < 6; ++n) components[n] = c[n];
+    }
+    AMREX_GPU_HOST_DEVICE static void FourierSpaceModification(
+        double re[6], double im[6], int kx, int ky, int kz, double dk, int dimN,
+        const double* params) {
+        const double k = std::sqrt((double)(kx * kx + ky * ky + kz * kz)) * dk;
+        for (int n = 0; n < 6; ++n) {
+            re[n] *= k;
+            im[n] *= k;
+        }
+    }
+};
+
+/** Spectrum of du_ij(x) + du_ij(x + d) for the distance d = (params[0],
+ *  params[1], params[2]) (same length unit as sim.L): with the forward
+ *  transform's exp(-i k.x), each mode times 1 + exp(i k.d). */
+struct SpectrumPlusTranslatedCopy {
+    static void SelectComponents(int components[6]) {
+        const int c[6] = {Gw::du_xx, Gw::du_xy, Gw::du_xz, Gw::du_yy, Gw::du_yz,
+                          Gw::du_zz};
+        for (int n = 0; n < 6; ++n) components[n] = c[n];
+    }
+    AMREX_GPU_HOST_DEVICE static void FourierSpaceModification(
+        double re[6], double im[6], int kx, int ky, int kz, double dk, int dimN,
+        const double* params) {
+        const double ph = (kx * params[0] + ky * params[1] + kz * params[2]) * dk;
+        const double c = std::cos(ph), s = std::sin(ph);
+        for (int n = 0; n < 6; ++n) {
+            const double r = re[n], q = im[n];
+            re[n] = r + (c * r - s * q);
+            im[n] = q + (s * r + c * q);
+        }
+    }
+};
+
+SLEDGEHAMR_GW_SPECTRUM_MODIFIERS(SpectrumUTimesK, SpectrumPlusTranslatedCopy)
 
 class PhaseTransitionGW : public sledgehamr::Sledgehamr {
   public:

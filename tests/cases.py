@@ -181,3 +181,75 @@ def amr_batched_boxes(seed, n1=2000, base=256, margin=6):
         lo2 = [2 * v + margin + 2 * int(rng.integers(0, room // 2 + 1)) for v in lo]
         lev2.append([*lo2, *(v + e2 - 1 for v in lo2)])
     return lev1, lev2
+
+
+# ---------------------------------------------------------------------------
+# Tagging with user criteria and truncation-error modifiers (ComplexScalar
+# with project.tag_strings / project.te_modifier).
+# ---------------------------------------------------------------------------
+
+def winding_tags(psi):
+    """Winding of psi (complex [z, y, x], periodic) around the plaquettes
+    with lower corner at each cell in the xy, xz and yz planes, from the
+    principal-value phase increments along the four edges.  For straight
+    edges that miss the origin this equals the signed ray-crossing count."""
+    def wind(a, b):
+        c0 = psi
+        c1 = np.roll(psi, -1, axis=a)
+        c2 = np.roll(c1, -1, axis=b)
+        c3 = np.roll(psi, -1, axis=b)
+        tot = sum(np.angle(y / x) for x, y in ((c0, c1), (c1, c2), (c2, c3), (c3, c0)))
+        return np.rint(tot / (2 * np.pi)).astype(int)
+    # axes of [z, y, x]: x = 2, y = 1, z = 0
+    return (wind(2, 1) != 0) | (wind(2, 0) != 0) | (wind(1, 0) != 0)
+
+
+def user_tag_case(N=8, scale=1.0, seed=5):
+    """Fine data (FP32-representable, generic: no corner value or plaquette
+    edge meets the origin of the complex plane), coarse data = child mean +
+    delta with deltas chosen per coarse cell and component far from the
+    decision thresholds of te_crit = 1e-3 with and without the modifiers
+    (te / max|Psi_n| for the fields, te * dt for the momenta; dt = 0.05), so
+    rounding of the child mean cannot move a decision."""
+    rng = np.random.default_rng(seed)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    fine = [scale * f32(rng.standard_normal((N, N, N))) for _ in range(2)]
+    fine += [f32(0.3 * rng.standard_normal((N, N, N))) for _ in range(2)]
+    h = N // 2
+    mean = [f.reshape(h, 2, h, 2, h, 2).mean(axis=(1, 3, 5)) for f in fine]
+    norms = [np.abs(f).max() for f in fine]
+    kind = rng.integers(0, 3, size=(4, h, h, h))
+    delta = np.zeros((4, h, h, h))
+    for c in range(4):
+        if c < 2:
+            # 1.5e-3 passes alone, fails relative to the norm (about 3);
+            # 3e-3 * norm passes both ways.
+            vals = (0.0, 1.5e-3, 3e-3 * norms[c])
+        else:
+            # 5e-3 passes alone, fails times dt; 0.1 passes both ways.
+            vals = (0.0, 5e-3, 0.1)
+        for t, v in enumerate(vals):
+            delta[c][kind[c] == t] = v * np.where(rng.random((h, h, h)) < 0.5, -1, 1)[kind[c] == t]
+    crse = [mean[c] + delta[c] for c in range(4)]
+    return fine, crse, delta, norms
+
+
+def expected_tags(fine, delta, norms, modifier, user_on, dt=0.05, crit=1e-3):
+    N = fine[0].shape[0]
+    h = N // 2
+    te = np.abs(delta)
+    passes = np.zeros((4, h, h, h), dtype=bool)
+    for c in range(4):
+        m = te[c]
+        if modifier:
+            m = m / norms[c] if c < 2 else m * dt
+        passes[c] = m >= crit
+    user = winding_tags(fine[0] + 1j * fine[1]) if user_on else \
+        np.zeros((N, N, N), dtype=bool)
+    anyp = passes.any(axis=0)
+    mask = user.copy()
+    mask[0::2, 0::2, 0::2] |= anyp
+    even_user = user[0::2, 0::2, 0::2]
+    total = int(user.sum()) + int((8 - even_user[anyp]).sum())
+    counts = [total, int(user.sum())] + [8 * int(passes[c].sum()) for c in range(4)]
+    return mask, np.array(counts)

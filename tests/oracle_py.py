@@ -57,6 +57,7 @@ def _load(project: str) -> C.CDLL:
         "orc_shadow_time": (d, [vp]),
         "orc_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "orc_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
+        "orc_gw_spectrum_mod": (i, [vp, i, i, i, i, i, dp, i, dp, dp, ip]),
         "orc_init_state": (i, [vp, i, i, C.c_ulonglong, dp, i]),
         "orc_r2c": (None, [i, dp, dp]),
         "orc_evolve": (i, [vp, i]),
@@ -220,13 +221,20 @@ class Oracle:
                                      _dp(te) if te is not None else None))
         return mask, counts, te
 
-    def gw_spectrum(self, mode=0, nbins=256, zero_padding=1, projection_type=2):
+    def gw_spectrum(self, mode=0, nbins=256, zero_padding=1, projection_type=2,
+                    modifier=-1, params=()):
+        """GW spectrum with an optional project spectrum modifier (by id) and
+        its parameters."""
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        pp = _dp(p) if p.size else None
         n = C.c_int(0)
-        self._ck(self.lib.orc_gw_spectrum(self.h, mode, nbins, zero_padding,
-                                          projection_type, None, None, C.byref(n)))
+        self._ck(self.lib.orc_gw_spectrum_mod(self.h, mode, nbins, zero_padding,
+                                              projection_type, modifier, pp, p.size,
+                                              None, None, C.byref(n)))
         b, k = np.zeros(n.value), np.zeros(n.value)
-        self._ck(self.lib.orc_gw_spectrum(self.h, mode, nbins, zero_padding,
-                                          projection_type, _dp(b), _dp(k), C.byref(n)))
+        self._ck(self.lib.orc_gw_spectrum_mod(self.h, mode, nbins, zero_padding,
+                                              projection_type, modifier, pp, p.size,
+                                              _dp(b), _dp(k), C.byref(n)))
         return b, k
 
     def init_state(self, lev, kind, seed, params=()):

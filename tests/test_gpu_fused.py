@@ -13,7 +13,7 @@ from cases import (complex_scalar, ensemble_boxes, ensemble_fields, ensemble_inp
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
-ALLP = [shm.PREC_FP64, shm.PREC_FP32C, shm.PREC_FP32C16]
+ALLP = [shm.PREC_FP64, shm.PREC_FP32C, shm.PREC_FP32C16, shm.PREC_FP32CS]
 
 
 def _pair(project, inp, prec, fv, graphs=False):
@@ -56,7 +56,7 @@ def test_fused_equals_two_launches_radius(order):
     _same(a, b, 0, 2, ("radius", order))
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16, shm.PREC_FP32CS])
 def test_fused_ensemble_small_boxes(prec):
     """Boxes of 16..48 cells: tiles wider and taller than the boxes wrap
     more than once."""
@@ -139,7 +139,7 @@ def test_fused_rkn_radius(order):
     _same(a, b, 0, 2, ("rkn radius", order))
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16, shm.PREC_FP32CS])
 def test_fused_rkn_ensemble_small_boxes(prec):
     boxes = ensemble_boxes(6, 16, 48, seed=13)
     inp = ensemble_inputs(boxes)

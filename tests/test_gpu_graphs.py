@@ -19,7 +19,7 @@ def _pair(project, inp, prec):
     return a, b
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16, shm.PREC_FP32CS])
 @pytest.mark.parametrize("integrator", [10, 21, 22, 11])
 def test_graph_steps_match_eager(integrator, prec):
     if integrator == 10:
@@ -83,7 +83,7 @@ def test_graph_subcycled_evolve_matches_eager():
     assert b._get_info()["graph_replays"] > 0
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16, shm.PREC_FP32CS])
 @pytest.mark.parametrize("integrator,interp", [(10, 2), (21, 2), (22, 2), (11, 2),
                                                (10, 4), (10, 1), (21, 0)])
 def test_graph_coarse_fine_levels_match_eager(integrator, interp, prec):

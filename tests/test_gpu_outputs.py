@@ -18,8 +18,10 @@ from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL = {shm.PREC_FP64: 1e-12, shm.PREC_FP32C: 1e-8, shm.PREC_FP32C16: 1e-8}
-TOL_STEPPED = {shm.PREC_FP64: 1e-12, shm.PREC_FP32C: 1e-6, shm.PREC_FP32C16: 1e-6}
+TOL = {shm.PREC_FP64: 1e-12, shm.PREC_FP32C: 1e-8, shm.PREC_FP32C16: 1e-8,
+       shm.PREC_FP32CS: 1e-8}
+TOL_STEPPED = {shm.PREC_FP64: 1e-12, shm.PREC_FP32C: 1e-6, shm.PREC_FP32C16: 1e-6,
+               shm.PREC_FP32CS: 1e-6}
 LEV1 = [[8, 8, 8, 15, 15, 15], [16, 8, 8, 23, 15, 15], [4, 20, 0, 11, 27, 7]]
 LEV2 = [[20, 20, 20, 27, 27, 27], [30, 20, 20, 43, 27, 27]]
 

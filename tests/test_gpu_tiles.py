@@ -33,7 +33,7 @@ def _run(integrator, prec, variant):
 
 
 @pytest.mark.parametrize("integrator", [10, 21, 11])
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16, shm.PREC_FP32CS])
 def test_tile_variants_agree(integrator, prec):
     ref = None
     for v in (1, 0, 2, 3, 4):

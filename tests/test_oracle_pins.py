@@ -17,7 +17,8 @@ import os
 import numpy as np
 import pytest
 
-from cases import phase_transition, real_scalar
+from cases import (complex_scalar, expected_tags, phase_transition, real_scalar,
+                   user_tag_case, winding_tags)
 from oracle_py import (Oracle, oracle_lambda, oracle_r2c, oracle_unique_k2,
                        spectral_lib)
 
@@ -599,6 +600,96 @@ def test_te_tag_hand_built_blocks():
     assert counts[3] == 8 * 2                      # dPhi: blocks 2, 5
 
 
+def _complex_inputs(N, **extra):
+    return complex_scalar(N=N, integrator=10, L=N * 0.25, cfl=0.2, **extra)
+
+
+def test_user_string_tags_match_phase_winding():
+    """ComplexScalar's user criterion (a string pierces one of the cell's
+    three forward plaquettes) on a random field, where about half of the
+    plaquettes wind: every cell, odd and even, is tagged as the winding
+    computed from phase increments says; total and user counters both count
+    the tagged cells (TagWithTruncationCpu without truncation errors)."""
+    N = 8
+    orc = Oracle("ComplexScalar", _complex_inputs(N, **{"project.tag_strings": 1}))
+    rng = np.random.default_rng(3)
+    re, im = rng.standard_normal((2, N, N, N))
+    for c, f in enumerate([re, im, np.zeros_like(re), np.zeros_like(re)]):
+        orc.upload(0, c, f.ravel())
+    mask, counts, _ = orc.te_tag(0, flags=4)
+    want = winding_tags(re + 1j * im)
+    assert 0.2 < want.mean() < 0.9
+    parity = (np.indices((N, N, N)).sum(axis=0) % 2).astype(bool)
+    assert want[parity].any() and want[~parity].any()
+    assert np.array_equal(mask.reshape(N, N, N).astype(bool), want)
+    assert counts[0] == counts[1] == want.sum()
+    # Switched off by the project input: no user tags.
+    orc0 = Oracle("ComplexScalar", _complex_inputs(N))
+    for c, f in enumerate([re, im, np.zeros_like(re), np.zeros_like(re)]):
+        orc0.upload(0, c, f.ravel())
+    m0, c0, _ = orc0.te_tag(0, flags=4)
+    assert m0.sum() == 0 and c0[0] == c0[1] == 0
+
+
+@pytest.mark.parametrize("modifier", [0, 1])
+@pytest.mark.parametrize("user_on", [0, 1])
+def test_te_tag_with_user_tags_and_modifiers_hand_built(modifier, user_on):
+    """TagWithTruncationCpu (macros.h:489-566) with a user criterion and
+    truncation-error modifiers: the mask is user(cell) or (even cell and
+    some component passes), ntags_trunc adds 8 per passing component,
+    ntags_user counts user tags and ntags_total adds them plus 8 - user(even
+    cell) per passing block (macros.h:509-527).  Expected values follow from
+    the chosen deltas by those rules, not from the oracle's loop."""
+    N = 8
+    fine, crse, delta, norms = user_tag_case(N)
+    inp = _complex_inputs(N, **{"amr.te_crit": 1e-3, "project.tag_strings": user_on,
+                                "project.te_modifier": modifier})
+    orc = Oracle("ComplexScalar", inp)
+    for c in range(4):
+        orc.upload(0, c, fine[c].ravel())
+        orc.upload(-1, c, crse[c].ravel())
+    mask, counts, te = orc.te_tag(0, flags=1 | 4, want_te=True)
+    assert np.allclose(te, np.abs(delta).reshape(4, -1), rtol=0, atol=1e-13)
+    want_mask, want_counts = expected_tags(fine, delta, norms, modifier, user_on)
+    assert np.array_equal(mask.reshape(N, N, N).astype(bool), want_mask)
+    assert np.array_equal(counts, want_counts), (counts, want_counts)
+    # Each decision class occurs, and user tags meet passing blocks.
+    assert counts[2] > 0 and counts[4] > 0
+    if user_on:
+        assert 0 < counts[1] < N ** 3
+        ev = winding_tags(fine[0] + 1j * fine[1])[0::2, 0::2, 0::2]
+        assert ev.any() and (~ev).any()
+
+
+def test_modifier_norm_is_taken_at_each_tagging():
+    """SetParamsTruncationModifier reads max|Psi_n| of level 0 at every
+    tagging (as FirstOrderPhaseTransition.cpp:133-157 does): with the same
+    truncation errors, fields scaled by 4 fail the relative criterion that
+    they passed before (a norm kept from the first tagging would pass them
+    again)."""
+    N = 8
+    inp = _complex_inputs(N, **{"amr.te_crit": 1e-3, "project.te_modifier": 1})
+    orc = Oracle("ComplexScalar", inp)
+    fine, crse, delta, norms = user_tag_case(N)
+    h = N // 2
+    got = []
+    for scale in (1.0, 4.0):
+        for c in range(4):
+            f = fine[c] * (scale if c < 2 else 1.0)
+            mean = f.reshape(h, 2, h, 2, h, 2).mean(axis=(1, 3, 5))
+            orc.upload(0, c, f.ravel())
+            orc.upload(-1, c, (mean + delta[c]).ravel())
+        _, counts, te = orc.te_tag(0, flags=1 | 4, want_te=True)
+        assert np.allclose(te, np.abs(delta).reshape(4, -1), rtol=0, atol=1e-13)
+        sn = [norms[0] * scale, norms[1] * scale, norms[2], norms[3]]
+        _, want = expected_tags([fine[0] * scale, fine[1] * scale, fine[2], fine[3]],
+                                 delta, sn, 1, 0)
+        assert np.array_equal(counts, want), (scale, counts, want)
+        got.append(counts)
+    assert got[0][2] > 0 and got[1][2] == 0 and got[1][3] == 0
+    assert np.array_equal(got[0][4:], got[1][4:])
+
+
 # ---------------------------------------------------------------------------
 # GW spectrum (gravitational_waves.cpp:52-322, gravitational_waves.h:101-133)
 # ---------------------------------------------------------------------------
@@ -671,6 +762,114 @@ def test_spectrum_modes_conserve_total():
     assert abs(spec0.sum() - 1.0) < 1e-12 and abs(spec2.sum() - 1.0) < 1e-12
     assert abs(spec0[int(np.searchsorted(k0, 25))] - 1.0) < 1e-12
     assert abs(spec1.sum() - 1.0) < 1e-12
+
+
+# GW spectrum modifiers (GravitationalWavesSpectrumModifier,
+# gravitational_waves.h:79-92; the project's SpectrumUTimesK and
+# SpectrumPlusTranslatedCopy) and zero padding (utils/fft.h:143-261).
+
+def _plane_wave_orc(comp, m, N=16, rand_seed=None):
+    orc = Oracle("PhaseTransitionGW", phase_transition(N=N, L=float(N)))
+    n = np.arange(N)
+    k, j, i = np.meshgrid(n, n, n, indexing="ij")
+    rng = np.random.default_rng(rand_seed) if rand_seed is not None else None
+    for c in range(orc.ncomp):
+        f = np.zeros(N ** 3) if rng is None else rng.standard_normal(N ** 3)
+        orc.upload(0, c, f)
+    orc.upload(0, comp, np.cos(2 * np.pi * m * k / N).ravel())
+    return orc
+
+
+def test_gw_modifier_u_times_k_plane_wave():
+    """SpectrumUTimesK selects u_ij (the du_ij, random here, do not enter)
+    and scales every mode by |n| dk, dk = 2 pi / L: a transverse-traceless
+    plane wave u_xy = cos(2 pi m z / N), whose du_ij spectrum would be 1 in
+    bin m^2, gives (m dk)^2 there and nothing elsewhere."""
+    N, m = 16, 3
+    orc = _plane_wave_orc(5, m, N)          # u_xy
+    for c in range(8, 14):                  # du_ij random
+        orc.upload(0, c, np.random.default_rng(c).standard_normal(N ** 3))
+    spec, kv = orc.gw_spectrum(mode=1, zero_padding=1, projection_type=3, modifier=0)
+    dk = 2 * np.pi / N
+    assert abs(spec[m] - (m * dk) ** 2) < 1e-12
+    assert np.max(np.abs(np.delete(spec, m))) < 1e-12
+    # Default: the du_ij (random) instead.
+    spec_d, _ = orc.gw_spectrum(mode=1, zero_padding=1, projection_type=3)
+    assert np.max(np.abs(np.delete(spec_d, m))) > 1e-3
+
+
+@pytest.mark.parametrize("delta", [0.0, 1.0, 2.5])
+def test_gw_modifier_translated_copy_plane_wave(delta):
+    """SpectrumPlusTranslatedCopy multiplies every mode by 1 + exp(i k.d):
+    for the TT plane wave du_xy = cos(2 pi m z / N) and d = (0, 0, delta)
+    the spectrum in bin m^2 is |1 + exp(i m dk delta)|^2 = 2 + 2 cos(m dk
+    delta) (both signs of k_z give the same)."""
+    N, m = 16, 3
+    orc = _plane_wave_orc(11, m, N)         # du_xy
+    spec, _ = orc.gw_spectrum(mode=1, zero_padding=1, projection_type=3, modifier=1,
+                              params=[0.0, 0.0, delta])
+    dk = 2 * np.pi / N
+    assert abs(spec[m] - (2 + 2 * np.cos(m * dk * delta))) < 1e-12
+    assert np.max(np.abs(np.delete(spec, m))) < 1e-12
+
+
+def _brute_force_gw_spectrum(fields, comps, N0, zp, L, ptype, utk):
+    """Independent restatement for the pin below: zero-padded explicit DFT
+    (no FFT), Lambda contracted in its trace form Tr(PAPA) - Tr(PA)^2 / 2
+    with P = 1 - d d^T / |d|^2, d = the gradient symbol of the padded grid,
+    multiplicity 2 off the x = 0 and x = N/2 planes, unique |n|^2 bins, over
+    N^6; with utk every mode times |n| 2 pi / (L zp)."""
+    N = N0 * zp
+    W = np.exp(-2j * np.pi * np.outer(np.arange(N), np.arange(N)) / N)
+    A = []
+    for c in comps:
+        pad = np.zeros((N, N, N))
+        pad[:N0, :N0, :N0] = fields[c].reshape(N0, N0, N0)
+        A.append(np.einsum("az,by,cx,zyx->abc", W, W, W, pad)[:, :, :N // 2 + 1])
+    f = np.array([a - N if a >= N // 2 else a for a in range(N)])
+    sym = np.array([_grad_symbol(ptype, 2 * np.pi * (a - N if a > N // 2 - 1 else a) / N)
+                    for a in range(N)])
+    k2 = sorted({int(x * x + y * y + z * z) for x in f for y in f for z in f})
+    spec = np.zeros(len(k2))
+    dk = 2 * np.pi / (L * zp)
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    for cz in range(N):
+        for by in range(N):
+            for ax in range(N // 2 + 1):
+                if ax == 0 and by == 0 and cz == 0:
+                    continue
+                d = np.array([sym[ax], sym[by], sym[cz]])
+                P = np.eye(3) - np.outer(d, d) / d.dot(d)
+                M = np.array([[A[idx[i][j]][cz, by, ax] for j in range(3)] for i in range(3)])
+                if utk:
+                    M = M * np.sqrt(f[ax] ** 2 + f[by] ** 2 + f[cz] ** 2) * dk
+                val = 0.0
+                for part in (M.real, M.imag):
+                    B = P @ part
+                    val += np.trace(B @ B) - 0.5 * np.trace(B) ** 2
+                mult = 1.0 if ax in (0, N // 2) else 2.0
+                spec[k2.index(int(f[ax] ** 2 + f[by] ** 2 + f[cz] ** 2))] += mult * val
+    return spec / N ** 6, np.array(k2, dtype=float)
+
+
+@pytest.mark.parametrize("utk", [False, True])
+def test_gw_zero_padding_matches_brute_force_dft(utk):
+    """zero_padding = 2 places the level at the corner of a 2N grid (the
+    other blocks zero, utils/fft.h:152-190), transforms the padded grid
+    with k tables of 2N and L -> 2L, and normalises by (2N)^6."""
+    N0, zp = 4, 2
+    L = N0 * 0.5
+    orc = Oracle("PhaseTransitionGW", phase_transition(N=N0, L=L))
+    rng = np.random.default_rng(11)
+    fields = [rng.standard_normal(N0 ** 3) for _ in range(orc.ncomp)]
+    for c, f in enumerate(fields):
+        orc.upload(0, c, f)
+    sel = [2, 5, 6, 3, 7, 4] if utk else [8, 11, 12, 9, 13, 10]
+    spec, kv = orc.gw_spectrum(mode=0, zero_padding=zp, projection_type=2,
+                               modifier=0 if utk else -1)
+    want, wk = _brute_force_gw_spectrum(fields, sel, N0, zp, L, 2, utk)
+    assert np.array_equal(kv, wk)
+    assert np.max(np.abs(spec - want)) < 1e-12 * max(1.0, np.max(np.abs(want)))
 
 
 def _golden():
