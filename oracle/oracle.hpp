@@ -8,7 +8,7 @@
 //
 // Functions and the reference code they follow:
 //   fill_rhs / fill_add_rhs   source/macros.h:301-485 (FillRhs, FillAddRhs)
-//   rkn                      source/integrators/rkn.cpp:91-186, tableaux :188-271
+//   rkn                      source/integrators/rkn.cpp:25-120, tableaux :124-205
 //   lsssprk3                 source/integrators/lsssprk3.cpp:14-31
 //   leapfrog                 source/integrators/leapfrog.cpp:13-57
 //   advance                  source/integrators/integrator.cpp:11-31
@@ -868,7 +868,7 @@ class Sim {
         fill_patch(mn, lev, t1);
     }
 
-    /** IntegratorRkn::Integrate (rkn.cpp:91-186). */
+    /** IntegratorRkn::Integrate (rkn.cpp:25-120). */
     void rkn(MultiFab &mo, MultiFab &mn, int lev, double dtl, double dxl) {
         const double t = mo.t;
         const int ng = cfg.nghost;
@@ -1311,6 +1311,12 @@ class Sim {
     struct Sched { int lowest; double t; };
     std::vector<Sched> sched;
     std::vector<double> last_regrid;
+    // Order of the level operations of evolve(): 'S' a regrid scheduled by
+    // lev for time t, 'H' shadow level created, 'A' lev advanced to t, 'D'
+    // lev+1 averaged onto lev, 'T' truncation errors of lev (with its
+    // average-down) and tags, 'R' regrid from lev at t.
+    struct Event { char kind; int lev; double t; };
+    std::vector<Event> events;
 
     int find_sched(double t) const {
         for (size_t i = 0; i < sched.size(); ++i)
@@ -1343,15 +1349,22 @@ class Sim {
             sched.push_back({lev, time + dt[lev]});
         else
             sched[id].lowest = std::min(lev, sched[id].lowest);
-        if (lev == 0) create_shadow();
+        events.push_back({'S', lev, time + dt[lev]});
+        if (lev == 0) {
+            create_shadow();
+            events.push_back({'H', -1, shadow.t});
+        }
     }
 
     void synchronize_levels(int lev) {
         const bool te = need_te(lev, grid_new[lev].t);
         const int finest = this->finest();
-        if (lev < finest && !(cfg.shadow_hierarchy && te))
+        if (lev < finest && !(cfg.shadow_hierarchy && te)) {
             average_down_to(grid_new[lev + 1], grid_new[lev]);
+            events.push_back({'D', lev, grid_new[lev].t});
+        }
         if (lev >= 1 - (cfg.shadow_hierarchy ? 1 : 0) && te) {
+            events.push_back({'T', lev, grid_new[lev].t});
             std::vector<double> t;
             te_avgdown(lev, t, true);
             std::vector<uint8_t> m;
@@ -1760,6 +1773,7 @@ class Sim {
     void do_regrid_if_scheduled(int lev) {
         const double time = grid_new[lev].t;
         if (!do_regrid(lev, time)) return;
+        events.push_back({'R', lev, time});
         const bool global = cfg.local_regrid && local_regrid(lev) == 0;
         for (int l = lev; l <= finest(); ++l) last_regrid[l] = time;
         const int id = find_sched(time);
@@ -1777,6 +1791,7 @@ class Sim {
         if (cfg.shadow_hierarchy) schedule_regrid(lev);
         if (lev == 0 && !shadow.defined()) prj.BeforeTimestep(grid_new[lev].t);
         advance(lev);
+        events.push_back({'A', lev, grid_new[lev].t});
         if (lev != finest()) {
             step(lev + 1);
             step(lev + 1);

@@ -198,6 +198,18 @@
                 counts[i] = it->second[i];                                     \
         });                                                                    \
     }                                                                          \
+    ORC_EXPORT int orc_events(void *h, char *kind, int *lev, double *t,       \
+                              int maxn, int *n) {                              \
+        return orc_guard([&] {                                                 \
+            const auto &ev = static_cast<OrcSim *>(h)->events;                 \
+            *n = (int)ev.size();                                               \
+            for (int i = 0; i < (int)ev.size() && i < maxn; ++i) {             \
+                kind[i] = ev[i].kind;                                          \
+                lev[i] = ev[i].lev;                                            \
+                t[i] = ev[i].t;                                                \
+            }                                                                  \
+        });                                                                    \
+    }                                                                          \
     ORC_EXPORT int orc_create_shadow(void *h) {                                \
         return orc_guard([&] { static_cast<OrcSim *>(h)->create_shadow(); });  \
     }                                                                          \

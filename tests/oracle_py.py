@@ -57,6 +57,7 @@ def _load(project: str) -> C.CDLL:
         "orc_shadow_time": (d, [vp]),
         "orc_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "orc_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
+        "orc_events": (i, [vp, C.c_char_p, ip, dp, i, ip]),
         "orc_gw_spectrum_mod": (i, [vp, i, i, i, i, i, dp, i, dp, dp, ip]),
         "orc_init_state": (i, [vp, i, i, C.c_ulonglong, dp, i]),
         "orc_r2c": (None, [i, dp, dp]),
@@ -206,6 +207,20 @@ class Oracle:
         self._ck(self.lib.orc_last_tags(self.h, lev,
                                         c.ctypes.data_as(C.POINTER(C.c_longlong)), c.size))
         return c
+
+    def events(self):
+        """Level operations of the evolves so far, in order: (kind, lev, t)
+        with kind S (regrid scheduled for t), H (shadow level), A (advanced
+        to t), D (lev+1 averaged onto lev), T (truncation errors and tags),
+        R (regrid)."""
+        n = C.c_int(0)
+        self._ck(self.lib.orc_events(self.h, None, None, None, 0, C.byref(n)))
+        k = C.create_string_buffer(max(n.value, 1))
+        lv = np.zeros(max(n.value, 1), dtype=np.int32)
+        t = np.zeros(max(n.value, 1))
+        self._ck(self.lib.orc_events(self.h, k, lv.ctypes.data_as(C.POINTER(C.c_int)),
+                                     _dp(t), n.value, C.byref(n)))
+        return [(k.raw[i:i + 1].decode(), int(lv[i]), float(t[i])) for i in range(n.value)]
 
     def create_shadow(self):
         self._ck(self.lib.orc_create_shadow(self.h))

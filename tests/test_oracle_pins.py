@@ -990,10 +990,7 @@ def test_string_loops_carry_conserved_unit_winding():
     assert np.count_nonzero(div) == 0
 
 
-def test_oracle_subcycled_schedule():
-    """Sub-cycling on a static 3-level layout: level l advances 2^l times
-    per coarse step, times stay synchronised, and a regrid scheduled on
-    level 0 runs truncation errors on every level."""
+def _three_level_oracle(ncoarse):
     from cases import amr_boxes, complex_scalar
     lev1, lev2 = amr_boxes(seed=2024)
     inp = complex_scalar(N=32, integrator=10, nghost=2, L=8.0, cfl=0.25,
@@ -1004,11 +1001,46 @@ def test_oracle_subcycled_schedule():
     orc.set_boxes(2, lev2)
     for lev in range(3):
         orc.init_state(lev, 5, 77, [4, 5.0, 10.0, 1.5])
-    orc.evolve(3)
+    orc.evolve(ncoarse)
+    return orc
+
+
+# The order of level operations of two coarse steps of a 3-level hierarchy
+# with a shadow level and regrids due as often as possible, worked out by
+# hand from TimeStepper::Advance / ScheduleRegrid / SynchronizeLevels /
+# DoRegridIfScheduled (time_stepper.cpp:90-171, 221-311) and the
+# RegridScheduler (regrid_scheduler.cpp:9-74); times in units of the coarse
+# dt after t0.
+#   Step 1: no regrid is scheduled on level 0 (grid_old[0] holds no state
+#   yet); level 1 schedules one at the end of its second step (odd istep),
+#   so level 2's second step and level 1 compute truncation errors instead
+#   of averaging down, level 1 regrids, level 0 then averages down.
+#   Step 2: level 0 schedules for t0 + 2 (shadow level advanced at once);
+#   level 1's own request merges into it (lowest level 0); every level
+#   computes truncation errors at t0 + 2, and level 0 regrids.
+HAND_LOG = [
+    ("A", 0, 1.0), ("A", 1, 0.5), ("A", 2, 0.25), ("A", 2, 0.5), ("D", 1, 0.5),
+    ("S", 1, 1.0), ("A", 1, 1.0), ("A", 2, 0.75), ("A", 2, 1.0), ("T", 2, 1.0),
+    ("T", 1, 1.0), ("R", 1, 1.0), ("D", 0, 1.0),
+    ("S", 0, 2.0), ("H", -1, 2.0), ("A", 0, 2.0), ("A", 1, 1.5), ("A", 2, 1.25),
+    ("A", 2, 1.5), ("D", 1, 1.5), ("S", 1, 2.0), ("A", 1, 2.0), ("A", 2, 1.75),
+    ("A", 2, 2.0), ("T", 2, 2.0), ("T", 1, 2.0), ("T", 0, 2.0), ("R", 0, 2.0),
+]
+
+
+def test_oracle_subcycled_event_order():
+    """Sub-cycling on a static 3-level layout with the shadow level: the
+    oracle's sequence of advances, average-downs, truncation errors,
+    schedulings and regrids over two coarse steps is the hand-derived one
+    (a wrong branch, e.g. averaging down where truncation errors are due, or
+    level 0 scheduling in the first step, changes the log)."""
+    orc = _three_level_oracle(2)
     dt0 = 0.25 * 8.0 / 32   # binary-exact times: the scheduler compares them exactly
+    got = [(k, lev, (t - 0.125) / dt0) for k, lev, t in orc.events()]
+    assert got == HAND_LOG
     for lev in range(3):
-        assert orc.time(lev) == 0.125 + 3 * dt0
-        assert orc.last_tags(lev)[0] >= 0
+        assert orc.time(lev) == 0.125 + 2 * dt0
+        assert orc.last_tags(lev)[0] > 0
 
 
 # ---------------------------------------------------------------------------
