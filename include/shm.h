@@ -134,6 +134,23 @@ int shm_create_shadow(shm_ctx *ctx);
 int shm_te_tag(shm_ctx *ctx, int lev, int flags, unsigned char *mask,
                long long *counts, double *te);
 
+/* shm_te_tag without device memory allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc/hipFree: counters, parameter records, the
+ * coarse/fine fill record and the tag mask live in a fixed device scratch
+ * array of 32 MiB per GPU, held until the process ends, so the level may
+ * hold at most 33 550 336 valid cells (larger levels: SHM_ERR_ARG), and at
+ * most 62 components and 64 tagging parameters each.  The level must be
+ * set up by an earlier allocating call (its shadow level or coarsened copy
+ * and coarse/fine fill plans; otherwise SHM_ERR_STATE).  Outputs as
+ * shm_te_tag without te; the level's kept mask used by shm_regrid_select
+ * is not updated.  Every call on a GPU shares that GPU's scratch array and
+ * leaves its contents behind: calls that overlap on one GPU (two contexts
+ * used from two host threads) race. */
+int shm_te_tag_noalloc(shm_ctx *ctx, int lev, int flags, unsigned char *mask,
+                       long long *counts);
+
 /* GravitationalWaves::ComputeSpectrum (gravitational_waves.cpp:52-322).
  * mode 0: unbinned (unique |k|^2), 1: rounded |k|, 2: nbins log bins.
  * bins/kvals must hold *nout values (call with bins=NULL to query *nout). */

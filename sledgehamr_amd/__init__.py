@@ -30,6 +30,7 @@ EXPORTS = (
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
     "shm_gw_spectrum_mod", "shm_gw_modifiers", "shm_gw_modifier_name",
+    "shm_te_tag_noalloc",
     "shm_output_functions", "shm_output_function_name", "shm_spectrum", "shm_projection",
     "shm_upload_array", "shm_fill_coarse_patch",
     "shm_regrid_select", "shm_add_boxes", "shm_remake_level", "shm_local_regrid",
@@ -103,6 +104,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_create_shadow": (i, [vp]),
         "shm_level_tag_counts": (i, [vp, i, llp]),
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
+        "shm_te_tag_noalloc": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "shm_gw_spectrum_mod": (i, [vp, i, i, i, i, i, dp, i, dp, dp, ip]),
         "shm_gw_modifiers": (i, [vp, ip]),
@@ -283,6 +285,17 @@ class Simulation:
             counts.ctypes.data_as(C.POINTER(C.c_longlong)),
             _dptr(te) if te is not None else None))
         return mask, counts, te
+
+    def te_tag_noalloc(self, lev: int, flags: int = TE_COMPUTE | TE_TAGS):
+        """shm_te_tag_noalloc: the tagging without device allocation (mask,
+        counters)."""
+        n = self.ncells(lev)
+        mask = np.zeros(n, dtype=np.uint8)
+        counts = np.zeros(2 + self.info["ncomp"], dtype=np.int64)
+        self._check(self.lib.shm_te_tag_noalloc(
+            self._ctx, lev, flags, mask.ctypes.data_as(C.POINTER(C.c_ubyte)),
+            counts.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return mask, counts
 
     def gw_spectrum(self, mode: int = 0, nbins: int = 256, zero_padding: int = 1,
                     projection_type: int = 2, modifier: int = -1, params=()):

@@ -265,6 +265,16 @@ __attribute__((visibility("default"))) int shm_te_tag(shm_ctx *ctx, int lev,
     });
 }
 
+__attribute__((visibility("default"))) int shm_te_tag_noalloc(shm_ctx *ctx, int lev,
+                                                              int flags,
+                                                              unsigned char *mask,
+                                                              long long *counts) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->te_tag_noalloc(lev, flags, mask, counts);
+    });
+}
+
 __attribute__((visibility("default"))) int shm_gw_spectrum(
     shm_ctx *ctx, int mode, int nbins, int zero_padding, int projection_type,
     double *bins, double *kvals, int *nout) {

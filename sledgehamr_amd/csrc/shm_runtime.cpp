@@ -18,6 +18,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <set>
 #include <sstream>
 
@@ -355,6 +357,38 @@ static std::vector<Box3> coarsen(const std::vector<Box3> &b) {
 }
 
 // ---------------------------------------------------------------------------
+// Device scratch array: the working memory of the allocation-free tagging
+// entry point (shm_te_tag_noalloc) and of the level norms.  One array per
+// GPU for the whole process, shared by every context of this library on
+// that GPU; its device address is looked up once per device.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static_assert(kScratchRecOff + sizeof(double) * kPrmBlock <= kScratchMaskOff,
+              "coarse/fine record overlaps the mask region");
+
+static void *scratch_on_current_device() {
+    static std::mutex mu;
+    static std::map<int, void *> ptrs;
+    int dev = 0;
+    SHM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw Error(2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptrs[dev] = p;
+    return p;
+}
+
+/** Zeroes n 64-bit words (counters that are summed atomically). */
+__global__ void __launch_bounds__(64) clear_words_kernel(unsigned long long *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
 // Runtime.
 // ---------------------------------------------------------------------------
 
@@ -376,6 +410,7 @@ Runtime::Runtime(sledgehamr::Sledgehamr *sim, const KernelTable &kt)
     prm_ev_.assign(kPrmSlots, nullptr);
     SHM_HIP(hipEventCreate(&ev0_));
     SHM_HIP(hipEventCreate(&ev1_));
+    scratch_ = static_cast<unsigned char *>(scratch_on_current_device());
     sim_->level_norm0_ = [this](int lev) { return level_norm0(lev); };
 }
 
@@ -384,7 +419,13 @@ std::vector<double> Runtime::level_norm0(int lev) {
     std::vector<double> out(ncomp_, 0.);
     const long long total = ls.L.nvalid();
     if (total == 0 || !ls.U[ls.cur]) return out;
-    upload_into(norm0_out_, std::vector<unsigned long long>(ncomp_, 0ull));
+    if (ncomp_ > kScratchWords) throw Error(4, "too many components for the level norm");
+    // The maxima live in the scratch array: a project's host hooks may call
+    // this from the allocation-free tagging.
+    unsigned long long *dmax =
+        reinterpret_cast<unsigned long long *>(scratch_ + kScratchNormOff);
+    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, stream_, dmax, ncomp_);
+    SHM_HIP(hipGetLastError());
     Norm0Args a{};
     a.boxes = ls.L.dbox.as<BoxDesc>();
     a.vprefix = ls.L.dvprefix.as<long long>();
@@ -394,7 +435,7 @@ std::vector<double> Runtime::level_norm0(int lev) {
     a.ncomp = ncomp_;
     a.hi = ls.U[ls.cur].get();
     a.lo = comp() ? ls.Ulo[ls.cur].get() : nullptr;
-    a.out = norm0_out_.as<unsigned long long>();
+    a.out = dmax;
     const int grid = (int)std::min<long long>((total + 255) / 256, 8LL * ncu_);
     switch (sprec()) {
         case Prec::F64:
@@ -410,8 +451,8 @@ std::vector<double> Runtime::level_norm0(int lev) {
     }
     SHM_HIP(hipGetLastError());
     std::vector<unsigned long long> bits(ncomp_);
-    SHM_HIP(hipMemcpyAsync(bits.data(), norm0_out_.get(), bits.size() * 8,
-                           hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipMemcpyAsync(bits.data(), dmax, bits.size() * 8, hipMemcpyDeviceToHost,
+                           stream_));
     SHM_HIP(hipStreamSynchronize(stream_));
     for (int c = 0; c < ncomp_; ++c) std::memcpy(&out[c], &bits[c], 8);
     return out;
@@ -1242,7 +1283,8 @@ const double *Runtime::push_record(const std::vector<double> &blk, int kind, dou
     return push_params(prm_host_, prm_ev_, prm_slot_, prm_dev_.as<double>(), stream_, blk);
 }
 
-CfArgs Runtime::cf_args(int lev, double time, bool first_state_only, bool step) {
+CfArgs Runtime::cf_args(int lev, double time, bool first_state_only, bool step,
+                        double *rec_dev) {
     LevelState &ls = levels_[lev];
     if (!ls.cf_built) build_cf(lev);
     LevelState &cs = levels_[lev - 1];
@@ -1265,7 +1307,15 @@ CfArgs Runtime::cf_args(int lev, double time, bool first_state_only, bool step) 
     }
     std::vector<double> rec(kPrmBlock, 0.);
     cf_record(rec.data(), lev, time, first_state_only);
-    a.rec = push_record(rec, first_state_only ? 2 : 1, time);
+    if (rec_dev) {
+        // Synchronous: the record lives on this stack frame (the null
+        // stream orders it with the context's blocking stream).
+        SHM_HIP(hipMemcpy(rec_dev, rec.data(), sizeof(double) * kPrmBlock,
+                          hipMemcpyHostToDevice));
+        a.rec = rec_dev;
+    } else {
+        a.rec = push_record(rec, first_state_only ? 2 : 1, time);
+    }
     return a;
 }
 
@@ -1346,9 +1396,9 @@ void Runtime::fill_coarse_full(int lev, int which, double time, bool first_state
 }
 
 void Runtime::cf_fill_state(int lev, double time, void *hi, void *lo,
-                            const std::vector<int> &comps, bool step) {
+                            const std::vector<int> &comps, bool step, double *rec_dev) {
     const std::vector<int> cs = comps.empty() ? all_comps() : comps;
-    CfArgs a = cf_args(lev, time, false, step);
+    CfArgs a = cf_args(lev, time, false, step, rec_dev);
     a.dst = hi;
     a.dstlo = lo;
     for (size_t c0 = 0; c0 < cs.size(); c0 += 32) {
@@ -1820,6 +1870,74 @@ void Runtime::copy_state(void *dst, void *dstlo, const Layout &dl, const void *s
 void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
                      double *te_host) {
     LevelState &fine = level(lev);
+    const sledgehamr::Sledgehamr &S = *sim_;
+    // Device buffers of the context, kept between calls.
+    TagBuffers tb;
+    upload_into(te_crit_, S.te_crit);
+    tb.crit = te_crit_.as<double>();
+    // The mask stays with the level for the local regrid.
+    const size_t nmask = (size_t)std::max<long long>(1, fine.L.nvalid());
+    if (flags & kTeTags) {
+        if (fine.tags.bytes() != nmask) fine.tags.alloc(nmask);
+        tb.mask = fine.tags.as<uint8_t>();
+        tb.keep_tags = true;
+    } else {
+        if (te_mask_.bytes() < nmask) te_mask_.alloc(nmask);
+        tb.mask = te_mask_.as<uint8_t>();
+    }
+    if (te_counts_.bytes() < sizeof(unsigned long long) * (2 + ncomp_))
+        te_counts_.alloc(sizeof(unsigned long long) * (2 + ncomp_));
+    tb.counts = te_counts_.as<unsigned long long>();
+    const long long total = fine.L.cprefix.back();
+    if (te_host && (flags & kTeCompute)) {
+        if (te_out_.bytes() < sizeof(double) * total * ncomp_)
+            te_out_.alloc(sizeof(double) * total * ncomp_);
+        tb.te = te_out_.as<double>();
+    }
+    tb.ptag = &te_ptag_;
+    tb.pmod = &te_pmod_;
+    te_tag_run(lev, flags, tb, mask, counts, te_host);
+}
+
+void Runtime::te_tag_noalloc(int lev, int flags, uint8_t *mask, long long *counts) {
+    if (lev < 0 || lev >= (int)levels_.size() || !has_level(lev))
+        throw Error(1, "no such level");
+    LevelState &fine = levels_[lev];
+    const sledgehamr::Sledgehamr &S = *sim_;
+    if (2 + ncomp_ > kScratchWords || (int)S.te_crit.size() > kScratchWords)
+        throw Error(4, "too many components for the allocation-free tagging");
+    if (fine.L.nvalid() > kScratchMaxCells)
+        throw Error(1, "level " + std::to_string(lev) + " holds " +
+                           std::to_string(fine.L.nvalid()) +
+                           " valid cells; the allocation-free tagging takes at most " +
+                           std::to_string(kScratchMaxCells));
+    // What the tagging would otherwise build on first use.
+    if (flags & kTeCompute) {
+        if (lev == 0 && !shadow_.defined())
+            throw Error(3, "no shadow level for truncation errors on level 0");
+        if (lev > 0 && (!fine.mirror || fine.mirror_epoch != epoch_))
+            throw Error(3, "the coarsened copy of level " + std::to_string(lev) +
+                               " is not set up: run shm_te_tag on the level first");
+    }
+    if (has_cf(lev) && !fine.cf_built)
+        throw Error(3, "the coarse/fine fill of level " + std::to_string(lev) +
+                           " is not set up: run shm_te_tag or a step on the level first");
+    TagBuffers tb;
+    tb.counts = reinterpret_cast<unsigned long long *>(scratch_ + kScratchCountsOff);
+    tb.crit = reinterpret_cast<double *>(scratch_ + kScratchCritOff);
+    tb.ptag_dev = reinterpret_cast<double *>(scratch_ + kScratchTagPrmOff);
+    tb.pmod_dev = reinterpret_cast<double *>(scratch_ + kScratchModPrmOff);
+    tb.cf_rec = reinterpret_cast<double *>(scratch_ + kScratchRecOff);
+    tb.mask = scratch_ + kScratchMaskOff;
+    if (!S.te_crit.empty())
+        SHM_HIP(hipMemcpy(tb.crit, S.te_crit.data(), sizeof(double) * S.te_crit.size(),
+                          hipMemcpyHostToDevice));
+    te_tag_run(lev, flags, tb, mask, counts, nullptr);
+}
+
+void Runtime::te_tag_run(int lev, int flags, TagBuffers &tb, uint8_t *mask,
+                         long long *counts, double *te_host) {
+    LevelState &fine = level(lev);
     const double time = fine.t;
     LevelState *crse = nullptr;
     if (flags & kTeCompute) {
@@ -1845,7 +1963,7 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     // at its time), then the same-level ghosts.
     if (has_cf(lev))
         cf_fill_state(lev, time, fine.U[fine.cur].get(),
-                      comp() ? fine.Ulo[fine.cur].get() : nullptr);
+                      comp() ? fine.Ulo[fine.cur].get() : nullptr, {}, false, tb.cf_rec);
     halo(fine, fine.U[fine.cur].get(), all_comps(), /*full=*/true);
     if (comp()) halo(fine, fine.Ulo[fine.cur].get(), all_comps(), true, (int)losize());
     fine.ghosts_ok[fine.cur] = false;
@@ -1853,28 +1971,31 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     std::vector<double> ptag, pmod;
     sim_->SetParamsTagCellForRefinement(ptag, time, lev);
     if (S.shadow_hierarchy) sim_->SetParamsTruncationModifier(pmod, time, lev);
-    DevMem &dcrit = te_crit_, &dptag = te_ptag_, &dpmod = te_pmod_, &dcounts = te_counts_;
-    upload_into(dcrit, S.te_crit);
     ptag.resize(std::max<size_t>(ptag.size(), 1));
     pmod.resize(std::max<size_t>(pmod.size(), 1));
-    upload_into(dptag, ptag);
-    upload_into(dpmod, pmod);
-    // The mask stays with the level for the local regrid.
-    if (flags & kTeTags) {
-        if (fine.tags.bytes() != (size_t)std::max<long long>(1, fine.L.nvalid()))
-            fine.tags.alloc(std::max<long long>(1, fine.L.nvalid()));
-        fine.tags_t = -DBL_MAX;
-    } else if (te_mask_.bytes() < (size_t)std::max<long long>(1, fine.L.nvalid())) {
-        te_mask_.alloc(std::max<long long>(1, fine.L.nvalid()));
+    const double *dptag, *dpmod;
+    if (tb.ptag) {
+        upload_into(*tb.ptag, ptag);
+        upload_into(*tb.pmod, pmod);
+        dptag = tb.ptag->as<double>();
+        dpmod = tb.pmod->as<double>();
+    } else {
+        if ((int)ptag.size() > kScratchWords || (int)pmod.size() > kScratchWords)
+            throw Error(4, "too many tagging parameters for the allocation-free tagging");
+        SHM_HIP(hipMemcpy(tb.ptag_dev, ptag.data(), sizeof(double) * ptag.size(),
+                          hipMemcpyHostToDevice));
+        SHM_HIP(hipMemcpy(tb.pmod_dev, pmod.data(), sizeof(double) * pmod.size(),
+                          hipMemcpyHostToDevice));
+        dptag = tb.ptag_dev;
+        dpmod = tb.pmod_dev;
     }
+    if (tb.keep_tags) fine.tags_t = -DBL_MAX;
     // Every valid cell's mask byte is written; the counters are summed
     // atomically, so they start at zero.
-    DevMem &mask_mem = (flags & kTeTags) ? fine.tags : te_mask_;
-    upload_into(dcounts, std::vector<unsigned long long>(2 + ncomp_, 0ull));
+    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, stream_, tb.counts,
+                       2 + ncomp_);
+    SHM_HIP(hipGetLastError());
     const long long total = fine.L.cprefix.back();
-    const bool want_te = te_host && (flags & kTeCompute);
-    if (want_te && te_out_.bytes() < sizeof(double) * total * ncomp_)
-        te_out_.alloc(sizeof(double) * total * ncomp_);
 
     TeTagArgs a{};
     a.fboxes = fine.L.dbox.as<BoxDesc>();
@@ -1890,12 +2011,12 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
     a.nbox = (int)fine.L.boxes.size();
     a.total = total;
     a.ncomp = ncomp_;
-    a.te_out = want_te ? te_out_.as<double>() : nullptr;
-    a.mask = mask_mem.as<uint8_t>();
-    a.counts = dcounts.as<unsigned long long>();
-    a.te_crit = dcrit.as<double>();
-    a.params_tag = dptag.as<double>();
-    a.params_mod = dpmod.as<double>();
+    a.te_out = tb.te;
+    a.mask = tb.mask;
+    a.counts = tb.counts;
+    a.te_crit = tb.crit;
+    a.params_tag = dptag;
+    a.params_mod = dpmod;
     a.lev = lev;
     a.time = time;
     a.dt = sim_->dt[lev];
@@ -1914,19 +2035,19 @@ void Runtime::te_tag(int lev, int flags, uint8_t *mask, long long *counts,
                    fine.mtot_out);
         cl.ghosts_ok[cl.cur] = false;
     }
-    if (flags & kTeTags) fine.tags_t = time;
+    if (tb.keep_tags && (flags & kTeTags)) fine.tags_t = time;
     if (mask && (flags & kTeTags))
-        SHM_HIP(hipMemcpyAsync(mask, mask_mem.get(), fine.L.nvalid(),
-                               hipMemcpyDeviceToHost, stream_));
-    std::vector<unsigned long long> hc(2 + ncomp_);
-    SHM_HIP(hipMemcpyAsync(hc.data(), dcounts.get(), hc.size() * 8,
+        SHM_HIP(hipMemcpyAsync(mask, tb.mask, fine.L.nvalid(), hipMemcpyDeviceToHost,
+                               stream_));
+    unsigned long long hc[kScratchWords];
+    SHM_HIP(hipMemcpyAsync(hc, tb.counts, sizeof(unsigned long long) * (2 + ncomp_),
                            hipMemcpyDeviceToHost, stream_));
-    if (want_te)
-        SHM_HIP(hipMemcpyAsync(te_host, te_out_.get(), sizeof(double) * total * ncomp_,
+    if (te_host && tb.te)
+        SHM_HIP(hipMemcpyAsync(te_host, tb.te, sizeof(double) * total * ncomp_,
                                hipMemcpyDeviceToHost, stream_));
     SHM_HIP(hipStreamSynchronize(stream_));
     if (counts)
-        for (size_t i = 0; i < hc.size(); ++i) counts[i] = (long long)hc[i];
+        for (int i = 0; i < 2 + ncomp_; ++i) counts[i] = (long long)hc[i];
 }
 
 // ---------------------------------------------------------------------------

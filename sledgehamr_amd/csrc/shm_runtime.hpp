@@ -21,6 +21,20 @@
 
 namespace shm {
 
+/** The per-GPU device scratch array (shm_te_tag_noalloc, level norms):
+ *  fixed regions at its head, then the tag mask of the allocation-free
+ *  tagging. */
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kScratchWords = 64;              // words per head region
+constexpr size_t kScratchCountsOff = 0;        // tag counters (u64)
+constexpr size_t kScratchCritOff = 512;        // te_crit (double)
+constexpr size_t kScratchTagPrmOff = 1024;     // TagCellForRefinement params
+constexpr size_t kScratchModPrmOff = 1536;     // TruncationModifier params
+constexpr size_t kScratchNormOff = 2048;       // level-norm maxima (u64 bits)
+constexpr size_t kScratchRecOff = 2560;        // coarse/fine fill record
+constexpr size_t kScratchMaskOff = 4096;       // tag mask, one byte per cell
+constexpr long long kScratchMaxCells = (long long)(kScratchBytes - kScratchMaskOff);
+
 struct Box3 {
     int lo[3], hi[3];  // inclusive, global index space of the level
     int n(int d) const { return hi[d] - lo[d] + 1; }
@@ -218,6 +232,10 @@ class Runtime {
     std::vector<double> level_norm0(int lev);
     void te_tag(int lev, int flags, uint8_t *mask, long long *counts,
                 double *te_host);
+    /** te_tag without device allocation: counters, parameters, the fill
+     *  record and the mask in the GPU's scratch array; the level's kept
+     *  mask (for the local regrid) is left as it is. */
+    void te_tag_noalloc(int lev, int flags, uint8_t *mask, long long *counts);
     /** GravitationalWaves::ComputeSpectrum with the project's spectrum
      *  modifier `modifier` (-1: default components, unmodified). */
     void gw_spectrum(int mode, int nbins, int zero_padding, int proj_type,
@@ -346,8 +364,11 @@ class Runtime {
     bool has_cf(int lev) const { return lev >= 1 && has_level(lev - 1); }
     void build_cf(int lev);
     /** Arguments of a coarse/fine fill of level lev at `time` (layout,
-     *  coarse states and their time weights); the caller adds the target. */
-    CfArgs cf_args(int lev, double time, bool first_state_only = false, bool step = false);
+     *  coarse states and their time weights); the caller adds the target.
+     *  rec_dev: device memory that takes the fill record instead of the
+     *  record ring (the allocation-free tagging). */
+    CfArgs cf_args(int lev, double time, bool first_state_only = false, bool step = false,
+                   double *rec_dev = nullptr);
     /** Coarse/fine record of level lev at `time` (see CfArgs::rec). */
     void cf_record(double *rec, int lev, double time, bool first_state_only) const;
     /** A parameter record for the device: the next slot of the ring, or,
@@ -359,7 +380,8 @@ class Runtime {
     /** kCfState fill of the components `comps` (empty: all) of a state of
      *  level lev; lo null: the leading words only. */
     void cf_fill_state(int lev, double time, void *hi, void *lo,
-                       const std::vector<int> &comps = {}, bool step = false);
+                       const std::vector<int> &comps = {}, bool step = false,
+                       double *rec_dev = nullptr);
     /** Components whose coarse/fine ghost cells a level step reads: the
      *  stencil components; every component with dissipation, or for RKN,
      *  whose stage states at ghost cells compose phi0 + c pi0 + ... */
@@ -461,7 +483,7 @@ class Runtime {
     // Tagging's inputs and outputs, kept between calls (each call ends with
     // a stream synchronisation) so that tagging allocates nothing per call.
     DevMem te_crit_, te_ptag_, te_pmod_, te_mask_, te_counts_, te_out_;
-    DevMem norm0_out_;
+    unsigned char *scratch_ = nullptr;  // this GPU's scratch array
     /** GW spectrum work space, kept between calls: the D2Z plan and buffers
      *  for one padded size, the k tables for one (size, projection type,
      *  binning). */
@@ -474,6 +496,20 @@ class Runtime {
         int nbin_eff = 0;
     } gwc_;
     void release_gw_cache();
+    /** Device buffers one tagging writes: context buffers (te_tag) or
+     *  regions of the scratch array (te_tag_noalloc). */
+    struct TagBuffers {
+        uint8_t *mask = nullptr;
+        unsigned long long *counts = nullptr;
+        double *crit = nullptr;
+        double *te = nullptr;
+        DevMem *ptag = nullptr, *pmod = nullptr;  // context parameter buffers
+        double *ptag_dev = nullptr, *pmod_dev = nullptr;  // or fixed regions
+        double *cf_rec = nullptr;  // coarse/fine record target (null: ring)
+        bool keep_tags = false;    // the mask is the level's kept mask
+    };
+    void te_tag_run(int lev, int flags, TagBuffers &tb, uint8_t *mask,
+                    long long *counts, double *te_host);
     /** RKN steps of level lev whose first stage also writes the second
      *  stage's field states (first node 0, three or more stages, no
      *  coarse/fine ghosts). */
