@@ -31,7 +31,8 @@ enum shm_status {
     SHM_ERR_HIP = 2,          /* HIP / hipFFT runtime failure */
     SHM_ERR_STATE = 3,        /* call not valid in the current state */
     SHM_ERR_UNSUPPORTED = 4,  /* configuration not built / not supported */
-    SHM_ERR_INTERNAL = 5
+    SHM_ERR_INTERNAL = 5,
+    SHM_ERR_IO = 6            /* a file could not be read or written */
 };
 
 enum shm_precision {
@@ -122,6 +123,17 @@ int shm_evolve(shm_ctx *ctx, int ncoarse);
  * ntags_user, ntags_trunc[ncomp], sledgehamr.cpp:280-348); the layout
  * itself stays static.  counts holds 2 + ncomp values. */
 int shm_level_tag_counts(shm_ctx *ctx, int lev, long long *counts);
+
+/* The order of the level operations shm_evolve ran so far (the sub-cycled
+ * schedule of TimeStepper::Advance, time_stepper.cpp:90-171, 221-311):
+ * kinds[i] is 'S' (a regrid scheduled by levs[i] for times[i]), 'H' (shadow
+ * level created), 'A' (levs[i] advanced to times[i]), 'D' (levs[i]+1
+ * averaged onto levs[i]), 'T' (truncation errors and tags of levs[i]) or
+ * 'R' (regrid from levs[i]).  No counterpart in the reference (its log
+ * messages); for tests and tracing.  Arrays hold maxn entries; *n receives
+ * the number of events (call with NULL arrays to query it). */
+int shm_event_log(shm_ctx *ctx, char *kinds, int *levs, double *times, int maxn,
+                  int *n);
 
 /* Sledgehamr::CreateShadowLevel (sledgehamr.cpp:234-251). */
 int shm_create_shadow(shm_ctx *ctx);
@@ -292,6 +304,80 @@ int shm_get_timing(shm_ctx *ctx, double *stage_ms, long long *stage_launches,
                    long long *stage_cells, double *halo_ms,
                    long long *halo_launches, double *fused_ms,
                    long long *fused_launches);
+
+/* ----------------------------------------------------------------------
+ * HDF5 files.  libhdf5 (1.10 or newer) is loaded at run time: the file named
+ * by $SHM_HDF5_LIB, else libhdf5.so on the loader's path, else common install
+ * paths; when none loads these return SHM_ERR_UNSUPPORTED.  Missing files,
+ * datasets of the wrong size and failed writes return SHM_ERR_IO.
+ * ---------------------------------------------------------------------- */
+
+/* Version of the loaded libhdf5 (major, minor, release).  Needs no context;
+ * returns SHM_ERR_UNSUPPORTED when the library cannot be loaded. */
+int shm_hdf5_version(int *version3);
+
+/* FillLevel::FromInitialStateFile / FromHdf5File (fill_level.cpp:17-190) for
+ * level lev, which must be defined.  path NULL or "": input.initial_state.
+ * A folder holding Meta.hdf5 restarts from that checkpoint (as
+ * shm_read_checkpoint without output counters).  Otherwise, per component
+ * <name> (the project's scalars, then u_xx .. du_yz): the file
+ * input.initial_state_<name>, else path; no file: zeros.  In that file,
+ * datasets <name>_<b> / data_<b> are per-box chunks (b = position in the box
+ * table; box-relative ind = i ly lz + j lz + k), else <name> / data is the
+ * whole level (ind = i N^2 + j N + k), else the component is set to zero.
+ * A folder path without Meta.hdf5 holds chunk files <name>_<b>.hdf5 with
+ * dataset <name> or data (the reference reads these and then leaves the
+ * level unchanged, fill_level.cpp:93-99; here they are applied).
+ * input.upsample > 1 returns SHM_ERR_UNSUPPORTED. */
+int shm_read_initial_state(shm_ctx *ctx, int lev, const char *path);
+
+/* Checkpoint::Write (output_types/checkpoint.cpp:18-98) into folder:
+ * Meta.hdf5 with the reference's datasets (Header = t, 1, finest level, N,
+ * nghost, ncomp, noutput, idx_checkpoints; blocking_factors; isteps; next_id
+ * and last_time_written of the noutput output types, entry idx_checkpoints
+ * counted as written), and per level Level_<lev>/Cell.hdf5 (in place of
+ * AMReX's MultiFab files): boxes (lo, hi), level (t, istep, N) and one FP64
+ * dataset per component, the valid cells box after box, x fastest.  next_id
+ * and last_time_written may be NULL when noutput is 0. */
+int shm_write_checkpoint(shm_ctx *ctx, const char *folder, const int *next_id,
+                         const double *last_time_written, int noutput,
+                         int idx_checkpoints);
+
+/* Checkpoint::Read + UpdateLevels + UpdateOutputModules (checkpoint.cpp:
+ * 122-187, 195-275): defines levels 0..min(finest, max level) with the
+ * checkpoint's boxes and data, every level at the checkpoint's time with its
+ * own step count; a blocking factor smaller than the inputs' marks a global
+ * regrid as wanted.  The component count must match (else SHM_ERR_ARG), as
+ * must noutput / idx_checkpoints when next_id is not NULL; next_id and
+ * last_time_written then receive the stored counters. */
+int shm_read_checkpoint(shm_ctx *ctx, const char *folder, int *next_id,
+                        double *last_time_written, int noutput,
+                        int idx_checkpoints);
+
+/* Checkpoint::Delete (checkpoint.cpp:277-308): removes the files of a
+ * checkpoint written by shm_write_checkpoint, then its folders if empty. */
+int shm_delete_checkpoint(shm_ctx *ctx, const char *folder);
+
+/* IOModule::WriteGravitationalWaveSpectrum (io_module.cpp:334-353) into
+ * prefix/spectra.hdf5: Header = t, N zp, number of bins, L zp, zp, unbinned;
+ * k (|k|^2 per bin, or with output.gw_spectra.log_bins = n > 0 the lower
+ * |k| edge of n log bins, plus a dataset log_bins); Spectrum.  Settings from
+ * output.gw_spectra.{projection_type, zero_padding_factor, unbinned}.
+ * modifier / params as shm_gw_spectrum_mod. */
+int shm_write_gw_spectrum(shm_ctx *ctx, const char *prefix, int modifier,
+                          const double *params, int nparams);
+
+/* IOModule::WriteSpectra (io_module.cpp:307-327, spectrum.cpp:101-112) into
+ * prefix/spectra.hdf5: Header = t, N, number of bins; k_sq; one dataset per
+ * output function fns[0..nfn), named after it. */
+int shm_write_spectra(shm_ctx *ctx, const char *prefix, const int *fns, int nfn);
+
+/* IOModule::WriteProjections (io_module.cpp:280-301, projection.cpp:76-86)
+ * into prefix/projections.hdf5: Header = t, N of the projected level;
+ * <name>_data and <name>_n per output function, with modes[i] (NULL: 0) and
+ * the level limit output.projections.max_level. */
+int shm_write_projections(shm_ctx *ctx, const char *prefix, const int *fns,
+                          const int *modes, int nfn);
 
 #ifdef __cplusplus
 }

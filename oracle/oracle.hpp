@@ -1845,6 +1845,7 @@ class Sim {
         }
         auto mi = [N0](double d) { return d - N0 * std::floor(d / N0 + 0.5); };
         for (auto &f : m.fabs)
+#pragma omp parallel for schedule(static)
             for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
                 for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
                     for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i) {
@@ -1888,6 +1889,7 @@ class Sim {
         MultiFab &m = grid_new[lev];
         const int N = dimN[lev];
         for (auto &f : m.fabs)
+#pragma omp parallel for schedule(static)
             for (int k = f.box.lo[2]; k <= f.box.hi[2]; ++k)
                 for (int j = f.box.lo[1]; j <= f.box.hi[1]; ++j)
                     for (int i = f.box.lo[0]; i <= f.box.hi[0]; ++i) {

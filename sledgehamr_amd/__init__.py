@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Simulation", "library_path", "PROJECTS", "ShmError",
+__all__ = ["Simulation", "library_path", "hdf5_version", "PROJECTS", "ShmError",
            "PREC_FP64", "PREC_FP32C", "PREC_FP32C16", "PREC_FP32CS", "TE_COMPUTE",
            "TE_AVGDOWN", "TE_TAGS"]
 
@@ -30,14 +30,28 @@ EXPORTS = (
     "shm_init_state", "shm_fill_rhs", "shm_download_rhs", "shm_advance_level",
     "shm_evolve", "shm_level_tag_counts", "shm_create_shadow", "shm_te_tag", "shm_gw_spectrum",
     "shm_gw_spectrum_mod", "shm_gw_modifiers", "shm_gw_modifier_name",
-    "shm_te_tag_noalloc",
+    "shm_te_tag_noalloc", "shm_event_log",
     "shm_output_functions", "shm_output_function_name", "shm_spectrum", "shm_projection",
     "shm_upload_array", "shm_fill_coarse_patch",
     "shm_regrid_select", "shm_add_boxes", "shm_remake_level", "shm_local_regrid",
     "shm_regrid_stats",
     "shm_sync", "shm_stream_probe", "shm_autotune", "shm_set_launch",
     "shm_get_launch", "shm_set_graphs", "shm_set_timing", "shm_get_timing",
+    "shm_hdf5_version", "shm_read_initial_state", "shm_write_checkpoint",
+    "shm_read_checkpoint", "shm_delete_checkpoint", "shm_write_gw_spectrum",
+    "shm_write_spectra", "shm_write_projections",
 )
+
+
+def hdf5_version(project: str = "RealScalar"):
+    """(major, minor, release) of the libhdf5 the library loads for its
+    file entry points; raises ShmError when none can be loaded."""
+    lib = load_library(project)
+    v = (C.c_int * 3)()
+    rc = lib.shm_hdf5_version(v)
+    if rc:
+        raise ShmError(rc, "libhdf5 could not be loaded (set SHM_HDF5_LIB)")
+    return tuple(v)
 
 
 class ShmError(RuntimeError):
@@ -105,6 +119,7 @@ def load_library(project: str) -> C.CDLL:
         "shm_level_tag_counts": (i, [vp, i, llp]),
         "shm_te_tag": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp, dp]),
         "shm_te_tag_noalloc": (i, [vp, i, i, C.POINTER(C.c_ubyte), llp]),
+        "shm_event_log": (i, [vp, C.c_char_p, ip, dp, i, ip]),
         "shm_gw_spectrum": (i, [vp, i, i, i, i, dp, dp, ip]),
         "shm_gw_spectrum_mod": (i, [vp, i, i, i, i, i, dp, i, dp, dp, ip]),
         "shm_gw_modifiers": (i, [vp, ip]),
@@ -128,6 +143,14 @@ def load_library(project: str) -> C.CDLL:
         "shm_set_graphs": (i, [vp, i]),
         "shm_set_timing": (i, [vp, i]),
         "shm_get_timing": (i, [vp, dp, llp, llp, dp, llp, dp, llp]),
+        "shm_hdf5_version": (i, [ip]),
+        "shm_read_initial_state": (i, [vp, i, C.c_char_p]),
+        "shm_write_checkpoint": (i, [vp, C.c_char_p, ip, dp, i, i]),
+        "shm_read_checkpoint": (i, [vp, C.c_char_p, ip, dp, i, i]),
+        "shm_delete_checkpoint": (i, [vp, C.c_char_p]),
+        "shm_write_gw_spectrum": (i, [vp, C.c_char_p, i, dp, i]),
+        "shm_write_spectra": (i, [vp, C.c_char_p, ip, i]),
+        "shm_write_projections": (i, [vp, C.c_char_p, ip, ip, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -286,6 +309,19 @@ class Simulation:
             _dptr(te) if te is not None else None))
         return mask, counts, te
 
+    def events(self) -> list:
+        """Level operations of the evolves so far: (kind, lev, t) with kind
+        S, H, A, D, T or R (see shm_event_log)."""
+        n = C.c_int(0)
+        self._check(self.lib.shm_event_log(self._ctx, None, None, None, 0, C.byref(n)))
+        m = max(n.value, 1)
+        k = C.create_string_buffer(m)
+        lv = np.zeros(m, dtype=np.int32)
+        t = np.zeros(m)
+        self._check(self.lib.shm_event_log(self._ctx, k, lv.ctypes.data_as(C.POINTER(C.c_int)),
+                                           _dptr(t), n.value, C.byref(n)))
+        return [(k.raw[i:i + 1].decode(), int(lv[i]), float(t[i])) for i in range(n.value)]
+
     def te_tag_noalloc(self, lev: int, flags: int = TE_COMPUTE | TE_TAGS):
         """shm_te_tag_noalloc: the tagging without device allocation (mask,
         counters)."""
@@ -425,6 +461,59 @@ class Simulation:
             self._ctx, fn, mode, max_level, _dptr(data),
             cnt.ctypes.data_as(C.POINTER(C.c_int)), C.byref(D)))
         return data.reshape(D.value, D.value), cnt.reshape(D.value, D.value)
+
+    # HDF5 files (libhdf5 loaded at run time by the library).
+    def read_initial_state(self, lev: int = 0, path: str | None = None) -> None:
+        """FillLevel::FromInitialStateFile for level lev (path None:
+        input.initial_state); a checkpoint folder restarts every level."""
+        self._check(self.lib.shm_read_initial_state(
+            self._ctx, lev, path.encode() if path else None))
+
+    def write_checkpoint(self, folder: str, next_id=(), last_time_written=(),
+                         idx_checkpoints: int = 0) -> None:
+        nid = np.ascontiguousarray(np.asarray(next_id, dtype=np.int32))
+        lt = np.ascontiguousarray(np.asarray(last_time_written, dtype=np.float64))
+        if nid.size != lt.size:
+            raise ValueError("next_id and last_time_written differ in length")
+        self._check(self.lib.shm_write_checkpoint(
+            self._ctx, folder.encode(),
+            nid.ctypes.data_as(C.POINTER(C.c_int)) if nid.size else None,
+            _dptr(lt) if lt.size else None, nid.size, idx_checkpoints))
+
+    def read_checkpoint(self, folder: str, noutput: int | None = None,
+                        idx_checkpoints: int = 0):
+        """Restart from a checkpoint; with noutput, returns the stored
+        (next_id, last_time_written) of the output types."""
+        if noutput is None:
+            self._check(self.lib.shm_read_checkpoint(self._ctx, folder.encode(), None,
+                                                     None, 0, 0))
+            return None
+        nid = np.zeros(max(noutput, 1), dtype=np.int32)
+        lt = np.zeros(max(noutput, 1))
+        self._check(self.lib.shm_read_checkpoint(
+            self._ctx, folder.encode(), nid.ctypes.data_as(C.POINTER(C.c_int)),
+            _dptr(lt), noutput, idx_checkpoints))
+        return nid[:noutput], lt[:noutput]
+
+    def delete_checkpoint(self, folder: str) -> None:
+        self._check(self.lib.shm_delete_checkpoint(self._ctx, folder.encode()))
+
+    def write_gw_spectrum(self, prefix: str, modifier: int = -1, params=()) -> None:
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        self._check(self.lib.shm_write_gw_spectrum(
+            self._ctx, prefix.encode(), modifier, _dptr(p) if p.size else None, p.size))
+
+    def write_spectra(self, prefix: str, fns) -> None:
+        f = np.ascontiguousarray(np.asarray(fns, dtype=np.int32))
+        self._check(self.lib.shm_write_spectra(
+            self._ctx, prefix.encode(), f.ctypes.data_as(C.POINTER(C.c_int)), f.size))
+
+    def write_projections(self, prefix: str, fns, modes=None) -> None:
+        f = np.ascontiguousarray(np.asarray(fns, dtype=np.int32))
+        m = None if modes is None else np.ascontiguousarray(np.asarray(modes, dtype=np.int32))
+        self._check(self.lib.shm_write_projections(
+            self._ctx, prefix.encode(), f.ctypes.data_as(C.POINTER(C.c_int)),
+            m.ctypes.data_as(C.POINTER(C.c_int)) if m is not None else None, f.size))
 
     def sync(self) -> None:
         self._check(self.lib.shm_sync(self._ctx))

@@ -55,7 +55,7 @@ COMMON_FLAGS = [
     "-I", INCLUDE, "-I", os.path.join(INCLUDE, "sledgehamr"), "-I", CSRC,
 ]
 
-RUNTIME_SOURCES = ["shm_runtime.cpp", "shm_capi.cpp"]
+RUNTIME_SOURCES = ["shm_runtime.cpp", "shm_capi.cpp", "shm_io.cpp"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -112,7 +112,7 @@ def _link(p: Project, pobjs: list[str], robjs: list[str]) -> str:
     out = os.path.join(LIB, f"libshm_{p.name}.so")
     tmp = out + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *pobjs, *robjs,
-          "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
+          "-L/opt/rocm/lib", "-lhipfft", "-ldl", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, out)
     return out
 

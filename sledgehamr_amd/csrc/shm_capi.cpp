@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 
+#include "shm_hdf5.hpp"
 #include "shm_runtime.hpp"
 
 extern "C" sledgehamr::Sledgehamr *shm_project_factory();
@@ -262,6 +263,126 @@ __attribute__((visibility("default"))) int shm_te_tag(shm_ctx *ctx, int lev,
     return guard(ctx, [&] {
         SHM_NEED_RT;
         ctx->rt->te_tag(lev, flags, mask, counts, te);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_event_log(shm_ctx *ctx, char *kinds,
+                                                         int *levs, double *times,
+                                                         int maxn, int *n) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!n) throw shm::Error(SHM_ERR_ARG, "null n");
+        const auto &ev = ctx->rt->events_;
+        *n = (int)ev.size();
+        if (kinds && levs && times)
+            for (int i = 0; i < (int)ev.size() && i < maxn; ++i) {
+                kinds[i] = ev[i].kind;
+                levs[i] = ev[i].lev;
+                times[i] = ev[i].t;
+            }
+    });
+}
+
+__attribute__((visibility("default"))) int shm_hdf5_version(int *version3) {
+    if (!version3) return SHM_ERR_ARG;
+    try {
+        const shm::h5::Api &a = shm::h5::api();
+        for (int i = 0; i < 3; ++i) version3[i] = (int)a.version[i];
+        return SHM_OK;
+    } catch (const shm::Error &e) {
+        return e.code;
+    } catch (...) {
+        return SHM_ERR_INTERNAL;
+    }
+}
+
+__attribute__((visibility("default"))) int shm_read_initial_state(shm_ctx *ctx, int lev,
+                                                                  const char *path) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        ctx->rt->read_initial_state(lev, path ? path : "");
+    });
+}
+
+__attribute__((visibility("default"))) int shm_write_checkpoint(
+    shm_ctx *ctx, const char *folder, const int *next_id, const double *last_time_written,
+    int noutput, int idx_checkpoints) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!folder) throw shm::Error(SHM_ERR_ARG, "null folder");
+        if (noutput < 0 || (noutput > 0 && (!next_id || !last_time_written)))
+            throw shm::Error(SHM_ERR_ARG, "output counters missing");
+        std::vector<int> nid(next_id, next_id + noutput);
+        std::vector<double> lt(last_time_written, last_time_written + noutput);
+        ctx->rt->write_checkpoint(folder, nid, lt, idx_checkpoints);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_read_checkpoint(
+    shm_ctx *ctx, const char *folder, int *next_id, double *last_time_written, int noutput,
+    int idx_checkpoints) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!folder) throw shm::Error(SHM_ERR_ARG, "null folder");
+        const bool want = next_id != nullptr;
+        if (want && noutput > 0 && !last_time_written)
+            throw shm::Error(SHM_ERR_ARG, "null last_time_written");
+        std::vector<int> nid;
+        std::vector<double> lt;
+        ctx->rt->read_checkpoint(folder, want ? noutput : -1, idx_checkpoints, nid, lt);
+        if (want)
+            for (int i = 0; i < noutput; ++i) {
+                next_id[i] = nid[i];
+                last_time_written[i] = lt[i];
+            }
+    });
+}
+
+__attribute__((visibility("default"))) int shm_delete_checkpoint(shm_ctx *ctx,
+                                                                 const char *folder) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!folder) throw shm::Error(SHM_ERR_ARG, "null folder");
+        ctx->rt->delete_checkpoint(folder);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_write_gw_spectrum(shm_ctx *ctx,
+                                                                 const char *prefix,
+                                                                 int modifier,
+                                                                 const double *params,
+                                                                 int nparams) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!prefix) throw shm::Error(SHM_ERR_ARG, "null prefix");
+        if (nparams < 0 || (nparams > 0 && !params))
+            throw shm::Error(SHM_ERR_ARG, "modifier parameters missing");
+        std::vector<double> mp(params, params + nparams);
+        ctx->rt->write_gw_spectrum_file(prefix, modifier, mp);
+    });
+}
+
+__attribute__((visibility("default"))) int shm_write_spectra(shm_ctx *ctx, const char *prefix,
+                                                             const int *fns, int nfn) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!prefix) throw shm::Error(SHM_ERR_ARG, "null prefix");
+        if (nfn < 0 || (nfn > 0 && !fns)) throw shm::Error(SHM_ERR_ARG, "null fns");
+        ctx->rt->write_spectra_file(prefix, std::vector<int>(fns, fns + nfn));
+    });
+}
+
+__attribute__((visibility("default"))) int shm_write_projections(shm_ctx *ctx,
+                                                                 const char *prefix,
+                                                                 const int *fns,
+                                                                 const int *modes, int nfn) {
+    return guard(ctx, [&] {
+        SHM_NEED_RT;
+        if (!prefix) throw shm::Error(SHM_ERR_ARG, "null prefix");
+        if (nfn < 0 || (nfn > 0 && !fns)) throw shm::Error(SHM_ERR_ARG, "null fns");
+        std::vector<int> m;
+        if (modes) m.assign(modes, modes + nfn);
+        ctx->rt->write_projections_file(prefix, std::vector<int>(fns, fns + nfn), m);
     });
 }
 

@@ -516,10 +516,19 @@ void Runtime::init_from_inputs(const std::string &inputs,
 
     pp.query("amr.nghost", S.nghost);
     pp.get("amr.coarse_level_grid_size", S.coarse_level_grid_size);
-    if (!sledgehamr::utils::IsPowerOfTwo(S.coarse_level_grid_size))
-        throw Error(1, "amr.coarse_level_grid_size needs to be a power of 2!");
     S.max_level = 0;
     pp.query("amr.max_refinement_levels", S.max_level);
+    // The reference requires a power of two (sledgehamr.cpp:438-444).
+    // shm.allow_any_grid_size lifts that for any even size (the shadow
+    // level halves the coarse level), e.g. a 384^3 level.
+    bool any_size = false;
+    pp.query("shm.allow_any_grid_size", any_size);
+    if (any_size) {
+        if (S.coarse_level_grid_size < 2 || (S.coarse_level_grid_size & 1))
+            throw Error(1, "amr.coarse_level_grid_size needs to be even");
+    } else if (!sledgehamr::utils::IsPowerOfTwo(S.coarse_level_grid_size)) {
+        throw Error(1, "amr.coarse_level_grid_size needs to be a power of 2!");
+    }
     pp.get("sim.t_start", S.t_start);
     pp.get("sim.t_end", S.t_end);
     pp.get("sim.L", S.L);
@@ -542,7 +551,8 @@ void Runtime::init_from_inputs(const std::string &inputs,
     ncomp_ = gw_ ? kt_.ngw : kt_.nscalars;
     if ((int)S.scalar_fields.size() != kt_.nscalars)
         throw Error(3, "scalar field registry does not match the kernel table");
-    std::vector<std::string> names;
+    std::vector<std::string> &names = names_;
+    names.clear();
     for (auto *f : S.scalar_fields) names.push_back(f->name);
     if (gw_)
         for (const char *n : {"u_xx", "u_yy", "u_zz", "u_xy", "u_xz", "u_yz",
@@ -2973,11 +2983,16 @@ void Runtime::scheduled_step(int lev) {
                 schedule_.emplace_back(lev, tnext);
             else
                 schedule_[id].first = std::min(schedule_[id].first, lev);
-            if (lev == 0) create_shadow_level();
+            events_.push_back({'S', lev, tnext});
+            if (lev == 0) {
+                create_shadow_level();
+                events_.push_back({'H', -1, shadow_.t});
+            }
         }
     }
     if (lev == 0 && !shadow_.defined()) S.BeforeTimestep(ls.t);
     advance(lev);
+    events_.push_back({'A', lev, ls.t});
     if (lev < S.finest_level) {
         scheduled_step(lev + 1);
         scheduled_step(lev + 1);
@@ -2992,6 +3007,7 @@ void Runtime::scheduled_step(int lev) {
     // local regrid's counters are left as they are (no regrid ran).
     for (size_t i = 0; i < schedule_.size(); ++i) {
         if (schedule_[i].second == ls.t && schedule_[i].first == lev) {
+            events_.push_back({'R', lev, ls.t});
             const bool global = local_regrid_on_ && local_regrid(lev) == 0;
             for (int l = lev; l <= S.finest_level; ++l) last_regrid_[l] = ls.t;
             schedule_.erase(schedule_.begin() + i);
@@ -3026,9 +3042,11 @@ void Runtime::synchronize_levels(int lev) {
             copy_state(ls.U[ls.cur].get(), ls.Ulo[ls.cur].get(), ls.L, m.U[0].get(),
                        m.Ulo[0].get(), m.L, fine.mplan_out, fine.mnseg_out, fine.mtot_out);
             ls.ghosts_ok[ls.cur] = false;
+            events_.push_back({'D', lev, ls.t});
         }
     }
     if (lev >= 1 - (S.shadow_hierarchy ? 1 : 0) && need_te) {
+        events_.push_back({'T', lev, ls.t});
         std::vector<long long> counts(2 + ncomp_, 0);
         te_tag(lev, kTeCompute | kTeAvgDown | kTeTags, nullptr, counts.data(), nullptr);
         last_tags_[lev] = counts;

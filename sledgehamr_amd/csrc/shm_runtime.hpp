@@ -484,6 +484,20 @@ class Runtime {
     // a stream synchronisation) so that tagging allocates nothing per call.
     DevMem te_crit_, te_ptag_, te_pmod_, te_mask_, te_counts_, te_out_;
     unsigned char *scratch_ = nullptr;  // this GPU's scratch array
+
+  public:
+    /** Order of the level operations of shm_evolve: 'S' a regrid scheduled
+     *  by lev for time t, 'H' shadow level created, 'A' lev advanced to t,
+     *  'D' lev+1 averaged onto lev, 'T' truncation errors and tags of lev,
+     *  'R' regrid from lev at t. */
+    struct Event {
+        char kind;
+        int lev;
+        double t;
+    };
+    std::vector<Event> events_;
+
+  private:
     /** GW spectrum work space, kept between calls: the D2Z plan and buffers
      *  for one padded size, the k tables for one (size, projection type,
      *  binning). */
@@ -496,6 +510,36 @@ class Runtime {
         int nbin_eff = 0;
     } gwc_;
     void release_gw_cache();
+    // Component names: the project's scalars, then the GW tensor.
+    std::vector<std::string> names_;
+
+  public:
+    // HDF5 files (shm_io.cpp).
+    /** FillLevel::FromInitialStateFile (fill_level.cpp:17-30): a checkpoint
+     *  folder restarts every level, any other path is read as initial-state
+     *  data for level lev. */
+    void read_initial_state(int lev, const std::string &path);
+    /** Checkpoint::Write (output_types/checkpoint.cpp:18-98). */
+    void write_checkpoint(const std::string &folder, const std::vector<int> &next_id,
+                          const std::vector<double> &last_time, int idx_checkpoints);
+    /** Checkpoint::Read + UpdateLevels + UpdateOutputModules
+     *  (checkpoint.cpp:122-187, 205-275). */
+    void read_checkpoint(const std::string &folder, int noutput, int idx_checkpoints,
+                         std::vector<int> &next_id, std::vector<double> &last_time);
+    /** Checkpoint::Delete (checkpoint.cpp:277-308). */
+    void delete_checkpoint(const std::string &folder);
+    /** IOModule::WriteGravitationalWaveSpectrum (io_module.cpp:334-353). */
+    void write_gw_spectrum_file(const std::string &prefix, int modifier,
+                                const std::vector<double> &mparams);
+    /** IOModule::WriteSpectra (io_module.cpp:307-327). */
+    void write_spectra_file(const std::string &prefix, const std::vector<int> &fns);
+    /** IOModule::WriteProjections (io_module.cpp:280-301). */
+    void write_projections_file(const std::string &prefix, const std::vector<int> &fns,
+                                const std::vector<int> &modes);
+    const std::vector<std::string> &comp_names() const { return names_; }
+
+  private:
+    void read_hdf5_level(int lev, const std::string &path);
     /** Device buffers one tagging writes: context buffers (te_tag) or
      *  regions of the scratch array (te_tag_noalloc). */
     struct TagBuffers {

@@ -183,6 +183,41 @@ def amr_batched_boxes(seed, n1=2000, base=256, margin=6):
     return lev1, lev2
 
 
+def amr_batched_boxes_wide(seed, base=256, margin=6, frac=0.6, split=0.8):
+    """Static nested layout of the batched AMR workload with level-1 edges
+    16..64.  The level-1 domain (2 base cells per axis) is cut into 64-cell
+    slots; a seeded fraction `frac` of them is used, each either holding one
+    box of even edge 34..64 or (with probability `split`) cut into eight
+    32-cell sub-slots holding one box of even edge 16..32 each, at even
+    offsets.  Level 2 holds one box per level-1 box, even edge 16 ..
+    min(64, 2 e1 - 2 margin), at least `margin` level-2 cells inside it
+    (margin 6 serves 3 ghost cells).  About 2000 boxes per fine level at
+    base 256."""
+    rng = np.random.default_rng(seed)
+    ns = 2 * base // 64
+    lev1, lev2 = [], []
+    slots = sorted(rng.choice(ns ** 3, int(frac * ns ** 3), replace=False))
+    for s in slots:
+        org = [64 * (s % ns), 64 * ((s // ns) % ns), 64 * (s // (ns * ns))]
+        if rng.random() < split:
+            subs = [(org[0] + 32 * a, org[1] + 32 * b, org[2] + 32 * c)
+                    for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+            cell, elo, ehi = 32, 16, 32
+        else:
+            subs = [tuple(org)]
+            cell, elo, ehi = 64, 34, 64
+        for sub in subs:
+            e1 = 2 * int(rng.integers(elo // 2, ehi // 2 + 1))
+            lo = [v + 2 * int(rng.integers(0, (cell - e1) // 2 + 1)) for v in sub]
+            lev1.append([*lo, *(v + e1 - 1 for v in lo)])
+            emax = min(64, 2 * e1 - 2 * margin)
+            e2 = 2 * int(rng.integers(8, emax // 2 + 1))
+            room = 2 * e1 - 2 * margin - e2
+            lo2 = [2 * v + margin + 2 * int(rng.integers(0, room // 2 + 1)) for v in lo]
+            lev2.append([*lo2, *(v + e2 - 1 for v in lo2)])
+    return lev1, lev2
+
+
 # ---------------------------------------------------------------------------
 # Tagging with user criteria and truncation-error modifiers (ComplexScalar
 # with project.tag_strings / project.te_modifier).

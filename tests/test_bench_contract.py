@@ -22,47 +22,62 @@ def test_more_than_one_gpu_is_refused():
 
 def test_fused_schedule_roofline():
     """512^3, 20 steps of one fused stage-1+2 launch (1.5 ms) and one
-    stage-3 launch (1.7 ms): stage 3 is the dominant kernel."""
+    stage-3 launch (1.7 ms): stage 3 is the dominant kernel, priced at what
+    it must move."""
     n, steps = 512 ** 3, 20
     tm = {"stage_launches": 2 * steps, "fused_launches": steps,
           "stage_cells": 3 * n * steps, "stage_ms": steps * (1.5 + 1.7),
           "fused_ms": steps * 1.5}
-    r = bench.kernel_roofline(tm, "fp32c16", True, 512)
+    launch = {"fused": 1, "fused_variant": 2, "variant": 1}
+    r = bench.headline_roofline(tm, "fp32c16", launch, n)
     assert r["kernel"].startswith("shm::stage_kernel")
     assert r["avg_launch_ms"] == pytest.approx(1.7)
-    # 4 components x (12 words of 4 B + a 2 B remainder read and written)
-    # per step, over 3 stages: 69.33 B per cell-stage.
-    alg = 4 * (12 * 4 + 2 * 2) / 3 * n
-    assert r["algorithmic_bytes_per_launch"] == pytest.approx(alg)
-    assert r["achieved"] == pytest.approx(alg / 1.7e-3 / 1e9)
-    assert r["frac"] == pytest.approx(r["achieved"] / 8000.0)
-    assert r["peak"] == 8000.0 and r["unit"] == "GB/s" and r["bound"] == "hbm"
     # Stage 3 reads u0 and k2 (4 B each) and u0's remainder (2 B), writes u
-    # (4 B) and its remainder (2 B): 16 B per component.
-    assert r["moved_bytes_per_launch"] == pytest.approx(4 * 16 * n)
+    # (4 B) and its remainder (2 B): 16 B per component, 4 components.
+    assert r["bytes_per_launch"] == pytest.approx(4 * 16 * n)
+    assert r["achieved"] == pytest.approx(4 * 16 * n / 1.7e-3 / 1e9)
+    assert r["frac"] == pytest.approx(r["achieved"] / 8000.0) and r["frac"] < 1
+    assert r["peak"] == 8000.0 and r["unit"] == "GB/s" and r["bound"] == "hbm"
     assert r["traffic"] == bench.TRAFFIC_BYTES_512["stage3"]
+    assert r["traffic_source"] == bench.TRAFFIC_PROFILE
     o = r["other_kernel"]
     assert o["avg_launch_ms"] == pytest.approx(1.5)
-    # Two cell-stages per cell; moves u0 in and k2 out (4 B each).
-    assert o["algorithmic_bytes_per_launch"] == pytest.approx(2 * alg)
-    assert o["moved_bytes_per_launch"] == pytest.approx(4 * 8 * n)
+    # Moves u0 in and k2 out (4 B each).
+    assert o["bytes_per_launch"] == pytest.approx(4 * 8 * n)
+    assert o["frac"] < 1
+    # Another launch shape than the profiled one: no traffic figure.
+    r2 = bench.headline_roofline(tm, "fp32c16", {"fused": 1, "fused_variant": 0,
+                                                 "variant": 4}, n)
+    assert r2["traffic"] is None
 
 
 def test_unfused_schedule_roofline():
     n = 64 ** 3
     tm = {"stage_launches": 30, "fused_launches": 0, "stage_cells": 30 * n,
           "stage_ms": 30 * 0.01, "fused_ms": 0.0}
-    r = bench.kernel_roofline(tm, "fp64", False, 64)
+    r = bench.headline_roofline(tm, "fp64", {"fused": 0}, n)
     assert r["avg_launch_ms"] == pytest.approx(0.01)
-    # 4 components x 12 words of 8 B per step over 3 stages: 128 B.
-    assert r["algorithmic_bytes_per_launch"] == pytest.approx(128 * n)
+    # Per component: stage 1 u0 in, k1 out; stage 2 u0, k1 in, k2 out;
+    # stage 3 u0, k2 in, u out: 8 words of 8 B over 3 launches.
+    assert r["bytes_per_launch"] == pytest.approx(4 * 8 * 8 * n / 3)
     assert r["traffic"] is None
 
 
-def test_ensemble_bytes():
+def test_compensated_stencil_bytes():
+    """With FP64 tiles the stages read u0's remainder wherever they read u0:
+    SSPRK3 stage one 4 + 2 + 4 B per component, stage three as FP32C16."""
+    assert bench.ssprk_bytes("fp32cs", "stage1", 4) == 4 * 10
+    assert bench.ssprk_bytes("fp32cs", "fused12", 4) == 4 * 10
+    assert bench.ssprk_bytes("fp32cs", "stage3", 4) == bench.ssprk_bytes("fp32c16", "stage3", 4)
+
+
+def test_rkn_bytes():
     """RKN4 on one field and its momentum: stage 1 reads 2 words and writes
     1, stage 2 reads 3 and writes 1, stage 3 reads 4 and writes 2; with FP32
-    + bf16 both remainders are read and written once (4 x 2 B)."""
+    + bf16 both remainders are read and written once (4 x 2 B); with FP64
+    tiles of the pair the first two stages also read both remainders."""
     words = (2 + 1) + (3 + 1) + (4 + 2)
-    assert bench.ENSEMBLE_BYTES_PER_CELL_STEP["fp32c16"] == words * 4 + 4 * 2
-    assert bench.ENSEMBLE_BYTES_PER_CELL_STEP["fp64"] == words * 8
+    assert bench.rkn_bytes_per_step("fp32c16", 1) == words * 4 + 4 * 2
+    assert bench.rkn_bytes_per_step("fp64", 1) == words * 8
+    assert bench.rkn_bytes_per_step("fp32cs", 1) == words * 4 + 4 * 2 + 4 * 2
+    assert bench.rkn_bytes_per_step("fp32c16", 7) == 7 * (words * 4 + 4 * 2)
