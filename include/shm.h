@@ -72,6 +72,8 @@ typedef struct shm_info_t {
     int shadow_hierarchy;
     double L, cfl, t_start, t_end;
     long long graph_replays;  /* level steps run from captured graphs */
+    int gw_groups;            /* GW stages run as three component groups
+                                 (shm.gw_split and the dependency probe) */
 } shm_info_t;
 
 /* Sledgehamr::Sledgehamr + InitSledgehamr (sledgehamr.cpp:15-77): build the

@@ -306,7 +306,10 @@ struct TileCtx {
 
 /** Probe context: records the reach of every component read by a cell
  *  function (run once per context to size halos and validate the tile
- *  assumptions).  Reads return a smooth positive value. */
+ *  assumptions).  Reads return a smooth positive value; with `hashed` set
+ *  they return values that vary with component and offset (in [0.75,
+ *  1.25)), plus pert[n] times a second such value, so that perturbing one
+ *  component shows which outputs read it. */
 struct ProbeCtx {
     static constexpr int kMaxComp = 64;
     int ci, cj, ck;
@@ -314,6 +317,18 @@ struct ProbeCtx {
     mutable int reach[kMaxComp];     // max |offset| along any axis
     mutable int offaxis[kMaxComp];   // read with more than one nonzero offset
     mutable int zreach[kMaxComp];    // max |dk|
+    int hashed = 0;
+    unsigned seed = 0;
+    double pert[kMaxComp];
+
+    AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE static double unit(unsigned h) {
+        h ^= h >> 16;
+        h *= 0x7feb352du;
+        h ^= h >> 15;
+        h *= 0x846ca68bu;
+        h ^= h >> 16;
+        return (double)(h >> 8) * (1.0 / 16777216.0);
+    }
 
     AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double get(int i, int j, int k,
                                                         int n) const {
@@ -327,6 +342,13 @@ struct ProbeCtx {
             if (ak > zreach[n]) zreach[n] = ak;
             int nz = (di != 0) + (dj != 0) + (dk != 0);
             if (nz > 1 && (dk != 0)) offaxis[n] = 1;
+        }
+        if (hashed && n >= 0 && n < kMaxComp) {
+            const unsigned key = seed ^ ((unsigned)n * 0x9e3779b1u) ^
+                                 ((unsigned)(di + 16) * 0x85ebca77u) ^
+                                 ((unsigned)(dj + 16) * 0xc2b2ae3du) ^
+                                 ((unsigned)(dk + 16) * 0x27d4eb2fu);
+            return 0.75 + 0.5 * unit(key) + pert[n] * (0.5 + unit(key ^ 0x5bd1e995u));
         }
         return 1.0 + 0.001 * (double)(n + 1);
     }

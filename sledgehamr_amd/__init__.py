@@ -77,7 +77,7 @@ class _Info(C.Structure):
         "precision", "max_level", "finest_level", "coarse_level_grid_size",
         "integrator_type", "with_gravitational_waves", "shadow_hierarchy")] + \
         [(n, C.c_double) for n in ("L", "cfl", "t_start", "t_end")] + \
-        [("graph_replays", C.c_longlong)]
+        [("graph_replays", C.c_longlong), ("gw_groups", C.c_int)]
 
 
 _LIBS: dict[str, C.CDLL] = {}

@@ -125,6 +125,7 @@ __attribute__((visibility("default"))) int shm_get_info(shm_ctx *ctx,
         pp.query("amr.te_crit", crit);
         info->shadow_hierarchy = crit != DBL_MAX;
         info->graph_replays = ctx->rt->graph_replays;
+        info->gw_groups = ctx->rt->gw_groups() ? 1 : 0;
     });
 }
 

@@ -107,6 +107,29 @@ hipError_t launch_stage(const StageArgs &a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+/** One group of a split GW stage (CompSet group G). */
+template <class Ops, int K, Prec P, int R, int V, int G>
+hipError_t launch_stage_grp(const StageArgs &a, int grid, hipStream_t s) {
+    constexpr TileVariant v = kVariants[V];
+    hipLaunchKernelGGL((stage_kernel<Ops, typename KindPol<K>::T, P, R, true, false,
+                                     v.nw, v.ry, v.tx, G>),
+                       dim3(grid), dim3(64 * v.nw), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class Ops, int K, Prec P, int R, int V, int G>
+int occupancy_stage_grp() {
+    constexpr TileVariant v = kVariants[V];
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb,
+            stage_kernel<Ops, typename KindPol<K>::T, P, R, true, false, v.nw, v.ry,
+                         v.tx, G>,
+            64 * v.nw, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
 template <class Ops, int K, Prec P, int R, bool GW, bool KO, int V>
 int occupancy_stage() {
     constexpr TileVariant v = kVariants[V];
@@ -144,6 +167,61 @@ void fill_variants(KernelTable &kt) {
     put(std::integral_constant<int, 2>{});
     put(std::integral_constant<int, 3>{});
     put(std::integral_constant<int, 4>{});
+}
+
+/** Split-stage groups of a GW kind: tile shapes 64 x 4, 64 x 8, 32 x 8 and
+ *  64 x 8 with two rows per thread (the 64 x 16 tile is left out to bound
+ *  the build). */
+template <class Ops, int K, Prec P, int R>
+void fill_group_variants(KernelTable &kt) {
+    auto put = [&](auto vc, auto gc) {
+        constexpr int V = decltype(vc)::value;
+        constexpr int G = decltype(gc)::value;
+        using TV = typename PrecTraits<P>::V;
+        constexpr int TX = kVariants[V].tx;
+        constexpr int TY = kVariants[V].nw * 64 / TX * kVariants[V].ry;
+        constexpr long lds = 2L * CompSet<Ops, true, false, G>::NS * (TY + 2 * R) *
+                             (TX + 2 * R) * (long)sizeof(TV);
+        if constexpr (lds <= 80 * 1024) {
+            StageEntry &e = kt.stage_grp[K][(int)P][R][G][V];
+            e.fn = &launch_stage_grp<Ops, K, P, R, V, G>;
+            e.occupancy = &occupancy_stage_grp<Ops, K, P, R, V, G>;
+            e.threads = 64 * kVariants[V].nw;
+            e.ty = TY;
+            e.tx = TX;
+        }
+    };
+    auto per_group = [&](auto vc) {
+        put(vc, std::integral_constant<int, 0>{});
+        put(vc, std::integral_constant<int, 1>{});
+        put(vc, std::integral_constant<int, 2>{});
+    };
+    per_group(std::integral_constant<int, 0>{});
+    per_group(std::integral_constant<int, 1>{});
+    per_group(std::integral_constant<int, 3>{});
+    per_group(std::integral_constant<int, 4>{});
+}
+
+/** The kinds a split GW step uses: SSPRK3 and RKN stages (fills, leapfrog
+ *  and dissipation run unsplit). */
+template <class Ops, Prec P, int R>
+void fill_group_kinds(KernelTable &kt) {
+    fill_group_variants<Ops, kSsprk1, P, R>(kt);
+    fill_group_variants<Ops, kSsprk2, P, R>(kt);
+    fill_group_variants<Ops, kSsprk3, P, R>(kt);
+    fill_group_variants<Ops, kRkn0, P, R>(kt);
+    fill_group_variants<Ops, kRkn1, P, R>(kt);
+    fill_group_variants<Ops, kRkn2, P, R>(kt);
+    fill_group_variants<Ops, kRkn3, P, R>(kt);
+    fill_group_variants<Ops, kRknF0, P, R>(kt);
+    fill_group_variants<Ops, kRknF1, P, R>(kt);
+    fill_group_variants<Ops, kRknF2, P, R>(kt);
+    fill_group_variants<Ops, kRknF3, P, R>(kt);
+    fill_group_variants<Ops, kRknF4, P, R>(kt);
+    if constexpr (std::is_same_v<typename PrecTraits<P>::V, typename PrecTraits<P>::S>) {
+        fill_group_variants<Ops, kRkn0Y, P, R>(kt);
+        fill_group_variants<Ops, kRkn1Y, P, R>(kt);
+    }
 }
 
 template <class Ops, Prec P, int R, bool GW, bool KO>
@@ -231,6 +309,7 @@ void fill_radius(KernelTable &kt) {
     }
     if constexpr (SHM_GW_MODES & 2) {
         fill_kinds<Ops, P, R, true, false>(kt);
+        fill_group_kinds<Ops, P, R>(kt);
         if constexpr (SHM_WITH_KO) fill_kinds<Ops, P, R, true, true>(kt);
     }
 }
@@ -252,6 +331,13 @@ template <class Ops>
 hipError_t launch_gw_bin(const BinArgs &a, int grid, hipStream_t s) {
     hipLaunchKernelGGL((gw_bin_kernel<typename Ops::GwModifiers, 2048>), dim3(grid),
                        dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class Ops>
+hipError_t launch_gwdep(unsigned long long *dep, const double *p, const double *pg,
+                        double t, double dt, double dx, hipStream_t s) {
+    hipLaunchKernelGGL((gwdep_kernel<Ops>), dim3(1), dim3(64), 0, s, dep, p, pg, t, dt, dx);
     return hipGetLastError();
 }
 
@@ -297,6 +383,13 @@ void fill_table_aux(KernelTable &kt) {
         kt.te_tag[1][1] = &launch_te_tag<ShmProjectOps, Prec::F32C, true>;
         kt.te_tag[2][1] = &launch_te_tag<ShmProjectOps, Prec::F32C16, true>;
         kt.probe[1] = &launch_probe<ShmProjectOps, true>;
+        kt.gwdep = &launch_gwdep<ShmProjectOps>;
+        kt.grp_out[0] = CompSet<ShmProjectOps, true, false, 0>::out_mask();
+        kt.grp_out[1] = CompSet<ShmProjectOps, true, false, 1>::out_mask();
+        kt.grp_out[2] = CompSet<ShmProjectOps, true, false, 2>::out_mask();
+        kt.grp_staged[0] = CompSet<ShmProjectOps, true, false, 0>::staged_mask();
+        kt.grp_staged[1] = CompSet<ShmProjectOps, true, false, 1>::staged_mask();
+        kt.grp_staged[2] = CompSet<ShmProjectOps, true, false, 2>::staged_mask();
         using Mods = typename ShmProjectOps::GwModifiers;
         kt.gw_bin = &launch_gw_bin<ShmProjectOps>;
         kt.ngwmod = Mods::count;

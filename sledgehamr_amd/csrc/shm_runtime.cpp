@@ -693,11 +693,14 @@ void Runtime::init_from_inputs(const std::string &inputs,
     pp.query("shm.graphs", graphs);
     pp.query("shm.fused", launch.fused);
     pp.query("shm.rkn_state", rkn_state_);
+    int gw_split = 1;
+    pp.query("shm.gw_split", gw_split);
     if (launch.variant >= kNVar) throw Error(1, "shm.tile_variant out of range");
 
     // Project initialisation, then the reach of its cell functions.
     sim_->Init();
     probe();
+    gw_groups_ = gw_split != 0 && check_gw_groups();
     if (R_ > S.nghost)
         throw Error(1, "cell functions reach " + std::to_string(R_) +
                            " cells but amr.nghost = " + std::to_string(S.nghost));
@@ -754,6 +757,31 @@ void Runtime::init_from_inputs(const std::string &inputs,
         regrid_dt_.push_back(reg_dt / std::pow(2., lev));
         last_regrid_.push_back(S.t_start);
     }
+}
+
+/** Whether the stages of a GW level may run as three component groups
+ *  (CompSet<..., G>): every output must read only the components its group
+ *  stages, by the dependency probe of the project's cell functions. */
+bool Runtime::check_gw_groups() {
+    if (!gw_ || ko_ || !kt_.gwdep) return false;
+    DevMem dep;
+    dep.alloc(sizeof(unsigned long long) * 64);
+    std::vector<double> blk(kPrmBlock, 0.);
+    fill_params(blk.data(), sim_->t_start, sim_->dt[0], sim_->dx[0], 0);
+    DevMem d;
+    upload_vec(d, blk);
+    SHM_HIP(kt_.gwdep(dep.as<unsigned long long>(), d.as<double>() + kPrmUser,
+                      d.as<double>() + kPrmGw, sim_->t_start, sim_->dt[0], sim_->dx[0],
+                      stream_));
+    std::vector<unsigned long long> h(64, 0);
+    SHM_HIP(hipMemcpyAsync(h.data(), dep.get(), sizeof(unsigned long long) * 64,
+                           hipMemcpyDeviceToHost, stream_));
+    SHM_HIP(hipStreamSynchronize(stream_));
+    gw_dep_.assign(h.begin(), h.begin() + ncomp_);
+    for (int g = 0; g < 3; ++g)
+        for (int n = 0; n < ncomp_; ++n)
+            if (((kt_.grp_out[g] >> n) & 1) && (h[n] & ~kt_.grp_staged[g])) return false;
+    return true;
 }
 
 void Runtime::probe() {
@@ -983,6 +1011,14 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
     }
     const StageEntry &e = row[v];
     if (!e.fn) throw Error(4, "stage kernel not built for this configuration");
+    // GW levels: three component-group launches when the split applies and
+    // the groups are built for this kind and tile shape.
+    const StageEntry *grp[3] = {nullptr, nullptr, nullptr};
+    if (gw_groups_) {
+        const auto &gr = kt_.stage_grp[kind][pi][R_];
+        if (gr[0][v].fn && gr[1][v].fn && gr[2][v].fn)
+            for (int g = 0; g < 3; ++g) grp[g] = &gr[g][v];
+    }
     int z = lc.tz;
     if (z <= 0) {
         // Enough tiles to fill every CU several times, long z runs otherwise.
@@ -1005,14 +1041,21 @@ void Runtime::launch_stage(int kind, LevelState &ls, StageArgs &a, int lev) {
     a.ko_order = sim_->dissipation_order;
     a.ghost = ls.L.g;
     if (ntiles == 0) return;
-    const int occ = e.occupancy();
-    int grid = lc.persistent ? (int)std::min<long long>(ntiles, (long long)occ * ncu_)
-                             : ntiles;
     a.xcd_split = lc.xcd_split ? 1 : 0;
-    if (a.xcd_split) grid = (int)round_up(grid, 8);
+    auto grid_of = [&](const StageEntry &x) {
+        const int occ = x.occupancy();
+        int grid = lc.persistent ? (int)std::min<long long>(ntiles, (long long)occ * ncu_)
+                                 : ntiles;
+        if (a.xcd_split) grid = (int)round_up(grid, 8);
+        return grid;
+    };
     EventPair *ep = timing_on ? timed_pair(0) : nullptr;
     if (ep) SHM_HIP(hipEventRecord(ep->a, stream_));
-    SHM_HIP(e.fn(a, grid, stream_));
+    if (grp[0]) {
+        for (int g = 0; g < 3; ++g) SHM_HIP(grp[g]->fn(a, grid_of(*grp[g]), stream_));
+    } else {
+        SHM_HIP(e.fn(a, grid_of(e), stream_));
+    }
     if (ep) {
         SHM_HIP(hipEventRecord(ep->b, stream_));
         timing.stage_cells += ls.L.nvalid();

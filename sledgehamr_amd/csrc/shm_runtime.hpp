@@ -297,6 +297,9 @@ class Runtime {
     int nfields() const { return kt_.nfields; }
     int radius() const { return R_; }
     int precision() const { return (int)prec_; }
+    /** GW level stages run as three component groups. */
+    bool gw_groups() const { return gw_groups_; }
+    const std::vector<unsigned long long> &gw_dependencies() const { return gw_dep_; }
     LevelState &level(int lev);
     bool has_level(int lev) const {
         return lev >= 0 && lev < (int)levels_.size() && levels_[lev].defined();
@@ -429,6 +432,7 @@ class Runtime {
     std::map<int, long> warm_;   // level -> epoch of its last eager step
     long epoch_ = 0;
     void probe();
+    bool check_gw_groups();
     std::vector<int> stencil_comps() const;
     // (field, momentum) pairs of the RKN / leapfrog integrators: the
     // project's fields, then the six GW tensor fields (rkn.cpp:32-44).
@@ -480,6 +484,10 @@ class Runtime {
     bool local_regrid_on_ = false;  // shm.local_regrid
     bool keep_layout_on_global_ = false;  // shm.keep_layout_on_global_regrid
     int rkn_state_ = 1;             // shm.rkn_state
+    // GW stages as three component groups (shm.gw_split and the dependency
+    // probe); the probe's masks per output.
+    bool gw_groups_ = false;
+    std::vector<unsigned long long> gw_dep_;
     // Tagging's inputs and outputs, kept between calls (each call ends with
     // a stream synchronisation) so that tagging allocates nothing per call.
     DevMem te_crit_, te_ptag_, te_pmod_, te_mask_, te_counts_, te_out_;

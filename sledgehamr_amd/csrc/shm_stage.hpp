@@ -128,31 +128,69 @@ struct StageArgs {
     double c[20];       // stage coefficients
 };
 
+// Component bookkeeping is evaluated at compile time: forced inlining lets
+// the unrolled loops fold every index, so the tile queues and the per-cell
+// arrays stay in registers (an out-of-line call would leave them in scratch
+// memory).
+#define SHM_CT __host__ __device__ __attribute__((always_inline))
+
 /** Which components need neighbour access (stencil) in the fused kernel.
  *  Default: the project's fields plus the six u_ij when GW are on; with
- *  Kreiss-Oliger dissipation every component is differenced. */
-template <class Ops, bool GW, bool KO>
+ *  Kreiss-Oliger dissipation every component is differenced.
+ *
+ *  G >= 0 selects one group of a split stage (GW levels without
+ *  dissipation, where the fourteen components of one pass need more
+ *  registers than a wavefront can hold at useful occupancy): each group is
+ *  a launch that computes and stores only its own outputs and stages only
+ *  the components they read.
+ *    G = 0: the project's scalars (Rhs and the GW backreaction);
+ *    G = 1: the pairs (u_xx, du_xx), (u_yy, du_yy), (u_zz, du_zz);
+ *    G = 2: the pairs (u_xy, du_xy), (u_xz, du_xz), (u_yz, du_yz);
+ *  groups 1 and 2 also stage the scalars (the GW source reads them).  A
+ *  component a group does not stage reads as NaN; the runtime splits only
+ *  when a dependency probe shows that every output reads its own group's
+ *  components alone (Runtime::check_gw_groups). */
+template <class Ops, bool GW, bool KO, int G = -1>
 struct CompSet {
+    static_assert(G < 0 || (GW && !KO && G < 3), "groups split GW levels only");
     static constexpr int NC = GW ? Ops::kNGw : Ops::kNScalars;
     static constexpr int NF = Ops::kNFields;
-    static constexpr bool stencil(int n) {
+    static constexpr int NSC = Ops::kNScalars;
+    static constexpr int kGroups = 3;
+    /** Outputs of this group. */
+    SHM_CT static constexpr bool out(int n) {
+        if (G < 0) return true;
+        if (G == 0) return n < NSC;
+        if (n < NSC) return false;
+        return G == 1 ? (n - NSC) % 6 < 3 : (n - NSC) % 6 >= 3;
+    }
+    /** Components this group loads. */
+    SHM_CT static constexpr bool staged(int n) { return G < 0 || out(n) || n < NSC; }
+    SHM_CT static constexpr bool base_stencil(int n) {
         return KO || n < NF ||
                (GW && n >= Ops::kNScalars && n < Ops::kNScalars + 6);
     }
-    static constexpr int count() {
+    SHM_CT static constexpr bool stencil(int n) { return base_stencil(n) && staged(n); }
+    SHM_CT static constexpr bool point(int n) { return !base_stencil(n) && staged(n); }
+    SHM_CT static constexpr int count() {
         int c = 0;
         for (int n = 0; n < NC; ++n) c += stencil(n) ? 1 : 0;
         return c;
     }
+    SHM_CT static constexpr int count_points() {
+        int c = 0;
+        for (int n = 0; n < NC; ++n) c += point(n) ? 1 : 0;
+        return c;
+    }
     static constexpr int NS = count();
-    static constexpr int NP = NC - NS;
-    static constexpr int slot(int n) {
+    static constexpr int NP = count_points();
+    SHM_CT static constexpr int slot(int n) {
         if (!stencil(n)) return -1;
         int c = 0;
         for (int m = 0; m < n; ++m) c += stencil(m) ? 1 : 0;
         return c;
     }
-    static constexpr int comp_of_slot(int s) {
+    SHM_CT static constexpr int comp_of_slot(int s) {
         int c = 0;
         for (int n = 0; n < NC; ++n)
             if (stencil(n)) {
@@ -161,10 +199,10 @@ struct CompSet {
             }
         return -1;
     }
-    static constexpr int comp_of_point(int p) {
+    SHM_CT static constexpr int comp_of_point(int p) {
         int c = 0;
         for (int n = 0; n < NC; ++n)
-            if (!stencil(n)) {
+            if (point(n)) {
                 if (c == p) return n;
                 ++c;
             }
@@ -172,19 +210,64 @@ struct CompSet {
     }
     // (field, momentum) pairs as the RKN/leapfrog integrators pair them
     // (rkn.cpp:32-44): first and second halves of the scalars, then of the
-    // twelve GW components.
+    // twelve GW components.  Pair indices address the force arrays.
     static constexpr int NPAIR = NF + (GW ? 6 : 0);
-    static constexpr int pair_field(int p) {
+    SHM_CT static constexpr int pair_field(int p) {
         return p < NF ? p : Ops::kNScalars + (p - NF);
     }
-    static constexpr int pair_mom(int p) {
+    SHM_CT static constexpr int pair_mom(int p) {
         return p < NF ? NF + p : Ops::kNScalars + 6 + (p - NF);
     }
-    static constexpr int pair_of_field(int n) {
+    SHM_CT static constexpr int pair_of_field(int n) {
         return n < NF ? n
                       : ((GW && n >= Ops::kNScalars && n < Ops::kNScalars + 6)
                              ? NF + (n - Ops::kNScalars)
                              : -1);
+    }
+    // The outputs a stage writes: components (SSPRK3, fills) and pairs
+    // (RKN, leapfrog) of this group, in index order.
+    SHM_CT static constexpr int count_out() {
+        int c = 0;
+        for (int n = 0; n < NC; ++n) c += out(n) ? 1 : 0;
+        return c;
+    }
+    SHM_CT static constexpr int count_out_pairs() {
+        int c = 0;
+        for (int p = 0; p < NPAIR; ++p) c += out(pair_field(p)) ? 1 : 0;
+        return c;
+    }
+    static constexpr int NO = count_out();
+    static constexpr int NOP = count_out_pairs();
+    SHM_CT static constexpr int out_comp(int i) {
+        int c = 0;
+        for (int n = 0; n < NC; ++n)
+            if (out(n)) {
+                if (c == i) return n;
+                ++c;
+            }
+        return -1;
+    }
+    SHM_CT static constexpr int out_pair(int i) {
+        int c = 0;
+        for (int p = 0; p < NPAIR; ++p)
+            if (out(pair_field(p))) {
+                if (c == i) return p;
+                ++c;
+            }
+        return -1;
+    }
+    /** Bit masks of out() and staged() (host-side group checks). */
+    static constexpr unsigned long long out_mask() {
+        unsigned long long m = 0;
+        for (int n = 0; n < NC; ++n)
+            if (out(n)) m |= 1ull << n;
+        return m;
+    }
+    static constexpr unsigned long long staged_mask() {
+        unsigned long long m = 0;
+        for (int n = 0; n < NC; ++n)
+            if (staged(n)) m |= 1ull << n;
+        return m;
     }
 };
 
@@ -345,7 +428,7 @@ struct PolSsprk {
     // the kept u0 is the whole stored value); fill-add previous rhs.
     template <class CS, class T>
     static constexpr int next() {
-        return CS::NC * ((K == kSsStage3 && T::comp && !T::exact) || K == kSsFillAdd ? 1 : 0);
+        return CS::NO * ((K == kSsStage3 && T::comp && !T::exact) || K == kSsFillAdd ? 1 : 0);
     }
     /** Stage state of component n: u0 + c2 k_in in the tile type (composed
      *  stages), else the source state. */
@@ -394,7 +477,8 @@ struct PolSsprk {
         using S = typename T::S;
         int i = 0;
 #pragma unroll
-        for (int n = 0; n < CS::NC; ++n) {
+        for (int o = 0; o < CS::NO; ++o) {
+            const int n = CS::out_comp(o);
             const long long b = org + n * a.cstride;
             if constexpr (K == kSsStage3 && T::comp && !T::exact)
                 e[i++] = (S)load_lo<T>(a.u0lo, b, off);
@@ -411,7 +495,8 @@ struct PolSsprk {
         using S = typename T::S;
         int i = 0;
 #pragma unroll
-        for (int n = 0; n < CS::NC; ++n) {
+        for (int o = 0; o < CS::NO; ++o) {
+            const int n = CS::out_comp(o);
             const long long b = org + n * a.cstride;
             double k = rhs[n];
             if constexpr (composed) k += a.c[0] * (double)kc[n][0];
@@ -447,7 +532,7 @@ struct PolRkn {
     // kept values are the whole stored values).
     template <class CS, class T>
     static constexpr int next() {
-        return FINAL && T::comp && !T::exact ? CS::NPAIR * 2 : 0;
+        return FINAL && T::comp && !T::exact ? CS::NOP * 2 : 0;
     }
 
     /** Stage state in the tile type (FP32 tiles hold FP32 values, so their
@@ -506,7 +591,8 @@ struct PolRkn {
         if constexpr (FINAL && T::comp && !T::exact) {
             int i = 0;
 #pragma unroll
-            for (int p = 0; p < CS::NPAIR; ++p) {
+            for (int o = 0; o < CS::NOP; ++o) {
+                const int p = CS::out_pair(o);
                 e[i++] = (S)load_lo<T>(a.u0lo, org + CS::pair_field(p) * a.cstride, off);
                 e[i++] = (S)load_lo<T>(a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
             }
@@ -522,7 +608,8 @@ struct PolRkn {
         using S = typename T::S;
         int i = 0;
 #pragma unroll
-        for (int p = 0; p < CS::NPAIR; ++p) {
+        for (int o = 0; o < CS::NOP; ++o) {
+            const int p = CS::out_pair(o);
             const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
             const double fi = rhs[nm];
             if constexpr (!FINAL) {
@@ -569,7 +656,8 @@ struct PolRkn0Y : PolRkn<0, false> {
         using S = typename T::S;
         PolRkn<0, false>::template store<CS, T, NK>(a, org, off, rhs, ctr, e, kc);
 #pragma unroll
-        for (int p = 0; p < CS::NPAIR; ++p) {
+        for (int o = 0; o < CS::NOP; ++o) {
+            const int p = CS::out_pair(o);
             const S f0 = (S)rhs[CS::pair_mom(p)];
             S y = (S)ctr[CS::pair_field(p)];
             y += (S)a.c[18] * (S)ctr[CS::pair_mom(p)];
@@ -615,7 +703,7 @@ struct PolLeapfrog {
     template <class CS, class T>
     static constexpr int nkeep() { return 0; }
     template <class CS, class T>
-    static constexpr int next() { return CS::NPAIR * (KICK2 ? 1 : 2); }
+    static constexpr int next() { return CS::NOP * (KICK2 ? 1 : 2); }
 
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
@@ -639,7 +727,8 @@ struct PolLeapfrog {
                                                        double *e) {
         int i = 0;
 #pragma unroll
-        for (int p = 0; p < CS::NPAIR; ++p) {
+        for (int o = 0; o < CS::NOP; ++o) {
+            const int p = CS::out_pair(o);
             if constexpr (!KICK2) {
                 e[i++] = load_state<T>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
                 e[i++] = load_state<T>(a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
@@ -657,7 +746,8 @@ struct PolLeapfrog {
                                                     const typename T::V (*)[NK]) {
         int i = 0;
 #pragma unroll
-        for (int p = 0; p < CS::NPAIR; ++p) {
+        for (int o = 0; o < CS::NOP; ++o) {
+            const int p = CS::out_pair(o);
             const int nf = CS::pair_field(p), nm = CS::pair_mom(p);
             const long long bp = org + p * a.cstride;
             const long long bf = org + nf * a.cstride, bm = org + nm * a.cstride;
@@ -700,12 +790,13 @@ __device__ AMREX_FORCE_INLINE void ring_xy(int h, int &x, int &y) {
     }
 }
 
-template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int NW, int RY, int TX>
+template <class Ops, class Pol, Prec P, int R, bool GW, bool KO, int NW, int RY, int TX,
+          int G = -1>
 __global__ void __launch_bounds__(64 * NW)
     stage_kernel(const StageArgs a) {
     using T = PrecTraits<P>;
     using V = typename T::V;  // tile values
-    using CS = CompSet<Ops, GW, KO>;
+    using CS = CompSet<Ops, GW, KO, G>;  // G >= 0: one group of a split stage
     constexpr int NC = CS::NC;
     constexpr int NS = CS::NS;
     constexpr int NP = CS::NP;
@@ -960,6 +1051,11 @@ __global__ void __launch_bounds__(64 * NW)
                 const int gi = bd.lo[0] + lx, gj = bd.lo[1] + td.y0 + w + r * ROWS,
                           gk = bd.lo[2] + z;
                 double ctr[NC];
+                if constexpr (G >= 0) {
+                    // Components of other groups: not loaded, read as NaN.
+#pragma unroll
+                    for (int n = 0; n < NC; ++n) ctr[n] = __builtin_nan("");
+                }
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                     ctr[CS::comp_of_slot(s)] = (double)q[s][r][R];
@@ -1093,6 +1189,61 @@ struct ProbeOut {
     int offaxis[ProbeCtx::kMaxComp];
     int tag_reach[ProbeCtx::kMaxComp];
 };
+
+/** Dependencies of the GW cell functions (Rhs, GravitationalWavesRhs and
+ *  the backreaction, summed per output as a stage sums them): dep[n] gets
+ *  bit m when perturbing component m (at every offset) changes output n, on
+ *  two sets of offset-dependent inputs.  An output that is not a number is
+ *  taken to read everything. */
+template <class Ops>
+__global__ void gwdep_kernel(unsigned long long *dep, const double *params,
+                             const double *gw_params, double time, double dt,
+                             double dx) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    constexpr int NC = Ops::kNGw;
+    static_assert(NC <= 64, "dependency masks hold 64 components");
+    ProbeCtx pc;
+    pc.ci = 7;
+    pc.cj = 9;
+    pc.ck = 11;
+    pc.ncomp = NC;
+    pc.hashed = 1;
+    for (int n = 0; n < ProbeCtx::kMaxComp; ++n) {
+        pc.reach[n] = 0;
+        pc.zreach[n] = 0;
+        pc.offaxis[n] = 0;
+        pc.pert[n] = 0.;
+    }
+    amrex::Array4<const double> sv;
+    sv.mode = amrex::Array4<const double>::kProbe;
+    sv.ext = &pc;
+    sv.ncomp = NC;
+    auto eval = [&](double *out) {
+        for (int n = 0; n < NC; ++n) out[n] = 0.;
+        amrex::Array4<double> rv;
+        rv.mode = amrex::Array4<double>::kSink;
+        rv.sink = out;
+        rv.ncomp = NC;
+        Ops::rhs(rv, sv, 7, 9, 11, 0, time, dt, dx, params);
+        Ops::gw_rhs(rv, sv, 7, 9, 11, 0, time, dt, dx, gw_params);
+        Ops::gw_backreaction(rv, sv, 7, 9, 11, 0, time, dt, dx, params, gw_params);
+    };
+    unsigned long long d[NC];
+    for (int n = 0; n < NC; ++n) d[n] = 0;
+    for (int trial = 0; trial < 2; ++trial) {
+        pc.seed = 0x2545f491u * (unsigned)(trial + 1);
+        double base[NC], o[NC];
+        eval(base);
+        for (int m = 0; m < NC; ++m) {
+            pc.pert[m] = 1e-3;
+            eval(o);
+            pc.pert[m] = 0.;
+            for (int n = 0; n < NC; ++n)
+                if (!(o[n] == base[n]) || !(base[n] == base[n])) d[n] |= 1ull << m;
+        }
+    }
+    for (int n = 0; n < NC; ++n) dep[n] = d[n];
+}
 
 template <class Ops, bool GW>
 __global__ void probe_kernel(ProbeOut *out, const double *params,

@@ -59,6 +59,11 @@ using ProbeFn = hipError_t (*)(ProbeOut *, const double *, const double *,
                                double, double, double, hipStream_t);
 using OutFn = hipError_t (*)(const OutArgs &, int grid, hipStream_t);
 using GwBinFn = hipError_t (*)(const BinArgs &, int grid, hipStream_t);
+/** Dependency probe of the cell functions: dep[n] = bit mask of the
+ *  components output n reads (by perturbation). */
+using GwDepFn = hipError_t (*)(unsigned long long *dep, const double *params,
+                               const double *gw_params, double time, double dt,
+                               double dx, hipStream_t);
 
 struct StageEntry {
     StageFn fn = nullptr;
@@ -84,6 +89,12 @@ struct KernelTable {
     StageEntry stage[kNKind][4][4][2][2][kNVar];
     FusedEntry fused12[4][4][2][kNVar];   // [precision][R][gw][variant]
     FusedEntry fusedrkn[4][4][2][kNVar];  // the same for RKN stages one and two
+    // Split stages of GW levels without dissipation: three group launches
+    // per stage ([kind][precision][R][group][variant]), the groups' output
+    // and staged component masks, and the probe that validates the split.
+    StageEntry stage_grp[kNKind][4][4][3][kNVar];
+    unsigned long long grp_out[3] = {}, grp_staged[3] = {};
+    GwDepFn gwdep = nullptr;
     // Auxiliary kernels per storage precision (F64, F32C, F32C16).
     TeTagFn te_tag[3][2] = {};  // [storage precision][gw]
     ProbeFn probe[2] = {};      // [gw]
