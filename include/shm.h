@@ -167,7 +167,8 @@ int shm_te_tag_noalloc(shm_ctx *ctx, int lev, int flags, unsigned char *mask,
 
 /* GravitationalWaves::ComputeSpectrum (gravitational_waves.cpp:52-322).
  * mode 0: unbinned (unique |k|^2), 1: rounded |k|, 2: nbins log bins.
- * bins/kvals must hold *nout values (call with bins=NULL to query *nout). */
+ * bins/kvals must hold *nout values (call with bins=NULL to query *nout;
+ * that call computes nothing). */
 int shm_gw_spectrum(shm_ctx *ctx, int mode, int nbins, int zero_padding,
                     int projection_type, double *bins, double *kvals,
                     int *nout);
@@ -247,13 +248,14 @@ int shm_output_function_name(shm_ctx *ctx, int id, char *buf, int len);
 /* Spectrum::Compute (output_types/spectrum.cpp:12-113) of output function fn
  * on level 0: bins[b] = sum over modes with |k|^2 = k_sq[b] of
  * mult t |F(k)|^2 / (dk N^6), dk = 2 pi / L.  bins / k_sq must hold *n
- * values (call with bins = NULL to query *n). */
+ * values (call with bins = NULL to query *n; that call computes nothing). */
 int shm_spectrum(shm_ctx *ctx, int fn, double *bins, long long *k_sq, int *n);
 
 /* Projection::Compute (output_types/projection.cpp:11-118) of output function
  * fn along z onto the finest level up to max_level (< 0: all levels).
  * mode 0: sum of ratio * value, 1: maximum (from 0).  data / counts hold
- * (*dimN)^2 values, x index slow (call with data = NULL to query *dimN). */
+ * (*dimN)^2 values, x index slow (call with data = NULL to query *dimN; that
+ * call computes nothing). */
 int shm_projection(shm_ctx *ctx, int fn, int mode, int max_level, double *data,
                    int *counts, int *dimN);
 

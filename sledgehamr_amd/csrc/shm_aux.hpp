@@ -1037,27 +1037,30 @@ __global__ void __launch_bounds__(256) init_kernel(const InitArgs a) {
         const uint64_t gidx = (uint64_t)(b.lo[0] + x) +
                               (uint64_t)a.N * ((uint64_t)(b.lo[1] + y) +
                                                (uint64_t)a.N * (uint64_t)(b.lo[2] + z));
-        double v[32];
-        for (int n = 0; n < a.ncomp && n < 32; ++n) v[n] = 0.;
         // Cell centre in level-0 cell units.
         const double px = (b.lo[0] + x + 0.5) * a.cell, py = (b.lo[1] + y + 0.5) * a.cell,
                      pz = (b.lo[2] + z + 0.5) * a.cell;
-        if (a.kind == kInitBubbles || a.kind == kInitAddBubbles) {
-            double phi = a.kind == kInitBubbles
-                             ? 0.
-                             : load_state_at<PrecTraits<P>>(a.arr, a.arrlo, e);
+        if (a.kind == kInitAddBubbles) {
+            // Injection: the first field only, read-modify-write.
+            double phi = load_state_at<PrecTraits<P>>(a.arr, a.arrlo, e);
             for (int o = 0; o < a.nobj; ++o) {
                 const double *ob = a.obj + 8 * o;
-                const bool now = a.kind == kInitBubbles
-                                     ? ob[3] <= a.p[7]
-                                     : (ob[3] > a.p[6] && ob[3] <= a.p[7]);
-                if (now)
+                if (ob[3] > a.p[6] && ob[3] <= a.p[7])
                     phi = fmax(phi, bubble_profile(ob, px, py, pz, a.N0, a.p[1],
                                                    a.p[2], a.p[3]));
             }
-            if (a.kind == kInitAddBubbles) {
-                store_state_at<PrecTraits<P>>(a.arr, a.arrlo, e, phi);
-                continue;
+            store_state_at<PrecTraits<P>>(a.arr, a.arrlo, e, phi);
+            continue;
+        }
+        double v[32];
+        for (int n = 0; n < a.ncomp && n < 32; ++n) v[n] = 0.;
+        if (a.kind == kInitBubbles) {
+            double phi = 0.;
+            for (int o = 0; o < a.nobj; ++o) {
+                const double *ob = a.obj + 8 * o;
+                if (ob[3] <= a.p[7])
+                    phi = fmax(phi, bubble_profile(ob, px, py, pz, a.N0, a.p[1],
+                                                   a.p[2], a.p[3]));
             }
             v[0] = phi;
         } else if (a.kind == kInitStringLoops) {

@@ -404,7 +404,8 @@ __attribute__((visibility("default"))) int shm_gw_spectrum(
         SHM_NEED_RT;
         if (!nout) throw shm::Error(SHM_ERR_ARG, "null nout");
         std::vector<double> b, k;
-        ctx->rt->gw_spectrum(mode, nbins, zero_padding, projection_type, b, k);
+        ctx->rt->gw_spectrum(mode, nbins, zero_padding, projection_type, b, k, -1, {},
+                             bins == nullptr);
         if (bins) {
             if (*nout < (int)b.size())
                 throw shm::Error(SHM_ERR_ARG, "output arrays too small");
@@ -425,7 +426,8 @@ __attribute__((visibility("default"))) int shm_gw_spectrum_mod(
         if (nparams < 0 || (nparams > 0 && !params))
             throw shm::Error(SHM_ERR_ARG, "bad modifier parameters");
         std::vector<double> p(params, params + nparams), b, k;
-        ctx->rt->gw_spectrum(mode, nbins, zero_padding, projection_type, b, k, modifier, p);
+        ctx->rt->gw_spectrum(mode, nbins, zero_padding, projection_type, b, k, modifier, p,
+                             bins == nullptr);
         if (bins) {
             if (*nout < (int)b.size())
                 throw shm::Error(SHM_ERR_ARG, "output arrays too small");
@@ -581,7 +583,7 @@ __attribute__((visibility("default"))) int shm_spectrum(shm_ctx *ctx, int fn,
         if (!n) throw shm::Error(SHM_ERR_ARG, "null n");
         std::vector<double> b;
         std::vector<long long> k;
-        ctx->rt->spectrum(fn, b, k);
+        ctx->rt->spectrum(fn, b, k, bins == nullptr);
         if (bins) {
             if (*n < (int)b.size()) throw shm::Error(SHM_ERR_ARG, "output arrays too small");
             std::memcpy(bins, b.data(), b.size() * sizeof(double));
@@ -600,7 +602,7 @@ __attribute__((visibility("default"))) int shm_projection(
         std::vector<double> d;
         std::vector<int> c;
         int D = 0;
-        ctx->rt->projection(fn, mode, max_level, d, c, D);
+        ctx->rt->projection(fn, mode, max_level, d, c, D, data == nullptr);
         if (data) {
             if (*dimN < D) throw shm::Error(SHM_ERR_ARG, "output arrays too small");
             std::memcpy(data, d.data(), d.size() * sizeof(double));

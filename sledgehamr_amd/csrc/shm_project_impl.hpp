@@ -95,6 +95,8 @@ template <> struct KindPol<kRknF3> { using T = PolRkn<3, true>; };
 template <> struct KindPol<kRknF4> { using T = PolRkn<4, true>; };
 template <> struct KindPol<kRkn0Y> { using T = PolRkn0Y; };
 template <> struct KindPol<kRkn1Y> { using T = PolRkn1Y; };
+template <> struct KindPol<kRkn1YY> { using T = PolRkn1YY; };
+template <> struct KindPol<kRknF2Y> { using T = PolRknF2Y; };
 template <> struct KindPol<kLeap1> { using T = PolLeapfrog<false>; };
 template <> struct KindPol<kLeap2> { using T = PolLeapfrog<true>; };
 
@@ -221,6 +223,8 @@ void fill_group_kinds(KernelTable &kt) {
     if constexpr (std::is_same_v<typename PrecTraits<P>::V, typename PrecTraits<P>::S>) {
         fill_group_variants<Ops, kRkn0Y, P, R>(kt);
         fill_group_variants<Ops, kRkn1Y, P, R>(kt);
+        fill_group_variants<Ops, kRkn1YY, P, R>(kt);
+        fill_group_variants<Ops, kRknF2Y, P, R>(kt);
     }
 }
 
@@ -245,6 +249,8 @@ void fill_kinds(KernelTable &kt) {
     if constexpr (std::is_same_v<typename PrecTraits<P>::V, typename PrecTraits<P>::S>) {
         fill_variants<Ops, kRkn0Y, P, R, GW, KO>(kt);
         fill_variants<Ops, kRkn1Y, P, R, GW, KO>(kt);
+        fill_variants<Ops, kRkn1YY, P, R, GW, KO>(kt);
+        fill_variants<Ops, kRknF2Y, P, R, GW, KO>(kt);
     }
     // Leapfrog writes u_new momenta in place while the stage reads them
     // point-wise; with dissipation the momenta are differenced, so that

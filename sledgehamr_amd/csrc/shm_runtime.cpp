@@ -463,6 +463,7 @@ Runtime::~Runtime() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     release_graphs();
     release_gw_cache();
+    release_out_cache();
     for (EventPair &p : ev_pool_) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -863,6 +864,8 @@ void Runtime::ensure_arrays(LevelState &ls, int integrator) {
             if (!ls.F[i]) ls.F[i].alloc(plane * npair * es);
         if (nodes_.size() >= 3 && nodes_[0] == 0. && !ls.Y2)
             ls.Y2.alloc(plane * npair * es);
+        if (gw_ && nodes_.size() == 3 && nodes_[0] == 0. && !ls.Y3)
+            ls.Y3.alloc(plane * npair * es);
     } else if (integrator == 11) {
         if (!ls.VH) ls.VH.alloc(plane * npair * 8);
     }
@@ -1587,6 +1590,13 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
             halo(ls, ls.F[1].get(), pairs);
             i0 = 2;
         }
+        // Three stages on GW levels: the second stage also writes the final
+        // stage's field states, which saves the final stage composing them
+        // from four arrays at every stencil point of seven stencil
+        // components.  With one or two stencil components the composed
+        // reads hit the caches and the extra array costs more than it saves
+        // (64-box RealScalar ensemble: 1.52 -> 1.66 ms per step).
+        const bool ry2 = ry && s == 3 && i0 == 0 && gw_;
         for (int i = i0; i < s; ++i) {
             StageArgs a{};
             a.prm = prm(st[i]);
@@ -1610,6 +1620,13 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                 } else if (ry && i == 1) {
                     kind = kRkn1Y;
                     a.y = ls.Y2.get();
+                    if (ry2) {
+                        kind = kRkn1YY;
+                        a.y2 = ls.Y3.get();
+                        a.c[12] = dt * nodes_[2];
+                        a.c[13] = dt * dt * tableau_[2][0];
+                        a.c[14] = dt * dt * tableau_[2][1];
+                    }
                 }
                 launch_stage(kind, ls, a, lev);
                 if (cf) {
@@ -1634,6 +1651,7 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                 }
                 halo(ls, ls.F[i].get(), pairs);
                 if (ry && i == 0) halo(ls, ls.Y2.get(), pairs);
+                if (ry2 && i == 1) halo(ls, ls.Y3.get(), pairs);
             } else {
                 a.c[6] = dt;
                 for (int j = 0; j < i; ++j) {
@@ -1644,7 +1662,12 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                 a.c[16] = dt * b_[i];
                 a.unew = ls.U[nw].get();
                 a.unewlo = ulo_new;
-                launch_stage(kRknF0 + i, ls, a, lev);
+                int kind = kRknF0 + i;
+                if (ry2) {
+                    kind = kRknF2Y;
+                    a.y2 = ls.Y3.get();
+                }
+                launch_stage(kind, ls, a, lev);
             }
         }
     } else if (integrator_ == 11) {
@@ -2280,6 +2303,17 @@ void Runtime::init_state(int lev, int kind, uint64_t seed, const double *p,
     } else if (kind != kInitComplexRandomPhase && kind != kInitZero) {
         throw Error(1, "unknown initial-state kind");
     }
+    if (kind == kInitAddBubbles) {
+        // Only the bubbles nucleated in (t_from, t_to] change the level
+        // (the kernel tests the same window): pass those alone, and leave
+        // the level untouched when there are none.
+        std::vector<double> now;
+        for (size_t o = 0; o < obj.size(); o += 8)
+            if (obj[o + 3] > p[6] && obj[o + 3] <= p[7])
+                now.insert(now.end(), obj.begin() + o, obj.begin() + o + 8);
+        if (now.empty()) return;
+        obj.swap(now);
+    }
     DevMem dobj;
     if (!obj.empty()) upload_vec(dobj, obj);
     a.obj = obj.empty() ? nullptr : dobj.as<double>();
@@ -2441,6 +2475,13 @@ void Runtime::autotune(int lev) {
                 r.y = ls.Y2.get();
                 r.c[18] = dt * nodes_[1];
                 r.c[19] = dt * dt * tableau_[1][0];
+                if (i == 1 && s == 3 && gw_) {
+                    kind = kRkn1YY;
+                    r.y2 = ls.Y3.get();
+                    r.c[12] = dt * nodes_[2];
+                    r.c[13] = dt * dt * tableau_[2][0];
+                    r.c[14] = dt * dt * tableau_[2][1];
+                }
             }
             runs.push_back({kind, r});
         }
@@ -2494,7 +2535,12 @@ void Runtime::autotune(int lev) {
         r.c[16] = dt * b_[i];
         r.unew = t3.get();
         r.unewlo = comp() ? t3lo.get() : nullptr;
-        runs.push_back({kRknF0 + i, r});
+        int kind = kRknF0 + i;
+        if (rkn_y(lev) && i == 2 && gw_) {
+            kind = kRknF2Y;
+            r.y2 = ls.Y3.get();
+        }
+        runs.push_back({kind, r});
     }
     if (tune3) {
         s3 = a;
@@ -2655,7 +2701,8 @@ void Runtime::release_gw_cache() {
  *  tables stay with the context for the next call of the same size. */
 void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
                           std::vector<double> &bins, std::vector<double> &kvals,
-                          int modifier, const std::vector<double> &mparams) {
+                          int modifier, const std::vector<double> &mparams,
+                          bool count_only) {
     if (independent_)
         throw Error(4, "the GW spectrum needs one periodic domain, not independent boxes");
     if (!gw_) throw Error(3, "gravitational waves are not enabled");
@@ -2736,6 +2783,10 @@ void Runtime::gw_spectrum(int mode, int nbins, int zp, int proj,
     }
     kvals = w.kvals;
     const int nb_eff = w.nbin_eff;
+    if (count_only) {
+        bins.assign(nb_eff, 0.);
+        return;
+    }
 
     // Zero padding: the level fills the corner [0, N/zp)^3 of the dense
     // array (its boxes tile it completely); the rest keeps the zeros of its
@@ -2827,18 +2878,56 @@ void Runtime::fill_output_ghosts(int lev) {
     if (comp()) halo(ls, lo, all_comps(), true, (int)losize());
 }
 
-void Runtime::spectrum(int fn, std::vector<double> &bins, std::vector<long long> &k_sq) {
+/** Sorted unique |n|^2 over the signed frequencies of an N^3 grid (the
+ *  k-table of ReadSpectrumKs, sledgehamr.cpp:554-601): every |f| in 0..N/2
+ *  occurs on each axis, so mark f1^2 + f2^2 + f3^2 for 0 <= f1 <= f2 <= f3
+ *  <= N/2. */
+static std::vector<long long> unique_k2(int N) {
+    const long long h = N / 2, maxsq = 3 * h * h;
+    std::vector<char> seen(maxsq + 1, 0);
+    for (long long a = 0; a <= h; ++a)
+        for (long long b = a; b <= h; ++b)
+            for (long long c = b; c <= h; ++c) seen[a * a + b * b + c * c] = 1;
+    std::vector<long long> v;
+    for (long long x = 0; x <= maxsq; ++x)
+        if (seen[x]) v.push_back(x);
+    return v;
+}
+
+void Runtime::spectrum(int fn, std::vector<double> &bins, std::vector<long long> &k_sq,
+                       bool count_only) {
     check_output_fn(fn);
     LevelState &l0 = level(0);
     const int N = l0.L.N, NH = N / 2 + 1;
+    OutSpecCache &w = osc_;
+    if (w.N != N) {
+        release_out_cache();
+        w.N = N;
+        w.k_sq = unique_k2(N);
+        const long long maxsq = 3LL * (N / 2) * (N / 2);
+        std::vector<int> sq2bin(maxsq + 1, -1);
+        for (size_t b = 0; b < w.k_sq.size(); ++b) sq2bin[w.k_sq[b]] = (int)b;
+        upload_vec(w.dsq, sq2bin);
+        w.dense.alloc(sizeof(double) * (size_t)N * N * N);
+        w.F.alloc(sizeof(double2) * (size_t)NH * N * N);
+        w.dbins.alloc(sizeof(double) * w.k_sq.size());
+        if (hipfftPlan3d(&w.plan, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS)
+            throw Error(2, "hipfftPlan3d failed");
+        w.have_plan = true;
+    }
+    k_sq = w.k_sq;
+    const int nbins = (int)k_sq.size();
+    if (count_only) {
+        bins.assign(nbins, 0.);
+        return;
+    }
     fill_output_ghosts(0);
     std::vector<double> p;
     sim_->SetParamsSpectra(p, l0.t);
     p.resize(std::max<size_t>(p.size(), 1), 0.);
-    DevMem dp, dense, F, dsq, dbins;
-    upload_vec(dp, p);
-    dense.alloc(sizeof(double) * (size_t)N * N * N);
-    SHM_HIP(hipMemsetAsync(dense.get(), 0, dense.bytes(), stream_));
+    upload_into(w.dparams, p);
+    SHM_HIP(hipMemsetAsync(w.dense.get(), 0, w.dense.bytes(), stream_));
+    SHM_HIP(hipMemsetAsync(w.dbins.get(), 0, w.dbins.bytes(), stream_));
     OutArgs a{};
     a.boxes = l0.L.dbox.as<BoxDesc>();
     a.vprefix = l0.L.dvprefix.as<long long>();
@@ -2849,59 +2938,33 @@ void Runtime::spectrum(int fn, std::vector<double> &bins, std::vector<long long>
     a.arr = l0.U[l0.cur].get();
     a.arrlo = comp() ? l0.Ulo[l0.cur].get() : nullptr;
     a.fn = fn;
-    a.params = dp.as<double>();
+    a.params = w.dparams.as<double>();
     a.lev = 0;
     a.time = l0.t;
     a.dt = sim_->dt[0];
     a.dx = sim_->dx[0];
     a.target = 0;
-    a.dense = dense.as<double>();
+    a.dense = w.dense.as<double>();
     a.N = N;
     const int grid = (int)std::min<long long>((a.total + 255) / 256, 8LL * ncu_);
     SHM_HIP(kt_.output[(int)sprec()](a, grid, stream_));
-    F.alloc(sizeof(double2) * (size_t)NH * N * N);
-    hipfftHandle plan;
-    if (hipfftPlan3d(&plan, N, N, N, HIPFFT_D2Z) != HIPFFT_SUCCESS)
-        throw Error(2, "hipfftPlan3d failed");
-    hipfftSetStream(plan, stream_);
-    const hipfftResult fr =
-        hipfftExecD2Z(plan, dense.as<hipfftDoubleReal>(), F.as<hipfftDoubleComplex>());
-    hipfftDestroy(plan);
-    if (fr != HIPFFT_SUCCESS) throw Error(2, "hipfftExecD2Z failed");
-    // Unique |k|^2 over the signed frequencies (the k-table of ReadSpectrumKs,
-    // sledgehamr.cpp:554-601).
-    std::set<long long> vals;
-    {
-        std::set<long long> u;
-        for (int q = 0; q < N; ++q) {
-            const long long f = q >= N / 2 ? q - N : q;
-            u.insert(f * f);
-        }
-        for (long long x : u)
-            for (long long y : u)
-                for (long long z : u) vals.insert(x + y + z);
-    }
-    const long long maxsq = 3LL * (N / 2) * (N / 2);
-    std::vector<int> sq2bin(maxsq + 1, -1);
-    k_sq.assign(vals.begin(), vals.end());
-    for (size_t b = 0; b < k_sq.size(); ++b)
-        if (k_sq[b] <= maxsq) sq2bin[k_sq[b]] = (int)b;
-    upload_vec(dsq, sq2bin);
-    const int nbins = (int)k_sq.size();
-    dbins.alloc(sizeof(double) * nbins);
-    SHM_HIP(hipMemsetAsync(dbins.get(), 0, dbins.bytes(), stream_));
+    if (hipfftSetStream(w.plan, stream_) != HIPFFT_SUCCESS)
+        throw Error(2, "hipfftSetStream failed");
+    if (hipfftExecD2Z(w.plan, w.dense.as<hipfftDoubleReal>(),
+                      w.F.as<hipfftDoubleComplex>()) != HIPFFT_SUCCESS)
+        throw Error(2, "hipfftExecD2Z failed");
     SpecBinArgs sb{};
-    sb.F = F.as<double2>();
+    sb.F = w.F.as<double2>();
     sb.N = N;
-    sb.sq_to_bin = dsq.as<int>();
+    sb.sq_to_bin = w.dsq.as<int>();
     sb.nbins = nbins;
-    sb.bins = dbins.as<double>();
+    sb.bins = w.dbins.as<double>();
     const long long modes = (long long)NH * N * N;
     const int bgrid = (int)std::min<long long>((modes + 255) / 256, 4LL * ncu_);
     hipLaunchKernelGGL(scalar_bin_kernel<2048>, dim3(bgrid), dim3(256), 0, stream_, sb);
     SHM_HIP(hipGetLastError());
     bins.assign(nbins, 0.);
-    SHM_HIP(hipMemcpyAsync(bins.data(), dbins.get(), sizeof(double) * nbins,
+    SHM_HIP(hipMemcpyAsync(bins.data(), w.dbins.get(), sizeof(double) * nbins,
                            hipMemcpyDeviceToHost, stream_));
     SHM_HIP(hipStreamSynchronize(stream_));
     // pre = t / (dk N^6), dk = 2 pi / L (spectrum.cpp:49-51).
@@ -2909,8 +2972,13 @@ void Runtime::spectrum(int fn, std::vector<double> &bins, std::vector<long long>
     for (double &v : bins) v *= pre;
 }
 
+void Runtime::release_out_cache() {
+    if (osc_.have_plan) (void)hipfftDestroy(osc_.plan);
+    osc_ = OutSpecCache();
+}
+
 void Runtime::projection(int fn, int mode, int max_level, std::vector<double> &data,
-                         std::vector<int> &count, int &D) {
+                         std::vector<int> &count, int &D, bool count_only) {
     check_output_fn(fn);
     if (mode != 0 && mode != 1) throw Error(1, "projection mode must be 0 (sum) or 1 (max)");
     int ml = max_level < 0 ? INT32_MAX : max_level;
@@ -2918,6 +2986,7 @@ void Runtime::projection(int fn, int mode, int max_level, std::vector<double> &d
     while (has_level(finest + 1)) ++finest;
     ml = std::min(ml, finest);
     D = sim_->dimN[ml];
+    if (count_only) return;
     std::vector<double> p;
     sim_->SetParamsProjections(p, level(0).t);
     p.resize(std::max<size_t>(p.size(), 1), 0.);

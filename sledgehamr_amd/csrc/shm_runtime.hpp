@@ -113,6 +113,7 @@ struct LevelState {
     DevMem K[2];               // SSPRK3 k accumulators (k1, k2)
     DevMem F[5];               // RKN forces
     DevMem Y2;                 // RKN second-stage field states (rkn_state)
+    DevMem Y3;                 // RKN third-stage field states (three stages)
     DevMem VH;                 // leapfrog half-step momenta (FP64)
     DevMem RHS;                // FillRhs target
     double t = -DBL_MAX;       // time of grid_new
@@ -240,15 +241,17 @@ class Runtime {
      *  modifier `modifier` (-1: default components, unmodified). */
     void gw_spectrum(int mode, int nbins, int zero_padding, int proj_type,
                      std::vector<double> &bins, std::vector<double> &kvals,
-                     int modifier = -1, const std::vector<double> &mparams = {});
+                     int modifier = -1, const std::vector<double> &mparams = {},
+                     bool count_only = false);
     void step(int ncoarse);              // TimeStepper::Advance(0)
     /** Spectrum::Compute of output function fn on level 0: bins per unique
      *  |k|^2 (k_sq). */
-    void spectrum(int fn, std::vector<double> &bins, std::vector<long long> &k_sq);
+    void spectrum(int fn, std::vector<double> &bins, std::vector<long long> &k_sq,
+                  bool count_only = false);
     /** Projection::Compute of output function fn along z onto the finest
      *  level up to max_level (< 0: all levels); D x D pixels, x index slow. */
     void projection(int fn, int mode, int max_level, std::vector<double> &data,
-                    std::vector<int> &count, int &D);
+                    std::vector<int> &count, int &D, bool count_only = false);
     int output_count() const { return kt_.nout; }
     const KernelTable &kernels() const { return kt_; }
     const char *output_name(int id) const {
@@ -518,6 +521,16 @@ class Runtime {
         int nbin_eff = 0;
     } gwc_;
     void release_gw_cache();
+    /** Scalar spectrum work space (plan, buffers, unique |k|^2 table) for
+     *  one level size, kept between calls. */
+    struct OutSpecCache {
+        int N = 0;
+        hipfftHandle plan = 0;
+        bool have_plan = false;
+        DevMem dense, F, dsq, dbins, dparams;
+        std::vector<long long> k_sq;
+    } osc_;
+    void release_out_cache();
     // Component names: the project's scalars, then the GW tensor.
     std::vector<std::string> names_;
 

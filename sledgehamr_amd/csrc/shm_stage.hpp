@@ -120,6 +120,7 @@ struct StageArgs {
     const void *kin;    // SSPRK3 k in / FillAddRhs previous rhs
     void *kout;         // SSPRK3 k out / FillRhs out
     void *y;            // SSPRK3 stage state out
+    void *y2;           // RKN third-stage field states (PolRkn1YY, PolRknF2Y)
     void *unew;         // new state (hi)
     void *unewlo;       // new state remainder
     const void *f[5];   // RKN forces of earlier stages
@@ -684,6 +685,75 @@ struct PolRkn1Y : PolRkn<1, false> {
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
         const StageArgs &a, long long org, int n, unsigned off, typename T::V *) {
         return load_st<CS, T>(a, org, n, off);
+    }
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_keep<CS, T>(a, org, n, off, keep);
+    }
+};
+
+/** Second stage of a three-stage RKN step (RKN4) that reads the field
+ *  states PolRkn0Y wrote and also writes, per pair, the final stage's field
+ *  state y2 = phi0 + c[12] pi0 + c[13] F0 + c[14] F1, formed in the storage
+ *  type exactly as PolRkn<2, true> forms it from the stored phi0, pi0, F0
+ *  and the F1 written here (same terms, order and rounding).  The final
+ *  stage (PolRknF2Y) then reads y2 at its stencil points instead of
+ *  composing it from four arrays there.  Tiles of the storage type only. */
+struct PolRkn1YY : PolRkn1Y {
+    template <class CS, class T, int NK>
+    __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
+                                                    long long org, unsigned off,
+                                                    const double *rhs,
+                                                    const double *ctr,
+                                                    const typename T::S *e,
+                                                    const typename T::V (*kc)[NK]) {
+        using S = typename T::S;
+        PolRkn<1, false>::template store<CS, T, NK>(a, org, off, rhs, ctr, e, kc);
+#pragma unroll
+        for (int o = 0; o < CS::NOP; ++o) {
+            const int p = CS::out_pair(o);
+            const S f1 = (S)rhs[CS::pair_mom(p)];
+            const S phi0 = ld<S>(a.u0, org + CS::pair_field(p) * a.cstride, off);
+            const S f0 = ld<S>(a.f[0], org + p * a.cstride, off);
+            S y = phi0;
+            y += (S)a.c[12] * (S)ctr[CS::pair_mom(p)];
+            y += (S)a.c[13] * f0;
+            y += (S)a.c[14] * f1;
+            st<S>(a.y2, org + p * a.cstride, off, y);
+        }
+    }
+};
+
+/** Final stage of a three-stage RKN step reading the field states
+ *  PolRkn1YY wrote at its stencil points; phi0, pi0, F0 and F1 are read at
+ *  the centre for the update, as PolRkn<2, true> keeps them. */
+struct PolRknF2Y : PolRkn<2, true> {
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
+                                                               long long org, int n,
+                                                               unsigned off) {
+        using S = typename T::S;
+        using V = typename T::V;
+        const int p = CS::pair_of_field(n);
+        if (p < 0) return ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
+        return (V)ld<S>(a.y2, org + p * a.cstride, off);
+    }
+    template <class CS, class T>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
+        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        using S = typename T::S;
+        using V = typename T::V;
+        const int p = CS::pair_of_field(n);
+        if (p < 0) return ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
+        // Fields a component group stages without updating (the scalars in
+        // the GW tensor groups) need no centre values.
+        if (!CS::out(n)) return (V)ld<S>(a.y2, org + p * a.cstride, off);
+        keep[0] = ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
+        keep[1] = ld_tile<T>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+        keep[2] = (V)ld<S>(a.f[0], org + p * a.cstride, off);
+        keep[3] = (V)ld<S>(a.f[1], org + p * a.cstride, off);
+        return (V)ld<S>(a.y2, org + p * a.cstride, off);
     }
     template <class CS, class T>
     __device__ AMREX_FORCE_INLINE static typename T::V load_pt(

@@ -36,6 +36,9 @@ enum StageKind : int {
     kRkn0Y,        // RKN first stage at node 0 that also writes the second
                    // stage's field states
     kRkn1Y,        // RKN second stage reading them
+    kRkn1YY,       // the same in a three-stage step, also writing the third
+                   // stage's field states
+    kRknF2Y,       // final stage of a three-stage step reading them
     kNKind
 };
 

@@ -15,8 +15,9 @@ from cases import bubble_params, phase_transition
 pytestmark = pytest.mark.gpu
 
 
-def _run(prec, integrator, split, steps=3, N=32):
-    inp = phase_transition(N=N, integrator=integrator, **{"shm.gw_split": split})
+def _run(prec, integrator, split, steps=3, N=32, rkn_state=1):
+    inp = phase_transition(N=N, integrator=integrator,
+                           **{"shm.gw_split": split, "shm.rkn_state": rkn_state})
     sim = shm.Simulation("PhaseTransitionGW", inp, precision=prec)
     sim.init_state(0, 3, 7, bubble_params(8, -1e30, 0.0))
     t = sim.level_info(0)[1]
@@ -37,5 +38,18 @@ def test_groups_equal_single_pass(prec, integrator):
     b, gb = _run(prec, integrator, 0)
     assert ga == 1 and gb == 0
     assert max(np.max(np.abs(x)) for x in a[2:]) > 0  # tensor sourced
+    for c in range(14):
+        assert np.array_equal(a[c], b[c]), c
+
+
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("split", [0, 1])
+def test_rkn_stored_stage_states_equal_composed(prec, split):
+    """RKN4 with the stage states stored by the stage before (the second
+    stage's by the first, the third's by the second; shm.rkn_state 1)
+    against states composed from phi0, pi0 and the forces at every stencil
+    point (0): identical bits, with and without component groups."""
+    a, _ = _run(prec, 21, split, rkn_state=1)
+    b, _ = _run(prec, 21, split, rkn_state=0)
     for c in range(14):
         assert np.array_equal(a[c], b[c]), c
