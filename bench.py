@@ -23,7 +23,10 @@ and writes u with its remainder (64 B per cell).  Launch times are HIP
 events on the library's stream inside the timed region.  `traffic` is the
 DRAM bytes of one launch of that kernel measured with rocprofv3 counters in
 the profile named by `traffic_source` (same workload, kernel and launch
-shape), not in this run; null when this run's shape differs.
+shape), not in this run; null when this run's shape differs.  The FP32 +
+bf16 headline runs in that profiled shape by default (the fastest of a sweep
+of the autotuner's candidates, within 1 % of its pick; `--launch auto` runs
+the autotuner); FP32CS and FP64 take the autotuner's shapes.
 
 Further blocks, each the one-GPU workload of a configuration with its own
 FP64 comparison, rate, stage-kernel HBM fraction and CPU baseline:
@@ -94,7 +97,12 @@ def rkn_bytes_per_step(key, npair, nstages=3):
 TRAFFIC_PROFILE = "profiles/stage_kernel_traffic_complex512.md"
 TRAFFIC_BYTES_512 = {"fused12": (74270667 + 67108864) * 32.0,
                      "stage3": (174360952 + 100663296) * 32.0}
-TRAFFIC_SHAPE = {"fused12": {"fused_variant": 2}, "stage3": {"variant": 1}}
+# The launch shape those counters were measured in (every field must match),
+# which is also the headline's default launch shape (HEADLINE_LAUNCH).
+PROFILED_SHAPE = {"variant": 2, "tile_z": 512, "xcd_split": 1, "persistent": 0,
+                  "zc_minor": 0, "fused": 1, "fused_variant": 2}
+TRAFFIC_SHAPE = {"fused12": PROFILED_SHAPE, "stage3": PROFILED_SHAPE}
+HEADLINE_LAUNCH = "2,512,1,0,0,1,2"
 
 
 def kernel_record(name, avg_ms, nbytes, traffic=None, source=None):
@@ -252,7 +260,10 @@ def headline(shm, a, launch):
                       ("fp64", shm.PREC_FP64)):
         if key != "fp32c16" and a.no_fp64:
             continue
-        sim, sec, tm = run_gpu(shm, inp, prec, a.steps, a.warmup, a.seed, launch)
+        # The headline precision runs in the given shape (by default the
+        # profiled one); the others take the autotuner's pick.
+        sim, sec, tm = run_gpu(shm, inp, prec, a.steps, a.warmup, a.seed,
+                               launch if key == "fp32c16" else None)
         cells = sim.ncells(0)
         ms = sec / a.steps * 1e3
         lc = sim.get_launch(0)
@@ -584,12 +595,15 @@ def main():
                          "(for profiling)")
     ap.add_argument("--amr-no-autotune", action="store_true",
                     help="default launch shapes in the batched AMR block")
-    ap.add_argument("--launch", default="",
-                    help="fixed stage launch shape 'variant,tile_z,xcd_split,"
-                         "persistent,zc_minor[,fused,fused_variant]' instead "
-                         "of the autotuner")
+    ap.add_argument("--launch", default=HEADLINE_LAUNCH,
+                    help="stage launch shape of the headline workload "
+                         "'variant,tile_z,xcd_split,persistent,zc_minor[,fused,"
+                         "fused_variant]' (default: the shape its DRAM traffic "
+                         "was profiled in, the fastest of a sweep of the "
+                         "autotuner's candidates); 'auto': the autotuner")
     a = ap.parse_args()
-    launch = [int(v) for v in a.launch.split(",")] if a.launch else None
+    launch = ([int(v) for v in a.launch.split(",")]
+              if a.launch and a.launch != "auto" else None)
     if launch is not None and len(launch) not in (5, 7):
         ap.error("--launch takes five or seven integers")
     if a.gpus != 1:

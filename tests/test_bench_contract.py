@@ -28,7 +28,7 @@ def test_fused_schedule_roofline():
     tm = {"stage_launches": 2 * steps, "fused_launches": steps,
           "stage_cells": 3 * n * steps, "stage_ms": steps * (1.5 + 1.7),
           "fused_ms": steps * 1.5}
-    launch = {"fused": 1, "fused_variant": 2, "variant": 1}
+    launch = dict(bench.PROFILED_SHAPE)
     r = bench.headline_roofline(tm, "fp32c16", launch, n)
     assert r["kernel"].startswith("shm::stage_kernel")
     assert r["avg_launch_ms"] == pytest.approx(1.7)
@@ -45,10 +45,16 @@ def test_fused_schedule_roofline():
     # Moves u0 in and k2 out (4 B each).
     assert o["bytes_per_launch"] == pytest.approx(4 * 8 * n)
     assert o["frac"] < 1
-    # Another launch shape than the profiled one: no traffic figure.
-    r2 = bench.headline_roofline(tm, "fp32c16", {"fused": 1, "fused_variant": 0,
-                                                 "variant": 4}, n)
-    assert r2["traffic"] is None
+    # Another launch shape than the profiled one (any field): no traffic.
+    for field, v in (("fused_variant", 0), ("variant", 4), ("xcd_split", 0),
+                     ("tile_z", 0)):
+        r2 = bench.headline_roofline(tm, "fp32c16", dict(launch, **{field: v}), n)
+        assert r2["traffic"] is None, field
+    # The headline's default launch string is the profiled shape.
+    keys = ("variant", "tile_z", "xcd_split", "persistent", "zc_minor", "fused",
+            "fused_variant")
+    assert dict(zip(keys, map(int, bench.HEADLINE_LAUNCH.split(",")))) == \
+        bench.PROFILED_SHAPE
 
 
 def test_unfused_schedule_roofline():
