@@ -96,8 +96,11 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     const int g = a.ghost;
     const double t1 = a.prm1[kPrmTime], t2 = a.prm2[kPrmTime];
     const double dt = a.prm1[kPrmDt], dx = a.prm1[kPrmDx];
-    const double *par1 = a.prm1 + kPrmUser, *par2 = a.prm2 + kPrmUser;
-    const double *gpar1 = a.prm1 + kPrmGw, *gpar2 = a.prm2 + kPrmGw;
+    // Parameters in registers, read once (a read in the plane loop would
+    // wait for the next plane's prefetch).
+    const ParamRegs<GW> pr1(a.prm1), pr2(a.prm2);
+    const double *par1 = pr1.user, *par2 = pr2.user;
+    const double *gpar1 = pr1.gw_ptr(a.prm1), *gpar2 = pr2.gw_ptr(a.prm2);
     const V cdt = (V)a.dt;
 
     // Stage-one points of this thread: its own cell (x = tx, y = w), then a

@@ -99,6 +99,29 @@ constexpr int kMaxParams = 64;
 constexpr int kPrmGw = kPrmUser + kMaxParams;  // SetParamsGravitationalWaveRhs()
 constexpr int kPrmBlock = kPrmGw + kMaxParams;
 
+/** Register copy of a stage record's user (and, with GW, GW) parameters,
+ *  taken at kernel entry.  The projects index them with constants, so the
+ *  copies of unused entries are removed and the rest stay in registers. */
+template <bool GW>
+struct ParamRegs {
+    double user[kMaxParams];
+    double gw[GW ? kMaxParams : 1];
+    __device__ AMREX_FORCE_INLINE explicit ParamRegs(const double *prm) {
+#pragma unroll
+        for (int i = 0; i < kMaxParams; ++i) user[i] = prm[kPrmUser + i];
+        if constexpr (GW) {
+#pragma unroll
+            for (int i = 0; i < kMaxParams; ++i) gw[i] = prm[kPrmGw + i];
+        }
+    }
+    __device__ AMREX_FORCE_INLINE const double *gw_ptr(const double *prm) const {
+        if constexpr (GW)
+            return gw;
+        else
+            return prm + kPrmGw;
+    }
+};
+
 /** Kernel argument block shared by all stage policies. */
 struct StageArgs {
     const BoxDesc *boxes;
@@ -902,8 +925,12 @@ __global__ void __launch_bounds__(64 * NW)
         hl[m] = (hy[m] + R) * PX + (hx[m] + R);
     }
     const double time = a.prm[kPrmTime], dt = a.prm[kPrmDt], dx = a.prm[kPrmDx];
-    const double *params = a.prm + kPrmUser;
-    const double *gw_params = a.prm + kPrmGw;
+    // The parameters are read once, here: a read inside the plane loop would
+    // make the right-hand side wait for every load in flight before it,
+    // the next plane's prefetch included.
+    const ParamRegs<GW> pr(a.prm);
+    const double *params = pr.user;
+    const double *gw_params = pr.gw_ptr(a.prm);
 
     // Persistent walk over the tile table.  With xcd_split the tiles are cut
     // into 8 contiguous ranges, one per XCD (workgroups are dealt to XCDs
