@@ -224,6 +224,33 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         S dk1[R + 1][NC];    // k1 of the own cell, planes zz-R .. zz
         V du1[R + 1][NPA];   // u1 of the point components, same planes
 
+        // A plane's global stores are issued at the top of the next plane,
+        // after its window stores have waited for the prefetched loads and
+        // before the next prefetch: a store between a load and its use
+        // would make that use wait for the store too (the wait counts
+        // loads and stores in issue order).
+        constexpr int NPR = CS::NPAIR > 0 ? CS::NPAIR : 1;
+        constexpr int NKS = RKN ? NPR : NC;
+        S pend_k[NKS];       // stage two's k2 (SSPRK3) or F1 (RKN)
+        S pend_f[NPR];       // RKN: stage one's F0
+        bool has_k = false, has_f = false;
+        unsigned o_k = 0, o_f = 0;
+        auto flush = [&]() {
+            if (has_k) {
+#pragma unroll
+                for (int n = 0; n < NKS; ++n) st<S>(a.kout, org + n * a.cstride, o_k, pend_k[n]);
+            }
+            if constexpr (RKN) {
+                if (has_f) {
+#pragma unroll
+                    for (int p = 0; p < CS::NPAIR; ++p)
+                        st<S>(a.f0out, org + p * a.cstride, o_f, pend_f[p]);
+                }
+            }
+            has_k = false;
+            has_f = false;
+        };
+
         for (int zz = td.z0 - R; zz < td.z1 + R; ++zz) {
             // Stage one runs on plane zz: window planes zz-R .. zz+R.
             store_window();
@@ -232,6 +259,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 #pragma unroll
                 for (int m = 0; m < L1; ++m)
                     if (e1ok[m]) p0[p][l1[m]] = vp[p][m];
+            flush();
             if (zz + 1 < td.z1 + R) {
                 load_window();
                 load_point();
@@ -320,12 +348,10 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                     if (m == 0) {
 #pragma unroll
                         for (int p = 0; p < NP; ++p) du1[R][p] = state2(CS::comp_of_point(p));
-                        if (own && zz >= td.z0 && zz < td.z1) {
-                            const unsigned o = cown + (unsigned)(zz + g) * ks;
+                        has_f = own && zz >= td.z0 && zz < td.z1;
+                        o_f = cown + (unsigned)(zz + g) * ks;
 #pragma unroll
-                            for (int p = 0; p < CS::NPAIR; ++p)
-                                st<S>(a.f0out, org + p * a.cstride, o, f0[p]);
-                        }
+                        for (int p = 0; p < CS::NPAIR; ++p) pend_f[p] = f0[p];
                     }
                 } else {
                     // k1 in the storage type, as the two-launch path stores
@@ -400,21 +426,17 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             }
             const unsigned zso = (unsigned)(zst + g) * ks;
             zst = next(zst, nz);
-            if (own) {
-                const unsigned o = cown + zso;
-                if constexpr (RKN) {
+            has_k = own;
+            o_k = cown + zso;
+            if constexpr (RKN) {
 #pragma unroll
-                    for (int p = 0; p < CS::NPAIR; ++p)
-                        st<S>(a.kout, org + p * a.cstride, o, (S)rhs[CS::pair_mom(p)]);
-                } else {
+                for (int p = 0; p < CS::NPAIR; ++p) pend_k[p] = (S)rhs[CS::pair_mom(p)];
+            } else {
 #pragma unroll
-                    for (int n = 0; n < NC; ++n) {
-                        const double k = rhs[n] + (double)dk1[0][n];
-                        st<S>(a.kout, org + n * a.cstride, o, (S)k);
-                    }
-                }
+                for (int n = 0; n < NC; ++n) pend_k[n] = (S)(rhs[n] + (double)dk1[0][n]);
             }
         }
+        flush();
         __syncthreads();  // the next tile's prologue rewrites the windows
     }
 }

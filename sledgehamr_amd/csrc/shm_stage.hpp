@@ -370,39 +370,142 @@ __device__ AMREX_FORCE_INLINE double pair_value(double u) {
             return (double)h + (double)bf16_to_f32(f32_to_bf16_rne(r));
     }
 }
+/** Memory sources of the load helpers and of the policies' load hooks.
+ *
+ *  MemLoad reads at once.  MemFetch issues the same loads and keeps their
+ *  raw words, in call order, without using them; MemReplay hands those
+ *  words to a second call of the same hook (same arguments), which then
+ *  forms its values.  The stage kernel fetches the next plane before a
+ *  plane's right-hand sides and replays after them, so the arithmetic that
+ *  forms a stage state from several arrays does not wait for the loads
+ *  before the plane's work.  One hook call reads at most kRawWords 32-bit
+ *  words: the largest is a final RKN stage of 5 stages in FP64 (phi0, pi0
+ *  and four forces, 12 words). */
+constexpr int kRawWords = 12;
+template <int K>
+struct RawWords {
+    uint32_t w[K];
+};
+struct MemLoad {
+    template <class S>
+    __device__ AMREX_FORCE_INLINE S ld(const void *p, long long b, unsigned off) {
+        return ::shm::ld<S>(p, b, off);
+    }
+    template <class S>
+    __device__ AMREX_FORCE_INLINE S ld_nt(const void *p, long long b, unsigned off) {
+        return ::shm::ld_nt<S>(p, b, off);
+    }
+};
+template <int K>
+struct MemFetch {
+    RawWords<K> &r;
+    int i = 0;
+    __device__ AMREX_FORCE_INLINE explicit MemFetch(RawWords<K> &raw) : r(raw) {}
+    template <class S>
+    __device__ AMREX_FORCE_INLINE void put(S v) {
+        if constexpr (sizeof(S) == 8) {
+            const unsigned long long u = (unsigned long long)__double_as_longlong((double)v);
+            r.w[i++] = (uint32_t)u;
+            r.w[i++] = (uint32_t)(u >> 32);
+        } else if constexpr (std::is_same_v<S, float>) {
+            r.w[i++] = __float_as_uint(v);
+        } else {
+            r.w[i++] = (uint32_t)v;
+        }
+    }
+    template <class S>
+    __device__ AMREX_FORCE_INLINE S ld(const void *p, long long b, unsigned off) {
+        put<S>(::shm::ld<S>(p, b, off));
+        return S();
+    }
+    template <class S>
+    __device__ AMREX_FORCE_INLINE S ld_nt(const void *p, long long b, unsigned off) {
+        put<S>(::shm::ld_nt<S>(p, b, off));
+        return S();
+    }
+};
+template <int K>
+struct MemReplay {
+    const RawWords<K> &r;
+    int i = 0;
+    __device__ AMREX_FORCE_INLINE explicit MemReplay(const RawWords<K> &raw) : r(raw) {}
+    template <class S>
+    __device__ AMREX_FORCE_INLINE S get() {
+        if constexpr (sizeof(S) == 8) {
+            const unsigned long long u =
+                (unsigned long long)r.w[i] | ((unsigned long long)r.w[i + 1] << 32);
+            i += 2;
+            return (S)__longlong_as_double((long long)u);
+        } else if constexpr (std::is_same_v<S, float>) {
+            return __uint_as_float(r.w[i++]);
+        } else {
+            return (S)r.w[i++];
+        }
+    }
+    template <class S>
+    __device__ AMREX_FORCE_INLINE S ld(const void *, long long, unsigned) {
+        return get<S>();
+    }
+    template <class S>
+    __device__ AMREX_FORCE_INLINE S ld_nt(const void *, long long, unsigned) {
+        return get<S>();
+    }
+};
+
 /** Remainder of the stored pair as FP64 (0 without compensation). */
-template <class T>
-__device__ AMREX_FORCE_INLINE double load_lo(const void *lo, long long base,
+template <class T, class M>
+__device__ AMREX_FORCE_INLINE double load_lo(M &m, const void *lo, long long base,
                                              unsigned off) {
     if constexpr (!T::comp)
         return 0.;
     else if constexpr (std::is_same_v<typename T::L, float>)
-        return (double)ld_nt<float>(lo, base, off);
+        return (double)m.template ld_nt<float>(lo, base, off);
     else
-        return (double)bf16_to_f32(ld_nt<uint16_t>(lo, base, off));
+        return (double)bf16_to_f32(m.template ld_nt<uint16_t>(lo, base, off));
+}
+template <class T>
+__device__ AMREX_FORCE_INLINE double load_lo(const void *lo, long long base,
+                                             unsigned off) {
+    MemLoad m;
+    return load_lo<T>(m, lo, base, off);
+}
+template <class T, class M>
+__device__ AMREX_FORCE_INLINE double load_state(M &m, const void *hi, const void *lo,
+                                                long long base, unsigned off) {
+    return (double)m.template ld<typename T::S>(hi, base, off) + load_lo<T>(m, lo, base, off);
 }
 template <class T>
 __device__ AMREX_FORCE_INLINE double load_state(const void *hi, const void *lo,
                                                 long long base, unsigned off) {
-    return (double)ld<typename T::S>(hi, base, off) + load_lo<T>(lo, base, off);
+    MemLoad m;
+    return load_state<T>(m, hi, lo, base, off);
 }
 /** Tile value (type V) of a stored state: the stored word for FP32 and FP64
  *  tiles, hi + lo for the FP64 tiles of a stored pair.  NT: the value is
  *  read at the centre only (non-temporal loads). */
-template <class T, bool NT = false>
-__device__ AMREX_FORCE_INLINE typename T::V ld_tile(const void *hi, const void *lo,
+template <class T, bool NT = false, class M>
+__device__ AMREX_FORCE_INLINE typename T::V ld_tile(M &m, const void *hi, const void *lo,
                                                     long long base, unsigned off) {
     using S = typename T::S;
     if constexpr (T::comp && T::exact) {
-        const double h = NT ? (double)ld_nt<S>(hi, base, off) : (double)ld<S>(hi, base, off);
+        const double h = NT ? (double)m.template ld_nt<S>(hi, base, off)
+                            : (double)m.template ld<S>(hi, base, off);
         if constexpr (std::is_same_v<typename T::L, float>)
-            return h + (double)(NT ? ld_nt<float>(lo, base, off) : ld<float>(lo, base, off));
+            return h + (double)(NT ? m.template ld_nt<float>(lo, base, off)
+                                   : m.template ld<float>(lo, base, off));
         else
-            return h + (double)bf16_to_f32(NT ? ld_nt<uint16_t>(lo, base, off)
-                                               : ld<uint16_t>(lo, base, off));
+            return h + (double)bf16_to_f32(NT ? m.template ld_nt<uint16_t>(lo, base, off)
+                                               : m.template ld<uint16_t>(lo, base, off));
     } else {
-        return (typename T::V)(NT ? ld_nt<S>(hi, base, off) : ld<S>(hi, base, off));
+        return (typename T::V)(NT ? m.template ld_nt<S>(hi, base, off)
+                                  : m.template ld<S>(hi, base, off));
     }
+}
+template <class T, bool NT = false>
+__device__ AMREX_FORCE_INLINE typename T::V ld_tile(const void *hi, const void *lo,
+                                                    long long base, unsigned off) {
+    MemLoad m;
+    return ld_tile<T, NT>(m, hi, lo, base, off);
 }
 /** Same, addressed by a 64-bit element index (auxiliary kernels). */
 template <class T>
@@ -456,8 +559,8 @@ struct PolSsprk {
     }
     /** Stage state of component n: u0 + c2 k_in in the tile type (composed
      *  stages), else the source state. */
-    template <class CS, class T, bool NT>
-    __device__ AMREX_FORCE_INLINE static typename T::V load_x(const StageArgs &a,
+    template <class CS, class T, bool NT, class M>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_x(M &m, const StageArgs &a,
                                                               long long org, int n,
                                                               unsigned off,
                                                               typename T::V *keep) {
@@ -465,37 +568,38 @@ struct PolSsprk {
         using V = typename T::V;
         const long long b = org + n * a.cstride;
         if constexpr (composed) {
-            const V k = (V)(NT ? ld_nt<S>(a.kin, b, off) : ld<S>(a.kin, b, off));
-            const V u = ld_tile<T, NT>(a.u0, a.u0lo, b, off);
+            const V k = (V)(NT ? m.template ld_nt<S>(a.kin, b, off)
+                               : m.template ld<S>(a.kin, b, off));
+            const V u = ld_tile<T, NT>(m, a.u0, a.u0lo, b, off);
             if (keep) {
                 keep[0] = k;
                 if constexpr (K == kSsStage3) keep[1] = u;
             }
             return fma_st<V>((V)a.c[2], k, u);
         } else {
-            return ld_tile<T, NT>(a.x, a.xlo, b, off);
+            return ld_tile<T, NT>(m, a.x, a.xlo, b, off);
         }
     }
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(M &m, const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        return load_x<CS, T, false>(a, org, n, off, nullptr);
+        return load_x<CS, T, false>(m, a, org, n, off, nullptr);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
-        return load_x<CS, T, false>(a, org, n, off, keep);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_x<CS, T, false>(m, a, org, n, off, keep);
     }
     /** load_keep for a point component: read at the centre only, so
      *  non-temporal (the stencil components' halos keep L2). */
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
-        return load_x<CS, T, true>(a, org, n, off, keep);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_x<CS, T, true>(m, a, org, n, off, keep);
     }
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static void load_ext(M &m, const StageArgs &a,
                                                        long long org, unsigned off,
                                                        typename T::S *e) {
         using S = typename T::S;
@@ -505,8 +609,8 @@ struct PolSsprk {
             const int n = CS::out_comp(o);
             const long long b = org + n * a.cstride;
             if constexpr (K == kSsStage3 && T::comp && !T::exact)
-                e[i++] = (S)load_lo<T>(a.u0lo, b, off);
-            if constexpr (K == kSsFillAdd) e[i++] = ld<S>(a.kin, b, off);
+                e[i++] = (S)load_lo<T>(m, a.u0lo, b, off);
+            if constexpr (K == kSsFillAdd) e[i++] = m.template ld<S>(a.kin, b, off);
         }
     }
     template <class CS, class T, int NK>
@@ -562,19 +666,19 @@ struct PolRkn {
     /** Stage state in the tile type (FP32 tiles hold FP32 values, so their
      *  sums are formed in FP32; FP64 tiles of stored pairs start from
      *  hi + lo). */
-    template <class CS, class T, bool NT>
+    template <class CS, class T, bool NT, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_y(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
         using S = typename T::S;
         using V = typename T::V;
         const int p = CS::pair_of_field(n);
-        const V phi0 = ld_tile<T, NT>(a.u0, a.u0lo, org + n * a.cstride, off);
+        const V phi0 = ld_tile<T, NT>(m, a.u0, a.u0lo, org + n * a.cstride, off);
         if (p < 0) return phi0;
         // A first stage at node 0 (RKN4, RKN5) has the state phi0: pi0 is
         // then read at the centre only.
         if constexpr (!FINAL && NPREV == 0)
             if (a.c[0] == 0.) return phi0;
-        const V pi0 = ld_tile<T, NT>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+        const V pi0 = ld_tile<T, NT>(m, a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
         V y = phi0;
         y += (V)a.c[0] * pi0;
         if constexpr (FINAL) {
@@ -583,32 +687,32 @@ struct PolRkn {
         }
 #pragma unroll
         for (int j = 0; j < NPREV; ++j) {
-            const V fj = (V)(NT ? ld_nt<S>(a.f[j], org + p * a.cstride, off)
-                                : ld<S>(a.f[j], org + p * a.cstride, off));
+            const V fj = (V)(NT ? m.template ld_nt<S>(a.f[j], org + p * a.cstride, off)
+                                : m.template ld<S>(a.f[j], org + p * a.cstride, off));
             if constexpr (FINAL) keep[2 + j] = fj;
             y += (V)a.c[1 + j] * fj;
         }
         return y;
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
-        return load_y<CS, T, false>(a, org, n, off, keep);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_y<CS, T, false>(m, a, org, n, off, keep);
     }
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(M &m, const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
         typename T::V keep[FINAL ? 2 + NPREV : 1];
-        return load_y<CS, T, false>(a, org, n, off, keep);
+        return load_y<CS, T, false>(m, a, org, n, off, keep);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
-        return load_y<CS, T, false>(a, org, n, off, keep);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_y<CS, T, false>(m, a, org, n, off, keep);
     }
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static void load_ext(M &m, const StageArgs &a,
                                                        long long org, unsigned off,
                                                        typename T::S *e) {
         using S = typename T::S;
@@ -617,8 +721,8 @@ struct PolRkn {
 #pragma unroll
             for (int o = 0; o < CS::NOP; ++o) {
                 const int p = CS::out_pair(o);
-                e[i++] = (S)load_lo<T>(a.u0lo, org + CS::pair_field(p) * a.cstride, off);
-                e[i++] = (S)load_lo<T>(a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+                e[i++] = (S)load_lo<T>(m, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
+                e[i++] = (S)load_lo<T>(m, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
             }
         }
     }
@@ -694,25 +798,25 @@ struct PolRkn0Y : PolRkn<0, false> {
 /** Second RKN stage reading the field states PolRkn0Y wrote; momenta stay
  *  at pi0.  Writes F1 as PolRkn<1, false> does. */
 struct PolRkn1Y : PolRkn<1, false> {
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(M &m, const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
         using S = typename T::S;
         using V = typename T::V;
         const int p = CS::pair_of_field(n);
-        if (p < 0) return ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
-        return (V)ld<S>(a.y, org + p * a.cstride, off);
+        if (p < 0) return ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
+        return (V)m.template ld<S>(a.y, org + p * a.cstride, off);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *) {
-        return load_st<CS, T>(a, org, n, off);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *) {
+        return load_st<CS, T>(m, a, org, n, off);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
-        return load_keep<CS, T>(a, org, n, off, keep);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_keep<CS, T>(m, a, org, n, off, keep);
     }
 };
 
@@ -752,36 +856,36 @@ struct PolRkn1YY : PolRkn1Y {
  *  PolRkn1YY wrote at its stencil points; phi0, pi0, F0 and F1 are read at
  *  the centre for the update, as PolRkn<2, true> keeps them. */
 struct PolRknF2Y : PolRkn<2, true> {
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(M &m, const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
         using S = typename T::S;
         using V = typename T::V;
         const int p = CS::pair_of_field(n);
-        if (p < 0) return ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
-        return (V)ld<S>(a.y2, org + p * a.cstride, off);
+        if (p < 0) return ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
+        return (V)m.template ld<S>(a.y2, org + p * a.cstride, off);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
         using S = typename T::S;
         using V = typename T::V;
         const int p = CS::pair_of_field(n);
-        if (p < 0) return ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
+        if (p < 0) return ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
         // Fields a component group stages without updating (the scalars in
         // the GW tensor groups) need no centre values.
-        if (!CS::out(n)) return (V)ld<S>(a.y2, org + p * a.cstride, off);
-        keep[0] = ld_tile<T>(a.u0, a.u0lo, org + n * a.cstride, off);
-        keep[1] = ld_tile<T>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
-        keep[2] = (V)ld<S>(a.f[0], org + p * a.cstride, off);
-        keep[3] = (V)ld<S>(a.f[1], org + p * a.cstride, off);
-        return (V)ld<S>(a.y2, org + p * a.cstride, off);
+        if (!CS::out(n)) return (V)m.template ld<S>(a.y2, org + p * a.cstride, off);
+        keep[0] = ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
+        keep[1] = ld_tile<T>(m, a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+        keep[2] = (V)m.template ld<S>(a.f[0], org + p * a.cstride, off);
+        keep[3] = (V)m.template ld<S>(a.f[1], org + p * a.cstride, off);
+        return (V)m.template ld<S>(a.y2, org + p * a.cstride, off);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
-        return load_keep<CS, T>(a, org, n, off, keep);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_keep<CS, T>(m, a, org, n, off, keep);
     }
 };
 
@@ -798,24 +902,24 @@ struct PolLeapfrog {
     template <class CS, class T>
     static constexpr int next() { return CS::NOP * (KICK2 ? 1 : 2); }
 
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static typename T::V load_st(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static typename T::V load_st(M &m, const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        return ld_tile<T>(a.x, a.xlo, org + n * a.cstride, off);
+        return ld_tile<T>(m, a.x, a.xlo, org + n * a.cstride, off);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *) {
-        return load_st<CS, T>(a, org, n, off);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *) {
+        return load_st<CS, T>(m, a, org, n, off);
     }
-    template <class CS, class T>
+    template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_pt(
-        const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
-        return load_keep<CS, T>(a, org, n, off, keep);
+        M &m, const StageArgs &a, long long org, int n, unsigned off, typename T::V *keep) {
+        return load_keep<CS, T>(m, a, org, n, off, keep);
     }
-    template <class CS, class T>
-    __device__ AMREX_FORCE_INLINE static void load_ext(const StageArgs &a,
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static void load_ext(M &m, const StageArgs &a,
                                                        long long org, unsigned off,
                                                        double *e) {
         int i = 0;
@@ -823,10 +927,10 @@ struct PolLeapfrog {
         for (int o = 0; o < CS::NOP; ++o) {
             const int p = CS::out_pair(o);
             if constexpr (!KICK2) {
-                e[i++] = load_state<T>(a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
-                e[i++] = load_state<T>(a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
+                e[i++] = load_state<T>(m, a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
+                e[i++] = load_state<T>(m, a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
             } else {
-                e[i++] = ld<double>(a.vh, org + p * a.cstride, off);
+                e[i++] = m.template ld<double>(a.vh, org + p * a.cstride, off);
             }
         }
     }
@@ -1017,62 +1121,96 @@ __global__ void __launch_bounds__(64 * NW)
         V nk[NS][RY][NKA];  // kept values of the queue head / of plane z+1
         V pk[NP > 0 ? NP : 1][RY][NKA];  // kept values of point components
 
-        // Prologue: planes z0-R .. z0+R-1 into q[..][1..2R], plane z0+R into nq.
+        // Prologue: planes z0-R .. z0+R-1 into q[..][1..2R].
 #pragma unroll
         for (int s = 0; s < NS; ++s)
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
 #pragma unroll
                 for (int m = 1; m < Q; ++m) {
+                    MemLoad ml;
                     const unsigned o = qoff[r] + zoff(td.z0 - R + m - 1);
                     if (DELAY && m > R)
                         q[s][r][m] = Pol::template load_keep<CS, T>(
-                            a, org, CS::comp_of_slot(s), o, kq[s][r][DELAY ? m - R : 0]);
+                            ml, a, org, CS::comp_of_slot(s), o, kq[s][r][DELAY ? m - R : 0]);
                     else
                         q[s][r][m] = Pol::template load_st<CS, T>(
-                            a, org, CS::comp_of_slot(s), o);
+                            ml, a, org, CS::comp_of_slot(s), o);
                 }
-                const unsigned o = qoff[r] + zoff(td.z0 + R);
-                if constexpr (DELAY)
-                    nq[s][r] = Pol::template load_keep<CS, T>(
-                        a, org, CS::comp_of_slot(s), o, nk[s][r]);
-                else
-                    nq[s][r] = Pol::template load_st<CS, T>(
-                        a, org, CS::comp_of_slot(s), o);
             }
-        auto load_halo = [&](int z) {
-            const unsigned zo = zoff(z);
+
+        // The next plane's loads (queue head zq, halo ring and point values
+        // of plane zp) in two steps: fetch() issues them and keeps the raw
+        // words, form() makes the stage states, kept values and store-time
+        // inputs from those words.  A plane fetches its successor before its
+        // right-hand sides and forms it after them, so the loads arrive
+        // while the plane is computed; the plane's stores follow form(), so
+        // its wait covers nothing issued after the loads.
+        constexpr int KE = 2 * (NE > 0 ? NE : 1);  // ext words: <= 2 per value
+        RawWords<kRawWords> rq[NS][RY], rh[NS][HPT], rp[NP > 0 ? NP : 1][RY];
+        RawWords<kRawWords> rk[NS][RY];
+        RawWords<KE> re[RY];
+        auto plane_loads = [&](auto tag, int zq, int zp) {
+            constexpr bool FORM = decltype(tag)::value;
+            using MQ = std::conditional_t<FORM, MemReplay<kRawWords>, MemFetch<kRawWords>>;
+            using ME = std::conditional_t<FORM, MemReplay<KE>, MemFetch<KE>>;
+            const unsigned oq = zoff(zq), op = zoff(zp);
+            V dk[NKA];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int r = 0; r < RY; ++r) {
+                    MQ mq(rq[s][r]);
+                    V y;
+                    if constexpr (DELAY)
+                        y = Pol::template load_keep<CS, T>(mq, a, org, CS::comp_of_slot(s),
+                                                           qoff[r] + oq, FORM ? nk[s][r] : dk);
+                    else
+                        y = Pol::template load_st<CS, T>(mq, a, org, CS::comp_of_slot(s),
+                                                         qoff[r] + oq);
+                    if constexpr (FORM) nq[s][r] = y;
+                }
 #pragma unroll
             for (int m = 0; m < HPT; ++m)
 #pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    hv[s][m] = Pol::template load_st<CS, T>(
-                        a, org, CS::comp_of_slot(s), hoff[m] + zo);
-        };
-        auto load_point = [&](int z) {
-            const unsigned zo = zoff(z);
+                for (int s = 0; s < NS; ++s) {
+                    MQ mh(rh[s][m]);
+                    const V y = Pol::template load_st<CS, T>(mh, a, org, CS::comp_of_slot(s),
+                                                             hoff[m] + op);
+                    if constexpr (FORM) hv[s][m] = y;
+                }
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int r = 0; r < RY; ++r)
-                    pv[p][r] = Pol::template load_pt<CS, T>(
-                        a, org, CS::comp_of_point(p), qoff[r] + zo, pk[p][r]);
+                for (int r = 0; r < RY; ++r) {
+                    MQ mp(rp[p][r]);
+                    const V y = Pol::template load_pt<CS, T>(mp, a, org, CS::comp_of_point(p),
+                                                             qoff[r] + op, FORM ? pk[p][r] : dk);
+                    if constexpr (FORM) pv[p][r] = y;
+                }
             if constexpr (NK > 0 && !DELAY) {
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
 #pragma unroll
-                    for (int r = 0; r < RY; ++r)
-                        (void)Pol::template load_keep<CS, T>(
-                            a, org, CS::comp_of_slot(s), qoff[r] + zo, nk[s][r]);
+                    for (int r = 0; r < RY; ++r) {
+                        MQ mk(rk[s][r]);
+                        (void)Pol::template load_keep<CS, T>(mk, a, org, CS::comp_of_slot(s),
+                                                             qoff[r] + op, FORM ? nk[s][r] : dk);
+                    }
             }
             if constexpr (NE > 0) {
+                E de[NE];
 #pragma unroll
-                for (int r = 0; r < RY; ++r)
-                    Pol::template load_ext<CS, T>(a, org, qoff[r] + zo, ev[r]);
+                for (int r = 0; r < RY; ++r) {
+                    ME me(re[r]);
+                    Pol::template load_ext<CS, T>(me, a, org, qoff[r] + op, FORM ? ev[r] : de);
+                }
             }
         };
-        load_halo(td.z0);
-        load_point(td.z0);
+        using Fetch = std::false_type;
+        using Form = std::true_type;
+        plane_loads(Fetch{}, td.z0 + R, td.z0);
+        plane_loads(Form{}, td.z0 + R, td.z0);
 
         for (int z = td.z0; z < td.z1; ++z, ++step) {
             const int buf = step & 1;
@@ -1122,43 +1260,29 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
                 for (int i = 0; i < NE; ++i) ec[r][i] = ev[r][i];
 
-            // Prefetch plane z+1 (queue head z+1+R, halo ring, point values).
-            if (z + 1 < td.z1) {
-                const unsigned zq = zoff(z + 1 + R);
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-#pragma unroll
-                    for (int r = 0; r < RY; ++r) {
-                        if constexpr (DELAY)
-                            nq[s][r] = Pol::template load_keep<CS, T>(
-                                a, org, CS::comp_of_slot(s), qoff[r] + zq, nk[s][r]);
-                        else
-                            nq[s][r] = Pol::template load_st<CS, T>(
-                                a, org, CS::comp_of_slot(s), qoff[r] + zq);
-                    }
-                load_halo(z + 1);
-                load_point(z + 1);
-            }
+            const bool more = z + 1 < td.z1;
+            if (more) plane_loads(Fetch{}, z + 1 + R, z + 1);
             __syncthreads();
 
             const unsigned zc = zoff(z);
+            double rhs[RY][NC];
+            double ctr[RY][NC];
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
                 if (!(cpx && cpy[r])) continue;
                 const int gi = bd.lo[0] + lx, gj = bd.lo[1] + td.y0 + w + r * ROWS,
                           gk = bd.lo[2] + z;
-                double ctr[NC];
                 if constexpr (G >= 0) {
                     // Components of other groups: not loaded, read as NaN.
 #pragma unroll
-                    for (int n = 0; n < NC; ++n) ctr[n] = __builtin_nan("");
+                    for (int n = 0; n < NC; ++n) ctr[r][n] = __builtin_nan("");
                 }
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    ctr[CS::comp_of_slot(s)] = (double)q[s][r][R];
+                    ctr[r][CS::comp_of_slot(s)] = (double)q[s][r][R];
 #pragma unroll
                 for (int p = 0; p < NP; ++p)
-                    ctr[CS::comp_of_point(p)] = (double)pc[p][r];
+                    ctr[r][CS::comp_of_point(p)] = (double)pc[p][r];
 
                 TileCtx tc;
                 tc.ci = gi;
@@ -1171,19 +1295,18 @@ __global__ void __launch_bounds__(64 * NW)
                 tc.zq_cstride = RY * Q;
                 tc.lds = &lds[buf][0][w + r * ROWS + R][tx + R];
                 tc.zq = &q[0][r][0];
-                tc.ctr = ctr;
+                tc.ctr = ctr[r];
                 tc.slot = slot;
                 amrex::Array4<const double> sv;
                 sv.mode = amrex::Array4<const double>::kTile;
                 sv.ext = &tc;
                 sv.ncomp = NC;
 
-                double rhs[NC];
 #pragma unroll
-                for (int n = 0; n < NC; ++n) rhs[n] = 0.;
+                for (int n = 0; n < NC; ++n) rhs[r][n] = 0.;
                 amrex::Array4<double> rv;
                 rv.mode = amrex::Array4<double>::kSink;
-                rv.sink = rhs;
+                rv.sink = rhs[r];
                 rv.ncomp = NC;
 
                 Ops::rhs(rv, sv, gi, gj, gk, a.lev, time, dt, dx, params);
@@ -1197,15 +1320,22 @@ __global__ void __launch_bounds__(64 * NW)
                     if (a.ko_order == 2) {
 #pragma unroll
                         for (int n = 0; n < NC; ++n)
-                            rhs[n] += sledgehamr::kernels::KreissOligerDissipation<2>(
+                            rhs[r][n] += sledgehamr::kernels::KreissOligerDissipation<2>(
                                 sv, gi, gj, gk, n, dx, a.ko_strength[n]);
                     } else if (a.ko_order == 3) {
 #pragma unroll
                         for (int n = 0; n < NC; ++n)
-                            rhs[n] += sledgehamr::kernels::KreissOligerDissipation<3>(
+                            rhs[r][n] += sledgehamr::kernels::KreissOligerDissipation<3>(
                                 sv, gi, gj, gk, n, dx, a.ko_strength[n]);
                     }
                 }
+            }
+
+            if (more) plane_loads(Form{}, z + 1 + R, z + 1);
+
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                if (!(cpx && cpy[r])) continue;
                 V kc[NC][NKA];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
@@ -1217,7 +1347,7 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
                     for (int j = 0; j < NK; ++j)
                         kc[CS::comp_of_point(p)][j] = pkc[p][r][j];
-                Pol::template store<CS, T, NKA>(a, org, coff[r] + zc, rhs, ctr,
+                Pol::template store<CS, T, NKA>(a, org, coff[r] + zc, rhs[r], ctr[r],
                                                 ec[r], kc);
             }
         }
