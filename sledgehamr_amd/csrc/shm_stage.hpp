@@ -382,6 +382,13 @@ __device__ AMREX_FORCE_INLINE double pair_value(double u) {
  *  words: the largest is a final RKN stage of 5 stages in FP64 (phi0, pi0
  *  and four forces, 12 words). */
 constexpr int kRawWords = 12;
+/** An empty instruction that reads and rewrites x in its VGPRs: the value
+ *  is complete at this point of the program, and the compiler neither
+ *  moves its computation later nor merges it with later work. */
+template <class X>
+__device__ AMREX_FORCE_INLINE void pin_reg(X &x) {
+    asm volatile("" : "+v"(x));
+}
 template <int K>
 struct RawWords {
     uint32_t w[K];
@@ -544,6 +551,12 @@ enum SsprkKind : int { kSsStage1 = 1, kSsStage2 = 2, kSsStage3 = 3, kSsFill = 4,
 template <int K>
 struct PolSsprk {
     static constexpr bool composed = (K == kSsStage2 || K == kSsStage3);
+    // The next plane's stage states are formed before this plane's stores,
+    // by every plane (after the last one from dummy loads), so that no wait
+    // for the loads drifts past the stores.  Measured faster for the SSPRK3
+    // stages; the RKN stages (less work per plane) do better when the
+    // formation is left where the compiler places it, after the stores.
+    static constexpr bool kFormEarly = true;
     template <class T>
     using Ext = typename T::S;
     // Kept per component: stage 2: k_in; stage 3: k_in, u0.
@@ -647,8 +660,9 @@ struct PolSsprk {
  *  pi = pi0 + sum_j c[12+j] F_j, with this stage's F (weights c[11], c[16])
  *  taken from registers.  The final stage keeps phi0, pi0 and the F_j that
  *  formed a field's stage state for its own update. */
-template <int NPREV, bool FINAL>
+template <int NPREV, bool FINAL, bool NODE0 = false>
 struct PolRkn {
+    static constexpr bool kFormEarly = false;
     static_assert(NPREV >= 0 && NPREV <= 4, "RKN supports up to 5 stages");
     template <class T>
     using Ext = typename T::S;
@@ -675,9 +689,14 @@ struct PolRkn {
         const V phi0 = ld_tile<T, NT>(m, a.u0, a.u0lo, org + n * a.cstride, off);
         if (p < 0) return phi0;
         // A first stage at node 0 (RKN4, RKN5) has the state phi0: pi0 is
-        // then read at the centre only.
-        if constexpr (!FINAL && NPREV == 0)
-            if (a.c[0] == 0.) return phi0;
+        // then read at the centre only.  NODE0: known when compiled, so the
+        // loads do not depend on a branch.
+        if constexpr (!FINAL && NPREV == 0) {
+            if constexpr (NODE0)
+                return phi0;
+            else if (a.c[0] == 0.)
+                return phi0;
+        }
         const V pi0 = ld_tile<T, NT>(m, a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
         V y = phi0;
         y += (V)a.c[0] * pi0;
@@ -773,7 +792,7 @@ struct PolRkn {
  *  stored phi0, pi0 and F0.  The second stage (PolRkn1Y) then reads y1 at
  *  its stencil points instead of composing it from three arrays there.
  *  Tiles of the storage type only: y1 is stored in that type. */
-struct PolRkn0Y : PolRkn<0, false> {
+struct PolRkn0Y : PolRkn<0, false, true> {
     template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
@@ -895,6 +914,7 @@ struct PolRknF2Y : PolRkn<2, true> {
  *  exactly as the reference's second Rhs sees them): pi1 = vh + c0*a(pi). */
 template <bool KICK2>
 struct PolLeapfrog {
+    static constexpr bool kFormEarly = false;
     template <class T>
     using Ext = double;
     template <class CS, class T>
@@ -1136,6 +1156,13 @@ __global__ void __launch_bounds__(64 * NW)
                     else
                         q[s][r][m] = Pol::template load_st<CS, T>(
                             ml, a, org, CS::comp_of_slot(s), o);
+                    if constexpr (Pol::kFormEarly) {
+                        pin_reg(q[s][r][m]);
+                        if (DELAY && m > R) {
+#pragma unroll
+                            for (int j = 0; j < NK; ++j) pin_reg(kq[s][r][DELAY ? m - R : 0][j]);
+                        }
+                    }
                 }
             }
 
@@ -1150,11 +1177,21 @@ __global__ void __launch_bounds__(64 * NW)
         RawWords<kRawWords> rq[NS][RY], rh[NS][HPT], rp[NP > 0 ? NP : 1][RY];
         RawWords<kRawWords> rk[NS][RY];
         RawWords<KE> re[RY];
-        auto plane_loads = [&](auto tag, int zq, int zp) {
+        // live = false (after the last plane): the same loads, all from the
+        // brick's first cell, and the formed values are not used.  Every
+        // plane then issues and waits for the same loads, so no wait for
+        // them is left to a later point of the program, where it would also
+        // wait for stores issued in between.
+        auto plane_loads = [&](auto tag, int zq, int zp, bool live) {
             constexpr bool FORM = decltype(tag)::value;
             using MQ = std::conditional_t<FORM, MemReplay<kRawWords>, MemFetch<kRawWords>>;
             using ME = std::conditional_t<FORM, MemReplay<KE>, MemFetch<KE>>;
-            const unsigned oq = zoff(zq), op = zoff(zp);
+            const unsigned oq = live ? zoff(zq) : 0u, op = live ? zoff(zp) : 0u;
+            unsigned qo[RY], ho[HPT];
+#pragma unroll
+            for (int r = 0; r < RY; ++r) qo[r] = live ? qoff[r] : 0u;
+#pragma unroll
+            for (int m = 0; m < HPT; ++m) ho[m] = live ? hoff[m] : 0u;
             V dk[NKA];
 #pragma unroll
             for (int s = 0; s < NS; ++s)
@@ -1164,10 +1201,10 @@ __global__ void __launch_bounds__(64 * NW)
                     V y;
                     if constexpr (DELAY)
                         y = Pol::template load_keep<CS, T>(mq, a, org, CS::comp_of_slot(s),
-                                                           qoff[r] + oq, FORM ? nk[s][r] : dk);
+                                                           qo[r] + oq, FORM ? nk[s][r] : dk);
                     else
                         y = Pol::template load_st<CS, T>(mq, a, org, CS::comp_of_slot(s),
-                                                         qoff[r] + oq);
+                                                         qo[r] + oq);
                     if constexpr (FORM) nq[s][r] = y;
                 }
 #pragma unroll
@@ -1176,7 +1213,7 @@ __global__ void __launch_bounds__(64 * NW)
                 for (int s = 0; s < NS; ++s) {
                     MQ mh(rh[s][m]);
                     const V y = Pol::template load_st<CS, T>(mh, a, org, CS::comp_of_slot(s),
-                                                             hoff[m] + op);
+                                                             ho[m] + op);
                     if constexpr (FORM) hv[s][m] = y;
                 }
 #pragma unroll
@@ -1185,7 +1222,7 @@ __global__ void __launch_bounds__(64 * NW)
                 for (int r = 0; r < RY; ++r) {
                     MQ mp(rp[p][r]);
                     const V y = Pol::template load_pt<CS, T>(mp, a, org, CS::comp_of_point(p),
-                                                             qoff[r] + op, FORM ? pk[p][r] : dk);
+                                                             qo[r] + op, FORM ? pk[p][r] : dk);
                     if constexpr (FORM) pv[p][r] = y;
                 }
             if constexpr (NK > 0 && !DELAY) {
@@ -1195,7 +1232,7 @@ __global__ void __launch_bounds__(64 * NW)
                     for (int r = 0; r < RY; ++r) {
                         MQ mk(rk[s][r]);
                         (void)Pol::template load_keep<CS, T>(mk, a, org, CS::comp_of_slot(s),
-                                                             qoff[r] + op, FORM ? nk[s][r] : dk);
+                                                             qo[r] + op, FORM ? nk[s][r] : dk);
                     }
             }
             if constexpr (NE > 0) {
@@ -1203,14 +1240,44 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
                 for (int r = 0; r < RY; ++r) {
                     ME me(re[r]);
-                    Pol::template load_ext<CS, T>(me, a, org, qoff[r] + op, FORM ? ev[r] : de);
+                    Pol::template load_ext<CS, T>(me, a, org, qo[r] + op, FORM ? ev[r] : de);
                 }
+            }
+            if constexpr (FORM && Pol::kFormEarly) {
+                // The formed values are complete here: the compiler would
+                // otherwise sink their arithmetic past the plane's stores
+                // (or into the next plane), and the waits for their loads
+                // would then also wait for those stores.
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int r = 0; r < RY; ++r) {
+                        pin_reg(nq[s][r]);
+#pragma unroll
+                        for (int j = 0; j < NK; ++j) pin_reg(nk[s][r][j]);
+                    }
+#pragma unroll
+                for (int m = 0; m < HPT; ++m)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) pin_reg(hv[s][m]);
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+#pragma unroll
+                    for (int r = 0; r < RY; ++r) {
+                        pin_reg(pv[p][r]);
+#pragma unroll
+                        for (int j = 0; j < NK; ++j) pin_reg(pk[p][r][j]);
+                    }
+#pragma unroll
+                for (int r = 0; r < RY; ++r)
+#pragma unroll
+                    for (int i = 0; i < NE; ++i) pin_reg(ev[r][i]);
             }
         };
         using Fetch = std::false_type;
         using Form = std::true_type;
-        plane_loads(Fetch{}, td.z0 + R, td.z0);
-        plane_loads(Form{}, td.z0 + R, td.z0);
+        plane_loads(Fetch{}, td.z0 + R, td.z0, true);
+        plane_loads(Form{}, td.z0 + R, td.z0, true);
 
         for (int z = td.z0; z < td.z1; ++z, ++step) {
             const int buf = step & 1;
@@ -1261,7 +1328,7 @@ __global__ void __launch_bounds__(64 * NW)
                 for (int i = 0; i < NE; ++i) ec[r][i] = ev[r][i];
 
             const bool more = z + 1 < td.z1;
-            if (more) plane_loads(Fetch{}, z + 1 + R, z + 1);
+            if (Pol::kFormEarly || more) plane_loads(Fetch{}, z + 1 + R, z + 1, more);
             __syncthreads();
 
             const unsigned zc = zoff(z);
@@ -1331,7 +1398,7 @@ __global__ void __launch_bounds__(64 * NW)
                 }
             }
 
-            if (more) plane_loads(Form{}, z + 1 + R, z + 1);
+            if (Pol::kFormEarly || more) plane_loads(Form{}, z + 1 + R, z + 1, more);
 
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
