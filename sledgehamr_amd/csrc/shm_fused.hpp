@@ -7,17 +7,21 @@
 // ring values are what the neighbouring tiles compute for themselves, so
 // the result equals the two-launch path bit for bit).  Per tile and plane:
 //   * u0 of the stencil components over the tile plus 2R cells sits in a
-//     circular LDS window of 2R+1 planes, u0 of the point-wise components of
-//     the current plane over the tile plus R cells in one LDS plane;
+//     circular LDS window of 2R+2 planes; u0 of the point-wise components
+//     stays in the registers of the thread that loaded it;
 //   * stage one at plane zz writes u1 of the stencil components into a
-//     second window (tile plus R) and keeps k1 and the point-wise u1 of the
-//     tile's own cells in a register delay line of R+1 planes;
-//   * stage two at plane zz - R reads the u1 window and the delay line and
-//     writes k2.
-// Planes are loaded one plane ahead; two barriers per plane.  Applies to
-// boxes that wrap periodically along every axis (a single box spanning the
-// domain, or an ensemble of independent boxes), where the extra reach needs
-// no ghost cells.
+//     second window (tile plus R, 2R+2 planes) and keeps k1 and the
+//     point-wise u1 of the tile's own cells in a register delay line of R+2
+//     planes;
+//   * stage two at plane zz - R - 1 reads the u1 window and the delay line
+//     and writes k2.
+// Stage two trails stage one by R+1 planes, so it reads only planes of
+// earlier passes, and each window holds one plane more than a stage reads:
+// one barrier per plane suffices (a wave that runs ahead into the next
+// plane writes slots no slower wave still reads).  Planes are loaded one
+// plane ahead.  Applies to boxes that wrap periodically along every axis (a
+// single box spanning the domain, or an ensemble of independent boxes),
+// where the extra reach needs no ghost cells.
 //
 // With RKN set, the same pass runs the first two stages of a Runge-Kutta-
 // Nystroem step whose first node is 0 (rkn.cpp:52-119; RKN4 and RKN5): the
@@ -56,18 +60,17 @@ struct FusedShape {
     using V = typename PrecTraits<P>::V;  // tile values (u0, u1)
     using CS = CompSet<Ops, GW, KO>;
     static constexpr int NC = CS::NC, NS = CS::NS, NP = CS::NP;
-    static constexpr int TX = 64, TY = NW, NT = 64 * NW, W = 2 * R + 1;
+    static constexpr int TX = 64, TY = NW, NT = 64 * NW, W = 2 * R + 1, WS = W + 1;
     static constexpr int E1X = TX + 2 * R, E1Y = TY + 2 * R, E1 = E1X * E1Y;
     static constexpr int E0X = TX + 4 * R, E0Y = TY + 4 * R, E0 = E0X * E0Y;
     static constexpr int L0 = (E0 + NT - 1) / NT;  // window points per thread
     static constexpr int L1 = (E1 + NT - 1) / NT;  // stage-one points per thread
-    static constexpr long lds_bytes =
-        (long)sizeof(V) * ((long)NS * W * E0 + (long)NS * W * E1 + (long)(NP > 0 ? NP : 1) * E1);
+    static constexpr long lds_bytes = (long)sizeof(V) * ((long)NS * WS * E0 + (long)NS * WS * E1);
     // Instantiated when the LDS fits one workgroup per CU and the delay
     // line stays small.
     static constexpr bool fits =
         !KO && lds_bytes <= 160 * 1024 && L1 <= 3 &&
-        (NC * (int)(sizeof(S) / 4) + NP * (int)(sizeof(V) / 4)) * (R + 1) <= 48;
+        (NC * (int)(sizeof(S) / 4) + NP * (int)(sizeof(V) / 4)) * (R + 2) <= 56;
 };
 
 template <class Ops, Prec P, int R, bool GW, bool KO, int NW, bool RKN>
@@ -78,14 +81,13 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     using V = typename F::V;
     using CS = typename F::CS;
     constexpr int NC = F::NC, NS = F::NS, NP = F::NP, NPA = NP > 0 ? NP : 1;
-    constexpr int W = F::W, NT = F::NT;
+    constexpr int W = F::W, WS = F::WS, NT = F::NT;
     constexpr int E0X = F::E0X, E0 = F::E0, E1X = F::E1X, E1 = F::E1;
     constexpr int L0 = F::L0, L1 = F::L1;
     static_assert(F::fits, "fused tile exceeds LDS or register budget");
 
-    __shared__ V w0[NS][W][E0];   // u0, stencil components, tile + 2R
-    __shared__ V w1[NS][W][E1];   // u1, stencil components, tile + R
-    __shared__ V p0[NPA][E1];     // u0, point components, tile + R
+    __shared__ V w0[NS][WS][E0];  // u0, stencil components, tile + 2R
+    __shared__ V w1[NS][WS][E1];  // u1, stencil components, tile + R
 
     signed char slot[NC];
 #pragma unroll
@@ -169,10 +171,10 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 
         // Plane bookkeeping without divisions in the z loop: wrapped plane
         // indices advance by one per plane, and window slots are those of
-        // consecutive planes modulo W (the lowest loaded plane z0 - 2R has
-        // slot 0).
+        // consecutive planes modulo WS (the lowest loaded plane z0 - 2R has
+        // slot 0; u1 of a plane has the slot of its u0).
         auto next = [](int v, int n) { return v + 1 == n ? 0 : v + 1; };
-        auto cyc = [](int v) { return v >= W ? v - W : v; };  // v < 2W
+        auto cyc = [](int v) { return v >= WS ? v - WS : v; };  // v < 2 WS
         int zwin = wrapi(td.z0 - 2 * R, nz);  // next window plane to load
         int zpt = wrapi(td.z0 - R, nz);       // next point plane to load
         int zst = wrapi(td.z0, nz);           // next plane to store
@@ -203,7 +205,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         };
         auto store_window = [&]() {
             const int sl = swin;
-            swin = next(swin, W);
+            swin = next(swin, WS);
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -221,8 +223,8 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         load_point();
         int slo = 0;  // slot of plane zz - R (lowest plane of stage one's window)
 
-        S dk1[R + 1][NC];    // k1 of the own cell, planes zz-R .. zz
-        V du1[R + 1][NPA];   // u1 of the point components, same planes
+        S dk1[R + 2][NC];    // k1 of the own cell, planes zz-R-1 .. zz
+        V du1[R + 2][NPA];   // u1 of the point components, same planes
 
         // A plane's global stores are issued at the top of the next plane,
         // after its window stores have waited for the prefetched loads and
@@ -251,14 +253,18 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             has_f = false;
         };
 
-        for (int zz = td.z0 - R; zz < td.z1 + R; ++zz) {
-            // Stage one runs on plane zz: window planes zz-R .. zz+R.
-            store_window();
+        for (int zz = td.z0 - R; zz <= td.z1 + R; ++zz) {
+            // Stage one runs on plane zz (window planes zz-R .. zz+R) up to
+            // the last plane stage two needs; stage two on plane zz-R-1.
+            const bool one = zz < td.z1 + R;
+            V vpc[NPA][L1];  // point components of plane zz
+            if (one) {
+                store_window();
 #pragma unroll
-            for (int p = 0; p < NP; ++p)
+                for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int m = 0; m < L1; ++m)
-                    if (e1ok[m]) p0[p][l1[m]] = vp[p][m];
+                    for (int m = 0; m < L1; ++m) vpc[p][m] = vp[p][m];
+            }
             flush();
             if (zz + 1 < td.z1 + R) {
                 load_window();
@@ -266,123 +272,124 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             }
             __syncthreads();
 
-            // ---- stage one at plane zz over the tile + R ----
-            const int sz = cyc(slo + R);
-            int zpl0[W];
 #pragma unroll
-            for (int d = 0; d < W; ++d) zpl0[d] = (cyc(slo + d) - sz) * E0;
-#pragma unroll
-            for (int s = 0; s < R; ++s)
+            for (int s = 0; s <= R; ++s)
 #pragma unroll
                 for (int n = 0; n < NC; ++n) dk1[s][n] = dk1[s + 1][n];
 #pragma unroll
-            for (int s = 0; s < R; ++s)
+            for (int s = 0; s <= R; ++s)
 #pragma unroll
                 for (int p = 0; p < NP; ++p) du1[s][p] = du1[s + 1][p];
+
+            // ---- stage one at plane zz over the tile + R ----
+            const int sz = cyc(slo + R);
+            if (one) {
+                int zpl0[W];
 #pragma unroll
-            for (int m = 0; m < L1; ++m) {
-                if (!e1ok[m]) continue;
-                const int lc = (e1y[m] + 2 * R) * E0X + (e1x[m] + 2 * R);
-                const int gi = bd.lo[0] + td.x0 + e1x[m], gj = bd.lo[1] + td.y0 + e1y[m],
-                          gk = bd.lo[2] + zz;
-                V u0v[NC];
-                double ctr[NC];
+                for (int d = 0; d < W; ++d) zpl0[d] = (cyc(slo + d) - sz) * E0;
 #pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    u0v[CS::comp_of_slot(s)] = w0[s][sz][lc];
-                    ctr[CS::comp_of_slot(s)] = (double)w0[s][sz][lc];
-                }
-#pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    u0v[CS::comp_of_point(p)] = p0[p][l1[m]];
-                    ctr[CS::comp_of_point(p)] = (double)p0[p][l1[m]];
-                }
-                TileCtx tc;
-                tc.ci = gi;
-                tc.cj = gj;
-                tc.ck = gk;
-                tc.R = R;
-                tc.f32 = std::is_same_v<V, float> ? 1 : 0;
-                tc.lds_pitch = E0X;
-                tc.lds_cstride = W * E0;
-                tc.zq_cstride = 0;
-                tc.lds = &w0[0][sz][lc];
-                tc.zq = nullptr;
-                tc.zplane = zpl0;
-                tc.ctr = ctr;
-                tc.slot = slot;
-                amrex::Array4<const double> sv;
-                sv.mode = amrex::Array4<const double>::kTile;
-                sv.ext = &tc;
-                sv.ncomp = NC;
-                double rhs[NC];
-#pragma unroll
-                for (int n = 0; n < NC; ++n) rhs[n] = 0.;
-                amrex::Array4<double> rv;
-                rv.mode = amrex::Array4<double>::kSink;
-                rv.sink = rhs;
-                rv.ncomp = NC;
-                Ops::rhs(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, par1);
-                if constexpr (GW) {
-                    Ops::gw_rhs(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, gpar1);
-                    Ops::gw_backreaction(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, par1, gpar1);
-                }
-                if constexpr (RKN) {
-                    // F0 per pair in the storage type, as the separate first
-                    // stage stores it.
-                    constexpr int NPR = CS::NPAIR > 0 ? CS::NPAIR : 1;
-                    S f0[NPR];
-#pragma unroll
-                    for (int p = 0; p < CS::NPAIR; ++p) f0[p] = (S)rhs[CS::pair_mom(p)];
-                    auto state2 = [&](int n) -> V {
-                        const int p = CS::pair_of_field(n);
-                        V y = u0v[n];
-                        if (p < 0) return y;
-                        y += (V)a.c0 * u0v[CS::pair_mom(p)];
-                        y += (V)a.c1 * (V)f0[p];
-                        return y;
-                    };
-#pragma unroll
-                    for (int s = 0; s < NS; ++s)
-                        w1[s][sz][l1[m]] = state2(CS::comp_of_slot(s));
-                    if (m == 0) {
-#pragma unroll
-                        for (int p = 0; p < NP; ++p) du1[R][p] = state2(CS::comp_of_point(p));
-                        has_f = own && zz >= td.z0 && zz < td.z1;
-                        o_f = cown + (unsigned)(zz + g) * ks;
-#pragma unroll
-                        for (int p = 0; p < CS::NPAIR; ++p) pend_f[p] = f0[p];
-                    }
-                } else {
-                    // k1 in the storage type, as the two-launch path stores
-                    // it; u1 = u0 + dt k1 in the storage type.
-                    S k1[NC];
-#pragma unroll
-                    for (int n = 0; n < NC; ++n) k1[n] = (S)rhs[n];
+                for (int m = 0; m < L1; ++m) {
+                    if (!e1ok[m]) continue;
+                    const int lc = (e1y[m] + 2 * R) * E0X + (e1x[m] + 2 * R);
+                    const int gi = bd.lo[0] + td.x0 + e1x[m], gj = bd.lo[1] + td.y0 + e1y[m],
+                              gk = bd.lo[2] + zz;
+                    V u0v[NC];
+                    double ctr[NC];
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
-                        const int n = CS::comp_of_slot(s);
-                        w1[s][sz][l1[m]] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
+                        u0v[CS::comp_of_slot(s)] = w0[s][sz][lc];
+                        ctr[CS::comp_of_slot(s)] = (double)w0[s][sz][lc];
                     }
-                    if (m == 0) {
 #pragma unroll
-                        for (int n = 0; n < NC; ++n) dk1[R][n] = k1[n];
+                    for (int p = 0; p < NP; ++p) {
+                        u0v[CS::comp_of_point(p)] = vpc[p][m];
+                        ctr[CS::comp_of_point(p)] = (double)vpc[p][m];
+                    }
+                    TileCtx tc;
+                    tc.ci = gi;
+                    tc.cj = gj;
+                    tc.ck = gk;
+                    tc.R = R;
+                    tc.f32 = std::is_same_v<V, float> ? 1 : 0;
+                    tc.lds_pitch = E0X;
+                    tc.lds_cstride = WS * E0;
+                    tc.zq_cstride = 0;
+                    tc.lds = &w0[0][sz][lc];
+                    tc.zq = nullptr;
+                    tc.zplane = zpl0;
+                    tc.ctr = ctr;
+                    tc.slot = slot;
+                    amrex::Array4<const double> sv;
+                    sv.mode = amrex::Array4<const double>::kTile;
+                    sv.ext = &tc;
+                    sv.ncomp = NC;
+                    double rhs[NC];
 #pragma unroll
-                        for (int p = 0; p < NP; ++p) {
-                            const int n = CS::comp_of_point(p);
-                            du1[R][p] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
+                    for (int n = 0; n < NC; ++n) rhs[n] = 0.;
+                    amrex::Array4<double> rv;
+                    rv.mode = amrex::Array4<double>::kSink;
+                    rv.sink = rhs;
+                    rv.ncomp = NC;
+                    Ops::rhs(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, par1);
+                    if constexpr (GW) {
+                        Ops::gw_rhs(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, gpar1);
+                        Ops::gw_backreaction(rv, sv, gi, gj, gk, a.lev, t1, dt, dx, par1, gpar1);
+                    }
+                    if constexpr (RKN) {
+                        // F0 per pair in the storage type, as the separate first
+                        // stage stores it.
+                        S f0[NPR];
+#pragma unroll
+                        for (int p = 0; p < CS::NPAIR; ++p) f0[p] = (S)rhs[CS::pair_mom(p)];
+                        auto state2 = [&](int n) -> V {
+                            const int p = CS::pair_of_field(n);
+                            V y = u0v[n];
+                            if (p < 0) return y;
+                            y += (V)a.c0 * u0v[CS::pair_mom(p)];
+                            y += (V)a.c1 * (V)f0[p];
+                            return y;
+                        };
+#pragma unroll
+                        for (int s = 0; s < NS; ++s)
+                            w1[s][sz][l1[m]] = state2(CS::comp_of_slot(s));
+                        if (m == 0) {
+#pragma unroll
+                            for (int p = 0; p < NP; ++p) du1[R + 1][p] = state2(CS::comp_of_point(p));
+                            has_f = own && zz >= td.z0 && zz < td.z1;
+                            o_f = cown + (unsigned)(zz + g) * ks;
+#pragma unroll
+                            for (int p = 0; p < CS::NPAIR; ++p) pend_f[p] = f0[p];
+                        }
+                    } else {
+                        // k1 in the storage type, as the two-launch path stores
+                        // it; u1 = u0 + dt k1 in the storage type.
+                        S k1[NC];
+#pragma unroll
+                        for (int n = 0; n < NC; ++n) k1[n] = (S)rhs[n];
+#pragma unroll
+                        for (int s = 0; s < NS; ++s) {
+                            const int n = CS::comp_of_slot(s);
+                            w1[s][sz][l1[m]] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
+                        }
+                        if (m == 0) {
+#pragma unroll
+                            for (int n = 0; n < NC; ++n) dk1[R + 1][n] = k1[n];
+#pragma unroll
+                            for (int p = 0; p < NP; ++p) {
+                                const int n = CS::comp_of_point(p);
+                                du1[R + 1][p] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
+                            }
                         }
                     }
                 }
             }
-            __syncthreads();
 
-            // ---- stage two at plane zc = zz - R on the tile ----
-            const int zc = zz - R;
-            const int s2 = cyc(slo + W - R);  // slot of plane zz - 2R
-            slo = next(slo, W);
+            // ---- stage two at plane zc = zz - R - 1 on the tile ----
+            const int zc = zz - R - 1;
+            const int sc = slo == 0 ? WS - 1 : slo - 1;  // slot of plane zc
+            const int s2 = cyc(slo + WS - 1 - R);        // slot of plane zc - R
+            slo = next(slo, WS);
             if (zc < td.z0) continue;
-            const int sc = slo == 0 ? W - 1 : slo - 1;  // slot of plane zc
             int zpl1[W];
 #pragma unroll
             for (int d = 0; d < W; ++d) zpl1[d] = (cyc(s2 + d) - sc) * E1;
@@ -401,7 +408,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             tc.R = R;
             tc.f32 = std::is_same_v<V, float> ? 1 : 0;
             tc.lds_pitch = E1X;
-            tc.lds_cstride = W * E1;
+            tc.lds_cstride = WS * E1;
             tc.zq_cstride = 0;
             tc.lds = &w1[0][sc][lc];
             tc.zq = nullptr;
