@@ -10,14 +10,15 @@
 //     circular LDS window of 2R+2 planes; u0 of the point-wise components
 //     stays in the registers of the thread that loaded it;
 //   * stage one at plane zz writes u1 of the stencil components into a
-//     second window (tile plus R, 2R+2 planes) and keeps k1 and the
-//     point-wise u1 of the tile's own cells in a register delay line of R+2
-//     planes;
-//   * stage two at plane zz - R - 1 reads the u1 window and the delay line
-//     and writes k2.
+//     second window (tile plus R, R+2 planes) for the x and y neighbours,
+//     and keeps u1 of the tile's own cells in a register queue of 2R+2
+//     planes (their z neighbours) and k1 and the point-wise u1 in a delay
+//     line of R+2 planes;
+//   * stage two at plane zz - R - 1 reads plane zz - R - 1 of the u1 window,
+//     the queue and the delay line, and writes k2.
 // Stage two trails stage one by R+1 planes, so it reads only planes of
-// earlier passes, and each window holds one plane more than a stage reads:
-// one barrier per plane suffices (a wave that runs ahead into the next
+// earlier passes, and each window holds one plane more than its readers
+// use: one barrier per plane suffices (a wave that runs ahead into the next
 // plane writes slots no slower wave still reads).  Planes are loaded one
 // plane ahead.  Applies to boxes that wrap periodically along every axis (a
 // single box spanning the domain, or an ensemble of independent boxes),
@@ -65,7 +66,8 @@ struct FusedShape {
     static constexpr int E0X = TX + 4 * R, E0Y = TY + 4 * R, E0 = E0X * E0Y;
     static constexpr int L0 = (E0 + NT - 1) / NT;  // window points per thread
     static constexpr int L1 = (E1 + NT - 1) / NT;  // stage-one points per thread
-    static constexpr long lds_bytes = (long)sizeof(V) * ((long)NS * WS * E0 + (long)NS * WS * E1);
+    static constexpr int WS1 = R + 2;  // u1 window planes
+    static constexpr long lds_bytes = (long)sizeof(V) * ((long)NS * WS * E0 + (long)NS * WS1 * E1);
     // Instantiated when the LDS fits one workgroup per CU and the delay
     // line stays small.
     static constexpr bool fits =
@@ -81,13 +83,13 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     using V = typename F::V;
     using CS = typename F::CS;
     constexpr int NC = F::NC, NS = F::NS, NP = F::NP, NPA = NP > 0 ? NP : 1;
-    constexpr int W = F::W, WS = F::WS, NT = F::NT;
+    constexpr int W = F::W, WS = F::WS, WS1 = F::WS1, NT = F::NT;
     constexpr int E0X = F::E0X, E0 = F::E0, E1X = F::E1X, E1 = F::E1;
     constexpr int L0 = F::L0, L1 = F::L1;
     static_assert(F::fits, "fused tile exceeds LDS or register budget");
 
     __shared__ V w0[NS][WS][E0];  // u0, stencil components, tile + 2R
-    __shared__ V w1[NS][WS][E1];  // u1, stencil components, tile + R
+    __shared__ V w1[NS][WS1][E1];  // u1, stencil components, tile + R
 
     signed char slot[NC];
 #pragma unroll
@@ -222,9 +224,11 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         load_window();
         load_point();
         int slo = 0;  // slot of plane zz - R (lowest plane of stage one's window)
+        int s1w = 0;  // u1 window slot of plane zz
 
         S dk1[R + 2][NC];    // k1 of the own cell, planes zz-R-1 .. zz
         V du1[R + 2][NPA];   // u1 of the point components, same planes
+        V uq[NS][2 * R + 2]; // u1 of the own cell, stencil components, planes zz-2R-1 .. zz
 
         // A plane's global stores are issued at the top of the next plane,
         // after its window stores have waited for the prefetched loads and
@@ -280,6 +284,10 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             for (int s = 0; s <= R; ++s)
 #pragma unroll
                 for (int p = 0; p < NP; ++p) du1[s][p] = du1[s + 1][p];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int j = 0; j < 2 * R + 1; ++j) uq[s][j] = uq[s][j + 1];
 
             // ---- stage one at plane zz over the tile + R ----
             const int sz = cyc(slo + R);
@@ -350,8 +358,11 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                             return y;
                         };
 #pragma unroll
-                        for (int s = 0; s < NS; ++s)
-                            w1[s][sz][l1[m]] = state2(CS::comp_of_slot(s));
+                        for (int s = 0; s < NS; ++s) {
+                            const V y = state2(CS::comp_of_slot(s));
+                            w1[s][s1w][l1[m]] = y;
+                            if (m == 0) uq[s][2 * R + 1] = y;
+                        }
                         if (m == 0) {
 #pragma unroll
                             for (int p = 0; p < NP; ++p) du1[R + 1][p] = state2(CS::comp_of_point(p));
@@ -369,7 +380,9 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 #pragma unroll
                         for (int s = 0; s < NS; ++s) {
                             const int n = CS::comp_of_slot(s);
-                            w1[s][sz][l1[m]] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
+                            const V y = fma_st<V>(cdt, (V)k1[n], u0v[n]);
+                            w1[s][s1w][l1[m]] = y;
+                            if (m == 0) uq[s][2 * R + 1] = y;
                         }
                         if (m == 0) {
 #pragma unroll
@@ -386,19 +399,16 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 
             // ---- stage two at plane zc = zz - R - 1 on the tile ----
             const int zc = zz - R - 1;
-            const int sc = slo == 0 ? WS - 1 : slo - 1;  // slot of plane zc
-            const int s2 = cyc(slo + WS - 1 - R);        // slot of plane zc - R
+            const int sc = next(s1w, WS1);  // u1 window slot of plane zc
+            s1w = sc;
             slo = next(slo, WS);
             if (zc < td.z0) continue;
-            int zpl1[W];
-#pragma unroll
-            for (int d = 0; d < W; ++d) zpl1[d] = (cyc(s2 + d) - sc) * E1;
             const int lc = l1[0];
             const int gi = bd.lo[0] + td.x0 + tx, gj = bd.lo[1] + td.y0 + w,
                       gk = bd.lo[2] + zc;
             double ctr[NC];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) ctr[CS::comp_of_slot(s)] = (double)w1[s][sc][lc];
+            for (int s = 0; s < NS; ++s) ctr[CS::comp_of_slot(s)] = (double)uq[s][R];
 #pragma unroll
             for (int p = 0; p < NP; ++p) ctr[CS::comp_of_point(p)] = (double)du1[0][p];
             TileCtx tc;
@@ -408,11 +418,11 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             tc.R = R;
             tc.f32 = std::is_same_v<V, float> ? 1 : 0;
             tc.lds_pitch = E1X;
-            tc.lds_cstride = WS * E1;
-            tc.zq_cstride = 0;
+            tc.lds_cstride = WS1 * E1;
+            tc.zq_cstride = 2 * R + 2;
             tc.lds = &w1[0][sc][lc];
-            tc.zq = nullptr;
-            tc.zplane = zpl1;
+            tc.zq = &uq[0][0];
+            tc.zplane = nullptr;
             tc.ctr = ctr;
             tc.slot = slot;
             amrex::Array4<const double> sv;
