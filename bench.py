@@ -95,8 +95,8 @@ def rkn_bytes_per_step(key, npair, nstages=3):
 # TCC_EA0_WRREQ_WRITE_DRAM_32B, x 32 B) of the headline kernels at 512^3,
 # FP32 + bf16 remainder, in the launch shapes the profile used.
 TRAFFIC_PROFILE = "profiles/stage_kernel_traffic_complex512.md"
-TRAFFIC_BYTES_512 = {"fused12": (74270667 + 67108864) * 32.0,
-                     "stage3": (174360952 + 100663296) * 32.0}
+TRAFFIC_BYTES_512 = {"fused12": 2.3767e9 + 2.1475e9,
+                     "stage3": 5.6065e9 + 3.2212e9}
 # The launch shape those counters were measured in (every field must match),
 # which is also the headline's default launch shape (HEADLINE_LAUNCH).
 PROFILED_SHAPE = {"variant": 2, "tile_z": 512, "xcd_split": 1, "persistent": 0,
