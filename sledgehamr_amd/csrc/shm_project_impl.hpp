@@ -220,12 +220,10 @@ void fill_group_kinds(KernelTable &kt) {
     fill_group_variants<Ops, kRknF2, P, R>(kt);
     fill_group_variants<Ops, kRknF3, P, R>(kt);
     fill_group_variants<Ops, kRknF4, P, R>(kt);
-    if constexpr (std::is_same_v<typename PrecTraits<P>::V, typename PrecTraits<P>::S>) {
-        fill_group_variants<Ops, kRkn0Y, P, R>(kt);
-        fill_group_variants<Ops, kRkn1Y, P, R>(kt);
-        fill_group_variants<Ops, kRkn1YY, P, R>(kt);
-        fill_group_variants<Ops, kRknF2Y, P, R>(kt);
-    }
+    fill_group_variants<Ops, kRkn0Y, P, R>(kt);
+    fill_group_variants<Ops, kRkn1Y, P, R>(kt);
+    fill_group_variants<Ops, kRkn1YY, P, R>(kt);
+    fill_group_variants<Ops, kRknF2Y, P, R>(kt);
 }
 
 template <class Ops, Prec P, int R, bool GW, bool KO>
@@ -244,14 +242,11 @@ void fill_kinds(KernelTable &kt) {
     fill_variants<Ops, kRknF2, P, R, GW, KO>(kt);
     fill_variants<Ops, kRknF3, P, R, GW, KO>(kt);
     fill_variants<Ops, kRknF4, P, R, GW, KO>(kt);
-    // The second-stage state is stored in the storage type: tiles of the
-    // storage type only.
-    if constexpr (std::is_same_v<typename PrecTraits<P>::V, typename PrecTraits<P>::S>) {
-        fill_variants<Ops, kRkn0Y, P, R, GW, KO>(kt);
-        fill_variants<Ops, kRkn1Y, P, R, GW, KO>(kt);
-        fill_variants<Ops, kRkn1YY, P, R, GW, KO>(kt);
-        fill_variants<Ops, kRknF2Y, P, R, GW, KO>(kt);
-    }
+    // Stored RKN stage states (in the tile type).
+    fill_variants<Ops, kRkn0Y, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn1Y, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRkn1YY, P, R, GW, KO>(kt);
+    fill_variants<Ops, kRknF2Y, P, R, GW, KO>(kt);
     // Leapfrog writes u_new momenta in place while the stage reads them
     // point-wise; with dissipation the momenta are differenced, so that
     // combination is not instantiated.

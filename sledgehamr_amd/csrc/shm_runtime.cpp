@@ -862,10 +862,11 @@ void Runtime::ensure_arrays(LevelState &ls, int integrator) {
     } else if (integrator >= 20) {
         for (size_t i = 0; i + 1 < nodes_.size(); ++i)
             if (!ls.F[i]) ls.F[i].alloc(plane * npair * es);
+        // Stored stage states are in the tile type (FP64 for FP64 tiles).
         if (nodes_.size() >= 3 && nodes_[0] == 0. && !ls.Y2)
-            ls.Y2.alloc(plane * npair * es);
+            ls.Y2.alloc(plane * npair * ysize());
         if (gw_ && nodes_.size() == 3 && nodes_[0] == 0. && !ls.Y3)
-            ls.Y3.alloc(plane * npair * es);
+            ls.Y3.alloc(plane * npair * ysize());
     } else if (integrator == 11) {
         if (!ls.VH) ls.VH.alloc(plane * npair * 8);
     }
@@ -915,10 +916,10 @@ LaunchCfg Runtime::resolved_launch(int lev) const {
 }
 
 bool Runtime::rkn_y(int lev) const {
-    // The second-stage state is stored in the storage type: not with FP64
-    // tiles of stored pairs.
+    // Stored stage states (in the tile type, ysize()); not across
+    // coarse/fine ghost cells, whose fills write the composed parts.
     return rkn_state_ != 0 && integrator_ >= 20 && nodes_.size() >= 3 && nodes_[0] == 0. &&
-           !has_cf(lev) && !tile_lo();
+           !has_cf(lev);
 }
 
 const FusedEntry *Runtime::fused_row() const {
@@ -1650,8 +1651,8 @@ void Runtime::integrate(LevelState &ls, int lev, double dt, double dx,
                     }
                 }
                 halo(ls, ls.F[i].get(), pairs);
-                if (ry && i == 0) halo(ls, ls.Y2.get(), pairs);
-                if (ry2 && i == 1) halo(ls, ls.Y3.get(), pairs);
+                if (ry && i == 0) halo(ls, ls.Y2.get(), pairs, false, (int)ysize());
+                if (ry2 && i == 1) halo(ls, ls.Y3.get(), pairs, false, (int)ysize());
             } else {
                 a.c[6] = dt;
                 for (int j = 0; j < i; ++j) {

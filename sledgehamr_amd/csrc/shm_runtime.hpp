@@ -343,6 +343,8 @@ class Runtime {
     /** Stencils read hi + lo (FP64 tiles of stored pairs): the remainders
      *  of the states a step reads must be valid at ghost cells too. */
     bool tile_lo() const { return prec_ == Prec::F32CS; }
+    // Element size of the stored RKN stage states: the tile type.
+    size_t ysize() const { return prec_ == Prec::F64 || tile_lo() ? 8 : 4; }
     /** Remainder array of state slot `which` when stencils read it. */
     void *tile_lo_of(LevelState &ls, int which) const {
         return tile_lo() ? ls.Ulo[which].get() : nullptr;

@@ -788,10 +788,11 @@ struct PolRkn {
 
 /** First RKN stage at node 0 (its state is u0) that also writes, per pair,
  *  the second stage's field state y1 = phi0 + c[18] pi0 + c[19] F0 into a.y,
- *  formed in the storage type exactly as PolRkn<1, false> forms it from the
+ *  formed in the tile type exactly as PolRkn<1, false> forms it from the
  *  stored phi0, pi0 and F0.  The second stage (PolRkn1Y) then reads y1 at
  *  its stencil points instead of composing it from three arrays there.
- *  Tiles of the storage type only: y1 is stored in that type. */
+ *  y1 is stored in the tile type (FP64 for the FP64 tiles of a stored
+ *  pair), so the stored value is the composed one. */
 struct PolRkn0Y : PolRkn<0, false, true> {
     template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
@@ -801,15 +802,16 @@ struct PolRkn0Y : PolRkn<0, false, true> {
                                                     const typename T::S *e,
                                                     const typename T::V (*kc)[NK]) {
         using S = typename T::S;
+        using Y = typename T::V;
         PolRkn<0, false>::template store<CS, T, NK>(a, org, off, rhs, ctr, e, kc);
 #pragma unroll
         for (int o = 0; o < CS::NOP; ++o) {
             const int p = CS::out_pair(o);
             const S f0 = (S)rhs[CS::pair_mom(p)];
-            S y = (S)ctr[CS::pair_field(p)];
-            y += (S)a.c[18] * (S)ctr[CS::pair_mom(p)];
-            y += (S)a.c[19] * f0;
-            st<S>(a.y, org + p * a.cstride, off, y);
+            Y y = (Y)ctr[CS::pair_field(p)];
+            y += (Y)a.c[18] * (Y)ctr[CS::pair_mom(p)];
+            y += (Y)a.c[19] * (Y)f0;
+            st<Y>(a.y, org + p * a.cstride, off, y);
         }
     }
 };
@@ -821,11 +823,10 @@ struct PolRkn1Y : PolRkn<1, false> {
     __device__ AMREX_FORCE_INLINE static typename T::V load_st(M &m, const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        using S = typename T::S;
         using V = typename T::V;
         const int p = CS::pair_of_field(n);
         if (p < 0) return ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
-        return (V)m.template ld<S>(a.y, org + p * a.cstride, off);
+        return m.template ld<V>(a.y, org + p * a.cstride, off);
     }
     template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
@@ -841,11 +842,11 @@ struct PolRkn1Y : PolRkn<1, false> {
 
 /** Second stage of a three-stage RKN step (RKN4) that reads the field
  *  states PolRkn0Y wrote and also writes, per pair, the final stage's field
- *  state y2 = phi0 + c[12] pi0 + c[13] F0 + c[14] F1, formed in the storage
+ *  state y2 = phi0 + c[12] pi0 + c[13] F0 + c[14] F1, formed in the tile
  *  type exactly as PolRkn<2, true> forms it from the stored phi0, pi0, F0
  *  and the F1 written here (same terms, order and rounding).  The final
  *  stage (PolRknF2Y) then reads y2 at its stencil points instead of
- *  composing it from four arrays there.  Tiles of the storage type only. */
+ *  composing it from four arrays there.  y2 is stored in the tile type. */
 struct PolRkn1YY : PolRkn1Y {
     template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
@@ -855,18 +856,19 @@ struct PolRkn1YY : PolRkn1Y {
                                                     const typename T::S *e,
                                                     const typename T::V (*kc)[NK]) {
         using S = typename T::S;
+        using Y = typename T::V;
         PolRkn<1, false>::template store<CS, T, NK>(a, org, off, rhs, ctr, e, kc);
 #pragma unroll
         for (int o = 0; o < CS::NOP; ++o) {
             const int p = CS::out_pair(o);
             const S f1 = (S)rhs[CS::pair_mom(p)];
-            const S phi0 = ld<S>(a.u0, org + CS::pair_field(p) * a.cstride, off);
+            const Y phi0 = ld_tile<T>(a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
             const S f0 = ld<S>(a.f[0], org + p * a.cstride, off);
-            S y = phi0;
-            y += (S)a.c[12] * (S)ctr[CS::pair_mom(p)];
-            y += (S)a.c[13] * f0;
-            y += (S)a.c[14] * f1;
-            st<S>(a.y2, org + p * a.cstride, off, y);
+            Y y = phi0;
+            y += (Y)a.c[12] * (Y)ctr[CS::pair_mom(p)];
+            y += (Y)a.c[13] * (Y)f0;
+            y += (Y)a.c[14] * (Y)f1;
+            st<Y>(a.y2, org + p * a.cstride, off, y);
         }
     }
 };
@@ -879,11 +881,10 @@ struct PolRknF2Y : PolRkn<2, true> {
     __device__ AMREX_FORCE_INLINE static typename T::V load_st(M &m, const StageArgs &a,
                                                                long long org, int n,
                                                                unsigned off) {
-        using S = typename T::S;
         using V = typename T::V;
         const int p = CS::pair_of_field(n);
         if (p < 0) return ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
-        return (V)m.template ld<S>(a.y2, org + p * a.cstride, off);
+        return m.template ld<V>(a.y2, org + p * a.cstride, off);
     }
     template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_keep(
@@ -894,12 +895,12 @@ struct PolRknF2Y : PolRkn<2, true> {
         if (p < 0) return ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
         // Fields a component group stages without updating (the scalars in
         // the GW tensor groups) need no centre values.
-        if (!CS::out(n)) return (V)m.template ld<S>(a.y2, org + p * a.cstride, off);
+        if (!CS::out(n)) return m.template ld<V>(a.y2, org + p * a.cstride, off);
         keep[0] = ld_tile<T>(m, a.u0, a.u0lo, org + n * a.cstride, off);
         keep[1] = ld_tile<T>(m, a.u0, a.u0lo, org + CS::pair_mom(p) * a.cstride, off);
         keep[2] = (V)m.template ld<S>(a.f[0], org + p * a.cstride, off);
         keep[3] = (V)m.template ld<S>(a.f[1], org + p * a.cstride, off);
-        return (V)m.template ld<S>(a.y2, org + p * a.cstride, off);
+        return m.template ld<V>(a.y2, org + p * a.cstride, off);
     }
     template <class CS, class T, class M>
     __device__ AMREX_FORCE_INLINE static typename T::V load_pt(

@@ -42,7 +42,7 @@ def test_groups_equal_single_pass(prec, integrator):
         assert np.array_equal(a[c], b[c]), c
 
 
-@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16, shm.PREC_FP32CS])
 @pytest.mark.parametrize("split", [0, 1])
 def test_rkn_stored_stage_states_equal_composed(prec, split):
     """RKN4 with the stage states stored by the stage before (the second
