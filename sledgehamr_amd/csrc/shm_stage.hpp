@@ -848,31 +848,55 @@ struct PolRkn1Y : PolRkn<1, false> {
  *  stage (PolRknF2Y) then reads y2 at its stencil points instead of
  *  composing it from four arrays there.  y2 is stored in the tile type. */
 struct PolRkn1YY : PolRkn1Y {
+    // phi0 and F0 of each output pair at the centre are store-time inputs,
+    // prefetched with the plane (loaded inside store() they would be
+    // waited for there), in the tile type.
+    template <class T>
+    using Ext = typename T::V;
+    template <class CS, class T>
+    static constexpr int next() {
+        return CS::NOP * 2;
+    }
+    template <class CS, class T, class M>
+    __device__ AMREX_FORCE_INLINE static void load_ext(M &m, const StageArgs &a,
+                                                       long long org, unsigned off,
+                                                       typename T::V *e) {
+        using S = typename T::S;
+        using V = typename T::V;
+        int i = 0;
+#pragma unroll
+        for (int o = 0; o < CS::NOP; ++o) {
+            const int p = CS::out_pair(o);
+            e[i++] = ld_tile<T>(m, a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
+            e[i++] = (V)m.template ld<S>(a.f[0], org + p * a.cstride, off);
+        }
+    }
     template <class CS, class T, int NK>
     __device__ AMREX_FORCE_INLINE static void store(const StageArgs &a,
                                                     long long org, unsigned off,
                                                     const double *rhs,
                                                     const double *ctr,
-                                                    const typename T::S *e,
+                                                    const typename T::V *e,
                                                     const typename T::V (*kc)[NK]) {
         using S = typename T::S;
         using Y = typename T::V;
-        PolRkn<1, false>::template store<CS, T, NK>(a, org, off, rhs, ctr, e, kc);
+        PolRkn<1, false>::template store<CS, T, NK>(a, org, off, rhs, ctr,
+                                                    (const S *)nullptr, kc);
+        int i = 0;
 #pragma unroll
         for (int o = 0; o < CS::NOP; ++o) {
             const int p = CS::out_pair(o);
             const S f1 = (S)rhs[CS::pair_mom(p)];
-            const Y phi0 = ld_tile<T>(a.u0, a.u0lo, org + CS::pair_field(p) * a.cstride, off);
-            const S f0 = ld<S>(a.f[0], org + p * a.cstride, off);
+            const Y phi0 = e[i++];
+            const Y f0 = e[i++];
             Y y = phi0;
             y += (Y)a.c[12] * (Y)ctr[CS::pair_mom(p)];
-            y += (Y)a.c[13] * (Y)f0;
+            y += (Y)a.c[13] * f0;
             y += (Y)a.c[14] * (Y)f1;
             st<Y>(a.y2, org + p * a.cstride, off, y);
         }
     }
 };
-
 /** Final stage of a three-stage RKN step reading the field states
  *  PolRkn1YY wrote at its stencil points; phi0, pi0, F0 and F1 are read at
  *  the centre for the update, as PolRkn<2, true> keeps them. */
