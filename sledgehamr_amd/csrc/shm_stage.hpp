@@ -1080,6 +1080,11 @@ __global__ void __launch_bounds__(64 * NW)
     const ParamRegs<GW> pr(a.prm);
     const double *params = pr.user;
     const double *gw_params = pr.gw_ptr(a.prm);
+    double ko_s[KO ? NC : 1];  // dissipation strengths, read once as well
+    if constexpr (KO) {
+#pragma unroll
+        for (int n = 0; n < NC; ++n) ko_s[n] = a.ko_strength[n];
+    }
 
     // Persistent walk over the tile table.  With xcd_split the tiles are cut
     // into 8 contiguous ranges, one per XCD (workgroups are dealt to XCDs
@@ -1413,12 +1418,12 @@ __global__ void __launch_bounds__(64 * NW)
 #pragma unroll
                         for (int n = 0; n < NC; ++n)
                             rhs[r][n] += sledgehamr::kernels::KreissOligerDissipation<2>(
-                                sv, gi, gj, gk, n, dx, a.ko_strength[n]);
+                                sv, gi, gj, gk, n, dx, ko_s[n]);
                     } else if (a.ko_order == 3) {
 #pragma unroll
                         for (int n = 0; n < NC; ++n)
                             rhs[r][n] += sledgehamr::kernels::KreissOligerDissipation<3>(
-                                sv, gi, gj, gk, n, dx, a.ko_strength[n]);
+                                sv, gi, gj, gk, n, dx, ko_s[n]);
                     }
                 }
             }
