@@ -408,16 +408,24 @@ struct MemFetch {
     RawWords<K> &r;
     int i = 0;
     __device__ AMREX_FORCE_INLINE explicit MemFetch(RawWords<K> &raw) : r(raw) {}
+    // Words beyond K are dropped (and replayed as 0): a hook that read
+    // more than K words would give wrong values, never an access outside
+    // the buffer.  The indices are constants once the hooks are inlined,
+    // so the checks fold away.
+    __device__ AMREX_FORCE_INLINE void word(uint32_t x) {
+        if (i < K) r.w[i] = x;
+        ++i;
+    }
     template <class S>
     __device__ AMREX_FORCE_INLINE void put(S v) {
         if constexpr (sizeof(S) == 8) {
             const unsigned long long u = (unsigned long long)__double_as_longlong((double)v);
-            r.w[i++] = (uint32_t)u;
-            r.w[i++] = (uint32_t)(u >> 32);
+            word((uint32_t)u);
+            word((uint32_t)(u >> 32));
         } else if constexpr (std::is_same_v<S, float>) {
-            r.w[i++] = __float_as_uint(v);
+            word(__float_as_uint(v));
         } else {
-            r.w[i++] = (uint32_t)v;
+            word((uint32_t)v);
         }
     }
     template <class S>
@@ -436,17 +444,21 @@ struct MemReplay {
     const RawWords<K> &r;
     int i = 0;
     __device__ AMREX_FORCE_INLINE explicit MemReplay(const RawWords<K> &raw) : r(raw) {}
+    __device__ AMREX_FORCE_INLINE uint32_t word() {
+        const uint32_t x = i < K ? r.w[i] : 0u;
+        ++i;
+        return x;
+    }
     template <class S>
     __device__ AMREX_FORCE_INLINE S get() {
         if constexpr (sizeof(S) == 8) {
-            const unsigned long long u =
-                (unsigned long long)r.w[i] | ((unsigned long long)r.w[i + 1] << 32);
-            i += 2;
-            return (S)__longlong_as_double((long long)u);
+            const unsigned long long lo = word();
+            const unsigned long long hi = word();
+            return (S)__longlong_as_double((long long)(lo | (hi << 32)));
         } else if constexpr (std::is_same_v<S, float>) {
-            return __uint_as_float(r.w[i++]);
+            return __uint_as_float(word());
         } else {
-            return (S)r.w[i++];
+            return (S)word();
         }
     }
     template <class S>
