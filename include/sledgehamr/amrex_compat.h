@@ -263,9 +263,11 @@ namespace shm {
 /** Tile view context: the fused stage kernel fills one per cell it updates.
  *  `lds` points at the cell's own position (stencil slot 0) in the staged
  *  centre plane, `zq` at the cell's register queue of z-neighbours
- *  (2R+1 entries per stencil slot, centre at index R), `ctr` at the widened
- *  centre values of every component.  Components whose slot is negative are
- *  point-wise: only the centre may be read. */
+ *  (zq_cstride entries per stencil slot; plane ck + dk at entry
+ *  dk + R + zq_phase, taken modulo zq_len, so a queue that rotates instead
+ *  of shifting passes its phase), `ctr` at the widened centre values of
+ *  every component.  Components whose slot is negative are point-wise: only
+ *  the centre may be read. */
 struct TileCtx {
     int ci, cj, ck;
     int R;
@@ -273,6 +275,8 @@ struct TileCtx {
     int lds_pitch;
     int lds_cstride;
     int zq_cstride;
+    int zq_phase = 0;
+    int zq_len = 0x7fffffff;
     const void *lds;
     const void *zq;
     const double *ctr;
@@ -280,6 +284,13 @@ struct TileCtx {
     // LDS-window mode (zq unused): element offset, relative to the centre
     // plane, of plane ck + dk for dk = -R..R.
     const int *zplane = nullptr;
+
+    /** Queue entry of plane ck + dk, |dk| <= R, in stencil slot s. */
+    AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE int zq_at(int s, int dk) const {
+        int q = dk + R + zq_phase;
+        if (q >= zq_len) q -= zq_len;
+        return s * zq_cstride + q;
+    }
 
     AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE double get(int i, int j, int k,
                                                         int n) const {
@@ -294,7 +305,7 @@ struct TileCtx {
         }
         if (dk != 0) {
             if (di != 0 || dj != 0) return __builtin_nan("");
-            const int q = s * zq_cstride + dk + R;
+            const int q = zq_at(s, dk);
             return f32 ? (double)static_cast<const float *>(zq)[q]
                        : static_cast<const double *>(zq)[q];
         }

@@ -68,10 +68,9 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE bool tile_lap_f32(
     constexpr detail::LapWeights W = detail::lap_weights(P);
     const float *L = static_cast<const float *>(tc.lds) + s * tc.lds_cstride;
     // z neighbours: the register queue, or the LDS window's other planes.
-    const float *Z = tc.zplane ? nullptr
-                               : static_cast<const float *>(tc.zq) + s * tc.zq_cstride + tc.R;
+    const float *Z = static_cast<const float *>(tc.zq);
     auto zv = [&](int m) -> double {
-        return tc.zplane ? (double)L[tc.zplane[tc.R + m]] : (double)Z[m];
+        return tc.zplane ? (double)L[tc.zplane[tc.R + m]] : (double)Z[tc.zq_at(s, m)];
     };
     double acc = W.w[0] * zv(0);
     constexpr_for<1, P + 1, 1>([&](auto m) {
@@ -137,10 +136,10 @@ Gradient(amrex::Array4<amrex::Real const> const &state, const int i,
                                          (double)L[tc.zplane[tc.R - m]]);
                     });
                 } else if (az) {
-                    const float *Z = static_cast<const float *>(tc.zq) +
-                                     s * tc.zq_cstride + tc.R;
+                    const float *Z = static_cast<const float *>(tc.zq);
                     constexpr_for<1, P + 1, 1>([&](auto m) {
-                        acc += W.g[m] * ((double)Z[m] - (double)Z[-m]);
+                        acc += W.g[m] * ((double)Z[tc.zq_at(s, m)] -
+                                         (double)Z[tc.zq_at(s, -m)]);
                     });
                 } else {
                     const float *L =

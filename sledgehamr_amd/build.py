@@ -47,10 +47,15 @@ PROJECTS = [
             "PhaseTransitionGW::PhaseTransitionGW", 0b1000, gw_modes=2),
 ]
 
+# Multiply-adds are fused per source expression (-ffp-contract=on), not
+# wherever the optimised code happens to place a multiply next to an add:
+# kernels that evaluate the same right-hand side in different code (the
+# fused stages 1 + 2 and the separate stage launches, the cells of a tile
+# and the ring cells of its neighbour) then round identically.
 COMMON_FLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast", "-munsafe-fp-atomics",
+    "-ffp-contract=on", "-munsafe-fp-atomics",
     "-Wno-unused-result", "-Wno-unknown-pragmas",
     "-I", INCLUDE, "-I", os.path.join(INCLUDE, "sledgehamr"), "-I", CSRC,
 ]

@@ -75,6 +75,13 @@ struct FusedShape {
         (NC * (int)(sizeof(S) / 4) + NP * (int)(sizeof(V) / 4)) * (R + 2) <= 56;
 };
 
+/** f(integral_constant<int, I>) for each I of the sequence, in order. */
+template <int... I, class Fn>
+__device__ __attribute__((always_inline)) inline void unroll_phases(
+    std::integer_sequence<int, I...>, Fn &&f) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
 template <class Ops, Prec P, int R, bool GW, bool KO, int NW, bool RKN>
 __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
     using F = FusedShape<Ops, P, R, GW, KO, NW>;
@@ -182,9 +189,10 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         int zst = wrapi(td.z0, nz);           // next plane to store
         int swin = 0;                         // slot of the next window plane
 
-        // Next plane's loads (registers).
+        // Next plane's loads (registers).  The point components alternate
+        // between two buffers: plane zz's are read while plane zz+1's load.
         V v0[NS][L0];
-        V vp[NPA][L1];
+        V vp[2][NPA][L1];
         auto load_window = [&]() {
             const unsigned zo = (unsigned)(zwin + g) * ks;
             zwin = next(zwin, nz);
@@ -195,15 +203,15 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                     v0[s][m] = ld_tile<T>(a.u0, a.u0lo, org + CS::comp_of_slot(s) * a.cstride,
                                           c0[m] + zo);
         };
-        auto load_point = [&]() {
+        auto load_point = [&](V (&dst)[NPA][L1]) {
             const unsigned zo = (unsigned)(zpt + g) * ks;
             zpt = next(zpt, nz);
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int m = 0; m < L1; ++m)
-                    vp[p][m] = ld_tile<T>(a.u0, a.u0lo, org + CS::comp_of_point(p) * a.cstride,
-                                          c1[m] + zo);
+                    dst[p][m] = ld_tile<T>(a.u0, a.u0lo, org + CS::comp_of_point(p) * a.cstride,
+                                           c1[m] + zo);
         };
         auto store_window = [&]() {
             const int sl = swin;
@@ -222,13 +230,19 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             store_window();
         }
         load_window();
-        load_point();
+        load_point(vp[0]);
         int slo = 0;  // slot of plane zz - R (lowest plane of stage one's window)
         int s1w = 0;  // u1 window slot of plane zz
 
-        S dk1[R + 2][NC];    // k1 of the own cell, planes zz-R-1 .. zz
-        V du1[R + 2][NPA];   // u1 of the point components, same planes
-        V uq[NS][2 * R + 2]; // u1 of the own cell, stencil components, planes zz-2R-1 .. zz
+        // Register queues of QL = 2R+2 planes.  They rotate instead of
+        // shifting: the plane loop is unrolled by QL, and the plane of phase
+        // P (its distance from plane z0 - R, modulo QL) keeps logical entry j
+        // (j = QL-1 the newest) in physical entry (j + P) mod QL, so a plane
+        // moves no register.
+        constexpr int QL = 2 * R + 2;
+        S dk1[QL][NC];   // k1 of the own cell, planes zz-2R-1 .. zz (stage two reads zz-R-1)
+        V du1[QL][NPA];  // u1 of the point components, same planes
+        V uq[NS][QL];    // u1 of the own cell, stencil components, same planes
 
         // A plane's global stores are issued at the top of the next plane,
         // after its window stores have waited for the prefetched loads and
@@ -257,37 +271,28 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             has_f = false;
         };
 
-        for (int zz = td.z0 - R; zz <= td.z1 + R; ++zz) {
+        auto plane = [&](auto phase, const int zz) __attribute__((always_inline)) {
+            constexpr int PH = decltype(phase)::value;
+            auto ix = [](int j) { return (j + PH) % QL; };  // logical -> physical
             // Stage one runs on plane zz (window planes zz-R .. zz+R) up to
             // the last plane stage two needs; stage two on plane zz-R-1.
             const bool one = zz < td.z1 + R;
-            V vpc[NPA][L1];  // point components of plane zz
-            if (one) {
-                store_window();
+            const V (&vpc)[NPA][L1] = vp[PH & 1];  // point components of plane zz
+            if (one) store_window();
+            // Wait for this plane's point components here, before the next
+            // plane's loads: a wait issued after those would wait for them
+            // too (the counter drains in issue order).  On the last plane
+            // the buffer holds an older plane, long arrived.
 #pragma unroll
-                for (int p = 0; p < NP; ++p)
+            for (int p = 0; p < NP; ++p)
 #pragma unroll
-                    for (int m = 0; m < L1; ++m) vpc[p][m] = vp[p][m];
-            }
+                for (int m = 0; m < L1; ++m) pin_reg(vp[PH & 1][p][m]);
             flush();
             if (zz + 1 < td.z1 + R) {
                 load_window();
-                load_point();
+                load_point(vp[(PH + 1) & 1]);
             }
             __syncthreads();
-
-#pragma unroll
-            for (int s = 0; s <= R; ++s)
-#pragma unroll
-                for (int n = 0; n < NC; ++n) dk1[s][n] = dk1[s + 1][n];
-#pragma unroll
-            for (int s = 0; s <= R; ++s)
-#pragma unroll
-                for (int p = 0; p < NP; ++p) du1[s][p] = du1[s + 1][p];
-#pragma unroll
-            for (int s = 0; s < NS; ++s)
-#pragma unroll
-                for (int j = 0; j < 2 * R + 1; ++j) uq[s][j] = uq[s][j + 1];
 
             // ---- stage one at plane zz over the tile + R ----
             const int sz = cyc(slo + R);
@@ -361,11 +366,11 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                         for (int s = 0; s < NS; ++s) {
                             const V y = state2(CS::comp_of_slot(s));
                             w1[s][s1w][l1[m]] = y;
-                            if (m == 0) uq[s][2 * R + 1] = y;
+                            if (m == 0) uq[s][ix(QL - 1)] = y;
                         }
                         if (m == 0) {
 #pragma unroll
-                            for (int p = 0; p < NP; ++p) du1[R + 1][p] = state2(CS::comp_of_point(p));
+                            for (int p = 0; p < NP; ++p) du1[ix(QL - 1)][p] = state2(CS::comp_of_point(p));
                             has_f = own && zz >= td.z0 && zz < td.z1;
                             o_f = cown + (unsigned)(zz + g) * ks;
 #pragma unroll
@@ -382,15 +387,15 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                             const int n = CS::comp_of_slot(s);
                             const V y = fma_st<V>(cdt, (V)k1[n], u0v[n]);
                             w1[s][s1w][l1[m]] = y;
-                            if (m == 0) uq[s][2 * R + 1] = y;
+                            if (m == 0) uq[s][ix(QL - 1)] = y;
                         }
                         if (m == 0) {
 #pragma unroll
-                            for (int n = 0; n < NC; ++n) dk1[R + 1][n] = k1[n];
+                            for (int n = 0; n < NC; ++n) dk1[ix(QL - 1)][n] = k1[n];
 #pragma unroll
                             for (int p = 0; p < NP; ++p) {
                                 const int n = CS::comp_of_point(p);
-                                du1[R + 1][p] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
+                                du1[ix(QL - 1)][p] = fma_st<V>(cdt, (V)k1[n], u0v[n]);
                             }
                         }
                     }
@@ -402,15 +407,15 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             const int sc = next(s1w, WS1);  // u1 window slot of plane zc
             s1w = sc;
             slo = next(slo, WS);
-            if (zc < td.z0) continue;
+            if (zc < td.z0) return;
             const int lc = l1[0];
             const int gi = bd.lo[0] + td.x0 + tx, gj = bd.lo[1] + td.y0 + w,
                       gk = bd.lo[2] + zc;
             double ctr[NC];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) ctr[CS::comp_of_slot(s)] = (double)uq[s][R];
+            for (int s = 0; s < NS; ++s) ctr[CS::comp_of_slot(s)] = (double)uq[s][ix(R)];
 #pragma unroll
-            for (int p = 0; p < NP; ++p) ctr[CS::comp_of_point(p)] = (double)du1[0][p];
+            for (int p = 0; p < NP; ++p) ctr[CS::comp_of_point(p)] = (double)du1[ix(R)][p];
             TileCtx tc;
             tc.ci = gi;
             tc.cj = gj;
@@ -419,7 +424,9 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             tc.f32 = std::is_same_v<V, float> ? 1 : 0;
             tc.lds_pitch = E1X;
             tc.lds_cstride = WS1 * E1;
-            tc.zq_cstride = 2 * R + 2;
+            tc.zq_cstride = QL;
+            tc.zq_phase = PH;
+            tc.zq_len = QL;
             tc.lds = &w1[0][sc][lc];
             tc.zq = &uq[0][0];
             tc.zplane = nullptr;
@@ -450,9 +457,13 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                 for (int p = 0; p < CS::NPAIR; ++p) pend_k[p] = (S)rhs[CS::pair_mom(p)];
             } else {
 #pragma unroll
-                for (int n = 0; n < NC; ++n) pend_k[n] = (S)(rhs[n] + (double)dk1[0][n]);
+                for (int n = 0; n < NC; ++n) pend_k[n] = (S)(rhs[n] + (double)dk1[ix(R)][n]);
             }
-        }
+        };
+        for (int zz = td.z0 - R; zz <= td.z1 + R; zz += QL)
+            unroll_phases(std::make_integer_sequence<int, QL>{}, [&](auto phase) {
+                if (zz + phase <= td.z1 + R) plane(phase, zz + phase);
+            });
         flush();
         __syncthreads();  // the next tile's prologue rewrites the windows
     }

@@ -37,7 +37,7 @@ pytestmark = pytest.mark.skipif(
 
 FLAGS = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-ffp-contract=fast", "-munsafe-fp-atomics", "-Wno-unused-result",
+         "-ffp-contract=on", "-munsafe-fp-atomics", "-Wno-unused-result",
          "-Wno-unknown-pragmas",
          "-I", os.path.join(ROOT, "include"),
          "-I", os.path.join(ROOT, "include", "sledgehamr"),
