@@ -12,8 +12,9 @@
 //   * stage one at plane zz writes u1 of the stencil components into a
 //     second window (tile plus R, R+2 planes) for the x and y neighbours,
 //     and keeps u1 of the tile's own cells in a register queue of 2R+2
-//     planes (their z neighbours) and k1 and the point-wise u1 in a delay
-//     line of R+2 planes;
+//     planes (their z neighbours) and k1 and the point-wise u1 in delay
+//     lines of as many planes (the queues rotate: the plane loop is
+//     unrolled by their length);
 //   * stage two at plane zz - R - 1 reads plane zz - R - 1 of the u1 window,
 //     the queue and the delay line, and writes k2.
 // Stage two trails stage one by R+1 planes, so it reads only planes of
@@ -179,15 +180,14 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         const unsigned cown = (unsigned)(td.x0 + tx + g) + (unsigned)(td.y0 + w + g) * js;
 
         // Plane bookkeeping without divisions in the z loop: wrapped plane
-        // indices advance by one per plane, and window slots are those of
+        // indices advance by one per plane.  Window slots are those of
         // consecutive planes modulo WS (the lowest loaded plane z0 - 2R has
-        // slot 0; u1 of a plane has the slot of its u0).
+        // slot 0), so with WS equal to the unroll length they are constants
+        // of the plane's phase; the u1 window (WS1 slots) keeps a counter.
         auto next = [](int v, int n) { return v + 1 == n ? 0 : v + 1; };
-        auto cyc = [](int v) { return v >= WS ? v - WS : v; };  // v < 2 WS
         int zwin = wrapi(td.z0 - 2 * R, nz);  // next window plane to load
         int zpt = wrapi(td.z0 - R, nz);       // next point plane to load
         int zst = wrapi(td.z0, nz);           // next plane to store
-        int swin = 0;                         // slot of the next window plane
 
         // Next plane's loads (registers).  The point components alternate
         // between two buffers: plane zz's are read while plane zz+1's load.
@@ -213,9 +213,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
                     dst[p][m] = ld_tile<T>(a.u0, a.u0lo, org + CS::comp_of_point(p) * a.cstride,
                                            c1[m] + zo);
         };
-        auto store_window = [&]() {
-            const int sl = swin;
-            swin = next(swin, WS);
+        auto store_window = [&](const int sl) {
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -225,14 +223,14 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
 
         // Prologue: planes z0-2R .. z0-1 of the window; prefetch plane z0
         // of the window and plane z0-R of the point components.
-        for (int z = td.z0 - 2 * R; z < td.z0; ++z) {
+        for (int z = 0; z < 2 * R; ++z) {
             load_window();
-            store_window();
+            store_window(z);
         }
         load_window();
         load_point(vp[0]);
-        int slo = 0;  // slot of plane zz - R (lowest plane of stage one's window)
-        int s1w = 0;  // u1 window slot of plane zz
+        int s1w = 0;       // u1 window slot of plane zz
+        int swin = 2 * R;  // window slot of plane zz + R
 
         // Register queues of QL = 2R+2 planes.  They rotate instead of
         // shifting: the plane loop is unrolled by QL, and the plane of phase
@@ -240,6 +238,7 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         // (j = QL-1 the newest) in physical entry (j + P) mod QL, so a plane
         // moves no register.
         constexpr int QL = 2 * R + 2;
+        static_assert(QL == WS, "window slots are constants of the phase");
         S dk1[QL][NC];   // k1 of the own cell, planes zz-2R-1 .. zz (stage two reads zz-R-1)
         V du1[QL][NPA];  // u1 of the point components, same planes
         V uq[NS][QL];    // u1 of the own cell, stencil components, same planes
@@ -278,7 +277,13 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             // the last plane stage two needs; stage two on plane zz-R-1.
             const bool one = zz < td.z1 + R;
             const V (&vpc)[NPA][L1] = vp[PH & 1];  // point components of plane zz
-            if (one) store_window();
+            // Plane zz + R enters window slot (zz + R - (z0 - 2R)) mod WS
+            // (a counter: as a constant of the phase its address would stay
+            // live across the unrolled loop).
+            if (one) {
+                store_window(swin);
+                swin = swin + 1 == WS ? 0 : swin + 1;
+            }
             // Wait for this plane's point components here, before the next
             // plane's loads: a wait issued after those would wait for them
             // too (the counter drains in issue order).  On the last plane
@@ -295,11 +300,12 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             __syncthreads();
 
             // ---- stage one at plane zz over the tile + R ----
-            const int sz = cyc(slo + R);
+            // Plane zz - R + d is in window slot (PH + d) mod WS.
+            constexpr int sz = (PH + R) % WS;
             if (one) {
                 int zpl0[W];
 #pragma unroll
-                for (int d = 0; d < W; ++d) zpl0[d] = (cyc(slo + d) - sz) * E0;
+                for (int d = 0; d < W; ++d) zpl0[d] = ((PH + d) % WS - sz) * E0;
 #pragma unroll
                 for (int m = 0; m < L1; ++m) {
                     if (!e1ok[m]) continue;
@@ -406,7 +412,6 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
             const int zc = zz - R - 1;
             const int sc = next(s1w, WS1);  // u1 window slot of plane zc
             s1w = sc;
-            slo = next(slo, WS);
             if (zc < td.z0) return;
             const int lc = l1[0];
             const int gi = bd.lo[0] + td.x0 + tx, gj = bd.lo[1] + td.y0 + w,
