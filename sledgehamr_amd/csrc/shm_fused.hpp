@@ -69,8 +69,9 @@ struct FusedShape {
     static constexpr int L1 = (E1 + NT - 1) / NT;  // stage-one points per thread
     static constexpr int WS1 = R + 2;  // u1 window planes
     static constexpr long lds_bytes = (long)sizeof(V) * ((long)NS * WS * E0 + (long)NS * WS1 * E1);
-    // Instantiated when the LDS fits one workgroup per CU and the delay
-    // line stays small.
+    // Instantiated when the LDS fits one workgroup per CU and k1 and the
+    // point-wise u1 over the R+2 planes stage two spans stay within 56
+    // registers (their queues are padded to 2R+2 planes to rotate).
     static constexpr bool fits =
         !KO && lds_bytes <= 160 * 1024 && L1 <= 3 &&
         (NC * (int)(sizeof(S) / 4) + NP * (int)(sizeof(V) / 4)) * (R + 2) <= 56;
