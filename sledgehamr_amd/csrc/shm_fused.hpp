@@ -255,16 +255,24 @@ __global__ void __launch_bounds__(64 * NW) fused12_kernel(const FusedArgs a) {
         S pend_f[NPR];       // RKN: stage one's F0
         bool has_k = false, has_f = false;
         unsigned o_k = 0, o_f = 0;
+        // Output bases per component, uniform across the workgroup: a store
+        // adds only the cell's 32-bit offset.
+        S *kb[NKS];
+        S *fb[NPR];
+#pragma unroll
+        for (int n = 0; n < NKS; ++n) kb[n] = static_cast<S *>(a.kout) + org + n * a.cstride;
+#pragma unroll
+        for (int p = 0; p < NPR; ++p)
+            fb[p] = RKN ? static_cast<S *>(a.f0out) + org + p * a.cstride : nullptr;
         auto flush = [&]() {
             if (has_k) {
 #pragma unroll
-                for (int n = 0; n < NKS; ++n) st<S>(a.kout, org + n * a.cstride, o_k, pend_k[n]);
+                for (int n = 0; n < NKS; ++n) st<S>(kb[n], 0, o_k, pend_k[n]);
             }
             if constexpr (RKN) {
                 if (has_f) {
 #pragma unroll
-                    for (int p = 0; p < CS::NPAIR; ++p)
-                        st<S>(a.f0out, org + p * a.cstride, o_f, pend_f[p]);
+                    for (int p = 0; p < CS::NPAIR; ++p) st<S>(fb[p], 0, o_f, pend_f[p]);
                 }
             }
             has_k = false;
