@@ -43,6 +43,48 @@ def test_fused_equals_two_launches_complex(prec, fv):
     _same(a, b, 0, 4, ("complex", prec, fv))
 
 
+@pytest.mark.parametrize("tz,persistent,xcd", [(16, 0, 0), (20, 1, 0), (5, 1, 1), (7, 1, 0)])
+@pytest.mark.parametrize("prec", [shm.PREC_FP64, shm.PREC_FP32C16])
+def test_fused_z_chunks_and_persistent_grids(prec, tz, persistent, xcd):
+    """The fused pass unrolls its plane loop by the register-queue length
+    (2R+2 = 6 here): z chunks of lengths that are not multiples of it, and
+    persistent grids where a workgroup runs several tiles (128^3 in 64 x 16
+    tiles and 5- or 7-plane chunks: 416 or 304 tiles for at most 256
+    workgroups; the queues restart at each tile), give the two-launch
+    result bit for bit."""
+    inp = complex_scalar(N=128)
+    a = shm.Simulation("ComplexScalar", inp, precision=prec)
+    b = shm.Simulation("ComplexScalar", inp, precision=prec)
+    # One call sets the whole launch shape.
+    a.set_launch(tile_z=tz, persistent=persistent, xcd_split=xcd, fused=0)
+    b.set_launch(tile_z=tz, persistent=persistent, xcd_split=xcd, fused=1,
+                 fused_variant=2)
+    for s in (a, b):
+        s.init_state(0, 1, 22, [0.05])
+    a.advance(0, 3)
+    b.advance(0, 3)
+    assert b.get_launch(0)["fused"] == 1 and a.get_launch(0)["fused"] == 0
+    _same(a, b, 0, 4, ("chunks", prec, tz, persistent, xcd))
+
+
+@pytest.mark.parametrize("order,tz", [(1, 5), (3, 12)])
+def test_fused_chunks_radius(order, tz):
+    """Queue lengths 4 and 8 (radius 1 and 3) on chunks of 5 and 12 planes."""
+    inp = real_scalar(N=32, order=order, integrator=10, lam=1.0)
+    a = shm.Simulation("RealScalar", inp, precision=shm.PREC_FP32C16)
+    b = shm.Simulation("RealScalar", inp, precision=shm.PREC_FP32C16)
+    a.set_launch(tile_z=tz, persistent=1, fused=0)
+    b.set_launch(tile_z=tz, persistent=1, fused=1, fused_variant=2)
+    phi = gaussian_random_field(32, seed=order + 7, kcut=8, std=0.2)
+    for s in (a, b):
+        s.upload(0, 0, phi)
+        s.upload(0, 1, np.zeros_like(phi))
+    a.advance(0, 3)
+    b.advance(0, 3)
+    assert b.get_launch(0)["fused"] == 1
+    _same(a, b, 0, 2, ("chunks radius", order, tz))
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_fused_equals_two_launches_radius(order):
     inp = real_scalar(N=32, order=order, integrator=10, lam=1.0)
