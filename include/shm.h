@@ -288,6 +288,9 @@ typedef struct shm_launch_t {
 /* Times the stage kernels of a level over every launch shape not fixed by
  * shm_set_launch and keeps the fastest for that level. */
 int shm_autotune(shm_ctx *ctx, int lev);
+/* Fixes the launch shape of every level: each call replaces the whole
+ * shape, and fields at -1 (tile_z 0) are left to the defaults and the
+ * autotuner. */
 int shm_set_launch(shm_ctx *ctx, const shm_launch_t *cfg);
 /* The launch shape a level's stage kernels use now. */
 int shm_get_launch(shm_ctx *ctx, int lev, shm_launch_t *cfg);

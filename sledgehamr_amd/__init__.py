@@ -530,7 +530,9 @@ class Simulation:
     def set_launch(self, variant: int = -1, tile_z: int = 0, xcd_split: int = -1,
                    persistent: int = -1, zc_minor: int = -1, fused: int = -1,
                    fused_variant: int = -1) -> None:
-        """Fixes fields of the stage-kernel launch shape (-1 / 0: not set)."""
+        """Fixes fields of the stage-kernel launch shape (-1 / 0: not set).
+        Each call replaces the whole shape: give every field to fix in one
+        call."""
         cfg = _Launch(variant, tile_z, xcd_split, persistent, zc_minor, fused,
                       fused_variant)
         self._check(self.lib.shm_set_launch(self._ctx, C.byref(cfg)))
